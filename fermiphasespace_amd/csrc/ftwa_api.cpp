@@ -1,0 +1,752 @@
+// ftwa_api.cpp -- the C ABI of include/fps_ftwa.h: handle, device buffers, run orchestration.
+//
+// One handle = one device + one HIP stream.  fpsftwa_run restates FHfTWASolver::runSimulation
+// (src/FHfTWASolver.cpp:222-556): sample every trajectory (:342-362), integrate the time grid
+// recording observables at each time point (:366-426), reduce the ensemble (:438-482) and keep the
+// sums for the reference's normalisation (:489-551), which fpsftwa_get_result applies.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fps_ftwa.h"
+#include "ftwa_kernels.hpp"
+#include "ftwa_internal.hpp"
+
+using namespace fpsamd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(FPSFTWA_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+// Device buffer owned by a handle.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t n)
+    {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, n ? n : 1);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+bool is_gfx950(int dev, std::string* name)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+    if (name) *name = prop.gcnArchName;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+}  // namespace
+
+struct fpsftwa_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // system (host copy)
+    int L = 0;
+    int coulomb_type = 0;
+    std::vector<double> tM, U, V, pot, disp;
+    int pulse_type = 0;
+    double A0 = 0, t0 = 0, td = 1, w0 = 0, phi = 0;
+    double pol[3] = {0, 0, 0};
+    std::vector<double> knots;
+    // parameters
+    double dt = 0.01, t_max = 2.0;
+    int nsub = 1;
+    int64_t B = 10;
+    uint64_t seed = 0;
+    int engine = FPSFTWA_ENGINE_FP64_BATCHED;
+    uint32_t flags = FPSFTWA_RES_DIAG | FPSFTWA_RES_ENERGIES;
+    std::vector<double> n_up, n_dn, rot_up, rot_dn;
+    std::vector<double> noise;
+    int nt_hint = 0;
+    int time_chunk = 0;
+    // results
+    bool has_result = false;
+    int T = 0;
+    int64_t B_run = 0;
+    std::vector<double> acc;  // [T][Q][2]
+    int64_t launches = 0;
+    double kernel_ms = 0.0, run_ms = 0.0;
+    // device buffers
+    DevBuf sys_buf, state, rec, part, acc_d, work, noise_d, rot_d, scratch, occ_d;
+    bool state_double = true;
+};
+
+namespace {
+
+int check_handle(fpsftwa_handle* h)
+{
+    if (!h) return fail(FPSFTWA_EINVAL, "null handle");
+    return FPSFTWA_OK;
+}
+
+// Upload the static system in precision Real into one device allocation and fill the view.
+template <typename Real>
+int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
+{
+    const int L = h->L;
+    // ELL neighbour lists of tM (off-diagonal non-zeros), entry-major
+    std::vector<std::vector<int>> nb(L);
+    int z = 0;
+    for (int i = 0; i < L; ++i) {
+        for (int k = 0; k < L; ++k)
+            if (k != i && h->tM[(size_t)i * L + k] != 0.0) nb[i].push_back(k);
+        z = std::max<int>(z, (int)nb[i].size());
+    }
+    z = std::max(z, 1);
+    const double* pol = h->pol;
+    auto pdot = [&](int i, int k) {
+        if (h->disp.empty()) return 0.0;
+        const double* d = &h->disp[((size_t)i * L + k) * 3];
+        return pol[0] * d[0] + pol[1] * d[1] + pol[2] * d[2];
+    };
+    std::vector<int> nk((size_t)z * L);
+    std::vector<Real> nt((size_t)z * L), npd((size_t)z * L);
+    for (int i = 0; i < L; ++i)
+        for (int n = 0; n < z; ++n) {
+            const bool on = n < (int)nb[i].size();
+            const int k = on ? nb[i][n] : i;
+            nk[(size_t)n * L + i] = k;
+            nt[(size_t)n * L + i] = on ? (Real)h->tM[(size_t)i * L + k] : Real(0);
+            npd[(size_t)n * L + i] = on ? (Real)pdot(i, k) : Real(0);
+        }
+    std::vector<Real> udiag(L), pdiag(L), tdiag(L), potv(L), Ud((size_t)L * L), Vd((size_t)L * L), tMd((size_t)L * L),
+        pdd((size_t)L * L);
+    for (int i = 0; i < L; ++i) {
+        udiag[i] = h->coulomb_type ? (Real)h->U[(size_t)i * L + i] : Real(0);
+        const double p = h->pot.empty() ? 0.0 : h->pot[i];
+        potv[i] = (Real)p;
+        tdiag[i] = (Real)h->tM[(size_t)i * L + i];
+        pdiag[i] = (Real)(p + h->tM[(size_t)i * L + i]);
+    }
+    for (size_t e = 0; e < (size_t)L * L; ++e) {
+        Ud[e] = h->U.empty() ? Real(0) : (Real)h->U[e];
+        Vd[e] = h->V.empty() ? Real(0) : (Real)h->V[e];
+        tMd[e] = (Real)h->tM[e];
+        pdd[e] = (Real)pdot((int)(e / L), (int)(e % L));
+    }
+    // pulse amplitude at every RK stage time: tau = g*hstep + c_s*hstep, c = (0, 1/3, 2/3, 1)
+    std::vector<double> amp;
+    if (h->pulse_type != 0) {
+        const double hs = h->dt / h->nsub;
+        const int64_t n = (int64_t)T * h->nsub;
+        amp.resize((size_t)n * 4);
+        PulseSpec pp{h->pulse_type, h->A0, h->t0, h->td, h->w0, h->phi, h->knots.data(), (int)h->knots.size(), h->t_max};
+        const double cs[4] = {0.0, 1.0 / 3.0, 2.0 / 3.0, 1.0};
+        for (int64_t g = 0; g < n; ++g)
+            for (int s = 0; s < 4; ++s) amp[(size_t)g * 4 + s] = pulse_amplitude(pp, (double)g * hs + cs[s] * hs);
+    }
+    // one allocation, 256-byte aligned sections
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_nk = take(nk.size() * sizeof(int)), o_nt = take(nt.size() * sizeof(Real)),
+                 o_npd = take(npd.size() * sizeof(Real)), o_ud = take(L * sizeof(Real)),
+                 o_pd = take(L * sizeof(Real)), o_td = take(L * sizeof(Real)), o_pot = take(L * sizeof(Real)),
+                 o_U = take(Ud.size() * sizeof(Real)), o_V = take(Vd.size() * sizeof(Real)),
+                 o_tM = take(tMd.size() * sizeof(Real)), o_pdd = take(pdd.size() * sizeof(Real)),
+                 o_amp = take(std::max<size_t>(amp.size(), 1) * sizeof(double));
+    HIP_TRY(h->sys_buf.ensure(off));
+    char* base = h->sys_buf.as<char>();
+    auto up = [&](size_t o, const void* src, size_t bytes) {
+        return hipMemcpyAsync(base + o, src, bytes, hipMemcpyHostToDevice, h->stream);
+    };
+    HIP_TRY(up(o_nk, nk.data(), nk.size() * sizeof(int)));
+    HIP_TRY(up(o_nt, nt.data(), nt.size() * sizeof(Real)));
+    HIP_TRY(up(o_npd, npd.data(), npd.size() * sizeof(Real)));
+    HIP_TRY(up(o_ud, udiag.data(), L * sizeof(Real)));
+    HIP_TRY(up(o_pd, pdiag.data(), L * sizeof(Real)));
+    HIP_TRY(up(o_td, tdiag.data(), L * sizeof(Real)));
+    HIP_TRY(up(o_pot, potv.data(), L * sizeof(Real)));
+    HIP_TRY(up(o_U, Ud.data(), Ud.size() * sizeof(Real)));
+    HIP_TRY(up(o_V, Vd.data(), Vd.size() * sizeof(Real)));
+    HIP_TRY(up(o_tM, tMd.data(), tMd.size() * sizeof(Real)));
+    HIP_TRY(up(o_pdd, pdd.data(), pdd.size() * sizeof(Real)));
+    if (!amp.empty()) HIP_TRY(up(o_amp, amp.data(), amp.size() * sizeof(double)));
+    HIP_TRY(hipStreamSynchronize(h->stream));  // host vectors go out of scope
+    ds.L = L;
+    ds.z = z;
+    ds.coulomb_type = h->coulomb_type;
+    ds.nbr_k = reinterpret_cast<const int*>(base + o_nk);
+    ds.nbr_t = reinterpret_cast<const Real*>(base + o_nt);
+    ds.nbr_pd = reinterpret_cast<const Real*>(base + o_npd);
+    ds.u_diag = reinterpret_cast<const Real*>(base + o_ud);
+    ds.pdiag = reinterpret_cast<const Real*>(base + o_pd);
+    ds.tdiag = reinterpret_cast<const Real*>(base + o_td);
+    ds.pot = reinterpret_cast<const Real*>(base + o_pot);
+    ds.U = reinterpret_cast<const Real*>(base + o_U);
+    ds.V = reinterpret_cast<const Real*>(base + o_V);
+    ds.tM = reinterpret_cast<const Real*>(base + o_tM);
+    ds.pd = reinterpret_cast<const Real*>(base + o_pdd);
+    ds.amp = amp.empty() ? nullptr : reinterpret_cast<const double*>(base + o_amp);
+    return FPSFTWA_OK;
+}
+
+int prepare_initial(fpsftwa_handle* h)
+{
+    const int L = h->L;
+    std::vector<double> occ(2 * (size_t)L);
+    for (int i = 0; i < L; ++i) {
+        occ[i] = h->n_up.empty() ? 0.0 : h->n_up[i];
+        occ[L + i] = h->n_dn.empty() ? 0.0 : h->n_dn[i];
+    }
+    HIP_TRY(h->occ_d.ensure(occ.size() * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(h->occ_d.p, occ.data(), occ.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (!h->noise.empty()) {
+        HIP_TRY(h->noise_d.ensure(h->noise.size() * sizeof(double)));
+        HIP_TRY(hipMemcpyAsync(h->noise_d.p, h->noise.data(), h->noise.size() * sizeof(double), hipMemcpyHostToDevice,
+                               h->stream));
+    }
+    if (!h->rot_up.empty()) {
+        const size_t LL2 = (size_t)L * L * 2;
+        std::vector<double> rot(2 * LL2);
+        std::memcpy(rot.data(), h->rot_up.data(), LL2 * sizeof(double));
+        std::memcpy(rot.data() + LL2, h->rot_dn.data(), LL2 * sizeof(double));
+        HIP_TRY(h->rot_d.ensure(rot.size() * sizeof(double)));
+        HIP_TRY(hipMemcpyAsync(h->rot_d.p, rot.data(), rot.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FPSFTWA_OK;
+}
+
+template <typename Real>
+int sample_states(fpsftwa_handle* h, const DevSystem<Real>& ds)
+{
+    const int L = h->L;
+    const size_t NE = 2 * (size_t)L * L;
+    HIP_TRY(h->state.ensure((size_t)h->B * NE * 2 * sizeof(Real)));
+    const double* occ = h->occ_d.as<double>();
+    HIP_TRY(launch_sample<Real>(ds, occ, occ + L, h->noise.empty() ? nullptr : h->noise_d.as<double>(), h->seed, 0, h->B,
+                                h->state.as<Real>(), h->stream));
+    if (!h->rot_up.empty()) {
+        HIP_TRY(h->scratch.ensure((size_t)h->B * NE * 2 * sizeof(Real)));
+        HIP_TRY(launch_rotate<Real>(L, h->rot_d.as<double>(), h->B, h->state.as<Real>(), h->scratch.as<Real>(),
+                                    h->stream));
+    }
+    return FPSFTWA_OK;
+}
+
+struct EventTimer {
+    std::vector<hipEvent_t> ev;
+    ~EventTimer()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    hipError_t mark(hipStream_t s)
+    {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        ev.push_back(e);
+        return hipEventRecord(e, s);
+    }
+    // sum of (ev[2k+1] - ev[2k])
+    double pairs_ms() const
+    {
+        double t = 0.0;
+        for (size_t k = 0; k + 1 < ev.size(); k += 2) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) == hipSuccess) t += ms;
+        }
+        return t;
+    }
+};
+
+template <typename Real>
+int run_batched(fpsftwa_handle* h, EventTimer& kt)
+{
+    DevSystem<Real> ds;
+    int rc = upload_system<Real>(h, ds, h->T);
+    if (rc) return rc;
+    int nt = h->nt_hint;
+    if (!stencil_fits<Real>(h->L, ds.z, &nt))
+        return fail(FPSFTWA_EUNSUPPORTED, "the batched engine does not support L=" + std::to_string(h->L) +
+                                              " with " + std::to_string(ds.z) +
+                                              " neighbours per site in this precision; use FPSFTWA_ENGINE_FP64_PLAIN");
+    if ((rc = sample_states<Real>(h, ds))) return rc;
+    const int L = h->L;
+    const int R = 2 * L + 8;
+    const int Q = 6 * L + 12;
+    int Tc = h->time_chunk;
+    if (Tc <= 0) {
+        const double per_tp = (double)h->B * R * sizeof(Real);
+        Tc = (int)std::max(1.0, std::min(64.0, 1.5e9 / per_tp));
+    }
+    Tc = std::min(Tc, h->T);
+    const int nchunk = reduce_chunks(h->B);
+    HIP_TRY(h->rec.ensure((size_t)Tc * h->B * R * sizeof(Real)));
+    HIP_TRY(h->part.ensure((size_t)Tc * nchunk * Q * 2 * sizeof(double)));
+    RunShape rs;
+    rs.B = h->B;
+    rs.T = h->T;
+    rs.nsub = h->nsub;
+    rs.h = h->dt / h->nsub;
+    rs.R = R;
+    for (int t0 = 0; t0 < h->T; t0 += Tc) {
+        const int t1 = std::min(h->T, t0 + Tc);
+        HIP_TRY(kt.mark(h->stream));
+        HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
+        HIP_TRY(kt.mark(h->stream));
+        ++h->launches;
+        HIP_TRY(launch_reduce<Real>(L, h->B, R, t1 - t0, t0, h->rec.as<Real>(), h->part.as<double>(), nchunk,
+                                    h->acc_d.as<double>(), h->stream));
+    }
+    return FPSFTWA_OK;
+}
+
+int run_plain(fpsftwa_handle* h, EventTimer& kt)
+{
+    DevSystem<double> ds;
+    int rc = upload_system<double>(h, ds, h->T);
+    if (rc) return rc;
+    if ((rc = sample_states<double>(h, ds))) return rc;
+    const int L = h->L;
+    const int R = 2 * L + 8;
+    const int Q = 6 * L + 12;
+    const int nchunk = reduce_chunks(h->B);
+    HIP_TRY(h->rec.ensure((size_t)h->B * R * sizeof(double)));
+    HIP_TRY(h->part.ensure((size_t)nchunk * Q * 2 * sizeof(double)));
+    HIP_TRY(h->work.ensure((size_t)h->B * plain_work_doubles(L) * sizeof(double)));
+    RunShape rs;
+    rs.B = h->B;
+    rs.T = h->T;
+    rs.nsub = h->nsub;
+    rs.h = h->dt / h->nsub;
+    rs.R = R;
+    for (int t = 0; t < h->T; ++t) {
+        HIP_TRY(kt.mark(h->stream));
+        HIP_TRY(launch_plain(ds, rs, t, h->state.as<double>(), h->work.as<double>(), h->rec.as<double>(), h->stream));
+        HIP_TRY(kt.mark(h->stream));
+        ++h->launches;
+        HIP_TRY(launch_reduce<double>(L, h->B, R, 1, t, h->rec.as<double>(), h->part.as<double>(), nchunk,
+                                      h->acc_d.as<double>(), h->stream));
+    }
+    return FPSFTWA_OK;
+}
+
+double kahan_value(const std::vector<double>& acc, int Q, int t, int q)
+{
+    const double* p = &acc[((size_t)t * Q + q) * 2];
+    return p[0] + p[1];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fpsftwa_last_error(void) { return g_last_error.c_str(); }
+
+int fpsftwa_version(int* major, int* minor, int* patch)
+{
+    if (major) *major = FPSFTWA_VERSION_MAJOR;
+    if (minor) *minor = FPSFTWA_VERSION_MINOR;
+    if (patch) *patch = FPSFTWA_VERSION_PATCH;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_device_info(char* name, size_t name_len, int* cu_count, double* hbm_gib)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(FPSFTWA_EDEVICE, "no HIP device visible");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (hbm_gib) *hbm_gib = prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0);
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_create(const fpsftwa_system* sys, fpsftwa_handle** out)
+{
+    if (!sys || !out) return fail(FPSFTWA_EINVAL, "null argument");
+    *out = nullptr;
+    const int L = sys->n_sites;
+    if (L < 1) return fail(FPSFTWA_EINVAL, "n_sites must be >= 1");
+    if (!sys->hopping) return fail(FPSFTWA_EINVAL, "hopping matrix is required");
+    if (sys->coulomb_type < 0 || sys->coulomb_type > 2) return fail(FPSFTWA_EINVAL, "bad coulomb_type");
+    if (sys->coulomb_type != FPSFTWA_COULOMB_NULL && !sys->coulomb_u) return fail(FPSFTWA_EINVAL, "coulomb_u is required");
+    if (sys->coulomb_type == FPSFTWA_COULOMB_GENERAL && !sys->coulomb_v)
+        return fail(FPSFTWA_EINVAL, "coulomb_v is required for COULOMB_GENERAL");
+    if (sys->pulse_type < 0 || sys->pulse_type > 2) return fail(FPSFTWA_EINVAL, "bad pulse_type");
+    if (sys->pulse_type != FPSFTWA_PULSE_NONE && !sys->displacement)
+        return fail(FPSFTWA_EINVAL, "the pulse needs the displacement array");
+    if (sys->pulse_type == FPSFTWA_PULSE_KNOTS && (sys->pulse_n_knots < 1 || !sys->pulse_knots))
+        return fail(FPSFTWA_EINVAL, "PULSE_KNOTS needs at least one knot");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(FPSFTWA_EDEVICE, "no HIP device visible: this engine runs only on a gfx950 GPU");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::string arch;
+    if (!is_gfx950(dev, &arch))
+        return fail(FPSFTWA_EDEVICE, "device " + std::to_string(dev) + " is " + arch + ", not gfx950");
+    auto* h = new (std::nothrow) fpsftwa_handle();
+    if (!h) return fail(FPSFTWA_ENOMEM, "host allocation failed");
+    h->device = dev;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(FPSFTWA_EDEVICE, "hipStreamCreate failed");
+    }
+    const size_t LL = (size_t)L * L;
+    h->L = L;
+    h->coulomb_type = sys->coulomb_type;
+    h->tM.assign(sys->hopping, sys->hopping + LL);
+    if (sys->coulomb_u) h->U.assign(sys->coulomb_u, sys->coulomb_u + LL);
+    else h->U.assign(LL, 0.0);
+    if (sys->coulomb_v && sys->coulomb_type == FPSFTWA_COULOMB_GENERAL) h->V.assign(sys->coulomb_v, sys->coulomb_v + LL);
+    else h->V.assign(LL, 0.0);
+    if (sys->potential) h->pot.assign(sys->potential, sys->potential + L);
+    h->pulse_type = sys->pulse_type;
+    h->A0 = sys->pulse_A0;
+    h->t0 = sys->pulse_t0;
+    h->td = sys->pulse_td;
+    h->w0 = sys->pulse_w0;
+    h->phi = sys->pulse_phi;
+    for (int k = 0; k < 3; ++k) h->pol[k] = sys->pulse_pol[k];
+    if (sys->pulse_type == FPSFTWA_PULSE_KNOTS) h->knots.assign(sys->pulse_knots, sys->pulse_knots + sys->pulse_n_knots);
+    if (sys->displacement) h->disp.assign(sys->displacement, sys->displacement + LL * 3);
+    h->n_up.assign(L, 0.0);
+    h->n_dn.assign(L, 0.0);
+    *out = h;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_destroy(fpsftwa_handle* h)
+{
+    if (!h) return FPSFTWA_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->sys_buf.release();
+    h->state.release();
+    h->rec.release();
+    h->part.release();
+    h->acc_d.release();
+    h->work.release();
+    h->noise_d.release();
+    h->rot_d.release();
+    h->scratch.release();
+    h->occ_d.release();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_set_params(fpsftwa_handle* h, double dt, double t_max, int32_t n_sub, int64_t n_traj, int32_t int_order,
+                       uint64_t seed, int32_t engine)
+{
+    if (int rc = check_handle(h)) return rc;
+    (void)int_order;  // the reference forces RK4 "3/8" (src/FHfTWASolver.cpp:90)
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(FPSFTWA_EINVAL, "dt must be > 0");
+    if (!(t_max >= 0.0) || !std::isfinite(t_max)) return fail(FPSFTWA_EINVAL, "t_max must be >= 0");
+    if (n_sub < 1) return fail(FPSFTWA_EINVAL, "n_sub must be >= 1");
+    if (n_traj < 1) return fail(FPSFTWA_EINVAL, "n_traj must be >= 1");
+    if (engine < 0 || engine > 2) return fail(FPSFTWA_EINVAL, "bad engine");
+    const double nt = std::floor(t_max / dt + 1.0);
+    if (nt > 2.0e9) return fail(FPSFTWA_EINVAL, "too many time points");
+    h->dt = dt;
+    h->t_max = t_max;
+    h->nsub = n_sub;
+    h->B = n_traj;
+    h->seed = seed;
+    h->engine = engine;
+    h->has_result = false;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_set_results(fpsftwa_handle* h, uint32_t flags)
+{
+    if (int rc = check_handle(h)) return rc;
+    h->flags = flags;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_set_initial_state(fpsftwa_handle* h, const double* n_up, const double* n_dn, const double* rot_up,
+                              const double* rot_dn)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (!n_up || !n_dn) return fail(FPSFTWA_EINVAL, "occupations are required");
+    const int L = h->L;
+    for (int i = 0; i < L; ++i)
+        if (!(n_up[i] >= 0.0 && n_up[i] <= 1.0 && n_dn[i] >= 0.0 && n_dn[i] <= 1.0))
+            return fail(FPSFTWA_EINVAL, "occupations must lie in [0, 1]");
+    if ((rot_up == nullptr) != (rot_dn == nullptr))
+        return fail(FPSFTWA_EINVAL, "give both rotations or neither");
+    h->n_up.assign(n_up, n_up + L);
+    h->n_dn.assign(n_dn, n_dn + L);
+    if (rot_up) {
+        h->rot_up.assign(rot_up, rot_up + (size_t)L * L * 2);
+        h->rot_dn.assign(rot_dn, rot_dn + (size_t)L * L * 2);
+    } else {
+        h->rot_up.clear();
+        h->rot_dn.clear();
+    }
+    h->has_result = false;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_set_noise(fpsftwa_handle* h, const double* noise, size_t n_values)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (!noise) {
+        h->noise.clear();
+        return FPSFTWA_OK;
+    }
+    const size_t need = (size_t)h->B * 2 * ((size_t)h->L * (h->L - 1) / 2) * 2;
+    if (n_values != need)
+        return fail(FPSFTWA_EINVAL, "noise must hold n_traj*2*L(L-1)/2*2 = " + std::to_string(need) + " values");
+    h->noise.assign(noise, noise + n_values);
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_sample_noise(fpsftwa_handle* h, int64_t first, int64_t count, double* out, size_t n_values)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (first < 0 || count < 0 || !out) return fail(FPSFTWA_EINVAL, "bad range");
+    const size_t need = (size_t)count * 2 * ((size_t)h->L * (h->L - 1) / 2) * 2;
+    if (n_values < need) return fail(FPSFTWA_EINVAL, "output too small");
+    if (need == 0) return FPSFTWA_OK;
+    HIP_TRY(hipSetDevice(h->device));
+    DevBuf buf;
+    HIP_TRY(buf.ensure(need * sizeof(double)));
+    HIP_TRY(launch_noise(h->L, h->seed, first, count, buf.as<double>(), h->stream));
+    HIP_TRY(hipMemcpyAsync(out, buf.p, need * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_run(fpsftwa_handle* h)
+{
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    if (!h->noise.empty()) {
+        const size_t need = (size_t)h->B * 2 * ((size_t)h->L * (h->L - 1) / 2) * 2;
+        if (h->noise.size() != need) return fail(FPSFTWA_EINVAL, "noise size does not match n_traj");
+    }
+    h->has_result = false;
+    h->T = (int)std::floor(h->t_max / h->dt + 1.0);  // src/FHSolver.cpp:100
+    const int Q = 6 * h->L + 12;
+    h->launches = 0;
+    h->kernel_ms = 0.0;
+    HIP_TRY(h->acc_d.ensure((size_t)h->T * Q * 2 * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(h->acc_d.p, 0, (size_t)h->T * Q * 2 * sizeof(double), h->stream));
+    int rc = prepare_initial(h);
+    if (rc) return rc;
+    EventTimer whole, kt;
+    HIP_TRY(whole.mark(h->stream));
+    if (h->engine == FPSFTWA_ENGINE_FP64_PLAIN) {
+        h->state_double = true;
+        rc = run_plain(h, kt);
+    } else if (h->engine == FPSFTWA_ENGINE_FP64_BATCHED) {
+        h->state_double = true;
+        rc = run_batched<double>(h, kt);
+    } else {
+        h->state_double = false;
+        rc = run_batched<float>(h, kt);
+    }
+    if (rc) {
+        (void)hipStreamSynchronize(h->stream);
+        return rc;
+    }
+    HIP_TRY(whole.mark(h->stream));
+    h->acc.resize((size_t)h->T * Q * 2);
+    HIP_TRY(hipMemcpyAsync(h->acc.data(), h->acc_d.p, h->acc.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->kernel_ms = kt.pairs_ms();
+    h->run_ms = whole.pairs_ms();
+    h->B_run = h->B;
+    h->has_result = true;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, double* run_ms)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (!h->has_result) return fail(FPSFTWA_ESTATE, "no completed run");
+    if (launches) *launches = h->launches;
+    if (kernel_ms) *kernel_ms = h->kernel_ms;
+    if (run_ms) *run_ms = h->run_ms;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_get_result(fpsftwa_handle* h, int32_t which, double* out, size_t n_values, size_t* needed)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (!h->has_result) return fail(FPSFTWA_ESTATE, "no completed run");
+    const int L = h->L, T = h->T, Q = 6 * L + 12;
+    const double B = (double)h->B_run;
+    size_t n = 0;
+    switch (which) {
+    case FPSFTWA_OUT_TIME: n = T; break;
+    case FPSFTWA_OUT_ENERGY: n = 5 * (size_t)T; break;
+    case FPSFTWA_OUT_ENERGY_ERR:
+    case FPSFTWA_OUT_SUM_ENERGY: n = 4 * (size_t)T; break;
+    case FPSFTWA_OUT_N1HU:
+    case FPSFTWA_OUT_N1HD: n = 2 * (size_t)T; break;
+    default:
+        if (which < 0 || which >= FPSFTWA_OUT_COUNT) return fail(FPSFTWA_EINVAL, "unknown result id");
+        n = (size_t)L * T;
+    }
+    if (needed) *needed = n;
+    if (!out) return FPSFTWA_OK;
+    if (n_values < n) return fail(FPSFTWA_EINVAL, "output buffer too small");
+    auto S = [&](int t, int q) { return kahan_value(h->acc, Q, t, q); };
+    auto stderr_of = [&](double s1, double s2) {
+        if (B < 2) return 0.0;
+        double var = (s2 - s1 * s1 / B) / (B - 1.0);
+        return var > 0 ? std::sqrt(var / B) : 0.0;
+    };
+    for (int t = 0; t < T; ++t) {
+        switch (which) {
+        case FPSFTWA_OUT_TIME: out[t] = t * h->dt; break;
+        case FPSFTWA_OUT_ENERGY: {
+            double tot = 0.0;
+            for (int k = 0; k < 4; ++k) {
+                const double e = S(t, 6 * L + k) / B;
+                out[(size_t)k * T + t] = e;
+                tot += e;
+            }
+            out[(size_t)4 * T + t] = tot;
+            break;
+        }
+        case FPSFTWA_OUT_ENERGY_ERR:
+            for (int k = 0; k < 4; ++k) out[(size_t)k * T + t] = stderr_of(S(t, 6 * L + k), S(t, 6 * L + 4 + k));
+            break;
+        case FPSFTWA_OUT_SUM_ENERGY:
+            for (int k = 0; k < 4; ++k) out[(size_t)k * T + t] = S(t, 6 * L + k);
+            break;
+        case FPSFTWA_OUT_N1HU:
+            out[t] = S(t, 6 * L + 8) / B + (L == 1 ? 0.5 : 0.0);
+            out[T + t] = S(t, 6 * L + 9) / B;
+            break;
+        case FPSFTWA_OUT_N1HD:
+            out[t] = S(t, 6 * L + 10) / B + (L == 1 ? 0.5 : 0.0);
+            out[T + t] = S(t, 6 * L + 11) / B;
+            break;
+        default:
+            for (int i = 0; i < L; ++i) {
+                const double su = S(t, 6 * i), sd = S(t, 6 * i + 1), suu = S(t, 6 * i + 2), sdd = S(t, 6 * i + 3),
+                             sud = S(t, 6 * i + 4), snn = S(t, 6 * i + 5);
+                double v = 0.0;
+                switch (which) {
+                case FPSFTWA_OUT_NIIU: v = su / B + 0.5; break;              // src/FHfTWASolver.cpp:509
+                case FPSFTWA_OUT_NIID: v = sd / B + 0.5; break;              // :510
+                case FPSFTWA_OUT_N2IIU: v = suu / B + (2.0 * su) / (2.0 * B) + 0.25; break;  // :520
+                case FPSFTWA_OUT_N2IID: v = sdd / B + (2.0 * sd) / (2.0 * B) + 0.25; break;  // :521
+                case FPSFTWA_OUT_NUND: v = sud / B + (su + sd) / (2.0 * B) + 0.25; break;    // :530
+                case FPSFTWA_OUT_NIIU_ERR: v = stderr_of(su, suu); break;
+                case FPSFTWA_OUT_NIID_ERR: v = stderr_of(sd, sdd); break;
+                case FPSFTWA_OUT_NUND_ERR: {
+                    const double sx = sud + (su + sd) / 2.0 + B / 4.0;  // sum of (u+1/2)(d+1/2)
+                    v = stderr_of(sx, snn);
+                    break;
+                }
+                case FPSFTWA_OUT_SUM_RHO_U: v = su; break;
+                case FPSFTWA_OUT_SUM_RHO_D: v = sd; break;
+                case FPSFTWA_OUT_SUM_RHO2_U: v = suu; break;
+                case FPSFTWA_OUT_SUM_RHO2_D: v = sdd; break;
+                case FPSFTWA_OUT_SUM_RHOUD: v = sud; break;
+                default: break;
+                }
+                out[(size_t)i * T + t] = v;
+            }
+        }
+    }
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_get_states(fpsftwa_handle* h, int64_t first, int64_t count, double* out, size_t n_values)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (!h->has_result) return fail(FPSFTWA_ESTATE, "no completed run");
+    if (first < 0 || count < 0 || first + count > h->B_run || !out) return fail(FPSFTWA_EINVAL, "bad range");
+    const size_t per = 2 * (size_t)h->L * h->L * 2;
+    if (n_values < per * count) return fail(FPSFTWA_EINVAL, "output too small");
+    HIP_TRY(hipSetDevice(h->device));
+    if (h->state_double) {
+        HIP_TRY(hipMemcpy(out, h->state.as<double>() + (size_t)first * per, per * count * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(per * count);
+        HIP_TRY(hipMemcpy(tmp.data(), h->state.as<float>() + (size_t)first * per, per * count * sizeof(float),
+                          hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < tmp.size(); ++k) out[k] = tmp[k];
+    }
+    return FPSFTWA_OK;
+}
+
+}  // extern "C"
+
+namespace fpsamd {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+double pulse_amplitude(const PulseSpec& p, double t)
+{
+    if (p.type == FPSFTWA_PULSE_GAUSSIAN) {  // src/FHExternalPotentialPulse.cpp:88
+        const double x = (t - p.t0) / p.td;
+        return p.A0 * std::exp(-(x * x) / 2.0) * std::cos(p.w0 * (t - p.t0) + p.phi);
+    }
+    if (p.type == FPSFTWA_PULSE_KNOTS) {  // piecewise linear through knots at k*t_max/(n-1)
+        const int n = p.n_knots;
+        if (n <= 0) return 0.0;
+        if (n == 1) return p.knots[0];
+        const double step = p.t_max / (double)(n - 1);
+        const double u = t / step;
+        if (u <= 0.0) return p.knots[0];
+        const int k = (int)std::floor(u);
+        if (k >= n - 1) return p.knots[n - 1];
+        const double f = u - (double)k;
+        return p.knots[k] * (1.0 - f) + p.knots[k + 1] * f;
+    }
+    return 0.0;
+}
+}  // namespace fpsamd
+
+extern "C" int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (block_threads < 0 || block_threads > 1024 || time_chunk < 0)
+        return fail(FPSFTWA_EINVAL, "block_threads must be in [0, 1024] and time_chunk >= 0");
+    if (block_threads > 0 && block_threads % h->L != 0)
+        return fail(FPSFTWA_EINVAL, "block_threads must be a multiple of the site count");
+    h->nt_hint = block_threads;
+    h->time_chunk = time_chunk;
+    return FPSFTWA_OK;
+}

@@ -1,0 +1,942 @@
+// ftwa_kernels.hip -- gfx950 kernels of the fTWA trajectory engine.
+//
+// Reference path restated: FHfTWASolver::runSimulation (src/FHfTWASolver.cpp:222-556) with
+// derivative (:568-600) and step_RK4_38 (:629-661).  Three kernel families:
+//
+//   sample   Gaussian initial states rho = diag(n - 1/2) + S.W (:342-362), noise from Philox4x32-10
+//   stencil  the product: one workgroup per trajectory, both spins' stage input resident in LDS,
+//            the commutator -i[rho, Delta] as a sparse stencil over the hopping's neighbour lists,
+//            RK4 "3/8" state (y, A, p) in registers, one launch covering many time points
+//   plain    the FP64 reference-order baseline: dense Delta, rho*Delta - Delta*rho from global memory,
+//            one launch per time point
+//   reduce   fixed-order, compensated FP64 ensemble sums of per-trajectory records
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "ftwa_kernels.hpp"
+
+#ifndef FTWA_REAL
+#define FTWA_REAL 2
+#endif
+
+namespace fpsamd {
+
+namespace {
+
+template <typename Real>
+using V2 = typename std::conditional<std::is_same<Real, float>::value, float2, double2>::type;
+
+// ---- Philox4x32-10 (Salmon et al., SC'11) -------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        const uint32_t lo0 = 0xD2511F53u * c[0];
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const uint32_t lo1 = 0xCD9E8D57u * c[2];
+        const uint32_t n0 = hi1 ^ c[1] ^ k0;
+        const uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Complex noise value for (trajectory, spin, pair): Box-Muller on two 53-bit uniforms,
+// E|w|^2 = 1 with independent N(0, 1/2) real and imaginary parts.
+__device__ __forceinline__ void noise_value(uint64_t seed, int64_t traj, int spin, uint32_t pair, double& re,
+                                            double& im)
+{
+    uint32_t c[4] = {pair, static_cast<uint32_t>(spin), static_cast<uint32_t>(static_cast<uint64_t>(traj)),
+                     static_cast<uint32_t>(static_cast<uint64_t>(traj) >> 32)};
+    philox4x32_10(c, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+    const double two_m53 = 1.1102230246251565404e-16;
+    const double u1 = (static_cast<double>(c[0] >> 5) * 67108864.0 + static_cast<double>(c[1] >> 6) + 1.0) * two_m53;
+    const double u2 = (static_cast<double>(c[2] >> 5) * 67108864.0 + static_cast<double>(c[3] >> 6)) * two_m53;
+    const double r = sqrt(-log(u1));
+    double sn, cs;
+    sincos(6.283185307179586232 * u2, &sn, &cs);
+    re = r * cs;
+    im = r * sn;
+}
+
+__device__ __forceinline__ int64_t pair_index(int L, int i, int j)  // i < j, row-major strictly upper
+{
+    return static_cast<int64_t>(i) * (2 * L - i - 1) / 2 + (j - i - 1);
+}
+
+// ---- sampling -------------------------------------------------------------------------
+template <typename Real>
+__global__ void sample_kernel(int L, const double* __restrict__ n_up, const double* __restrict__ n_dn,
+                              const double* __restrict__ noise, uint64_t seed, int64_t b0, int64_t B,
+                              Real* __restrict__ state)
+{
+    const int64_t LL = static_cast<int64_t>(L) * L;
+    const int64_t total = B * 2 * LL;
+    const int64_t npairs = static_cast<int64_t>(L) * (L - 1) / 2;
+    for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t b = e / (2 * LL);
+        const int64_t rem = e - b * 2 * LL;
+        const int sg = static_cast<int>(rem / LL);
+        const int64_t ij = rem - sg * LL;
+        const int i = static_cast<int>(ij / L);
+        const int j = static_cast<int>(ij - static_cast<int64_t>(i) * L);
+        const double* n = sg ? n_dn : n_up;
+        double re = 0.0, im = 0.0;
+        if (i == j) {
+            re = n[i] - 0.5;
+        } else {
+            const int a = i < j ? i : j;
+            const int c = i < j ? j : i;
+            const int64_t p = pair_index(L, a, c);
+            double wr, wi;
+            if (noise) {
+                const double* w = noise + ((b * 2 + sg) * npairs + p) * 2;
+                wr = w[0];
+                wi = w[1];
+            } else {
+                noise_value(seed, b0 + b, sg, static_cast<uint32_t>(p), wr, wi);
+            }
+            const double sa = n[a], sc = n[c];
+            const double sij = sqrt((sa + sc - 2.0 * sa * sc) / 2.0);
+            re = sij * wr;
+            im = (i < j) ? sij * wi : -sij * wi;
+        }
+        state[2 * e] = static_cast<Real>(re);
+        state[2 * e + 1] = static_cast<Real>(im);
+    }
+}
+
+#if FTWA_REAL != 1
+__global__ void noise_kernel(int L, uint64_t seed, int64_t b0, int64_t B, double* __restrict__ out)
+{
+    const int64_t npairs = static_cast<int64_t>(L) * (L - 1) / 2;
+    const int64_t total = B * 2 * npairs;
+    for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t b = e / (2 * npairs);
+        const int64_t rem = e - b * 2 * npairs;
+        const int sg = static_cast<int>(rem / npairs);
+        const int64_t p = rem - sg * npairs;
+        double re, im;
+        noise_value(seed, b0 + b, sg, static_cast<uint32_t>(p), re, im);
+        out[2 * e] = re;
+        out[2 * e + 1] = im;
+    }
+}
+#endif
+
+// rho <- R rho R^dagger for one (trajectory, spin) per workgroup; tmp holds R rho.
+template <typename Real>
+__global__ void rotate_kernel(int L, const double* __restrict__ rot, Real* __restrict__ state, Real* __restrict__ tmp)
+{
+    const int64_t LL = static_cast<int64_t>(L) * L;
+    const int sg = blockIdx.x & 1;
+    Real* M = state + static_cast<int64_t>(blockIdx.x) * LL * 2;
+    Real* W = tmp + static_cast<int64_t>(blockIdx.x) * LL * 2;
+    const double* R = rot + sg * LL * 2;
+    for (int64_t e = threadIdx.x; e < LL; e += blockDim.x) {
+        const int i = static_cast<int>(e / L), j = static_cast<int>(e % L);
+        double ar = 0.0, ai = 0.0;
+        for (int k = 0; k < L; ++k) {
+            const double rr = R[2 * (i * L + k)], ri = R[2 * (i * L + k) + 1];
+            const double mr = M[2 * (k * L + j)], mi = M[2 * (k * L + j) + 1];
+            ar += rr * mr - ri * mi;
+            ai += rr * mi + ri * mr;
+        }
+        W[2 * e] = static_cast<Real>(ar);
+        W[2 * e + 1] = static_cast<Real>(ai);
+    }
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < LL; e += blockDim.x) {
+        const int i = static_cast<int>(e / L), j = static_cast<int>(e % L);
+        double ar = 0.0, ai = 0.0;
+        for (int k = 0; k < L; ++k) {
+            const double wr = W[2 * (i * L + k)], wi = W[2 * (i * L + k) + 1];
+            const double rr = R[2 * (j * L + k)], ri = -R[2 * (j * L + k) + 1];  // conj(R_jk)
+            ar += wr * rr - wi * ri;
+            ai += wr * ri + wi * rr;
+        }
+        M[2 * e] = static_cast<Real>(ar);
+        M[2 * e + 1] = static_cast<Real>(ai);
+    }
+}
+
+// ---- workgroup reduction helpers ----------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ---- stencil integrator ------------------------------------------------------------------
+//
+// Element (sigma, i, j) of the trajectory's state is owned by thread tid for f = tid + NT*m,
+// f = sigma*L*L + i*L + j.  NT is a multiple of L, so a thread's column j never changes and the
+// neighbour list of j lives in registers.  Per RK stage:
+//   compute  k_ij = -i [ sum_{k in nb(j)} Y_ik T_kj - sum_{k in nb(i)} T_ik Y_kj + (h_j - h_i) Y_ij ]
+//            and fold it into the registers: A (b-weighted sum of k) and p (k1, then k1-k2, then k1-k2+k3)
+//   barrier
+//   write    the next stage input, derived from (y, A, p) alone, into LDS; the owners of diagonal
+//            elements refresh the Hartree potential h of the other spin (on-site U)
+//   barrier
+// The stage inputs of the 3/8 rule follow from (y, A, p):
+//   Y2 = y + h p/3,  Y3 = y + h (A - 3p)/6,  Y4 = y + h p,  y' = y + h A/8.
+template <typename Real, int EPT, int ZMAX, bool PULSE>
+__global__ void __launch_bounds__(EPT >= 32 ? 256 : (EPT >= 16 ? 512 : 1024))
+    stencil_kernel(DevSystem<Real> s, RunShape r, int t0, int t1, Real* __restrict__ state, Real* __restrict__ rec)
+{
+    using C = V2<Real>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int L = s.L;
+    const int LL = L * L;
+    const int NE = 2 * LL;
+    const int z = s.z;
+    const int NT = blockDim.x;
+    const int tid = threadIdx.x;
+    const int nwave = (NT + 63) / 64;
+    const int64_t b = blockIdx.x;
+    const bool general = (s.coulomb_type == 2);
+
+    C* Y = reinterpret_cast<C*>(smem);                                 // [2][L][L]
+    C* Tc = Y + NE;                                                    // [z][L] (pulse)
+    Real* hb = reinterpret_cast<Real*>(PULSE ? (Tc + z * L) : Tc);     // [2][L]
+    Real* rt = hb + 2 * L;                                             // [z][L]
+    Real* ew = rt + z * L;                                             // [nwave][4]
+    int* rk = reinterpret_cast<int*>(ew + 4 * nwave);                  // [z][L]
+
+    const int j = tid % L;
+    int ck[ZMAX];
+    Real ct[ZMAX];
+    Real cpd[ZMAX];
+#pragma unroll
+    for (int n = 0; n < ZMAX; ++n) {
+        const bool on = n < z;
+        ck[n] = on ? s.nbr_k[n * L + j] : j;
+        ct[n] = on ? s.nbr_t[n * L + j] : Real(0);
+        cpd[n] = (PULSE && on) ? s.nbr_pd[n * L + j] : Real(0);
+    }
+    for (int e = tid; e < z * L; e += NT) {
+        rk[e] = s.nbr_k[e];
+        rt[e] = s.nbr_t[e];
+    }
+
+    C y[EPT], A[EPT], p[EPT];
+    const C* gs = reinterpret_cast<const C*>(state) + b * NE;
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int f = tid + NT * m;
+        if (f < NE) {
+            y[m] = gs[f];
+            Y[f] = y[m];
+        }
+        A[m] = C{0, 0};
+        p[m] = C{0, 0};
+    }
+    const double h = r.h;
+    const Real hR = static_cast<Real>(h);
+    int64_t gst = static_cast<int64_t>(t0) * r.nsub * 4;  // global RK-stage counter (pulse amplitude index)
+
+    auto pulse_table = [&](int64_t stage_index) {
+        if constexpr (PULSE) {
+            const Real a = static_cast<Real>(s.amp[stage_index]);
+            for (int e = tid; e < z * L; e += NT) {
+                Real sn, cs;
+                if constexpr (std::is_same<Real, float>::value)
+                    __sincosf(a * s.nbr_pd[e], &sn, &cs);
+                else
+                    sincos(a * s.nbr_pd[e], &sn, &cs);
+                const Real t = s.nbr_t[e];
+                Tc[e] = C{t * (Real(1) + cs), t * sn};
+            }
+        }
+    };
+    // Hartree potential for both spins from the LDS diagonal (general U, V or at start-up)
+    auto h_phase = [&]() {
+        for (int e = tid; e < 2 * L; e += NT) {
+            const int sg = e / L, i = e - sg * L;
+            Real v = s.pdiag[i];
+            if (s.coulomb_type == 1) {
+                v += s.u_diag[i] * (Y[(1 - sg) * LL + i * L + i].x + Real(0.5));
+            } else if (general) {
+                Real a = 0;
+                for (int jj = 0; jj < L; ++jj)
+                    a += s.U[i * L + jj] * (Y[(1 - sg) * LL + jj * L + jj].x + Real(0.5)) +
+                         s.V[i * L + jj] * (Y[sg * LL + jj * L + jj].x + Real(0.5));
+                v += a;
+            }
+            hb[e] = v;
+        }
+    };
+
+    pulse_table(gst);
+    __syncthreads();
+    h_phase();
+    __syncthreads();
+
+    Real* recb = rec;
+    const int64_t Bn = r.B;
+    const int R = r.R;
+    for (int t = t0; t < t1; ++t) {
+        Real* rr = recb + (static_cast<int64_t>(t - t0) * Bn + b) * R;
+        // ---- record the state at time point t (src/FHfTWASolver.cpp:366-418) ----
+        {
+            if (tid < 2 * L) {
+                const int sg = tid / L, i = tid - sg * L;
+                rr[tid] = Y[sg * LL + i * L + i].x;
+            }
+            if (tid == 0) {
+                const C cu = Y[L - 1], cd = Y[LL + L - 1];
+                rr[2 * L + 4] = cu.x;
+                rr[2 * L + 5] = cu.y;
+                rr[2 * L + 6] = cd.x;
+                rr[2 * L + 7] = cd.y;
+            }
+            Real eth = 0, eu = 0, ev = 0, epe = 0;
+            if (tid < L) {
+                const int i = tid;
+                const Real nu = Y[i * L + i].x + Real(0.5);
+                const Real nd = Y[LL + i * L + i].x + Real(0.5);
+                for (int n = 0; n < z; ++n) {
+                    const int k = rk[n * L + i];
+                    const Real tt = rt[n * L + i];
+                    const C a = Y[i * L + k], c = Y[LL + i * L + k];
+                    eth += (a.x + c.x) * tt;
+                    if constexpr (PULSE) {
+                        const C P = Tc[n * L + i];  // t (1 + e^{i phi}); extPot = P - t
+                        epe += (a.x + c.x) * (P.x - tt) - (a.y + c.y) * P.y;
+                    }
+                }
+                eth += s.tdiag[i] * (nu + nd);
+                epe += s.pot[i] * (nu + nd);
+                if (s.coulomb_type == 1) {
+                    eu = s.u_diag[i] * nu * nd;
+                } else if (general) {
+                    Real su = 0, sd = 0, vu = 0, vd = 0;
+                    for (int jj = 0; jj < L; ++jj) {
+                        const Real nuj = Y[jj * L + jj].x + Real(0.5);
+                        const Real ndj = Y[LL + jj * L + jj].x + Real(0.5);
+                        su += s.U[i * L + jj] * ndj;
+                        sd += s.U[i * L + jj] * nuj;
+                        vu += s.V[i * L + jj] * nuj;
+                        vd += s.V[i * L + jj] * ndj;
+                    }
+                    eu = (nu * su + nd * sd) / 2;
+                    ev = (nu * vu + nd * vd) / 2;
+                }
+            }
+            eth = wave_sum(eth);
+            eu = wave_sum(eu);
+            ev = wave_sum(ev);
+            epe = wave_sum(epe);
+            if ((tid & 63) == 0) {
+                ew[(tid >> 6) * 4 + 0] = eth;
+                ew[(tid >> 6) * 4 + 1] = eu;
+                ew[(tid >> 6) * 4 + 2] = ev;
+                ew[(tid >> 6) * 4 + 3] = epe;
+            }
+        }
+        auto flush_energies = [&]() {
+            if (tid == 0) {
+                Real e4[4] = {0, 0, 0, 0};
+                for (int w = 0; w < nwave; ++w)
+                    for (int q = 0; q < 4; ++q) e4[q] += ew[w * 4 + q];
+                for (int q = 0; q < 4; ++q) rr[2 * L + q] = e4[q];
+            }
+        };
+        if (t == r.T - 1) {  // no step after the last record changes an output
+            __syncthreads();
+            flush_energies();
+            break;
+        }
+        for (int sub = 0; sub < r.nsub; ++sub) {
+#pragma unroll 1
+            for (int st = 0; st < 4; ++st, ++gst) {
+                // ---- compute phase ----
+                C cT[ZMAX];
+                if constexpr (PULSE) {
+#pragma unroll
+                    for (int n = 0; n < ZMAX; ++n) {
+                        const C v = (n < z) ? Tc[n * L + j] : C{0, 0};
+                        cT[n] = C{v.x, -v.y};  // T_kj = conj(T_jk)
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int f = tid + NT * m;
+                    if (f >= NE) continue;
+                    const int sg = f >= LL ? 1 : 0;
+                    const int i = (f - sg * LL) / L;
+                    const C* Ys = Y + sg * LL;
+                    const C yij = Ys[i * L + j];
+                    const Real dh = hb[sg * L + j] - hb[sg * L + i];
+                    Real ax = dh * yij.x, ay = dh * yij.y;
+#pragma unroll
+                    for (int n = 0; n < ZMAX; ++n) {
+                        if (n < z) {
+                            const C v = Ys[i * L + ck[n]];
+                            if constexpr (PULSE) {
+                                ax += v.x * cT[n].x - v.y * cT[n].y;
+                                ay += v.x * cT[n].y + v.y * cT[n].x;
+                            } else {
+                                ax += v.x * ct[n];
+                                ay += v.y * ct[n];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int n = 0; n < ZMAX; ++n) {
+                        if (n < z) {
+                            const int k = rk[n * L + i];
+                            const C v = Ys[k * L + j];
+                            if constexpr (PULSE) {
+                                const C T = Tc[n * L + i];
+                                ax -= T.x * v.x - T.y * v.y;
+                                ay -= T.x * v.y + T.y * v.x;
+                            } else {
+                                const Real tt = rt[n * L + i];
+                                ax -= tt * v.x;
+                                ay -= tt * v.y;
+                            }
+                        }
+                    }
+                    const C k = C{ay, -ax};  // -i * (ax + i ay)
+                    if (st == 0) {
+                        A[m] = k;
+                        p[m] = k;
+                    } else if (st == 1) {
+                        A[m] = C{A[m].x + Real(3) * k.x, A[m].y + Real(3) * k.y};
+                        p[m] = C{p[m].x - k.x, p[m].y - k.y};
+                    } else if (st == 2) {
+                        A[m] = C{A[m].x + Real(3) * k.x, A[m].y + Real(3) * k.y};
+                        p[m] = C{p[m].x + k.x, p[m].y + k.y};
+                    } else {
+                        A[m] = C{A[m].x + k.x, A[m].y + k.y};
+                        y[m] = C{y[m].x + hR / Real(8) * A[m].x, y[m].y + hR / Real(8) * A[m].y};
+                    }
+                }
+                if (st == 0 && sub == 0) {
+                    __syncthreads();
+                    flush_energies();
+                } else {
+                    __syncthreads();
+                }
+                // ---- write phase: next stage input ----
+#pragma unroll
+                for (int m = 0; m < EPT; ++m) {
+                    const int f = tid + NT * m;
+                    if (f >= NE) continue;
+                    C yn;
+                    if (st == 0) {
+                        yn = C{y[m].x + hR / Real(3) * p[m].x, y[m].y + hR / Real(3) * p[m].y};
+                    } else if (st == 1) {
+                        yn = C{y[m].x + hR * (A[m].x - Real(3) * p[m].x) / Real(6),
+                               y[m].y + hR * (A[m].y - Real(3) * p[m].y) / Real(6)};
+                    } else if (st == 2) {
+                        yn = C{y[m].x + hR * p[m].x, y[m].y + hR * p[m].y};
+                    } else {
+                        yn = y[m];
+                    }
+                    Y[f] = yn;
+                    if (!general) {
+                        const int sg = f >= LL ? 1 : 0;
+                        const int i = (f - sg * LL) / L;
+                        if (i == j) {
+                            const Real uc = (s.coulomb_type == 1) ? s.u_diag[i] : Real(0);
+                            hb[(1 - sg) * L + i] = uc * (yn.x + Real(0.5)) + s.pdiag[i];
+                        }
+                    }
+                }
+                pulse_table(gst + 1);
+                __syncthreads();
+                if (general) {
+                    h_phase();
+                    __syncthreads();
+                }
+            }
+        }
+    }
+    C* go = reinterpret_cast<C*>(state) + b * NE;
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int f = tid + NT * m;
+        if (f < NE) go[f] = y[m];
+    }
+}
+
+// ---- plain FP64 path ---------------------------------------------------------------------
+//
+// The reference's arithmetic as a straightforward GPU program: one workgroup per trajectory,
+// the dense L x L Delta built per stage (src/FHfTWASolver.cpp:571-596), the two dense products
+// rho*Delta - Delta*rho read from global memory (:598-599), the RK4 "3/8" combinations written back
+// (:644-660), and one launch per time point.  It is the FP64 baseline the batched engine is timed
+// against, and the GPU twin of the CPU oracle.
+#if FTWA_REAL != 1
+constexpr int kPlainThreads = 256;
+
+__device__ __forceinline__ void record_energies_tail(double* rr, int L, double* ew, int nwave)
+{
+    if (threadIdx.x == 0) {
+        double e4[4] = {0, 0, 0, 0};
+        for (int w = 0; w < nwave; ++w)
+            for (int q = 0; q < 4; ++q) e4[q] += ew[w * 4 + q];
+        for (int q = 0; q < 4; ++q) rr[2 * L + q] = e4[q];
+    }
+}
+
+__global__ void __launch_bounds__(kPlainThreads)
+    plain_kernel(DevSystem<double> s, RunShape r, int t, double* __restrict__ state, double* __restrict__ work,
+                 double* __restrict__ rec)
+{
+    __shared__ double ew[(kPlainThreads / 64) * 4];
+    const int L = s.L;
+    const int LL = L * L;
+    const int NE = 2 * LL;
+    const int tid = threadIdx.x;
+    const int NT = blockDim.x;
+    const int64_t b = blockIdx.x;
+    double* y = state + b * NE * 2;
+    double* wk = work + b * static_cast<int64_t>(plain_work_doubles(L));
+    double* Ys = wk;
+    double* K = Ys + NE * 2;       // [4][NE][2]
+    double* D = K + 4 * NE * 2;    // [2][L][L][2]
+    double* hc = D + NE * 2;       // [2][L][2]
+    const bool general = s.coulomb_type == 2;
+    const double h = r.h;
+
+    // ---- record (src/FHfTWASolver.cpp:366-418) ----
+    double* rr = rec + b * r.R;
+    {
+        for (int e = tid; e < 2 * L; e += NT) {
+            const int sg = e / L, i = e - sg * L;
+            rr[e] = y[2 * (sg * LL + i * L + i)];
+        }
+        if (tid == 0) {
+            rr[2 * L + 4] = y[2 * (L - 1)];
+            rr[2 * L + 5] = y[2 * (L - 1) + 1];
+            rr[2 * L + 6] = y[2 * (LL + L - 1)];
+            rr[2 * L + 7] = y[2 * (LL + L - 1) + 1];
+        }
+        double eth = 0, eu = 0, ev = 0, epe = 0;
+        const double a = s.amp ? s.amp[static_cast<int64_t>(t) * r.nsub * 4] : 0.0;
+        for (int i = tid; i < L; i += NT) {
+            for (int jj = 0; jj < L; ++jj) {
+                const double nr = y[2 * (i * L + jj)] + y[2 * (LL + i * L + jj)] + (i == jj ? 1.0 : 0.0);
+                const double ni = y[2 * (i * L + jj) + 1] + y[2 * (LL + i * L + jj) + 1];
+                const double tt = s.tM[i * L + jj];
+                eth += nr * tt;
+                if (s.amp && tt != 0.0) {
+                    double sn, cs;
+                    sincos(a * s.pd[i * L + jj], &sn, &cs);
+                    epe += nr * tt * cs - ni * tt * sn;
+                }
+            }
+            const double nu = y[2 * (i * L + i)] + 0.5, nui = y[2 * (i * L + i) + 1];
+            const double nd = y[2 * (LL + i * L + i)] + 0.5, ndi = y[2 * (LL + i * L + i) + 1];
+            epe += s.pot[i] * (nu + nd);
+            if (s.coulomb_type != 0) {
+                for (int jj = 0; jj < L; ++jj) {
+                    const double nuj = y[2 * (jj * L + jj)] + 0.5, nujj = y[2 * (jj * L + jj) + 1];
+                    const double ndj = y[2 * (LL + jj * L + jj)] + 0.5, ndjj = y[2 * (LL + jj * L + jj) + 1];
+                    const double u = s.U[i * L + jj];
+                    eu += u * ((nu * ndj - nui * ndjj) + (nd * nuj - ndi * nujj)) / 2.0;
+                    if (general) {
+                        const double v = s.V[i * L + jj];
+                        ev += v * ((nu * nuj - nui * nujj) + (nd * ndj - ndi * ndjj)) / 2.0;
+                    }
+                }
+            }
+        }
+        eth = wave_sum(eth);
+        eu = wave_sum(eu);
+        ev = wave_sum(ev);
+        epe = wave_sum(epe);
+        if ((tid & 63) == 0) {
+            ew[(tid >> 6) * 4 + 0] = eth;
+            ew[(tid >> 6) * 4 + 1] = eu;
+            ew[(tid >> 6) * 4 + 2] = ev;
+            ew[(tid >> 6) * 4 + 3] = epe;
+        }
+        __syncthreads();
+        record_energies_tail(rr, L, ew, (NT + 63) / 64);
+    }
+    if (t == r.T - 1) return;
+
+    for (int sub = 0; sub < r.nsub; ++sub) {
+        const int64_t g = static_cast<int64_t>(t) * r.nsub + sub;
+        for (int st = 0; st < 4; ++st) {
+            const double* Yin = (st == 0) ? y : Ys;
+            // Hartree potential, complex as in the reference (U (rho_diag + 1/2))
+            for (int e = tid; e < 2 * L; e += NT) {
+                const int sg = e / L, i = e - sg * L;
+                double hr = 0.0, hi = 0.0;
+                if (s.coulomb_type != 0) {
+                    for (int jj = 0; jj < L; ++jj) {
+                        const double u = s.U[i * L + jj];
+                        hr += u * (Yin[2 * ((1 - sg) * LL + jj * L + jj)] + 0.5);
+                        hi += u * Yin[2 * ((1 - sg) * LL + jj * L + jj) + 1];
+                    }
+                    if (general)
+                        for (int jj = 0; jj < L; ++jj) {
+                            const double v = s.V[i * L + jj];
+                            hr += v * (Yin[2 * (sg * LL + jj * L + jj)] + 0.5);
+                            hi += v * Yin[2 * (sg * LL + jj * L + jj) + 1];
+                        }
+                }
+                hc[2 * e] = hr + s.pot[i];
+                hc[2 * e + 1] = hi;
+            }
+            __syncthreads();
+            const double a = s.amp ? s.amp[g * 4 + st] : 0.0;
+            for (int e = tid; e < NE; e += NT) {
+                const int sg = e / LL, ij = e - sg * LL, i = ij / L, jj = ij - i * L;
+                const double tt = s.tM[ij];
+                double dr = tt, di = 0.0;
+                if (i == jj) {
+                    dr += hc[2 * (sg * L + i)];
+                    di += hc[2 * (sg * L + i) + 1];
+                }
+                if (s.amp && tt != 0.0) {
+                    double sn, cs;
+                    sincos(a * s.pd[ij], &sn, &cs);
+                    dr += tt * cs;
+                    di += tt * sn;
+                }
+                D[2 * e] = dr;
+                D[2 * e + 1] = di;
+            }
+            __syncthreads();
+            double* Ko = K + st * NE * 2;
+            for (int e = tid; e < NE; e += NT) {
+                const int sg = e / LL, ij = e - sg * LL, i = ij / L, jj = ij - i * L;
+                const double* Ym = Yin + 2 * sg * LL;
+                const double* Dm = D + 2 * sg * LL;
+                double ar = 0, ai = 0, br = 0, bi = 0;
+                for (int k = 0; k < L; ++k) {
+                    const double r1r = Ym[2 * (i * L + k)], r1i = Ym[2 * (i * L + k) + 1];
+                    const double d1r = Dm[2 * (k * L + jj)], d1i = Dm[2 * (k * L + jj) + 1];
+                    ar += r1r * d1r - r1i * d1i;
+                    ai += r1r * d1i + r1i * d1r;
+                    const double d2r = Dm[2 * (i * L + k)], d2i = Dm[2 * (i * L + k) + 1];
+                    const double r2r = Ym[2 * (k * L + jj)], r2i = Ym[2 * (k * L + jj) + 1];
+                    br += d2r * r2r - d2i * r2i;
+                    bi += d2r * r2i + d2i * r2r;
+                }
+                Ko[2 * e] = ai - bi;       // -i (c_r + i c_i) = c_i - i c_r
+                Ko[2 * e + 1] = -(ar - br);
+            }
+            __syncthreads();
+            const double* K1 = K;
+            const double* K2 = K + NE * 2;
+            const double* K3 = K + 2 * NE * 2;
+            const double* K4 = K + 3 * NE * 2;
+            for (int e = tid; e < 2 * NE; e += NT) {
+                if (st == 0)
+                    Ys[e] = y[e] + K1[e] * h / 3.0;
+                else if (st == 1)
+                    Ys[e] = y[e] + (-K1[e] / 3.0 + K2[e]) * h;
+                else if (st == 2)
+                    Ys[e] = y[e] + (K1[e] - K2[e] + K3[e]) * h;
+                else
+                    y[e] = y[e] + (K1[e] + K2[e] * 3.0 + K3[e] * 3.0 + K4[e]) * h / 8.0;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+#endif
+
+// ---- ensemble reduction ---------------------------------------------------------------------
+struct Neumaier {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double x)
+    {
+        const double t = s + x;
+        if (fabs(s) >= fabs(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+};
+
+template <typename Real>
+__global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, const Real* __restrict__ rec,
+                                      double* __restrict__ part)
+{
+    const int t = blockIdx.x;
+    const int c = blockIdx.y;
+    const int64_t CB = (B + nchunk - 1) / nchunk;
+    const int64_t bb0 = static_cast<int64_t>(c) * CB;
+    const int64_t bb1 = (bb0 + CB < B) ? bb0 + CB : B;
+    const int Q = 6 * L + 12;
+    const Real* rt = rec + static_cast<int64_t>(t) * B * R;
+    double* po = part + (static_cast<int64_t>(t) * nchunk + c) * Q * 2;
+    for (int slot = threadIdx.x; slot < L + 8; slot += blockDim.x) {
+        if (slot < L) {
+            Neumaier su, sd, suu, sdd, sud, snn;
+            for (int64_t bi = bb0; bi < bb1; ++bi) {
+                const double u = static_cast<double>(rt[bi * R + slot]);
+                const double d = static_cast<double>(rt[bi * R + L + slot]);
+                su.add(u);
+                sd.add(d);
+                suu.add(u * u);
+                sdd.add(d * d);
+                sud.add(u * d);
+                const double nn = (u + 0.5) * (d + 0.5);
+                snn.add(nn * nn);
+            }
+            const Neumaier* all[6] = {&su, &sd, &suu, &sdd, &sud, &snn};
+            for (int k = 0; k < 6; ++k) {
+                po[(6 * slot + k) * 2] = all[k]->s;
+                po[(6 * slot + k) * 2 + 1] = all[k]->c;
+            }
+        } else if (slot < L + 4) {
+            const int k = slot - L;
+            Neumaier se, se2;
+            for (int64_t bi = bb0; bi < bb1; ++bi) {
+                const double e = static_cast<double>(rt[bi * R + 2 * L + k]);
+                se.add(e);
+                se2.add(e * e);
+            }
+            po[(6 * L + k) * 2] = se.s;
+            po[(6 * L + k) * 2 + 1] = se.c;
+            po[(6 * L + 4 + k) * 2] = se2.s;
+            po[(6 * L + 4 + k) * 2 + 1] = se2.c;
+        } else {
+            const int k = slot - L - 4;
+            Neumaier sc;
+            for (int64_t bi = bb0; bi < bb1; ++bi) sc.add(static_cast<double>(rt[bi * R + 2 * L + 4 + k]));
+            po[(6 * L + 8 + k) * 2] = sc.s;
+            po[(6 * L + 8 + k) * 2 + 1] = sc.c;
+        }
+    }
+}
+
+__global__ void reduce_final_kernel(int Q, int nchunk, int nt, int t0, const double* __restrict__ part,
+                                    double* __restrict__ acc)
+{
+    const int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (idx >= static_cast<int64_t>(nt) * Q) return;
+    const int t = static_cast<int>(idx / Q);
+    const int q = static_cast<int>(idx - static_cast<int64_t>(t) * Q);
+    Neumaier a;
+    for (int c = 0; c < nchunk; ++c) {
+        const double* pp = part + ((static_cast<int64_t>(t) * nchunk + c) * Q + q) * 2;
+        a.add(pp[0]);
+        a.add(pp[1]);
+    }
+    double* o = acc + (static_cast<int64_t>(t0 + t) * Q + q) * 2;
+    o[0] = a.s;
+    o[1] = a.c;
+}
+
+// ---- stencil dispatch ------------------------------------------------------------------------
+constexpr int kEpts[] = {1, 2, 4, 8, 16};
+
+constexpr int launch_bound(int ept) { return ept >= 32 ? 256 : (ept >= 16 ? 512 : 1024); }
+
+template <typename Real>
+size_t stencil_lds_bytes(int L, int z, int nt, bool pulse)
+{
+    const size_t C = 2 * sizeof(Real);
+    const int nwave = (nt + 63) / 64;
+    return 2 * static_cast<size_t>(L) * L * C + (pulse ? static_cast<size_t>(z) * L * C : 0) +
+           2 * static_cast<size_t>(L) * sizeof(Real) + static_cast<size_t>(z) * L * sizeof(Real) +
+           static_cast<size_t>(nwave) * 4 * sizeof(Real) + static_cast<size_t>(z) * L * sizeof(int);
+}
+
+constexpr size_t kLdsLimit = 160 * 1024;
+
+// Block size for (L, ept): a multiple of L covering 2 L^2 elements in ept slots per thread.
+inline int nt_for(int L, int ept) { return L * ((2 * L + ept - 1) / ept); }
+
+int choose_ept(int L, int nt_hint, int* nt)
+{
+    if (nt_hint > 0 && nt_hint % L == 0 && nt_hint >= 2 * L) {
+        const int need = (2 * L * L + nt_hint - 1) / nt_hint;
+        for (int e : kEpts)
+            if (e >= need && nt_hint <= launch_bound(e)) {
+                *nt = nt_hint;
+                return e;
+            }
+    }
+    for (int e : kEpts) {
+        const int n = nt_for(L, e);
+        if (n <= launch_bound(e)) {
+            *nt = n;
+            return e;
+        }
+    }
+    return 0;
+}
+
+template <typename Real, int EPT, int ZMAX, bool PULSE>
+hipError_t run_stencil(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec, int nt,
+                       hipStream_t st)
+{
+    auto kern = stencil_kernel<Real, EPT, ZMAX, PULSE>;
+    const size_t lds = stencil_lds_bytes<Real>(s.L, s.z, nt, PULSE);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(nt), lds, st, s, r, t0, t1, state, rec);
+    return hipGetLastError();
+}
+
+template <typename Real, int EPT, bool PULSE>
+hipError_t run_stencil_z(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec, int nt,
+                         hipStream_t st)
+{
+    if (s.z <= 4) return run_stencil<Real, EPT, 4, PULSE>(s, r, t0, t1, state, rec, nt, st);
+    if (s.z <= 8) return run_stencil<Real, EPT, 8, PULSE>(s, r, t0, t1, state, rec, nt, st);
+    return run_stencil<Real, EPT, 16, PULSE>(s, r, t0, t1, state, rec, nt, st);
+}
+
+template <typename Real, bool PULSE>
+hipError_t run_stencil_e(int ept, const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
+                         int nt, hipStream_t st)
+{
+    switch (ept) {
+    case 1: return run_stencil_z<Real, 1, PULSE>(s, r, t0, t1, state, rec, nt, st);
+    case 2: return run_stencil_z<Real, 2, PULSE>(s, r, t0, t1, state, rec, nt, st);
+    case 4: return run_stencil_z<Real, 4, PULSE>(s, r, t0, t1, state, rec, nt, st);
+    case 8: return run_stencil_z<Real, 8, PULSE>(s, r, t0, t1, state, rec, nt, st);
+    case 16: return run_stencil_z<Real, 16, PULSE>(s, r, t0, t1, state, rec, nt, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+inline unsigned grid_for(int64_t total, int block)
+{
+    int64_t g = (total + block - 1) / block;
+    if (g > (1 << 20)) g = 1 << 20;
+    if (g < 1) g = 1;
+    return static_cast<unsigned>(g);
+}
+
+}  // namespace
+
+template <typename Real>
+bool stencil_fits(int L, int z, int* nt)
+{
+    if (L < 2 || z > 16) return false;
+    int n = 0;
+    const int e = choose_ept(L, *nt, &n);
+    if (!e) return false;
+    if (stencil_lds_bytes<Real>(L, z, n, true) > kLdsLimit) return false;
+    *nt = n;
+    return true;
+}
+
+template <typename Real>
+hipError_t launch_stencil(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
+                          int nt_hint, hipStream_t st)
+{
+    int nt = 0;
+    const int ept = choose_ept(s.L, nt_hint, &nt);
+    if (!ept || s.z > 16 || r.B <= 0) return hipErrorInvalidValue;
+    if (stencil_lds_bytes<Real>(s.L, s.z, nt, s.amp != nullptr) > kLdsLimit) return hipErrorInvalidValue;
+    if (s.amp) return run_stencil_e<Real, true>(ept, s, r, t0, t1, state, rec, nt, st);
+    return run_stencil_e<Real, false>(ept, s, r, t0, t1, state, rec, nt, st);
+}
+
+template <typename Real>
+hipError_t launch_sample(const DevSystem<Real>& s, const double* n_up, const double* n_dn, const double* noise,
+                         uint64_t seed, int64_t b0, int64_t B, Real* state, hipStream_t st)
+{
+    const int64_t total = B * 2 * static_cast<int64_t>(s.L) * s.L;
+    hipLaunchKernelGGL(sample_kernel<Real>, dim3(grid_for(total, 256)), dim3(256), 0, st, s.L, n_up, n_dn, noise, seed,
+                       b0, B, state);
+    return hipGetLastError();
+}
+
+#if FTWA_REAL != 1
+hipError_t launch_noise(int L, uint64_t seed, int64_t b0, int64_t B, double* out, hipStream_t st)
+{
+    const int64_t total = B * 2 * (static_cast<int64_t>(L) * (L - 1) / 2);
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(noise_kernel, dim3(grid_for(total, 256)), dim3(256), 0, st, L, seed, b0, B, out);
+    return hipGetLastError();
+}
+#endif
+
+template <typename Real>
+hipError_t launch_rotate(int L, const double* rot, int64_t B, Real* state, Real* scratch, hipStream_t st)
+{
+    hipLaunchKernelGGL(rotate_kernel<Real>, dim3(static_cast<unsigned>(B * 2)), dim3(256), 0, st, L, rot, state,
+                       scratch);
+    return hipGetLastError();
+}
+
+#if FTWA_REAL != 1
+hipError_t launch_plain(const DevSystem<double>& s, const RunShape& r, int t, double* state, double* work, double* rec,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL(plain_kernel, dim3(static_cast<unsigned>(r.B)), dim3(kPlainThreads), 0, st, s, r, t, state,
+                       work, rec);
+    return hipGetLastError();
+}
+#endif
+
+#if FTWA_REAL != 1
+int reduce_chunks(int64_t B)
+{
+    int64_t n = (B + 1023) / 1024;
+    if (n > 256) n = 256;
+    if (n < 1) n = 1;
+    return static_cast<int>(n);
+}
+#endif
+
+template <typename Real>
+hipError_t launch_reduce(int L, int64_t B, int R, int nt, int t0, const Real* rec, double* part, int nchunk,
+                         double* acc, hipStream_t st)
+{
+    const int Q = 6 * L + 12;
+    const int threads = ((L + 8 + 63) / 64) * 64 > 256 ? 256 : ((L + 8 + 63) / 64) * 64;
+    hipLaunchKernelGGL(reduce_partial_kernel<Real>, dim3(nt, nchunk), dim3(threads), 0, st, L, B, R, nchunk, rec,
+                       part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t total = static_cast<int64_t>(nt) * Q;
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(grid_for(total, 256)), dim3(256), 0, st, Q, nchunk, nt, t0, part, acc);
+    return hipGetLastError();
+}
+
+// explicit instantiations: FTWA_REAL selects the precision of this translation unit (1 = float,
+// 2 = double); the FP64-only entry points live in the double unit.
+#if FTWA_REAL == 1
+template bool stencil_fits<float>(int, int, int*);
+template hipError_t launch_stencil<float>(const DevSystem<float>&, const RunShape&, int, int, float*, float*, int,
+                                          hipStream_t);
+template hipError_t launch_sample<float>(const DevSystem<float>&, const double*, const double*, const double*, uint64_t,
+                                         int64_t, int64_t, float*, hipStream_t);
+template hipError_t launch_rotate<float>(int, const double*, int64_t, float*, float*, hipStream_t);
+template hipError_t launch_reduce<float>(int, int64_t, int, int, int, const float*, double*, int, double*, hipStream_t);
+#else
+template bool stencil_fits<double>(int, int, int*);
+template hipError_t launch_stencil<double>(const DevSystem<double>&, const RunShape&, int, int, double*, double*, int,
+                                           hipStream_t);
+template hipError_t launch_sample<double>(const DevSystem<double>&, const double*, const double*, const double*,
+                                          uint64_t, int64_t, int64_t, double*, hipStream_t);
+template hipError_t launch_rotate<double>(int, const double*, int64_t, double*, double*, hipStream_t);
+template hipError_t launch_reduce<double>(int, int64_t, int, int, int, const double*, double*, int, double*,
+                                          hipStream_t);
+#endif
+
+}  // namespace fpsamd

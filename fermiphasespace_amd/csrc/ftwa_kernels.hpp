@@ -1,0 +1,88 @@
+// ftwa_kernels.hpp -- host-side interface to the gfx950 fTWA kernels (ftwa_kernels.hip).
+//
+// Data layout in HBM (one handle):
+//   state  [B][2 spins][L][L] complex<Real>, row-major, interleaved (re, im)
+//   rec    [Tc][B][R] Real, R = 2L + 8 per-trajectory records of one time point:
+//            [0,L) rho_up_ii  [L,2L) rho_dn_ii  2L+0..3 E_tH, E_U, E_V, E_pe  2L+4..7 corner re/im up, dn
+//   part   [Tc][nchunk][Q][2] double   partial ensemble sums (sum, compensation)
+//   acc    [T][Q][2] double            ensemble sums (sum, compensation), Q = 6L + 12:
+//            per site i: S_u, S_d, S_uu, S_dd, S_ud, S_(nu nd)^2 at [6i .. 6i+5];
+//            6L + k (k<4): S_Ek;  6L + 4 + k: S_Ek^2;  6L + 8 + k (k<4): corner sums
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fpsamd {
+
+// Device view of the static system.  Neighbour lists are ELL with z entries per site,
+// stored entry-major ([n][L]) so consecutive sites are consecutive addresses; unused entries
+// point at the site itself with zero hopping.
+template <typename Real>
+struct DevSystem {
+    int L = 0;
+    int z = 0;
+    int coulomb_type = 0;          // 0 null, 1 onSite, 2 general
+    const int* nbr_k = nullptr;    // [z][L]  neighbour k of site i (tM_ik != 0, k != i)
+    const Real* nbr_t = nullptr;   // [z][L]  tM_ik
+    const Real* nbr_pd = nullptr;  // [z][L]  pol . (r_i - r_k)  (pulse phase per unit amplitude)
+    const Real* u_diag = nullptr;  // [L]     U_ii
+    const Real* U = nullptr;       // [L][L]  dense U (general)
+    const Real* V = nullptr;       // [L][L]  dense V (general)
+    const Real* pdiag = nullptr;   // [L]     constant diagonal potential + tM_ii (enters h)
+    const Real* tdiag = nullptr;   // [L]     tM_ii (energy bookkeeping)
+    const Real* pot = nullptr;     // [L]     constant diagonal potential (energy bookkeeping)
+    const double* amp = nullptr;   // [T*nsub*4] pulse amplitude a(tau) at each RK stage time, or null
+    const Real* tM = nullptr;      // [L][L]  dense hopping (plain path)
+    const Real* pd = nullptr;      // [L][L]  pol . (r_i - r_j) dense (plain path, pulse)
+};
+
+struct RunShape {
+    int64_t B = 0;        // trajectories in this launch
+    int64_t b0 = 0;       // global index of the first trajectory (noise counter)
+    int T = 0;            // time points of the whole run
+    int nsub = 1;
+    double h = 0.0;       // dt / nsub
+    int R = 0;            // record width
+};
+
+// Sampling: rho_sigma = diag(n - 1/2) + S.W (rotation applied separately).  noise (optional,
+// FP64 [B][2][npairs][2]) replaces the counter-based generator.
+template <typename Real>
+hipError_t launch_sample(const DevSystem<Real>& s, const double* n_up, const double* n_dn, const double* noise,
+                         uint64_t seed, int64_t b0, int64_t B, Real* state, hipStream_t st);
+
+// Export of the generator's noise (FP64, layout as above).
+hipError_t launch_noise(int L, uint64_t seed, int64_t b0, int64_t B, double* out, hipStream_t st);
+
+// rho <- R rho R^dagger per trajectory and spin (R FP64 complex [2][L][L]).
+template <typename Real>
+hipError_t launch_rotate(int L, const double* rot, int64_t B, Real* state, Real* scratch, hipStream_t st);
+
+// Batched LDS stencil integrator: time points [t0, t1) for B trajectories, recording each.
+// Returns hipErrorInvalidValue when the shape does not fit the kernel family.
+template <typename Real>
+hipError_t launch_stencil(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
+                          int nt_hint, hipStream_t st);
+// Whether the stencil kernel supports this L (LDS capacity) and its chosen block size.
+template <typename Real>
+bool stencil_fits(int L, int z, int* nt);
+
+// Plain FP64 path: one launch per time point, dense Delta, global-memory products.
+hipError_t launch_plain(const DevSystem<double>& s, const RunShape& r, int t, double* state, double* work,
+                        double* rec, hipStream_t st);
+// Per-trajectory FP64 scratch of the plain path: stage input, k1..k4, Delta (2 spins each) and h.
+__host__ __device__ inline size_t plain_work_doubles(int L)
+{
+    const size_t LL = static_cast<size_t>(L) * L;
+    return 6 * 2 * LL * 2 + 4 * static_cast<size_t>(L);
+}
+
+// Ensemble reduction of records for time points [t0, t0+nt) into acc.
+template <typename Real>
+hipError_t launch_reduce(int L, int64_t B, int R, int nt, int t0, const Real* rec, double* part, int nchunk,
+                         double* acc, hipStream_t st);
+int reduce_chunks(int64_t B);
+
+}  // namespace fpsamd

@@ -1,0 +1,357 @@
+// solver.cpp -- FHSystem / FHfTWASolver facade, the results-file writer, and the JSON-level
+// C ABI entry points (fpsftwa_run_json_file, fpsftwa_load_system_json).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "../../include/fps_ftwa_solver.hpp"
+#include "ftwa_internal.hpp"
+
+using fpsjson::Value;
+
+namespace {
+
+// Version of the reference format the results file follows (include/FHSystem.h:21-23).
+const char* kFormatVersion = "0.5.0";
+
+Value rowOf(const std::vector<double>& a, size_t row, size_t T)
+{
+    Value v = Value::array();
+    for (size_t t = 0; t < T; ++t) v.append(a[row * T + t]);
+    return v;
+}
+
+Value rowsOf(const std::vector<double>& a, size_t rows, size_t T)
+{
+    Value v = Value::array();
+    for (size_t r = 0; r < rows; ++r) v.append(rowOf(a, r, T));
+    return v;
+}
+
+std::vector<double> fetch(fpsftwa_handle* h, int which)
+{
+    size_t n = 0;
+    if (fpsftwa_get_result(h, which, nullptr, 0, &n) != FPSFTWA_OK) return {};
+    std::vector<double> v(n);
+    if (fpsftwa_get_result(h, which, v.data(), n, nullptr) != FPSFTWA_OK) return {};
+    return v;
+}
+
+const char* engineName(int e)
+{
+    switch (e) {
+    case FPSFTWA_ENGINE_FP64_PLAIN: return "fp64_plain";
+    case FPSFTWA_ENGINE_FP64_BATCHED: return "fp64";
+    default: return "fp32";
+    }
+}
+
+}  // namespace
+
+FHSystem::FHSystem(const Value& systemJson, const std::string& folderName)
+{
+    std::string err;
+    if (!fpsamd::buildSystem(systemJson, folderName, sys_, &err)) throw std::runtime_error(err);
+}
+
+FHfTWASolver::FHfTWASolver(FHSystem* fhSystem) : sys_(fhSystem)
+{
+    if (!sys_) throw std::invalid_argument("FHfTWASolver needs a system");
+}
+
+FHfTWASolver::FHfTWASolver(FHfTWASolver&& o) noexcept
+    : sys_(o.sys_), h_(o.h_), json_(std::move(o.json_)), sim_(std::move(o.sim_)), info_(std::move(o.info_)),
+      workName_(std::move(o.workName_)), folderName_(std::move(o.folderName_)), writeJson_(o.writeJson_),
+      talkative_(o.talkative_), initiated_(o.initiated_), tMax_(o.tMax_), dt_(o.dt_), nSub_(o.nSub_), nTraj_(o.nTraj_),
+      flags_(o.flags_), seed_(o.seed_), engine_(o.engine_)
+{
+    o.h_ = nullptr;
+}
+
+FHfTWASolver::~FHfTWASolver()
+{
+    if (h_) fpsftwa_destroy(h_);
+}
+
+void FHfTWASolver::check(int rc, const char* what) const
+{
+    if (rc != FPSFTWA_OK) throw std::runtime_error(std::string(what) + ": " + fpsftwa_last_error());
+}
+
+void FHfTWASolver::setIntegratorOrder(int order)
+{
+    if (order != 2 && order != 4)
+        std::cout << "Integrator order " << order << " not implemented. Switching to default 2." << std::endl;
+}
+
+void FHfTWASolver::setfTWAParameters(unsigned int nTraj) { nTraj_ = nTraj; }
+
+void FHfTWASolver::setTimeParameters(double t_max, double dt, unsigned nPerTS)
+{
+    tMax_ = t_max;
+    dt_ = dt;
+    nSub_ = nPerTS;
+}
+
+void FHfTWASolver::setMethodParameters(const Value& p)
+{
+    const double dt = p.get("dt", Value(0.01)).asDouble();
+    const double tm = p.get("t_max", Value(2.0)).asDouble();
+    // the reference reads "stepsParTimePoints" (src/FHfTWASolver.cpp:53); the documented
+    // "stepsPerTimePoints" is ignored there, and here too
+    const int64_t sptp = p.get("stepsParTimePoints", Value(1)).asInt();
+    if (p.isMember("stepsPerTimePoints") && !p.isMember("stepsParTimePoints"))
+        std::cout << "   -- note: 'stepsPerTimePoints' is not read by the fTWA solver; 'stepsParTimePoints' is."
+                  << std::endl;
+    const int64_t nTraj = p.get("n_traj", Value(10)).asInt();
+    setTimeParameters(tm, dt, static_cast<unsigned>(std::max<int64_t>(1, sptp)));
+    setIntegratorOrder(static_cast<int>(p.get("int_order", Value(2)).asInt()));
+    setfTWAParameters(static_cast<unsigned>(std::max<int64_t>(1, nTraj)));
+    const Value& m = p["mi355x"];
+    if (m.isObject()) {
+        seed_ = static_cast<uint64_t>(m.get("seed", Value(0)).asInt());
+        const std::string e = m.get("engine", Value("fp64")).asString();
+        if (e == "fp32") engine_ = FPSFTWA_ENGINE_FP32_BATCHED;
+        else if (e == "fp64_plain") engine_ = FPSFTWA_ENGINE_FP64_PLAIN;
+        else engine_ = FPSFTWA_ENGINE_FP64_BATCHED;
+    }
+}
+
+void FHfTWASolver::setResultsToSave(const Value& res)
+{
+    uint32_t f = 0;
+    if (res.get("diagDMatrix", Value(true)).asBool()) f |= FPSFTWA_RES_DIAG;
+    if (res.get("diag2DMatrix", Value(false)).asBool()) f |= FPSFTWA_RES_DIAG2;
+    if (res.get("onSiteDoublons", Value(false)).asBool()) f |= FPSFTWA_RES_DOUBLONS;
+    if (res.get("wholeDMatrix", Value(false)).asBool()) f |= FPSFTWA_RES_WHOLE;
+    if (res.get("energies", Value(true)).asBool()) f |= FPSFTWA_RES_ENERGIES;
+    if (res.get("tbStates", Value(false)).asBool() || res.get("mfStates", Value(false)).asBool())
+        throw std::runtime_error("tbStates / mfStates (momentum-space occupations) are not supported by this build");
+    flags_ = f;
+}
+
+void FHfTWASolver::setResultFile(std::string workName, std::string folderName)
+{
+    workName_ = std::move(workName);
+    folderName_ = std::move(folderName);
+    talkative_ = false;
+}
+
+void FHfTWASolver::setResultsWriter(std::string how, bool yn)
+{
+    if (how == "json") writeJson_ = yn;
+    else if (how == "txt" || how == "hdf5")
+        std::cout << "   -- note: the '" << how << "' writer is not provided; use json." << std::endl;
+}
+
+void FHfTWASolver::setJsonFile(const Value& file) { json_ = file; }
+
+void FHfTWASolver::initiateSolver()
+{
+    std::cout << "   -- fTWA Solver initiation." << std::endl;
+    const fpsamd::FtwaSystem& s = sys_->data();
+    if (!h_) {
+        fpsftwa_system d{};
+        d.n_sites = s.L;
+        d.hopping = s.tM.data();
+        d.coulomb_type = s.coulomb_type;
+        d.coulomb_u = s.U.data();
+        d.coulomb_v = s.V.data();
+        d.potential = s.potential.empty() ? nullptr : s.potential.data();
+        d.pulse_type = s.pulse_type;
+        d.pulse_A0 = s.pulse_A0;
+        d.pulse_t0 = s.pulse_t0;
+        d.pulse_td = s.pulse_td;
+        d.pulse_w0 = s.pulse_w0;
+        d.pulse_phi = s.pulse_phi;
+        for (int k = 0; k < 3; ++k) d.pulse_pol[k] = s.pulse_pol[k];
+        d.pulse_n_knots = static_cast<int32_t>(s.pulse_knots.size());
+        d.pulse_knots = s.pulse_knots.empty() ? nullptr : s.pulse_knots.data();
+        d.displacement = s.displacement.data();
+        check(fpsftwa_create(&d, &h_), "fpsftwa_create");
+    }
+    check(fpsftwa_set_initial_state(h_, s.n_up.data(), s.n_dn.data(), nullptr, nullptr), "initial state");
+    initiated_ = true;
+}
+
+void FHfTWASolver::runSimulation()
+{
+    if (!initiated_) initiateSolver();
+    std::cout << "   -- Start simulation." << std::endl;
+    check(fpsftwa_set_params(h_, dt_, tMax_, static_cast<int32_t>(nSub_), nTraj_, 4, seed_, engine_), "parameters");
+    check(fpsftwa_set_results(h_, flags_), "results");
+    check(fpsftwa_run(h_), "run");
+    const int L = sys_->data().L;
+    std::vector<double> t = fetch(h_, FPSFTWA_OUT_TIME);
+    const size_t T = t.size();
+    Value sim = Value::object();
+    Value tv = Value::array();
+    for (double x : t) tv.append(x);
+    sim["t"] = tv;
+    if (flags_ & FPSFTWA_RES_WHOLE) {  // the single corner element N(0, L-1)(t) (src/FHSolver.cpp:196-197)
+        for (int which : {FPSFTWA_OUT_N1HU, FPSFTWA_OUT_N1HD}) {
+            std::vector<double> c = fetch(h_, which);
+            Value o = Value::object();
+            o["r"] = rowOf(c, 0, T);
+            o["i"] = rowOf(c, 1, T);
+            sim[which == FPSFTWA_OUT_N1HU ? "N1hu" : "N1hd"] = o;
+        }
+    }
+    if (flags_ & FPSFTWA_RES_DIAG) {
+        sim["Niiu"] = rowsOf(fetch(h_, FPSFTWA_OUT_NIIU), L, T);
+        sim["Niid"] = rowsOf(fetch(h_, FPSFTWA_OUT_NIID), L, T);
+    }
+    if (flags_ & FPSFTWA_RES_DIAG2) {
+        sim["N2iiu"] = rowsOf(fetch(h_, FPSFTWA_OUT_N2IIU), L, T);
+        sim["N2iid"] = rowsOf(fetch(h_, FPSFTWA_OUT_N2IID), L, T);
+    }
+    if (flags_ & FPSFTWA_RES_DOUBLONS) sim["NuNd"] = rowsOf(fetch(h_, FPSFTWA_OUT_NUND), L, T);
+    if (flags_ & FPSFTWA_RES_ENERGIES) {
+        std::vector<double> e = fetch(h_, FPSFTWA_OUT_ENERGY);
+        Value E = Value::object();
+        E["tH"] = rowOf(e, 0, T);
+        E["U"] = rowOf(e, 1, T);
+        E["V"] = rowOf(e, 2, T);
+        E["pe"] = rowOf(e, 3, T);
+        E["tot"] = rowOf(e, 4, T);
+        sim["E"] = E;
+    }
+    sim_ = sim;
+    int64_t launches = 0;
+    double kms = 0, rms = 0;
+    fpsftwa_get_timing(h_, &launches, &kms, &rms);
+    info_ = Value::object();
+    info_["engine"] = engineName(engine_);
+    info_["seed"] = static_cast<int64_t>(seed_);
+    info_["kernel_ms"] = kms;
+    info_["run_ms"] = rms;
+    info_["traj_steps_per_s"] =
+        rms > 0 ? static_cast<double>(nTraj_) * static_cast<double>(T) * nSub_ / (rms * 1e-3) : 0.0;
+    info_["energies_note"] = "E is computed for every system (the reference writes E only for core-hole systems)";
+    Value err = Value::object();
+    if (flags_ & FPSFTWA_RES_DIAG) {
+        err["Niiu"] = rowsOf(fetch(h_, FPSFTWA_OUT_NIIU_ERR), L, T);
+        err["Niid"] = rowsOf(fetch(h_, FPSFTWA_OUT_NIID_ERR), L, T);
+    }
+    if (flags_ & FPSFTWA_RES_DOUBLONS) err["NuNd"] = rowsOf(fetch(h_, FPSFTWA_OUT_NUND_ERR), L, T);
+    info_["statistical_error"] = err;
+    if (writeJson_) writeResults();
+}
+
+Value FHfTWASolver::resultsDocument() const
+{
+    Value res = json_;
+    if (!res.isObject()) res = Value::object();
+    res["fpsVersion"] = kFormatVersion;
+    res["simulation"] = sim_;
+    res["mi355x"] = info_;
+    return res;
+}
+
+void FHfTWASolver::writeResults()
+{
+    const std::string path = folderName_ + workName_ + ".json";
+    std::ofstream f(path);
+    if (!f.good()) throw std::runtime_error("cannot write " + path);
+    f << fpsjson::write(resultsDocument(), "\t");
+    std::cout << "   -- Results written in " << folderName_ << workName_ << ".json" << std::endl;
+}
+
+// ---- JSON-level C ABI ---------------------------------------------------------------------------
+namespace {
+
+bool splitPath(const std::string& path, std::string& folder, std::string& file)
+{
+    const size_t p = path.find_last_of('/');
+    folder = (p == std::string::npos) ? "" : path.substr(0, p + 1);
+    file = (p == std::string::npos) ? path : path.substr(p + 1);
+    return true;
+}
+
+}  // namespace
+
+extern "C" int fpsftwa_run_json_file(const char* path, const char* out_path)
+{
+    if (!path) {
+        fpsamd::set_last_error("null path");
+        return FPSFTWA_EINVAL;
+    }
+    try {
+        Value root;
+        std::string err;
+        if (!fpsjson::parseFile(path, root, &err)) {
+            fpsamd::set_last_error(err);
+            return FPSFTWA_EIO;
+        }
+        std::string folder, file;
+        splitPath(path, folder, file);
+        const std::string simName = root.get("sim_name", Value("defaultSimName")).asString();
+        const std::string method = root.get("method", Value("HF")).asString();
+        if (method != "fTWA") {
+            fpsamd::set_last_error("method '" + method + "' is not provided by this engine (only fTWA)");
+            return FPSFTWA_EUNSUPPORTED;
+        }
+        FHSystem sys(root["system"], folder);
+        for (const auto& w : sys.data().warnings) std::cout << "   -- note: " << w << std::endl;
+        FHfTWASolver solver(&sys);
+        std::string work = simName + "_fTWA_res";
+        std::string outFolder = folder;
+        if (out_path) {
+            std::string f2, w2;
+            splitPath(out_path, f2, w2);
+            outFolder = f2;
+            work = w2.size() > 5 && w2.compare(w2.size() - 5, 5, ".json") == 0 ? w2.substr(0, w2.size() - 5) : w2;
+        }
+        solver.setResultFile(work, outFolder);
+        solver.setJsonFile(root);
+        solver.setResultsWriter("json");
+        solver.setResultsToSave(root["results"]);
+        solver.setMethodParameters(root["parameters"]);
+        solver.initiateSolver();
+        solver.runSimulation();
+    } catch (const std::exception& e) {
+        fpsamd::set_last_error(e.what());
+        return FPSFTWA_EINVAL;
+    }
+    return FPSFTWA_OK;
+}
+
+extern "C" int fpsftwa_load_system_json(const char* path, int32_t* n_sites, int32_t* coulomb_type, double* hopping,
+                                        double* coulomb_u, double* coulomb_v, double* potential, double* n_up,
+                                        double* n_dn, double* positions)
+{
+    if (!path || !n_sites) {
+        fpsamd::set_last_error("null argument");
+        return FPSFTWA_EINVAL;
+    }
+    Value root;
+    std::string err;
+    if (!fpsjson::parseFile(path, root, &err)) {
+        fpsamd::set_last_error(err);
+        return FPSFTWA_EIO;
+    }
+    std::string folder, file;
+    splitPath(path, folder, file);
+    fpsamd::FtwaSystem s;
+    if (!fpsamd::buildSystem(root["system"], folder, s, &err)) {
+        fpsamd::set_last_error(err);
+        return FPSFTWA_EINVAL;
+    }
+    *n_sites = s.L;
+    if (coulomb_type) *coulomb_type = s.coulomb_type;
+    const size_t LL = (size_t)s.L * s.L;
+    if (hopping) std::memcpy(hopping, s.tM.data(), LL * sizeof(double));
+    if (coulomb_u) std::memcpy(coulomb_u, s.U.data(), LL * sizeof(double));
+    if (coulomb_v) std::memcpy(coulomb_v, s.V.data(), LL * sizeof(double));
+    if (potential) {
+        for (int i = 0; i < s.L; ++i) potential[i] = s.potential.empty() ? 0.0 : s.potential[i];
+    }
+    if (n_up) std::memcpy(n_up, s.n_up.data(), s.L * sizeof(double));
+    if (n_dn) std::memcpy(n_dn, s.n_dn.data(), s.L * sizeof(double));
+    if (positions) std::memcpy(positions, s.positions.data(), 3 * (size_t)s.L * sizeof(double));
+    return FPSFTWA_OK;
+}

@@ -1,0 +1,266 @@
+// system.cpp -- JSON -> fTWA system (see system.hpp for the reference functions restated).
+#include "system.hpp"
+
+#include <cmath>
+#include <fstream>
+#include <sstream>
+
+namespace fpsamd {
+
+namespace {
+
+bool readXyz(const std::string& path, std::vector<double>& pos, int& L, std::string* err)
+{
+    // .xyz: site count, comment line, then "name x [y [z]]" per site (src/FHLattice.cpp:646-710)
+    std::ifstream f(path);
+    if (!f.good()) {
+        if (err) *err = "cannot open positions file " + path;
+        return false;
+    }
+    std::string line;
+    if (!std::getline(f, line)) {
+        if (err) *err = "empty positions file " + path;
+        return false;
+    }
+    int n = std::atoi(line.c_str());
+    std::getline(f, line);  // comment
+    pos.clear();
+    int found = 0;
+    while (std::getline(f, line)) {
+        std::stringstream ss(line);
+        std::string name;
+        if (!(ss >> name)) continue;  // blank line
+        double c[3] = {0, 0, 0};
+        std::string tok;
+        int k = 0;
+        while (ss >> tok) {
+            if (k < 3) c[k] = std::strtod(tok.c_str(), nullptr);
+            ++k;
+        }
+        pos.push_back(c[0]);
+        pos.push_back(c[1]);
+        pos.push_back(c[2]);
+        ++found;
+    }
+    if (found != n) {
+        if (err) *err = "positions file " + path + ": " + std::to_string(n) + " sites announced, " +
+                        std::to_string(found) + " found";
+        return false;
+    }
+    L = n;
+    return true;
+}
+
+inline double sq(double x) { return x * x; }
+
+}  // namespace
+
+double siteDistance(const FtwaSystem& s, int i, int j, double* disp3)
+{
+    const double* a = &s.positions[3 * i];
+    const double* b = &s.positions[3 * j];
+    double d0[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    double best[3] = {d0[0], d0[1], d0[2]};
+    double dmin = std::sqrt(sq(d0[0]) + sq(d0[1]) + sq(d0[2]));
+    if (s.periodic) {
+        const double* m = s.lattice_matrix;  // m[r*3+c]; column c is lattice vector c
+        for (int z = -1; z < 2; ++z)
+            for (int y = -1; y < 2; ++y)
+                for (int x = -1; x < 2; ++x) {
+                    double d[3];
+                    for (int r = 0; r < 3; ++r) d[r] = d0[r] + x * m[r * 3 + 0] + y * m[r * 3 + 1] + z * m[r * 3 + 2];
+                    double n = std::sqrt(sq(d[0]) + sq(d[1]) + sq(d[2]));
+                    if (n < dmin) {
+                        dmin = n;
+                        best[0] = d[0]; best[1] = d[1]; best[2] = d[2];
+                    }
+                }
+    }
+    if (disp3) { disp3[0] = best[0]; disp3[1] = best[1]; disp3[2] = best[2]; }
+    return dmin;
+}
+
+bool buildSystem(const fpsjson::Value& sys, const std::string& folder, FtwaSystem& out, std::string* err)
+{
+    using fpsjson::Value;
+    FtwaSystem s;
+    // ---- lattice (src/FHLattice.cpp:101-167) ----
+    const Value& lat = sys["lattice"];
+    std::string posFile = lat.get("positionsFile", Value("")).asString();
+    if (!folder.empty()) posFile = folder + "/" + posFile;
+    if (!readXyz(posFile, s.positions, s.L, err)) return false;
+    if (s.L <= 0) {
+        if (err) *err = "lattice has no sites";
+        return false;
+    }
+    s.periodic = lat.get("periodic", Value(false)).asBool();
+    const Value& lm = lat["latticeMatrix"];
+    if (!lm.isNull())
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) s.lattice_matrix[r * 3 + c] = lm[r][c].asDouble();
+    const int L = s.L;
+    s.distance.assign((size_t)L * L, 0.0);
+    s.displacement.assign((size_t)L * L * 3, 0.0);
+    for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j)
+            s.distance[(size_t)i * L + j] = siteDistance(s, i, j, &s.displacement[((size_t)i * L + j) * 3]);
+    const double ths = s.same_point_distance;
+
+    // ---- tunnelling (src/FHTunnelingInteraction.cpp:16-125, addTunneling :293-312) ----
+    s.tM.assign((size_t)L * L, 0.0);
+    auto addTunneling = [&](double dist, double E) {
+        for (int i = 0; i < L; ++i)
+            for (int j = i + 1; j < L; ++j)
+                if (sq(s.distance[(size_t)i * L + j] - dist) < ths) {
+                    s.tM[(size_t)i * L + j] += E;
+                    s.tM[(size_t)j * L + i] += E;
+                }
+    };
+    const Value& tun = sys["tunnelingInteraction"];
+    std::string ttype = tun.get("type", Value("null")).asString();
+    if (ttype == "nearestNeighbors") {
+        addTunneling(tun.get("neighborDistance", Value(1.0)).asDouble(),
+                     tun.get("neighborInteraction", Value(-1.0)).asDouble());
+    } else if (ttype == "distanceInteraction") {
+        const Value& ds = tun["distances"];
+        const Value& es = tun["energies"];
+        if (!ds.isNull()) {
+            if (ds.size() == es.size()) {
+                for (size_t k = 0; k < ds.size(); ++k) addTunneling(ds[k].asDouble(), es[k].asDouble());
+            } else {
+                s.warnings.push_back("Problem in the tunneling energy parameters.");
+            }
+        }
+    } else if (ttype == "distanceInteraction1" || ttype == "distanceInteraction2" || ttype == "given") {
+        s.warnings.push_back("tunnelingInteraction type '" + ttype + "' adds no hopping (as in the reference)");
+    } else if (sys.isMember("EnergyT") || sys.isMember("parametersT")) {
+        s.warnings.push_back("legacy keys EnergyT/parametersT are not read (as in the reference): zero hopping");
+    }
+
+    // ---- Coulomb (src/FHCoulombInteraction.cpp:13-76, :240-288) ----
+    s.U.assign((size_t)L * L, 0.0);
+    s.V.assign((size_t)L * L, 0.0);
+    auto addCoulomb = [&](double dist, double E) {
+        for (int i = 0; i < L; ++i)
+            for (int j = 0; j < L; ++j)
+                if (sq(s.distance[(size_t)i * L + j] - dist) < ths) {
+                    s.U[(size_t)i * L + j] += E;
+                    if (i != j) s.V[(size_t)i * L + j] += E;
+                }
+    };
+    const Value& cou = sys["coulombInteraction"];
+    std::string ctype = cou.get("type", Value("null")).asString();
+    if (ctype == "onSite") {
+        s.coulomb_type = 1;
+        addCoulomb(0.0, cou.get("onSiteInteraction", Value(1.0)).asDouble());
+    } else if (ctype == "distanceInteraction") {
+        // JSON form of the distance-table constructor (:57-76), keys as its writer emits them (:160-177)
+        s.coulomb_type = 2;
+        const Value& ds = cou["distances"];
+        const Value& es = cou["energies"];
+        size_t n = std::max(ds.size(), es.size());
+        for (size_t k = 0; k < n; ++k) addCoulomb(ds[k].asDouble(), es[k].asDouble());
+    } else {
+        s.coulomb_type = 0;
+        if (ctype != "null") s.warnings.push_back("coulombInteraction type '" + ctype + "' is not parsed: no interaction");
+    }
+
+    // ---- external potential (src/FHSystem.cpp:146-169) ----
+    const Value& ep = sys["externalPotential"];
+    if (!ep.isNull()) {
+        std::string etype = ep.get("type", Value("null")).asString();
+        if (etype == "constant") {
+            s.potential.assign(L, 0.0);
+            const Value& dg = ep["diagonal"];  // the key the reference writer emits (src/FHExternalPotential.cpp:146-164)
+            if (dg.isArray()) {
+                for (int i = 0; i < L && i < (int)dg.size(); ++i) s.potential[i] = dg[i].asDouble();
+            } else {
+                s.warnings.push_back("constant potential without 'diagonal': zero potential");
+            }
+        } else if (etype == "coreHole") {
+            s.potential.assign(L, 0.0);
+            int64_t site = ep.get("site", Value(0)).asInt();
+            double hp = ep.get("holePotential", Value(-1.0)).asDouble();
+            if (site >= 0 && site < L) s.potential[site] = hp;
+        } else if (etype == "pulse") {
+            // Peierls phase on the hopping (src/FHExternalPotentialPulse.cpp:86-110); keys as its writer (:112-122)
+            s.pulse_A0 = ep.get("A0", Value(0.0)).asDouble();
+            s.pulse_t0 = ep.get("t0", Value(0.0)).asDouble();
+            s.pulse_td = ep.get("td", Value(1.0)).asDouble();
+            s.pulse_w0 = ep.get("w0", Value(0.0)).asDouble();
+            s.pulse_phi = ep.get("phi", Value(0.0)).asDouble();
+            const Value& pol = ep["polarization"];
+            if (pol.isArray()) {
+                for (int k = 0; k < 3; ++k) s.pulse_pol[k] = pol[k].asDouble();
+            } else {  // makePotential_pulse's polarisation (src/FHSystem.cpp:818-828)
+                s.pulse_pol[0] = 1.0 / std::sqrt(2.0);
+                s.pulse_pol[1] = 1.0 / std::sqrt(2.0);
+            }
+            const Value& kn = ep["knots"];
+            if (kn.isArray() && kn.size() > 0) {
+                s.pulse_type = 2;
+                for (size_t k = 0; k < kn.size(); ++k) s.pulse_knots.push_back(kn[k].asDouble());
+            } else {
+                s.pulse_type = (s.pulse_A0 != 0.0) ? 1 : 0;
+            }
+            const Value& dg = ep["diagonal"];
+            if (dg.isArray()) {
+                s.potential.assign(L, 0.0);
+                for (int i = 0; i < L && i < (int)dg.size(); ++i) s.potential[i] = dg[i].asDouble();
+            }
+        }
+    }
+
+    // ---- initial condition (src/FHSystem.cpp:173-293, :536-595) ----
+    s.n_up.assign(L, 0.0);
+    s.n_dn.assign(L, 0.0);
+    const Value& ic = sys["IC"];
+    if (!ic.isNull()) {
+        std::string type = ic.get("type", Value("givenDiagonal")).asString();
+        s.ic_type = type;
+        int64_t nUp = ic.get("nPartUp", Value((int64_t)std::ceil(L / 2.0))).asInt();
+        int64_t nDn = ic.get("nPartDown", Value((int64_t)std::floor(L / 2.0))).asInt();
+        if (type == "givenDiagonal") {
+            std::string sub = ic.get("subtype", Value("spinWave")).asString();
+            s.ic_type += "/" + sub;
+            if (sub == "spinWave") {
+                for (int64_t i = 0; i < std::min<int64_t>(L, 2 * nUp); i += 2) s.n_up[i] = 1.0;
+                for (int64_t i = 1; i < std::min<int64_t>(L, 2 * nDn); i += 2) s.n_dn[i] = 1.0;
+            } else if (sub == "chargeWave") {
+                for (int64_t i = 0; i < std::min<int64_t>(L, 2 * nUp); i += 2) s.n_up[i] = 1.0;
+                for (int64_t i = 0; i < std::min<int64_t>(L, 2 * nDn); i += 2) s.n_dn[i] = 1.0;
+            } else if (sub == "spinWave3" || sub == "chargeWave3") {
+                int mid = L / 2;  // std::ceil(getnSites()/2) on an integer quotient
+                for (int i = 0; i < mid; ++i) s.n_up[i] = 1.0;
+                if (sub == "spinWave3")
+                    for (int i = mid; i < L; ++i) s.n_dn[i] = 1.0;
+                else
+                    for (int i = 0; i < mid; ++i) s.n_dn[i] = 1.0;
+            } else if (sub == "givenInJson") {
+                const Value& nu = ic["IC_NuDiag"];
+                const Value& nd = ic["IC_NdDiag"];
+                if (nu.isArray())
+                    for (int i = 0; i < L && i < (int)nu.size(); ++i) s.n_up[i] = nu[i].asBool() ? 1.0 : 0.0;
+                if (nd.isArray())
+                    for (int i = 0; i < L && i < (int)nd.size(); ++i) s.n_dn[i] = nd[i].asBool() ? 1.0 : 0.0;
+            } else if (sub == "givenInFile") {
+                s.warnings.push_back("IC subtype givenInFile reads no file (as in the reference): empty state");
+            }
+        } else if (type == "tbGroundState" || type == "mfGroundState" || type == "groundState") {
+            if (err) *err = "initial condition '" + type + "' (eigenbasis rotation) is not supported by this build";
+            return false;
+        } else {
+            s.warnings.push_back("initial condition type '" + type + "' sets no occupation");
+        }
+    } else {
+        s.warnings.push_back("No Initial condition in the json file.");
+    }
+    if (sys.isMember("coreHole") && sys["coreHole"].get("presence", Value(false)).asBool()) {
+        if (err) *err = "core-hole systems are not supported by this build";
+        return false;
+    }
+    out = std::move(s);
+    return true;
+}
+
+}  // namespace fpsamd

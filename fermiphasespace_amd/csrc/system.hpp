@@ -1,0 +1,46 @@
+// system.hpp -- the fTWA system built from the reference's simulation JSON.
+//
+// Restates, for the fTWA path only, what FHSystem(Json, folder) (src/FHSystem.cpp:123-326)
+// builds: lattice positions from the .xyz file (src/FHLattice.cpp:646-710), distances with the
+// periodic minimum image over +-1 cells (:516-539), the hopping tM (src/FHTunnelingInteraction.cpp:
+// 16-125, :293-312), U and V (src/FHCoulombInteraction.cpp:13-76, :240-288), the external potential
+// (src/FHSystem.cpp:146-169) and the initial occupations (src/FHSystem.cpp:173-293, :536-595).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "json.hpp"
+
+namespace fpsamd {
+
+struct FtwaSystem {
+    int L = 0;
+    std::vector<double> positions;               // [L][3]
+    bool periodic = false;
+    double lattice_matrix[9] = {0};              // [3][3]; columns are the lattice vectors
+    double same_point_distance = 1e-4;           // include/FHLattice.h:77
+    std::vector<double> distance;                // [L][L]
+    std::vector<double> tM;                      // [L][L]
+    int coulomb_type = 0;                        // 0 null, 1 onSite, 2 general
+    std::vector<double> U, V;                    // [L][L]
+    std::vector<double> potential;               // [L] or empty (EPnull)
+    int pulse_type = 0;                          // 0 none, 1 gaussian, 2 knots
+    double pulse_A0 = 0, pulse_t0 = 0, pulse_td = 1, pulse_w0 = 0, pulse_phi = 0;
+    double pulse_pol[3] = {0, 0, 0};
+    std::vector<double> pulse_knots;
+    std::vector<double> displacement;            // [L][L][3] minimum-image r_i - r_j
+    std::vector<double> n_up, n_dn;              // [L] initial occupations
+    std::string ic_type;                         // as read (for messages)
+    std::vector<std::string> warnings;           // reference behaviours the build reproduces or deviates from
+};
+
+// Builds the system from the "system" object of a simulation file.  folder is the directory of
+// the simulation file (with trailing '/'), prefixed to positionsFile as the reference does.
+// Returns false with err set on unreadable input or an unsupported request.
+bool buildSystem(const fpsjson::Value& systemJson, const std::string& folder, FtwaSystem& out, std::string* err);
+
+// Minimum-image distance between two sites (src/FHLattice.cpp:516-539).
+double siteDistance(const FtwaSystem& s, int i, int j, double* disp3);
+
+}  // namespace fpsamd

@@ -1,0 +1,183 @@
+/*
+ * fps_ftwa.h -- C ABI of the MI355X (gfx950) fTWA trajectory engine.
+ *
+ * The reference has no FFI: its boundary is the C++ class FHfTWASolver, driven by
+ * fps.cpp:192-207 (construct, five setters, initiateSolver(), runSimulation()).
+ * Each entry point below replaces one piece of that class; the citation next to it
+ * names the reference member it stands for.  Plain pointers and sizes only: no C++
+ * or torch types cross this boundary, and no exception escapes it.
+ *
+ * Conventions
+ *   - Every call returns FPSFTWA_OK (0) or a negative error code; the message of the
+ *     last failure on the calling thread is fpsftwa_last_error().
+ *   - Matrices are row-major, M[i*L + j].  Complex arrays are interleaved (re, im).
+ *   - Input arrays are copied in: the caller keeps ownership.
+ *   - A handle owns one device, one HIP stream and its device buffers.  Handles are
+ *     not shared between threads; different handles may run concurrently.
+ *   - There is no CPU fallback: without a usable gfx950 device, fpsftwa_create()
+ *     fails with FPSFTWA_EDEVICE.
+ */
+#ifndef FPS_FTWA_H
+#define FPS_FTWA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FPSFTWA_VERSION_MAJOR 0
+#define FPSFTWA_VERSION_MINOR 1
+#define FPSFTWA_VERSION_PATCH 0
+
+/* ---- error codes ------------------------------------------------------------ */
+#define FPSFTWA_OK 0
+#define FPSFTWA_EINVAL -1       /* bad argument / inconsistent sizes            */
+#define FPSFTWA_EDEVICE -2      /* no gfx950 device, or a HIP runtime failure   */
+#define FPSFTWA_ENOMEM -3       /* device allocation failed                     */
+#define FPSFTWA_ESTATE -4       /* call out of order (e.g. result before run)   */
+#define FPSFTWA_EUNSUPPORTED -5 /* valid request this build does not implement  */
+#define FPSFTWA_EIO -6          /* file could not be read / written             */
+
+/* ---- Coulomb interaction type (FHCoulombInteractionType, include/FHCoulombInteraction.h:14-19) */
+#define FPSFTWA_COULOMB_NULL 0    /* CEnull: no Hartree term at all                 */
+#define FPSFTWA_COULOMB_ONSITE 1  /* CEonSite: U only (V skipped)                   */
+#define FPSFTWA_COULOMB_GENERAL 2 /* CEdistanceInteraction / CEgiven: U and V       */
+
+/* ---- time-dependent potential (FHExternalPotentialPulse, src/FHExternalPotentialPulse.cpp:86-110) */
+#define FPSFTWA_PULSE_NONE 0
+#define FPSFTWA_PULSE_GAUSSIAN 1 /* A(t) = A0 exp(-((t-t0)/td)^2/2) cos(w0 (t-t0) + phi) */
+#define FPSFTWA_PULSE_KNOTS 2    /* A(t) piecewise linear through equally spaced knots   */
+
+/* ---- trajectory engine (precision / kernel family) ----------------------------- */
+#define FPSFTWA_ENGINE_FP64_PLAIN 0   /* dense Delta, reference operation order, one launch per time point */
+#define FPSFTWA_ENGINE_FP64_BATCHED 1 /* LDS sparse-stencil kernel, FP64 trajectories                     */
+#define FPSFTWA_ENGINE_FP32_BATCHED 2 /* LDS sparse-stencil kernel, FP32 trajectories, FP64 ensemble sums */
+
+/* ---- results to save (FHSolver::setResultsToSave, src/FHSolver.cpp:34-45) -------- */
+#define FPSFTWA_RES_DIAG 0x01     /* "diagDMatrix"   -> Niiu, Niid          */
+#define FPSFTWA_RES_DIAG2 0x02    /* "diag2DMatrix"  -> N2iiu, N2iid        */
+#define FPSFTWA_RES_DOUBLONS 0x04 /* "onSiteDoublons"-> NuNd                */
+#define FPSFTWA_RES_ENERGIES 0x08 /* "energies"      -> E {tH,U,V,pe,tot}   */
+#define FPSFTWA_RES_WHOLE 0x10    /* "wholeDMatrix"  -> N1hu, N1hd (corner) */
+
+/* ---- result arrays for fpsftwa_get_result ----------------------------------------- */
+#define FPSFTWA_OUT_TIME 0      /* [T]            t = n*dt                                   */
+#define FPSFTWA_OUT_NIIU 1      /* [L][T]         Niiu  (src/FHfTWASolver.cpp:509)           */
+#define FPSFTWA_OUT_NIID 2      /* [L][T]         Niid  (:510)                               */
+#define FPSFTWA_OUT_N2IIU 3     /* [L][T]         N2iiu (:520)                               */
+#define FPSFTWA_OUT_N2IID 4     /* [L][T]         N2iid (:521)                               */
+#define FPSFTWA_OUT_NUND 5      /* [L][T]         NuNd  (:530)                               */
+#define FPSFTWA_OUT_ENERGY 6    /* [5][T]         tH, U, V, pe, tot (:399-412, :543-547)     */
+#define FPSFTWA_OUT_N1HU 7      /* [2][T]         re, im of <N_{0,L-1}> up   (FHSolver.cpp:196) */
+#define FPSFTWA_OUT_N1HD 8      /* [2][T]         re, im of <N_{0,L-1}> down (FHSolver.cpp:197) */
+#define FPSFTWA_OUT_NIIU_ERR 9  /* [L][T]         statistical error sigma/sqrt(B) of Niiu    */
+#define FPSFTWA_OUT_NIID_ERR 10 /* [L][T]                                          of Niid    */
+#define FPSFTWA_OUT_NUND_ERR 11 /* [L][T]                                          of NuNd    */
+#define FPSFTWA_OUT_ENERGY_ERR 12 /* [4][T]                                        of E rows  */
+/* raw ensemble sums (before the reference's normalisation), for parity checks */
+#define FPSFTWA_OUT_SUM_RHO_U 13   /* [L][T] sum_b rho_up_ii                        */
+#define FPSFTWA_OUT_SUM_RHO_D 14   /* [L][T] sum_b rho_dn_ii                        */
+#define FPSFTWA_OUT_SUM_RHO2_U 15  /* [L][T] sum_b |rho_up_ii|^2                    */
+#define FPSFTWA_OUT_SUM_RHO2_D 16  /* [L][T] sum_b |rho_dn_ii|^2                    */
+#define FPSFTWA_OUT_SUM_RHOUD 17   /* [L][T] sum_b rho_up_ii rho_dn_ii               */
+#define FPSFTWA_OUT_SUM_ENERGY 18  /* [4][T] sum_b E_k                               */
+#define FPSFTWA_OUT_COUNT 19
+
+/* System matrices, as FHSystem hands them to the solver through getUMatrix / gettMatrix /
+ * getVMatrix / getExtPot (src/FHSystem.cpp:387-391).  All values are real: the reference's
+ * JSON front end only ever builds real U, V, tM and a real diagonal extPot. */
+typedef struct fpsftwa_system {
+    int32_t n_sites;               /* L                                                       */
+    const double* hopping;         /* [L][L] tM, real symmetric (FHTunnelingInteraction)     */
+    int32_t coulomb_type;          /* FPSFTWA_COULOMB_*                                      */
+    const double* coulomb_u;       /* [L][L] U (opposite spin); NULL only for COULOMB_NULL   */
+    const double* coulomb_v;       /* [L][L] V (same spin); read only for COULOMB_GENERAL    */
+    const double* potential;       /* [L] diagonal of a constant extPot, or NULL (EPnull)    */
+    /* Peierls pulse on the hopping: extPot_ij(t) = tM_ij exp(i A(t).(r_i - r_j)), added to tM */
+    int32_t pulse_type;            /* FPSFTWA_PULSE_*                                         */
+    double pulse_A0, pulse_t0, pulse_td, pulse_w0, pulse_phi;
+    double pulse_pol[3];           /* polarisation vector                                     */
+    int32_t pulse_n_knots;         /* PULSE_KNOTS: amplitudes a_k at t_k = k*t_max/(n-1)      */
+    const double* pulse_knots;     /* [n_knots]                                               */
+    const double* displacement;    /* [L][L][3] r_i - r_j (minimum image when periodic), or NULL when no pulse */
+} fpsftwa_system;
+
+typedef struct fpsftwa_handle fpsftwa_handle;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fpsftwa_last_error(void);
+
+/* Library version and the device it runs on (name buffer may be NULL). */
+int fpsftwa_version(int* major, int* minor, int* patch);
+int fpsftwa_device_info(char* name, size_t name_len, int* cu_count, double* hbm_gib);
+
+/* FHfTWASolver::FHfTWASolver(FHSystem*) -- include/FHfTWASolver.h:28, src/FHfTWASolver.cpp:4-16.
+ * Copies the system in; fails with FPSFTWA_EDEVICE when no gfx950 device is usable. */
+int fpsftwa_create(const fpsftwa_system* system, fpsftwa_handle** out);
+int fpsftwa_destroy(fpsftwa_handle* h);
+
+/* FHfTWASolver::setMethodParameters / setTimeParameters / setfTWAParameters
+ * (src/FHfTWASolver.cpp:49-69, src/FHSolver.cpp:95-101).  nTimePoints = floor(t_max/dt + 1);
+ * each time point is followed by n_sub RK4 "3/8" steps of dt/n_sub.  int_order is accepted and
+ * ignored, as the reference forces RK4-3/8 in initiateSolver (src/FHfTWASolver.cpp:90).
+ * seed keys the counter-based noise; engine is one of FPSFTWA_ENGINE_*. */
+int fpsftwa_set_params(fpsftwa_handle* h, double dt, double t_max, int32_t n_sub, int64_t n_traj,
+                       int32_t int_order, uint64_t seed, int32_t engine);
+
+/* FHSolver::setResultsToSave (src/FHSolver.cpp:34-45); flags = FPSFTWA_RES_* bitmask. */
+int fpsftwa_set_results(fpsftwa_handle* h, uint32_t flags);
+
+/* Initial state, as initiateSolver builds it (src/FHfTWASolver.cpp:99-118, :187-198):
+ * occupations n_up[L], n_dn[L] in [0,1] and optional unitary rotations R_up, R_dn
+ * ([L][L] complex interleaved; NULL = identity, the givenDiagonal/spinWave/chargeWave case). */
+int fpsftwa_set_initial_state(fpsftwa_handle* h, const double* n_up, const double* n_dn,
+                              const double* rot_up, const double* rot_dn);
+
+/* Oracle mode: use the caller's noise instead of the counter-based generator.
+ * noise = [n_traj][2 spins][L(L-1)/2 pairs][re, im], pair p enumerating i<j row by row;
+ * the sampled state is rho_sigma = R (diag(n - 1/2) + S.W) R^dagger with
+ * W_ij = noise, W_ji = conj(noise), S_ij = sqrt((n_i + n_j - 2 n_i n_j)/2)
+ * (src/FHfTWASolver.cpp:342-362).  Pass NULL to return to the generator. */
+int fpsftwa_set_noise(fpsftwa_handle* h, const double* noise, size_t n_values);
+
+/* The generator's noise for trajectories [first, first+count), same layout as above. */
+int fpsftwa_sample_noise(fpsftwa_handle* h, int64_t first, int64_t count, double* out, size_t n_values);
+
+/* FHfTWASolver::runSimulation (src/FHfTWASolver.cpp:222-556) without the file write:
+ * samples, integrates every trajectory, reduces the ensemble.  Synchronous. */
+int fpsftwa_run(fpsftwa_handle* h);
+
+/* Result arrays (FPSFTWA_OUT_*).  With out == NULL only *needed (element count) is set. */
+int fpsftwa_get_result(fpsftwa_handle* h, int32_t which, double* out, size_t n_values, size_t* needed);
+
+/* Final per-trajectory states rho_up, rho_dn ([count][2][L][L] complex interleaved, FP64)
+ * after the last recorded time point. */
+int fpsftwa_get_states(fpsftwa_handle* h, int64_t first, int64_t count, double* out, size_t n_values);
+
+/* Device time of the dominant trajectory kernel over the last fpsftwa_run: launches and
+ * summed milliseconds measured with HIP events on the handle's stream, plus whole-run ms. */
+int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, double* run_ms);
+
+/* Engine tuning (no reference counterpart; the autotuner's knobs): threads per trajectory
+ * workgroup for the batched engine (a multiple of L, 0 = automatic) and time points per launch
+ * (0 = automatic). */
+int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk);
+
+/* fps.cpp:88-207 for "method":"fTWA": read <path>, build the system, run, and write
+ * <folder><sim_name>_fTWA_res.json (the reference's results format, src/FHSolver.cpp:171-307).
+ * out_path overrides the output file name when not NULL. */
+int fpsftwa_run_json_file(const char* path, const char* out_path);
+
+/* The system and initial occupations the JSON front end builds from a simulation file
+ * (FHSystem(Json, folder), src/FHSystem.cpp:123-326).  Arrays may be NULL; *n_sites is set
+ * first, so call once with NULL arrays to size them. */
+int fpsftwa_load_system_json(const char* path, int32_t* n_sites, int32_t* coulomb_type,
+                             double* hopping, double* coulomb_u, double* coulomb_v, double* potential,
+                             double* n_up, double* n_dn, double* positions);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FPS_FTWA_H */

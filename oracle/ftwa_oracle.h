@@ -1,0 +1,80 @@
+/*
+ * ftwa_oracle.h -- CPU FP64 restatement of the reference fTWA algorithm.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (fermiphasespace_amd/) links, loads or
+ * calls this code; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg do,
+ * and only as the checker / the timed CPU baseline.
+ *
+ * It follows, function by function:
+ *   oracle_philox4x32_10  Salmon et al., "Parallel random numbers: as easy as 1, 2, 3", SC'11
+ *                         (Philox4x32 with 10 rounds; pinned by the Random123 known-answer vectors)
+ *   oracle_noise          the complex white noise of src/FHfTWASolver.cpp:343-346, drawn from a
+ *                         counter-based stream instead of Armadillo's randomly seeded one
+ *   oracle_run            FHfTWASolver::initiateSolver (src/FHfTWASolver.cpp:99-118, :187-198),
+ *                         runSimulation (:222-556), derivative (:568-600), step_RK4_38 (:629-661)
+ */
+#ifndef FTWA_ORACLE_H
+#define FTWA_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_system {
+    int L;
+    const double* tM;    /* [L][L] real symmetric hopping                         */
+    int coulomb_type;    /* 0 null, 1 onSite, 2 general (U and V)                 */
+    const double* U;     /* [L][L]                                                */
+    const double* V;     /* [L][L] (general only)                                 */
+    const double* pot;   /* [L] constant diagonal potential or NULL               */
+    int pulse_type;      /* 0 none, 1 gaussian, 2 knots                           */
+    double A0, t0, td, w0, phi;
+    double pol[3];
+    int n_knots;
+    const double* knots;
+    const double* disp;  /* [L][L][3] r_i - r_j                                   */
+} or_system;
+
+typedef struct or_params {
+    double dt, t_max;
+    int n_sub;
+    int64_t n_traj;
+    const double* n_up;   /* [L] */
+    const double* n_dn;   /* [L] */
+    const double* rot_up; /* [L][L] complex interleaved or NULL (identity) */
+    const double* rot_dn;
+} or_params;
+
+/* Raw ensemble sums (real parts), each [rows][T], any pointer may be NULL. */
+typedef struct or_out {
+    double* sum_u;      /* [L][T] sum rho_up_ii            */
+    double* sum_d;      /* [L][T] sum rho_dn_ii            */
+    double* sum2_u;     /* [L][T] sum |rho_up_ii|^2        */
+    double* sum2_d;     /* [L][T] sum |rho_dn_ii|^2        */
+    double* sum_ud;     /* [L][T] sum rho_up_ii rho_dn_ii  */
+    double* sum_e;      /* [4][T] tH, U, V, pe             */
+    double* sum_corner; /* [4][T] re/im rho_up(0,L-1), re/im rho_dn(0,L-1) */
+    double* final_states; /* [B][2][L][L][2]               */
+} or_out;
+
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+int64_t oracle_n_time_points(double dt, double t_max);
+/* noise for trajectories [first, first+count): [count][2][L(L-1)/2][2] */
+void oracle_noise(int L, int64_t first, int64_t count, uint64_t seed, double* out);
+/* pulse amplitude a(t) (scalar multiplying pol) */
+double oracle_pulse_amplitude(const or_system* s, double t, double t_max);
+/* Runs B = p->n_traj trajectories from the given noise ([B][2][npairs][2]), nthreads OpenMP threads
+ * (0 = all).  Returns 0 on success. */
+int oracle_run(const or_system* s, const or_params* p, const double* noise, int nthreads, or_out* o);
+/* Same, drawing the noise from the counter-based generator (seed) trajectory by trajectory,
+ * which keeps memory O(threads) for large B (used for the CPU baseline). */
+int oracle_run_seeded(const or_system* s, const or_params* p, uint64_t seed, int nthreads, or_out* o);
+/* One derivative evaluation f = -i(rho Delta - Delta rho) for both spins (for tests). */
+void oracle_derivative(const or_system* s, double t, double t_max, const double* rho_u, const double* rho_d,
+                       double* f_u, double* f_d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
