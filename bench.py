@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark: fTWA RK4 trajectory-steps/s on the 8x8 periodic Hubbard lattice, 65,536 trajectories.
+
+One "step" is one RK4 "3/8" step (4 stages, both spins) of the whole 65,536-trajectory batch.
+The timed region is one fpsftwa_run of K steps of the FP32 batched engine (sampling, K steps with an
+observable record at every time point, and the FP64 ensemble reduction); it prints one JSON line.
+
+Beside it, on the same workload:
+  * the plain FP64 GPU path (dense Delta, one launch per time point), timed over a few steps: the
+    speedup denominator of the headline metric;
+  * the FP64 batched engine over the same K steps: the FP64 reference for the FP32 error check;
+  * the roofline of the dominant kernel (the stencil integrator), from HIP events on its stream;
+  * the CPU baseline: the FP64 oracle (a C/OpenMP restatement of the reference algorithm) on a bounded
+    sample of the same workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fTWA RK4 traj-steps/s, 8x8 Hubbard 64k traj; speedup vs plain FP64 GPU at 1e-5 err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(seed=202):
+    from fermiphasespace_amd import lattices as lat
+    pos, box = lat.grid(8, 8)
+    U = 2.0 + 6.0 * lat.uniform_draw(seed, 0)
+    tM, Um = lat.system_arrays(pos, box, True, t=-1.0, U=U)
+    occ = np.array(lat.checkerboard(8, 8)["IC_NuDiag"], dtype=float)
+    return {"tM": tM, "U": Um, "occ": occ, "Uval": U, "L": 64}
+
+
+def traffic_from_profile(path):
+    """HBM bytes per launch of the stencil kernel on this workload, measured with rocprofv3 counters."""
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch"), d
+    except (OSError, ValueError):
+        return None, None
+
+
+def cpu_baseline(w, dt, target_s=15.0):
+    from oracle import oracle as O
+    nthreads = os.cpu_count() or 1
+    sysO = O.System(tM=w["tM"], coulomb_type=1, U=w["U"])
+    # calibrate on a tiny sample, then size the sample to about target_s seconds
+    B0, K0 = nthreads, 2
+    t0 = time.perf_counter()
+    O.run(sysO, dt, K0 * dt, 1, w["occ"], w["occ"], n_traj=B0, seed=202, nthreads=nthreads)
+    c0 = time.perf_counter() - t0
+    rate0 = B0 * K0 / max(c0, 1e-9)
+    K = max(4, int(round(target_s * rate0 / (4 * nthreads))))
+    B = 4 * nthreads
+    t0 = time.perf_counter()
+    O.run(sysO, dt, K * dt, 1, w["occ"], w["occ"], n_traj=B, seed=202, nthreads=nthreads)
+    el = time.perf_counter() - t0
+    return {"value": B * K / el, "unit": "traj-steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"FP64 oracle (dense Delta, reference algorithm) on the same 8x8 workload: {B} trajectories x "
+                      f"{K} RK4 steps, {el:.1f} s on {nthreads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000, help="timed RK4 steps of the 65,536-trajectory batch")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n-traj", type=int, default=65536)
+    ap.add_argument("--block", type=int, default=0, help="threads per trajectory workgroup (0, 512, 1024)")
+    ap.add_argument("--time-chunk", type=int, default=0)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 batched reference run")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); trajectories shard trivially "
+              "across GPUs but multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    from fermiphasespace_amd import _lib
+    try:
+        import torch  # only to bracket the timed region with a device-wide synchronize
+
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing here, not required
+        sync = lambda: None  # noqa: E731
+
+    dt, seed = 0.005, 202
+    B, K, W = args.n_traj, args.steps, args.warmup
+    w = workload(seed)
+    L = w["L"]
+    info = _lib.device_info()
+    log("device", info, "U =", w["Uval"])
+    eng = _lib.Engine(w["tM"], coulomb_type=_lib.COULOMB_ONSITE, U=w["U"])
+    eng.set_initial_state(w["occ"], w["occ"])
+    eng.set_tuning(args.block, args.time_chunk)
+    eng.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES)
+
+    # ---- warmup ----
+    eng.set_params(dt, W * dt, 1, B, seed=seed, engine="fp32")
+    eng.run()
+    log(f"warmup {W} steps done")
+
+    # ---- timed: K steps of the FP32 batched engine ----
+    eng.set_params(dt, K * dt, 1, B, seed=seed, engine="fp32")
+    sync()
+    t0 = time.perf_counter()
+    eng.run()
+    sync()
+    wall = time.perf_counter() - t0
+    tm = eng.timing()
+    value = B * K / wall
+    log(f"fp32 batched: {K} steps x {B} traj in {wall:.3f} s -> {value:.4e} traj-steps/s; kernel {tm}")
+    res32 = {k: eng.result(k) for k in ("Niiu", "Niid", "NuNd", "E", "Niiu_err", "NuNd_err")}
+
+    # roofline of the stencil kernel: algorithmic bytes 32 L^2 per trajectory-step (FP32)
+    alg_bytes = 32.0 * L * L * B * K
+    kern_s = tm["kernel_ms"] * 1e-3
+    achieved = alg_bytes / kern_s / 1e9
+    traffic, prof = traffic_from_profile(os.path.join(ROOT, "profiles", "stencil_fp32_hubbard8x8_traffic.json"))
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "stencil_kernel<float,16,8,4,false>", "launches": tm["launches"],
+            "avg_launch_ms": round(tm["kernel_ms"] / max(1, tm["launches"]), 4),
+            "alg_bytes_per_traj_step": 32 * L * L}
+
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "traj-steps/s", "n_gpus": 1, "steps": K,
+           "warmup": W, "ms_per_step": round(wall * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": "hubbard8x8_periodic_cdw", "L": L, "n_traj": B, "dt": dt, "U": round(w["Uval"], 6),
+                      "seed": seed, "steps_timed": K, "t_timed": K * dt, "integrator": "rk4_3_8",
+                      "records_every_step": True},
+           "roofline": roof}
+
+    # ---- FP64 batched reference over the same steps: FP32 error vs FP64 ----
+    if not args.no_check:
+        eng.set_params(dt, K * dt, 1, B, seed=seed, engine="fp64")
+        t0 = time.perf_counter()
+        eng.run()
+        el64 = time.perf_counter() - t0
+        res64 = {k: eng.result(k) for k in ("Niiu", "Niid", "NuNd", "E")}
+        err = {k: float(np.max(np.abs(res32[k] - res64[k]))) for k in ("Niiu", "Niid", "NuNd")}
+        err["E"] = float(np.max(np.abs(res32["E"] - res64["E"])))
+        stat = {"Niiu": float(np.max(res32["Niiu_err"])), "NuNd": float(np.max(res32["NuNd_err"]))}
+        out["fp64_batched"] = {"value": round(B * K / el64, 1), "unit": "traj-steps/s", "seconds": round(el64, 3)}
+        out["err_vs_fp64"] = {"max_abs": err, "stat_err_max": stat, "tolerance": 1e-5,
+                              "within_tolerance": bool(max(err.values()) <= 1e-5)}
+        log("fp32 vs fp64 max abs err:", err, "stat err:", stat)
+
+    # ---- plain FP64 GPU path: the speedup denominator ----
+    if not args.no_plain:
+        P = args.plain_steps
+        eng.set_params(dt, P * dt, 1, B, seed=seed, engine="fp64_plain")
+        t0 = time.perf_counter()
+        eng.run()
+        elp = time.perf_counter() - t0
+        tp = eng.timing()
+        # both sides on the same footing: wall time of a whole run (sampling, steps, records, reduction),
+        # and kernel time of the integrator launches alone
+        plain_value = B * P / elp
+        plain_kernel_value = B * P / (tp["kernel_ms"] * 1e-3)
+        out["fp64_plain"] = {"value": round(plain_value, 1), "unit": "traj-steps/s", "steps": P,
+                             "seconds": round(elp, 3), "kernel_ms": round(tp["kernel_ms"], 3)}
+        out["speedup_vs_fp64_plain"] = round(value / plain_value, 2)
+        out["speedup_vs_fp64_plain_kernel_only"] = round((B * K / kern_s) / plain_kernel_value, 2)
+        log(f"fp64 plain: {P} steps in {elp:.3f} s -> {plain_value:.4e} traj-steps/s; speedup {value / plain_value:.1f}x")
+    eng.close()
+
+    # ---- CPU baseline (oracle, host cores) ----
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, dt, args.cpu_seconds)
+        log("cpu baseline", out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

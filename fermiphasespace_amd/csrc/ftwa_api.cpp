@@ -742,10 +742,8 @@ double pulse_amplitude(const PulseSpec& p, double t)
 extern "C" int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk)
 {
     if (int rc = check_handle(h)) return rc;
-    if (block_threads < 0 || block_threads > 1024 || time_chunk < 0)
-        return fail(FPSFTWA_EINVAL, "block_threads must be in [0, 1024] and time_chunk >= 0");
-    if (block_threads > 0 && block_threads % h->L != 0)
-        return fail(FPSFTWA_EINVAL, "block_threads must be a multiple of the site count");
+    if ((block_threads != 0 && block_threads != 512 && block_threads != 1024) || time_chunk < 0)
+        return fail(FPSFTWA_EINVAL, "block_threads must be 0 (automatic), 512 or 1024, and time_chunk >= 0");
     h->nt_hint = block_threads;
     h->time_chunk = time_chunk;
     return FPSFTWA_OK;

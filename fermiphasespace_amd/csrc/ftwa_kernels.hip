@@ -1,12 +1,10 @@
 // ftwa_kernels.hip -- gfx950 kernels of the fTWA trajectory engine.
 //
 // Reference path restated: FHfTWASolver::runSimulation (src/FHfTWASolver.cpp:222-556) with
-// derivative (:568-600) and step_RK4_38 (:629-661).  Three kernel families:
+// derivative (:568-600) and step_RK4_38 (:629-661).  Kernel families in this file:
 //
 //   sample   Gaussian initial states rho = diag(n - 1/2) + S.W (:342-362), noise from Philox4x32-10
-//   stencil  the product: one workgroup per trajectory, both spins' stage input resident in LDS,
-//            the commutator -i[rho, Delta] as a sparse stencil over the hopping's neighbour lists,
-//            RK4 "3/8" state (y, A, p) in registers, one launch covering many time points
+//   (the batched stencil integrator lives in ftwa_stencil.hip)
 //   plain    the FP64 reference-order baseline: dense Delta, rho*Delta - Delta*rho from global memory,
 //            one launch per time point
 //   reduce   fixed-order, compensated FP64 ensemble sums of per-trajectory records
@@ -15,6 +13,7 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "ftwa_device.hpp"
 #include "ftwa_kernels.hpp"
 
 #ifndef FTWA_REAL
@@ -25,8 +24,8 @@ namespace fpsamd {
 
 namespace {
 
-template <typename Real>
-using V2 = typename std::conditional<std::is_same<Real, float>::value, float2, double2>::type;
+using dev::V2;
+using dev::wave_sum;
 
 // ---- Philox4x32-10 (Salmon et al., SC'11) -------------------------------------------
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
@@ -166,310 +165,6 @@ __global__ void rotate_kernel(int L, const double* __restrict__ rot, Real* __res
         }
         M[2 * e] = static_cast<Real>(ar);
         M[2 * e + 1] = static_cast<Real>(ai);
-    }
-}
-
-// ---- workgroup reduction helpers ----------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// ---- stencil integrator ------------------------------------------------------------------
-//
-// Element (sigma, i, j) of the trajectory's state is owned by thread tid for f = tid + NT*m,
-// f = sigma*L*L + i*L + j.  NT is a multiple of L, so a thread's column j never changes and the
-// neighbour list of j lives in registers.  Per RK stage:
-//   compute  k_ij = -i [ sum_{k in nb(j)} Y_ik T_kj - sum_{k in nb(i)} T_ik Y_kj + (h_j - h_i) Y_ij ]
-//            and fold it into the registers: A (b-weighted sum of k) and p (k1, then k1-k2, then k1-k2+k3)
-//   barrier
-//   write    the next stage input, derived from (y, A, p) alone, into LDS; the owners of diagonal
-//            elements refresh the Hartree potential h of the other spin (on-site U)
-//   barrier
-// The stage inputs of the 3/8 rule follow from (y, A, p):
-//   Y2 = y + h p/3,  Y3 = y + h (A - 3p)/6,  Y4 = y + h p,  y' = y + h A/8.
-template <typename Real, int EPT, int ZMAX, bool PULSE>
-__global__ void __launch_bounds__(EPT >= 32 ? 256 : (EPT >= 16 ? 512 : 1024))
-    stencil_kernel(DevSystem<Real> s, RunShape r, int t0, int t1, Real* __restrict__ state, Real* __restrict__ rec)
-{
-    using C = V2<Real>;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int L = s.L;
-    const int LL = L * L;
-    const int NE = 2 * LL;
-    const int z = s.z;
-    const int NT = blockDim.x;
-    const int tid = threadIdx.x;
-    const int nwave = (NT + 63) / 64;
-    const int64_t b = blockIdx.x;
-    const bool general = (s.coulomb_type == 2);
-
-    C* Y = reinterpret_cast<C*>(smem);                                 // [2][L][L]
-    C* Tc = Y + NE;                                                    // [z][L] (pulse)
-    Real* hb = reinterpret_cast<Real*>(PULSE ? (Tc + z * L) : Tc);     // [2][L]
-    Real* rt = hb + 2 * L;                                             // [z][L]
-    Real* ew = rt + z * L;                                             // [nwave][4]
-    int* rk = reinterpret_cast<int*>(ew + 4 * nwave);                  // [z][L]
-
-    const int j = tid % L;
-    int ck[ZMAX];
-    Real ct[ZMAX];
-    Real cpd[ZMAX];
-#pragma unroll
-    for (int n = 0; n < ZMAX; ++n) {
-        const bool on = n < z;
-        ck[n] = on ? s.nbr_k[n * L + j] : j;
-        ct[n] = on ? s.nbr_t[n * L + j] : Real(0);
-        cpd[n] = (PULSE && on) ? s.nbr_pd[n * L + j] : Real(0);
-    }
-    for (int e = tid; e < z * L; e += NT) {
-        rk[e] = s.nbr_k[e];
-        rt[e] = s.nbr_t[e];
-    }
-
-    C y[EPT], A[EPT], p[EPT];
-    const C* gs = reinterpret_cast<const C*>(state) + b * NE;
-#pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-        const int f = tid + NT * m;
-        if (f < NE) {
-            y[m] = gs[f];
-            Y[f] = y[m];
-        }
-        A[m] = C{0, 0};
-        p[m] = C{0, 0};
-    }
-    const double h = r.h;
-    const Real hR = static_cast<Real>(h);
-    int64_t gst = static_cast<int64_t>(t0) * r.nsub * 4;  // global RK-stage counter (pulse amplitude index)
-
-    auto pulse_table = [&](int64_t stage_index) {
-        if constexpr (PULSE) {
-            const Real a = static_cast<Real>(s.amp[stage_index]);
-            for (int e = tid; e < z * L; e += NT) {
-                Real sn, cs;
-                if constexpr (std::is_same<Real, float>::value)
-                    __sincosf(a * s.nbr_pd[e], &sn, &cs);
-                else
-                    sincos(a * s.nbr_pd[e], &sn, &cs);
-                const Real t = s.nbr_t[e];
-                Tc[e] = C{t * (Real(1) + cs), t * sn};
-            }
-        }
-    };
-    // Hartree potential for both spins from the LDS diagonal (general U, V or at start-up)
-    auto h_phase = [&]() {
-        for (int e = tid; e < 2 * L; e += NT) {
-            const int sg = e / L, i = e - sg * L;
-            Real v = s.pdiag[i];
-            if (s.coulomb_type == 1) {
-                v += s.u_diag[i] * (Y[(1 - sg) * LL + i * L + i].x + Real(0.5));
-            } else if (general) {
-                Real a = 0;
-                for (int jj = 0; jj < L; ++jj)
-                    a += s.U[i * L + jj] * (Y[(1 - sg) * LL + jj * L + jj].x + Real(0.5)) +
-                         s.V[i * L + jj] * (Y[sg * LL + jj * L + jj].x + Real(0.5));
-                v += a;
-            }
-            hb[e] = v;
-        }
-    };
-
-    pulse_table(gst);
-    __syncthreads();
-    h_phase();
-    __syncthreads();
-
-    Real* recb = rec;
-    const int64_t Bn = r.B;
-    const int R = r.R;
-    for (int t = t0; t < t1; ++t) {
-        Real* rr = recb + (static_cast<int64_t>(t - t0) * Bn + b) * R;
-        // ---- record the state at time point t (src/FHfTWASolver.cpp:366-418) ----
-        {
-            if (tid < 2 * L) {
-                const int sg = tid / L, i = tid - sg * L;
-                rr[tid] = Y[sg * LL + i * L + i].x;
-            }
-            if (tid == 0) {
-                const C cu = Y[L - 1], cd = Y[LL + L - 1];
-                rr[2 * L + 4] = cu.x;
-                rr[2 * L + 5] = cu.y;
-                rr[2 * L + 6] = cd.x;
-                rr[2 * L + 7] = cd.y;
-            }
-            Real eth = 0, eu = 0, ev = 0, epe = 0;
-            if (tid < L) {
-                const int i = tid;
-                const Real nu = Y[i * L + i].x + Real(0.5);
-                const Real nd = Y[LL + i * L + i].x + Real(0.5);
-                for (int n = 0; n < z; ++n) {
-                    const int k = rk[n * L + i];
-                    const Real tt = rt[n * L + i];
-                    const C a = Y[i * L + k], c = Y[LL + i * L + k];
-                    eth += (a.x + c.x) * tt;
-                    if constexpr (PULSE) {
-                        const C P = Tc[n * L + i];  // t (1 + e^{i phi}); extPot = P - t
-                        epe += (a.x + c.x) * (P.x - tt) - (a.y + c.y) * P.y;
-                    }
-                }
-                eth += s.tdiag[i] * (nu + nd);
-                epe += s.pot[i] * (nu + nd);
-                if (s.coulomb_type == 1) {
-                    eu = s.u_diag[i] * nu * nd;
-                } else if (general) {
-                    Real su = 0, sd = 0, vu = 0, vd = 0;
-                    for (int jj = 0; jj < L; ++jj) {
-                        const Real nuj = Y[jj * L + jj].x + Real(0.5);
-                        const Real ndj = Y[LL + jj * L + jj].x + Real(0.5);
-                        su += s.U[i * L + jj] * ndj;
-                        sd += s.U[i * L + jj] * nuj;
-                        vu += s.V[i * L + jj] * nuj;
-                        vd += s.V[i * L + jj] * ndj;
-                    }
-                    eu = (nu * su + nd * sd) / 2;
-                    ev = (nu * vu + nd * vd) / 2;
-                }
-            }
-            eth = wave_sum(eth);
-            eu = wave_sum(eu);
-            ev = wave_sum(ev);
-            epe = wave_sum(epe);
-            if ((tid & 63) == 0) {
-                ew[(tid >> 6) * 4 + 0] = eth;
-                ew[(tid >> 6) * 4 + 1] = eu;
-                ew[(tid >> 6) * 4 + 2] = ev;
-                ew[(tid >> 6) * 4 + 3] = epe;
-            }
-        }
-        auto flush_energies = [&]() {
-            if (tid == 0) {
-                Real e4[4] = {0, 0, 0, 0};
-                for (int w = 0; w < nwave; ++w)
-                    for (int q = 0; q < 4; ++q) e4[q] += ew[w * 4 + q];
-                for (int q = 0; q < 4; ++q) rr[2 * L + q] = e4[q];
-            }
-        };
-        if (t == r.T - 1) {  // no step after the last record changes an output
-            __syncthreads();
-            flush_energies();
-            break;
-        }
-        for (int sub = 0; sub < r.nsub; ++sub) {
-#pragma unroll 1
-            for (int st = 0; st < 4; ++st, ++gst) {
-                // ---- compute phase ----
-                C cT[ZMAX];
-                if constexpr (PULSE) {
-#pragma unroll
-                    for (int n = 0; n < ZMAX; ++n) {
-                        const C v = (n < z) ? Tc[n * L + j] : C{0, 0};
-                        cT[n] = C{v.x, -v.y};  // T_kj = conj(T_jk)
-                    }
-                }
-#pragma unroll
-                for (int m = 0; m < EPT; ++m) {
-                    const int f = tid + NT * m;
-                    if (f >= NE) continue;
-                    const int sg = f >= LL ? 1 : 0;
-                    const int i = (f - sg * LL) / L;
-                    const C* Ys = Y + sg * LL;
-                    const C yij = Ys[i * L + j];
-                    const Real dh = hb[sg * L + j] - hb[sg * L + i];
-                    Real ax = dh * yij.x, ay = dh * yij.y;
-#pragma unroll
-                    for (int n = 0; n < ZMAX; ++n) {
-                        if (n < z) {
-                            const C v = Ys[i * L + ck[n]];
-                            if constexpr (PULSE) {
-                                ax += v.x * cT[n].x - v.y * cT[n].y;
-                                ay += v.x * cT[n].y + v.y * cT[n].x;
-                            } else {
-                                ax += v.x * ct[n];
-                                ay += v.y * ct[n];
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int n = 0; n < ZMAX; ++n) {
-                        if (n < z) {
-                            const int k = rk[n * L + i];
-                            const C v = Ys[k * L + j];
-                            if constexpr (PULSE) {
-                                const C T = Tc[n * L + i];
-                                ax -= T.x * v.x - T.y * v.y;
-                                ay -= T.x * v.y + T.y * v.x;
-                            } else {
-                                const Real tt = rt[n * L + i];
-                                ax -= tt * v.x;
-                                ay -= tt * v.y;
-                            }
-                        }
-                    }
-                    const C k = C{ay, -ax};  // -i * (ax + i ay)
-                    if (st == 0) {
-                        A[m] = k;
-                        p[m] = k;
-                    } else if (st == 1) {
-                        A[m] = C{A[m].x + Real(3) * k.x, A[m].y + Real(3) * k.y};
-                        p[m] = C{p[m].x - k.x, p[m].y - k.y};
-                    } else if (st == 2) {
-                        A[m] = C{A[m].x + Real(3) * k.x, A[m].y + Real(3) * k.y};
-                        p[m] = C{p[m].x + k.x, p[m].y + k.y};
-                    } else {
-                        A[m] = C{A[m].x + k.x, A[m].y + k.y};
-                        y[m] = C{y[m].x + hR / Real(8) * A[m].x, y[m].y + hR / Real(8) * A[m].y};
-                    }
-                }
-                if (st == 0 && sub == 0) {
-                    __syncthreads();
-                    flush_energies();
-                } else {
-                    __syncthreads();
-                }
-                // ---- write phase: next stage input ----
-#pragma unroll
-                for (int m = 0; m < EPT; ++m) {
-                    const int f = tid + NT * m;
-                    if (f >= NE) continue;
-                    C yn;
-                    if (st == 0) {
-                        yn = C{y[m].x + hR / Real(3) * p[m].x, y[m].y + hR / Real(3) * p[m].y};
-                    } else if (st == 1) {
-                        yn = C{y[m].x + hR * (A[m].x - Real(3) * p[m].x) / Real(6),
-                               y[m].y + hR * (A[m].y - Real(3) * p[m].y) / Real(6)};
-                    } else if (st == 2) {
-                        yn = C{y[m].x + hR * p[m].x, y[m].y + hR * p[m].y};
-                    } else {
-                        yn = y[m];
-                    }
-                    Y[f] = yn;
-                    if (!general) {
-                        const int sg = f >= LL ? 1 : 0;
-                        const int i = (f - sg * LL) / L;
-                        if (i == j) {
-                            const Real uc = (s.coulomb_type == 1) ? s.u_diag[i] : Real(0);
-                            hb[(1 - sg) * L + i] = uc * (yn.x + Real(0.5)) + s.pdiag[i];
-                        }
-                    }
-                }
-                pulse_table(gst + 1);
-                __syncthreads();
-                if (general) {
-                    h_phase();
-                    __syncthreads();
-                }
-            }
-        }
-    }
-    C* go = reinterpret_cast<C*>(state) + b * NE;
-#pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-        const int f = tid + NT * m;
-        if (f < NE) go[f] = y[m];
     }
 }
 
@@ -741,86 +436,6 @@ __global__ void reduce_final_kernel(int Q, int nchunk, int nt, int t0, const dou
     o[1] = a.c;
 }
 
-// ---- stencil dispatch ------------------------------------------------------------------------
-constexpr int kEpts[] = {1, 2, 4, 8, 16};
-
-constexpr int launch_bound(int ept) { return ept >= 32 ? 256 : (ept >= 16 ? 512 : 1024); }
-
-template <typename Real>
-size_t stencil_lds_bytes(int L, int z, int nt, bool pulse)
-{
-    const size_t C = 2 * sizeof(Real);
-    const int nwave = (nt + 63) / 64;
-    return 2 * static_cast<size_t>(L) * L * C + (pulse ? static_cast<size_t>(z) * L * C : 0) +
-           2 * static_cast<size_t>(L) * sizeof(Real) + static_cast<size_t>(z) * L * sizeof(Real) +
-           static_cast<size_t>(nwave) * 4 * sizeof(Real) + static_cast<size_t>(z) * L * sizeof(int);
-}
-
-constexpr size_t kLdsLimit = 160 * 1024;
-
-// Block size for (L, ept): a multiple of L covering 2 L^2 elements in ept slots per thread.
-inline int nt_for(int L, int ept) { return L * ((2 * L + ept - 1) / ept); }
-
-int choose_ept(int L, int nt_hint, int* nt)
-{
-    if (nt_hint > 0 && nt_hint % L == 0 && nt_hint >= 2 * L) {
-        const int need = (2 * L * L + nt_hint - 1) / nt_hint;
-        for (int e : kEpts)
-            if (e >= need && nt_hint <= launch_bound(e)) {
-                *nt = nt_hint;
-                return e;
-            }
-    }
-    for (int e : kEpts) {
-        const int n = nt_for(L, e);
-        if (n <= launch_bound(e)) {
-            *nt = n;
-            return e;
-        }
-    }
-    return 0;
-}
-
-template <typename Real, int EPT, int ZMAX, bool PULSE>
-hipError_t run_stencil(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec, int nt,
-                       hipStream_t st)
-{
-    auto kern = stencil_kernel<Real, EPT, ZMAX, PULSE>;
-    const size_t lds = stencil_lds_bytes<Real>(s.L, s.z, nt, PULSE);
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(nt), lds, st, s, r, t0, t1, state, rec);
-    return hipGetLastError();
-}
-
-template <typename Real, int EPT, bool PULSE>
-hipError_t run_stencil_z(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec, int nt,
-                         hipStream_t st)
-{
-    if (s.z <= 4) return run_stencil<Real, EPT, 4, PULSE>(s, r, t0, t1, state, rec, nt, st);
-    if (s.z <= 8) return run_stencil<Real, EPT, 8, PULSE>(s, r, t0, t1, state, rec, nt, st);
-    return run_stencil<Real, EPT, 16, PULSE>(s, r, t0, t1, state, rec, nt, st);
-}
-
-template <typename Real, bool PULSE>
-hipError_t run_stencil_e(int ept, const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
-                         int nt, hipStream_t st)
-{
-    switch (ept) {
-    case 1: return run_stencil_z<Real, 1, PULSE>(s, r, t0, t1, state, rec, nt, st);
-    case 2: return run_stencil_z<Real, 2, PULSE>(s, r, t0, t1, state, rec, nt, st);
-    case 4: return run_stencil_z<Real, 4, PULSE>(s, r, t0, t1, state, rec, nt, st);
-    case 8: return run_stencil_z<Real, 8, PULSE>(s, r, t0, t1, state, rec, nt, st);
-    case 16: return run_stencil_z<Real, 16, PULSE>(s, r, t0, t1, state, rec, nt, st);
-    default: return hipErrorInvalidValue;
-    }
-}
-
 inline unsigned grid_for(int64_t total, int block)
 {
     int64_t g = (total + block - 1) / block;
@@ -830,30 +445,6 @@ inline unsigned grid_for(int64_t total, int block)
 }
 
 }  // namespace
-
-template <typename Real>
-bool stencil_fits(int L, int z, int* nt)
-{
-    if (L < 2 || z > 16) return false;
-    int n = 0;
-    const int e = choose_ept(L, *nt, &n);
-    if (!e) return false;
-    if (stencil_lds_bytes<Real>(L, z, n, true) > kLdsLimit) return false;
-    *nt = n;
-    return true;
-}
-
-template <typename Real>
-hipError_t launch_stencil(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
-                          int nt_hint, hipStream_t st)
-{
-    int nt = 0;
-    const int ept = choose_ept(s.L, nt_hint, &nt);
-    if (!ept || s.z > 16 || r.B <= 0) return hipErrorInvalidValue;
-    if (stencil_lds_bytes<Real>(s.L, s.z, nt, s.amp != nullptr) > kLdsLimit) return hipErrorInvalidValue;
-    if (s.amp) return run_stencil_e<Real, true>(ept, s, r, t0, t1, state, rec, nt, st);
-    return run_stencil_e<Real, false>(ept, s, r, t0, t1, state, rec, nt, st);
-}
 
 template <typename Real>
 hipError_t launch_sample(const DevSystem<Real>& s, const double* n_up, const double* n_dn, const double* noise,
@@ -921,17 +512,11 @@ hipError_t launch_reduce(int L, int64_t B, int R, int nt, int t0, const Real* re
 // explicit instantiations: FTWA_REAL selects the precision of this translation unit (1 = float,
 // 2 = double); the FP64-only entry points live in the double unit.
 #if FTWA_REAL == 1
-template bool stencil_fits<float>(int, int, int*);
-template hipError_t launch_stencil<float>(const DevSystem<float>&, const RunShape&, int, int, float*, float*, int,
-                                          hipStream_t);
 template hipError_t launch_sample<float>(const DevSystem<float>&, const double*, const double*, const double*, uint64_t,
                                          int64_t, int64_t, float*, hipStream_t);
 template hipError_t launch_rotate<float>(int, const double*, int64_t, float*, float*, hipStream_t);
 template hipError_t launch_reduce<float>(int, int64_t, int, int, int, const float*, double*, int, double*, hipStream_t);
 #else
-template bool stencil_fits<double>(int, int, int*);
-template hipError_t launch_stencil<double>(const DevSystem<double>&, const RunShape&, int, int, double*, double*, int,
-                                           hipStream_t);
 template hipError_t launch_sample<double>(const DevSystem<double>&, const double*, const double*, const double*,
                                           uint64_t, int64_t, int64_t, double*, hipStream_t);
 template hipError_t launch_rotate<double>(int, const double*, int64_t, double*, double*, hipStream_t);

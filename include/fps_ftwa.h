@@ -161,7 +161,7 @@ int fpsftwa_get_states(fpsftwa_handle* h, int64_t first, int64_t count, double* 
 int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, double* run_ms);
 
 /* Engine tuning (no reference counterpart; the autotuner's knobs): threads per trajectory
- * workgroup for the batched engine (a multiple of L, 0 = automatic) and time points per launch
+ * workgroup for the batched engine (512 or 1024; 0 = automatic) and time points per launch
  * (0 = automatic). */
 int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk);
 

@@ -1,0 +1,276 @@
+"""GPU parity of the gfx950 engine against the oracle, through the C ABI (include/fps_ftwa.h).
+
+Bars (stated per test): FP64 engines match the oracle per trajectory and per observable to 1e-10
+relative on identical samples; FP32 trajectories match to the tolerances written next to each check.
+The oracle is used only as the checker, on the same seeded inputs; committed fixtures under
+tests/golden/ hold oracle outputs so part of the suite needs no oracle library at all.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fermiphasespace_amd import _lib
+from fermiphasespace_amd import lattices as lat
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+FP64_RTOL = 1e-10
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def chain(L, t=-1.0, periodic=False):
+    tM = np.zeros((L, L))
+    for i in range(L - 1):
+        tM[i, i + 1] = tM[i + 1, i] = t
+    if periodic and L > 2:
+        tM[0, L - 1] = tM[L - 1, 0] = t
+    return tM
+
+
+def engine_run(eng, dt, t_max, n_sub, B, engine, seed=0, noise=None):
+    eng.set_params(dt, t_max, n_sub, B, seed=seed, engine=engine)
+    eng.set_noise(noise)
+    eng.run()
+    out = {k: eng.result(k) for k in ("sum_u", "sum_d", "sum2_u", "sum_ud", "sum_e")}
+    out["states"] = eng.states()
+    return out
+
+
+def oracle_system(O, tM, coulomb_type, U=None, V=None, pot=None, pulse=None):
+    pulse = pulse or {}
+    return O.System(tM=tM, coulomb_type=coulomb_type, U=U, V=V, pot=pot,
+                    pulse_type=pulse.get("pulse_type", 0), A0=pulse.get("A0", 0.0), t0=pulse.get("t0", 0.0),
+                    td=pulse.get("td", 1.0), w0=pulse.get("w0", 0.0), phi=pulse.get("phi", 0.0),
+                    pol=pulse.get("pol", (0.0, 0.0, 0.0)), knots=pulse.get("knots"), disp=pulse.get("disp"))
+
+
+def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, pulse=None, dt=0.02, t_max=0.2,
+            n_sub=1, B=6, seed=3, engines=("fp64_plain", "fp64", "fp32"), fp32_tol=2e-5, rot=None):
+    L = tM.shape[0]
+    eng = _lib.Engine(tM, coulomb_type=coulomb_type, U=U, V=V, pot=pot, **(pulse or {}))
+    if rot is None:
+        eng.set_initial_state(n_up, n_dn)
+    else:
+        eng.set_initial_state(n_up, n_dn, rot[0], rot[1])
+    eng.set_params(dt, t_max, n_sub, B, seed=seed)
+    noise = eng.sample_noise(0, B)
+    ref = O.run(oracle_system(O, tM, coulomb_type, U, V, pot, pulse), dt, t_max, n_sub, n_up, n_dn,
+                noise_arr=noise, keep_states=True, rot_up=None if rot is None else rot[0],
+                rot_dn=None if rot is None else rot[1])
+    for e in engines:
+        got = engine_run(eng, dt, t_max, n_sub, B, e, seed=seed, noise=noise)
+        tol = fp32_tol if e == "fp32" else FP64_RTOL
+        print(f"L={L} {e}: states {rel(got['states'], ref.final_states):.2e}")
+        assert rel(got["states"], ref.final_states) < tol, e
+        assert rel(got["sum_u"], ref.sum_u) < tol, e
+        assert rel(got["sum_d"], ref.sum_d) < tol, e
+        assert rel(got["sum2_u"], ref.sum2_u) < tol, e
+        assert rel(got["sum_ud"], ref.sum_ud) < tol, e
+        assert rel(got["sum_e"], ref.sum_e) < tol, e
+    eng.close()
+
+
+def test_device_noise_matches_oracle(oracle_mod):
+    L, B = 9, 40
+    eng = _lib.Engine(np.zeros((L, L)), U=np.eye(L))
+    eng.set_params(0.1, 0.1, 1, B, seed=0xDEADBEEF12345)
+    w = eng.sample_noise(0, B)
+    wo = oracle_mod.noise(L, 0, B, 0xDEADBEEF12345)
+    assert rel(w, wo) < 1e-14  # same Philox integers; log/sincos differ by at most an ulp or two
+    np.testing.assert_array_equal(eng.sample_noise(7, 3), w[7:10])  # counter-based slices
+
+
+def test_golden_fixtures():
+    for name in ("chain6_onsite", "ring6_general"):
+        f = np.load(os.path.join(GOLDEN, name + ".npz"))
+        ct = int(f["coulomb_type"])
+        eng = _lib.Engine(f["tM"], coulomb_type=ct, U=f["U"], V=f["V"] if ct == 2 else None,
+                          pot=f["pot"] if np.any(f["pot"]) else None)
+        eng.set_initial_state(f["n_up"], f["n_dn"])
+        B = int(f["B"])
+        for e in ("fp64_plain", "fp64"):
+            got = engine_run(eng, float(f["dt"]), float(f["t_max"]), int(f["n_sub"]), B, e, noise=f["noise"])
+            assert rel(got["states"], f["final_states"]) < FP64_RTOL, (name, e)
+            for k in ("sum_u", "sum_d", "sum2_u", "sum_ud", "sum_e"):
+                assert rel(got[k], f[k]) < FP64_RTOL, (name, e, k)
+        eng.close()
+
+
+def test_chain10_workload_shape(oracle_mod):
+    """The 10-site chain workload (t=-1, U=4, Neel, dt=0.01, 10 sub-steps), shortened to t=0.3."""
+    L = 10
+    n_up = np.array([1.0, 0] * 5)
+    compare(oracle_mod, chain(L), n_up, 1 - n_up, U=4.0 * np.eye(L), dt=0.01, t_max=0.3, n_sub=10, B=12)
+
+
+def test_hubbard8x8_shape(oracle_mod):
+    pos, box = lat.grid(8, 8)
+    tM, Um = lat.system_arrays(pos, box, True, t=-1.0, U=5.3)
+    occ = np.array(lat.checkerboard(8, 8)["IC_NuDiag"], dtype=float)
+    compare(oracle_mod, tM, occ, occ, U=Um, dt=0.005, t_max=0.05, B=8, seed=202)
+
+
+@pytest.mark.parametrize("L", [2, 5, 24, 36, 48, 54])
+def test_site_counts(oracle_mod, L):
+    rng = np.random.default_rng(L)
+    n_up = (rng.uniform(size=L) < 0.5).astype(float)
+    n_dn = (rng.uniform(size=L) < 0.5).astype(float)
+    compare(oracle_mod, chain(L, periodic=True), n_up, n_dn, U=3.0 * np.eye(L), dt=0.02, t_max=0.1, B=3,
+            engines=("fp64", "fp32"))
+
+
+def test_general_coulomb_and_potential(oracle_mod):
+    L = 12
+    rng = np.random.default_rng(5)
+    U = rng.uniform(0.2, 1.0, (L, L))
+    U = (U + U.T) / 2
+    V = rng.uniform(0.0, 0.5, (L, L))
+    V = (V + V.T) / 2
+    np.fill_diagonal(V, 0)
+    pot = rng.uniform(-1, 1, L)
+    n_up = np.array([1.0, 0] * 6)
+    compare(oracle_mod, chain(L, periodic=True), n_up, n_up[::-1].copy(), coulomb_type=2, U=U, V=V, pot=pot, B=5)
+
+
+def test_null_coulomb_and_fractional_occupations(oracle_mod):
+    L = 7
+    n = np.linspace(0.1, 0.9, L)
+    compare(oracle_mod, chain(L), n, n[::-1].copy(), coulomb_type=0, B=4)
+
+
+@pytest.mark.parametrize("kind", ["gaussian", "knots"])
+def test_pulse(oracle_mod, kind):
+    L = 8
+    pos, box = lat.chain(L)
+    tM = chain(L, periodic=True)
+    d = pos[:, None, :] - pos[None, :, :]
+    d[0, L - 1, 0], d[L - 1, 0, 0] = 1.0, -1.0  # minimum image across the ring
+    if kind == "gaussian":
+        pulse = dict(pulse_type=1, A0=0.5, t0=0.1, td=0.2, w0=1.5, phi=0.3, pol=(0.6, 0.8, 0.0), disp=d)
+    else:
+        pulse = dict(pulse_type=2, knots=np.array([0.0, 0.8, -0.4, 0.2]), pol=(1.0, 0.0, 0.0), disp=d)
+    n_up = np.array([1.0, 0] * 4)
+    compare(oracle_mod, tM, n_up, 1 - n_up, U=2.0 * np.eye(L), pulse=pulse, dt=0.02, t_max=0.3, n_sub=2, B=4)
+
+
+def test_rotated_initial_state(oracle_mod):
+    L = 6
+    rng = np.random.default_rng(1)
+    Q = []
+    for _ in range(2):
+        q, _ = np.linalg.qr(rng.normal(size=(L, L)) + 1j * rng.normal(size=(L, L)))
+        Q.append(q)
+    n_up = np.array([1.0, 1, 1, 0, 0, 0])
+    compare(oracle_mod, chain(L), n_up, n_up, U=4.0 * np.eye(L), B=4, rot=Q, engines=("fp64_plain", "fp64"))
+
+
+def test_few_trajectories_and_single_time_point(oracle_mod):
+    L = 4
+    n_up = np.array([1.0, 0, 1, 0])
+    compare(oracle_mod, chain(L), n_up, 1 - n_up, U=np.eye(L), B=1, t_max=0.0)
+    compare(oracle_mod, chain(L), n_up, 1 - n_up, U=np.eye(L), B=3, t_max=0.06)
+
+
+def test_tuning_and_chunking_are_bitwise_invariant():
+    """Block shape and time chunking change the schedule, not the arithmetic of any element."""
+    pos, box = lat.grid(8, 8)
+    tM, Um = lat.system_arrays(pos, box, True, U=4.0)
+    occ = np.array(lat.checkerboard(8, 8)["IC_NuDiag"], dtype=float)
+    eng = _lib.Engine(tM, U=Um)
+    eng.set_initial_state(occ, occ)
+    ref = None
+    for block, chunk in [(0, 0), (512, 1), (1024, 7), (512, 3)]:
+        eng.set_tuning(block, chunk)
+        got = engine_run(eng, 0.005, 0.1, 1, 64, "fp32", seed=9)
+        if ref is None:
+            ref = got
+        else:
+            for k in ref:
+                np.testing.assert_array_equal(got[k], ref[k], err_msg=f"{block} {chunk} {k}")
+    again = engine_run(eng, 0.005, 0.1, 1, 64, "fp32", seed=9)
+    np.testing.assert_array_equal(again["sum_u"], ref["sum_u"])  # deterministic run to run
+
+
+def test_fp32_tracks_fp64_on_identical_samples():
+    pos, box = lat.grid(8, 8)
+    tM, Um = lat.system_arrays(pos, box, True, U=4.0)
+    occ = np.array(lat.checkerboard(8, 8)["IC_NuDiag"], dtype=float)
+    eng = _lib.Engine(tM, U=Um)
+    eng.set_initial_state(occ, occ)
+    eng.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES)
+    res = {}
+    for e in ("fp32", "fp64"):
+        eng.set_params(0.005, 0.5, 1, 256, seed=4, engine=e)
+        eng.run()
+        res[e] = {k: eng.result(k) for k in ("Niiu", "NuNd", "E")}
+    for k in ("Niiu", "NuNd"):
+        assert np.max(np.abs(res["fp32"][k] - res["fp64"][k])) < 1e-5, k
+    assert np.max(np.abs(res["fp32"]["E"] - res["fp64"]["E"])) < 1e-4
+
+
+def test_full_batch_invariants():
+    """65,536 trajectories of the 8x8 workload (short): exact t=0 values and conservation laws."""
+    sim, pos = lat.workload_hubbard8x8(n_traj=65536)
+    pos, box = lat.grid(8, 8)
+    U = sim["system"]["coulombInteraction"]["onSiteInteraction"]
+    tM, Um = lat.system_arrays(pos, box, True, U=U)
+    occ = np.array(sim["system"]["IC"]["IC_NuDiag"], dtype=float)
+    eng = _lib.Engine(tM, U=Um)
+    eng.set_initial_state(occ, occ)
+    eng.set_params(0.005, 0.1, 1, 65536, seed=202, engine="fp32")
+    eng.run()
+    Niiu, Niid, NuNd, E = (eng.result(k) for k in ("Niiu", "Niid", "NuNd", "E"))
+    np.testing.assert_array_equal(Niiu[:, 0], occ)
+    np.testing.assert_array_equal(NuNd[:, 0], occ * occ)
+    np.testing.assert_allclose(Niiu.sum(axis=0), 32.0, atol=2e-4)  # particle number per spin
+    np.testing.assert_allclose(Niid.sum(axis=0), 32.0, atol=2e-4)
+    np.testing.assert_allclose(E[4], E[4, 0], atol=1e-3 * max(1.0, abs(E[4, 0])))  # mean-field energy
+    err = eng.result("Niiu_err")
+    assert np.all(err[:, 0] == 0) and np.all(err[:, -1] > 0)  # sigma/sqrt(B) of sampled densities
+    t = eng.timing()
+    assert t["launches"] >= 1 and t["kernel_ms"] > 0
+
+
+def test_json_run_writes_reference_format(tmp_path):
+    sim, pos = lat.workload_chain10(n_traj=20, seed=5, engine="fp64")
+    sim["parameters"]["t_max"] = 0.2
+    sim["results"]["diag2DMatrix"] = True
+    sim["results"]["wholeDMatrix"] = True
+    path = lat.write_simulation(str(tmp_path), sim, pos)
+    _lib.run_json_file(path)
+    out = json.load(open(tmp_path / "chain10_fTWA_res.json"))
+    assert out["fpsVersion"] == "0.5.0" and out["method"] == "fTWA" and out["system"] == sim["system"]
+    s = out["simulation"]
+    T = 21
+    assert len(s["t"]) == T and s["t"][1] == pytest.approx(0.01)
+    for k in ("Niiu", "Niid", "N2iiu", "N2iid", "NuNd"):
+        assert np.asarray(s[k]).shape == (10, T), k
+    assert set(s["E"]) == {"tH", "U", "V", "pe", "tot"} and len(s["E"]["tot"]) == T
+    assert set(s["N1hu"]) == {"r", "i"}
+    # the same run through the C ABI directly gives the same numbers
+    d = _lib.load_system_json(path)
+    eng = _lib.Engine(d["tM"], U=d["U"])
+    eng.set_initial_state(d["n_up"], d["n_dn"])
+    eng.set_params(0.01, 0.2, 10, 20, seed=5, engine="fp64")
+    eng.run()
+    np.testing.assert_allclose(np.asarray(s["Niiu"]), eng.result("Niiu"), rtol=0, atol=0)
+    np.testing.assert_allclose(np.asarray(s["NuNd"]), eng.result("NuNd"), rtol=0, atol=0)
+
+
+def test_plain_path_handles_states_beyond_stencil_capacity(oracle_mod):
+    """L = 80 exceeds the batched engine's LDS capacity: it refuses, the plain FP64 path runs."""
+    L = 80
+    n_up = np.array([1.0, 0] * 40)
+    eng = _lib.Engine(chain(L), U=np.eye(L))
+    eng.set_initial_state(n_up, 1 - n_up)
+    eng.set_params(0.02, 0.02, 1, 2, engine="fp64")
+    with pytest.raises(_lib.FtwaError) as e:
+        eng.run()
+    assert e.value.code == _lib.EUNSUPPORTED
+    compare(oracle_mod, chain(L), n_up, 1 - n_up, U=np.eye(L), B=2, t_max=0.04, engines=("fp64_plain",))

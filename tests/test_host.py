@@ -1,0 +1,213 @@
+"""CPU tests of the host side: the C ABI library's exports and the JSON front end.
+
+The JSON front end restates FHSystem(Json, folder) (src/FHSystem.cpp:123-326): positions from .xyz
+(src/FHLattice.cpp:646-710), minimum-image distances over +-1 cells (:516-539), hopping shells
+(src/FHTunnelingInteraction.cpp:293-312), Coulomb shells (src/FHCoulombInteraction.cpp:240-288),
+external potentials (src/FHSystem.cpp:146-169) and initial occupations (:173-293, :536-595).
+These tests need no GPU: fpsftwa_load_system_json is host code.
+"""
+import json
+import math
+import os
+import re
+import shutil
+
+import numpy as np
+import pytest
+
+from fermiphasespace_amd import _lib
+from fermiphasespace_amd import lattices as lat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "fps_ftwa.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+(fpsftwa_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    syms = header_symbols()
+    assert len(syms) >= 15
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTED) == syms
+
+
+def test_version_and_last_error():
+    import ctypes
+    lib = _lib.load()
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert lib.fpsftwa_version(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)) == 0
+    assert (a.value, b.value) == (0, 1)
+    assert isinstance(_lib.last_error(), str)
+
+
+def write(tmp_path, sim, pos):
+    return lat.write_simulation(str(tmp_path), sim, pos)
+
+
+def load(tmp_path, sim, pos):
+    return _lib.load_system_json(write(tmp_path, sim, pos))
+
+
+def test_chain_open(tmp_path):
+    pos, _ = lat.chain(10)
+    d = load(tmp_path, lat.simulation("c10", pos, None, False, t=-1.0, U=4.0), pos)
+    assert d["L"] == 10 and d["coulomb_type"] == _lib.COULOMB_ONSITE
+    tM = d["tM"]
+    assert np.count_nonzero(tM) == 18
+    assert np.all(np.diag(tM, 1) == -1.0) and np.array_equal(tM, tM.T)
+    np.testing.assert_array_equal(d["U"], 4.0 * np.eye(10))
+    np.testing.assert_array_equal(d["V"], 0.0)
+    np.testing.assert_array_equal(d["n_up"], [1, 0] * 5)  # spinWave: up on even sites
+    np.testing.assert_array_equal(d["n_dn"], [0, 1] * 5)
+
+
+@pytest.mark.parametrize("n", [2, 3, 8])
+def test_periodic_square_min_image(tmp_path, n):
+    pos, box = lat.grid(n, n)
+    d = load(tmp_path, lat.simulation("g", pos, box, True), pos)
+    tM = d["tM"]
+    assert np.array_equal(tM, tM.T)
+    # each pair is added once, even when two images are at the same distance (n = 2)
+    assert set(np.unique(tM)) <= {0.0, -1.0}
+    deg = np.count_nonzero(tM, axis=1)
+    assert np.all(deg == (2 if n == 2 else 4))
+    if n == 8:  # torus: site (x, y) bonds to (x +- 1, y), (x, y +- 1) mod 8
+        for x, y in [(0, 0), (7, 3), (4, 7)]:
+            i = x + 8 * y
+            want = {((x + 1) % 8) + 8 * y, ((x - 1) % 8) + 8 * y, x + 8 * ((y + 1) % 8), x + 8 * ((y - 1) % 8)}
+            assert set(np.nonzero(tM[i])[0]) == want
+
+
+@pytest.mark.parametrize("maker,args,z,L", [(lat.honeycomb, (2, 2), 3, 8), (lat.kagome, (4, 4), 4, 48),
+                                            (lat.bcc, (3,), 8, 54), (lat.grid, (10, 10), 4, 100)])
+def test_lattice_generators_coordination(tmp_path, maker, args, z, L):
+    pos, box = maker(*args)
+    d = load(tmp_path, lat.simulation("x", pos, box, True), pos)
+    assert d["L"] == L
+    assert np.all(np.count_nonzero(d["tM"], axis=1) == z)
+    # the Python generator's minimum-image distances agree with the C++ front end's bonds
+    tM, _ = lat.system_arrays(pos, box, True)
+    np.testing.assert_array_equal(tM, d["tM"])
+
+
+def test_distance_tables(tmp_path):
+    pos, box = lat.grid(4, 4)
+    sim = lat.simulation("dt", pos, box, True,
+                         tunneling={"type": "distanceInteraction", "distances": [1.0, math.sqrt(2)],
+                                    "energies": [-1.0, -0.25]},
+                         coulomb={"type": "distanceInteraction", "distances": [0.0, 1.0], "energies": [3.0, 0.5]})
+    d = load(tmp_path, sim, pos)
+    assert d["coulomb_type"] == _lib.COULOMB_GENERAL
+    assert np.count_nonzero(d["tM"] == -1.0) == 16 * 4
+    assert np.count_nonzero(d["tM"] == -0.25) == 16 * 4
+    np.testing.assert_array_equal(np.diag(d["U"]), 3.0)   # distance 0 enters U's diagonal
+    np.testing.assert_array_equal(np.diag(d["V"]), 0.0)   # ... but never V's (fillDiagonal = false)
+    assert np.count_nonzero(d["V"] == 0.5) == 16 * 4 and np.count_nonzero(d["U"] == 0.5) == 16 * 4
+
+
+def test_initial_condition_subtypes(tmp_path):
+    pos, _ = lat.chain(6)
+    cases = {
+        ("spinWave", None): ([1, 0, 1, 0, 1, 0], [0, 1, 0, 1, 0, 1]),
+        ("chargeWave", None): ([1, 0, 1, 0, 1, 0], [1, 0, 1, 0, 1, 0]),  # index parity (src/FHSystem.cpp:571-583)
+        ("spinWave3", None): ([1, 1, 1, 0, 0, 0], [0, 0, 0, 1, 1, 1]),
+        ("chargeWave3", None): ([1, 1, 1, 0, 0, 0], [1, 1, 1, 0, 0, 0]),
+        ("spinWave", (2, 1)): ([1, 0, 1, 0, 0, 0], [0, 1, 0, 0, 0, 0]),
+        ("givenInJson", None): ([1, 1, 0, 0, 0, 1], [0, 0, 1, 1, 0, 0]),
+    }
+    for (sub, npart), (nu, nd) in cases.items():
+        ic = {"type": "givenDiagonal", "subtype": sub}
+        if npart:
+            ic["nPartUp"], ic["nPartDown"] = npart
+        if sub == "givenInJson":
+            ic["IC_NuDiag"], ic["IC_NdDiag"] = nu, [False, 0, True, 1, 0, False]
+        d = load(tmp_path, lat.simulation("ic", pos, None, False, ic=ic), pos)
+        np.testing.assert_array_equal(d["n_up"], nu, err_msg=sub)
+        np.testing.assert_array_equal(d["n_dn"], nd, err_msg=sub)
+
+
+def test_external_potentials(tmp_path):
+    pos, _ = lat.chain(4)
+    d = load(tmp_path, lat.simulation("p", pos, None, False,
+                                      external={"type": "constant", "diagonal": [0.5, -0.5, 0.25, 1.0]}), pos)
+    np.testing.assert_array_equal(d["pot"], [0.5, -0.5, 0.25, 1.0])
+    d = load(tmp_path, lat.simulation("p", pos, None, False,
+                                      external={"type": "coreHole", "site": 2, "holePotential": -3.0}), pos)
+    np.testing.assert_array_equal(d["pot"], [0, 0, -3.0, 0])
+
+
+def test_reference_sample_with_legacy_keys(tmp_path):
+    """grid_10.json (shipped with the reference) uses EnergyT/parametersT keys that FHSystem(Json) never
+    reads: zero hopping and a null Coulomb interaction (src/FHSystem.cpp:136-143)."""
+    for f in ("grid_10.json", "grid_10.xyz", "sym_grid_10.json"):
+        shutil.copy(os.path.join(GOLDEN, "reference_inputs", f), tmp_path / f)
+    d = _lib.load_system_json(str(tmp_path / "grid_10.json"))
+    assert d["L"] == 10 and d["coulomb_type"] == _lib.COULOMB_NULL
+    np.testing.assert_array_equal(d["tM"], 0.0)
+    np.testing.assert_array_equal(d["n_up"], [1, 0] * 5)
+    d2 = _lib.load_system_json(str(tmp_path / "sym_grid_10.json"))
+    np.testing.assert_array_equal(d2["positions"][:, 0], np.arange(10))
+
+
+def test_front_end_errors(tmp_path):
+    pos, _ = lat.chain(4)
+    sim = lat.simulation("e", pos, None, False)
+    path = write(tmp_path, sim, pos)
+    os.remove(tmp_path / "e.xyz")
+    with pytest.raises(_lib.FtwaError) as e:
+        _lib.load_system_json(path)
+    assert "e.xyz" in str(e.value)
+    sim = lat.simulation("g", pos, None, False, ic={"type": "tbGroundState"})
+    with pytest.raises(_lib.FtwaError) as e:
+        load(tmp_path, sim, pos)
+    assert e.value.code == _lib.EINVAL and "tbGroundState" in str(e.value)
+    bad = tmp_path / "bad.json"
+    bad.write_text('{"system": {"lattice": ')
+    with pytest.raises(_lib.FtwaError) as e:
+        _lib.load_system_json(str(bad))
+    assert e.value.code == _lib.EIO
+
+
+def test_xyz_count_mismatch(tmp_path):
+    pos, _ = lat.chain(3)
+    path = write(tmp_path, lat.simulation("m", pos, None, False), pos)
+    (tmp_path / "m.xyz").write_text("4\n\nC 0 0 0\nC 1 0 0\nC 2 0 0\n")
+    with pytest.raises(_lib.FtwaError) as e:
+        _lib.load_system_json(path)
+    assert "4 sites announced, 3 found" in str(e.value)
+
+
+def test_configuration_draw_is_reproducible():
+    u = lat.uniform_draw(202, 0)
+    assert 0 <= u < 1 and u == lat.uniform_draw(202, 0) and u != lat.uniform_draw(203, 0)
+    sim, pos = lat.workload_hubbard8x8(n_traj=16)
+    U = sim["system"]["coulombInteraction"]["onSiteInteraction"]
+    assert 2.0 <= U <= 8.0
+    assert sum(sim["system"]["IC"]["IC_NuDiag"]) == 32  # half-filled checkerboard
+
+
+def test_engine_refuses_without_device():
+    """No CPU fallback: creating an engine without a gfx950 device fails loudly."""
+    try:
+        info = _lib.device_info()
+        if "gfx950" in info["name"]:
+            pytest.skip("a gfx950 device is present")
+    except _lib.FtwaError:
+        pass
+    tM = np.zeros((2, 2))
+    with pytest.raises(_lib.FtwaError) as e:
+        _lib.Engine(tM, U=np.eye(2))
+    assert e.value.code == _lib.EDEVICE
+
+
+def test_create_argument_checks():
+    with pytest.raises(_lib.FtwaError) as e:
+        _lib.Engine(np.zeros((3, 3)), coulomb_type=_lib.COULOMB_GENERAL, U=np.eye(3))
+    assert e.value.code == _lib.EINVAL
+    with pytest.raises(_lib.FtwaError):
+        _lib.Engine(np.zeros((3, 3)), U=np.eye(3), pulse_type=1, A0=1.0)  # pulse without displacements

@@ -1,0 +1,283 @@
+"""Pins of the CPU oracle (oracle/ftwa_oracle.c) against what the mathematics fixes.
+
+The reference ships no golden vectors for this path and cannot be built here (it needs Armadillo),
+so each oracle function is checked against known answers, closed forms and invariants:
+
+  philox4x32_10   Random123 known-answer vectors
+  n_time_points   floor(t_max/dt + 1) in FP64, incl. the 0.3/0.1 floor quirk (src/FHSolver.cpp:100)
+  noise/sampling  closed-form moments of the fTWA Wigner sample (src/FHfTWASolver.cpp:187-198, :342-362)
+  derivative+RK4  atomic limit (tM = 0): each rho_ij is multiplied by the RK4 stability polynomial of
+                  -i(h_j - h_i) h per step, with h_sigma = U (n_other) -- pins the Hartree term, its spin,
+                  the +1/2 and the sign of the commutator;
+                  free fermions: the exact RK4 propagator in the eigenbasis of Delta, the exact
+                  e^{i Delta t} rho e^{-i Delta t} to RK4 accuracy, and order-4 convergence
+  pulse           A = 0 doubles the hopping (extPot added to tM); a static Peierls dimer closed form
+  observables     t = 0 known answers; particle number, energy, Hermiticity and spectrum conservation
+"""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+from scipy import stats
+
+
+def rk4_poly(z):
+    return 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+
+
+def initial_states(L, n_up, n_dn, noise):
+    """rho = diag(n - 1/2) + S.W with W_ij = noise_p (i<j), W_ji = conj, S_ij = sqrt((ni+nj-2ninj)/2)."""
+    B = noise.shape[0]
+    out = np.zeros((B, 2, L, L), dtype=complex)
+    iu = np.triu_indices(L, 1)
+    for s, n in enumerate((n_up, n_dn)):
+        S = np.sqrt((n[:, None] + n[None, :] - 2 * n[:, None] * n[None, :]) / 2)
+        for b in range(B):
+            w = noise[b, s, :, 0] + 1j * noise[b, s, :, 1]
+            M = np.zeros((L, L), dtype=complex)
+            M[iu] = w * S[iu]
+            M = M + M.conj().T
+            M[np.diag_indices(L)] = n - 0.5
+            out[b, s] = M
+    return out
+
+
+def chain_tM(L, t=-1.0):
+    tM = np.zeros((L, L))
+    for i in range(L - 1):
+        tM[i, i + 1] = tM[i + 1, i] = t
+    return tM
+
+
+def test_philox_known_answers(oracle_mod):
+    O = oracle_mod
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in kat:
+        assert list(O.philox4x32_10(ctr, key)) == want
+
+
+def test_time_grid(oracle_mod):
+    O = oracle_mod
+    assert O.n_time_points(0.01, 10) == 1001
+    assert O.n_time_points(0.005, 20) == 4001
+    assert O.n_time_points(0.1, 0.3) == 3  # 0.3/0.1 = 2.9999999999999996 in FP64
+    assert O.n_time_points(0.01, 2) == 201
+    assert O.n_time_points(0.05, 0) == 1
+
+
+def test_noise_moments(oracle_mod):
+    w = oracle_mod.noise(6, 0, 4000, 7)  # 4000 x 2 x 15 complex values
+    re, im = w[..., 0].ravel(), w[..., 1].ravel()
+    assert abs(np.mean(re)) < 0.01 and abs(np.mean(im)) < 0.01
+    assert abs(np.var(re) - 0.5) < 0.01 and abs(np.var(im) - 0.5) < 0.01
+    assert abs(np.corrcoef(re, im)[0, 1]) < 0.01
+    # each component is N(0, 1/2)
+    assert stats.kstest(re / np.sqrt(0.5), "norm").pvalue > 1e-3
+    assert stats.kstest(im / np.sqrt(0.5), "norm").pvalue > 1e-3
+    # the two spins and different trajectories are independent streams
+    assert abs(np.corrcoef(w[:, 0, :, 0].ravel(), w[:, 1, :, 0].ravel())[0, 1]) < 0.01
+    # counter-based: a slice reproduces the same values
+    np.testing.assert_array_equal(oracle_mod.noise(6, 100, 5, 7), w[100:105])
+
+
+def test_sampled_state_moments(oracle_mod):
+    O = oracle_mod
+    L = 4
+    n_up = np.array([1.0, 0.0, 1.0, 0.0])
+    n_dn = np.array([1.0, 1.0, 0.0, 0.0])
+    B = 20000
+    s = O.System(tM=np.zeros((L, L)), coulomb_type=1, U=np.eye(L))
+    r = O.run(s, 0.1, 0.0, 1, n_up, n_dn, n_traj=B, seed=11, keep_states=True)
+    rho = r.final_states  # T = 1: the sampled initial states
+    for sg, n in enumerate((n_up, n_dn)):
+        R = rho[:, sg]
+        np.testing.assert_allclose(R, np.conj(np.transpose(R, (0, 2, 1))), atol=0)  # Hermitian
+        np.testing.assert_array_equal(np.diagonal(R, axis1=1, axis2=2).real, np.broadcast_to(n - 0.5, (B, L)))
+        want = (n[:, None] + n[None, :] - 2 * n[:, None] * n[None, :]) / 2
+        got = np.mean(np.abs(R) ** 2, axis=0)
+        off = ~np.eye(L, dtype=bool)
+        np.testing.assert_allclose(got[off], want[off], atol=0.03)
+        assert np.all(np.abs(R[:, (want == 0) & off]) == 0)  # no noise between equal occupations
+        np.testing.assert_allclose(np.mean(R, axis=0)[off], 0, atol=0.02)
+    # up and down noise are independent
+    c = np.corrcoef(rho[:, 0, 0, 1].real, rho[:, 1, 0, 2].real)[0, 1]
+    assert abs(c) < 0.03
+
+
+def test_atomic_limit_closed_form(oracle_mod):
+    """tM = 0: diag(rho) is constant and rho_ij picks up R(-i(h_j - h_i) dt) per step."""
+    O = oracle_mod
+    L, B = 4, 3
+    U = np.diag([1.0, 2.5, 3.0, 4.25])
+    n_up = np.array([1.0, 0.0, 1.0, 0.0])
+    n_dn = np.array([0.0, 1.0, 1.0, 0.0])
+    noise = O.noise(L, 0, B, 5)
+    s = O.System(tM=np.zeros((L, L)), coulomb_type=1, U=U)
+    dt, t_max = 0.1, 2.0
+    r = O.run(s, dt, t_max, 1, n_up, n_dn, noise_arr=noise, keep_states=True)
+    nsteps = r.T - 1
+    rho0 = initial_states(L, n_up, n_dn, noise)
+    h = [np.diag(U) * n_dn, np.diag(U) * n_up]  # h_up = U (rho_dn + 1/2) = U n_dn
+    for sg in range(2):
+        z = -1j * (h[sg][None, :] - h[sg][:, None]) * dt
+        want = rho0[:, sg] * rk4_poly(z)[None] ** nsteps
+        np.testing.assert_allclose(r.final_states[:, sg], want, rtol=0, atol=1e-13)
+
+
+def _free_expected(tM, pot, rho0, dt, nsteps):
+    D = tM + np.diag(pot)
+    lam, V = np.linalg.eigh(D)
+    rt = V.conj().T @ rho0 @ V
+    z = 1j * (lam[:, None] - lam[None, :]) * dt  # d rho/dt = i[D, rho]
+    return V @ (rt * rk4_poly(z) ** nsteps) @ V.conj().T, V @ (rt * np.exp(z * nsteps)) @ V.conj().T
+
+
+def test_free_fermions_closed_form(oracle_mod):
+    O = oracle_mod
+    L, B = 5, 2
+    rng = np.random.default_rng(3)
+    tM = chain_tM(L)
+    tM[0, 3] = tM[3, 0] = 0.3
+    pot = rng.uniform(-1, 1, L)
+    n_up = np.array([1.0, 0.0, 1.0, 0.0, 1.0])
+    n_dn = 1 - n_up
+    noise = O.noise(L, 0, B, 9)
+    s = O.System(tM=tM, coulomb_type=0, pot=pot)
+    dt, t_max = 0.05, 1.0
+    r = O.run(s, dt, t_max, 1, n_up, n_dn, noise_arr=noise, keep_states=True)
+    rho0 = initial_states(L, n_up, n_dn, noise)
+    for b in range(B):
+        for sg in range(2):
+            exact_rk4, exact = _free_expected(tM, pot, rho0[b, sg], dt, r.T - 1)
+            np.testing.assert_allclose(r.final_states[b, sg], exact_rk4, atol=1e-13)
+            np.testing.assert_allclose(r.final_states[b, sg], exact, atol=5e-4)  # RK4 truncation at dt=0.05
+
+
+def test_rk4_order_four(oracle_mod):
+    O = oracle_mod
+    L = 4
+    tM = chain_tM(L)
+    n_up = np.array([1.0, 0.0, 1.0, 0.0])
+    noise = O.noise(L, 0, 1, 1)
+    s = O.System(tM=tM, coulomb_type=0)
+    rho0 = initial_states(L, n_up, 1 - n_up, noise)[0, 0]
+    errs = []
+    for nsub in (2, 4, 8):
+        r = O.run(s, 0.4, 0.4, nsub, n_up, 1 - n_up, noise_arr=noise, keep_states=True)
+        _, exact = _free_expected(tM, np.zeros(L), rho0, 0.4 / nsub, nsub)
+        errs.append(np.max(np.abs(r.final_states[0, 0] - exact)))
+    for a, b in zip(errs, errs[1:]):
+        assert 13 < a / b < 19, errs
+
+
+def test_pulse_zero_amplitude_doubles_hopping(oracle_mod):
+    O = oracle_mod
+    L, B = 5, 2
+    tM = chain_tM(L)
+    pos = np.arange(L, dtype=float)[:, None] * np.array([1.0, 0.0, 0.0])
+    disp = pos[:, None, :] - pos[None, :, :]
+    n_up = np.array([1.0, 0.0, 1.0, 0.0, 1.0])
+    noise = O.noise(L, 0, B, 4)
+    U = 3.0 * np.eye(L)
+    with_pulse = O.System(tM=tM, coulomb_type=1, U=U, pulse_type=1, A0=0.0, t0=1.0, td=1.0, w0=2.0,
+                          pol=(1.0, 0.0, 0.0), disp=disp)
+    doubled = O.System(tM=2 * tM, coulomb_type=1, U=U)
+    a = O.run(with_pulse, 0.05, 0.5, 1, n_up, 1 - n_up, noise_arr=noise, keep_states=True)
+    b = O.run(doubled, 0.05, 0.5, 1, n_up, 1 - n_up, noise_arr=noise, keep_states=True)
+    np.testing.assert_array_equal(a.final_states, b.final_states)
+    # with A = 0 the pulse energy is the hopping energy (extPot = tM)
+    np.testing.assert_allclose(a.sum_e[3], a.sum_e[0], rtol=1e-12, atol=1e-12)
+
+
+def test_static_peierls_dimer(oracle_mod):
+    """Two sites, U = 0, constant amplitude a along x: Delta_01 = t (1 + e^{i a (x0 - x1)})."""
+    O = oracle_mod
+    t, a = -1.0, 0.7
+    tM = np.array([[0.0, t], [t, 0.0]])
+    pos = np.array([[0.0, 0, 0], [1.0, 0, 0]])
+    disp = pos[:, None, :] - pos[None, :, :]
+    s = O.System(tM=tM, coulomb_type=0, pulse_type=2, knots=np.array([a]), pol=(1.0, 0.0, 0.0), disp=disp)
+    n_up = np.array([1.0, 0.0])
+    noise = O.noise(2, 0, 1, 2)
+    dt = 0.05
+    r = O.run(s, dt, 1.0, 1, n_up, 1 - n_up, noise_arr=noise, keep_states=True)
+    D = np.array([[0, t * (1 + np.exp(-1j * a))], [t * (1 + np.exp(1j * a)), 0]])
+    rho0 = initial_states(2, n_up, 1 - n_up, noise)[0, 0]
+    lam, V = np.linalg.eigh(D)
+    z = 1j * (lam[:, None] - lam[None, :]) * dt
+    want = V @ ((V.conj().T @ rho0 @ V) * rk4_poly(z) ** (r.T - 1)) @ V.conj().T
+    np.testing.assert_allclose(r.final_states[0, 0], want, atol=1e-13)
+    assert abs(O.pulse_amplitude(s, 0.3, 1.0) - a) == 0
+
+
+def test_pulse_amplitude_forms(oracle_mod):
+    O = oracle_mod
+    g = O.System(tM=np.zeros((1, 1)), pulse_type=1, A0=0.5, t0=5.0, td=2.0, w0=1.5, phi=0.2)
+    for t in (0.0, 3.0, 5.0, 7.5):
+        want = 0.5 * np.exp(-(((t - 5.0) / 2.0) ** 2) / 2) * np.cos(1.5 * (t - 5.0) + 0.2)
+        assert abs(O.pulse_amplitude(g, t, 10.0) - want) < 1e-15
+    k = O.System(tM=np.zeros((1, 1)), pulse_type=2, knots=np.array([0.0, 1.0, -1.0]))
+    assert O.pulse_amplitude(k, 2.5, 10.0) == pytest.approx(0.5)
+    assert O.pulse_amplitude(k, 7.5, 10.0) == pytest.approx(0.0)
+    assert O.pulse_amplitude(k, 10.0, 10.0) == -1.0
+
+
+def test_time_zero_observables(oracle_mod):
+    O = oracle_mod
+    L, B = 6, 50
+    n_up = np.array([1.0, 0, 1, 0, 1, 1])
+    n_dn = np.array([1.0, 1, 0, 0, 1, 0])
+    s = O.System(tM=chain_tM(L), coulomb_type=1, U=4 * np.eye(L))
+    r = O.run(s, 0.01, 0.05, 2, n_up, n_dn, n_traj=B, seed=3)
+    out = O.normalise(r, B)
+    np.testing.assert_array_equal(out["Niiu"][:, 0], n_up)
+    np.testing.assert_array_equal(out["Niid"][:, 0], n_dn)
+    np.testing.assert_array_equal(out["NuNd"][:, 0], n_up * n_dn)
+    np.testing.assert_allclose(out["N2iiu"][:, 0], n_up, atol=1e-15)  # (n - 1/2)^2 + (n - 1/2) + 1/4 = n^2
+    # E_U(0) = U sum_i n_up n_dn; E_tH(0) = 0 (noise is off-diagonal with zero mean per trajectory pair? no:
+    # per trajectory E_tH = sum_ij N_ij t_ij = 2 t Re sum_bonds rho_ij, nonzero; only its mean vanishes)
+    np.testing.assert_allclose(out["E"][1, 0], 4 * np.sum(n_up * n_dn), rtol=1e-14)
+
+
+def test_conservation_laws(oracle_mod):
+    O = oracle_mod
+    L, B = 6, 4
+    rng = np.random.default_rng(8)
+    tM = chain_tM(L)
+    Ug = rng.uniform(0.5, 1.5, (L, L))
+    Ug = (Ug + Ug.T) / 2
+    Vg = rng.uniform(0.0, 0.5, (L, L))
+    Vg = (Vg + Vg.T) / 2
+    np.fill_diagonal(Vg, 0)
+    pot = rng.uniform(-1, 1, L)
+    n_up = np.array([1.0, 0, 1, 0, 1, 0])
+    noise = O.noise(L, 0, B, 12)
+    for s in (O.System(tM=tM, coulomb_type=1, U=4 * np.eye(L)),
+              O.System(tM=tM, coulomb_type=2, U=Ug, V=Vg, pot=pot)):
+        r = O.run(s, 0.01, 2.0, 1, n_up, 1 - n_up, noise_arr=noise, keep_states=True)
+        Nup = r.sum_u.sum(axis=0)
+        np.testing.assert_allclose(Nup, Nup[0], atol=1e-11)  # particle number per spin
+        E = r.sum_e.sum(axis=0)  # tH + U + V + pe: the mean-field energy of every trajectory
+        np.testing.assert_allclose(E, E[0], atol=1e-7 * max(1.0, abs(E[0])))
+        rho0 = initial_states(L, n_up, 1 - n_up, noise)
+        for b in range(B):
+            for sg in range(2):
+                R = r.final_states[b, sg]
+                np.testing.assert_allclose(R, R.conj().T, atol=1e-13)
+                # isospectral flow, up to the RK4 truncation error (~h^4 t)
+                np.testing.assert_allclose(np.linalg.eigvalsh(R), np.linalg.eigvalsh(rho0[b, sg]), atol=2e-6)
+
+
+def test_seeded_equals_explicit_noise(oracle_mod):
+    O = oracle_mod
+    L, B = 5, 6
+    s = O.System(tM=chain_tM(L), coulomb_type=1, U=2 * np.eye(L))
+    n_up = np.array([1.0, 0, 1, 0, 1])
+    a = O.run(s, 0.02, 0.2, 1, n_up, 1 - n_up, n_traj=B, seed=77, nthreads=2)
+    b = O.run(s, 0.02, 0.2, 1, n_up, 1 - n_up, noise_arr=O.noise(L, 0, B, 77), nthreads=3)
+    np.testing.assert_allclose(a.sum_u, b.sum_u, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(a.sum_e, b.sum_e, rtol=1e-13, atol=1e-13)
