@@ -124,6 +124,7 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
         z = std::max<int>(z, (int)nb[i].size());
     }
     z = std::max(z, 1);
+    if (const int zp = stencil_padded_z(z)) z = zp;  // the batched kernel's compiled width
     const double* pol = h->pol;
     auto pdot = [&](int i, int k) {
         if (h->disp.empty()) return 0.0;
@@ -140,6 +141,23 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
             nt[(size_t)n * L + i] = on ? (Real)h->tM[(size_t)i * L + k] : Real(0);
             npd[(size_t)n * L + i] = on ? (Real)pdot(i, k) : Real(0);
         }
+    // per-site records for the batched kernel (z is 4 or 8 there; other widths use the plain path)
+    std::vector<unsigned char> rows;
+    auto pack_rows = [&](auto rec_tag) {
+        using RecT = decltype(rec_tag);
+        constexpr int ZR = sizeof(RecT{}.k) / sizeof(int);
+        rows.assign(sizeof(RecT) * L, 0);
+        for (int i = 0; i < L; ++i) {
+            RecT rr{};
+            for (int n = 0; n < ZR; ++n) {
+                rr.k[n] = nk[(size_t)n * L + i];
+                rr.t[n] = nt[(size_t)n * L + i];
+            }
+            std::memcpy(rows.data() + sizeof(RecT) * i, &rr, sizeof(RecT));
+        }
+    };
+    if (z == 4) pack_rows(RowNbrs<Real, 4>{});
+    else if (z == 8) pack_rows(RowNbrs<Real, 8>{});
     std::vector<Real> udiag(L), pdiag(L), tdiag(L), potv(L), Ud((size_t)L * L), Vd((size_t)L * L), tMd((size_t)L * L),
         pdd((size_t)L * L);
     for (int i = 0; i < L; ++i) {
@@ -178,7 +196,8 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
                  o_pd = take(L * sizeof(Real)), o_td = take(L * sizeof(Real)), o_pot = take(L * sizeof(Real)),
                  o_U = take(Ud.size() * sizeof(Real)), o_V = take(Vd.size() * sizeof(Real)),
                  o_tM = take(tMd.size() * sizeof(Real)), o_pdd = take(pdd.size() * sizeof(Real)),
-                 o_amp = take(std::max<size_t>(amp.size(), 1) * sizeof(double));
+                 o_amp = take(std::max<size_t>(amp.size(), 1) * sizeof(double)),
+                 o_rows = take(std::max<size_t>(rows.size(), 1));
     HIP_TRY(h->sys_buf.ensure(off));
     char* base = h->sys_buf.as<char>();
     auto up = [&](size_t o, const void* src, size_t bytes) {
@@ -196,6 +215,7 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
     HIP_TRY(up(o_tM, tMd.data(), tMd.size() * sizeof(Real)));
     HIP_TRY(up(o_pdd, pdd.data(), pdd.size() * sizeof(Real)));
     if (!amp.empty()) HIP_TRY(up(o_amp, amp.data(), amp.size() * sizeof(double)));
+    if (!rows.empty()) HIP_TRY(up(o_rows, rows.data(), rows.size()));
     HIP_TRY(hipStreamSynchronize(h->stream));  // host vectors go out of scope
     ds.L = L;
     ds.z = z;
@@ -212,6 +232,7 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
     ds.tM = reinterpret_cast<const Real*>(base + o_tM);
     ds.pd = reinterpret_cast<const Real*>(base + o_pdd);
     ds.amp = amp.empty() ? nullptr : reinterpret_cast<const double*>(base + o_amp);
+    ds.rows = rows.empty() ? nullptr : base + o_rows;
     return FPSFTWA_OK;
 }
 

@@ -16,6 +16,14 @@
 
 namespace fpsamd {
 
+// Neighbour list of one site as one aligned record, so a wave-uniform site index reads it with a
+// single scalar load: k[n] the neighbour, t[n] the hopping tM_ik (unused entries: the site, 0).
+template <typename Real, int Z>
+struct alignas(Z * (sizeof(int) + sizeof(Real)) > 32 ? 64 : 32) RowNbrs {
+    int k[Z];
+    Real t[Z];
+};
+
 // Device view of the static system.  Neighbour lists are ELL with z entries per site,
 // stored entry-major ([n][L]) so consecutive sites are consecutive addresses; unused entries
 // point at the site itself with zero hopping.
@@ -36,6 +44,7 @@ struct DevSystem {
     const double* amp = nullptr;   // [T*nsub*4] pulse amplitude a(tau) at each RK stage time, or null
     const Real* tM = nullptr;      // [L][L]  dense hopping (plain path)
     const Real* pd = nullptr;      // [L][L]  pol . (r_i - r_j) dense (plain path, pulse)
+    const void* rows = nullptr;    // [L] RowNbrs<Real, z> (batched path)
 };
 
 struct RunShape {
@@ -65,6 +74,8 @@ hipError_t launch_rotate(int L, const double* rot, int64_t B, Real* state, Real*
 template <typename Real>
 hipError_t launch_stencil(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
                           int nt_hint, hipStream_t st);
+// Neighbour-list width the batched kernel is compiled for (4 or 8); 0 when z is too large.
+inline int stencil_padded_z(int z) { return z <= 4 ? 4 : (z <= 8 ? 8 : 0); }
 // Whether the stencil kernel supports this L (LDS capacity) and its chosen block size.
 template <typename Real>
 bool stencil_fits(int L, int z, int* nt);

@@ -14,13 +14,16 @@
 //   * the RK4 "3/8" state per element is (y, A, p) in registers: A accumulates b-weighted stages and
 //     p = k1, k1 - k2, k1 - k2 + k3, so every stage input follows from (y, A, p) alone:
 //        Y2 = y + h p/3,   Y3 = y + h (A - 3p)/6,   Y4 = y + h p,   y' = y + h A/8;
-//   * each stage is compute (read LDS) -> barrier -> write next stage input (and, for an on-site U, the
-//     owners of diagonal elements refresh the other spin's Hartree potential) -> barrier;
+//   * the four stages are unrolled at compile time; each is compute (read LDS) -> barrier -> write the
+//     next stage input (and, for an on-site U, the owners of diagonal elements refresh the other spin's
+//     Hartree potential) -> barrier;
 //   * every time point writes one record per trajectory (diagonals, energies, corner element) that the
 //     reduction kernels sum in fixed order.
+// Neighbour lists are padded on the host to exactly Z entries (padding: the site itself, zero hopping).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "ftwa_device.hpp"
 #include "ftwa_kernels.hpp"
@@ -39,43 +42,50 @@ using dev::wave_sum;
 constexpr int kRow = 64;  // LDS row length: sites padded to one wavefront
 constexpr size_t kLdsLimit = 160 * 1024;
 
-template <typename Real, int W, int EPT, int ZMAX, bool PULSE>
+template <int N>
+using Ic = std::integral_constant<int, N>;
+
+// LT > 0 fixes the site count at compile time (all index arithmetic folds); LT = 0 reads it from s.L.
+template <typename Real, int W, int EPT, int Z, bool PULSE, int LT>
 __global__ void __launch_bounds__(64 * W)
     stencil_kernel(DevSystem<Real> s, RunShape r, int t0, int t1, Real* __restrict__ state, Real* __restrict__ rec)
 {
     using C = V2<Real>;
     constexpr int NT = 64 * W;
+    // Folding the spin of each element into the code (when rows exactly tile the wavefronts) looks
+    // attractive but makes the register allocator spill heavily; the spin stays a uniform runtime value.
+    constexpr bool kExact = false;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int L = s.L;
+    const int L = LT > 0 ? LT : s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
-    const int z = s.z;
     const int tid = threadIdx.x;
     const int j = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const bool jon = j < L;
+    __builtin_assume(w >= 0 && w < W);
+    const bool jon = (LT == kRow) ? true : (j < L);
     const int64_t b = blockIdx.x;
     const bool general = s.coulomb_type == 2;
     const int* __restrict__ nbk = s.nbr_k;
     const Real* __restrict__ nbt = s.nbr_t;
     const Real* __restrict__ udiag = s.u_diag;
     const Real* __restrict__ pdiag = s.pdiag;
+    const RowNbrs<Real, Z>* __restrict__ rows = static_cast<const RowNbrs<Real, Z>*>(s.rows);
 
     C* Y = reinterpret_cast<C*>(smem);                                  // [2L][64]
-    C* Tc = Y + R2 * kRow;                                              // [z][L] (pulse)
-    Real* hb = reinterpret_cast<Real*>(PULSE ? (Tc + z * L) : Tc);      // [2][64]
+    C* Tc = Y + R2 * kRow;                                              // [Z][L] (pulse)
+    Real* hb = reinterpret_cast<Real*>(PULSE ? (Tc + Z * L) : Tc);      // [2][64]
     Real* ew = hb + 2 * kRow;                                           // [W][4]
 
     for (int e = tid; e < R2 * kRow; e += NT) Y[e] = C{0, 0};
     for (int e = tid; e < 2 * kRow; e += NT) hb[e] = Real(0);
 
-    int ck[ZMAX];
-    Real ct[ZMAX];
+    int ck[Z];
+    Real ct[Z];
 #pragma unroll
-    for (int n = 0; n < ZMAX; ++n) {
-        const bool on = jon && n < z;
-        ck[n] = on ? nbk[n * L + j] : j;
-        ct[n] = on ? nbt[n * L + j] : Real(0);
+    for (int n = 0; n < Z; ++n) {
+        ck[n] = jon ? nbk[n * L + j] : j;
+        ct[n] = jon ? nbt[n * L + j] : Real(0);
     }
     __syncthreads();
 
@@ -87,20 +97,21 @@ __global__ void __launch_bounds__(64 * W)
         y[m] = C{0, 0};
         A[m] = C{0, 0};
         p[m] = C{0, 0};
-        if (row < R2 && jon) {
-            const int sg = row >= L ? 1 : 0;
+        if ((kExact || row < R2) && jon) {
+            const int sg = kExact ? (m >= EPT / 2 ? 1 : 0) : (row >= L ? 1 : 0);
             const int i = row - sg * L;
             y[m] = gs[sg * LL + i * L + j];
             Y[row * kRow + j] = y[m];
         }
     }
-    const Real hR = static_cast<Real>(r.h);
+    const Real h1 = static_cast<Real>(r.h);
+    const Real h3 = static_cast<Real>(r.h / 3.0), h6 = static_cast<Real>(r.h / 6.0), h8 = static_cast<Real>(r.h / 8.0);
     int64_t gst = static_cast<int64_t>(t0) * r.nsub * 4;  // global RK-stage index (pulse amplitude table)
 
     auto pulse_table = [&](int64_t stage_index) {
         if constexpr (PULSE) {
             const Real a = static_cast<Real>(s.amp[stage_index]);
-            for (int e = tid; e < z * L; e += NT) {
+            for (int e = tid; e < Z * L; e += NT) {
                 Real sn, cs;
                 if constexpr (std::is_same<Real, float>::value)
                     __sincosf(a * s.nbr_pd[e], &sn, &cs);
@@ -134,10 +145,145 @@ __global__ void __launch_bounds__(64 * W)
     h_phase();
     __syncthreads();
 
+    Real* rr = nullptr;
+    auto flush_energies = [&]() {
+        if (tid == 0) {
+            Real e4[4] = {0, 0, 0, 0};
+            for (int ww = 0; ww < W; ++ww)
+                for (int q = 0; q < 4; ++q) e4[q] += ew[ww * 4 + q];
+            for (int q = 0; q < 4; ++q) rr[R2 + q] = e4[q];
+        }
+    };
+
+    // One RK stage: compute k for every owned element, fold it into (A, p, y), then publish the next
+    // stage input.  ST is the stage (0..3); flush writes the record's energies after the first barrier.
+    auto stage = [&](auto stc, int64_t gidx, bool flush) {
+        constexpr int ST = decltype(stc)::value;
+        // the row records are read per element through the scalar cache; the row base is made opaque
+        // per stage so the compiler does not pin every row's record in registers for the whole run.
+        // Likewise the lane's column and its neighbour columns, so LDS addresses are formed per stage
+        // instead of being hoisted, one register each per (element, neighbour), out of the time loop
+        int jo = j;
+        int cko[Z];
+        int wo = w;
+        asm volatile("" : "+v"(jo), "+s"(wo));
+#pragma unroll
+        for (int n = 0; n < Z; ++n) {
+            cko[n] = ck[n];
+            asm volatile("" : "+v"(cko[n]));
+        }
+        const Real hj0 = hb[j], hj1 = hb[kRow + j];
+        C cT[Z];
+        if constexpr (PULSE) {
+#pragma unroll
+            for (int n = 0; n < Z; ++n) {
+                const C v = jon ? Tc[n * L + j] : C{0, 0};
+                cT[n] = C{v.x, -v.y};  // t_kj(t) = conj(t_jk(t))
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int row = wo + W * m;
+            if (!kExact && row >= R2) continue;
+            const int sg = kExact ? (m >= EPT / 2 ? 1 : 0) : (row >= L ? 1 : 0);
+            const int i = row - sg * L;
+            const RowNbrs<Real, Z> nb = rows[i];
+            const C* Yr = Y + row * kRow;
+            const C* Ys = Y + sg * L * kRow;
+            const C yij = (ST == 0) ? y[m] : Yr[jo];
+            const Real dh = (sg ? hj1 : hj0) - hb[sg * kRow + i];
+            Real ax = dh * yij.x, ay = dh * yij.y;
+#pragma unroll
+            for (int n = 0; n < Z; ++n) {
+                const C v = Yr[cko[n]];
+                if constexpr (PULSE) {
+                    ax += v.x * cT[n].x - v.y * cT[n].y;
+                    ay += v.x * cT[n].y + v.y * cT[n].x;
+                } else {
+                    ax += v.x * ct[n];
+                    ay += v.y * ct[n];
+                }
+            }
+#pragma unroll
+            for (int n = 0; n < Z; ++n) {
+                const C v = Ys[nb.k[n] * kRow + jo];
+                if constexpr (PULSE) {
+                    const C T = Tc[n * L + i];
+                    ax -= T.x * v.x - T.y * v.y;
+                    ay -= T.x * v.y + T.y * v.x;
+                } else {
+                    const Real tt = nb.t[n];
+                    ax -= tt * v.x;
+                    ay -= tt * v.y;
+                }
+            }
+            const Real kx = ay, ky = -ax;  // k = -i (ax + i ay)
+            if constexpr (ST == 0) {
+                A[m] = C{kx, ky};
+                p[m] = C{kx, ky};
+            } else if constexpr (ST == 1) {
+                A[m] = C{A[m].x + Real(3) * kx, A[m].y + Real(3) * ky};
+                p[m] = C{p[m].x - kx, p[m].y - ky};
+            } else if constexpr (ST == 2) {
+                A[m] = C{A[m].x + Real(3) * kx, A[m].y + Real(3) * ky};
+                p[m] = C{p[m].x + kx, p[m].y + ky};
+            } else {
+                A[m] = C{A[m].x + kx, A[m].y + ky};
+                y[m] = C{y[m].x + h8 * A[m].x, y[m].y + h8 * A[m].y};
+            }
+            // keep each element's loads and arithmetic together: unbounded interleaving of the
+            // unrolled elements makes the scheduler hoist loads until registers spill
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+        if (ST == 0 && flush) flush_energies();
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int row = w + W * m;
+            if (!kExact && row >= R2) continue;
+            C yn;
+            if constexpr (ST == 0) {
+                yn = C{y[m].x + h3 * p[m].x, y[m].y + h3 * p[m].y};
+            } else if constexpr (ST == 1) {
+                yn = C{y[m].x + h6 * (A[m].x - Real(3) * p[m].x), y[m].y + h6 * (A[m].y - Real(3) * p[m].y)};
+            } else if constexpr (ST == 2) {
+                yn = C{y[m].x + h1 * p[m].x, y[m].y + h1 * p[m].y};
+            } else {
+                yn = y[m];
+            }
+            Y[row * kRow + jo] = yn;
+            {
+                // on-site Hartree potential of the other spin from this row's diagonal element, which
+                // lane i holds: fetched wave-wide and stored by every lane (same address, same value),
+                // so there is no divergent branch.  With a general U/V, h_phase overwrites it below.
+                const int sg = kExact ? (m >= EPT / 2 ? 1 : 0) : (row >= L ? 1 : 0);
+                const int i = row - sg * L;
+                Real dv;
+                if constexpr (std::is_same<Real, float>::value) {
+                    dv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, yn.x), i));
+                } else {
+                    const long long bits = __builtin_bit_cast(long long, yn.x);
+                    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), i);
+                    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), i);
+                    dv = __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) |
+                                                        static_cast<unsigned int>(lo));
+                }
+                const Real uc = (s.coulomb_type == 1) ? udiag[i] : Real(0);
+                hb[(1 - sg) * kRow + i] = uc * (dv + Real(0.5)) + pdiag[i];
+            }
+        }
+        pulse_table(gidx + 1);
+        __syncthreads();
+        if (general) {
+            h_phase();
+            __syncthreads();
+        }
+    };
+
     const int64_t Bn = r.B;
     const int R = r.R;
     for (int t = t0; t < t1; ++t) {
-        Real* rr = rec + (static_cast<int64_t>(t - t0) * Bn + b) * R;
+        rr = rec + (static_cast<int64_t>(t - t0) * Bn + b) * R;
         // ---- record time point t (src/FHfTWASolver.cpp:366-418) ----
         {
             for (int e = tid; e < R2; e += NT) {
@@ -156,7 +302,7 @@ __global__ void __launch_bounds__(64 * W)
                 const int i = tid;
                 const Real nu = Y[i * kRow + i].x + Real(0.5);
                 const Real nd = Y[(L + i) * kRow + i].x + Real(0.5);
-                for (int n = 0; n < z; ++n) {
+                for (int n = 0; n < Z; ++n) {
                     const int k = nbk[n * L + i];
                     const Real tt = nbt[n * L + i];
                     const C a = Y[i * kRow + k], c = Y[(L + i) * kRow + k];
@@ -195,134 +341,25 @@ __global__ void __launch_bounds__(64 * W)
                 ew[w * 4 + 3] = epe;
             }
         }
-        auto flush_energies = [&]() {
-            if (tid == 0) {
-                Real e4[4] = {0, 0, 0, 0};
-                for (int ww = 0; ww < W; ++ww)
-                    for (int q = 0; q < 4; ++q) e4[q] += ew[ww * 4 + q];
-                for (int q = 0; q < 4; ++q) rr[R2 + q] = e4[q];
-            }
-        };
         if (t == r.T - 1) {  // no step after the last record changes an output
             __syncthreads();
             flush_energies();
             break;
         }
         for (int sub = 0; sub < r.nsub; ++sub) {
-#pragma unroll 1
-            for (int st = 0; st < 4; ++st, ++gst) {
-                // ---- compute phase: k for every owned element ----
-                // The row base is made opaque here so the row tables are re-read (scalar cache) each
-                // stage instead of being hoisted out of the time loop into registers.
-                int wo = w;
-                asm volatile("" : "+s"(wo));
-                const Real hj0 = hb[j], hj1 = hb[kRow + j];
-                C cT[ZMAX];
-                if constexpr (PULSE) {
-#pragma unroll
-                    for (int n = 0; n < ZMAX; ++n) {
-                        const C v = (jon && n < z) ? Tc[n * L + j] : C{0, 0};
-                        cT[n] = C{v.x, -v.y};  // t_kj(t) = conj(t_jk(t))
-                    }
-                }
-#pragma unroll
-                for (int m = 0; m < EPT; ++m) {
-                    const int row = wo + W * m;
-                    if (row >= R2) continue;
-                    const int sg = row >= L ? 1 : 0;
-                    const int i = row - sg * L;
-                    const C* Yr = Y + row * kRow;
-                    const C* Ys = Y + sg * L * kRow;
-                    const C yij = Yr[j];
-                    const Real dh = (sg ? hj1 : hj0) - hb[sg * kRow + i];
-                    Real ax = dh * yij.x, ay = dh * yij.y;
-#pragma unroll
-                    for (int n = 0; n < ZMAX; ++n) {
-                        if (n < z) {
-                            const C v = Yr[ck[n]];
-                            if constexpr (PULSE) {
-                                ax += v.x * cT[n].x - v.y * cT[n].y;
-                                ay += v.x * cT[n].y + v.y * cT[n].x;
-                            } else {
-                                ax += v.x * ct[n];
-                                ay += v.y * ct[n];
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int n = 0; n < ZMAX; ++n) {
-                        if (n < z) {
-                            const int k = nbk[n * L + i];
-                            const C v = Ys[k * kRow + j];
-                            if constexpr (PULSE) {
-                                const C T = Tc[n * L + i];
-                                ax -= T.x * v.x - T.y * v.y;
-                                ay -= T.x * v.y + T.y * v.x;
-                            } else {
-                                const Real tt = nbt[n * L + i];
-                                ax -= tt * v.x;
-                                ay -= tt * v.y;
-                            }
-                        }
-                    }
-                    const C k = C{ay, -ax};  // -i (ax + i ay)
-                    if (st == 0) {
-                        A[m] = k;
-                        p[m] = k;
-                    } else if (st == 1) {
-                        A[m] = C{A[m].x + Real(3) * k.x, A[m].y + Real(3) * k.y};
-                        p[m] = C{p[m].x - k.x, p[m].y - k.y};
-                    } else if (st == 2) {
-                        A[m] = C{A[m].x + Real(3) * k.x, A[m].y + Real(3) * k.y};
-                        p[m] = C{p[m].x + k.x, p[m].y + k.y};
-                    } else {
-                        A[m] = C{A[m].x + k.x, A[m].y + k.y};
-                        y[m] = C{y[m].x + hR / Real(8) * A[m].x, y[m].y + hR / Real(8) * A[m].y};
-                    }
-                }
-                __syncthreads();
-                if (st == 0 && sub == 0) flush_energies();
-                // ---- write phase: next stage input ----
-#pragma unroll
-                for (int m = 0; m < EPT; ++m) {
-                    const int row = wo + W * m;
-                    if (row >= R2) continue;
-                    C yn;
-                    if (st == 0) {
-                        yn = C{y[m].x + hR / Real(3) * p[m].x, y[m].y + hR / Real(3) * p[m].y};
-                    } else if (st == 1) {
-                        yn = C{y[m].x + hR * (A[m].x - Real(3) * p[m].x) / Real(6),
-                               y[m].y + hR * (A[m].y - Real(3) * p[m].y) / Real(6)};
-                    } else if (st == 2) {
-                        yn = C{y[m].x + hR * p[m].x, y[m].y + hR * p[m].y};
-                    } else {
-                        yn = y[m];
-                    }
-                    Y[row * kRow + j] = yn;
-                    if (!general) {
-                        const int sg = row >= L ? 1 : 0;
-                        const int i = row - sg * L;
-                        if (i == j) {
-                            const Real uc = (s.coulomb_type == 1) ? udiag[i] : Real(0);
-                            hb[(1 - sg) * kRow + i] = uc * (yn.x + Real(0.5)) + pdiag[i];
-                        }
-                    }
-                }
-                pulse_table(gst + 1);
-                __syncthreads();
-                if (general) {
-                    h_phase();
-                    __syncthreads();
-                }
-            }
+            stage(Ic<0>{}, gst, sub == 0);
+            stage(Ic<1>{}, gst + 1, false);
+            stage(Ic<2>{}, gst + 2, false);
+            stage(Ic<3>{}, gst + 3, false);
+            gst += 4;
         }
     }
     C* go = reinterpret_cast<C*>(state) + b * 2 * LL;
 #pragma unroll
     for (int m = 0; m < EPT; ++m) {
         const int row = w + W * m;
-        if (row < R2 && jon) {
-            const int sg = row >= L ? 1 : 0;
+        if ((kExact || row < R2) && jon) {
+            const int sg = kExact ? (m >= EPT / 2 ? 1 : 0) : (row >= L ? 1 : 0);
             const int i = row - sg * L;
             go[sg * LL + i * L + j] = y[m];
         }
@@ -353,10 +390,10 @@ int choose_shape(int L, int nt_hint, int* W)
     return 0;
 }
 
-template <typename Real, int W, int EPT, int ZMAX, bool PULSE>
+template <typename Real, int W, int EPT, int Z, bool PULSE, int LT>
 hipError_t run(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec, hipStream_t st)
 {
-    auto kern = stencil_kernel<Real, W, EPT, ZMAX, PULSE>;
+    auto kern = stencil_kernel<Real, W, EPT, Z, PULSE, LT>;
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -372,8 +409,14 @@ hipError_t run(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real
 template <typename Real, int W, int EPT, bool PULSE>
 hipError_t run_z(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec, hipStream_t st)
 {
-    if (s.z <= 4) return run<Real, W, EPT, 4, PULSE>(s, r, t0, t1, state, rec, st);
-    return run<Real, W, EPT, 8, PULSE>(s, r, t0, t1, state, rec, st);
+    if (s.z == 4) {
+        // the 8x8-lattice shape: site count folded into the code
+        if constexpr (!PULSE && ((W == 16 && EPT == 8) || (W == 8 && EPT == 16)))
+            if (s.L == 64) return run<Real, W, EPT, 4, false, 64>(s, r, t0, t1, state, rec, st);
+        return run<Real, W, EPT, 4, PULSE, 0>(s, r, t0, t1, state, rec, st);
+    }
+    if (s.z == 8) return run<Real, W, EPT, 8, PULSE, 0>(s, r, t0, t1, state, rec, st);
+    return hipErrorInvalidValue;
 }
 
 template <typename Real, int W, bool PULSE>
@@ -394,10 +437,10 @@ hipError_t run_e(int ept, const DevSystem<Real>& s, const RunShape& r, int t0, i
 template <typename Real>
 bool stencil_fits(int L, int z, int* nt)
 {
-    if (L < 1 || L > kRow || z > 8) return false;
+    if (L < 1 || L > kRow || stencil_padded_z(z) == 0) return false;
     int W = 0;
     if (!choose_shape<Real>(L, *nt, &W)) return false;
-    if (lds_bytes<Real>(L, z, W, true) > kLdsLimit) return false;
+    if (lds_bytes<Real>(L, stencil_padded_z(z), W, true) > kLdsLimit) return false;
     *nt = 64 * W;
     return true;
 }
@@ -408,7 +451,7 @@ hipError_t launch_stencil(const DevSystem<Real>& s, const RunShape& r, int t0, i
 {
     int W = 0;
     const int ept = choose_shape<Real>(s.L, nt_hint, &W);
-    if (!ept || s.z > 8 || s.L > kRow || r.B <= 0) return hipErrorInvalidValue;
+    if (!ept || (s.z != 4 && s.z != 8) || s.L > kRow || r.B <= 0) return hipErrorInvalidValue;
     const bool pulse = s.amp != nullptr;
     if (lds_bytes<Real>(s.L, s.z, W, pulse) > kLdsLimit) return hipErrorInvalidValue;
     if (W == 16) {
