@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--n-traj", type=int, default=65536)
     ap.add_argument("--block", type=int, default=0, help="threads per trajectory workgroup (0, 512, 1024)")
     ap.add_argument("--time-chunk", type=int, default=0)
+    ap.add_argument("--kernel", default="auto", choices=["auto", "full", "hermitian"],
+                    help="FP32 integrator: upper-triangle (hermitian), full-matrix, or automatic")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 batched reference run")
     ap.add_argument("--no-plain", action="store_true")
@@ -120,7 +122,7 @@ def main():
     log("device", info, "U =", w["Uval"])
     eng = _lib.Engine(w["tM"], coulomb_type=_lib.COULOMB_ONSITE, U=w["U"])
     eng.set_initial_state(w["occ"], w["occ"])
-    eng.set_tuning(args.block, args.time_chunk)
+    eng.set_tuning(args.block, args.time_chunk, args.kernel)
     eng.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES)
 
     # ---- warmup ----
@@ -147,7 +149,8 @@ def main():
     traffic, prof = traffic_from_profile(os.path.join(ROOT, "profiles", "stencil_fp32_hubbard8x8_traffic.json"))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "stencil_kernel<float,16,8,4,false>", "launches": tm["launches"],
+            "kernel": ("stencil_kernel<float,16,8,4,false,64>" if args.kernel == "full"
+                       else "herm_kernel<8,5,4,false>"), "launches": tm["launches"],
             "avg_launch_ms": round(tm["kernel_ms"] / max(1, tm["launches"]), 4),
             "alg_bytes_per_traj_step": 32 * L * L}
 

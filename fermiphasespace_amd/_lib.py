@@ -20,6 +20,8 @@ COULOMB_NULL, COULOMB_ONSITE, COULOMB_GENERAL = 0, 1, 2
 PULSE_NONE, PULSE_GAUSSIAN, PULSE_KNOTS = 0, 1, 2
 ENGINE_FP64_PLAIN, ENGINE_FP64_BATCHED, ENGINE_FP32_BATCHED = 0, 1, 2
 ENGINES = {"fp64_plain": ENGINE_FP64_PLAIN, "fp64": ENGINE_FP64_BATCHED, "fp32": ENGINE_FP32_BATCHED}
+KERNEL_AUTO, KERNEL_FULL, KERNEL_HERMITIAN = 0, 1, 2
+KERNELS = {"auto": KERNEL_AUTO, "full": KERNEL_FULL, "hermitian": KERNEL_HERMITIAN}
 
 RES_DIAG, RES_DIAG2, RES_DOUBLONS, RES_ENERGIES, RES_WHOLE = 0x01, 0x02, 0x04, 0x08, 0x10
 
@@ -92,7 +94,7 @@ def load(path: str = LIB_PATH):
         "fpsftwa_get_result": ([H, i32, P(d), sz, P(sz)], ctypes.c_int),
         "fpsftwa_get_states": ([H, i64, i64, P(d), sz], ctypes.c_int),
         "fpsftwa_get_timing": ([H, P(i64), P(d), P(d)], ctypes.c_int),
-        "fpsftwa_set_tuning": ([H, i32, i32], ctypes.c_int),
+        "fpsftwa_set_tuning": ([H, i32, i32, i32], ctypes.c_int),
         "fpsftwa_run_json_file": ([ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
         "fpsftwa_load_system_json": ([ctypes.c_char_p, P(i32), P(i32), P(d), P(d), P(d), P(d), P(d), P(d), P(d)],
                                      ctypes.c_int),
@@ -237,8 +239,9 @@ class Engine:
         check(self._lib.fpsftwa_sample_noise(self._h, first, count, _dptr(out), out.size), "sample_noise")
         return out
 
-    def set_tuning(self, block_threads=0, time_chunk=0):
-        check(self._lib.fpsftwa_set_tuning(self._h, block_threads, time_chunk), "set_tuning")
+    def set_tuning(self, block_threads=0, time_chunk=0, kernel="auto"):
+        k = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        check(self._lib.fpsftwa_set_tuning(self._h, block_threads, time_chunk, k), "set_tuning")
 
     def run(self):
         check(self._lib.fpsftwa_run(self._h), "run")

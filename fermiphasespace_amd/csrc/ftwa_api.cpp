@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fps_ftwa.h"
@@ -90,6 +91,8 @@ struct fpsftwa_handle {
     std::vector<double> noise;
     int nt_hint = 0;
     int time_chunk = 0;
+    int kernel = FPSFTWA_KERNEL_AUTO;
+    bool used_herm = false;
     // results
     bool has_result = false;
     int T = 0;
@@ -313,7 +316,16 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     int rc = upload_system<Real>(h, ds, h->T);
     if (rc) return rc;
     int nt = h->nt_hint;
-    if (!stencil_fits<Real>(h->L, ds.z, &nt))
+    bool herm = false;
+    if constexpr (std::is_same<Real, float>::value) {
+        herm = h->kernel != FPSFTWA_KERNEL_FULL && herm_fits(h->L, ds.z);
+        if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !herm)
+            return fail(FPSFTWA_EUNSUPPORTED, "the upper-triangle kernel does not fit L=" + std::to_string(h->L));
+    } else if (h->kernel == FPSFTWA_KERNEL_HERMITIAN) {
+        return fail(FPSFTWA_EUNSUPPORTED, "the upper-triangle kernel is FP32 only");
+    }
+    h->used_herm = herm;
+    if (!herm && !stencil_fits<Real>(h->L, ds.z, &nt))
         return fail(FPSFTWA_EUNSUPPORTED, "the batched engine does not support L=" + std::to_string(h->L) +
                                               " with " + std::to_string(ds.z) +
                                               " neighbours per site in this precision; use FPSFTWA_ENGINE_FP64_PLAIN");
@@ -339,7 +351,12 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     for (int t0 = 0; t0 < h->T; t0 += Tc) {
         const int t1 = std::min(h->T, t0 + Tc);
         HIP_TRY(kt.mark(h->stream));
-        HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
+        if constexpr (std::is_same<Real, float>::value) {
+            if (herm) HIP_TRY(launch_herm(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->nt_hint, h->stream));
+            else HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
+        } else {
+            HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
+        }
         HIP_TRY(kt.mark(h->stream));
         ++h->launches;
         HIP_TRY(launch_reduce<Real>(L, h->B, R, t1 - t0, t0, h->rec.as<Real>(), h->part.as<double>(), nchunk,
@@ -760,11 +777,13 @@ double pulse_amplitude(const PulseSpec& p, double t)
 }
 }  // namespace fpsamd
 
-extern "C" int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk)
+extern "C" int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel)
 {
     if (int rc = check_handle(h)) return rc;
-    if ((block_threads != 0 && block_threads != 512 && block_threads != 1024) || time_chunk < 0)
-        return fail(FPSFTWA_EINVAL, "block_threads must be 0 (automatic), 512 or 1024, and time_chunk >= 0");
+    if (kernel < FPSFTWA_KERNEL_AUTO || kernel > FPSFTWA_KERNEL_HERMITIAN) return fail(FPSFTWA_EINVAL, "bad kernel");
+    h->kernel = kernel;
+    if ((block_threads != 0 && block_threads != 512 && block_threads != 832 && block_threads != 1024) || time_chunk < 0)
+        return fail(FPSFTWA_EINVAL, "block_threads must be 0 (automatic), 512, 832 or 1024, and time_chunk >= 0");
     h->nt_hint = block_threads;
     h->time_chunk = time_chunk;
     return FPSFTWA_OK;

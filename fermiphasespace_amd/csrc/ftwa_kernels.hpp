@@ -80,6 +80,11 @@ inline int stencil_padded_z(int z) { return z <= 4 ? 4 : (z <= 8 ? 8 : 0); }
 template <typename Real>
 bool stencil_fits(int L, int z, int* nt);
 
+// FP32 batched integrator on the upper triangle (X = T rho exchanged through LDS); L <= 64, z <= 8.
+bool herm_fits(int L, int z);
+hipError_t launch_herm(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
+                       int nt_hint, hipStream_t st);
+
 // Plain FP64 path: one launch per time point, dense Delta, global-memory products.
 hipError_t launch_plain(const DevSystem<double>& s, const RunShape& r, int t, double* state, double* work,
                         double* rec, hipStream_t st);

@@ -45,10 +45,24 @@ constexpr size_t kLdsLimit = 160 * 1024;
 template <int N>
 using Ic = std::integral_constant<int, N>;
 
+// Value of v held by lane `lane` (wave-uniform index), broadcast to the whole wavefront.
+__device__ __forceinline__ float lane_value(float v, int lane)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+__device__ __forceinline__ double lane_value(double v, int lane)
+{
+    const long long bits = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), lane);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
 // LT > 0 fixes the site count at compile time (all index arithmetic folds); LT = 0 reads it from s.L.
 template <typename Real, int W, int EPT, int Z, bool PULSE, int LT>
 __global__ void __launch_bounds__(64 * W)
-    stencil_kernel(DevSystem<Real> s, RunShape r, int t0, int t1, Real* __restrict__ state, Real* __restrict__ rec)
+    stencil_kernel(DevSystem<Real> s, RunShape r, int t0, int t1, Real* __restrict__ state, Real* __restrict__ rec,
+                   const RowNbrs<Real, Z>* __restrict__ rows)
 {
     using C = V2<Real>;
     constexpr int NT = 64 * W;
@@ -70,7 +84,6 @@ __global__ void __launch_bounds__(64 * W)
     const Real* __restrict__ nbt = s.nbr_t;
     const Real* __restrict__ udiag = s.u_diag;
     const Real* __restrict__ pdiag = s.pdiag;
-    const RowNbrs<Real, Z>* __restrict__ rows = static_cast<const RowNbrs<Real, Z>*>(s.rows);
 
     C* Y = reinterpret_cast<C*>(smem);                                  // [2L][64]
     C* Tc = Y + R2 * kRow;                                              // [Z][L] (pulse)
@@ -87,6 +100,9 @@ __global__ void __launch_bounds__(64 * W)
         ck[n] = jon ? nbk[n * L + j] : j;
         ct[n] = jon ? nbt[n * L + j] : Real(0);
     }
+    // lane j keeps U_jj and the diagonal potential of site j: the Hartree update of row i reads lane i
+    const Real uj = (jon && s.coulomb_type == 1) ? udiag[j] : Real(0);
+    const Real pj = jon ? pdiag[j] : Real(0);
     __syncthreads();
 
     C y[EPT], A[EPT], p[EPT];
@@ -191,7 +207,8 @@ __global__ void __launch_bounds__(64 * W)
             const C* Yr = Y + row * kRow;
             const C* Ys = Y + sg * L * kRow;
             const C yij = (ST == 0) ? y[m] : Yr[jo];
-            const Real dh = (sg ? hj1 : hj0) - hb[sg * kRow + i];
+            const Real hjs = sg ? hj1 : hj0;
+            const Real dh = hjs - lane_value(hjs, i);  // h_j - h_i, h_i from lane i
             Real ax = dh * yij.x, ay = dh * yij.y;
 #pragma unroll
             for (int n = 0; n < Z; ++n) {
@@ -258,18 +275,7 @@ __global__ void __launch_bounds__(64 * W)
                 // so there is no divergent branch.  With a general U/V, h_phase overwrites it below.
                 const int sg = kExact ? (m >= EPT / 2 ? 1 : 0) : (row >= L ? 1 : 0);
                 const int i = row - sg * L;
-                Real dv;
-                if constexpr (std::is_same<Real, float>::value) {
-                    dv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, yn.x), i));
-                } else {
-                    const long long bits = __builtin_bit_cast(long long, yn.x);
-                    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), i);
-                    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), i);
-                    dv = __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) |
-                                                        static_cast<unsigned int>(lo));
-                }
-                const Real uc = (s.coulomb_type == 1) ? udiag[i] : Real(0);
-                hb[(1 - sg) * kRow + i] = uc * (dv + Real(0.5)) + pdiag[i];
+                hb[(1 - sg) * kRow + i] = lane_value(uj * (yn.x + Real(0.5)) + pj, i);
             }
         }
         pulse_table(gidx + 1);
@@ -402,7 +408,8 @@ hipError_t run(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real
         attr_set = true;
     }
     const size_t lds = lds_bytes<Real>(s.L, s.z, W, PULSE);
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(64 * W), lds, st, s, r, t0, t1, state, rec);
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(64 * W), lds, st, s, r, t0, t1, state, rec,
+                       static_cast<const RowNbrs<Real, Z>*>(s.rows));
     return hipGetLastError();
 }
 

@@ -161,9 +161,12 @@ int fpsftwa_get_states(fpsftwa_handle* h, int64_t first, int64_t count, double* 
 int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, double* run_ms);
 
 /* Engine tuning (no reference counterpart; the autotuner's knobs): threads per trajectory
- * workgroup for the batched engine (512 or 1024; 0 = automatic) and time points per launch
- * (0 = automatic). */
-int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk);
+ * workgroup for the full-matrix batched kernel (512 or 1024; 0 = automatic), time points per launch
+ * (0 = automatic) and the FP32 kernel variant (FPSFTWA_KERNEL_*). */
+#define FPSFTWA_KERNEL_AUTO 0      /* upper-triangle kernel where it fits, else full-matrix       */
+#define FPSFTWA_KERNEL_FULL 1      /* full-matrix stencil kernel                                   */
+#define FPSFTWA_KERNEL_HERMITIAN 2 /* upper-triangle kernel (FP32, L <= 64)                         */
+int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel);
 
 /* fps.cpp:88-207 for "method":"fTWA": read <path>, build the system, run, and write
  * <folder><sim_name>_fTWA_res.json (the reference's results format, src/FHSolver.cpp:171-307).

@@ -63,7 +63,14 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
     ref = O.run(oracle_system(O, tM, coulomb_type, U, V, pot, pulse), dt, t_max, n_sub, n_up, n_dn,
                 noise_arr=noise, keep_states=True, rot_up=None if rot is None else rot[0],
                 rot_dn=None if rot is None else rot[1])
+    runs = []
     for e in engines:
+        if e == "fp32":  # both FP32 kernels: full-matrix and upper-triangle (where it fits)
+            runs += [("fp32", "full")] + ([("fp32", "hermitian")] if L <= 64 else [])
+        else:
+            runs.append((e, "auto"))
+    for e, kern in runs:
+        eng.set_tuning(0, 0, kern)
         got = engine_run(eng, dt, t_max, n_sub, B, e, seed=seed, noise=noise)
         tol = fp32_tol if e == "fp32" else FP64_RTOL
         print(f"L={L} {e}: states {rel(got['states'], ref.final_states):.2e}")
@@ -185,8 +192,8 @@ def test_tuning_and_chunking_are_bitwise_invariant():
     eng = _lib.Engine(tM, U=Um)
     eng.set_initial_state(occ, occ)
     ref = None
-    for block, chunk in [(0, 0), (512, 1), (1024, 7), (512, 3)]:
-        eng.set_tuning(block, chunk)
+    for block, chunk, kern in [(0, 0, "full"), (512, 1, "full"), (1024, 7, "full"), (512, 3, "full")]:
+        eng.set_tuning(block, chunk, kern)
         got = engine_run(eng, 0.005, 0.1, 1, 64, "fp32", seed=9)
         if ref is None:
             ref = got
@@ -195,6 +202,18 @@ def test_tuning_and_chunking_are_bitwise_invariant():
                 np.testing.assert_array_equal(got[k], ref[k], err_msg=f"{block} {chunk} {k}")
     again = engine_run(eng, 0.005, 0.1, 1, 64, "fp32", seed=9)
     np.testing.assert_array_equal(again["sum_u"], ref["sum_u"])  # deterministic run to run
+    # the upper-triangle kernel: chunking changes nothing either, and it agrees with the full kernel
+    href = None
+    for chunk, block in ((0, 0), (1, 832), (6, 512), (3, 1024)):
+        eng.set_tuning(block, chunk, "hermitian")
+        got = engine_run(eng, 0.005, 0.1, 1, 64, "fp32", seed=9)
+        if href is None:
+            href = got
+        else:
+            for k in href:
+                np.testing.assert_array_equal(got[k], href[k], err_msg=f"hermitian {chunk} {k}")
+    assert rel(href["states"], ref["states"]) < 1e-5
+    assert np.max(np.abs(href["states"] - np.conj(np.transpose(href["states"], (0, 1, 3, 2))))) == 0
 
 
 def test_fp32_tracks_fp64_on_identical_samples():
