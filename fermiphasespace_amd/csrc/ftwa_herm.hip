@@ -4,20 +4,29 @@
 // organised around X = T rho, the hopping applied from the left:
 //
 //   rho T - T rho = X^dagger - X      (rho and T Hermitian: (rho T)_ij = conj(X_ji))
-//   k_ij = -i [ conj(X_ji) - X_ij + (h_j - h_i) rho_ij ]
+//   k_ij = -i [ conj(X_ji) - X_ij + (h_j - h_i) rho_ij ]          k_ii = -2 Im X_ii
 //
-// Per RK stage and trajectory (one workgroup):
+// One workgroup integrates one trajectory.  Per RK stage:
 //   phase A  wavefront w forms full rows of X, lane j = column j:  X_ij = sum_{k in nb(i)} t_ik Y_kj.
-//            The row index is wave-uniform, so nb(i) and t_ik are one scalar load; every read of Y is a
-//            contiguous row segment.  X goes to LDS.                                      -> barrier
-//   phase B  each thread owns upper-triangle elements (i <= j) of both spins, balanced over the
-//            workgroup, with their RK4 "3/8" registers (y, A, p).  It reads X_ij and X_ji, forms k,
-//            updates (y, A, p), and writes the next stage input Y_ij and its mirror Y_ji = conj(Y_ij).
-//            The owners of diagonal elements write the other spin's next Hartree potential into the
-//            other half of a double-buffered h.                                           -> barrier
-// Only the upper triangle carries RK state: half the registers and half the update arithmetic of the
-// full-matrix kernel, and the state stays exactly Hermitian (real diagonal).
-// LDS rows are padded to 65 complex so the transposed accesses of phase B hit distinct banks.
+//            The row index is wave-uniform: nb(i) and t_ik come from one scalar load, every read of Y is
+//            a contiguous row segment, X goes to LDS.                                     -> barrier
+//   phase B  the strict upper triangle of both spins is packed into 64-lane "element rows", followed
+//            by the diagonals; element row r belongs to wave r % W, slot r / W.  Each thread keeps, per
+//            slot, the RK4 "3/8" registers (y, A, p) and the LDS byte addresses of its element and of
+//            the mirror, so a stage is: read X_ij, X_ji, h_i, h_j, form k, update, write Y_ij and
+//            Y_ji = conj(Y_ij).  Slot kinds are wave-uniform; diagonal slots also write the other
+//            spin's next Hartree potential.                                               -> barrier
+// Only the upper triangle carries RK state (half the registers and update arithmetic of the
+// full-matrix kernel) and the state stays exactly Hermitian (real diagonal).
+//
+// LDS (bytes):
+//   [0, 1280)     h double buffer [2][160] floats: entry sg*64 + i, entries 128.. absorb padding lanes
+//   [1280, 1792)  u_ii [64] (on-site Coulomb; 0 otherwise), pdiag [64]
+//   [1792, 2048)  energy partials [W][4]
+//   [2048, ...)   rows r = sg*L + i (2L of them, plus a spare row 2L), 129 complex each:
+//                 Y_r,0..63 | X_r,0..63 | pad.  X_ij sits 512 bytes after Y_ij; the odd stride keeps
+//                 the transposed (column) accesses of phase B on distinct banks.
+//   then          pulse hoppings t (1 + e^{i phi}) [Z][L] complex
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -33,16 +42,30 @@ namespace {
 using dev::wave_sum;
 using C = float2;
 
-constexpr int kLP = 65;  // LDS row stride in complex elements
+constexpr int kRowBytes = 129 * 8;  // LDS row: Y (64 complex), X (64 complex), pad
+constexpr int kXOff = 64 * 8;       // X_ij - Y_ij, bytes
+constexpr int kHB = 0;              // h buffers
+constexpr int kHBStride = 160 * 4;  // bytes per h buffer
+constexpr int kCoefU = 1280;        // u_ii
+constexpr int kCoefP = 1536;        // pdiag
+constexpr int kEW = 1792;           // energy partials
+constexpr int kRows = 2048;         // first row
 constexpr size_t kLdsLimit = 160 * 1024;
 
 template <int N>
 using Ic = std::integral_constant<int, N>;
 
-__device__ __forceinline__ C cmul(C a, C b) { return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ C lds_c(const unsigned char* base, int off)
+{
+    return *reinterpret_cast<const C*>(base + off);
+}
+__device__ __forceinline__ float lds_f(const unsigned char* base, int off)
+{
+    return *reinterpret_cast<const float*>(base + off);
+}
 
-// EPTA: rows per wavefront in phase A; EPTB: upper-triangle elements per thread in phase B.
-template <int W, int EPTA, int EPTB, int Z, bool PULSE>
+// EPTA: rows per wavefront in phase A; S: element slots per thread in phase B.
+template <int W, int EPTA, int S, int Z, bool PULSE>
 __global__ void __launch_bounds__(64 * W)
     herm_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state, float* __restrict__ rec,
                 const RowNbrs<float, Z>* __restrict__ rows)
@@ -52,61 +75,87 @@ __global__ void __launch_bounds__(64 * W)
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
-    const int NU = L * (L + 1) / 2;  // upper-triangle elements per spin
+    const int NS = L * (L - 1) / 2;             // strict upper elements per spin
+    const int n_off = (2 * NS + 63) / 64;       // element rows of off-diagonal elements
+    const int n_dia = (R2 + 63) / 64;           // element rows of diagonal elements
     const int tid = threadIdx.x;
     const int j = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t b = blockIdx.x;
     const bool general = s.coulomb_type == 2;
+    const int spare = kRows + R2 * kRowBytes;  // spare row: target of padding lanes and rows
+    C* Tc = reinterpret_cast<C*>(smem + kRows + (R2 + 1) * kRowBytes);  // [Z][L]
+    float* ew = reinterpret_cast<float*>(smem + kEW);
 
-    C* Y = reinterpret_cast<C*>(smem);                // [2L][kLP] stage input, both triangles
-    C* X = Y + R2 * kLP;                              // [2L][kLP] X = T Y
-    C* Tc = X + R2 * kLP;                             // [Z][L] pulse hopping t (1 + e^{i phi})
-    float* hb = reinterpret_cast<float*>(PULSE ? (Tc + Z * L) : Tc);  // [2 buffers][2 spins][64]
-    float* ew = hb + 4 * 64;                          // [W][4]
+    auto yat = [&](int row, int col) -> C& {
+        return *reinterpret_cast<C*>(smem + kRows + row * kRowBytes + col * 8);
+    };
 
-    // ---- phase-B ownership: element e = tid + kNT*q, spin-major, row-major upper triangle ----
-    int own[EPTB];  // packed (spin << 16) | (i << 8) | j, or -1
-#pragma unroll
-    for (int q = 0; q < EPTB; ++q) {
-        const int e = tid + kNT * q;
-        if (e < 2 * NU) {
-            const int sg = e >= NU ? 1 : 0;
-            int u = e - sg * NU;
-            // row i: the largest i with i*L - i*(i-1)/2 <= u
-            const float Lf = static_cast<float>(L) + 0.5f;
-            int i = static_cast<int>(Lf - sqrtf(Lf * Lf - 2.0f * static_cast<float>(u)));
-            if (i < 0) i = 0;
-            while (i > 0 && i * L - i * (i - 1) / 2 > u) --i;
-            while ((i + 1) * L - (i + 1) * i / 2 <= u) ++i;
-            const int jj = i + (u - (i * L - i * (i - 1) / 2));
-            own[q] = (sg << 16) | (i << 8) | jj;
-        } else {
-            own[q] = -1;
-        }
+    // ---- clear the rows (columns past L stay finite), h buffers and tables ----
+    {
+        const int n16 = (kRows + (R2 + 1) * kRowBytes) / 16;
+        for (int e = tid; e < n16; e += kNT) reinterpret_cast<float4*>(smem)[e] = float4{0, 0, 0, 0};
+    }
+    __syncthreads();
+    for (int i = tid; i < L; i += kNT) {
+        *reinterpret_cast<float*>(smem + kCoefU + i * 4) = s.coulomb_type == 1 ? s.u_diag[i] : 0.0f;
+        *reinterpret_cast<float*>(smem + kCoefP + i * 4) = s.pdiag[i];
     }
 
-    for (int e = tid; e < 4 * 64; e += kNT) hb[e] = 0.0f;
-    C y[EPTB], A[EPTB], p[EPTB];
+    // ---- element slots: LDS byte addresses of Y_ij (aij) and of the mirror Y_ji (aji), h offsets ----
+    // Off-diagonal: ah = (sg*64 + i)*4 | (sg*64 + j)*4 << 16.  Diagonal: aji = i*4 (coefficient
+    // offset), ah = ((1 - sg)*64 + i)*4 (the other spin's next h).  Padding lanes point at the spare
+    // row and at h entry 128.
+    int aij[S], aji[S], ah[S];
+    C y[S], A[S], p[S];
     const C* gs = reinterpret_cast<const C*>(state) + b * 2 * LL;
 #pragma unroll
-    for (int q = 0; q < EPTB; ++q) {
+    for (int q = 0; q < S; ++q) {
+        const int er = q * W + w;
+        aij[q] = spare;
+        aji[q] = spare;
+        ah[q] = 128 * 4;
         y[q] = C{0, 0};
         A[q] = C{0, 0};
         p[q] = C{0, 0};
-        if (own[q] >= 0) {
-            const int sg = own[q] >> 16, i = (own[q] >> 8) & 255, jj = own[q] & 255;
-            y[q] = gs[sg * LL + i * L + jj];
-            if (i == jj) y[q].y = 0.0f;
-            Y[(sg * L + i) * kLP + jj] = y[q];
-            Y[(sg * L + jj) * kLP + i] = C{y[q].x, -y[q].y};
+        if (er < n_off) {
+            const int e = er * 64 + j;
+            if (e < 2 * NS) {
+                const int sg = e >= NS ? 1 : 0;
+                const int u = e - sg * NS;
+                // row i of the strict upper triangle: the largest i with i*(2L - i - 1)/2 <= u
+                const float c = static_cast<float>(2 * L - 1);
+                int i = static_cast<int>(0.5f * (c - sqrtf(fmaxf(c * c - 8.0f * static_cast<float>(u), 0.0f))));
+                if (i < 0) i = 0;
+                while (i > 0 && i * (2 * L - i - 1) / 2 > u) --i;
+                while ((i + 1) * (2 * L - i - 2) / 2 <= u) ++i;
+                const int jj = i + 1 + (u - i * (2 * L - i - 1) / 2);
+                aij[q] = kRows + (sg * L + i) * kRowBytes + jj * 8;
+                aji[q] = kRows + (sg * L + jj) * kRowBytes + i * 8;
+                ah[q] = (sg * 64 + i) * 4 | ((sg * 64 + jj) * 4) << 16;
+                y[q] = gs[sg * LL + i * L + jj];
+                *reinterpret_cast<C*>(smem + aij[q]) = y[q];
+                *reinterpret_cast<C*>(smem + aji[q]) = C{y[q].x, -y[q].y};
+            }
+        } else if (er < n_off + n_dia) {
+            const int d = (er - n_off) * 64 + j;
+            if (d < R2) {
+                const int sg = d >= L ? 1 : 0;
+                const int i = d - sg * L;
+                aij[q] = kRows + (sg * L + i) * kRowBytes + i * 8;
+                aji[q] = i * 4;
+                ah[q] = ((1 - sg) * 64 + i) * 4;
+                y[q] = C{gs[sg * LL + i * L + i].x, 0.0f};
+                *reinterpret_cast<C*>(smem + aij[q]) = y[q];
+            } else {
+                aji[q] = 0;
+            }
         }
     }
     const float h1 = static_cast<float>(r.h);
     const float h3 = static_cast<float>(r.h / 3.0), h6 = static_cast<float>(r.h / 6.0),
                 h8 = static_cast<float>(r.h / 8.0);
     int64_t gst = static_cast<int64_t>(t0) * r.nsub * 4;
-    int hbuf = 0;  // which half of hb holds the current stage's Hartree potential
 
     auto pulse_table = [&](int64_t stage_index) {
         if constexpr (PULSE) {
@@ -119,18 +168,19 @@ __global__ void __launch_bounds__(64 * W)
             }
         }
     };
+    // h of both spins from the diagonal in Y, into buffer buf
     auto h_phase = [&](int buf) {
-        float* h = hb + buf * 128;
+        float* h = reinterpret_cast<float*>(smem + kHB + buf * kHBStride);
         for (int e = tid; e < R2; e += kNT) {
             const int sg = e >= L ? 1 : 0, i = e - sg * L;
             float v = s.pdiag[i];
             if (s.coulomb_type == 1) {
-                v += s.u_diag[i] * (Y[((1 - sg) * L + i) * kLP + i].x + 0.5f);
+                v += s.u_diag[i] * (yat((1 - sg) * L + i, i).x + 0.5f);
             } else if (general) {
                 float a = 0.0f;
                 for (int jj = 0; jj < L; ++jj)
-                    a += s.U[i * L + jj] * (Y[((1 - sg) * L + jj) * kLP + jj].x + 0.5f) +
-                         s.V[i * L + jj] * (Y[(sg * L + jj) * kLP + jj].x + 0.5f);
+                    a += s.U[i * L + jj] * (yat((1 - sg) * L + jj, jj).x + 0.5f) +
+                         s.V[i * L + jj] * (yat(sg * L + jj, jj).x + 0.5f);
                 v += a;
             }
             h[sg * 64 + i] = v;
@@ -151,25 +201,33 @@ __global__ void __launch_bounds__(64 * W)
         }
     };
 
+    // Stage ST reads h buffer ST & 1 and writes buffer (ST + 1) & 1: four stages per step, so every
+    // step starts on buffer 0.
     auto stage = [&](auto stc, int64_t gidx, bool flush) {
         constexpr int ST = decltype(stc)::value;
+        constexpr int HC = kHB + (ST & 1) * kHBStride;
+        constexpr int HN = kHB + ((ST + 1) & 1) * kHBStride;
         // ---- phase A: X = T Y, full rows ----
         {
             int wo = w;
             int jo = j;
             asm volatile("" : "+s"(wo), "+v"(jo));
+            const unsigned char* Yl = smem + kRows + jo * 8;
+            unsigned char* Xl = smem + kRows + kXOff + jo * 8;
+            // branch-free over the wave's rows, so all their LDS reads can be in flight together: rows
+            // past 2L reuse the last row's neighbours and write the spare row, which nothing reads
 #pragma unroll
             for (int m = 0; m < EPTA; ++m) {
                 const int row = wo + W * m;
-                if (row >= R2) break;
-                const int sg = row >= L ? 1 : 0;
-                const int i = row - sg * L;
+                const int rw = min(row, R2);
+                const int rs = min(row, R2 - 1);
+                const int sg = rs >= L ? 1 : 0;
+                const int i = rs - sg * L;
                 const RowNbrs<float, Z> nb = rows[i];
-                const C* Ys = Y + sg * L * kLP;
                 float ax = 0.0f, ay = 0.0f;
 #pragma unroll
                 for (int n = 0; n < Z; ++n) {
-                    const C v = Ys[nb.k[n] * kLP + jo];
+                    const C v = lds_c(Yl, (sg * L + nb.k[n]) * kRowBytes);
                     if constexpr (PULSE) {
                         const C T = Tc[n * L + i];
                         ax += T.x * v.x - T.y * v.y;
@@ -179,77 +237,83 @@ __global__ void __launch_bounds__(64 * W)
                         ay += nb.t[n] * v.y;
                     }
                 }
-                X[row * kLP + jo] = C{ax, ay};
+                *reinterpret_cast<C*>(Xl + rw * kRowBytes) = C{ax, ay};
             }
         }
         __syncthreads();
         if (ST == 0 && flush) flush_energies();
-        // ---- phase B: upper-triangle elements ----
-        const float* hc = hb + hbuf * 128;
-        float* hn = hb + (hbuf ^ 1) * 128;
-        // element indices are made opaque per stage: their LDS addresses are formed here, not hoisted
-        // out of the time loop into registers for every element
-        int ow[EPTB];
+        // ---- phase B ----
 #pragma unroll
-        for (int q = 0; q < EPTB; ++q) {
-            ow[q] = own[q];
-            asm volatile("" : "+v"(ow[q]));
-        }
-#pragma unroll
-        for (int q = 0; q < EPTB; ++q) {
-            if (ow[q] < 0) continue;
-            const int sg = ow[q] >> 16, i = (ow[q] >> 8) & 255, jj = ow[q] & 255;
-            const C xij = X[(sg * L + i) * kLP + jj];
-            const C xji = X[(sg * L + jj) * kLP + i];
-            const float dh = hc[sg * 64 + jj] - hc[sg * 64 + i];
-            // current stage input from the registers
-            C yc;
-            if constexpr (ST == 0) {
-                yc = y[q];
-            } else if constexpr (ST == 1) {
-                yc = C{y[q].x + h3 * p[q].x, y[q].y + h3 * p[q].y};
-            } else if constexpr (ST == 2) {
-                yc = C{y[q].x + h6 * (A[q].x - 3.0f * p[q].x), y[q].y + h6 * (A[q].y - 3.0f * p[q].y)};
-            } else {
-                yc = C{y[q].x + h1 * p[q].x, y[q].y + h1 * p[q].y};
-            }
-            // c = conj(X_ji) - X_ij + dh * Y_ij;  k = -i c
-            const float cx = xji.x - xij.x + dh * yc.x;
-            const float cy = -xji.y - xij.y + dh * yc.y;
-            const float kx = cy, ky = -cx;
-            C yn;
-            if constexpr (ST == 0) {
-                A[q] = C{kx, ky};
-                p[q] = C{kx, ky};
-                yn = C{y[q].x + h3 * kx, y[q].y + h3 * ky};
-            } else if constexpr (ST == 1) {
-                A[q] = C{A[q].x + 3.0f * kx, A[q].y + 3.0f * ky};
-                p[q] = C{p[q].x - kx, p[q].y - ky};
-                yn = C{y[q].x + h6 * (A[q].x - 3.0f * p[q].x), y[q].y + h6 * (A[q].y - 3.0f * p[q].y)};
-            } else if constexpr (ST == 2) {
-                A[q] = C{A[q].x + 3.0f * kx, A[q].y + 3.0f * ky};
-                p[q] = C{p[q].x + kx, p[q].y + ky};
-                yn = C{y[q].x + h1 * p[q].x, y[q].y + h1 * p[q].y};
-            } else {
-                A[q] = C{A[q].x + kx, A[q].y + ky};
-                y[q] = C{y[q].x + h8 * A[q].x, y[q].y + h8 * A[q].y};
-                yn = y[q];
-            }
-            if (i == jj) {
-                yn.y = 0.0f;
-                Y[(sg * L + i) * kLP + i] = yn;
-                const float uc = (s.coulomb_type == 1) ? s.u_diag[i] : 0.0f;
-                hn[(1 - sg) * 64 + i] = uc * (yn.x + 0.5f) + s.pdiag[i];
-            } else {
-                Y[(sg * L + i) * kLP + jj] = yn;
-                Y[(sg * L + jj) * kLP + i] = C{yn.x, -yn.y};
+        for (int q = 0; q < S; ++q) {
+            const int er = q * W + w;
+            if (er < n_off) {
+                const C xij = lds_c(smem, aij[q] + kXOff);
+                const C xji = lds_c(smem, aji[q] + kXOff);
+                const float dh = lds_f(smem, HC + (ah[q] >> 16)) - lds_f(smem, HC + (ah[q] & 0xffff));
+                C yc;  // current stage input
+                if constexpr (ST == 0) {
+                    yc = y[q];
+                } else if constexpr (ST == 1) {
+                    yc = C{y[q].x + h3 * p[q].x, y[q].y + h3 * p[q].y};
+                } else if constexpr (ST == 2) {
+                    yc = C{y[q].x + h6 * (A[q].x - 3.0f * p[q].x), y[q].y + h6 * (A[q].y - 3.0f * p[q].y)};
+                } else {
+                    yc = C{y[q].x + h1 * p[q].x, y[q].y + h1 * p[q].y};
+                }
+                // c = conj(X_ji) - X_ij + dh * Y_ij;  k = -i c
+                const float cx = xji.x - xij.x + dh * yc.x;
+                const float cy = -xji.y - xij.y + dh * yc.y;
+                const float kx = cy, ky = -cx;
+                C yn;
+                if constexpr (ST == 0) {
+                    A[q] = C{kx, ky};
+                    p[q] = C{kx, ky};
+                    yn = C{y[q].x + h3 * kx, y[q].y + h3 * ky};
+                } else if constexpr (ST == 1) {
+                    A[q] = C{A[q].x + 3.0f * kx, A[q].y + 3.0f * ky};
+                    p[q] = C{p[q].x - kx, p[q].y - ky};
+                    yn = C{y[q].x + h6 * (A[q].x - 3.0f * p[q].x), y[q].y + h6 * (A[q].y - 3.0f * p[q].y)};
+                } else if constexpr (ST == 2) {
+                    A[q] = C{A[q].x + 3.0f * kx, A[q].y + 3.0f * ky};
+                    p[q] = C{p[q].x + kx, p[q].y + ky};
+                    yn = C{y[q].x + h1 * p[q].x, y[q].y + h1 * p[q].y};
+                } else {
+                    A[q] = C{A[q].x + kx, A[q].y + ky};
+                    y[q] = C{y[q].x + h8 * A[q].x, y[q].y + h8 * A[q].y};
+                    yn = y[q];
+                }
+                *reinterpret_cast<C*>(smem + aij[q]) = yn;
+                *reinterpret_cast<C*>(smem + aji[q]) = C{yn.x, -yn.y};
+            } else if (er < n_off + n_dia) {
+                // diagonal: k = -2 Im X_ii, real RK registers in .x
+                const float k = -2.0f * lds_f(smem, aij[q] + kXOff + 4);
+                float yn;
+                if constexpr (ST == 0) {
+                    A[q].x = k;
+                    p[q].x = k;
+                    yn = y[q].x + h3 * k;
+                } else if constexpr (ST == 1) {
+                    A[q].x += 3.0f * k;
+                    p[q].x -= k;
+                    yn = y[q].x + h6 * (A[q].x - 3.0f * p[q].x);
+                } else if constexpr (ST == 2) {
+                    A[q].x += 3.0f * k;
+                    p[q].x += k;
+                    yn = y[q].x + h1 * p[q].x;
+                } else {
+                    A[q].x += k;
+                    y[q].x += h8 * A[q].x;
+                    yn = y[q].x;
+                }
+                *reinterpret_cast<float*>(smem + aij[q]) = yn;
+                *reinterpret_cast<float*>(smem + HN + ah[q]) =
+                    lds_f(smem, kCoefU + aji[q]) * (yn + 0.5f) + lds_f(smem, kCoefP + aji[q]);
             }
         }
         pulse_table(gidx + 1);
-        hbuf ^= 1;
         __syncthreads();
         if (general) {
-            h_phase(hbuf);
+            h_phase((ST + 1) & 1);
             __syncthreads();
         }
     };
@@ -261,10 +325,10 @@ __global__ void __launch_bounds__(64 * W)
         {
             for (int e = tid; e < R2; e += kNT) {
                 const int sg = e >= L ? 1 : 0, i = e - sg * L;
-                rr[e] = Y[(sg * L + i) * kLP + i].x;
+                rr[e] = yat(sg * L + i, i).x;
             }
             if (tid == 0) {
-                const C cu = Y[L - 1], cd = Y[L * kLP + L - 1];
+                const C cu = yat(0, L - 1), cd = yat(L, L - 1);
                 rr[R2 + 4] = cu.x;
                 rr[R2 + 5] = cu.y;
                 rr[R2 + 6] = cd.x;
@@ -273,12 +337,12 @@ __global__ void __launch_bounds__(64 * W)
             float eth = 0, eu = 0, ev = 0, epe = 0;
             if (tid < L) {
                 const int i = tid;
-                const float nu = Y[i * kLP + i].x + 0.5f;
-                const float nd = Y[(L + i) * kLP + i].x + 0.5f;
+                const float nu = yat(i, i).x + 0.5f;
+                const float nd = yat(L + i, i).x + 0.5f;
                 for (int n = 0; n < Z; ++n) {
                     const int k = s.nbr_k[n * L + i];
                     const float tt = s.nbr_t[n * L + i];
-                    const C a = Y[i * kLP + k], c = Y[(L + i) * kLP + k];
+                    const C a = yat(i, k), c = yat(L + i, k);
                     eth += (a.x + c.x) * tt;
                     if constexpr (PULSE) {
                         const C P = Tc[n * L + i];
@@ -292,8 +356,8 @@ __global__ void __launch_bounds__(64 * W)
                 } else if (general) {
                     float su = 0, sd = 0, vu = 0, vd = 0;
                     for (int jj = 0; jj < L; ++jj) {
-                        const float nuj = Y[jj * kLP + jj].x + 0.5f;
-                        const float ndj = Y[(L + jj) * kLP + jj].x + 0.5f;
+                        const float nuj = yat(jj, jj).x + 0.5f;
+                        const float ndj = yat(L + jj, jj).x + 0.5f;
                         su += s.U[i * L + jj] * ndj;
                         sd += s.U[i * L + jj] * nuj;
                         vu += s.V[i * L + jj] * nuj;
@@ -327,45 +391,55 @@ __global__ void __launch_bounds__(64 * W)
             gst += 4;
         }
     }
+    // ---- state back to HBM, from the RK registers ----
     C* go = reinterpret_cast<C*>(state) + b * 2 * LL;
 #pragma unroll
-    for (int q = 0; q < EPTB; ++q) {
-        if (own[q] < 0) continue;
-        const int sg = own[q] >> 16, i = (own[q] >> 8) & 255, jj = own[q] & 255;
-        go[sg * LL + i * L + jj] = y[q];
-        if (i != jj) go[sg * LL + jj * L + i] = C{y[q].x, -y[q].y};
+    for (int q = 0; q < S; ++q) {
+        if (aij[q] == spare) continue;
+        const int er = q * W + w;
+        const int row = (aij[q] - kRows) / kRowBytes;
+        const int col = (aij[q] - kRows - row * kRowBytes) / 8;
+        const int sg = row >= L ? 1 : 0, i = row - sg * L;
+        if (er < n_off) {
+            go[sg * LL + i * L + col] = y[q];
+            go[sg * LL + col * L + i] = C{y[q].x, -y[q].y};
+        } else {
+            go[sg * LL + i * L + i] = C{y[q].x, 0.0f};
+        }
     }
 }
 
-size_t herm_lds_bytes(int L, int z, bool pulse, int W)
+size_t herm_lds_bytes(int L, int z, bool pulse)
 {
-    return 2 * static_cast<size_t>(2 * L) * kLP * sizeof(C) + (pulse ? static_cast<size_t>(z) * L * sizeof(C) : 0) +
-           4 * 64 * sizeof(float) + static_cast<size_t>(W) * 4 * sizeof(float);
+    return kRows + static_cast<size_t>(2 * L + 1) * kRowBytes + (pulse ? static_cast<size_t>(z) * L * sizeof(C) : 0);
 }
 
-// Shape (waves W, rows per wave in phase A, upper elements per thread in phase B) for L sites.
-// 16 waves for any L <= 64; 13 waves (exactly 5 upper elements per thread at L = 64) and 8 waves are
-// alternatives for the autotuner at L = 64 (block_threads 832 or 512).
+// Shape for L sites: 16 waves; rows per wave in phase A (EA) and element slots per thread (S).
 struct HShape {
-    int W, ea, eb;
+    int W, ea, s;
 };
-HShape herm_shape(int L, int nt_hint)
+HShape herm_shape(int L)
 {
-    if (L > 64 || L < 1) return {0, 0, 0};
-    if (nt_hint == 832 && L * (L + 1) <= 832 * 5 && 2 * L <= 13 * 10) return {13, 10, 5};
-    if (nt_hint == 512 && L * (L + 1) <= 512 * 9 && 2 * L <= 8 * 16) return {8, 16, 9};
-    const int need_a = (2 * L + 15) / 16;
-    const int need_b = (L * (L + 1) + 1023) / 1024;
+    if (L > 64 || L < 2) return {0, 0, 0};
+    const int W = 16;
+    const int need_a = (2 * L + W - 1) / W;
+    const int rows_b = (L * (L - 1) + 63) / 64 + (2 * L + 63) / 64;
+    const int need_s = (rows_b + W - 1) / W;
     const int ea = need_a <= 2 ? 2 : (need_a <= 4 ? 4 : (need_a <= 8 ? 8 : 0));
-    const int eb = need_b <= 1 ? 1 : (need_b <= 2 ? 2 : (need_b <= 3 ? 3 : (need_b <= 5 ? 5 : 0)));
-    if (!ea || !eb) return {0, 0, 0};
-    return {16, ea, eb};
+    if (!ea || need_s > 5) return {0, 0, 0};
+    // compiled (EA, S) pairs: (2, 1), (4, 2), (8, 3..5)
+    int s = need_s;
+    if (ea == 2) s = need_s <= 1 ? 1 : 0;
+    else if (ea == 4) s = need_s <= 2 ? 2 : 0;
+    else s = need_s <= 3 ? 3 : need_s;
+    if (!s) return {0, 0, 0};
+    return {W, ea, s};
 }
 
-template <int W, int EA, int EB, int Z, bool PULSE>
+template <int W, int EA, int S, int Z, bool PULSE>
 hipError_t run(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec, hipStream_t st)
 {
-    auto kern = herm_kernel<W, EA, EB, Z, PULSE>;
+    auto kern = herm_kernel<W, EA, S, Z, PULSE>;
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -373,17 +447,17 @@ hipError_t run(const DevSystem<float>& s, const RunShape& r, int t0, int t1, flo
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(64 * W), herm_lds_bytes(s.L, s.z, PULSE, W), st,
-                       s, r, t0, t1, state, rec, static_cast<const RowNbrs<float, Z>*>(s.rows));
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(64 * W), herm_lds_bytes(s.L, s.z, PULSE), st, s,
+                       r, t0, t1, state, rec, static_cast<const RowNbrs<float, Z>*>(s.rows));
     return hipGetLastError();
 }
 
-template <int W, int EA, int EB, bool PULSE>
+template <int EA, int S, bool PULSE>
 hipError_t run_z(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                  hipStream_t st)
 {
-    if (s.z == 4) return run<W, EA, EB, 4, PULSE>(s, r, t0, t1, state, rec, st);
-    if (s.z == 8) return run<W, EA, EB, 8, PULSE>(s, r, t0, t1, state, rec, st);
+    if (s.z == 4) return run<16, EA, S, 4, PULSE>(s, r, t0, t1, state, rec, st);
+    if (s.z == 8) return run<16, EA, S, 8, PULSE>(s, r, t0, t1, state, rec, st);
     return hipErrorInvalidValue;
 }
 
@@ -391,18 +465,11 @@ template <bool PULSE>
 hipError_t run_shape(const HShape& h, const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state,
                      float* rec, hipStream_t st)
 {
-    if (h.W == 16) {
-        if (h.ea == 2 && h.eb == 1) return run_z<16, 2, 1, PULSE>(s, r, t0, t1, state, rec, st);
-        if (h.ea == 4 && h.eb == 1) return run_z<16, 4, 1, PULSE>(s, r, t0, t1, state, rec, st);
-        if (h.ea == 4 && h.eb == 2) return run_z<16, 4, 2, PULSE>(s, r, t0, t1, state, rec, st);
-        if (h.ea == 8 && h.eb == 2) return run_z<16, 8, 2, PULSE>(s, r, t0, t1, state, rec, st);
-        if (h.ea == 8 && h.eb == 3) return run_z<16, 8, 3, PULSE>(s, r, t0, t1, state, rec, st);
-        if (h.ea == 8 && h.eb == 5) return run_z<16, 8, 5, PULSE>(s, r, t0, t1, state, rec, st);
-    }
-    if (!PULSE && s.z == 4) {
-        if (h.W == 13) return run<13, 10, 5, 4, false>(s, r, t0, t1, state, rec, st);
-        if (h.W == 8) return run<8, 16, 9, 4, false>(s, r, t0, t1, state, rec, st);
-    }
+    if (h.ea == 2 && h.s == 1) return run_z<2, 1, PULSE>(s, r, t0, t1, state, rec, st);
+    if (h.ea == 4 && h.s == 2) return run_z<4, 2, PULSE>(s, r, t0, t1, state, rec, st);
+    if (h.ea == 8 && h.s == 3) return run_z<8, 3, PULSE>(s, r, t0, t1, state, rec, st);
+    if (h.ea == 8 && h.s == 4) return run_z<8, 4, PULSE>(s, r, t0, t1, state, rec, st);
+    if (h.ea == 8 && h.s == 5) return run_z<8, 5, PULSE>(s, r, t0, t1, state, rec, st);
     return hipErrorInvalidValue;
 }
 
@@ -410,17 +477,17 @@ hipError_t run_shape(const HShape& h, const DevSystem<float>& s, const RunShape&
 
 bool herm_fits(int L, int z)
 {
-    const HShape h = herm_shape(L, 0);
+    const HShape h = herm_shape(L);
     const int zp = stencil_padded_z(z);
-    return h.W && zp && herm_lds_bytes(L, zp, true, h.W) <= kLdsLimit;
+    return h.W && zp && herm_lds_bytes(L, zp, true) <= kLdsLimit;
 }
 
 hipError_t launch_herm(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                        int nt_hint, hipStream_t st)
 {
-    HShape h = herm_shape(s.L, nt_hint);
-    if ((h.W == 13 || h.W == 8) && (s.amp || s.z != 4)) h = herm_shape(s.L, 0);
-    if (!h.W || (s.z != 4 && s.z != 8) || r.B <= 0) return hipErrorInvalidValue;
+    (void)nt_hint;  // one workgroup shape (16 waves) for every L
+    const HShape h = herm_shape(s.L);
+    if (!h.W || (s.z != 4 && s.z != 8) || r.B <= 0 || !s.rows) return hipErrorInvalidValue;
     if (s.amp) return run_shape<true>(h, s, r, t0, t1, state, rec, st);
     return run_shape<false>(h, s, r, t0, t1, state, rec, st);
 }
