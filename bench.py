@@ -4,12 +4,16 @@
 One "step" is one RK4 "3/8" step (4 stages, both spins) of the whole 65,536-trajectory batch.
 The timed region is one fpsftwa_run of K steps of the FP32 batched engine (sampling, K steps with an
 observable record at every time point, and the FP64 ensemble reduction); it prints one JSON line.
+The default K = 4000 is the whole run at dt = 0.005, t_max = 20 (4001 time points).
 
 Beside it, on the same workload:
   * the plain FP64 GPU path (dense Delta, one launch per time point), timed over a few steps: the
     speedup denominator of the headline metric;
-  * the FP64 batched engine over the same K steps: the FP64 reference for the FP32 error check;
-  * the roofline of the dominant kernel (the stencil integrator), from HIP events on its stream;
+  * the FP64 batched engine over the same K steps: the FP64 reference for the FP32 error check, and
+    itself timed.  The error report gives, per observable, the largest difference and the time up
+    to which every observable stays within 1e-5 (the trajectories are chaotic, so rounding
+    differences grow exponentially in t);
+  * the roofline of the dominant kernel (the upper-triangle integrator), from HIP events on its stream;
   * the CPU baseline: the FP64 oracle (a C/OpenMP restatement of the reference algorithm) on a bounded
     sample of the same workload on the host cores.
 """
@@ -53,6 +57,31 @@ def traffic_from_profile(path):
         return None, None
 
 
+def error_report(r32, r64, dt, tol=1e-5):
+    """Per-observable FP32-vs-FP64 ensemble differences, and the time up to which all stay within tol.
+
+    Both runs start from the same samples.  The per-trajectory dynamics is chaotic, so any rounding
+    difference grows roughly exponentially in t; the horizon is where that growth reaches tol.
+    """
+    diffs = {k: np.abs(r32[k] - r64[k]) for k in ("Niiu", "Niid", "NuNd")}
+    per_t = np.max(np.stack([d.max(axis=0) for d in diffs.values()]), axis=0)  # [T]
+    above = np.nonzero(per_t > tol)[0]
+    t_bad = int(above[0]) if above.size else per_t.shape[0]
+    e_rel = np.abs(r32["E"] - r64["E"]).max(axis=0) / np.maximum(np.abs(r64["E"]).max(axis=0), 1e-30)
+    rep = {"tolerance": tol,
+           "max_abs": {k: float(d.max()) for k, d in diffs.items()},
+           "E_max_rel": float(e_rel.max()),
+           "stat_err_max": {"Niiu": float(np.max(r32["Niiu_err"])), "NuNd": float(np.max(r32["NuNd_err"]))},
+           "within_tolerance": bool(above.size == 0),
+           "t_within_tolerance": round(t_bad * dt if above.size else (per_t.shape[0] - 1) * dt, 4),
+           "max_abs_by_t": [[round(i * dt, 3), float(per_t[i])]
+                            for i in np.linspace(0, per_t.shape[0] - 1, 11).astype(int)]}
+    if t_bad > 0:
+        rep["max_abs_within_horizon"] = {k: float(d[:, :t_bad].max()) for k, d in diffs.items()}
+        rep["E_max_rel_within_horizon"] = float(e_rel[:t_bad].max())
+    return rep
+
+
 def cpu_baseline(w, dt, target_s=15.0):
     from oracle import oracle as O
     nthreads = os.cpu_count() or 1
@@ -76,7 +105,8 @@ def cpu_baseline(w, dt, target_s=15.0):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000, help="timed RK4 steps of the 65,536-trajectory batch")
+    ap.add_argument("--steps", type=int, default=4000,
+                    help="timed RK4 steps of the 65,536-trajectory batch (4000 = t_max 20 at dt 0.005)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n-traj", type=int, default=65536)
     ap.add_argument("--block", type=int, default=0, help="threads per trajectory workgroup (0, 512, 1024)")
@@ -146,37 +176,43 @@ def main():
     alg_bytes = 32.0 * L * L * B * K
     kern_s = tm["kernel_ms"] * 1e-3
     achieved = alg_bytes / kern_s / 1e9
-    traffic, prof = traffic_from_profile(os.path.join(ROOT, "profiles", "stencil_fp32_hubbard8x8_traffic.json"))
+    traffic, prof = traffic_from_profile(os.path.join(ROOT, "profiles", "herm_fp32_hubbard8x8_traffic.json"))
+    kname = "stencil_kernel<float,16,8,4,false,64>" if args.kernel == "full" else "herm_kernel<16,8,5,4,false>"
+    if args.kernel == "full" or (prof and prof.get("kernel") != kname):
+        traffic = None  # the committed counter figures belong to the other kernel
+    z = 4
+    valu_flop = (64 * z + 104) * L * L  # reference-formula flops per trajectory-step, sparse tM
+    valu_achieved = valu_flop * B * K / kern_s / 1e12
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": ("stencil_kernel<float,16,8,4,false,64>" if args.kernel == "full"
-                       else "herm_kernel<16,8,5,4,false>"), "launches": tm["launches"],
+            "kernel": kname, "launches": tm["launches"],
             "avg_launch_ms": round(tm["kernel_ms"] / max(1, tm["launches"]), 4),
-            "alg_bytes_per_traj_step": 32 * L * L}
+            "alg_bytes_per_traj_step": 32 * L * L,
+            # the state stays on chip for a whole launch, so HBM is not what binds: the arithmetic
+            # figure by the same fixed formula, against the FP32 vector peak
+            "valu": {"achieved": round(valu_achieved, 2), "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(valu_achieved / 157.3, 4), "alg_flop_per_traj_step": valu_flop}}
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "traj-steps/s", "n_gpus": 1, "steps": K,
            "warmup": W, "ms_per_step": round(wall * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
            "config": {"workload": "hubbard8x8_periodic_cdw", "L": L, "n_traj": B, "dt": dt, "U": round(w["Uval"], 6),
-                      "seed": seed, "steps_timed": K, "t_timed": K * dt, "integrator": "rk4_3_8",
+                      "seed": seed, "steps_timed": K, "t_timed": K * dt, "t_max": K * dt, "integrator": "rk4_3_8",
                       "records_every_step": True},
            "roofline": roof}
 
     # ---- FP64 batched reference over the same steps: FP32 error vs FP64 ----
     eng.set_tuning(0, 0, "auto")  # the FP32 kernel choice does not apply to the FP64 engines
+    el64 = None
     if not args.no_check:
         eng.set_params(dt, K * dt, 1, B, seed=seed, engine="fp64")
         t0 = time.perf_counter()
         eng.run()
         el64 = time.perf_counter() - t0
         res64 = {k: eng.result(k) for k in ("Niiu", "Niid", "NuNd", "E")}
-        err = {k: float(np.max(np.abs(res32[k] - res64[k]))) for k in ("Niiu", "Niid", "NuNd")}
-        err["E"] = float(np.max(np.abs(res32["E"] - res64["E"])))
-        stat = {"Niiu": float(np.max(res32["Niiu_err"])), "NuNd": float(np.max(res32["NuNd_err"]))}
         out["fp64_batched"] = {"value": round(B * K / el64, 1), "unit": "traj-steps/s", "seconds": round(el64, 3)}
-        out["err_vs_fp64"] = {"max_abs": err, "stat_err_max": stat, "tolerance": 1e-5,
-                              "within_tolerance": bool(max(err.values()) <= 1e-5)}
-        log("fp32 vs fp64 max abs err:", err, "stat err:", stat)
+        out["err_vs_fp64"] = error_report(res32, res64, dt)
+        log("fp32 vs fp64:", json.dumps(out["err_vs_fp64"]))
 
     # ---- plain FP64 GPU path: the speedup denominator ----
     if not args.no_plain:
@@ -194,6 +230,8 @@ def main():
                              "seconds": round(elp, 3), "kernel_ms": round(tp["kernel_ms"], 3)}
         out["speedup_vs_fp64_plain"] = round(value / plain_value, 2)
         out["speedup_vs_fp64_plain_kernel_only"] = round((B * K / kern_s) / plain_kernel_value, 2)
+        if el64 is not None:
+            out["fp64_batched"]["speedup_vs_fp64_plain"] = round((B * K / el64) / plain_value, 2)
         log(f"fp64 plain: {P} steps in {elp:.3f} s -> {plain_value:.4e} traj-steps/s; speedup {value / plain_value:.1f}x")
     eng.close()
 

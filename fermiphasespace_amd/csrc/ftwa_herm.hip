@@ -7,9 +7,10 @@
 //   k_ij = -i [ conj(X_ji) - X_ij + (h_j - h_i) rho_ij ]          k_ii = -2 Im X_ii
 //
 // One workgroup integrates one trajectory.  Per RK stage:
-//   phase A  wavefront w forms full rows of X, lane j = column j:  X_ij = sum_{k in nb(i)} t_ik Y_kj.
-//            The row index is wave-uniform: nb(i) and t_ik come from one scalar load, every read of Y is
-//            a contiguous row segment, X goes to LDS.                                     -> barrier
+//   phase A  X_ij = sum_{k in nb(i)} t_ik Y_kj with lane = row i and a wave-uniform column: every
+//            lane keeps the LDS addresses of its neighbour rows and t_ik in registers, the column
+//            enters as an immediate offset, so the stage costs no address arithmetic.  X goes to
+//            LDS.                                                                          -> barrier
 //   phase B  the strict upper triangle of both spins is packed into 64-lane "element rows", followed
 //            by the diagonals; element row r belongs to wave r % W, slot r / W.  Each thread keeps, per
 //            slot, the RK4 "3/8" registers (y, A, p) and the LDS byte addresses of its element and of
@@ -21,9 +22,9 @@
 //
 // LDS (bytes):
 //   [0, 1280)     h double buffer [2][160] floats: entry sg*64 + i, entries 128.. absorb padding lanes
-//   [1280, 1792)  u_ii [64] (on-site Coulomb; 0 otherwise), pdiag [64]
-//   [1792, 2048)  energy partials [W][4]
-//   [2048, ...)   rows r = sg*L + i (2L of them, plus a spare row 2L), 129 complex each:
+//   [1280, 1536)  energy partials [W][4]
+//   [1536, 2304)  per-site constants of the energies: tM_ii, potential, U_ii ([64] each)
+//   [2304, ...)   rows r = sg*L + i (2L of them, plus a spare row 2L), 129 complex each:
 //                 Y_r,0..63 | X_r,0..63 | pad.  X_ij sits 512 bytes after Y_ij; the odd stride keeps
 //                 the transposed (column) accesses of phase B on distinct banks.
 //   then          pulse hoppings t (1 + e^{i phi}) [Z][L] complex
@@ -46,10 +47,9 @@ constexpr int kRowBytes = 129 * 8;  // LDS row: Y (64 complex), X (64 complex), 
 constexpr int kXOff = 64 * 8;       // X_ij - Y_ij, bytes
 constexpr int kHB = 0;              // h buffers
 constexpr int kHBStride = 160 * 4;  // bytes per h buffer
-constexpr int kCoefU = 1280;        // u_ii
-constexpr int kCoefP = 1536;        // pdiag
-constexpr int kEW = 1792;           // energy partials
-constexpr int kRows = 2048;         // first row
+constexpr int kEW = 1280;           // energy partials
+constexpr int kSite = 1536;         // tM_ii [64], potential [64], U_ii [64]
+constexpr int kRows = 2304;         // first row
 constexpr size_t kLdsLimit = 160 * 1024;
 
 template <int N>
@@ -64,11 +64,10 @@ __device__ __forceinline__ float lds_f(const unsigned char* base, int off)
     return *reinterpret_cast<const float*>(base + off);
 }
 
-// EPTA: rows per wavefront in phase A; S: element slots per thread in phase B.
+// EPTA: column groups per wavefront in phase A (half per spin); S: element slots per thread in phase B.
 template <int W, int EPTA, int S, int Z, bool PULSE>
 __global__ void __launch_bounds__(64 * W)
-    herm_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state, float* __restrict__ rec,
-                const RowNbrs<float, Z>* __restrict__ rows)
+    herm_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state, float* __restrict__ rec)
 {
     constexpr int kNT = 64 * W;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -98,14 +97,16 @@ __global__ void __launch_bounds__(64 * W)
     }
     __syncthreads();
     for (int i = tid; i < L; i += kNT) {
-        *reinterpret_cast<float*>(smem + kCoefU + i * 4) = s.coulomb_type == 1 ? s.u_diag[i] : 0.0f;
-        *reinterpret_cast<float*>(smem + kCoefP + i * 4) = s.pdiag[i];
+        *reinterpret_cast<float*>(smem + kSite + i * 4) = s.tdiag[i];
+        *reinterpret_cast<float*>(smem + kSite + 256 + i * 4) = s.pot[i];
+        *reinterpret_cast<float*>(smem + kSite + 512 + i * 4) = s.u_diag[i];
     }
 
     // ---- element slots: LDS byte addresses of Y_ij (aij) and of the mirror Y_ji (aji), h offsets ----
-    // Off-diagonal: ah = (sg*64 + i)*4 | (sg*64 + j)*4 << 16.  Diagonal: aji = i*4 (coefficient
-    // offset), ah = ((1 - sg)*64 + i)*4 (the other spin's next h).  Padding lanes point at the spare
-    // row and at h entry 128.
+    // Off-diagonal: ah = (sg*64 + i)*4 | (sg*64 + j)*4 << 16.  Diagonal: aji = aij,
+    // ah = ((1 - sg)*64 + i)*4 (the other spin's next h), and the unused imaginary RK registers hold
+    // the constants of that h: A.y = U_ii (on-site Coulomb, else 0), p.y = pdiag_i.  Padding lanes
+    // point at the spare row and at h entry 128.
     int aij[S], aji[S], ah[S];
     C y[S], A[S], p[S];
     const C* gs = reinterpret_cast<const C*>(state) + b * 2 * LL;
@@ -143,15 +144,31 @@ __global__ void __launch_bounds__(64 * W)
                 const int sg = d >= L ? 1 : 0;
                 const int i = d - sg * L;
                 aij[q] = kRows + (sg * L + i) * kRowBytes + i * 8;
-                aji[q] = i * 4;
+                aji[q] = aij[q];
                 ah[q] = ((1 - sg) * 64 + i) * 4;
                 y[q] = C{gs[sg * LL + i * L + i].x, 0.0f};
+                A[q].y = s.coulomb_type == 1 ? s.u_diag[i] : 0.0f;
+                p[q].y = s.pdiag[i];
                 *reinterpret_cast<C*>(smem + aij[q]) = y[q];
-            } else {
-                aji[q] = 0;
             }
         }
     }
+    // ---- phase-A addresses: lane j is row i = j of each spin; wave w covers columns w + W*mm ----
+    // kb{sg}[n]: byte address of Y(sg*L + k_n(i), w); xb{sg}: of X(sg*L + i, w).  Lanes past L read
+    // row 0 with t = 0 and write the spare row.
+    const int il = min(j, L - 1);
+    int kb0[Z], kb1[Z];
+    float tv[Z];
+#pragma unroll
+    for (int n = 0; n < Z; ++n) {
+        const int k = j < L ? s.nbr_k[n * L + il] : 0;
+        kb0[n] = kRows + k * kRowBytes + w * 8;
+        kb1[n] = kb0[n] + L * kRowBytes;
+        tv[n] = j < L ? s.nbr_t[n * L + il] : 0.0f;
+    }
+    const int xb0 = (j < L ? kRows + il * kRowBytes : spare) + kXOff + w * 8;
+    const int xb1 = (j < L ? kRows + (L + il) * kRowBytes : spare) + kXOff + w * 8;
+
     const float h1 = static_cast<float>(r.h);
     const float h3 = static_cast<float>(r.h / 3.0), h6 = static_cast<float>(r.h / 6.0),
                 h8 = static_cast<float>(r.h / 8.0);
@@ -207,49 +224,54 @@ __global__ void __launch_bounds__(64 * W)
         constexpr int ST = decltype(stc)::value;
         constexpr int HC = kHB + (ST & 1) * kHBStride;
         constexpr int HN = kHB + ((ST + 1) & 1) * kHBStride;
-        // ---- phase A: X = T Y, full rows ----
+        // ---- phase A: X = T Y; lane = row i, column group (spin sg, columns w + W*mm) per step ----
         {
-            int wo = w;
-            int jo = j;
-            asm volatile("" : "+s"(wo), "+v"(jo));
-            const unsigned char* Yl = smem + kRows + jo * 8;
-            unsigned char* Xl = smem + kRows + kXOff + jo * 8;
-            // branch-free over the wave's rows, so all their LDS reads can be in flight together: rows
-            // past 2L reuse the last row's neighbours and write the spare row, which nothing reads
+            C T[Z];
+            if constexpr (PULSE) {
+#pragma unroll
+                for (int n = 0; n < Z; ++n) T[n] = Tc[n * L + il];
+            }
 #pragma unroll
             for (int m = 0; m < EPTA; ++m) {
-                const int row = wo + W * m;
-                const int rw = min(row, R2);
-                const int rs = min(row, R2 - 1);
-                const int sg = rs >= L ? 1 : 0;
-                const int i = rs - sg * L;
-                const RowNbrs<float, Z> nb = rows[i];
+                constexpr int NCG = EPTA / 2;
+                const int sg = m / NCG;
+                const int co = (m % NCG) * W * 8;  // immediate column offset
                 float ax = 0.0f, ay = 0.0f;
 #pragma unroll
                 for (int n = 0; n < Z; ++n) {
-                    const C v = lds_c(Yl, (sg * L + nb.k[n]) * kRowBytes);
+                    const C v = lds_c(smem, (sg ? kb1[n] : kb0[n]) + co);
                     if constexpr (PULSE) {
-                        const C T = Tc[n * L + i];
-                        ax += T.x * v.x - T.y * v.y;
-                        ay += T.x * v.y + T.y * v.x;
+                        ax += T[n].x * v.x - T[n].y * v.y;
+                        ay += T[n].x * v.y + T[n].y * v.x;
                     } else {
-                        ax += nb.t[n] * v.x;
-                        ay += nb.t[n] * v.y;
+                        ax += tv[n] * v.x;
+                        ay += tv[n] * v.y;
                     }
                 }
-                *reinterpret_cast<C*>(Xl + rw * kRowBytes) = C{ax, ay};
+                *reinterpret_cast<C*>(smem + (sg ? xb1 : xb0) + co) = C{ax, ay};
             }
         }
         __syncthreads();
         if (ST == 0 && flush) flush_energies();
-        // ---- phase B ----
+        // ---- phase B: every slot's reads first, so they are in flight together ----
+        C xij[S], xji[S];
+        float hi[S], hj[S];
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            if (q * W + w < n_off + n_dia) {
+                xij[q] = lds_c(smem, aij[q] + kXOff);
+                xji[q] = lds_c(smem, aji[q] + kXOff);
+                int ahq = ah[q];  // opaque: decoded here, not kept decoded across the time loop
+                asm volatile("" : "+v"(ahq));
+                hi[q] = lds_f(smem, HC + (ahq & 0xffff));
+                hj[q] = lds_f(smem, HC + (ahq >> 16));
+            }
+        }
 #pragma unroll
         for (int q = 0; q < S; ++q) {
             const int er = q * W + w;
             if (er < n_off) {
-                const C xij = lds_c(smem, aij[q] + kXOff);
-                const C xji = lds_c(smem, aji[q] + kXOff);
-                const float dh = lds_f(smem, HC + (ah[q] >> 16)) - lds_f(smem, HC + (ah[q] & 0xffff));
+                const float dh = hj[q] - hi[q];
                 C yc;  // current stage input
                 if constexpr (ST == 0) {
                     yc = y[q];
@@ -261,8 +283,8 @@ __global__ void __launch_bounds__(64 * W)
                     yc = C{y[q].x + h1 * p[q].x, y[q].y + h1 * p[q].y};
                 }
                 // c = conj(X_ji) - X_ij + dh * Y_ij;  k = -i c
-                const float cx = xji.x - xij.x + dh * yc.x;
-                const float cy = -xji.y - xij.y + dh * yc.y;
+                const float cx = xji[q].x - xij[q].x + dh * yc.x;
+                const float cy = -xji[q].y - xij[q].y + dh * yc.y;
                 const float kx = cy, ky = -cx;
                 C yn;
                 if constexpr (ST == 0) {
@@ -285,8 +307,8 @@ __global__ void __launch_bounds__(64 * W)
                 *reinterpret_cast<C*>(smem + aij[q]) = yn;
                 *reinterpret_cast<C*>(smem + aji[q]) = C{yn.x, -yn.y};
             } else if (er < n_off + n_dia) {
-                // diagonal: k = -2 Im X_ii, real RK registers in .x
-                const float k = -2.0f * lds_f(smem, aij[q] + kXOff + 4);
+                // diagonal: k = -2 Im X_ii, real RK registers in .x; A.y = u_ii, p.y = pdiag_i
+                const float k = -2.0f * xij[q].y;
                 float yn;
                 if constexpr (ST == 0) {
                     A[q].x = k;
@@ -306,8 +328,7 @@ __global__ void __launch_bounds__(64 * W)
                     yn = y[q].x;
                 }
                 *reinterpret_cast<float*>(smem + aij[q]) = yn;
-                *reinterpret_cast<float*>(smem + HN + ah[q]) =
-                    lds_f(smem, kCoefU + aji[q]) * (yn + 0.5f) + lds_f(smem, kCoefP + aji[q]);
+                *reinterpret_cast<float*>(smem + HN + ah[q]) = A[q].y * (yn + 0.5f) + p[q].y;
             }
         }
         pulse_table(gidx + 1);
@@ -323,7 +344,11 @@ __global__ void __launch_bounds__(64 * W)
     for (int t = t0; t < t1; ++t) {
         rr = rec + (static_cast<int64_t>(t - t0) * Bn + b) * R;
         {
-            for (int e = tid; e < R2; e += kNT) {
+            // lane indices made opaque here, so the addresses of this section are formed here and not
+            // kept in registers across the whole time loop
+            int jt = j, tt = tid;
+            asm volatile("" : "+v"(jt), "+v"(tt));
+            for (int e = tt; e < R2; e += kNT) {
                 const int sg = e >= L ? 1 : 0, i = e - sg * L;
                 rr[e] = yat(sg * L + i, i).x;
             }
@@ -334,25 +359,27 @@ __global__ void __launch_bounds__(64 * W)
                 rr[R2 + 6] = cd.x;
                 rr[R2 + 7] = cd.y;
             }
+            // energies of this time point, lane = site i: wave 0 the hopping terms (neighbour addresses
+            // from the phase-A tables: Y(k, i) = conj(Y(i, k))), wave 1 the on-site terms
             float eth = 0, eu = 0, ev = 0, epe = 0;
-            if (tid < L) {
-                const int i = tid;
-                const float nu = yat(i, i).x + 0.5f;
-                const float nd = yat(L + i, i).x + 0.5f;
+            if (w == 0) {
+#pragma unroll
                 for (int n = 0; n < Z; ++n) {
-                    const int k = s.nbr_k[n * L + i];
-                    const float tt = s.nbr_t[n * L + i];
-                    const C a = yat(i, k), c = yat(L + i, k);
-                    eth += (a.x + c.x) * tt;
+                    const C a = lds_c(smem, kb0[n] + jt * 8), c = lds_c(smem, kb1[n] + jt * 8);
+                    eth += (a.x + c.x) * tv[n];
                     if constexpr (PULSE) {
-                        const C P = Tc[n * L + i];
-                        epe += (a.x + c.x) * (P.x - tt) - (a.y + c.y) * P.y;
+                        const C P = Tc[n * L + min(jt, L - 1)];
+                        if (jt < L) epe += (a.x + c.x) * (P.x - tv[n]) + (a.y + c.y) * P.y;
                     }
                 }
-                eth += s.tdiag[i] * (nu + nd);
-                epe += s.pot[i] * (nu + nd);
+            } else if (w == 1 && jt < L) {
+                const int i = jt;
+                const float nu = yat(i, i).x + 0.5f;
+                const float nd = yat(L + i, i).x + 0.5f;
+                eth = lds_f(smem, kSite + i * 4) * (nu + nd);
+                epe = lds_f(smem, kSite + 256 + i * 4) * (nu + nd);
                 if (s.coulomb_type == 1) {
-                    eu = s.u_diag[i] * nu * nd;
+                    eu = lds_f(smem, kSite + 512 + i * 4) * nu * nd;
                 } else if (general) {
                     float su = 0, sd = 0, vu = 0, vd = 0;
                     for (int jj = 0; jj < L; ++jj) {
@@ -414,7 +441,7 @@ size_t herm_lds_bytes(int L, int z, bool pulse)
     return kRows + static_cast<size_t>(2 * L + 1) * kRowBytes + (pulse ? static_cast<size_t>(z) * L * sizeof(C) : 0);
 }
 
-// Shape for L sites: 16 waves; rows per wave in phase A (EA) and element slots per thread (S).
+// Shape for L sites: 16 waves; column groups per wave in phase A (EA) and element slots per thread (S).
 struct HShape {
     int W, ea, s;
 };
@@ -448,7 +475,7 @@ hipError_t run(const DevSystem<float>& s, const RunShape& r, int t0, int t1, flo
         attr_set = true;
     }
     hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(64 * W), herm_lds_bytes(s.L, s.z, PULSE), st, s,
-                       r, t0, t1, state, rec, static_cast<const RowNbrs<float, Z>*>(s.rows));
+                       r, t0, t1, state, rec);
     return hipGetLastError();
 }
 
@@ -487,7 +514,7 @@ hipError_t launch_herm(const DevSystem<float>& s, const RunShape& r, int t0, int
 {
     (void)nt_hint;  // one workgroup shape (16 waves) for every L
     const HShape h = herm_shape(s.L);
-    if (!h.W || (s.z != 4 && s.z != 8) || r.B <= 0 || !s.rows) return hipErrorInvalidValue;
+    if (!h.W || (s.z != 4 && s.z != 8) || r.B <= 0) return hipErrorInvalidValue;
     if (s.amp) return run_shape<true>(h, s, r, t0, t1, state, rec, st);
     return run_shape<false>(h, s, r, t0, t1, state, rec, st);
 }
