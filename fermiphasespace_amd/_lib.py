@@ -36,7 +36,7 @@ EXPORTED = [
     "fpsftwa_last_error", "fpsftwa_version", "fpsftwa_device_info", "fpsftwa_create", "fpsftwa_destroy",
     "fpsftwa_set_params", "fpsftwa_set_results", "fpsftwa_set_initial_state", "fpsftwa_set_noise",
     "fpsftwa_sample_noise", "fpsftwa_run", "fpsftwa_get_result", "fpsftwa_get_states", "fpsftwa_get_timing",
-    "fpsftwa_set_tuning", "fpsftwa_run_json_file", "fpsftwa_load_system_json",
+    "fpsftwa_set_tuning", "fpsftwa_run_json_file", "fpsftwa_load_system_json", "fpsftwa_run_adjoint",
 ]
 
 
@@ -44,6 +44,10 @@ class FtwaError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[{code}] {msg}")
         self.code = code
+
+
+class _AdjointSpec(ctypes.Structure):
+    _fields_ = [("objective", ctypes.c_int32), ("checkpoint_every", ctypes.c_int32)]
 
 
 class _System(ctypes.Structure):
@@ -96,6 +100,7 @@ def load(path: str = LIB_PATH):
         "fpsftwa_get_timing": ([H, P(i64), P(d), P(d)], ctypes.c_int),
         "fpsftwa_set_tuning": ([H, i32, i32, i32], ctypes.c_int),
         "fpsftwa_run_json_file": ([ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
+        "fpsftwa_run_adjoint": ([H, P(_AdjointSpec), P(d), P(d), P(d), P(d), sz], ctypes.c_int),
         "fpsftwa_load_system_json": ([ctypes.c_char_p, P(i32), P(i32), P(d), P(d), P(d), P(d), P(d), P(d), P(d)],
                                      ctypes.c_int),
     }
@@ -185,6 +190,7 @@ class Engine:
         s.displacement = _dptr(keep[5])
         h = ctypes.c_void_p()
         check(self._lib.fpsftwa_create(ctypes.byref(s), ctypes.byref(h)), "create")
+        self._n_pulse = (s.pulse_n_knots if pulse_type == PULSE_KNOTS else (1 if pulse_type == PULSE_GAUSSIAN else 0))
         self._h = h
         self.n_traj = 0
 
@@ -270,6 +276,22 @@ class Engine:
         out = np.zeros((count, 2, self.L, self.L, 2))
         check(self._lib.fpsftwa_get_states(self._h, first, count, _dptr(out), out.size), "get_states")
         return out[..., 0] + 1j * out[..., 1]
+
+    def adjoint(self, checkpoint_every: int = 50) -> dict:
+        """Discrete adjoint of the ensemble double occupancy at the last time point.
+
+        Returns {"J", "grad_u" [L], "grad_t" [L, L], "grad_pulse" [n] (knots) / [1] (Gaussian A0) / None}.
+        """
+        L = self.L
+        spec = _AdjointSpec(0, int(checkpoint_every))
+        J = ctypes.c_double(0)
+        gu = np.zeros(L)
+        gt = np.zeros((L, L))
+        npulse = self._n_pulse
+        gp = np.zeros(max(npulse, 1))
+        check(self._lib.fpsftwa_run_adjoint(self._h, ctypes.byref(spec), ctypes.byref(J), _dptr(gu), _dptr(gt),
+                                            _dptr(gp) if npulse else None, npulse), "run_adjoint")
+        return {"J": J.value, "grad_u": gu, "grad_t": gt, "grad_pulse": gp[:npulse] if npulse else None}
 
     def timing(self) -> dict:
         n = ctypes.c_int64(0)

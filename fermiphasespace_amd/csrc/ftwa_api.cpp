@@ -161,14 +161,18 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
     };
     if (z == 4) pack_rows(RowNbrs<Real, 4>{});
     else if (z == 8) pack_rows(RowNbrs<Real, 8>{});
-    std::vector<Real> udiag(L), pdiag(L), tdiag(L), potv(L), Ud((size_t)L * L), Vd((size_t)L * L), tMd((size_t)L * L),
+    std::vector<Real> udiag(L), pdiag(L), tdiag(L), potv(L), epot(L), Ud((size_t)L * L), Vd((size_t)L * L), tMd((size_t)L * L),
         pdd((size_t)L * L);
     for (int i = 0; i < L; ++i) {
         udiag[i] = h->coulomb_type ? (Real)h->U[(size_t)i * L + i] : Real(0);
         const double p = h->pot.empty() ? 0.0 : h->pot[i];
+        // with a pulse the diagonal hopping also enters the pulse potential (t_ii e^{i A.0} = t_ii,
+        // src/FHExternalPotentialPulse.cpp:96-100): twice in h, once more in E_pe
+        const double tii = h->tM[(size_t)i * L + i];
         potv[i] = (Real)p;
-        tdiag[i] = (Real)h->tM[(size_t)i * L + i];
-        pdiag[i] = (Real)(p + h->tM[(size_t)i * L + i]);
+        epot[i] = (Real)(p + (h->pulse_type != 0 ? tii : 0.0));
+        tdiag[i] = (Real)tii;
+        pdiag[i] = (Real)(p + (h->pulse_type != 0 ? 2.0 : 1.0) * tii);
     }
     for (size_t e = 0; e < (size_t)L * L; ++e) {
         Ud[e] = h->U.empty() ? Real(0) : (Real)h->U[e];
@@ -196,7 +200,7 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
     };
     const size_t o_nk = take(nk.size() * sizeof(int)), o_nt = take(nt.size() * sizeof(Real)),
                  o_npd = take(npd.size() * sizeof(Real)), o_ud = take(L * sizeof(Real)),
-                 o_pd = take(L * sizeof(Real)), o_td = take(L * sizeof(Real)), o_pot = take(L * sizeof(Real)),
+                 o_pd = take(L * sizeof(Real)), o_td = take(L * sizeof(Real)), o_pot = take(L * sizeof(Real)), o_epot = take(L * sizeof(Real)),
                  o_U = take(Ud.size() * sizeof(Real)), o_V = take(Vd.size() * sizeof(Real)),
                  o_tM = take(tMd.size() * sizeof(Real)), o_pdd = take(pdd.size() * sizeof(Real)),
                  o_amp = take(std::max<size_t>(amp.size(), 1) * sizeof(double)),
@@ -213,6 +217,7 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
     HIP_TRY(up(o_pd, pdiag.data(), L * sizeof(Real)));
     HIP_TRY(up(o_td, tdiag.data(), L * sizeof(Real)));
     HIP_TRY(up(o_pot, potv.data(), L * sizeof(Real)));
+    HIP_TRY(up(o_epot, epot.data(), L * sizeof(Real)));
     HIP_TRY(up(o_U, Ud.data(), Ud.size() * sizeof(Real)));
     HIP_TRY(up(o_V, Vd.data(), Vd.size() * sizeof(Real)));
     HIP_TRY(up(o_tM, tMd.data(), tMd.size() * sizeof(Real)));
@@ -230,6 +235,7 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
     ds.pdiag = reinterpret_cast<const Real*>(base + o_pd);
     ds.tdiag = reinterpret_cast<const Real*>(base + o_td);
     ds.pot = reinterpret_cast<const Real*>(base + o_pot);
+    ds.epot = reinterpret_cast<const Real*>(base + o_epot);
     ds.U = reinterpret_cast<const Real*>(base + o_U);
     ds.V = reinterpret_cast<const Real*>(base + o_V);
     ds.tM = reinterpret_cast<const Real*>(base + o_tM);
@@ -399,6 +405,116 @@ double kahan_value(const std::vector<double>& acc, int Q, int t, int q)
 {
     const double* p = &acc[((size_t)t * Q + q) * 2];
     return p[0] + p[1];
+}
+
+// dJ/d(pulse parameters) from dJ/da at every RK stage time: knots by their linear-interpolation
+// weights (the same rule as pulse_amplitude), the Gaussian amplitude A0 by the envelope a/A0.
+void pulse_gradient(const fpsftwa_handle* h, const std::vector<double>& dJda, double* out, size_t n_out)
+{
+    const double hs = h->dt / h->nsub;
+    const double cs[4] = {0.0, 1.0 / 3.0, 2.0 / 3.0, 1.0};
+    for (size_t k = 0; k < n_out; ++k) out[k] = 0.0;
+    const int n = (int)h->knots.size();
+    for (size_t q = 0; q < dJda.size(); ++q) {
+        const double t = (double)(q / 4) * hs + cs[q % 4] * hs;
+        if (h->pulse_type == FPSFTWA_PULSE_KNOTS) {
+            if (n == 1) {
+                out[0] += dJda[q];
+                continue;
+            }
+            const double u = t / (h->t_max / (double)(n - 1));
+            if (u <= 0.0) {
+                out[0] += dJda[q];
+                continue;
+            }
+            const int k = (int)std::floor(u);
+            if (k >= n - 1) {
+                out[n - 1] += dJda[q];
+                continue;
+            }
+            const double f = u - (double)k;
+            out[k] += dJda[q] * (1.0 - f);
+            out[k + 1] += dJda[q] * f;
+        } else if (h->pulse_type == FPSFTWA_PULSE_GAUSSIAN) {
+            const double x = (t - h->t0) / h->td;
+            out[0] += dJda[q] * std::exp(-(x * x) / 2.0) * std::cos(h->w0 * (t - h->t0) + h->phi);
+        }
+    }
+}
+
+template <typename Real>
+int run_adjoint(fpsftwa_handle* h, int ck_every, double* objective, double* grad_u, double* grad_t,
+                double* grad_pulse, size_t n_pulse)
+{
+    DevSystem<Real> ds;
+    int rc = upload_system<Real>(h, ds, h->T);
+    if (rc) return rc;
+    if ((rc = sample_states<Real>(h, ds))) return rc;
+    const int L = h->L;
+    const int64_t B = h->B;
+    const size_t NE = 2 * (size_t)L * L;
+    const int64_t nst = (int64_t)(h->T - 1) * h->nsub;  // steps up to the last time point
+    const int64_t K = std::max<int64_t>(1, std::min<int64_t>(ck_every, std::max<int64_t>(nst, 1)));
+    const int64_t nseg = (nst + K - 1) / K;
+    const int64_t n_stage = ds.amp ? nst * 4 : 0;
+    const int64_t G = 2 * (int64_t)L + (int64_t)ds.z * L + n_stage;
+    RunShape rs;
+    rs.B = B;
+    rs.T = h->T;
+    rs.nsub = h->nsub;
+    rs.h = h->dt / h->nsub;
+    DevBuf ck, wk, gacc, jb, out;
+    HIP_TRY(ck.ensure(std::max<int64_t>(nseg, 1) * B * NE * 2 * sizeof(Real)));
+    HIP_TRY(wk.ensure(adjoint_work_bytes<Real>(L, B, K)));
+    HIP_TRY(gacc.ensure((size_t)B * G * sizeof(double)));
+    HIP_TRY(jb.ensure((size_t)B * sizeof(double)));
+    HIP_TRY(out.ensure((size_t)(G + 1) * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(gacc.p, 0, (size_t)B * G * sizeof(double), h->stream));
+    Real* state = h->state.as<Real>();
+    Real* work = wk.as<Real>();
+    Real* segbuf = work + (size_t)B * NE * 2 * 10;
+    const size_t ck_bytes = (size_t)B * NE * 2 * sizeof(Real);
+    EventTimer kt;
+    HIP_TRY(kt.mark(h->stream));
+    for (int64_t s = 0; s < nseg; ++s) {
+        HIP_TRY(hipMemcpyAsync(ck.as<char>() + s * ck_bytes, state, ck_bytes, hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(launch_adjoint_forward<Real>(ds, rs, s * K, std::min(nst, (s + 1) * K), state, work, h->stream));
+    }
+    HIP_TRY(launch_adjoint_terminal<Real>(ds, B, state, work, jb.as<double>(), h->stream));
+    for (int64_t s = nseg - 1; s >= 0; --s)
+        HIP_TRY(launch_adjoint_backward<Real>(ds, rs, s * K, std::min(nst, (s + 1) * K),
+                                              reinterpret_cast<const Real*>(ck.as<char>() + s * ck_bytes), segbuf,
+                                              work, gacc.as<double>(), n_stage, h->stream));
+    HIP_TRY(launch_adjoint_reduce(gacc.as<double>(), B, G, out.as<double>(), h->stream));
+    HIP_TRY(launch_adjoint_reduce(jb.as<double>(), B, 1, out.as<double>() + G, h->stream));
+    HIP_TRY(kt.mark(h->stream));
+    std::vector<double> g(G + 1);
+    HIP_TRY(hipMemcpyAsync(g.data(), out.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->kernel_ms = kt.pairs_ms();
+    h->launches = 2 * nseg + 1;
+    const double inv = 1.0 / (double)B;
+    if (objective) *objective = g[G] * inv;
+    if (grad_u)
+        for (int i = 0; i < L; ++i) grad_u[i] = g[i] * inv;
+    if (grad_t) {
+        for (size_t e = 0; e < (size_t)L * L; ++e) grad_t[e] = 0.0;
+        for (int k = 0; k < L; ++k) grad_t[(size_t)k * L + k] = g[L + k] * inv;
+        // the ELL order of upload_system: off-diagonal non-zeros of row k in column order
+        for (int k = 0; k < L; ++k) {
+            int n = 0;
+            for (int l = 0; l < L && n < ds.z; ++l)
+                if (l != k && h->tM[(size_t)k * L + l] != 0.0) grad_t[(size_t)k * L + l] = g[2 * L + (size_t)n++ * L + k] * inv;
+        }
+    }
+    if (grad_pulse && n_pulse && n_stage) {
+        std::vector<double> dJda(g.begin() + 2 * L + (size_t)ds.z * L, g.begin() + G);
+        for (double& v : dJda) v *= inv;
+        pulse_gradient(h, dJda, grad_pulse, n_pulse);
+    } else if (grad_pulse) {
+        for (size_t k = 0; k < n_pulse; ++k) grad_pulse[k] = 0.0;
+    }
+    return FPSFTWA_OK;
 }
 
 }  // namespace
@@ -632,6 +748,40 @@ int fpsftwa_run(fpsftwa_handle* h)
     h->B_run = h->B;
     h->has_result = true;
     return FPSFTWA_OK;
+}
+
+int fpsftwa_run_adjoint(fpsftwa_handle* h, const fpsftwa_adjoint_spec* spec, double* objective, double* grad_u,
+                        double* grad_t, double* grad_pulse, size_t n_pulse)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (spec && spec->objective != FPSFTWA_OBJ_DOUBLE_OCCUPANCY) return fail(FPSFTWA_EINVAL, "unknown objective");
+    if (h->coulomb_type == FPSFTWA_COULOMB_GENERAL)
+        return fail(FPSFTWA_EUNSUPPORTED, "the adjoint supports null and on-site Coulomb interactions");
+    if (h->L > 256) return fail(FPSFTWA_EUNSUPPORTED, "the adjoint supports L <= 256");
+    if (grad_pulse && n_pulse) {
+        const size_t need = h->pulse_type == FPSFTWA_PULSE_KNOTS ? h->knots.size()
+                                                                 : (h->pulse_type == FPSFTWA_PULSE_GAUSSIAN ? 1 : 0);
+        if (n_pulse != need) return fail(FPSFTWA_EINVAL, "grad_pulse length does not match the pulse parameters");
+    }
+    if (!h->noise.empty()) {
+        const size_t need = (size_t)h->B * 2 * ((size_t)h->L * (h->L - 1) / 2) * 2;
+        if (h->noise.size() != need) return fail(FPSFTWA_EINVAL, "noise size does not match n_traj");
+    }
+    HIP_TRY(hipSetDevice(h->device));
+    h->has_result = false;
+    h->T = (int)std::floor(h->t_max / h->dt + 1.0);  // src/FHSolver.cpp:100
+    int rc = prepare_initial(h);
+    if (rc) return rc;
+    const int ck = spec && spec->checkpoint_every > 0 ? spec->checkpoint_every : 50;
+    if (h->engine == FPSFTWA_ENGINE_FP32_BATCHED) {
+        h->state_double = false;
+        rc = run_adjoint<float>(h, ck, objective, grad_u, grad_t, grad_pulse, n_pulse);
+    } else {
+        h->state_double = true;
+        rc = run_adjoint<double>(h, ck, objective, grad_u, grad_t, grad_pulse, n_pulse);
+    }
+    (void)hipStreamSynchronize(h->stream);
+    return rc;
 }
 
 int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, double* run_ms)

@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(64 * W)
     __syncthreads();
     for (int i = tid; i < L; i += kNT) {
         *reinterpret_cast<float*>(smem + kSite + i * 4) = s.tdiag[i];
-        *reinterpret_cast<float*>(smem + kSite + 256 + i * 4) = s.pot[i];
+        *reinterpret_cast<float*>(smem + kSite + 256 + i * 4) = s.epot[i];
         *reinterpret_cast<float*>(smem + kSite + 512 + i * 4) = s.u_diag[i];
     }
 

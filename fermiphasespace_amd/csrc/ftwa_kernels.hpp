@@ -38,9 +38,12 @@ struct DevSystem {
     const Real* u_diag = nullptr;  // [L]     U_ii
     const Real* U = nullptr;       // [L][L]  dense U (general)
     const Real* V = nullptr;       // [L][L]  dense V (general)
-    const Real* pdiag = nullptr;   // [L]     constant diagonal potential + tM_ii (enters h)
+    const Real* pdiag = nullptr;   // [L]     diagonal of Delta besides the Hartree term: pot + tM_ii
+                                   //         (+ tM_ii again from the pulse potential when present)
     const Real* tdiag = nullptr;   // [L]     tM_ii (energy bookkeeping)
-    const Real* pot = nullptr;     // [L]     constant diagonal potential (energy bookkeeping)
+    const Real* pot = nullptr;     // [L]     constant diagonal potential (plain path: h and E_pe)
+    const Real* epot = nullptr;    // [L]     diagonal of the potential in E_pe (batched paths): pot, plus
+                                   //         tM_ii when a pulse is present
     const double* amp = nullptr;   // [T*nsub*4] pulse amplitude a(tau) at each RK stage time, or null
     const Real* tM = nullptr;      // [L][L]  dense hopping (plain path)
     const Real* pd = nullptr;      // [L][L]  pol . (r_i - r_j) dense (plain path, pulse)
@@ -94,6 +97,23 @@ __host__ __device__ inline size_t plain_work_doubles(int L)
     const size_t LL = static_cast<size_t>(L) * L;
     return 6 * 2 * LL * 2 + 4 * static_cast<size_t>(L);
 }
+
+// Discrete adjoint (ftwa_adjoint.hip).  Work area per trajectory: 10 + seg_steps + 1 complex matrices
+// of both spins; gradient accumulators per trajectory: [U_ii (L)][tM_kk (L)][ELL tM (z L)][dJ/da per
+// RK stage (n_stage, 0 without a pulse)].
+template <typename Real>
+size_t adjoint_work_bytes(int L, int64_t B, int64_t seg_steps);
+template <typename Real>
+hipError_t launch_adjoint_forward(const DevSystem<Real>& s, const RunShape& r, int64_t n0, int64_t n1, Real* state,
+                                  Real* work, hipStream_t st);
+template <typename Real>
+hipError_t launch_adjoint_terminal(const DevSystem<Real>& s, int64_t B, const Real* state, Real* work, double* jb,
+                                   hipStream_t st);
+template <typename Real>
+hipError_t launch_adjoint_backward(const DevSystem<Real>& s, const RunShape& r, int64_t n0, int64_t n1,
+                                   const Real* ckpt, Real* segbuf, Real* work, double* gacc, int64_t n_stage,
+                                   hipStream_t st);
+hipError_t launch_adjoint_reduce(const double* src, int64_t B, int64_t n, double* out, hipStream_t st);
 
 // Ensemble reduction of records for time points [t0, t0+nt) into acc.
 template <typename Real>

@@ -46,11 +46,11 @@ template <int N>
 using Ic = std::integral_constant<int, N>;
 
 // Value of v held by lane `lane` (wave-uniform index), broadcast to the whole wavefront.
-__device__ __forceinline__ float lane_value(float v, int lane)
+[[maybe_unused]] __device__ __forceinline__ float lane_value(float v, int lane)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
-__device__ __forceinline__ double lane_value(double v, int lane)
+[[maybe_unused]] __device__ __forceinline__ double lane_value(double v, int lane)
 {
     const long long bits = __builtin_bit_cast(long long, v);
     const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), lane);
@@ -319,7 +319,7 @@ __global__ void __launch_bounds__(64 * W)
                     }
                 }
                 eth += s.tdiag[i] * (nu + nd);
-                epe += s.pot[i] * (nu + nd);
+                epe += s.epot[i] * (nu + nd);
                 if (s.coulomb_type == 1) {
                     eu = udiag[i] * nu * nd;
                 } else if (general) {

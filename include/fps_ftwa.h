@@ -168,6 +168,30 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
 #define FPSFTWA_KERNEL_HERMITIAN 2 /* upper-triangle kernel (FP32, L <= 64)                         */
 int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel);
 
+/* Discrete adjoint of a run (no reference counterpart; it differentiates what runSimulation
+ * computes, src/FHfTWASolver.cpp:342-426 with the derivative :568-600 and the step :629-661).
+ * Objective: the ensemble double occupancy at the last time point,
+ *   J = (1/L) sum_i NuNd_i(t_max) = (1/B) sum_b (1/L) sum_i (rho_up,ii + 1/2)(rho_dn,ii + 1/2)
+ * (NuNd as in :395-398, :524-531). The same samples as fpsftwa_run (seed, initial state, rotations)
+ * are integrated with the textbook 3/8 tableau, the state is checkpointed every checkpoint_every
+ * steps, and each segment is recomputed and reversed. Precision follows the engine set by
+ * fpsftwa_set_params (FP32 batched: FP32 trajectories, FP64 gradient sums; otherwise FP64).
+ * Outputs (each may be NULL):
+ *   objective   J;
+ *   grad_u[L]   dJ/dU_ii (on-site Coulomb);
+ *   grad_t[L*L] dJ/dtM_kl for the entries of the hopping pattern (tM_kl != 0) and the diagonal,
+ *               each entry treated as independent (a symmetric bond change is the sum of kl and lk);
+ *               0 elsewhere;
+ *   grad_pulse  knot pulse: dJ/dknot_k [n_knots]; Gaussian pulse: dJ/dA0 [1]; n_pulse its length.
+ * Unsupported: general (distance) Coulomb interactions, L > 256. */
+#define FPSFTWA_OBJ_DOUBLE_OCCUPANCY 0
+typedef struct fpsftwa_adjoint_spec {
+    int32_t objective;        /* FPSFTWA_OBJ_DOUBLE_OCCUPANCY */
+    int32_t checkpoint_every; /* steps between checkpoints, 0 = 50 */
+} fpsftwa_adjoint_spec;
+int fpsftwa_run_adjoint(fpsftwa_handle* h, const fpsftwa_adjoint_spec* spec, double* objective, double* grad_u,
+                        double* grad_t, double* grad_pulse, size_t n_pulse);
+
 /* fps.cpp:88-207 for "method":"fTWA": read <path>, build the system, run, and write
  * <folder><sim_name>_fTWA_res.json (the reference's results format, src/FHSolver.cpp:171-307).
  * out_path overrides the output file name when not NULL. */

@@ -156,6 +156,7 @@ def test_pulse(oracle_mod, kind):
     L = 8
     pos, box = lat.chain(L)
     tM = chain(L, periodic=True)
+    tM[2, 2] = 0.25  # a diagonal hopping also enters the pulse potential (t_ii e^{i A.0})
     d = pos[:, None, :] - pos[None, :, :]
     d[0, L - 1, 0], d[L - 1, 0, 0] = 1.0, -1.0  # minimum image across the ring
     if kind == "gaussian":
