@@ -53,8 +53,6 @@ __device__ __forceinline__ V2<Real> ccmul(V2<Real> a, V2<Real> b)  // conj(a) * 
 {
     return V2<Real>{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ void sc(float x, float* s, float* c) { sincosf(x, s, c); }
-__device__ __forceinline__ void sc(double x, double* s, double* c) { sincos(x, s, c); }
 
 // Per-trajectory work area, in complex elements of NE = 2 L^2 (both spins, row-major).
 template <typename Real>
@@ -78,15 +76,13 @@ __device__ Work<Real> work_of(V2<Real>* base, int64_t b, int NE)
     return w;
 }
 
-// hopping T_ik of the n-th neighbour of site i at amplitude a
+// hopping T_ik of the n-th neighbour of site i: from the stage's table Tst [z][L] with a pulse,
+// tM_ik without
 template <typename Real>
-__device__ __forceinline__ V2<Real> hop(const DevSystem<Real>& s, int n, int i, double a, bool pulse)
+__device__ __forceinline__ V2<Real> hop(const DevSystem<Real>& s, const V2<Real>* Tst, int n, int i)
 {
-    const Real t = s.nbr_t[n * s.L + i];
-    if (!pulse) return V2<Real>{t, Real(0)};
-    Real sn, cs;
-    sc(static_cast<Real>(a) * s.nbr_pd[n * s.L + i], &sn, &cs);
-    return V2<Real>{t * (Real(1) + cs), t * sn};
+    if (Tst) return Tst[n * s.L + i];
+    return V2<Real>{s.nbr_t[n * s.L + i], Real(0)};
 }
 
 // h of both spins from the stage input Y, into sh[2L]
@@ -107,7 +103,7 @@ __device__ void hartree(const DevSystem<Real>& s, const V2<Real>* Y, Real* sh)
 // linear part).
 // ACC adds into out instead of overwriting.
 template <typename Real, bool PLUS_I, bool ACC>
-__device__ void commutator(const DevSystem<Real>& s, double a, bool pulse, const V2<Real>* X, const Real* sh,
+__device__ void commutator(const DevSystem<Real>& s, const V2<Real>* Tst, const V2<Real>* X, const Real* sh,
                            V2<Real>* out)
 {
     const int L = s.L, LL = L * L, NE = 2 * LL, z = s.z;
@@ -116,13 +112,13 @@ __device__ void commutator(const DevSystem<Real>& s, double a, bool pulse, const
         const V2<Real>* Xs = X + sg * LL;
         V2<Real> acc{Real(0), Real(0)};
         for (int n = 0; n < z; ++n) {  // (Delta X)_ij
-            const V2<Real> T = hop(s, n, i, a, pulse);
+            const V2<Real> T = hop(s, Tst, n, i);
             const V2<Real> v = cmul<Real>(T, Xs[s.nbr_k[n * L + i] * L + j]);
             acc.x += v.x;
             acc.y += v.y;
         }
         for (int n = 0; n < z; ++n) {  // (X Delta)_ij, T_kj = conj(T_jk)
-            const V2<Real> T = hop(s, n, j, a, pulse);
+            const V2<Real> T = hop(s, Tst, n, j);
             const V2<Real> v = cmulc<Real>(Xs[i * L + s.nbr_k[n * L + j]], T);
             acc.x -= v.x;
             acc.y -= v.y;
@@ -165,37 +161,36 @@ __device__ void combine(int NE, V2<Real>* dst, const V2<Real>* y, Real c1, const
 
 // Stage inputs Y1..Y4 and slopes k1..k3 of the step from y (k4 too when want_k4).
 template <typename Real>
-__device__ void stages(const DevSystem<Real>& s, const RunShape& r, int64_t g, const V2<Real>* y, Work<Real>& w,
-                       Real* sh, bool want_k4)
+__device__ void stages(const DevSystem<Real>& s, const V2<Real>* ttab, const RunShape& r, int64_t g, const V2<Real>* y,
+                       Work<Real>& w, Real* sh, bool want_k4)
 {
     const int NE = 2 * s.L * s.L;
     const Real h = static_cast<Real>(r.h);
-    const bool pulse = s.amp != nullptr;
-    auto amp = [&](int st) { return pulse ? s.amp[g * 4 + st] : 0.0; };
+    auto tst = [&](int st) { return ttab ? ttab + (g * 4 + st) * s.z * s.L : nullptr; };
     for (int e = threadIdx.x; e < NE; e += blockDim.x) w.Y[0][e] = y[e];
     __syncthreads();
     hartree(s, w.Y[0], sh);
-    commutator<Real, true, false>(s, amp(0), pulse, w.Y[0], sh, w.K[0]);
+    commutator<Real, true, false>(s, tst(0), w.Y[0], sh, w.K[0]);
     combine<Real>(NE, w.Y[1], y, h / Real(3), w.K[0], Real(0), nullptr, Real(0), nullptr);
     hartree(s, w.Y[1], sh);
-    commutator<Real, true, false>(s, amp(1), pulse, w.Y[1], sh, w.K[1]);
+    commutator<Real, true, false>(s, tst(1), w.Y[1], sh, w.K[1]);
     combine<Real>(NE, w.Y[2], y, -h / Real(3), w.K[0], h, w.K[1], Real(0), nullptr);
     hartree(s, w.Y[2], sh);
-    commutator<Real, true, false>(s, amp(2), pulse, w.Y[2], sh, w.K[2]);
+    commutator<Real, true, false>(s, tst(2), w.Y[2], sh, w.K[2]);
     combine<Real>(NE, w.Y[3], y, h, w.K[0], -h, w.K[1], h, w.K[2]);
     if (want_k4) {
         hartree(s, w.Y[3], sh);
-        commutator<Real, true, false>(s, amp(3), pulse, w.Y[3], sh, w.K[3]);
+        commutator<Real, true, false>(s, tst(3), w.Y[3], sh, w.K[3]);
     }
 }
 
 // One forward step in place: y <- y + h (k1 + 3 k2 + 3 k3 + k4) / 8.
 template <typename Real>
-__device__ void step_forward(const DevSystem<Real>& s, const RunShape& r, int64_t g, V2<Real>* y, Work<Real>& w,
-                             Real* sh)
+__device__ void step_forward(const DevSystem<Real>& s, const V2<Real>* ttab, const RunShape& r, int64_t g,
+                             V2<Real>* y, Work<Real>& w, Real* sh)
 {
     const int NE = 2 * s.L * s.L;
-    stages(s, r, g, y, w, sh, true);
+    stages(s, ttab, r, g, y, w, sh, true);
     const Real h8 = static_cast<Real>(r.h / 8.0), h38 = static_cast<Real>(3.0 * r.h / 8.0);
     for (int e = threadIdx.x; e < NE; e += blockDim.x) {
         V2<Real> v = y[e];
@@ -230,13 +225,14 @@ struct GradLayout {
 
 // VJP of f at stage input Y for cotangent kappa: w.tmp = rho_bar, gradient terms into gacc.
 template <typename Real>
-__device__ void vjp(const DevSystem<Real>& s, double a, bool pulse, const V2<Real>* Y, const V2<Real>* kap,
+__device__ void vjp(const DevSystem<Real>& s, const V2<Real>* Tst, const V2<Real>* Y, const V2<Real>* kap,
                     V2<Real>* out, Real* sh, double* gsh, double* red, double* gacc, const GradLayout& gl,
                     int64_t stage_index)
 {
     const int L = s.L, LL = L * L, z = s.z;
     hartree(s, Y, sh);
-    commutator<Real, false, false>(s, a, pulse, kap, sh, out);
+    const bool pulse = Tst != nullptr;
+    commutator<Real, false, false>(s, Tst, kap, sh, out);
     // G_s,k = Re Q_s,kk = Im( sum_m conj(kap_mk) Y_mk - sum_m Y_km conj(kap_km) )
     for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
         const int sg = e >= L ? 1 : 0, k = e - sg * L;
@@ -270,8 +266,12 @@ __device__ void vjp(const DevSystem<Real>& s, double a, bool pulse, const V2<Rea
         const Real t = s.nbr_t[n * L + k];
         if (t == Real(0)) continue;
         const int l = s.nbr_k[n * L + k];
-        Real sn = 0, cs = 1;
-        if (pulse) sc(static_cast<Real>(a) * s.nbr_pd[n * L + k], &sn, &cs);
+        double sn = 0.0, cs = 1.0;  // e^{i phi} = T / tM - 1
+        if (pulse) {
+            const V2<Real> T = Tst[n * L + k];
+            cs = static_cast<double>(T.x) / static_cast<double>(t) - 1.0;
+            sn = static_cast<double>(T.y) / static_cast<double>(t);
+        }
         double gt = 0.0;
         for (int sg = 0; sg < 2; ++sg) {
             const V2<Real>* Ys = Y + sg * LL;
@@ -305,14 +305,15 @@ __device__ void vjp(const DevSystem<Real>& s, double a, bool pulse, const V2<Rea
 
 template <typename Real>
 __global__ void __launch_bounds__(kAdjThreads)
-    adj_forward_kernel(DevSystem<Real> s, RunShape r, int64_t n0, int64_t n1, V2<Real>* state, V2<Real>* work)
+    adj_forward_kernel(DevSystem<Real> s, const V2<Real>* ttab, RunShape r, int64_t n0, int64_t n1, V2<Real>* state,
+                       V2<Real>* work)
 {
     __shared__ Real sh[2 * 256];
     const int NE = 2 * s.L * s.L;
     const int64_t b = blockIdx.x;
     V2<Real>* y = state + b * NE;
     Work<Real> w = work_of<Real>(work, b, NE);
-    for (int64_t g = n0; g < n1; ++g) step_forward(s, r, g, y, w, sh);
+    for (int64_t g = n0; g < n1; ++g) step_forward(s, ttab, r, g, y, w, sh);
 }
 
 // J_b and the terminal cotangent lambda = dJ_b/d rho at the last time point
@@ -342,7 +343,7 @@ __global__ void __launch_bounds__(kAdjThreads)
 // One segment [n0, n1): recompute its states from the checkpoint, then reverse it.
 template <typename Real>
 __global__ void __launch_bounds__(kAdjThreads)
-    adj_backward_kernel(DevSystem<Real> s, RunShape r, int64_t n0, int64_t n1, const V2<Real>* ckpt,
+    adj_backward_kernel(DevSystem<Real> s, const V2<Real>* ttab, RunShape r, int64_t n0, int64_t n1, const V2<Real>* ckpt,
                         V2<Real>* segbuf, V2<Real>* work, double* gacc_all, GradLayout gl)
 {
     __shared__ Real sh[2 * 256];
@@ -362,14 +363,13 @@ __global__ void __launch_bounds__(kAdjThreads)
         V2<Real>* nxt = cur + NE;
         for (int e = threadIdx.x; e < NE; e += blockDim.x) nxt[e] = cur[e];
         __syncthreads();
-        step_forward(s, r, g, nxt, w, sh);
+        step_forward(s, ttab, r, g, nxt, w, sh);
     }
     const Real h = static_cast<Real>(r.h);
     const Real h8 = static_cast<Real>(r.h / 8.0), h38 = static_cast<Real>(3.0 * r.h / 8.0);
-    const bool pulse = s.amp != nullptr;
     for (int64_t g = n1 - 1; g >= n0; --g) {
         const V2<Real>* yg = seg + (g - n0) * NE;
-        stages(s, r, g, yg, w, sh, false);
+        stages(s, ttab, r, g, yg, w, sh, false);
         // kbar_s = h b_s lambda (the K buffers now hold cotangents)
         for (int e = threadIdx.x; e < NE; e += blockDim.x) {
             const V2<Real> l = w.lam[e];
@@ -379,9 +379,9 @@ __global__ void __launch_bounds__(kAdjThreads)
             w.K[3][e] = V2<Real>{h8 * l.x, h8 * l.y};
         }
         __syncthreads();
-        auto amp = [&](int st) { return pulse ? s.amp[g * 4 + st] : 0.0; };
+        auto tst = [&](int st) { return ttab ? ttab + (g * 4 + st) * s.z * s.L : nullptr; };
         // stage 4: Y4 = y + h (k1 - k2 + k3)
-        vjp(s, amp(3), pulse, w.Y[3], w.K[3], w.tmp, sh, gsh, red, gacc, gl, g * 4 + 3);
+        vjp(s, tst(3), w.Y[3], w.K[3], w.tmp, sh, gsh, red, gacc, gl, g * 4 + 3);
         for (int e = threadIdx.x; e < NE; e += blockDim.x) {
             const V2<Real> v = w.tmp[e];
             w.lam[e].x += v.x;
@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(kAdjThreads)
         }
         __syncthreads();
         // stage 3: Y3 = y + h (k2 - k1/3)
-        vjp(s, amp(2), pulse, w.Y[2], w.K[2], w.tmp, sh, gsh, red, gacc, gl, g * 4 + 2);
+        vjp(s, tst(2), w.Y[2], w.K[2], w.tmp, sh, gsh, red, gacc, gl, g * 4 + 2);
         for (int e = threadIdx.x; e < NE; e += blockDim.x) {
             const V2<Real> v = w.tmp[e];
             w.lam[e].x += v.x;
@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(kAdjThreads)
         }
         __syncthreads();
         // stage 2: Y2 = y + h k1/3
-        vjp(s, amp(1), pulse, w.Y[1], w.K[1], w.tmp, sh, gsh, red, gacc, gl, g * 4 + 1);
+        vjp(s, tst(1), w.Y[1], w.K[1], w.tmp, sh, gsh, red, gacc, gl, g * 4 + 1);
         for (int e = threadIdx.x; e < NE; e += blockDim.x) {
             const V2<Real> v = w.tmp[e];
             w.lam[e].x += v.x;
@@ -417,13 +417,27 @@ __global__ void __launch_bounds__(kAdjThreads)
         }
         __syncthreads();
         // stage 1: Y1 = y
-        vjp(s, amp(0), pulse, w.Y[0], w.K[0], w.tmp, sh, gsh, red, gacc, gl, g * 4 + 0);
+        vjp(s, tst(0), w.Y[0], w.K[0], w.tmp, sh, gsh, red, gacc, gl, g * 4 + 0);
         for (int e = threadIdx.x; e < NE; e += blockDim.x) {
             w.lam[e].x += w.tmp[e].x;
             w.lam[e].y += w.tmp[e].y;
         }
         __syncthreads();
     }
+}
+
+// Hopping with the pulse at every RK stage: ttab[q][n][i] = tM_ik (1 + e^{i a(tau_q) pd_ik}), k = nb_n(i).
+template <typename Real>
+__global__ void adj_table_kernel(DevSystem<Real> s, int64_t n_stage, V2<Real>* ttab)
+{
+    const int64_t zl = static_cast<int64_t>(s.z) * s.L;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= n_stage * zl) return;
+    const int64_t q = idx / zl, e = idx - q * zl;
+    const double t = static_cast<double>(s.nbr_t[e]);
+    double sn, cs;
+    sincos(s.amp[q] * static_cast<double>(s.nbr_pd[e]), &sn, &cs);
+    ttab[idx] = V2<Real>{static_cast<Real>(t * (1.0 + cs)), static_cast<Real>(t * sn)};
 }
 
 // Fixed-order sum over trajectories: out[q] = sum_b src[b][q] (Neumaier), for n quantities.
@@ -451,12 +465,23 @@ size_t adjoint_work_bytes(int L, int64_t B, int64_t seg_steps)
 }
 
 template <typename Real>
-hipError_t launch_adjoint_forward(const DevSystem<Real>& s, const RunShape& r, int64_t n0, int64_t n1, Real* state,
-                                  Real* work, hipStream_t st)
+hipError_t launch_adjoint_table(const DevSystem<Real>& s, int64_t n_stage, Real* ttab, hipStream_t st)
+{
+    const int64_t n = n_stage * s.z * s.L;
+    if (n <= 0 || !s.amp) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(adj_table_kernel<Real>, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, s,
+                       n_stage, reinterpret_cast<V2<Real>*>(ttab));
+    return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_adjoint_forward(const DevSystem<Real>& s, const Real* ttab, const RunShape& r, int64_t n0,
+                                  int64_t n1, Real* state, Real* work, hipStream_t st)
 {
     if (s.L > 256 || r.B <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(adj_forward_kernel<Real>, dim3(static_cast<unsigned>(r.B)), dim3(kAdjThreads), 0, st, s, r, n0,
-                       n1, reinterpret_cast<V2<Real>*>(state), reinterpret_cast<V2<Real>*>(work));
+    hipLaunchKernelGGL(adj_forward_kernel<Real>, dim3(static_cast<unsigned>(r.B)), dim3(kAdjThreads), 0, st, s,
+                       reinterpret_cast<const V2<Real>*>(ttab), r, n0, n1, reinterpret_cast<V2<Real>*>(state),
+                       reinterpret_cast<V2<Real>*>(work));
     return hipGetLastError();
 }
 
@@ -470,15 +495,15 @@ hipError_t launch_adjoint_terminal(const DevSystem<Real>& s, int64_t B, const Re
 }
 
 template <typename Real>
-hipError_t launch_adjoint_backward(const DevSystem<Real>& s, const RunShape& r, int64_t n0, int64_t n1,
-                                   const Real* ckpt, Real* segbuf, Real* work, double* gacc, int64_t n_stage,
-                                   hipStream_t st)
+hipError_t launch_adjoint_backward(const DevSystem<Real>& s, const Real* ttab, const RunShape& r, int64_t n0,
+                                   int64_t n1, const Real* ckpt, Real* segbuf, Real* work, double* gacc,
+                                   int64_t n_stage, hipStream_t st)
 {
     if (s.L > 256 || r.B <= 0 || n1 <= n0) return hipErrorInvalidValue;
     GradLayout gl{s.L, s.z, n_stage};
-    hipLaunchKernelGGL(adj_backward_kernel<Real>, dim3(static_cast<unsigned>(r.B)), dim3(kAdjThreads), 0, st, s, r, n0,
-                       n1, reinterpret_cast<const V2<Real>*>(ckpt), reinterpret_cast<V2<Real>*>(segbuf),
-                       reinterpret_cast<V2<Real>*>(work), gacc, gl);
+    hipLaunchKernelGGL(adj_backward_kernel<Real>, dim3(static_cast<unsigned>(r.B)), dim3(kAdjThreads), 0, st, s,
+                       reinterpret_cast<const V2<Real>*>(ttab), r, n0, n1, reinterpret_cast<const V2<Real>*>(ckpt),
+                       reinterpret_cast<V2<Real>*>(segbuf), reinterpret_cast<V2<Real>*>(work), gacc, gl);
     return hipGetLastError();
 }
 
@@ -491,17 +516,21 @@ hipError_t launch_adjoint_reduce(const double* src, int64_t B, int64_t n, double
 
 template size_t adjoint_work_bytes<float>(int, int64_t, int64_t);
 template size_t adjoint_work_bytes<double>(int, int64_t, int64_t);
-template hipError_t launch_adjoint_forward<float>(const DevSystem<float>&, const RunShape&, int64_t, int64_t, float*,
-                                                  float*, hipStream_t);
-template hipError_t launch_adjoint_forward<double>(const DevSystem<double>&, const RunShape&, int64_t, int64_t,
-                                                   double*, double*, hipStream_t);
+template hipError_t launch_adjoint_table<float>(const DevSystem<float>&, int64_t, float*, hipStream_t);
+template hipError_t launch_adjoint_table<double>(const DevSystem<double>&, int64_t, double*, hipStream_t);
+template hipError_t launch_adjoint_forward<float>(const DevSystem<float>&, const float*, const RunShape&, int64_t,
+                                                  int64_t, float*, float*, hipStream_t);
+template hipError_t launch_adjoint_forward<double>(const DevSystem<double>&, const double*, const RunShape&, int64_t,
+                                                   int64_t, double*, double*, hipStream_t);
 template hipError_t launch_adjoint_terminal<float>(const DevSystem<float>&, int64_t, const float*, float*, double*,
                                                    hipStream_t);
 template hipError_t launch_adjoint_terminal<double>(const DevSystem<double>&, int64_t, const double*, double*,
                                                     double*, hipStream_t);
-template hipError_t launch_adjoint_backward<float>(const DevSystem<float>&, const RunShape&, int64_t, int64_t,
-                                                   const float*, float*, float*, double*, int64_t, hipStream_t);
-template hipError_t launch_adjoint_backward<double>(const DevSystem<double>&, const RunShape&, int64_t, int64_t,
-                                                    const double*, double*, double*, double*, int64_t, hipStream_t);
+template hipError_t launch_adjoint_backward<float>(const DevSystem<float>&, const float*, const RunShape&, int64_t,
+                                                   int64_t, const float*, float*, float*, double*, int64_t,
+                                                   hipStream_t);
+template hipError_t launch_adjoint_backward<double>(const DevSystem<double>&, const double*, const RunShape&, int64_t,
+                                                    int64_t, const double*, double*, double*, double*, int64_t,
+                                                    hipStream_t);
 
 }  // namespace fpsamd

@@ -463,13 +463,19 @@ int run_adjoint(fpsftwa_handle* h, int ck_every, double* objective, double* grad
     rs.T = h->T;
     rs.nsub = h->nsub;
     rs.h = h->dt / h->nsub;
-    DevBuf ck, wk, gacc, jb, out;
+    DevBuf ck, wk, gacc, jb, out, tt;
     HIP_TRY(ck.ensure(std::max<int64_t>(nseg, 1) * B * NE * 2 * sizeof(Real)));
     HIP_TRY(wk.ensure(adjoint_work_bytes<Real>(L, B, K)));
     HIP_TRY(gacc.ensure((size_t)B * G * sizeof(double)));
     HIP_TRY(jb.ensure((size_t)B * sizeof(double)));
     HIP_TRY(out.ensure((size_t)(G + 1) * sizeof(double)));
     HIP_TRY(hipMemsetAsync(gacc.p, 0, (size_t)B * G * sizeof(double), h->stream));
+    Real* ttab = nullptr;
+    if (n_stage) {
+        HIP_TRY(tt.ensure((size_t)n_stage * ds.z * L * 2 * sizeof(Real)));
+        ttab = tt.as<Real>();
+        HIP_TRY(launch_adjoint_table<Real>(ds, n_stage, ttab, h->stream));
+    }
     Real* state = h->state.as<Real>();
     Real* work = wk.as<Real>();
     Real* segbuf = work + (size_t)B * NE * 2 * 10;
@@ -478,11 +484,11 @@ int run_adjoint(fpsftwa_handle* h, int ck_every, double* objective, double* grad
     HIP_TRY(kt.mark(h->stream));
     for (int64_t s = 0; s < nseg; ++s) {
         HIP_TRY(hipMemcpyAsync(ck.as<char>() + s * ck_bytes, state, ck_bytes, hipMemcpyDeviceToDevice, h->stream));
-        HIP_TRY(launch_adjoint_forward<Real>(ds, rs, s * K, std::min(nst, (s + 1) * K), state, work, h->stream));
+        HIP_TRY(launch_adjoint_forward<Real>(ds, ttab, rs, s * K, std::min(nst, (s + 1) * K), state, work, h->stream));
     }
     HIP_TRY(launch_adjoint_terminal<Real>(ds, B, state, work, jb.as<double>(), h->stream));
     for (int64_t s = nseg - 1; s >= 0; --s)
-        HIP_TRY(launch_adjoint_backward<Real>(ds, rs, s * K, std::min(nst, (s + 1) * K),
+        HIP_TRY(launch_adjoint_backward<Real>(ds, ttab, rs, s * K, std::min(nst, (s + 1) * K),
                                               reinterpret_cast<const Real*>(ck.as<char>() + s * ck_bytes), segbuf,
                                               work, gacc.as<double>(), n_stage, h->stream));
     HIP_TRY(launch_adjoint_reduce(gacc.as<double>(), B, G, out.as<double>(), h->stream));

@@ -103,16 +103,19 @@ __host__ __device__ inline size_t plain_work_doubles(int L)
 // RK stage (n_stage, 0 without a pulse)].
 template <typename Real>
 size_t adjoint_work_bytes(int L, int64_t B, int64_t seg_steps);
+// Pulse hopping of every RK stage, [n_stage][z][L] complex (ttab is null without a pulse).
 template <typename Real>
-hipError_t launch_adjoint_forward(const DevSystem<Real>& s, const RunShape& r, int64_t n0, int64_t n1, Real* state,
-                                  Real* work, hipStream_t st);
+hipError_t launch_adjoint_table(const DevSystem<Real>& s, int64_t n_stage, Real* ttab, hipStream_t st);
+template <typename Real>
+hipError_t launch_adjoint_forward(const DevSystem<Real>& s, const Real* ttab, const RunShape& r, int64_t n0,
+                                  int64_t n1, Real* state, Real* work, hipStream_t st);
 template <typename Real>
 hipError_t launch_adjoint_terminal(const DevSystem<Real>& s, int64_t B, const Real* state, Real* work, double* jb,
                                    hipStream_t st);
 template <typename Real>
-hipError_t launch_adjoint_backward(const DevSystem<Real>& s, const RunShape& r, int64_t n0, int64_t n1,
-                                   const Real* ckpt, Real* segbuf, Real* work, double* gacc, int64_t n_stage,
-                                   hipStream_t st);
+hipError_t launch_adjoint_backward(const DevSystem<Real>& s, const Real* ttab, const RunShape& r, int64_t n0,
+                                   int64_t n1, const Real* ckpt, Real* segbuf, Real* work, double* gacc,
+                                   int64_t n_stage, hipStream_t st);
 hipError_t launch_adjoint_reduce(const double* src, int64_t B, int64_t n, double* out, hipStream_t st);
 
 // Ensemble reduction of records for time points [t0, t0+nt) into acc.
