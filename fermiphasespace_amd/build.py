@@ -49,6 +49,7 @@ def build(verbose: bool = False) -> str:
         ("ftwa_stencil.hip", "ftwa_stencil_f64.o", ["-DFTWA_REAL=2"]),
         ("ftwa_herm.hip", "ftwa_herm.o", []),
         ("ftwa_adjoint.hip", "ftwa_adjoint.o", []),
+        ("ftwa_dense.hip", "ftwa_dense.o", []),
         ("ftwa_api.cpp", "ftwa_api.o", ["-x", "hip"]),
     ]
     cxx_units = [("json.cpp", "json.o"), ("system.cpp", "system.o"), ("solver.cpp", "solver.o")]
@@ -63,7 +64,8 @@ def build(verbose: bool = False) -> str:
             f.result()
     objs = [os.path.join(OUT, o) for _, o, _ in hip_units] + [os.path.join(OUT, o) for _, o in cxx_units]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib", "-lrocblas",
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     _run([cxx, *CXX_FLAGS, os.path.join(CSRC, "fps_main.cpp"), "-o", CLI + ".tmp", "-L" + PKG, "-lfps_ftwa",
           "-Wl,-rpath,$ORIGIN"])

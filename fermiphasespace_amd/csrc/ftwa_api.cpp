@@ -93,6 +93,8 @@ struct fpsftwa_handle {
     int time_chunk = 0;
     int kernel = FPSFTWA_KERNEL_AUTO;
     bool used_herm = false;
+    bool used_dense = false;
+    DenseCtx* dense = nullptr;  // GEMM library context of the dense path, created on first use
     // results
     bool has_result = false;
     int T = 0;
@@ -101,7 +103,7 @@ struct fpsftwa_handle {
     int64_t launches = 0;
     double kernel_ms = 0.0, run_ms = 0.0;
     // device buffers
-    DevBuf sys_buf, state, rec, part, acc_d, work, noise_d, rot_d, scratch, occ_d;
+    DevBuf sys_buf, state, rec, part, acc_d, work, noise_d, rot_d, scratch, occ_d, dense_buf;
     bool state_double = true;
 };
 
@@ -322,19 +324,30 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     int rc = upload_system<Real>(h, ds, h->T);
     if (rc) return rc;
     int nt = h->nt_hint;
-    bool herm = false;
-    if constexpr (std::is_same<Real, float>::value) {
+    bool herm = false, dense = h->kernel == FPSFTWA_KERNEL_DENSE;
+    if (dense) {
+        // forced
+    } else if constexpr (std::is_same<Real, float>::value) {
         herm = h->kernel != FPSFTWA_KERNEL_FULL && herm_fits(h->L, ds.z);
         if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !herm)
             return fail(FPSFTWA_EUNSUPPORTED, "the upper-triangle kernel does not fit L=" + std::to_string(h->L));
     } else if (h->kernel == FPSFTWA_KERNEL_HERMITIAN) {
         return fail(FPSFTWA_EUNSUPPORTED, "the upper-triangle kernel is FP32 only");
     }
+    if (!dense && !herm && !stencil_fits<Real>(h->L, ds.z, &nt)) {
+        if (h->kernel == FPSFTWA_KERNEL_FULL)
+            return fail(FPSFTWA_EUNSUPPORTED, "the full-matrix stencil kernel does not support L=" +
+                                                  std::to_string(h->L) + " with " + std::to_string(ds.z) +
+                                                  " neighbours per site");
+        dense = true;  // large or long-range lattices: one GEMM per stage for the batch
+    }
+    if (dense && h->L > 1024) return fail(FPSFTWA_EUNSUPPORTED, "the batched engines support L <= 1024");
     h->used_herm = herm;
-    if (!herm && !stencil_fits<Real>(h->L, ds.z, &nt))
-        return fail(FPSFTWA_EUNSUPPORTED, "the batched engine does not support L=" + std::to_string(h->L) +
-                                              " with " + std::to_string(ds.z) +
-                                              " neighbours per site in this precision; use FPSFTWA_ENGINE_FP64_PLAIN");
+    h->used_dense = dense;
+    if (dense && !h->dense) {
+        h->dense = dense_create(h->stream);
+        if (!h->dense) return fail(FPSFTWA_EDEVICE, "rocBLAS handle creation failed");
+    }
     if ((rc = sample_states<Real>(h, ds))) return rc;
     const int L = h->L;
     const int R = 2 * L + 8;
@@ -354,10 +367,14 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     rs.nsub = h->nsub;
     rs.h = h->dt / h->nsub;
     rs.R = R;
+    if (dense) HIP_TRY(h->dense_buf.ensure(dense_buffer_elems<Real>(L, h->B) * sizeof(Real)));
     for (int t0 = 0; t0 < h->T; t0 += Tc) {
         const int t1 = std::min(h->T, t0 + Tc);
         HIP_TRY(kt.mark(h->stream));
-        if constexpr (std::is_same<Real, float>::value) {
+        if (dense) {
+            HIP_TRY(dense_run<Real>(h->dense, ds, rs, h->dense_buf.as<Real>(), h->state.as<Real>(), h->rec.as<Real>(),
+                                    t0, t1, t0 == 0, t1 == h->T));
+        } else if constexpr (std::is_same<Real, float>::value) {
             if (herm) HIP_TRY(launch_herm(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->nt_hint, h->stream));
             else HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
         } else {
@@ -621,6 +638,9 @@ int fpsftwa_destroy(fpsftwa_handle* h)
     h->work.release();
     h->noise_d.release();
     h->rot_d.release();
+    h->dense_buf.release();
+    dense_destroy(h->dense);
+    h->dense = nullptr;
     h->scratch.release();
     h->occ_d.release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -936,7 +956,7 @@ double pulse_amplitude(const PulseSpec& p, double t)
 extern "C" int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel)
 {
     if (int rc = check_handle(h)) return rc;
-    if (kernel < FPSFTWA_KERNEL_AUTO || kernel > FPSFTWA_KERNEL_HERMITIAN) return fail(FPSFTWA_EINVAL, "bad kernel");
+    if (kernel < FPSFTWA_KERNEL_AUTO || kernel > FPSFTWA_KERNEL_DENSE) return fail(FPSFTWA_EINVAL, "bad kernel");
     h->kernel = kernel;
     if ((block_threads != 0 && block_threads != 512 && block_threads != 832 && block_threads != 1024) || time_chunk < 0)
         return fail(FPSFTWA_EINVAL, "block_threads must be 0 (automatic), 512, 832 or 1024, and time_chunk >= 0");

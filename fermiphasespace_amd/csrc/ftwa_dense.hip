@@ -1,0 +1,461 @@
+// ftwa_dense.hip -- batched fTWA integrator for large or long-range lattices (L up to 1024).
+//
+// Same physics and records as the stencil engines (src/FHfTWASolver.cpp:366-426, :568-600, :629-661),
+// organised around one matrix product per RK stage for the whole batch:
+//
+//   X = Y T  for every (trajectory, spin) at once,  T = hopping (+ pulse Peierls term), shared
+//   k_ij = -i [ X_ij - conj(X_ji) + (h_j - h_i) Y_ij ]          (Y T - T Y = X - X^dagger)
+//
+// The product is a plain GEMM with the batch stacked into the rows, so it goes to the vendor library
+// (rocBLAS, MFMA): rows are (trajectory, spin, i, plane) with the real and imaginary parts of row i
+// stored as two planes, [b][s][i][plane][L].  Without a pulse T is real and X = Z T with Z the
+// [4 B L x L] matrix of planes; with a pulse T = Tr + i Ti and [X_re X_im] = [Y_re Y_im] [[Tr, Ti],
+// [-Ti, Tr]] with K = N = 2L.  Everything else is hand-written:
+//   hartree kernel   h_s,i = pdiag_i + sum_j U_ij (Re rho_s',jj + 1/2) [+ V_ij (Re rho_s,jj + 1/2)]
+//   update kernel    per 32x32 tile pair {(I,J), (J,I)}, I <= J: both X tiles through LDS, the RK4
+//                    "3/8" registers (y, A, p) of the upper-triangle elements, next stage input
+//                    written to both triangles (exactly Hermitian)
+//   record kernel    per time point: diagonal, corner, energies (plain-path formulas)
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "ftwa_device.hpp"
+#include "ftwa_kernels.hpp"
+
+namespace fpsamd {
+
+namespace {
+
+using dev::wave_sum;
+
+constexpr int kTile = 32;
+
+template <typename Real>
+__device__ __forceinline__ size_t row_off(int64_t b, int sg, int i, int L)  // start of row i (plane 0)
+{
+    return ((static_cast<size_t>(b) * 2 + sg) * L + i) * 2 * static_cast<size_t>(L);
+}
+
+// interleaved [b][s][i][j] complex <-> planes [b][s][i][plane][j]; to_planes also sets Ycur = y
+template <typename Real>
+__global__ void dense_pack_kernel(int L, int64_t B, const Real* inter, Real* y, Real* ycur)
+{
+    const int64_t n = B * 2 * static_cast<int64_t>(L) * L;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t bs = e / (static_cast<int64_t>(L) * L);
+        const int ij = static_cast<int>(e - bs * L * L), i = ij / L, j = ij - i * L;
+        const size_t o = (static_cast<size_t>(bs) * L + i) * 2 * L + j;
+        const Real re = inter[2 * e], im = inter[2 * e + 1];
+        y[o] = re;
+        y[o + L] = im;
+        ycur[o] = re;
+        ycur[o + L] = im;
+    }
+}
+
+template <typename Real>
+__global__ void dense_unpack_kernel(int L, int64_t B, const Real* y, Real* inter)
+{
+    const int64_t n = B * 2 * static_cast<int64_t>(L) * L;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t bs = e / (static_cast<int64_t>(L) * L);
+        const int ij = static_cast<int>(e - bs * L * L), i = ij / L, j = ij - i * L;
+        // the upper triangle carries the state; the lower is its mirror
+        const int a = i <= j ? i : j, c = i <= j ? j : i;
+        const size_t o = (static_cast<size_t>(bs) * L + a) * 2 * L + c;
+        inter[2 * e] = y[o];
+        inter[2 * e + 1] = (i <= j ? Real(1) : Real(-1)) * y[o + L];
+        if (i == j) inter[2 * e + 1] = Real(0);
+    }
+}
+
+// the GEMM's right factor for stage amplitude a: real T [L][L] (no pulse) or the real 2L x 2L block
+// form of T = Tr + i Ti (pulse); the diagonal of the hopping is carried by h
+template <typename Real>
+__global__ void dense_hop_kernel(DevSystem<Real> s, int64_t stage, Real* M)
+{
+    const int L = s.L;
+    const bool pulse = s.amp != nullptr;
+    const double a = pulse ? s.amp[stage] : 0.0;
+    const int64_t n = static_cast<int64_t>(L) * L;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int k = static_cast<int>(e / L), l = static_cast<int>(e - static_cast<int64_t>(k) * L);
+        const double t = k == l ? 0.0 : static_cast<double>(s.tM[e]);
+        if (!pulse) {
+            M[e] = static_cast<Real>(t);
+            continue;
+        }
+        double sn = 0.0, cs = 0.0;
+        if (t != 0.0) sincos(a * static_cast<double>(s.pd[e]), &sn, &cs);
+        const Real tr = static_cast<Real>(t * (1.0 + cs)), ti = static_cast<Real>(t * sn);
+        const int64_t L2 = 2 * static_cast<int64_t>(L);
+        M[k * L2 + l] = tr;
+        M[k * L2 + L + l] = ti;
+        M[(L + k) * L2 + l] = -ti;
+        M[(L + k) * L2 + L + l] = tr;
+    }
+}
+
+// h of both spins of one trajectory per workgroup, from the diagonal of Ycur
+template <typename Real>
+__global__ void dense_hartree_kernel(DevSystem<Real> s, const Real* ycur, Real* hv)
+{
+    extern __shared__ unsigned char dsm[];
+    Real* nd = reinterpret_cast<Real*>(dsm);  // [2][L] occupations n = Re rho_ii + 1/2
+    const int L = s.L;
+    const int64_t b = blockIdx.x;
+    for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
+        const int sg = e >= L ? 1 : 0, i = e - sg * L;
+        nd[e] = ycur[row_off<Real>(b, sg, i, L) + i] + Real(0.5);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
+        const int sg = e >= L ? 1 : 0, i = e - sg * L;
+        Real v = s.pdiag[i];
+        if (s.coulomb_type == 1) {
+            v += s.u_diag[i] * nd[(1 - sg) * L + i];
+        } else if (s.coulomb_type == 2) {
+            Real acc = 0;
+            for (int j = 0; j < L; ++j) acc += s.U[i * L + j] * nd[(1 - sg) * L + j] + s.V[i * L + j] * nd[sg * L + j];
+            v += acc;
+        }
+        hv[(b * 2 + sg) * L + i] = v;
+    }
+}
+
+// RK stage ST on the upper triangle.  Grid: (tile pairs x B, 2 spins); 256 threads.
+template <typename Real, int ST>
+__global__ void __launch_bounds__(256)
+    dense_update_kernel(int L, int npair, double hstep, const int2* pairs, const Real* X, const Real* hv, Real* y,
+                        Real* A, Real* p, Real* ycur)
+{
+    __shared__ Real xa[2][kTile][kTile + 1];  // X tile (I, J), planes re/im
+    __shared__ Real xb[2][kTile][kTile + 1];  // X tile (J, I)
+    const int64_t b = blockIdx.x / npair;
+    const int2 pr = pairs[blockIdx.x - b * npair];
+    const int sg = blockIdx.y;
+    const int I0 = pr.x * kTile, J0 = pr.y * kTile;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32 per pass
+    for (int r = ty; r < kTile; r += 8) {
+        const int i = I0 + r, j = J0 + tx;
+        if (i < L && j < L) {
+            const size_t o = row_off<Real>(b, sg, i, L) + j;
+            xa[0][r][tx] = X[o];
+            xa[1][r][tx] = X[o + L];
+        }
+        const int i2 = J0 + r, j2 = I0 + tx;
+        if (i2 < L && j2 < L) {
+            const size_t o = row_off<Real>(b, sg, i2, L) + j2;
+            xb[0][r][tx] = X[o];
+            xb[1][r][tx] = X[o + L];
+        }
+    }
+    __syncthreads();
+    const Real h = static_cast<Real>(hstep);
+    const Real h3 = static_cast<Real>(hstep / 3.0), h6 = static_cast<Real>(hstep / 6.0),
+               h8 = static_cast<Real>(hstep / 8.0);
+    const Real* hs = hv + (b * 2 + sg) * L;
+    for (int r = ty; r < kTile; r += 8) {
+        const int i = I0 + r, j = J0 + tx;
+        if (i >= L || j >= L || i > j) continue;
+        const size_t o = row_off<Real>(b, sg, i, L) + j;
+        const Real xr = xa[0][r][tx], xi = xa[1][r][tx];    // X_ij
+        const Real tr = xb[0][tx][r], ti = xb[1][tx][r];    // X_ji
+        const Real yr = y[o], yi = y[o + L];
+        Real Ar = 0, Ai = 0, pr_ = 0, pi = 0;
+        if (ST > 0) {
+            Ar = A[o];
+            Ai = A[o + L];
+            pr_ = p[o];
+            pi = p[o + L];
+        }
+        Real cr, ci;  // current stage input
+        if (ST == 0) {
+            cr = yr;
+            ci = yi;
+        } else if (ST == 1) {
+            cr = yr + h3 * pr_;
+            ci = yi + h3 * pi;
+        } else if (ST == 2) {
+            cr = yr + h6 * (Ar - Real(3) * pr_);
+            ci = yi + h6 * (Ai - Real(3) * pi);
+        } else {
+            cr = yr + h * pr_;
+            ci = yi + h * pi;
+        }
+        // c = X_ij - conj(X_ji) + (h_j - h_i) Y_ij;  k = -i c
+        const Real dh = hs[j] - hs[i];
+        const Real cx = xr - tr + dh * cr, cy = xi + ti + dh * ci;
+        Real kx = cy, ky = -cx;
+        if (i == j) ky = 0;  // the diagonal slope is real
+        Real nr, ni;
+        if (ST == 0) {
+            Ar = kx;
+            Ai = ky;
+            pr_ = kx;
+            pi = ky;
+            nr = yr + h3 * kx;
+            ni = yi + h3 * ky;
+        } else if (ST == 1) {
+            Ar += Real(3) * kx;
+            Ai += Real(3) * ky;
+            pr_ -= kx;
+            pi -= ky;
+            nr = yr + h6 * (Ar - Real(3) * pr_);
+            ni = yi + h6 * (Ai - Real(3) * pi);
+        } else if (ST == 2) {
+            Ar += Real(3) * kx;
+            Ai += Real(3) * ky;
+            pr_ += kx;
+            pi += ky;
+            nr = yr + h * pr_;
+            ni = yi + h * pi;
+        } else {
+            Ar += kx;
+            Ai += ky;
+            nr = yr + h8 * Ar;
+            ni = yi + h8 * Ai;
+            y[o] = nr;
+            y[o + L] = ni;
+        }
+        if (ST < 3) {
+            A[o] = Ar;
+            A[o + L] = Ai;
+            p[o] = pr_;
+            p[o + L] = pi;
+        }
+        ycur[o] = nr;
+        ycur[o + L] = ni;
+        if (i != j) {
+            const size_t om = row_off<Real>(b, sg, j, L) + i;
+            ycur[om] = nr;
+            ycur[om + L] = -ni;
+        }
+    }
+}
+
+// records of one time point, one trajectory per workgroup (formulas of the plain path)
+template <typename Real>
+__global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real* ycur, Real* rec, int R)
+{
+    __shared__ double ew[4 * 4];
+    const int L = s.L;
+    const int64_t b = blockIdx.x;
+    Real* rr = rec + b * R;
+    for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
+        const int sg = e >= L ? 1 : 0, i = e - sg * L;
+        rr[e] = ycur[row_off<Real>(b, sg, i, L) + i];
+    }
+    if (threadIdx.x == 0) {
+        const size_t ou = row_off<Real>(b, 0, 0, L) + L - 1, od = row_off<Real>(b, 1, 0, L) + L - 1;
+        rr[2 * L + 4] = ycur[ou];
+        rr[2 * L + 5] = ycur[ou + L];
+        rr[2 * L + 6] = ycur[od];
+        rr[2 * L + 7] = ycur[od + L];
+    }
+    double eth = 0, eu = 0, ev = 0, epe = 0;
+    const bool pulse = s.amp != nullptr;
+    const double a = pulse ? s.amp[stage] : 0.0;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        const size_t ou = row_off<Real>(b, 0, i, L), od = row_off<Real>(b, 1, i, L);
+        for (int j = 0; j < L; ++j) {
+            const double tt = static_cast<double>(s.tM[i * L + j]);
+            if (tt == 0.0) continue;
+            const double nr = static_cast<double>(ycur[ou + j]) + static_cast<double>(ycur[od + j]) + (i == j ? 1.0 : 0.0);
+            const double ni = static_cast<double>(ycur[ou + L + j]) + static_cast<double>(ycur[od + L + j]);
+            eth += nr * tt;
+            if (pulse) {
+                double sn, cs;
+                sincos(a * static_cast<double>(s.pd[i * L + j]), &sn, &cs);
+                epe += nr * tt * cs - ni * tt * sn;
+            }
+        }
+        const double nu = static_cast<double>(ycur[ou + i]) + 0.5, ndn = static_cast<double>(ycur[od + i]) + 0.5;
+        epe += static_cast<double>(s.pot[i]) * (nu + ndn);
+        if (s.coulomb_type == 1) {
+            eu += static_cast<double>(s.u_diag[i]) * nu * ndn;
+        } else if (s.coulomb_type == 2) {
+            for (int j = 0; j < L; ++j) {
+                const double nuj = static_cast<double>(ycur[row_off<Real>(b, 0, j, L) + j]) + 0.5;
+                const double ndj = static_cast<double>(ycur[row_off<Real>(b, 1, j, L) + j]) + 0.5;
+                eu += static_cast<double>(s.U[i * L + j]) * (nu * ndj + ndn * nuj) / 2.0;
+                ev += static_cast<double>(s.V[i * L + j]) * (nu * nuj + ndn * ndj) / 2.0;
+            }
+        }
+    }
+    eth = wave_sum(eth);
+    eu = wave_sum(eu);
+    ev = wave_sum(ev);
+    epe = wave_sum(epe);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) / 64;
+    if ((threadIdx.x & 63) == 0) {
+        ew[w * 4 + 0] = eth;
+        ew[w * 4 + 1] = eu;
+        ew[w * 4 + 2] = ev;
+        ew[w * 4 + 3] = epe;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0;
+        for (int k = 0; k < nw; ++k) t += ew[k * 4 + threadIdx.x];
+        rr[2 * L + threadIdx.x] = static_cast<Real>(t);
+    }
+}
+
+unsigned grid_for(int64_t n, int bs)
+{
+    int64_t g = (n + bs - 1) / bs;
+    if (g > 65535 * 16) g = 65535 * 16;
+    return static_cast<unsigned>(g < 1 ? 1 : g);
+}
+
+rocblas_status gemm(rocblas_handle hb, int m, int n, int k, const float* A, int lda, const float* B, int ldb,
+                    float* C, int ldc)
+{
+    const float one = 1.0f, zero = 0.0f;
+    return rocblas_sgemm(hb, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero, C,
+                         ldc);
+}
+rocblas_status gemm(rocblas_handle hb, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                    double* C, int ldc)
+{
+    const double one = 1.0, zero = 0.0;
+    return rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero, C,
+                         ldc);
+}
+
+}  // namespace
+
+struct DenseCtx {
+    rocblas_handle blas = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+DenseCtx* dense_create(hipStream_t st)
+{
+    DenseCtx* c = new DenseCtx;
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+        delete c;
+        return nullptr;
+    }
+    rocblas_set_stream(c->blas, st);
+    c->stream = st;
+    return c;
+}
+
+void dense_destroy(DenseCtx* c)
+{
+    if (!c) return;
+    if (c->blas) rocblas_destroy_handle(c->blas);
+    delete c;
+}
+
+template <typename Real>
+size_t dense_buffer_elems(int L, int64_t B)
+{
+    // y, A, p, Ycur, X: 4 L^2 values per trajectory each; the GEMM factor (2L)^2; h: 2 L per trajectory;
+    // tile pairs
+    const size_t per = 4 * static_cast<size_t>(L) * L * static_cast<size_t>(B);
+    const int nt = (L + kTile - 1) / kTile;
+    const size_t pairs = static_cast<size_t>(nt) * (nt + 1) / 2;
+    return 5 * per + 4 * static_cast<size_t>(L) * L + 2 * static_cast<size_t>(L) * B + pairs * 2 + 64;
+}
+
+template <typename Real>
+hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, Real* buf, Real* state, Real* rec,
+                     int t0, int t1, bool first, bool last)
+{
+    const int L = s.L;
+    const int64_t B = r.B;
+    const size_t per = 4 * static_cast<size_t>(L) * L * static_cast<size_t>(B);
+    Real* y = buf;
+    Real* A = y + per;
+    Real* p = A + per;
+    Real* ycur = p + per;
+    Real* X = ycur + per;
+    Real* M = X + per;
+    Real* hv = M + 4 * static_cast<size_t>(L) * L;
+    int2* pairs = reinterpret_cast<int2*>(hv + 2 * static_cast<size_t>(L) * B);
+    const int nt = (L + kTile - 1) / kTile;
+    const int npair = nt * (nt + 1) / 2;
+    hipStream_t st = c->stream;
+    if (first) {
+        int2 hp[(1024 / kTile) * (1024 / kTile + 1) / 2];
+        int q = 0;
+        for (int I = 0; I < nt; ++I)
+            for (int J = I; J < nt; ++J) hp[q++] = int2{I, J};
+        hipError_t e = hipMemcpyAsync(pairs, hp, npair * sizeof(int2), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(st);  // hp goes out of scope
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(dense_pack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B, state, y,
+                           ycur);
+    }
+    const bool pulse = s.amp != nullptr;
+    if (first && !pulse)
+        hipLaunchKernelGGL(dense_hop_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)), dim3(256), 0,
+                           st, s, 0, M);
+    for (int t = t0; t < t1; ++t) {
+        hipLaunchKernelGGL(dense_record_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256), 0, st, s,
+                           static_cast<int64_t>(t) * r.nsub * 4, ycur, rec + static_cast<int64_t>(t - t0) * B * r.R,
+                           r.R);
+        if (t == r.T - 1) break;
+        for (int sub = 0; sub < r.nsub; ++sub) {
+            for (int stg = 0; stg < 4; ++stg) {
+                const int64_t g = (static_cast<int64_t>(t) * r.nsub + sub) * 4 + stg;
+                hipLaunchKernelGGL(dense_hartree_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256),
+                                   2 * L * sizeof(Real), st, s, ycur, hv);
+                if (pulse)
+                    hipLaunchKernelGGL(dense_hop_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)),
+                                       dim3(256), 0, st, s, g, M);
+                rocblas_status bs;
+                if (!pulse) {
+                    // rows (b, s, i, plane) of length L:  X = Z T, row-major -> column-major X^T = T^T Z^T
+                    bs = gemm(c->blas, L, static_cast<int>(4 * B * L), L, M, L, ycur, L, X, L);
+                } else {
+                    bs = gemm(c->blas, 2 * L, static_cast<int>(2 * B * L), 2 * L, M, 2 * L, ycur, 2 * L, X, 2 * L);
+                }
+                if (bs != rocblas_status_success) return hipErrorLaunchFailure;
+                const dim3 grid(static_cast<unsigned>(npair * B), 2);
+                switch (stg) {
+                    case 0:
+                        hipLaunchKernelGGL((dense_update_kernel<Real, 0>), grid, dim3(256), 0, st, L, npair, r.h, pairs, X,
+                                           hv, y, A, p, ycur);
+                        break;
+                    case 1:
+                        hipLaunchKernelGGL((dense_update_kernel<Real, 1>), grid, dim3(256), 0, st, L, npair, r.h, pairs, X,
+                                           hv, y, A, p, ycur);
+                        break;
+                    case 2:
+                        hipLaunchKernelGGL((dense_update_kernel<Real, 2>), grid, dim3(256), 0, st, L, npair, r.h, pairs, X,
+                                           hv, y, A, p, ycur);
+                        break;
+                    default:
+                        hipLaunchKernelGGL((dense_update_kernel<Real, 3>), grid, dim3(256), 0, st, L, npair, r.h, pairs, X,
+                                           hv, y, A, p, ycur);
+                        break;
+                }
+            }
+        }
+    }
+    if (last) hipLaunchKernelGGL(dense_unpack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B, y,
+                                 state);
+    return hipGetLastError();
+}
+
+}  // namespace fpsamd
+
+namespace fpsamd {
+template size_t dense_buffer_elems<float>(int, int64_t);
+template size_t dense_buffer_elems<double>(int, int64_t);
+template hipError_t dense_run<float>(DenseCtx*, const DevSystem<float>&, const RunShape&, float*, float*, float*, int,
+                                     int, bool, bool);
+template hipError_t dense_run<double>(DenseCtx*, const DevSystem<double>&, const RunShape&, double*, double*, double*,
+                                      int, int, bool, bool);
+}  // namespace fpsamd

@@ -88,6 +88,20 @@ bool herm_fits(int L, int z);
 hipError_t launch_herm(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                        int nt_hint, hipStream_t st);
 
+// Dense batched path (ftwa_dense.hip): one library GEMM X = Y T per RK stage for the whole batch,
+// hand-written Hartree, commutator/RK update and record kernels.  L <= 1024, any neighbour count.
+struct DenseCtx;
+DenseCtx* dense_create(hipStream_t st);
+void dense_destroy(DenseCtx* c);
+// elements of Real the dense path needs for B trajectories (one buffer)
+template <typename Real>
+size_t dense_buffer_elems(int L, int64_t B);
+// time points [t0, t1), records into rec; first packs `state` into the buffer's planes, last unpacks
+// the final state back to `state`
+template <typename Real>
+hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, Real* buf, Real* state, Real* rec,
+                     int t0, int t1, bool first, bool last);
+
 // Plain FP64 path: one launch per time point, dense Delta, global-memory products.
 hipError_t launch_plain(const DevSystem<double>& s, const RunShape& r, int t, double* state, double* work,
                         double* rec, hipStream_t st);

@@ -166,6 +166,8 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
 #define FPSFTWA_KERNEL_AUTO 0      /* upper-triangle kernel where it fits, else full-matrix       */
 #define FPSFTWA_KERNEL_FULL 1      /* full-matrix stencil kernel                                   */
 #define FPSFTWA_KERNEL_HERMITIAN 2 /* upper-triangle kernel (FP32, L <= 64)                         */
+#define FPSFTWA_KERNEL_DENSE 3     /* one GEMM per RK stage for the batch (any precision, L <= 1024);
+                                    * chosen automatically for L > 64 or more than 8 neighbours      */
 int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel);
 
 /* Discrete adjoint of a run (no reference counterpart; it differentiates what runSimulation

@@ -64,9 +64,13 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
                 noise_arr=noise, keep_states=True, rot_up=None if rot is None else rot[0],
                 rot_dn=None if rot is None else rot[1])
     runs = []
+    z = int(((tM != 0) & ~np.eye(L, dtype=bool)).sum(axis=1).max())
+    stencil = L <= 64 and z <= 8  # where the LDS stencil kernels apply
     for e in engines:
-        if e == "fp32":  # both FP32 kernels: full-matrix and upper-triangle (where it fits)
-            runs += [("fp32", "full")] + ([("fp32", "hermitian")] if L <= 64 else [])
+        if e == "fp32":  # every FP32 kernel: full-matrix and upper-triangle (where they fit), dense GEMM
+            runs += ([("fp32", "full"), ("fp32", "hermitian")] if stencil else []) + [("fp32", "dense")]
+        elif e == "fp64":  # the stencil kernel where it fits, and the dense GEMM path
+            runs += ([("fp64", "auto")] if stencil else []) + [("fp64", "dense")]
         else:
             runs.append((e, "auto"))
     for e, kern in runs:
@@ -283,14 +287,25 @@ def test_json_run_writes_reference_format(tmp_path):
     np.testing.assert_allclose(np.asarray(s["NuNd"]), eng.result("NuNd"), rtol=0, atol=0)
 
 
-def test_plain_path_handles_states_beyond_stencil_capacity(oracle_mod):
-    """L = 80 exceeds the batched engine's LDS capacity: it refuses, the plain FP64 path runs."""
+def test_beyond_stencil_capacity_runs_dense(oracle_mod):
+    """L = 80 exceeds the stencil kernels' LDS capacity: the batched engines take the dense GEMM path."""
     L = 80
     n_up = np.array([1.0, 0] * 40)
-    eng = _lib.Engine(chain(L), U=np.eye(L))
-    eng.set_initial_state(n_up, 1 - n_up)
-    eng.set_params(0.02, 0.02, 1, 2, engine="fp64")
-    with pytest.raises(_lib.FtwaError) as e:
-        eng.run()
-    assert e.value.code == _lib.EUNSUPPORTED
-    compare(oracle_mod, chain(L), n_up, 1 - n_up, U=np.eye(L), B=2, t_max=0.04, engines=("fp64_plain",))
+    compare(oracle_mod, chain(L), n_up, 1 - n_up, U=np.eye(L), B=2, t_max=0.04)
+
+
+def test_long_range_hopping_general_coulomb_and_pulse_dense(oracle_mod):
+    """All-to-all hopping (more than 8 neighbours per site), distance Coulomb U and V, site potentials
+    and a Gaussian pulse on a 2D patch: the long-range workload's structure at a small L."""
+    pos, box = lat.grid(5, 4)
+    L = pos.shape[0]
+    d = pos[:, None, :] - pos[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    tM = np.where((r > 0) & (r <= 2.5), -np.exp(-(r - 1.0) / 0.3), 0.0)
+    V = np.where(r > 0, 1.5 / (1.0 + r), 0.0)
+    U = 3.0 * np.eye(L) + V
+    pot = np.linspace(-1.0, 1.0, L)
+    pulse = dict(pulse_type=1, A0=0.5, t0=0.05, td=0.1, w0=1.5, phi=0.0, pol=(2 ** -0.5, 2 ** -0.5, 0.0), disp=d)
+    n_up = np.array([1.0, 0.0] * (L // 2))
+    compare(oracle_mod, tM, n_up, 1 - n_up, coulomb_type=2, U=U, V=V, pot=pot, pulse=pulse, dt=0.01, t_max=0.1,
+            B=4)
