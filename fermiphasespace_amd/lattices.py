@@ -61,6 +61,52 @@ def bcc(n: int, a: float | None = None):
     return np.array(pos), np.diag([n * c] * 3)
 
 
+def graphene(nx: int, ny: int, a: float = 1.0):
+    """Graphene patch of nx x ny two-site cells, carbon distance a (makeGraphene, src/FHLattice.cpp:1093-1130):
+    cell (i, j) holds (0, -a/2) and (0, a/2) shifted by i*(sqrt3 a, 0) + j*(sqrt3 a/2, 3a/2)."""
+    s3 = math.sqrt(3.0)
+    pos = []
+    for j in range(ny):
+        for i in range(nx):
+            x = s3 * a * i + s3 / 2 * a * j
+            y = 1.5 * a * j
+            pos += [[x, y - 0.5 * a, 0.0], [x, y + 0.5 * a, 0.0]]
+    box = np.array([[s3 * a * nx, s3 / 2 * a * ny, 0.0], [0.0, 1.5 * a * ny, 0.0], [0.0, 0.0, 10.0 * a]])
+    return np.array(pos), box
+
+
+def moire(nx: int, ny: int, a: float = 1.0, b: float = 2.35, theta: float = 5.0):
+    """Twisted bilayer (makeMoireGraphene, src/FHLattice.cpp:1272-1307): two graphene patches centred on
+    the middle of their cell box, the second rotated by theta degrees about z and lifted by b.  Open
+    boundaries (the twist breaks the cell's periodicity)."""
+    p0, box = graphene(nx, ny, a)
+    mid = (box[:, 0] + box[:, 1]) / 2.0
+    layers = []
+    for layer in range(2):
+        c, s_ = math.cos(layer * math.radians(theta)), math.sin(layer * math.radians(theta))
+        R = np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]])
+        q = (p0 - mid) @ R.T
+        q[:, 2] = layer * b
+        layers.append(q)
+    return np.vstack(layers), box
+
+
+def workload_moire196(A0=0.5, td=2.0, w0=1.5, t0=5.0, U=3.0, a=1.0):
+    """Long-range twisted bilayer, 2 x 7x7 graphene cells (L = 196): t_ij = -exp(-(r_ij - a)/0.3a) for
+    r_ij <= 4a, V_ij = U/(1 + r_ij/a) (on-site U), Gaussian pulse, half filling (spinWave by site index)."""
+    pos, _ = moire(7, 7, a)
+    L = len(pos)
+    d = pos[:, None, :] - pos[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    off = ~np.eye(L, dtype=bool)
+    tM = np.where(off & (r <= 4.0 * a + 1e-9), -np.exp(-(r - a) / (0.3 * a)), 0.0)
+    V = np.where(off, U / (1.0 + r / a), 0.0)
+    Umat = np.diag(np.full(L, U)) + V
+    n_up = np.array([1.0 if i % 2 == 0 else 0.0 for i in range(L)])
+    pulse = dict(pulse_type=1, A0=A0, t0=t0, td=td, w0=w0, phi=0.0, pol=(2 ** -0.5, 2 ** -0.5, 0.0), disp=d)
+    return {"tM": tM, "U": Umat, "V": V, "n_up": n_up, "n_dn": 1.0 - n_up, "pulse": pulse, "pos": pos, "L": L}
+
+
 def min_image_distances(pos: np.ndarray, box: np.ndarray | None):
     """Minimum-image distances over +-1 cells (src/FHLattice.cpp:516-539)."""
     d = pos[:, None, :] - pos[None, :, :]

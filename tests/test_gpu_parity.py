@@ -309,3 +309,11 @@ def test_long_range_hopping_general_coulomb_and_pulse_dense(oracle_mod):
     n_up = np.array([1.0, 0.0] * (L // 2))
     compare(oracle_mod, tM, n_up, 1 - n_up, coulomb_type=2, U=U, V=V, pot=pot, pulse=pulse, dt=0.01, t_max=0.1,
             B=4)
+
+
+def test_moire196_shape(oracle_mod):
+    """The long-range twisted bilayer (L = 196, 17-65 neighbours per site, dense distance Coulomb U and V,
+    Gaussian pulse) through the dense path, two trajectories over a few steps."""
+    w = lat.workload_moire196(t0=0.005, td=0.01)
+    compare(oracle_mod, w["tM"], w["n_up"], w["n_dn"], coulomb_type=2, U=w["U"], V=w["V"], pulse=w["pulse"],
+            dt=0.005, t_max=0.015, B=2, seed=404, engines=("fp64", "fp32"))
