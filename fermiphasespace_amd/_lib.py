@@ -37,6 +37,7 @@ EXPORTED = [
     "fpsftwa_set_params", "fpsftwa_set_results", "fpsftwa_set_initial_state", "fpsftwa_set_noise",
     "fpsftwa_sample_noise", "fpsftwa_run", "fpsftwa_get_result", "fpsftwa_get_states", "fpsftwa_get_timing",
     "fpsftwa_set_tuning", "fpsftwa_run_json_file", "fpsftwa_load_system_json", "fpsftwa_run_adjoint",
+    "fpsftwa_autotune", "fpsftwa_run_many",
 ]
 
 
@@ -101,6 +102,8 @@ def load(path: str = LIB_PATH):
         "fpsftwa_set_tuning": ([H, i32, i32, i32], ctypes.c_int),
         "fpsftwa_run_json_file": ([ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
         "fpsftwa_run_adjoint": ([H, P(_AdjointSpec), P(d), P(d), P(d), P(d), sz], ctypes.c_int),
+        "fpsftwa_autotune": ([H, i32, P(i32), P(i32), P(i32), P(d)], ctypes.c_int),
+        "fpsftwa_run_many": ([P(H), i32], ctypes.c_int),
         "fpsftwa_load_system_json": ([ctypes.c_char_p, P(i32), P(i32), P(d), P(d), P(d), P(d), P(d), P(d), P(d)],
                                      ctypes.c_int),
     }
@@ -158,6 +161,13 @@ def load_system_json(path: str) -> dict:
     out["L"] = n
     out["coulomb_type"] = ct.value
     return out
+
+
+def run_many(engines) -> None:
+    """Run several engines at once, each on its own stream (fpsftwa_run_many)."""
+    lib = load()
+    arr = (ctypes.c_void_p * len(engines))(*[e._h.value for e in engines])
+    check(lib.fpsftwa_run_many(arr, len(engines)), "run_many")
 
 
 def run_json_file(path: str, out_path: str | None = None) -> None:
@@ -276,6 +286,15 @@ class Engine:
         out = np.zeros((count, 2, self.L, self.L, 2))
         check(self._lib.fpsftwa_get_states(self._h, first, count, _dptr(out), out.size), "get_states")
         return out[..., 0] + 1j * out[..., 1]
+
+    def autotune(self, trial_steps: int = 8) -> dict:
+        """Time the batched kernels on a shortened run and keep the fastest; returns the choice."""
+        k, b, c = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        r = ctypes.c_double(0)
+        check(self._lib.fpsftwa_autotune(self._h, trial_steps, ctypes.byref(k), ctypes.byref(b), ctypes.byref(c),
+                                         ctypes.byref(r)), "autotune")
+        names = {v: n for n, v in KERNELS.items()}
+        return {"kernel": names[k.value], "block_threads": b.value, "time_chunk": c.value, "traj_steps_per_s": r.value}
 
     def adjoint(self, checkpoint_every: int = 50) -> dict:
         """Discrete adjoint of the ensemble double occupancy at the last time point.

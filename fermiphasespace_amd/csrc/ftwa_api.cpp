@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <thread>
 #include <vector>
 
 #include "../../include/fps_ftwa.h"
@@ -962,5 +963,94 @@ extern "C" int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int3
         return fail(FPSFTWA_EINVAL, "block_threads must be 0 (automatic), 512, 832 or 1024, and time_chunk >= 0");
     h->nt_hint = block_threads;
     h->time_chunk = time_chunk;
+    return FPSFTWA_OK;
+}
+
+
+extern "C" int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t* kernel, int32_t* block_threads,
+                                int32_t* time_chunk, double* traj_steps_per_s)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (trial_steps < 1) return fail(FPSFTWA_EINVAL, "trial_steps must be >= 1");
+    if (h->engine == FPSFTWA_ENGINE_FP64_PLAIN) return fail(FPSFTWA_EINVAL, "the plain engine has no tuning");
+    struct Cand {
+        int kernel, block, chunk;
+    };
+    std::vector<Cand> cands;
+    const bool fp32 = h->engine == FPSFTWA_ENGINE_FP32_BATCHED;
+    if (fp32) cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 0, 0});
+    for (int blk : {0, 512, 1024}) cands.push_back({FPSFTWA_KERNEL_FULL, blk, 0});
+    cands.push_back({FPSFTWA_KERNEL_DENSE, 0, 0});
+    if (fp32) cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 0, 8});
+    // a shortened copy of the configured run: trial_steps steps of at most 16,384 trajectories
+    const double t_max = h->t_max;
+    const int64_t B = h->B;
+    const int nsub = h->nsub;
+    const std::vector<double> noise = h->noise;
+    const int saved_kernel = h->kernel, saved_block = h->nt_hint, saved_chunk = h->time_chunk;
+    h->t_max = h->dt * trial_steps;
+    h->B = std::min<int64_t>(B, 16384);
+    h->nsub = 1;
+    if (!noise.empty()) h->noise.clear();
+    double best = -1.0;
+    Cand pick{saved_kernel, saved_block, saved_chunk};
+    const std::string keep_err = g_last_error;
+    for (const Cand& c : cands) {
+        h->kernel = c.kernel;
+        h->nt_hint = c.block;
+        h->time_chunk = c.chunk;
+        double rate = -1.0;
+        for (int rep = 0; rep < 2; ++rep) {  // the first run pays one-time costs (library set-up)
+            if (fpsftwa_run(h) != FPSFTWA_OK) {
+                rate = -1.0;
+                break;
+            }
+            rate = (double)h->B * trial_steps / std::max(h->kernel_ms * 1e-3, 1e-12);
+        }
+        if (rate > best) {
+            best = rate;
+            pick = c;
+        }
+    }
+    g_last_error = keep_err;  // unsupported candidates are expected failures
+    h->t_max = t_max;
+    h->B = B;
+    h->nsub = nsub;
+    h->noise = noise;
+    h->has_result = false;
+    if (best <= 0.0) {
+        h->kernel = saved_kernel;
+        h->nt_hint = saved_block;
+        h->time_chunk = saved_chunk;
+        return fail(FPSFTWA_EUNSUPPORTED, "no batched kernel runs this system");
+    }
+    h->kernel = pick.kernel;
+    h->nt_hint = pick.block;
+    h->time_chunk = pick.chunk;
+    if (kernel) *kernel = pick.kernel;
+    if (block_threads) *block_threads = pick.block;
+    if (time_chunk) *time_chunk = pick.chunk;
+    if (traj_steps_per_s) *traj_steps_per_s = best;
+    return FPSFTWA_OK;
+}
+
+extern "C" int fpsftwa_run_many(fpsftwa_handle** handles, int32_t n)
+{
+    if (!handles || n < 0) return fail(FPSFTWA_EINVAL, "bad handle list");
+    for (int k = 0; k < n; ++k)
+        for (int m = 0; m < k; ++m)
+            if (handles[k] == handles[m]) return fail(FPSFTWA_EINVAL, "a handle appears twice");
+    std::vector<int> rc(n, FPSFTWA_OK);
+    std::vector<std::string> msg(n);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int k = 0; k < n; ++k)
+        th.emplace_back([&, k]() {
+            rc[k] = fpsftwa_run(handles[k]);
+            if (rc[k] != FPSFTWA_OK) msg[k] = g_last_error;  // thread-local message of that thread
+        });
+    for (auto& t : th) t.join();
+    for (int k = 0; k < n; ++k)
+        if (rc[k] != FPSFTWA_OK) return fail(rc[k], "system " + std::to_string(k) + ": " + msg[k]);
     return FPSFTWA_OK;
 }

@@ -107,6 +107,24 @@ def workload_moire196(A0=0.5, td=2.0, w0=1.5, t0=5.0, U=3.0, a=1.0):
     return {"tM": tM, "U": Umat, "V": V, "n_up": n_up, "n_dn": 1.0 - n_up, "pulse": pulse, "pos": pos, "L": L}
 
 
+def workload_mixed(seed=303, t=-1.0, U=4.0):
+    """Uneven batch of four lattices: honeycomb L=24 (z=3), kagome 4x4 L=48 (z=4), 10x10 square L=100
+    (z=4), bcc 3x3x3 L=54 (z=8), all periodic, with site potentials Uniform[-1, 1] drawn from the seed
+    (counter stream 1000*k + 1 + i for site i of system k), spinWave initial state (up on even sites,
+    down on odd, src/FHSystem.cpp:559-570) and trajectory counts 32,768 / 16,384 / 4,096 / 8,192."""
+    specs = [("honeycomb24", honeycomb(4, 3), 32768), ("kagome48", kagome(4, 4), 16384),
+             ("square100", grid(10, 10), 4096), ("bcc54", bcc(3), 8192)]
+    out = []
+    for k, (name, (pos, box), B) in enumerate(specs):
+        L = len(pos)
+        tM, Um = system_arrays(pos, box, True, t=t, U=U)
+        pot = np.array([-1.0 + 2.0 * uniform_draw(seed, 1000 * k + 1 + i) for i in range(L)])
+        n_up = np.array([1.0 if i % 2 == 0 else 0.0 for i in range(L)])
+        out.append({"name": name, "L": L, "tM": tM, "U": Um, "pot": pot, "n_up": n_up, "n_dn": 1.0 - n_up,
+                    "n_traj": B, "z": int((tM != 0).sum(axis=1).max())})
+    return out
+
+
 def min_image_distances(pos: np.ndarray, box: np.ndarray | None):
     """Minimum-image distances over +-1 cells (src/FHLattice.cpp:516-539)."""
     d = pos[:, None, :] - pos[None, :, :]

@@ -170,6 +170,20 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
                                     * chosen automatically for L > 64 or more than 8 neighbours      */
 int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel);
 
+/* Autotuner (no reference counterpart): times every batched kernel family and shape that can run
+ * this system (upper-triangle, full-matrix stencil at 512/1024 threads, dense GEMM; two time-chunk
+ * lengths) on a shortened copy of the configured run -- trial_steps steps of at most 16,384
+ * trajectories, each candidate run twice and the second timed -- and keeps the fastest as the
+ * handle's tuning (fpsftwa_set_tuning). Outputs may be NULL. Requires a batched engine. */
+int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t* kernel, int32_t* block_threads,
+                     int32_t* time_chunk, double* traj_steps_per_s);
+
+/* Runs n handles at once (no reference counterpart: the reference runs one system per process,
+ * fps.cpp:192-207). Each handle runs fpsftwa_run from its own host thread on its own stream, so
+ * systems of different size and lattice share the GPU; results are per handle, as from fpsftwa_run.
+ * Returns the first failure (its message names the system). */
+int fpsftwa_run_many(fpsftwa_handle** handles, int32_t n);
+
 /* Discrete adjoint of a run (no reference counterpart; it differentiates what runSimulation
  * computes, src/FHfTWASolver.cpp:342-426 with the derivative :568-600 and the step :629-661).
  * Objective: the ensemble double occupancy at the last time point,

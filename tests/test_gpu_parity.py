@@ -317,3 +317,46 @@ def test_moire196_shape(oracle_mod):
     w = lat.workload_moire196(t0=0.005, td=0.01)
     compare(oracle_mod, w["tM"], w["n_up"], w["n_dn"], coulomb_type=2, U=w["U"], V=w["V"], pulse=w["pulse"],
             dt=0.005, t_max=0.015, B=2, seed=404, engines=("fp64", "fp32"))
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+def test_mixed_batch_lattices(oracle_mod, k):
+    """Each lattice of the uneven batch (honeycomb 24, kagome 48, square 100, bcc 54, with site
+    potentials) against the oracle on every engine that runs it."""
+    w = lat.workload_mixed()[k]
+    compare(oracle_mod, w["tM"], w["n_up"], w["n_dn"], U=w["U"], pot=w["pot"], dt=0.01, t_max=0.05, B=3, seed=303)
+
+
+def test_run_many_matches_sequential_runs():
+    """fpsftwa_run_many (the four systems at once, one stream each) gives bitwise the results of
+    running them one after another."""
+    ws = lat.workload_mixed()
+    engines = []
+    for w in ws:
+        e = _lib.Engine(w["tM"], U=w["U"], pot=w["pot"])
+        e.set_initial_state(w["n_up"], w["n_dn"])
+        e.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES)
+        e.set_params(0.01, 0.1, 1, 64, seed=303, engine="fp32")
+        engines.append(e)
+    _lib.run_many(engines)
+    together = [{k: e.result(k) for k in ("Niiu", "NuNd", "E")} for e in engines]
+    for e, got in zip(engines, together):
+        e.run()
+        for k in ("Niiu", "NuNd", "E"):
+            assert np.array_equal(e.result(k), got[k]), k
+        e.close()
+
+
+def test_autotune_picks_a_kernel_that_reproduces_the_run():
+    """The autotuner's choice runs the same FP64 dynamics (1e-10) as the default choice."""
+    w = lat.workload_mixed()[1]
+    e = _lib.Engine(w["tM"], U=w["U"], pot=w["pot"])
+    e.set_initial_state(w["n_up"], w["n_dn"])
+    e.set_params(0.01, 0.05, 1, 256, seed=303, engine="fp64")
+    e.run()
+    ref = e.result("sum_u")
+    choice = e.autotune(trial_steps=3)
+    assert choice["kernel"] in ("full", "dense") and choice["traj_steps_per_s"] > 0
+    e.run()
+    assert rel(e.result("sum_u"), ref) < FP64_RTOL
+    e.close()
