@@ -240,6 +240,279 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Hopping of the stage, T_ik for the ELL entries [z][L] (k = nb_n(i)): tM_ik (1 + e^{i a pd_ik}) with a
+// pulse, tM_ik without.
+template <typename Real>
+__global__ void dense_ell_table_kernel(DevSystem<Real> s, int64_t stage, Real* tt)
+{
+    const int n = s.z * s.L;
+    const bool pulse = s.amp != nullptr;
+    const double a = pulse ? s.amp[stage] : 0.0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const double t = static_cast<double>(s.nbr_t[e]);
+        double sn = 0.0, cs = 0.0;
+        if (pulse) sincos(a * static_cast<double>(s.nbr_pd[e]), &sn, &cs);
+        tt[2 * e] = static_cast<Real>(pulse ? t * (1.0 + cs) : t);
+        tt[2 * e + 1] = static_cast<Real>(pulse ? t * sn : 0.0);
+    }
+}
+
+// RK stage ST for short-range lattices without the product: X_ij = sum_n Y_{i,k_n(j)} conj(T_{j,k_n(j)})
+// from full rows of the stage input staged in LDS (rows of tile I and of tile J), then the same update
+// as dense_update_kernel.  The stage input is double-buffered (yin -> yout) because other workgroups
+// still read rows this one rewrites.  Grid: (tile pairs x B, 2 spins); 256 threads.
+template <typename Real, int ST>
+__global__ void __launch_bounds__(256)
+    dense_stencil_update_kernel(int L, int z, int npair, double hstep, const int2* pairs, const int* nbk,
+                                const Real* ttab, const Real* hv, const Real* yin, Real* y, Real* A, Real* p,
+                                Real* yout)
+{
+    extern __shared__ unsigned char dsm[];
+    Real* rowsI = reinterpret_cast<Real*>(dsm);           // [32][2][L]
+    Real* rowsJ = rowsI + kTile * 2 * L;                   // [32][2][L]
+    Real* Ts = rowsJ + kTile * 2 * L;                      // [z][L][2]
+    int* ks = reinterpret_cast<int*>(Ts + 2 * z * L);      // [z][L]
+    const int64_t b = blockIdx.x / npair;
+    const int2 pr = pairs[blockIdx.x - b * npair];
+    const int sg = blockIdx.y;
+    const int I0 = pr.x * kTile, J0 = pr.y * kTile;
+    const int nI = min(kTile, L - I0), nJ = min(kTile, L - J0);
+    for (int e = threadIdx.x; e < nI * 2 * L; e += blockDim.x) rowsI[e] = yin[row_off<Real>(b, sg, I0, L) + e];
+    for (int e = threadIdx.x; e < nJ * 2 * L; e += blockDim.x) rowsJ[e] = yin[row_off<Real>(b, sg, J0, L) + e];
+    for (int e = threadIdx.x; e < z * L; e += blockDim.x) {
+        ks[e] = nbk[e];
+        Ts[2 * e] = ttab[2 * e];
+        Ts[2 * e + 1] = ttab[2 * e + 1];
+    }
+    __syncthreads();
+    const Real h = static_cast<Real>(hstep);
+    const Real h3 = static_cast<Real>(hstep / 3.0), h6 = static_cast<Real>(hstep / 6.0),
+               h8 = static_cast<Real>(hstep / 8.0);
+    const Real* hs = hv + (b * 2 + sg) * L;
+    for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+        const int r = e >> 5, c = e & 31;
+        const int i = I0 + r, j = J0 + c;
+        if (i >= L || j >= L || i > j) continue;
+        // X_ij from row i (tile I), X_ji from row j (tile J)
+        const Real* ri = rowsI + r * 2 * L;
+        const Real* rj = rowsJ + c * 2 * L;
+        Real xr = 0, xi = 0, tr = 0, ti = 0;
+        for (int n = 0; n < z; ++n) {
+            const int kj = ks[n * L + j], ki = ks[n * L + i];
+            const Real Tjr = Ts[2 * (n * L + j)], Tji = Ts[2 * (n * L + j) + 1];  // T_{j,kj}; T_{kj,j} = conj
+            const Real Tir = Ts[2 * (n * L + i)], Tii = Ts[2 * (n * L + i) + 1];
+            const Real ar = ri[kj], ai = ri[L + kj];   // Y_{i,kj}
+            const Real br = rj[ki], bi = rj[L + ki];   // Y_{j,ki}
+            xr += ar * Tjr + ai * Tji;                  // Y_{i,kj} conj(T_{j,kj})
+            xi += ai * Tjr - ar * Tji;
+            tr += br * Tir + bi * Tii;
+            ti += bi * Tir - br * Tii;
+        }
+        const size_t o = row_off<Real>(b, sg, i, L) + j;
+        const Real yr = y[o], yi = y[o + L];
+        Real Ar = 0, Ai = 0, pr_ = 0, pi = 0;
+        if (ST > 0) {
+            Ar = A[o];
+            Ai = A[o + L];
+            pr_ = p[o];
+            pi = p[o + L];
+        }
+        const Real cr = ri[j], ci = ri[L + j];  // current stage input Y_ij
+        const Real dh = hs[j] - hs[i];
+        const Real cx = xr - tr + dh * cr, cy = xi + ti + dh * ci;
+        Real kx = cy, ky = -cx;
+        if (i == j) ky = 0;
+        Real nr, ni;
+        if (ST == 0) {
+            Ar = kx;
+            Ai = ky;
+            pr_ = kx;
+            pi = ky;
+            nr = yr + h3 * kx;
+            ni = yi + h3 * ky;
+        } else if (ST == 1) {
+            Ar += Real(3) * kx;
+            Ai += Real(3) * ky;
+            pr_ -= kx;
+            pi -= ky;
+            nr = yr + h6 * (Ar - Real(3) * pr_);
+            ni = yi + h6 * (Ai - Real(3) * pi);
+        } else if (ST == 2) {
+            Ar += Real(3) * kx;
+            Ai += Real(3) * ky;
+            pr_ += kx;
+            pi += ky;
+            nr = yr + h * pr_;
+            ni = yi + h * pi;
+        } else {
+            Ar += kx;
+            Ai += ky;
+            nr = yr + h8 * Ar;
+            ni = yi + h8 * Ai;
+            y[o] = nr;
+            y[o + L] = ni;
+        }
+        if (ST < 3) {
+            A[o] = Ar;
+            A[o + L] = Ai;
+            p[o] = pr_;
+            p[o + L] = pi;
+        }
+        yout[o] = nr;
+        yout[o + L] = ni;
+        if (i != j) {
+            const size_t om = row_off<Real>(b, sg, j, L) + i;
+            yout[om] = nr;
+            yout[om + L] = -ni;
+        }
+    }
+}
+
+// dst[0, n) = src[0, n) by the whole workgroup, 16-byte accesses where both are 16-byte aligned
+template <typename Real>
+__device__ __forceinline__ void copy_block(Real* dst, const Real* src, int n)
+{
+    constexpr int V = 16 / sizeof(Real);
+    int done = 0;
+    if ((reinterpret_cast<uintptr_t>(dst) % 16) == 0 && (reinterpret_cast<uintptr_t>(src) % 16) == 0) {
+        const int nv = n / V;
+        for (int e = threadIdx.x; e < nv; e += blockDim.x)
+            reinterpret_cast<float4*>(dst)[e] = reinterpret_cast<const float4*>(src)[e];
+        done = nv * V;
+    }
+    for (int e = done + threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
+}
+
+// RK stage ST with one workgroup per (trajectory, spin) holding that spin's whole stage input in LDS
+// (short-range lattices up to L ~ 140 in FP32): read once, X_ij and X_ji gathered from LDS for the
+// thread's upper-triangle elements (at most QMAX each), a barrier, the RK update against HBM
+// registers, the next stage input written in place (both triangles) and streamed out coalesced.
+// TAB: neighbour tables staged in LDS (false: read through the caches, so two workgroups fit a CU)
+template <typename Real, int ST, int QMAX, bool TAB>
+__global__ void __launch_bounds__(1024)
+    dense_wg_update_kernel(int L, int z, double hstep, const int* __restrict__ nbk, const Real* __restrict__ ttab,
+                           const Real* hv, const Real* yin, Real* y, Real* A, Real* p, Real* yout)
+{
+    extern __shared__ unsigned char dsm[];
+    Real* Ys = reinterpret_cast<Real*>(dsm);              // [L][2][L] as in HBM
+    const Real* Ts = ttab;                                // [z][L][2]
+    const int* ks = nbk;                                  // [z][L]
+    const int64_t b = blockIdx.x >> 1;
+    const int sg = blockIdx.x & 1;
+    const size_t base = row_off<Real>(b, sg, 0, L);
+    const int n2 = 2 * L * L;
+    copy_block(Ys, yin + base, n2);
+    if constexpr (TAB) {
+        Real* Tl = Ys + 2 * L * L;
+        int* kl = reinterpret_cast<int*>(Tl + 2 * z * L);
+        for (int e = threadIdx.x; e < z * L; e += blockDim.x) {
+            kl[e] = nbk[e];
+            Tl[2 * e] = ttab[2 * e];
+            Tl[2 * e + 1] = ttab[2 * e + 1];
+        }
+        Ts = Tl;
+        ks = kl;
+    }
+    __syncthreads();
+    const int NU = L * (L + 1) / 2;
+    const Real* hs = hv + (b * 2 + sg) * L;
+    int ii[QMAX], jj[QMAX];
+    Real kx[QMAX], ky[QMAX];
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+        const int u = threadIdx.x + q * blockDim.x;
+        ii[q] = -1;
+        if (u >= NU) continue;
+        // row i of the upper triangle (with diagonal): the largest i with i*(2L - i + 1)/2 <= u
+        const float c = static_cast<float>(2 * L + 1);
+        int i = static_cast<int>(0.5f * (c - sqrtf(fmaxf(c * c - 8.0f * static_cast<float>(u), 0.0f))));
+        i = max(0, min(i, L - 1));
+        while (i > 0 && i * (2 * L - i + 1) / 2 > u) --i;
+        while (i + 1 < L && (i + 1) * (2 * L - i) / 2 <= u) ++i;
+        const int j = i + (u - i * (2 * L - i + 1) / 2);
+        ii[q] = i;
+        jj[q] = j;
+        const Real* ri = Ys + i * 2 * L;
+        const Real* rj = Ys + j * 2 * L;
+        Real xr = 0, xi = 0, tr = 0, ti = 0;
+        for (int n = 0; n < z; ++n) {
+            const int kj = ks[n * L + j], ki = ks[n * L + i];
+            const Real Tjr = Ts[2 * (n * L + j)], Tji = Ts[2 * (n * L + j) + 1];
+            const Real Tir = Ts[2 * (n * L + i)], Tii = Ts[2 * (n * L + i) + 1];
+            const Real ar = ri[kj], ai = ri[L + kj];
+            const Real br = rj[ki], bi = rj[L + ki];
+            xr += ar * Tjr + ai * Tji;
+            xi += ai * Tjr - ar * Tji;
+            tr += br * Tir + bi * Tii;
+            ti += bi * Tir - br * Tii;
+        }
+        const Real dh = hs[j] - hs[i];
+        const Real cx = xr - tr + dh * ri[j], cy = xi + ti + dh * ri[L + j];
+        kx[q] = cy;
+        ky[q] = i == j ? Real(0) : -cx;
+    }
+    __syncthreads();  // every gather of this stage input is done before it is overwritten
+    const Real h = static_cast<Real>(hstep);
+    const Real h3 = static_cast<Real>(hstep / 3.0), h6 = static_cast<Real>(hstep / 6.0),
+               h8 = static_cast<Real>(hstep / 8.0);
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+        if (ii[q] < 0) continue;
+        const int i = ii[q], j = jj[q];
+        const size_t o = base + static_cast<size_t>(i) * 2 * L + j;
+        const Real yr = y[o], yi = y[o + L];
+        Real Ar = 0, Ai = 0, pr_ = 0, pi = 0;
+        if (ST > 0) {
+            Ar = A[o];
+            Ai = A[o + L];
+            pr_ = p[o];
+            pi = p[o + L];
+        }
+        Real nr, ni;
+        if (ST == 0) {
+            Ar = kx[q];
+            Ai = ky[q];
+            pr_ = kx[q];
+            pi = ky[q];
+            nr = yr + h3 * kx[q];
+            ni = yi + h3 * ky[q];
+        } else if (ST == 1) {
+            Ar += Real(3) * kx[q];
+            Ai += Real(3) * ky[q];
+            pr_ -= kx[q];
+            pi -= ky[q];
+            nr = yr + h6 * (Ar - Real(3) * pr_);
+            ni = yi + h6 * (Ai - Real(3) * pi);
+        } else if (ST == 2) {
+            Ar += Real(3) * kx[q];
+            Ai += Real(3) * ky[q];
+            pr_ += kx[q];
+            pi += ky[q];
+            nr = yr + h * pr_;
+            ni = yi + h * pi;
+        } else {
+            Ar += kx[q];
+            Ai += ky[q];
+            nr = yr + h8 * Ar;
+            ni = yi + h8 * Ai;
+            y[o] = nr;
+            y[o + L] = ni;
+        }
+        if (ST < 3) {
+            A[o] = Ar;
+            A[o + L] = Ai;
+            p[o] = pr_;
+            p[o + L] = pi;
+        }
+        Ys[i * 2 * L + j] = nr;
+        Ys[i * 2 * L + L + j] = ni;
+        Ys[j * 2 * L + i] = nr;
+        Ys[j * 2 * L + L + i] = -ni;
+    }
+    __syncthreads();
+    copy_block(yout + base, Ys, n2);
+}
+
 // records of one time point, one trajectory per workgroup (formulas of the plain path)
 template <typename Real>
 __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real* ycur, Real* rec, int R)
@@ -264,15 +537,16 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
     const double a = pulse ? s.amp[stage] : 0.0;
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
         const size_t ou = row_off<Real>(b, 0, i, L), od = row_off<Real>(b, 1, i, L);
-        for (int j = 0; j < L; ++j) {
-            const double tt = static_cast<double>(s.tM[i * L + j]);
+        for (int n = 0; n <= s.z; ++n) {  // the neighbour list of i, then the diagonal entry
+            const int j = n < s.z ? s.nbr_k[n * L + i] : i;
+            const double tt = n < s.z ? static_cast<double>(s.nbr_t[n * L + i]) : static_cast<double>(s.tdiag[i]);
             if (tt == 0.0) continue;
             const double nr = static_cast<double>(ycur[ou + j]) + static_cast<double>(ycur[od + j]) + (i == j ? 1.0 : 0.0);
             const double ni = static_cast<double>(ycur[ou + L + j]) + static_cast<double>(ycur[od + L + j]);
             eth += nr * tt;
             if (pulse) {
-                double sn, cs;
-                sincos(a * static_cast<double>(s.pd[i * L + j]), &sn, &cs);
+                double sn = 0.0, cs = 1.0;
+                if (n < s.z) sincos(a * static_cast<double>(s.nbr_pd[n * L + i]), &sn, &cs);
                 epe += nr * tt * cs - ni * tt * sn;
             }
         }
@@ -398,9 +672,51 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                            ycur);
     }
     const bool pulse = s.amp != nullptr;
-    if (first && !pulse)
-        hipLaunchKernelGGL(dense_hop_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)), dim3(256), 0,
-                           st, s, 0, M);
+    // short-range lattices skip the product: X comes from rows of the stage input staged in LDS
+    const size_t lds_st = 2 * kTile * 2 * static_cast<size_t>(L) * sizeof(Real) +
+                          static_cast<size_t>(s.z) * L * (2 * sizeof(Real) + sizeof(int));
+    const bool stencil = 4 * s.z < L && lds_st <= 160 * 1024;
+    // the whole spin block in one workgroup's LDS when it fits (upper elements <= 8 per thread)
+    const size_t lds_wg = 2 * static_cast<size_t>(L) * L * sizeof(Real) +
+                          static_cast<size_t>(s.z) * L * (2 * sizeof(Real) + sizeof(int));
+    const bool whole = stencil && lds_wg <= 160 * 1024 && L * (L + 1) / 2 <= 8 * 1024;
+    const size_t lds_half = 2 * static_cast<size_t>(L) * L * sizeof(Real);
+    const bool two = whole && 2 * lds_half <= 160 * 1024;  // tables through the caches, two workgroups per CU
+    const dim3 grid(static_cast<unsigned>(npair * B), 2);
+    if (first && !pulse) {
+        if (stencil)
+            hipLaunchKernelGGL(dense_ell_table_kernel<Real>, dim3(grid_for(static_cast<int64_t>(s.z) * L, 256)),
+                               dim3(256), 0, st, s, 0, M);
+        else
+            hipLaunchKernelGGL(dense_hop_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)), dim3(256), 0,
+                               st, s, 0, M);
+    }
+    if (stencil) {
+        static bool attr = false;  // one-time opt-in to more than 64 KB of dynamic LDS
+        if (!attr) {
+            const void* ks[12] = {
+                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 0, 8, false>),
+                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 1, 8, false>),
+                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 2, 8, false>),
+                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 3, 8, false>),reinterpret_cast<const void*>(dense_stencil_update_kernel<Real, 0>),
+                                 reinterpret_cast<const void*>(dense_stencil_update_kernel<Real, 1>),
+                                 reinterpret_cast<const void*>(dense_stencil_update_kernel<Real, 2>),
+                                 reinterpret_cast<const void*>(dense_stencil_update_kernel<Real, 3>),
+                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 0, 8, true>),
+                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 1, 8, true>),
+                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 2, 8, true>),
+                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 3, 8, true>)};
+            for (const void* k : ks) {
+                hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                if (e != hipSuccess) return e;
+            }
+            attr = true;
+        }
+    }
+    // the stage input alternates between ycur and X in the stencil form (four swaps per step, so every
+    // time point finds it in ycur)
+    Real* yin = ycur;
+    Real* yout = X;
     for (int t = t0; t < t1; ++t) {
         hipLaunchKernelGGL(dense_record_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256), 0, st, s,
                            static_cast<int64_t>(t) * r.nsub * 4, ycur, rec + static_cast<int64_t>(t - t0) * B * r.R,
@@ -410,7 +726,34 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
             for (int stg = 0; stg < 4; ++stg) {
                 const int64_t g = (static_cast<int64_t>(t) * r.nsub + sub) * 4 + stg;
                 hipLaunchKernelGGL(dense_hartree_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256),
-                                   2 * L * sizeof(Real), st, s, ycur, hv);
+                                   2 * L * sizeof(Real), st, s, yin, hv);
+                if (stencil) {
+                    if (pulse)
+                        hipLaunchKernelGGL(dense_ell_table_kernel<Real>,
+                                           dim3(grid_for(static_cast<int64_t>(s.z) * L, 256)), dim3(256), 0, st, s, g,
+                                           M);
+#define FPS_STENCIL_STAGE(STG)                                                                                   \
+    if (two)                                                                                                       \
+        hipLaunchKernelGGL((dense_wg_update_kernel<Real, STG, 8, false>), dim3(static_cast<unsigned>(2 * B)),       \
+                           dim3(1024), lds_half, st, L, s.z, r.h, s.nbr_k, M, hv, yin, y, A, p, yout);             \
+    else if (whole)                                                                                                \
+        hipLaunchKernelGGL((dense_wg_update_kernel<Real, STG, 8, true>), dim3(static_cast<unsigned>(2 * B)),        \
+                           dim3(1024), lds_wg, st, L, s.z, r.h, s.nbr_k, M, hv, yin, y, A, p, yout);               \
+    else                                                                                                           \
+        hipLaunchKernelGGL((dense_stencil_update_kernel<Real, STG>), grid, dim3(256), lds_st, st, L, s.z, npair,    \
+                           r.h, pairs, s.nbr_k, M, hv, yin, y, A, p, yout)
+                    switch (stg) {
+                        case 0: FPS_STENCIL_STAGE(0); break;
+                        case 1: FPS_STENCIL_STAGE(1); break;
+                        case 2: FPS_STENCIL_STAGE(2); break;
+                        default: FPS_STENCIL_STAGE(3); break;
+                    }
+#undef FPS_STENCIL_STAGE
+                    Real* tmp = yin;
+                    yin = yout;
+                    yout = tmp;
+                    continue;
+                }
                 if (pulse)
                     hipLaunchKernelGGL(dense_hop_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)),
                                        dim3(256), 0, st, s, g, M);
@@ -422,7 +765,6 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                     bs = gemm(c->blas, 2 * L, static_cast<int>(2 * B * L), 2 * L, M, 2 * L, ycur, 2 * L, X, 2 * L);
                 }
                 if (bs != rocblas_status_success) return hipErrorLaunchFailure;
-                const dim3 grid(static_cast<unsigned>(npair * B), 2);
                 switch (stg) {
                     case 0:
                         hipLaunchKernelGGL((dense_update_kernel<Real, 0>), grid, dim3(256), 0, st, L, npair, r.h, pairs, X,
