@@ -111,7 +111,7 @@ def main():
     ap.add_argument("--n-traj", type=int, default=65536)
     ap.add_argument("--block", type=int, default=0, help="threads per trajectory workgroup (0, 512, 1024)")
     ap.add_argument("--time-chunk", type=int, default=0)
-    ap.add_argument("--kernel", default="auto", choices=["auto", "full", "hermitian"],
+    ap.add_argument("--kernel", default="auto", choices=["auto", "full", "hermitian", "pipelined"],
                     help="FP32 integrator: upper-triangle (hermitian), full-matrix, or automatic")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 batched reference run")

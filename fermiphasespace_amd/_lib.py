@@ -20,8 +20,9 @@ COULOMB_NULL, COULOMB_ONSITE, COULOMB_GENERAL = 0, 1, 2
 PULSE_NONE, PULSE_GAUSSIAN, PULSE_KNOTS = 0, 1, 2
 ENGINE_FP64_PLAIN, ENGINE_FP64_BATCHED, ENGINE_FP32_BATCHED = 0, 1, 2
 ENGINES = {"fp64_plain": ENGINE_FP64_PLAIN, "fp64": ENGINE_FP64_BATCHED, "fp32": ENGINE_FP32_BATCHED}
-KERNEL_AUTO, KERNEL_FULL, KERNEL_HERMITIAN, KERNEL_DENSE = 0, 1, 2, 3
-KERNELS = {"auto": KERNEL_AUTO, "full": KERNEL_FULL, "hermitian": KERNEL_HERMITIAN, "dense": KERNEL_DENSE}
+KERNEL_AUTO, KERNEL_FULL, KERNEL_HERMITIAN, KERNEL_DENSE, KERNEL_PIPELINED = 0, 1, 2, 3, 4
+KERNELS = {"auto": KERNEL_AUTO, "full": KERNEL_FULL, "hermitian": KERNEL_HERMITIAN, "dense": KERNEL_DENSE,
+           "pipelined": KERNEL_PIPELINED}
 
 RES_DIAG, RES_DIAG2, RES_DOUBLONS, RES_ENERGIES, RES_WHOLE = 0x01, 0x02, 0x04, 0x08, 0x10
 

@@ -436,6 +436,335 @@ __global__ void __launch_bounds__(64 * W)
     }
 }
 
+// Spin-pipelined variant (no pulse; null or on-site Coulomb).  Waves 0-7 carry spin up, waves 8-15
+// spin down, and the down group runs half a stage behind, so every interval between two barriers pairs
+// one spin's LDS-heavy phase A with the other spin's VALU-heavy phase B:
+//   even interval of stage s:  up: A(s)      down: B(s - 1)
+//   odd interval of stage s:   up: B(s)      down: A(s)
+// B(s) of one spin reads that spin's h(s) and writes the other spin's h(s + 1) (buffers by stage
+// parity), so each h is produced one interval before it is read and overwritten two intervals after.
+// Records are taken after the even interval of a step's first stage, when both spins sit at the time
+// point.  Per spin: phase A covers 64 columns with 8 waves (8 column groups each), phase B packs the
+// spin's strict upper triangle and its diagonal into element rows owned by wave r % 8, slot r / 8.
+template <int S, int Z>
+__global__ void __launch_bounds__(1024)
+    herm_pipe_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state,
+                     float* __restrict__ rec)
+{
+    constexpr int kNT = 1024;
+    constexpr int kNCG = 8;  // column groups per wave in phase A
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int L = s.L;
+    const int LL = L * L;
+    const int R2 = 2 * L;
+    const int NS = L * (L - 1) / 2;
+    const int n_off = (NS + 63) / 64;  // element rows per spin
+    const int n_dia = (L + 63) / 64;
+    const int tid = threadIdx.x;
+    const int j = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int sg = w >> 3;  // this wave's spin
+    const int gw = w & 7;   // wave within the spin group
+    const int64_t b = blockIdx.x;
+    const int spare = kRows + R2 * kRowBytes;
+    float* ew = reinterpret_cast<float*>(smem + kEW);
+    auto yat = [&](int row, int col) -> C& {
+        return *reinterpret_cast<C*>(smem + kRows + row * kRowBytes + col * 8);
+    };
+    {
+        const int n16 = (kRows + (R2 + 1) * kRowBytes) / 16;
+        for (int e = tid; e < n16; e += kNT) reinterpret_cast<float4*>(smem)[e] = float4{0, 0, 0, 0};
+    }
+    __syncthreads();
+    for (int i = tid; i < L; i += kNT) {
+        *reinterpret_cast<float*>(smem + kSite + i * 4) = s.tdiag[i];
+        *reinterpret_cast<float*>(smem + kSite + 256 + i * 4) = s.epot[i];
+        *reinterpret_cast<float*>(smem + kSite + 512 + i * 4) = s.u_diag[i];
+    }
+    // ---- element slots of this wave's spin (layout as in herm_kernel) ----
+    int aij[S], aji[S], ah[S];
+    C y[S], A[S], p[S];
+    const C* gs = reinterpret_cast<const C*>(state) + b * 2 * LL;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const int er = q * 8 + gw;
+        aij[q] = spare;
+        aji[q] = spare;
+        ah[q] = 128 * 4;
+        y[q] = C{0, 0};
+        A[q] = C{0, 0};
+        p[q] = C{0, 0};
+        if (er < n_off) {
+            const int u = er * 64 + j;
+            if (u < NS) {
+                const float c = static_cast<float>(2 * L - 1);
+                int i = static_cast<int>(0.5f * (c - sqrtf(fmaxf(c * c - 8.0f * static_cast<float>(u), 0.0f))));
+                if (i < 0) i = 0;
+                while (i > 0 && i * (2 * L - i - 1) / 2 > u) --i;
+                while ((i + 1) * (2 * L - i - 2) / 2 <= u) ++i;
+                const int jj = i + 1 + (u - i * (2 * L - i - 1) / 2);
+                aij[q] = kRows + (sg * L + i) * kRowBytes + jj * 8;
+                aji[q] = kRows + (sg * L + jj) * kRowBytes + i * 8;
+                ah[q] = (sg * 64 + i) * 4 | ((sg * 64 + jj) * 4) << 16;
+                y[q] = gs[sg * LL + i * L + jj];
+                *reinterpret_cast<C*>(smem + aij[q]) = y[q];
+                *reinterpret_cast<C*>(smem + aji[q]) = C{y[q].x, -y[q].y};
+            }
+        } else if (er < n_off + n_dia) {
+            const int i = (er - n_off) * 64 + j;
+            if (i < L) {
+                aij[q] = kRows + (sg * L + i) * kRowBytes + i * 8;
+                aji[q] = aij[q];
+                ah[q] = ((1 - sg) * 64 + i) * 4;
+                y[q] = C{gs[sg * LL + i * L + i].x, 0.0f};
+                A[q].y = s.coulomb_type == 1 ? s.u_diag[i] : 0.0f;
+                p[q].y = s.pdiag[i];
+                *reinterpret_cast<C*>(smem + aij[q]) = y[q];
+            }
+        }
+    }
+    // ---- phase-A addresses for this wave's spin: lane j = row i, columns gw + 8 mm ----
+    const int il = min(j, L - 1);
+    int kb[Z];
+    float tv[Z];
+#pragma unroll
+    for (int n = 0; n < Z; ++n) {
+        const int k = j < L ? s.nbr_k[n * L + il] : 0;
+        kb[n] = kRows + (sg * L + k) * kRowBytes + gw * 8;
+        tv[n] = j < L ? s.nbr_t[n * L + il] : 0.0f;
+    }
+    const int xb = (j < L ? kRows + (sg * L + il) * kRowBytes : spare) + kXOff + gw * 8;
+    const float h1 = static_cast<float>(r.h);
+    const float h3 = static_cast<float>(r.h / 3.0), h6 = static_cast<float>(r.h / 6.0),
+                h8 = static_cast<float>(r.h / 8.0);
+    // h of both spins at the first stage, buffer 0, from the diagonal the slots above wrote
+    __syncthreads();
+    for (int e = tid; e < R2; e += kNT) {
+        const int sgh = e >= L ? 1 : 0, i = e - sgh * L;
+        float v = s.pdiag[i];
+        if (s.coulomb_type == 1) v += s.u_diag[i] * (yat((1 - sgh) * L + i, i).x + 0.5f);
+        *reinterpret_cast<float*>(smem + kHB + (sgh * 64 + i) * 4) = v;
+    }
+    __syncthreads();
+
+    auto phase_a = [&]() {
+#pragma unroll
+        for (int mm = 0; mm < kNCG; ++mm) {
+            const int co = mm * 64;  // 8 columns of 8 bytes
+            float ax = 0.0f, ay = 0.0f;
+#pragma unroll
+            for (int n = 0; n < Z; ++n) {
+                const C v = lds_c(smem, kb[n] + co);
+                ax += tv[n] * v.x;
+                ay += tv[n] * v.y;
+            }
+            *reinterpret_cast<C*>(smem + xb + co) = C{ax, ay};
+        }
+    };
+    auto phase_b = [&](auto stc) {
+        constexpr int ST = decltype(stc)::value;
+        constexpr int HC = kHB + (ST & 1) * kHBStride;
+        constexpr int HN = kHB + ((ST + 1) & 1) * kHBStride;
+        C xij[S], xji[S];
+        float hi[S], hj[S];
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            if (q * 8 + gw < n_off + n_dia) {
+                xij[q] = lds_c(smem, aij[q] + kXOff);
+                xji[q] = lds_c(smem, aji[q] + kXOff);
+                int ahq = ah[q];
+                asm volatile("" : "+v"(ahq));
+                hi[q] = lds_f(smem, HC + (ahq & 0xffff));
+                hj[q] = lds_f(smem, HC + (ahq >> 16));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            const int er = q * 8 + gw;
+            if (er < n_off) {
+                const float dh = hj[q] - hi[q];
+                C yc;
+                if constexpr (ST == 0) {
+                    yc = y[q];
+                } else if constexpr (ST == 1) {
+                    yc = C{y[q].x + h3 * p[q].x, y[q].y + h3 * p[q].y};
+                } else if constexpr (ST == 2) {
+                    yc = C{y[q].x + h6 * (A[q].x - 3.0f * p[q].x), y[q].y + h6 * (A[q].y - 3.0f * p[q].y)};
+                } else {
+                    yc = C{y[q].x + h1 * p[q].x, y[q].y + h1 * p[q].y};
+                }
+                const float cx = xji[q].x - xij[q].x + dh * yc.x;
+                const float cy = -xji[q].y - xij[q].y + dh * yc.y;
+                const float kx = cy, ky = -cx;
+                C yn;
+                if constexpr (ST == 0) {
+                    A[q] = C{kx, ky};
+                    p[q] = C{kx, ky};
+                    yn = C{y[q].x + h3 * kx, y[q].y + h3 * ky};
+                } else if constexpr (ST == 1) {
+                    A[q] = C{A[q].x + 3.0f * kx, A[q].y + 3.0f * ky};
+                    p[q] = C{p[q].x - kx, p[q].y - ky};
+                    yn = C{y[q].x + h6 * (A[q].x - 3.0f * p[q].x), y[q].y + h6 * (A[q].y - 3.0f * p[q].y)};
+                } else if constexpr (ST == 2) {
+                    A[q] = C{A[q].x + 3.0f * kx, A[q].y + 3.0f * ky};
+                    p[q] = C{p[q].x + kx, p[q].y + ky};
+                    yn = C{y[q].x + h1 * p[q].x, y[q].y + h1 * p[q].y};
+                } else {
+                    A[q] = C{A[q].x + kx, A[q].y + ky};
+                    y[q] = C{y[q].x + h8 * A[q].x, y[q].y + h8 * A[q].y};
+                    yn = y[q];
+                }
+                *reinterpret_cast<C*>(smem + aij[q]) = yn;
+                *reinterpret_cast<C*>(smem + aji[q]) = C{yn.x, -yn.y};
+            } else if (er < n_off + n_dia) {
+                const float k = -2.0f * xij[q].y;
+                float yn;
+                if constexpr (ST == 0) {
+                    A[q].x = k;
+                    p[q].x = k;
+                    yn = y[q].x + h3 * k;
+                } else if constexpr (ST == 1) {
+                    A[q].x += 3.0f * k;
+                    p[q].x -= k;
+                    yn = y[q].x + h6 * (A[q].x - 3.0f * p[q].x);
+                } else if constexpr (ST == 2) {
+                    A[q].x += 3.0f * k;
+                    p[q].x += k;
+                    yn = y[q].x + h1 * p[q].x;
+                } else {
+                    A[q].x += k;
+                    y[q].x += h8 * A[q].x;
+                    yn = y[q].x;
+                }
+                *reinterpret_cast<float*>(smem + aij[q]) = yn;
+                *reinterpret_cast<float*>(smem + HN + ah[q]) = A[q].y * (yn + 0.5f) + p[q].y;
+            }
+        }
+    };
+
+    float* rr = nullptr;
+    auto record = [&]() {
+        int jt = j, tt = tid;
+        asm volatile("" : "+v"(jt), "+v"(tt));
+        for (int e = tt; e < R2; e += kNT) {
+            const int sgr = e >= L ? 1 : 0, i = e - sgr * L;
+            rr[e] = yat(sgr * L + i, i).x;
+        }
+        if (tid == 0) {
+            const C cu = yat(0, L - 1), cd = yat(L, L - 1);
+            rr[R2 + 4] = cu.x;
+            rr[R2 + 5] = cu.y;
+            rr[R2 + 6] = cd.x;
+            rr[R2 + 7] = cd.y;
+        }
+        float eth = 0, eu = 0, epe = 0;
+        if (w == 0) {  // spin-up tables of wave 0 (column 0): Y(k, i) = conj(Y(i, k)); spin down L rows on
+#pragma unroll
+            for (int n = 0; n < Z; ++n) {
+                const C a = lds_c(smem, kb[n] + jt * 8), c = lds_c(smem, kb[n] + L * kRowBytes + jt * 8);
+                eth += (a.x + c.x) * tv[n];
+            }
+        } else if (w == 1 && jt < L) {
+            const int i = jt;
+            const float nu = yat(i, i).x + 0.5f;
+            const float nd = yat(L + i, i).x + 0.5f;
+            eth = lds_f(smem, kSite + i * 4) * (nu + nd);
+            epe = lds_f(smem, kSite + 256 + i * 4) * (nu + nd);
+            if (s.coulomb_type == 1) eu = lds_f(smem, kSite + 512 + i * 4) * nu * nd;
+        }
+        eth = wave_sum(eth);
+        eu = wave_sum(eu);
+        epe = wave_sum(epe);
+        if (j == 0) {
+            ew[w * 4 + 0] = eth;
+            ew[w * 4 + 1] = eu;
+            ew[w * 4 + 2] = 0.0f;
+            ew[w * 4 + 3] = epe;
+        }
+    };
+    auto flush_energies = [&]() {
+        if (tid == 0) {
+            float e4[4] = {0, 0, 0, 0};
+            for (int ww = 0; ww < 16; ++ww)
+                for (int qq = 0; qq < 4; ++qq) e4[qq] += ew[ww * 4 + qq];
+            for (int qq = 0; qq < 4; ++qq) rr[R2 + qq] = e4[qq];
+        }
+    };
+
+    const int64_t Bn = r.B;
+    const int R = r.R;
+    bool pending = false;  // the down spin still owes phase B of the previous step's last stage
+    for (int t = t0; t < t1; ++t) {
+        rr = rec + (static_cast<int64_t>(t - t0) * Bn + b) * R;
+        if (t == r.T - 1) {
+            if (pending) {
+                if (sg) phase_b(Ic<3>{});
+                __syncthreads();
+                pending = false;
+            }
+            record();
+            __syncthreads();
+            flush_energies();
+            break;
+        }
+        for (int sub = 0; sub < r.nsub; ++sub) {
+            // stage 0
+            if (sg == 0) phase_a();
+            else if (pending) phase_b(Ic<3>{});
+            __syncthreads();
+            if (sub == 0) {
+                record();
+                __syncthreads();
+            }
+            if (sg == 0) phase_b(Ic<0>{});
+            else phase_a();
+            __syncthreads();
+            if (sub == 0) flush_energies();
+            // stage 1
+            if (sg == 0) phase_a();
+            else phase_b(Ic<0>{});
+            __syncthreads();
+            if (sg == 0) phase_b(Ic<1>{});
+            else phase_a();
+            __syncthreads();
+            // stage 2
+            if (sg == 0) phase_a();
+            else phase_b(Ic<1>{});
+            __syncthreads();
+            if (sg == 0) phase_b(Ic<2>{});
+            else phase_a();
+            __syncthreads();
+            // stage 3
+            if (sg == 0) phase_a();
+            else phase_b(Ic<2>{});
+            __syncthreads();
+            if (sg == 0) phase_b(Ic<3>{});
+            else phase_a();
+            __syncthreads();
+            pending = true;
+        }
+    }
+    if (pending) {
+        if (sg) phase_b(Ic<3>{});
+        __syncthreads();
+    }
+    C* go = reinterpret_cast<C*>(state) + b * 2 * LL;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        if (aij[q] == spare) continue;
+        const int er = q * 8 + gw;
+        const int row = (aij[q] - kRows) / kRowBytes;
+        const int col = (aij[q] - kRows - row * kRowBytes) / 8;
+        const int i = row - sg * L;
+        if (er < n_off) {
+            go[sg * LL + i * L + col] = y[q];
+            go[sg * LL + col * L + i] = C{y[q].x, -y[q].y};
+        } else {
+            go[sg * LL + i * L + i] = C{y[q].x, 0.0f};
+        }
+    }
+}
+
 size_t herm_lds_bytes(int L, int z, bool pulse)
 {
     return kRows + static_cast<size_t>(2 * L + 1) * kRowBytes + (pulse ? static_cast<size_t>(z) * L * sizeof(C) : 0);
@@ -500,7 +829,59 @@ hipError_t run_shape(const HShape& h, const DevSystem<float>& s, const RunShape&
     return hipErrorInvalidValue;
 }
 
+int pipe_slots(int L)
+{
+    if (L < 2 || L > 64) return 0;
+    const int rows = (L * (L - 1) / 2 + 63) / 64 + (L + 63) / 64;
+    return (rows + 7) / 8;
+}
+
+template <int S, int Z>
+hipError_t run_pipe(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
+                    hipStream_t st)
+{
+    auto kern = herm_pipe_kernel<S, Z>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(1024), herm_lds_bytes(s.L, s.z, false), st, s, r,
+                       t0, t1, state, rec);
+    return hipGetLastError();
+}
+
+template <int Z>
+hipError_t run_pipe_s(int S, const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
+                      hipStream_t st)
+{
+    switch (S) {
+        case 1: return run_pipe<1, Z>(s, r, t0, t1, state, rec, st);
+        case 2: return run_pipe<2, Z>(s, r, t0, t1, state, rec, st);
+        case 3: return run_pipe<3, Z>(s, r, t0, t1, state, rec, st);
+        case 4: return run_pipe<4, Z>(s, r, t0, t1, state, rec, st);
+        case 5: return run_pipe<5, Z>(s, r, t0, t1, state, rec, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
+
+bool herm_pipe_fits(const DevSystem<float>& s)
+{
+    return !s.amp && s.coulomb_type != 2 && (s.z == 4 || s.z == 8) && pipe_slots(s.L) >= 1 && pipe_slots(s.L) <= 5;
+}
+
+hipError_t launch_herm_pipe(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
+                            hipStream_t st)
+{
+    if (!herm_pipe_fits(s) || r.B <= 0) return hipErrorInvalidValue;
+    const int S = pipe_slots(s.L);
+    if (s.z == 4) return run_pipe_s<4>(S, s, r, t0, t1, state, rec, st);
+    return run_pipe_s<8>(S, s, r, t0, t1, state, rec, st);
+}
 
 bool herm_fits(int L, int z)
 {

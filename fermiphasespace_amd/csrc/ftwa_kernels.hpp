@@ -87,6 +87,10 @@ bool stencil_fits(int L, int z, int* nt);
 bool herm_fits(int L, int z);
 hipError_t launch_herm(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                        int nt_hint, hipStream_t st);
+// Spin-pipelined variant of the upper-triangle kernel (no pulse; null or on-site Coulomb; L <= 64).
+bool herm_pipe_fits(const DevSystem<float>& s);
+hipError_t launch_herm_pipe(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
+                            hipStream_t st);
 
 // Dense batched path (ftwa_dense.hip): one library GEMM X = Y T per RK stage for the whole batch,
 // hand-written Hartree, commutator/RK update and record kernels.  L <= 1024, any neighbour count.

@@ -168,10 +168,12 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
 #define FPSFTWA_KERNEL_HERMITIAN 2 /* upper-triangle kernel (FP32, L <= 64)                         */
 #define FPSFTWA_KERNEL_DENSE 3     /* one GEMM per RK stage for the batch (any precision, L <= 1024);
                                     * chosen automatically for L > 64 or more than 8 neighbours      */
+#define FPSFTWA_KERNEL_PIPELINED 4 /* upper-triangle kernel with the two spins half a stage apart
+                                    * (FP32, L <= 64, no pulse, null or on-site Coulomb)              */
 int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel);
 
 /* Autotuner (no reference counterpart): times every batched kernel family and shape that can run
- * this system (upper-triangle, full-matrix stencil at 512/1024 threads, dense GEMM; two time-chunk
+ * this system (upper-triangle plain and spin-pipelined, full-matrix stencil at 512/1024 threads, dense; two time-chunk
  * lengths) on a shortened copy of the configured run -- trial_steps steps of at most 16,384
  * trajectories, each candidate run twice and the second timed -- and keeps the fastest as the
  * handle's tuning (fpsftwa_set_tuning). Outputs may be NULL. Requires a batched engine. */

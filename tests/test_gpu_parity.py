@@ -69,6 +69,8 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
     for e in engines:
         if e == "fp32":  # every FP32 kernel: full-matrix and upper-triangle (where they fit), dense GEMM
             runs += ([("fp32", "full"), ("fp32", "hermitian")] if stencil else []) + [("fp32", "dense")]
+            if stencil and (pulse or {}).get("pulse_type", 0) == 0 and coulomb_type != 2:
+                runs.append(("fp32", "pipelined"))
         elif e == "fp64":  # the stencil kernel where it fits, and the dense GEMM path
             runs += ([("fp64", "auto")] if stencil else []) + [("fp64", "dense")]
         else:
@@ -219,6 +221,12 @@ def test_tuning_and_chunking_are_bitwise_invariant():
                 np.testing.assert_array_equal(got[k], href[k], err_msg=f"hermitian {chunk} {k}")
     assert rel(href["states"], ref["states"]) < 1e-5
     assert np.max(np.abs(href["states"] - np.conj(np.transpose(href["states"], (0, 1, 3, 2))))) == 0
+    # the spin-pipelined kernel performs the same operations in the same order: bitwise equal
+    for chunk in (0, 1, 6):
+        eng.set_tuning(0, chunk, "pipelined")
+        got = engine_run(eng, 0.005, 0.1, 1, 64, "fp32", seed=9)
+        for k in href:
+            np.testing.assert_array_equal(got[k], href[k], err_msg=f"pipelined {chunk} {k}")
 
 
 def test_fp32_tracks_fp64_on_identical_samples():
