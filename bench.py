@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--states-steps", type=int, default=0,
+                    help="also time this many FP32 steps recording the tight-binding eigenstate occupations "
+                         "(tbStates), to report their cost")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (torch), the reference kernel for HBM counter scaling")
     args = ap.parse_args()
@@ -177,7 +180,7 @@ def main():
     kern_s = tm["kernel_ms"] * 1e-3
     achieved = alg_bytes / kern_s / 1e9
     traffic, prof = traffic_from_profile(os.path.join(ROOT, "profiles", "herm_fp32_hubbard8x8_traffic.json"))
-    kname = "stencil_kernel<float,16,8,4,false,64>" if args.kernel == "full" else "herm_kernel<16,8,5,4,false>"
+    kname = "stencil_kernel<float,16,8,4,false,64>" if args.kernel == "full" else "herm_kernel<16,8,5,4,false,false>"
     if args.kernel == "full" or (prof and prof.get("kernel") != kname):
         traffic = None  # the committed counter figures belong to the other kernel
     z = 4
@@ -200,6 +203,27 @@ def main():
                       "seed": seed, "steps_timed": K, "t_timed": K * dt, "t_max": K * dt, "integrator": "rk4_3_8",
                       "records_every_step": True},
            "roofline": roof}
+
+    # ---- optional: the same FP32 run recording the tight-binding eigenstate occupations ----
+    if args.states_steps > 0:
+        Ks = args.states_steps
+        Vt = np.linalg.eigh(w["tM"])[1]
+        eng.set_bases(np.stack([np.stack([Vt, Vt])]))
+        eng.set_params(dt, Ks * dt, 1, B, seed=seed, engine="fp32")
+        sync()
+        t0 = time.perf_counter()
+        eng.run()
+        sync()
+        els = time.perf_counter() - t0
+        ts = eng.timing()
+        M = eng.result("sum_basis")[0]
+        tr_err = float(np.abs(M[0].sum(axis=0) - eng.result("sum_u").sum(axis=0)).max())
+        out["tb_states"] = {"value": round(B * Ks / els, 1), "unit": "traj-steps/s", "steps": Ks,
+                            "seconds": round(els, 3), "kernel_ms": round(ts["kernel_ms"], 3),
+                            "trace_check_max_abs": tr_err}
+        log(f"fp32 with tbStates: {Ks} steps in {els:.3f} s -> {B * Ks / els:.4e} traj-steps/s; "
+            f"trace check {tr_err:.2e}")
+        eng.set_bases(None)
 
     # ---- FP64 batched reference over the same steps: FP32 error vs FP64 ----
     eng.set_tuning(0, 0, "auto")  # the FP32 kernel choice does not apply to the FP64 engines

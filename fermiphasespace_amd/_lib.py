@@ -30,6 +30,7 @@ OUT = {
     "time": 0, "Niiu": 1, "Niid": 2, "N2iiu": 3, "N2iid": 4, "NuNd": 5, "E": 6, "N1hu": 7, "N1hd": 8,
     "Niiu_err": 9, "Niid_err": 10, "NuNd_err": 11, "E_err": 12,
     "sum_u": 13, "sum_d": 14, "sum2_u": 15, "sum2_d": 16, "sum_ud": 17, "sum_e": 18,
+    "basis": 19, "sum_basis": 20,
 }
 
 # every symbol include/fps_ftwa.h declares
@@ -38,7 +39,8 @@ EXPORTED = [
     "fpsftwa_set_params", "fpsftwa_set_results", "fpsftwa_set_initial_state", "fpsftwa_set_noise",
     "fpsftwa_sample_noise", "fpsftwa_run", "fpsftwa_get_result", "fpsftwa_get_states", "fpsftwa_get_timing",
     "fpsftwa_set_tuning", "fpsftwa_run_json_file", "fpsftwa_load_system_json", "fpsftwa_run_adjoint",
-    "fpsftwa_autotune", "fpsftwa_run_many",
+    "fpsftwa_autotune", "fpsftwa_run_many", "fpsftwa_load_initial_state_json", "fpsftwa_ground_state",
+    "fpsftwa_set_bases",
 ]
 
 
@@ -107,6 +109,9 @@ def load(path: str = LIB_PATH):
         "fpsftwa_run_many": ([P(H), i32], ctypes.c_int),
         "fpsftwa_load_system_json": ([ctypes.c_char_p, P(i32), P(i32), P(d), P(d), P(d), P(d), P(d), P(d), P(d)],
                                      ctypes.c_int),
+        "fpsftwa_load_initial_state_json": ([ctypes.c_char_p, P(i32), P(d), P(d), P(d), P(d)], ctypes.c_int),
+        "fpsftwa_ground_state": ([i32, P(d), P(d), i32, i32, i32, P(d), P(d), P(i32), P(d)], ctypes.c_int),
+        "fpsftwa_set_bases": ([H, i32, P(d)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -162,6 +167,38 @@ def load_system_json(path: str) -> dict:
     out["L"] = n
     out["coulomb_type"] = ct.value
     return out
+
+
+def load_initial_state_json(path: str) -> dict:
+    """Occupations and rotations of the initial state the JSON front end builds (host only)."""
+    lib = load()
+    L = ctypes.c_int32(0)
+    P = ctypes.POINTER(ctypes.c_double)
+    check(lib.fpsftwa_load_initial_state_json(path.encode(), ctypes.byref(L), P(), P(), P(), P()),
+          "load_initial_state_json")
+    n = L.value
+    out = {"n_up": np.zeros(n), "n_dn": np.zeros(n), "rot_up": np.zeros((n, n, 2)), "rot_dn": np.zeros((n, n, 2))}
+    check(lib.fpsftwa_load_initial_state_json(path.encode(), ctypes.byref(L), _dptr(out["n_up"]), _dptr(out["n_dn"]),
+                                              _dptr(out["rot_up"]), _dptr(out["rot_dn"])), "load_initial_state_json")
+    for k in ("rot_up", "rot_dn"):
+        out[k] = out[k][..., 0] + 1j * out[k][..., 1]
+    return out
+
+
+def ground_state(tM, U=None, n_up=None, n_dn=None, mean_field=False) -> dict:
+    """Tight-binding or mean-field eigenbasis (fpsftwa_ground_state): real [L][L] eigenvector columns,
+    ascending eigenvalues, each column's largest-magnitude entry positive.  Host only."""
+    lib = load()
+    tM = _f64(tM)
+    L = tM.shape[0]
+    U = _f64(U) if U is not None else (np.zeros((L, L)) if mean_field else None)
+    n_up = int(np.ceil(L / 2)) if n_up is None else int(n_up)
+    n_dn = int(np.floor(L / 2)) if n_dn is None else int(n_dn)
+    ru, rd = np.zeros((L, L)), np.zeros((L, L))
+    it, res = ctypes.c_int32(0), ctypes.c_double(0)
+    check(lib.fpsftwa_ground_state(L, _dptr(tM), _dptr(U), n_up, n_dn, 1 if mean_field else 0, _dptr(ru), _dptr(rd),
+                                   ctypes.byref(it), ctypes.byref(res)), "ground_state")
+    return {"rot_up": ru, "rot_dn": rd, "iterations": it.value, "residue": res.value}
 
 
 def run_many(engines) -> None:
@@ -272,7 +309,20 @@ class Engine:
         T = self.T
         if name == "time":
             return out
+        if name in ("basis", "sum_basis"):  # [n_bases][2][L][T]
+            return out.reshape(-1, 2, self.L, T)
         return out.reshape(-1, T)
+
+    def set_bases(self, bases):
+        """Bases whose occupations diag(V^T rho V) are recorded: [n_bases][2][L][L] real (columns the
+        basis states), or None to stop recording them."""
+        if bases is None:
+            check(self._lib.fpsftwa_set_bases(self._h, 0, None), "set_bases")
+            return
+        b = np.ascontiguousarray(bases, dtype=np.float64)
+        if b.ndim != 4 or b.shape[1:] != (2, self.L, self.L):
+            raise ValueError(f"bases must have shape (n, 2, {self.L}, {self.L})")
+        check(self._lib.fpsftwa_set_bases(self._h, b.shape[0], _dptr(b)), "set_bases")
 
     @property
     def T(self) -> int:

@@ -52,7 +52,8 @@ def build(verbose: bool = False) -> str:
         ("ftwa_dense.hip", "ftwa_dense.o", []),
         ("ftwa_api.cpp", "ftwa_api.o", ["-x", "hip"]),
     ]
-    cxx_units = [("json.cpp", "json.o"), ("system.cpp", "system.o"), ("solver.cpp", "solver.o")]
+    cxx_units = [("json.cpp", "json.o"), ("system.cpp", "system.o"), ("eigen.cpp", "eigen.o"),
+                 ("solver.cpp", "solver.o")]
     jobs = []
     for src, obj, extra in hip_units:
         jobs.append([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", os.path.join(OUT, obj)])

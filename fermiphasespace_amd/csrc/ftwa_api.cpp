@@ -89,6 +89,8 @@ struct fpsftwa_handle {
     int engine = FPSFTWA_ENGINE_FP64_BATCHED;
     uint32_t flags = FPSFTWA_RES_DIAG | FPSFTWA_RES_ENERGIES;
     std::vector<double> n_up, n_dn, rot_up, rot_dn;
+    int n_bases = 0;
+    std::vector<double> bases;  // [n_bases][2][L][L] real
     std::vector<double> noise;
     int nt_hint = 0;
     int time_chunk = 0;
@@ -101,10 +103,13 @@ struct fpsftwa_handle {
     int T = 0;
     int64_t B_run = 0;
     std::vector<double> acc;  // [T][Q][2]
+    std::vector<double> basis_acc;  // [n_bases][2][L][T] ensemble sums of diag(V^T rho V)
+    int n_bases_run = 0;
     int64_t launches = 0;
     double kernel_ms = 0.0, run_ms = 0.0;
     // device buffers
     DevBuf sys_buf, state, rec, part, acc_d, work, noise_d, rot_d, scratch, occ_d, dense_buf;
+    DevBuf bases_d, dens_part, dens_S, basis_work, basis_out;  // basis occupations
     bool state_double = true;
 };
 
@@ -318,6 +323,40 @@ struct EventTimer {
     }
 };
 
+// Basis occupations: device copies of the bases, the output and the chunk buffers for nt_max time
+// points per reduction.
+int prepare_bases(fpsftwa_handle* h, int nt_max)
+{
+    if (h->n_bases == 0) return FPSFTWA_OK;
+    const int L = h->L;
+    if (!h->dense) {
+        h->dense = dense_create(h->stream);
+        if (!h->dense) return fail(FPSFTWA_EDEVICE, "rocBLAS handle creation failed");
+    }
+    HIP_TRY(h->bases_d.ensure(h->bases.size() * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(h->bases_d.p, h->bases.data(), h->bases.size() * sizeof(double), hipMemcpyHostToDevice,
+                           h->stream));
+    const size_t nout = (size_t)h->n_bases * 2 * L * h->T;
+    HIP_TRY(h->basis_out.ensure(nout * sizeof(double)));
+    const size_t P2 = (size_t)L * (L + 1);
+    HIP_TRY(h->dens_part.ensure((size_t)nt_max * reduce_chunks(h->B) * P2 * 2 * sizeof(double)));
+    HIP_TRY(h->dens_S.ensure((size_t)nt_max * P2 * sizeof(double)));
+    HIP_TRY(h->basis_work.ensure(basis_work_doubles(L, nt_max) * sizeof(double)));
+    return FPSFTWA_OK;
+}
+
+// Density sums of records [t0, t0 + nt) and their projection on every basis.
+template <typename Real>
+int reduce_bases(fpsftwa_handle* h, int R, int nt, int t0)
+{
+    if (h->n_bases == 0) return FPSFTWA_OK;
+    HIP_TRY(launch_density_reduce<Real>(h->L, h->B, R, nt, h->rec.as<Real>(), h->dens_part.as<double>(),
+                                        reduce_chunks(h->B), h->dens_S.as<double>(), h->stream));
+    HIP_TRY(basis_project(h->dense, h->L, nt, t0, h->T, h->n_bases, h->bases_d.as<double>(), h->dens_S.as<double>(),
+                          h->basis_work.as<double>(), h->basis_out.as<double>()));
+    return FPSFTWA_OK;
+}
+
 template <typename Real>
 int run_batched(fpsftwa_handle* h, EventTimer& kt)
 {
@@ -358,23 +397,29 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     }
     if ((rc = sample_states<Real>(h, ds))) return rc;
     const int L = h->L;
-    const int R = 2 * L + 8;
+    const int dens = h->n_bases > 0 ? 1 : 0;
+    const int R = 2 * L + 8 + dens * L * (L + 1);
     const int Q = 6 * L + 12;
     int Tc = h->time_chunk;
     if (Tc <= 0) {
+        // records of up to 64 time points per launch; with the density part (basis occupations) the
+        // records are L/2 times wider and get a larger budget, so the trajectory state is still
+        // carried across several time points per launch
         const double per_tp = (double)h->B * R * sizeof(Real);
-        Tc = (int)std::max(1.0, std::min(64.0, 1.5e9 / per_tp));
+        Tc = (int)std::max(1.0, std::min(64.0, (dens ? 8e9 : 1.5e9) / per_tp));
     }
     Tc = std::min(Tc, h->T);
     const int nchunk = reduce_chunks(h->B);
     HIP_TRY(h->rec.ensure((size_t)Tc * h->B * R * sizeof(Real)));
     HIP_TRY(h->part.ensure((size_t)Tc * nchunk * Q * 2 * sizeof(double)));
+    if ((rc = prepare_bases(h, Tc))) return rc;
     RunShape rs;
     rs.B = h->B;
     rs.T = h->T;
     rs.nsub = h->nsub;
     rs.h = h->dt / h->nsub;
     rs.R = R;
+    rs.dens = dens;
     if (dense) HIP_TRY(h->dense_buf.ensure(dense_buffer_elems<Real>(L, h->B) * sizeof(Real)));
     for (int t0 = 0; t0 < h->T; t0 += Tc) {
         const int t1 = std::min(h->T, t0 + Tc);
@@ -393,6 +438,7 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
         ++h->launches;
         HIP_TRY(launch_reduce<Real>(L, h->B, R, t1 - t0, t0, h->rec.as<Real>(), h->part.as<double>(), nchunk,
                                     h->acc_d.as<double>(), h->stream));
+        if ((rc = reduce_bases<Real>(h, R, t1 - t0, t0))) return rc;
     }
     return FPSFTWA_OK;
 }
@@ -404,18 +450,21 @@ int run_plain(fpsftwa_handle* h, EventTimer& kt)
     if (rc) return rc;
     if ((rc = sample_states<double>(h, ds))) return rc;
     const int L = h->L;
-    const int R = 2 * L + 8;
+    const int dens = h->n_bases > 0 ? 1 : 0;
+    const int R = 2 * L + 8 + dens * L * (L + 1);
     const int Q = 6 * L + 12;
     const int nchunk = reduce_chunks(h->B);
     HIP_TRY(h->rec.ensure((size_t)h->B * R * sizeof(double)));
     HIP_TRY(h->part.ensure((size_t)nchunk * Q * 2 * sizeof(double)));
     HIP_TRY(h->work.ensure((size_t)h->B * plain_work_doubles(L) * sizeof(double)));
+    if ((rc = prepare_bases(h, 1))) return rc;
     RunShape rs;
     rs.B = h->B;
     rs.T = h->T;
     rs.nsub = h->nsub;
     rs.h = h->dt / h->nsub;
     rs.R = R;
+    rs.dens = dens;
     for (int t = 0; t < h->T; ++t) {
         HIP_TRY(kt.mark(h->stream));
         HIP_TRY(launch_plain(ds, rs, t, h->state.as<double>(), h->work.as<double>(), h->rec.as<double>(), h->stream));
@@ -423,6 +472,7 @@ int run_plain(fpsftwa_handle* h, EventTimer& kt)
         ++h->launches;
         HIP_TRY(launch_reduce<double>(L, h->B, R, 1, t, h->rec.as<double>(), h->part.as<double>(), nchunk,
                                       h->acc_d.as<double>(), h->stream));
+        if ((rc = reduce_bases<double>(h, R, 1, t))) return rc;
     }
     return FPSFTWA_OK;
 }
@@ -652,6 +702,11 @@ int fpsftwa_destroy(fpsftwa_handle* h)
     h->dense = nullptr;
     h->scratch.release();
     h->occ_d.release();
+    h->bases_d.release();
+    h->dens_part.release();
+    h->dens_S.release();
+    h->basis_work.release();
+    h->basis_out.release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return FPSFTWA_OK;
@@ -706,6 +761,21 @@ int fpsftwa_set_initial_state(fpsftwa_handle* h, const double* n_up, const doubl
         h->rot_up.clear();
         h->rot_dn.clear();
     }
+    h->has_result = false;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_set_bases(fpsftwa_handle* h, int32_t n_bases, const double* bases)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (n_bases < 0 || n_bases > 2) return fail(FPSFTWA_EINVAL, "n_bases must lie in [0, 2]");
+    if (n_bases > 0 && !bases) return fail(FPSFTWA_EINVAL, "bases are required when n_bases > 0");
+    if (n_bases > 0 && h->L > 1024) return fail(FPSFTWA_EUNSUPPORTED, "basis occupations support L <= 1024");
+    const size_t n = (size_t)n_bases * 2 * h->L * h->L;
+    for (size_t e = 0; e < n; ++e)
+        if (!std::isfinite(bases[e])) return fail(FPSFTWA_EINVAL, "bases must be finite");
+    h->n_bases = n_bases;
+    h->bases.assign(bases, bases + n);
     h->has_result = false;
     return FPSFTWA_OK;
 }
@@ -777,6 +847,11 @@ int fpsftwa_run(fpsftwa_handle* h)
     h->acc.resize((size_t)h->T * Q * 2);
     HIP_TRY(hipMemcpyAsync(h->acc.data(), h->acc_d.p, h->acc.size() * sizeof(double), hipMemcpyDeviceToHost,
                            h->stream));
+    h->n_bases_run = h->n_bases;
+    h->basis_acc.assign((size_t)h->n_bases * 2 * h->L * h->T, 0.0);
+    if (h->n_bases)
+        HIP_TRY(hipMemcpyAsync(h->basis_acc.data(), h->basis_out.p, h->basis_acc.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->kernel_ms = kt.pairs_ms();
     h->run_ms = whole.pairs_ms();
@@ -843,6 +918,11 @@ int fpsftwa_get_result(fpsftwa_handle* h, int32_t which, double* out, size_t n_v
     case FPSFTWA_OUT_SUM_ENERGY: n = 4 * (size_t)T; break;
     case FPSFTWA_OUT_N1HU:
     case FPSFTWA_OUT_N1HD: n = 2 * (size_t)T; break;
+    case FPSFTWA_OUT_BASIS:
+    case FPSFTWA_OUT_SUM_BASIS:
+        if (h->n_bases_run == 0) return fail(FPSFTWA_ESTATE, "the run recorded no basis occupations (fpsftwa_set_bases)");
+        n = h->basis_acc.size();
+        break;
     default:
         if (which < 0 || which >= FPSFTWA_OUT_COUNT) return fail(FPSFTWA_EINVAL, "unknown result id");
         n = (size_t)L * T;
@@ -850,6 +930,11 @@ int fpsftwa_get_result(fpsftwa_handle* h, int32_t which, double* out, size_t n_v
     if (needed) *needed = n;
     if (!out) return FPSFTWA_OK;
     if (n_values < n) return fail(FPSFTWA_EINVAL, "output buffer too small");
+    if (which == FPSFTWA_OUT_BASIS || which == FPSFTWA_OUT_SUM_BASIS) {
+        for (size_t e = 0; e < n; ++e)  // Mkk = mkk / nTraj + 1/2 (src/FHfTWASolver.cpp:531-540)
+            out[e] = which == FPSFTWA_OUT_BASIS ? h->basis_acc[e] / B + 0.5 : h->basis_acc[e];
+        return FPSFTWA_OK;
+    }
     auto S = [&](int t, int q) { return kahan_value(h->acc, Q, t, q); };
     auto stderr_of = [&](double s1, double s2) {
         if (B < 2) return 0.0;

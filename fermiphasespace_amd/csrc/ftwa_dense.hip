@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(1024)
 
 // records of one time point, one trajectory per workgroup (formulas of the plain path)
 template <typename Real>
-__global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real* ycur, Real* rec, int R)
+__global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real* ycur, Real* rec, int R, int dens)
 {
     __shared__ double ew[4 * 4];
     const int L = s.L;
@@ -532,6 +532,11 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
         rr[2 * L + 6] = ycur[od];
         rr[2 * L + 7] = ycur[od + L];
     }
+    if (dens)  // block of 256 threads: 4 waves
+        dev::record_density(rr + 2 * L + 8, L, threadIdx.x & 63, threadIdx.x >> 6, 4, [&](int row, int col) {
+            const int sg = row >= L ? 1 : 0;
+            return ycur[row_off<Real>(b, sg, row - sg * L, L) + col];
+        });
     double eth = 0, eu = 0, ev = 0, epe = 0;
     const bool pulse = s.amp != nullptr;
     const double a = pulse ? s.amp[stage] : 0.0;
@@ -698,7 +703,8 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                                  reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 0, 8, false>),
                                  reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 1, 8, false>),
                                  reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 2, 8, false>),
-                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 3, 8, false>),reinterpret_cast<const void*>(dense_stencil_update_kernel<Real, 0>),
+                                 reinterpret_cast<const void*>(dense_wg_update_kernel<Real, 3, 8, false>),
+                                 reinterpret_cast<const void*>(dense_stencil_update_kernel<Real, 0>),
                                  reinterpret_cast<const void*>(dense_stencil_update_kernel<Real, 1>),
                                  reinterpret_cast<const void*>(dense_stencil_update_kernel<Real, 2>),
                                  reinterpret_cast<const void*>(dense_stencil_update_kernel<Real, 3>),
@@ -720,7 +726,7 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     for (int t = t0; t < t1; ++t) {
         hipLaunchKernelGGL(dense_record_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256), 0, st, s,
                            static_cast<int64_t>(t) * r.nsub * 4, ycur, rec + static_cast<int64_t>(t - t0) * B * r.R,
-                           r.R);
+                           r.R, r.dens);
         if (t == r.T - 1) break;
         for (int sub = 0; sub < r.nsub; ++sub) {
             for (int stg = 0; stg < 4; ++stg) {
@@ -789,6 +795,75 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     if (last) hipLaunchKernelGGL(dense_unpack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B, y,
                                  state);
     return hipGetLastError();
+}
+
+// ---- occupations of fixed bases (tbStates / mfStates) ----
+//
+// The reference sums diag(V^dagger rho V) over trajectories at each time point
+// (src/FHfTWASolver.cpp:385-394).  The bases here are real (eigenvectors of real symmetric
+// matrices) and rho is Hermitian, so (V^T rho V)_kk = sum_ij V_ik V_jk Re rho_ij: linear in rho, and
+// the ensemble sum is diag(V^T S V) with S the ensemble sum of Re rho.  The records carry Re rho's
+// upper triangle, launch_density_reduce forms S per time point, and here one FP64 GEMM per (basis,
+// spin) takes every time point of a chunk at once: P = S V with the time points stacked into the
+// rows, then M_k = sum_i V_ik P_ik.
+
+namespace {
+
+// full[t][i][j] = S_sg(min(i, j), max(i, j)) for nt time points
+__global__ void basis_unpack_kernel(int L, int nt, int sg, const double* __restrict__ S, double* __restrict__ full)
+{
+    const int64_t LL = static_cast<int64_t>(L) * L;
+    const int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (idx >= nt * LL) return;
+    const int t = static_cast<int>(idx / LL);
+    const int e = static_cast<int>(idx - t * LL);
+    const int i = e / L, j = e - (e / L) * L;
+    const int a = i < j ? i : j, b = i < j ? j : i;
+    const int P = L * (L + 1) / 2;
+    full[idx] = S[static_cast<int64_t>(t) * 2 * P + sg * P + dev::packed_row(L, a) + b];
+}
+
+// out[k][t0 + t] (row stride T) = sum_i V[i][k] Pm[t][i][k]
+__global__ void basis_diag_kernel(int L, int nt, int t0, int T, const double* __restrict__ V,
+                                  const double* __restrict__ Pm, double* __restrict__ out)
+{
+    const int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (idx >= static_cast<int64_t>(nt) * L) return;
+    const int t = static_cast<int>(idx / L);
+    const int k = static_cast<int>(idx - static_cast<int64_t>(t) * L);
+    const double* pt = Pm + static_cast<int64_t>(t) * L * L;
+    double m = 0.0;
+    for (int i = 0; i < L; ++i) m += V[static_cast<int64_t>(i) * L + k] * pt[static_cast<int64_t>(i) * L + k];
+    out[static_cast<int64_t>(k) * T + t0 + t] = m;
+}
+
+}  // namespace
+
+size_t basis_work_doubles(int L, int nt) { return 2 * static_cast<size_t>(nt) * L * L; }
+
+hipError_t basis_project(DenseCtx* c, int L, int nt, int t0, int T, int nb, const double* bases, const double* S,
+                         double* work, double* out)
+{
+    const int64_t n = static_cast<int64_t>(nt) * L * L;
+    double* full = work;
+    double* prod = work + n;
+    for (int b = 0; b < nb; ++b) {
+        for (int sg = 0; sg < 2; ++sg) {
+            const double* V = bases + (static_cast<int64_t>(b) * 2 + sg) * L * L;
+            hipLaunchKernelGGL(basis_unpack_kernel, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, L, nt, sg, S,
+                               full);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            // row-major P[nt L][L] = full[nt L][L] V[L][L], i.e. column-major P^T = V^T full^T
+            if (gemm(c->blas, L, nt * L, L, V, L, full, L, prod, L) != rocblas_status_success)
+                return hipErrorLaunchFailure;
+            hipLaunchKernelGGL(basis_diag_kernel, dim3(grid_for(static_cast<int64_t>(nt) * L, 256)), dim3(256), 0,
+                               c->stream, L, nt, t0, T, V, prod, out + (static_cast<int64_t>(b) * 2 + sg) * L * T);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
 }
 
 }  // namespace fpsamd

@@ -64,8 +64,10 @@ __device__ __forceinline__ float lds_f(const unsigned char* base, int off)
     return *reinterpret_cast<const float*>(base + off);
 }
 
-// EPTA: column groups per wavefront in phase A (half per spin); S: element slots per thread in phase B.
-template <int W, int EPTA, int S, int Z, bool PULSE>
+// EPTA: column groups per wavefront in phase A (half per spin); S: element slots per thread in phase B;
+// DENS: records carry the packed real density matrices (a compile-time switch, so the variant without
+// them keeps its register allocation).
+template <int W, int EPTA, int S, int Z, bool PULSE, bool DENS>
 __global__ void __launch_bounds__(64 * W)
     herm_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state, float* __restrict__ rec)
 {
@@ -359,6 +361,8 @@ __global__ void __launch_bounds__(64 * W)
                 rr[R2 + 6] = cd.x;
                 rr[R2 + 7] = cd.y;
             }
+            if constexpr (DENS)
+                dev::record_density(rr + R2 + 8, L, jt, w, kNT / 64, [&](int row, int col) { return yat(row, col).x; });
             // energies of this time point, lane = site i: wave 0 the hopping terms (neighbour addresses
             // from the phase-A tables: Y(k, i) = conj(Y(i, k))), wave 1 the on-site terms
             float eth = 0, eu = 0, ev = 0, epe = 0;
@@ -657,6 +661,8 @@ __global__ void __launch_bounds__(1024)
             rr[R2 + 6] = cd.x;
             rr[R2 + 7] = cd.y;
         }
+        if (r.dens)
+            dev::record_density(rr + R2 + 8, L, jt, w, kNT / 64, [&](int row, int col) { return yat(row, col).x; });
         float eth = 0, eu = 0, epe = 0;
         if (w == 0) {  // spin-up tables of wave 0 (column 0): Y(k, i) = conj(Y(i, k)); spin down L rows on
 #pragma unroll
@@ -792,10 +798,10 @@ HShape herm_shape(int L)
     return {W, ea, s};
 }
 
-template <int W, int EA, int S, int Z, bool PULSE>
+template <int W, int EA, int S, int Z, bool PULSE, bool DENS>
 hipError_t run(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec, hipStream_t st)
 {
-    auto kern = herm_kernel<W, EA, S, Z, PULSE>;
+    auto kern = herm_kernel<W, EA, S, Z, PULSE, DENS>;
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -812,8 +818,13 @@ template <int EA, int S, bool PULSE>
 hipError_t run_z(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                  hipStream_t st)
 {
-    if (s.z == 4) return run<16, EA, S, 4, PULSE>(s, r, t0, t1, state, rec, st);
-    if (s.z == 8) return run<16, EA, S, 8, PULSE>(s, r, t0, t1, state, rec, st);
+    if (r.dens) {
+        if (s.z == 4) return run<16, EA, S, 4, PULSE, true>(s, r, t0, t1, state, rec, st);
+        if (s.z == 8) return run<16, EA, S, 8, PULSE, true>(s, r, t0, t1, state, rec, st);
+    } else {
+        if (s.z == 4) return run<16, EA, S, 4, PULSE, false>(s, r, t0, t1, state, rec, st);
+        if (s.z == 8) return run<16, EA, S, 8, PULSE, false>(s, r, t0, t1, state, rec, st);
+    }
     return hipErrorInvalidValue;
 }
 

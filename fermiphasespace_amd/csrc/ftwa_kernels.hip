@@ -25,6 +25,7 @@ namespace fpsamd {
 namespace {
 
 using dev::V2;
+using dev::Neumaier;
 using dev::wave_sum;
 
 // ---- Philox4x32-10 (Salmon et al., SC'11) -------------------------------------------
@@ -221,6 +222,9 @@ __global__ void __launch_bounds__(kPlainThreads)
             rr[2 * L + 6] = y[2 * (LL + L - 1)];
             rr[2 * L + 7] = y[2 * (LL + L - 1) + 1];
         }
+        if (r.dens)
+            dev::record_density(rr + 2 * L + 8, L, tid & 63, tid >> 6, (NT + 63) / 64,
+                                [&](int row, int col) { return y[2 * (row * L + col)]; });
         double eth = 0, eu = 0, ev = 0, epe = 0;
         const double a = s.amp ? s.amp[static_cast<int64_t>(t) * r.nsub * 4] : 0.0;
         for (int i = tid; i < L; i += NT) {
@@ -352,18 +356,6 @@ __global__ void __launch_bounds__(kPlainThreads)
 #endif
 
 // ---- ensemble reduction ---------------------------------------------------------------------
-struct Neumaier {
-    double s = 0.0, c = 0.0;
-    __device__ __forceinline__ void add(double x)
-    {
-        const double t = s + x;
-        if (fabs(s) >= fabs(x))
-            c += (s - t) + x;
-        else
-            c += (x - t) + s;
-        s = t;
-    }
-};
 
 template <typename Real>
 __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, const Real* __restrict__ rec,
@@ -436,6 +428,45 @@ __global__ void reduce_final_kernel(int Q, int nchunk, int nt, int t0, const dou
     o[1] = a.c;
 }
 
+// Ensemble sums of the density part of the records ([2L+8, R): packed Re rho, both spins), per time
+// point: partial sums over fixed trajectory chunks (grid nt x nchunk x column blocks, one column per
+// thread, reads coalesced along a record), then a fixed-order compensated sum over the chunks.
+template <typename Real>
+__global__ void density_partial_kernel(int L, int64_t B, int R, int nchunk, const Real* __restrict__ rec,
+                                       double* __restrict__ part)
+{
+    const int t = blockIdx.x;
+    const int c = blockIdx.y;
+    const int P2 = L * (L + 1);
+    const int col = blockIdx.z * blockDim.x + threadIdx.x;
+    if (col >= P2) return;
+    const int64_t CB = (B + nchunk - 1) / nchunk;
+    const int64_t bb0 = static_cast<int64_t>(c) * CB;
+    const int64_t bb1 = (bb0 + CB < B) ? bb0 + CB : B;
+    const Real* rt = rec + static_cast<int64_t>(t) * B * R + 2 * L + 8 + col;
+    Neumaier a;
+    for (int64_t bi = bb0; bi < bb1; ++bi) a.add(static_cast<double>(rt[bi * R]));
+    double* po = part + ((static_cast<int64_t>(t) * nchunk + c) * P2 + col) * 2;
+    po[0] = a.s;
+    po[1] = a.c;
+}
+
+__global__ void density_final_kernel(int P2, int nchunk, int nt, const double* __restrict__ part,
+                                     double* __restrict__ S)
+{
+    const int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (idx >= static_cast<int64_t>(nt) * P2) return;
+    const int t = static_cast<int>(idx / P2);
+    const int q = static_cast<int>(idx - static_cast<int64_t>(t) * P2);
+    Neumaier a;
+    for (int c = 0; c < nchunk; ++c) {
+        const double* pp = part + ((static_cast<int64_t>(t) * nchunk + c) * P2 + q) * 2;
+        a.add(pp[0]);
+        a.add(pp[1]);
+    }
+    S[idx] = a.s + a.c;
+}
+
 inline unsigned grid_for(int64_t total, int block)
 {
     int64_t g = (total + block - 1) / block;
@@ -495,6 +526,20 @@ int reduce_chunks(int64_t B)
 #endif
 
 template <typename Real>
+hipError_t launch_density_reduce(int L, int64_t B, int R, int nt, const Real* rec, double* part, int nchunk,
+                                 double* S, hipStream_t st)
+{
+    const int P2 = L * (L + 1);
+    hipLaunchKernelGGL(density_partial_kernel<Real>, dim3(nt, nchunk, (P2 + 255) / 256), dim3(256), 0, st, L, B, R,
+                       nchunk, rec, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(density_final_kernel, dim3(grid_for(static_cast<int64_t>(nt) * P2, 256)), dim3(256), 0, st, P2,
+                       nchunk, nt, part, S);
+    return hipGetLastError();
+}
+
+template <typename Real>
 hipError_t launch_reduce(int L, int64_t B, int R, int nt, int t0, const Real* rec, double* part, int nchunk,
                          double* acc, hipStream_t st)
 {
@@ -516,12 +561,16 @@ template hipError_t launch_sample<float>(const DevSystem<float>&, const double*,
                                          int64_t, int64_t, float*, hipStream_t);
 template hipError_t launch_rotate<float>(int, const double*, int64_t, float*, float*, hipStream_t);
 template hipError_t launch_reduce<float>(int, int64_t, int, int, int, const float*, double*, int, double*, hipStream_t);
+template hipError_t launch_density_reduce<float>(int, int64_t, int, int, const float*, double*, int, double*,
+                                                 hipStream_t);
 #else
 template hipError_t launch_sample<double>(const DevSystem<double>&, const double*, const double*, const double*,
                                           uint64_t, int64_t, int64_t, double*, hipStream_t);
 template hipError_t launch_rotate<double>(int, const double*, int64_t, double*, double*, hipStream_t);
 template hipError_t launch_reduce<double>(int, int64_t, int, int, int, const double*, double*, int, double*,
                                           hipStream_t);
+template hipError_t launch_density_reduce<double>(int, int64_t, int, int, const double*, double*, int, double*,
+                                                  hipStream_t);
 #endif
 
 }  // namespace fpsamd

@@ -4,6 +4,8 @@
 //   state  [B][2 spins][L][L] complex<Real>, row-major, interleaved (re, im)
 //   rec    [Tc][B][R] Real, R = 2L + 8 per-trajectory records of one time point:
 //            [0,L) rho_up_ii  [L,2L) rho_dn_ii  2L+0..3 E_tH, E_U, E_V, E_pe  2L+4..7 corner re/im up, dn
+//            and, when basis occupations are requested, R = 2L + 8 + L(L+1): [2L+8, R) Re rho_sigma(i, j),
+//            j >= i, spin-major, rows packed (dev::packed_row)
 //   part   [Tc][nchunk][Q][2] double   partial ensemble sums (sum, compensation)
 //   acc    [T][Q][2] double            ensemble sums (sum, compensation), Q = 6L + 12:
 //            per site i: S_u, S_d, S_uu, S_dd, S_ud, S_(nu nd)^2 at [6i .. 6i+5];
@@ -57,6 +59,7 @@ struct RunShape {
     int nsub = 1;
     double h = 0.0;       // dt / nsub
     int R = 0;            // record width
+    int dens = 0;         // 1: records also carry the packed real density matrices (R = 2L + 8 + L(L+1))
 };
 
 // Sampling: rho_sigma = diag(n - 1/2) + S.W (rotation applied separately).  noise (optional,
@@ -141,5 +144,18 @@ template <typename Real>
 hipError_t launch_reduce(int L, int64_t B, int R, int nt, int t0, const Real* rec, double* part, int nchunk,
                          double* acc, hipStream_t st);
 int reduce_chunks(int64_t B);
+
+// Ensemble sums S[t][L(L+1)] (FP64) of the records' density part for nt time points (records with
+// RunShape::dens); part holds nt * nchunk * L(L+1) * 2 doubles.
+template <typename Real>
+hipError_t launch_density_reduce(int L, int64_t B, int R, int nt, const Real* rec, double* part, int nchunk,
+                                 double* S, hipStream_t st);
+
+// Occupations of nb fixed bases, M[b][sg][k] = sum_ij V_ik V_jk S_sg(i, j) = diag(V^T S V) with V =
+// bases[b][sg] (real [L][L], columns the basis states), for nt time points of density sums S (as
+// above), written to out[((b * 2 + sg) * L + k) * T + t0 + t].  work holds basis_work_doubles(L, nt).
+hipError_t basis_project(DenseCtx* c, int L, int nt, int t0, int T, int nb, const double* bases, const double* S,
+                         double* work, double* out);
+size_t basis_work_doubles(int L, int nt);
 
 }  // namespace fpsamd

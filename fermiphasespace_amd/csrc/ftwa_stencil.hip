@@ -303,6 +303,8 @@ __global__ void __launch_bounds__(64 * W)
                 rr[R2 + 6] = cd.x;
                 rr[R2 + 7] = cd.y;
             }
+            if (r.dens)
+                dev::record_density(rr + R2 + 8, L, j, w, NT / 64, [&](int row, int col) { return Y[row * kRow + col].x; });
             Real eth = 0, eu = 0, ev = 0, epe = 0;
             if (tid < L) {
                 const int i = tid;

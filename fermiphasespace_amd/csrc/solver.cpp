@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/fps_ftwa_solver.hpp"
+#include "eigen.hpp"
 #include "ftwa_internal.hpp"
 
 using fpsjson::Value;
@@ -67,7 +68,8 @@ FHfTWASolver::FHfTWASolver(FHfTWASolver&& o) noexcept
     : sys_(o.sys_), h_(o.h_), json_(std::move(o.json_)), sim_(std::move(o.sim_)), info_(std::move(o.info_)),
       workName_(std::move(o.workName_)), folderName_(std::move(o.folderName_)), writeJson_(o.writeJson_),
       talkative_(o.talkative_), initiated_(o.initiated_), tMax_(o.tMax_), dt_(o.dt_), nSub_(o.nSub_), nTraj_(o.nTraj_),
-      flags_(o.flags_), seed_(o.seed_), engine_(o.engine_)
+      flags_(o.flags_), seed_(o.seed_), engine_(o.engine_), saveTB_(o.saveTB_), saveMF_(o.saveMF_),
+      basisTB_(o.basisTB_), basisMF_(o.basisMF_)
 {
     o.h_ = nullptr;
 }
@@ -129,8 +131,8 @@ void FHfTWASolver::setResultsToSave(const Value& res)
     if (res.get("onSiteDoublons", Value(false)).asBool()) f |= FPSFTWA_RES_DOUBLONS;
     if (res.get("wholeDMatrix", Value(false)).asBool()) f |= FPSFTWA_RES_WHOLE;
     if (res.get("energies", Value(true)).asBool()) f |= FPSFTWA_RES_ENERGIES;
-    if (res.get("tbStates", Value(false)).asBool() || res.get("mfStates", Value(false)).asBool())
-        throw std::runtime_error("tbStates / mfStates (momentum-space occupations) are not supported by this build");
+    saveTB_ = res.get("tbStates", Value(false)).asBool();
+    saveMF_ = res.get("mfStates", Value(false)).asBool();
     flags_ = f;
 }
 
@@ -174,7 +176,48 @@ void FHfTWASolver::initiateSolver()
         d.displacement = s.displacement.data();
         check(fpsftwa_create(&d, &h_), "fpsftwa_create");
     }
-    check(fpsftwa_set_initial_state(h_, s.n_up.data(), s.n_dn.data(), nullptr, nullptr), "initial state");
+    check(fpsftwa_set_initial_state(h_, s.n_up.data(), s.n_dn.data(), s.rot_up.empty() ? nullptr : s.rot_up.data(),
+                                    s.rot_dn.empty() ? nullptr : s.rot_dn.data()),
+          "initial state");
+    // bases of the "tbStates" / "mfStates" observables: the eigenbases the system already holds
+    // (eigenbasis initial conditions), else computed here as initiateSolver does
+    // (src/FHfTWASolver.cpp:80-87) for the system's particle numbers
+    const size_t LL = (size_t)s.L * s.L;
+    std::vector<double> bases;
+    basisTB_ = basisMF_ = -1;
+    auto add_basis = [&](bool mean_field) {
+        std::vector<double> ru, rd;
+        if (!mean_field && !s.tb_rot.empty()) {
+            ru = rd = s.tb_rot;
+        } else if (mean_field && !s.mf_rot_up.empty()) {
+            ru = s.mf_rot_up;
+            rd = s.mf_rot_dn;
+        } else {
+            if (s.part_up < 0 || s.part_dn < 0 || s.part_up > s.L || s.part_dn > s.L)
+                throw std::runtime_error("particle numbers outside [0, L] for the ground-state bases");
+            ru.assign(LL, 0.0);
+            rd.assign(LL, 0.0);
+            int it = 0;
+            double res = 0.0;
+            if (fpsamd::ground_state_rotation(s.L, s.tM.data(), s.U.data(), s.part_up, s.part_dn, mean_field, ru.data(),
+                                              rd.data(), &it, &res) != 0)
+                throw std::runtime_error("eigen-decomposition of the hopping matrix did not converge");
+            if (mean_field && it >= 100)
+                std::cout << "!! the MF ground state computation has not converged. Residue=" << res << std::endl;
+        }
+        bases.insert(bases.end(), ru.begin(), ru.end());
+        bases.insert(bases.end(), rd.begin(), rd.end());
+    };
+    int nb = 0;
+    if (saveTB_) {
+        add_basis(false);
+        basisTB_ = nb++;
+    }
+    if (saveMF_) {
+        add_basis(true);
+        basisMF_ = nb++;
+    }
+    check(fpsftwa_set_bases(h_, nb, nb ? bases.data() : nullptr), "bases");
     initiated_ = true;
 }
 
@@ -210,6 +253,22 @@ void FHfTWASolver::runSimulation()
         sim["N2iid"] = rowsOf(fetch(h_, FPSFTWA_OUT_N2IID), L, T);
     }
     if (flags_ & FPSFTWA_RES_DOUBLONS) sim["NuNd"] = rowsOf(fetch(h_, FPSFTWA_OUT_NUND), L, T);
+    if (basisTB_ >= 0 || basisMF_ >= 0) {  // src/FHSolver.cpp:242-265
+        const std::vector<double> m = fetch(h_, FPSFTWA_OUT_BASIS);
+        auto block = [&](int b, int sg) {
+            std::vector<double> v(m.begin() + ((size_t)(b * 2 + sg) * L) * T,
+                                  m.begin() + ((size_t)(b * 2 + sg + 1) * L) * T);
+            return rowsOf(v, L, T);
+        };
+        if (basisTB_ >= 0) {
+            sim["Mkku"] = block(basisTB_, 0);
+            sim["Mkkd"] = block(basisTB_, 1);
+        }
+        if (basisMF_ >= 0) {
+            sim["Mukku"] = block(basisMF_, 0);
+            sim["Mukkd"] = block(basisMF_, 1);
+        }
+    }
     if (flags_ & FPSFTWA_RES_ENERGIES) {
         std::vector<double> e = fetch(h_, FPSFTWA_OUT_ENERGY);
         Value E = Value::object();
@@ -353,5 +412,67 @@ extern "C" int fpsftwa_load_system_json(const char* path, int32_t* n_sites, int3
     if (n_up) std::memcpy(n_up, s.n_up.data(), s.L * sizeof(double));
     if (n_dn) std::memcpy(n_dn, s.n_dn.data(), s.L * sizeof(double));
     if (positions) std::memcpy(positions, s.positions.data(), 3 * (size_t)s.L * sizeof(double));
+    return FPSFTWA_OK;
+}
+
+extern "C" int fpsftwa_load_initial_state_json(const char* path, int32_t* n_sites, double* n_up, double* n_dn,
+                                               double* rot_up, double* rot_dn)
+{
+    if (!path || !n_sites) {
+        fpsamd::set_last_error("null argument");
+        return FPSFTWA_EINVAL;
+    }
+    Value root;
+    std::string err;
+    if (!fpsjson::parseFile(path, root, &err)) {
+        fpsamd::set_last_error(err);
+        return FPSFTWA_EIO;
+    }
+    std::string folder, file;
+    splitPath(path, folder, file);
+    fpsamd::FtwaSystem s;
+    if (!fpsamd::buildSystem(root["system"], folder, s, &err)) {
+        fpsamd::set_last_error(err);
+        return FPSFTWA_EINVAL;
+    }
+    *n_sites = s.L;
+    const size_t LL = (size_t)s.L * s.L;
+    if (n_up) std::memcpy(n_up, s.n_up.data(), s.L * sizeof(double));
+    if (n_dn) std::memcpy(n_dn, s.n_dn.data(), s.L * sizeof(double));
+    for (int spin = 0; spin < 2; ++spin) {
+        double* out = spin == 0 ? rot_up : rot_dn;
+        const std::vector<double>& r = spin == 0 ? s.rot_up : s.rot_dn;
+        if (!out) continue;
+        if (!r.empty()) {
+            std::memcpy(out, r.data(), 2 * LL * sizeof(double));
+        } else {
+            std::memset(out, 0, 2 * LL * sizeof(double));
+            for (int i = 0; i < s.L; ++i) out[2 * ((size_t)i * s.L + i)] = 1.0;
+        }
+    }
+    return FPSFTWA_OK;
+}
+
+extern "C" int fpsftwa_ground_state(int32_t n_sites, const double* hopping, const double* coulomb_u, int32_t n_up,
+                                    int32_t n_dn, int32_t mean_field, double* rot_up, double* rot_dn,
+                                    int32_t* iterations, double* residue)
+{
+    if (n_sites <= 0 || !hopping || !rot_up || !rot_dn || (mean_field && !coulomb_u)) {
+        fpsamd::set_last_error("fpsftwa_ground_state: null argument or n_sites <= 0");
+        return FPSFTWA_EINVAL;
+    }
+    if (n_up < 0 || n_dn < 0 || n_up > n_sites || n_dn > n_sites) {
+        fpsamd::set_last_error("fpsftwa_ground_state: particle numbers must lie in [0, n_sites]");
+        return FPSFTWA_EINVAL;
+    }
+    int it = 0;
+    double res = 0.0;
+    if (fpsamd::ground_state_rotation(n_sites, hopping, coulomb_u, n_up, n_dn, mean_field != 0, rot_up, rot_dn, &it,
+                                      &res) != 0) {
+        fpsamd::set_last_error("fpsftwa_ground_state: eigen-decomposition did not converge");
+        return FPSFTWA_EINVAL;
+    }
+    if (iterations) *iterations = it;
+    if (residue) *residue = res;
     return FPSFTWA_OK;
 }

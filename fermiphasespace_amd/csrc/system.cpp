@@ -1,6 +1,8 @@
 // system.cpp -- JSON -> fTWA system (see system.hpp for the reference functions restated).
 #include "system.hpp"
 
+#include "eigen.hpp"
+
 #include <cmath>
 #include <fstream>
 #include <sstream>
@@ -78,6 +80,31 @@ double siteDistance(const FtwaSystem& s, int i, int j, double* disp3)
     }
     if (disp3) { disp3[0] = best[0]; disp3[1] = best[1]; disp3[2] = best[2]; }
     return dmin;
+}
+
+bool computeGroundState(FtwaSystem& s, bool mean_field, int n_up, int n_dn, std::string* err)
+{
+    const int L = s.L;
+    std::vector<double> ru((size_t)L * L), rd((size_t)L * L);
+    int it = 0;
+    double res = 0.0;
+    if (ground_state_rotation(L, s.tM.data(), s.U.empty() ? nullptr : s.U.data(), n_up, n_dn, mean_field, ru.data(),
+                              rd.data(), &it, &res) != 0) {
+        if (err) *err = "eigen-decomposition of the hopping matrix did not converge";
+        return false;
+    }
+    if (!mean_field) {
+        s.tb_rot = std::move(ru);
+        return true;
+    }
+    s.mf_rot_up = std::move(ru);
+    s.mf_rot_dn = std::move(rd);
+    s.mf_iterations = it;
+    s.mf_residue = res;
+    if (it >= 100)
+        s.warnings.push_back("the mean-field ground state did not converge in 100 iterations (residue " +
+                             std::to_string(res) + "); its last iterate is used, as in the reference");
+    return true;
 }
 
 bool buildSystem(const fpsjson::Value& sys, const std::string& folder, FtwaSystem& out, std::string* err)
@@ -223,6 +250,12 @@ bool buildSystem(const fpsjson::Value& sys, const std::string& folder, FtwaSyste
         if (type == "givenDiagonal") {
             std::string sub = ic.get("subtype", Value("spinWave")).asString();
             s.ic_type += "/" + sub;
+            // particle numbers: nPart for spinWave / chargeWave (:536-545), the occupied counts for
+            // spinWave3 / chargeWave3 (:196-227), none set for givenInJson / givenInFile
+            if (sub == "spinWave" || sub == "chargeWave") {
+                s.part_up = (int)std::min<int64_t>(nUp, 1 << 30);
+                s.part_dn = (int)std::min<int64_t>(nDn, 1 << 30);
+            }
             if (sub == "spinWave") {
                 for (int64_t i = 0; i < std::min<int64_t>(L, 2 * nUp); i += 2) s.n_up[i] = 1.0;
                 for (int64_t i = 1; i < std::min<int64_t>(L, 2 * nDn); i += 2) s.n_dn[i] = 1.0;
@@ -231,6 +264,8 @@ bool buildSystem(const fpsjson::Value& sys, const std::string& folder, FtwaSyste
                 for (int64_t i = 0; i < std::min<int64_t>(L, 2 * nDn); i += 2) s.n_dn[i] = 1.0;
             } else if (sub == "spinWave3" || sub == "chargeWave3") {
                 int mid = L / 2;  // std::ceil(getnSites()/2) on an integer quotient
+                s.part_up = mid;
+                s.part_dn = sub == "spinWave3" ? L - mid : mid;
                 for (int i = 0; i < mid; ++i) s.n_up[i] = 1.0;
                 if (sub == "spinWave3")
                     for (int i = mid; i < L; ++i) s.n_dn[i] = 1.0;
@@ -246,8 +281,31 @@ bool buildSystem(const fpsjson::Value& sys, const std::string& folder, FtwaSyste
             } else if (sub == "givenInFile") {
                 s.warnings.push_back("IC subtype givenInFile reads no file (as in the reference): empty state");
             }
-        } else if (type == "tbGroundState" || type == "mfGroundState" || type == "groundState") {
-            if (err) *err = "initial condition '" + type + "' (eigenbasis rotation) is not supported by this build";
+        } else if (type == "tbGroundState" || type == "mfGroundState") {
+            // setInitialConditions(type, nPart) (src/FHSystem.cpp:536-595) keeps nPartUp/nPartDown and
+            // computes the eigenbasis; initiateSolver (src/FHfTWASolver.cpp:120-153) occupies the lowest
+            // nPart states and rotates the sampled state with the eigenvector matrix.
+            if (nUp < 0 || nDn < 0 || nUp > L || nDn > L) {
+                if (err) *err = "initial condition '" + type + "': nPartUp/nPartDown must lie in [0, " + std::to_string(L) + "]";
+                return false;
+            }
+            s.part_up = (int)nUp;
+            s.part_dn = (int)nDn;
+            if (!computeGroundState(s, type == "mfGroundState", (int)nUp, (int)nDn, err)) return false;
+            const std::vector<double>& ru = type == "mfGroundState" ? s.mf_rot_up : s.tb_rot;
+            const std::vector<double>& rd = type == "mfGroundState" ? s.mf_rot_dn : s.tb_rot;
+            for (int64_t i = 0; i < nUp; ++i) s.n_up[i] = 1.0;
+            for (int64_t i = 0; i < nDn; ++i) s.n_dn[i] = 1.0;
+            s.rot_up.assign(2 * (size_t)L * L, 0.0);
+            s.rot_dn.assign(2 * (size_t)L * L, 0.0);
+            for (size_t k = 0; k < (size_t)L * L; ++k) {
+                s.rot_up[2 * k] = ru[k];
+                s.rot_dn[2 * k] = rd[k];
+            }
+        } else if (type == "groundState") {
+            // the reference computes the mean-field state for this type but initiateSolver has no
+            // branch for it (src/FHfTWASolver.cpp:99-153), leaving the rotation matrices empty
+            if (err) *err = "initial condition 'groundState' has no fTWA initial state (use tbGroundState or mfGroundState)";
             return false;
         } else {
             s.warnings.push_back("initial condition type '" + type + "' sets no occupation");

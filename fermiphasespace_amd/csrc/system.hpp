@@ -31,6 +31,12 @@ struct FtwaSystem {
     std::vector<double> pulse_knots;
     std::vector<double> displacement;            // [L][L][3] minimum-image r_i - r_j
     std::vector<double> n_up, n_dn;              // [L] initial occupations
+    std::vector<double> rot_up, rot_dn;          // [L][L][re,im] initial-state rotations; empty = identity
+    std::vector<double> tb_rot;                  // [L][L] tight-binding eigenvectors (columns), when computed
+    std::vector<double> mf_rot_up, mf_rot_dn;    // [L][L] mean-field eigenvectors per spin, when computed
+    int part_up = 0, part_dn = 0;                // upSpinParticleNumber / downSpinParticleNumber (ground-state bases)
+    int mf_iterations = 0;                       // self-consistency iterations of the mean-field state
+    double mf_residue = 0;                       // its final residue
     std::string ic_type;                         // as read (for messages)
     std::vector<std::string> warnings;           // reference behaviours the build reproduces or deviates from
 };
@@ -39,6 +45,10 @@ struct FtwaSystem {
 // the simulation file (with trailing '/'), prefixed to positionsFile as the reference does.
 // Returns false with err set on unreadable input or an unsupported request.
 bool buildSystem(const fpsjson::Value& systemJson, const std::string& folder, FtwaSystem& out, std::string* err);
+
+// The tight-binding (mean_field false: src/FHSystem.cpp:909-937) or mean-field (true: :940-1019)
+// eigenbasis of the built system for n_up / n_dn particles, into tb_rot or mf_rot_up/mf_rot_dn.
+bool computeGroundState(FtwaSystem& s, bool mean_field, int n_up, int n_dn, std::string* err);
 
 // Minimum-image distance between two sites (src/FHLattice.cpp:516-539).
 double siteDistance(const FtwaSystem& s, int i, int j, double* disp3);

@@ -25,6 +25,8 @@ class FHSystem:
     pot: np.ndarray | None = None
     n_up: np.ndarray | None = None
     n_dn: np.ndarray | None = None
+    rot_up: np.ndarray | None = None  # [L][L] initial-state rotations (tbGroundState / mfGroundState); None = identity
+    rot_dn: np.ndarray | None = None
     pulse: dict = field(default_factory=dict)  # keys of Engine(...): pulse_type, A0, t0, td, w0, phi, pol, knots, disp
 
     @property
@@ -35,12 +37,40 @@ class FHSystem:
     def from_json_file(cls, path: str) -> "FHSystem":
         """Build from a simulation file's "system" object with the engine's JSON front end."""
         d = _lib.load_system_json(path)
+        ic = _lib.load_initial_state_json(path)
+        eye = np.eye(d["L"])
+        rot = ic["rot_up"] if not (np.allclose(ic["rot_up"], eye) and np.allclose(ic["rot_dn"], eye)) else None
         pot = d["pot"] if np.any(d["pot"]) else None
         return cls(tM=d["tM"], U=d["U"], V=d["V"], coulomb_type=d["coulomb_type"], pot=pot, n_up=d["n_up"],
-                   n_dn=d["n_dn"])
+                   n_dn=d["n_dn"], rot_up=rot, rot_dn=None if rot is None else ic["rot_dn"])
 
     def getnSites(self) -> int:
         return self.L
+
+    def computeTBGroundState(self) -> np.ndarray:
+        """Eigenvectors of tM (src/FHSystem.cpp:909-937), columns in ascending eigenvalue order."""
+        return _lib.ground_state(self.tM)["rot_up"]
+
+    def computeMFGroundState(self, n_up: int | None = None, n_dn: int | None = None):
+        """Self-consistent Hartree eigenvectors per spin (src/FHSystem.cpp:940-1019)."""
+        g = _lib.ground_state(self.tM, self.U, n_up, n_dn, mean_field=True)
+        return g["rot_up"], g["rot_dn"]
+
+    def setGroundStateInitialConditions(self, kind: str, n_up: int | None = None, n_dn: int | None = None) -> None:
+        """tbGroundState / mfGroundState initial state: the lowest n_up / n_dn eigenstates occupied and the
+        sampled state rotated into that basis (src/FHSystem.cpp:536-595, src/FHfTWASolver.cpp:120-153)."""
+        L = self.L
+        n_up = int(np.ceil(L / 2)) if n_up is None else int(n_up)
+        n_dn = int(np.floor(L / 2)) if n_dn is None else int(n_dn)
+        if kind == "tbGroundState":
+            ru = rd = self.computeTBGroundState()
+        elif kind == "mfGroundState":
+            ru, rd = self.computeMFGroundState(n_up, n_dn)
+        else:
+            raise ValueError(f"unknown eigenbasis initial condition {kind!r}")
+        self.n_up = (np.arange(L) < n_up).astype(np.float64)
+        self.n_dn = (np.arange(L) < n_dn).astype(np.float64)
+        self.rot_up, self.rot_dn = ru.astype(np.complex128), rd.astype(np.complex128)
 
 
 class FHfTWASolver:
@@ -58,6 +88,8 @@ class FHfTWASolver:
         self.jsonFile: dict = {}
         self._engine: _lib.Engine | None = None
         self.results: dict = {}
+        self.saveTB = self.saveMF = False   # "tbStates" / "mfStates" (src/FHSolver.cpp:41-42)
+        self._basis_index: dict = {}
 
     # ---- setters (src/FHfTWASolver.cpp:20-69, src/FHSolver.cpp:11-112) ----
     def setIntegratorOrder(self, order: int) -> None:
@@ -89,8 +121,8 @@ class FHfTWASolver:
             f |= _lib.RES_WHOLE
         if res.get("energies", True):
             f |= _lib.RES_ENERGIES
-        if res.get("tbStates", False) or res.get("mfStates", False):
-            raise _lib.FtwaError(_lib.EUNSUPPORTED, "tbStates / mfStates are not supported by this build")
+        self.saveTB = bool(res.get("tbStates", False))
+        self.saveMF = bool(res.get("mfStates", False))
         self.flags = f
 
     def setResultFile(self, workName: str, folderName: str) -> None:
@@ -111,7 +143,20 @@ class FHfTWASolver:
         L = s.L
         nu = np.zeros(L) if s.n_up is None else s.n_up
         nd = np.zeros(L) if s.n_dn is None else s.n_dn
-        self._engine.set_initial_state(nu, nd)
+        self._engine.set_initial_state(nu, nd, s.rot_up, s.rot_dn)
+        # bases of the tbStates / mfStates observables (src/FHfTWASolver.cpp:80-87), for the system's
+        # particle numbers (the occupied counts of the initial state)
+        bases, self._basis_index = [], {}
+        n_up, n_dn = int(round(float(np.sum(nu)))), int(round(float(np.sum(nd))))
+        if self.saveTB:
+            V = s.computeTBGroundState()
+            self._basis_index["tb"] = len(bases)
+            bases.append(np.stack([V, V]))
+        if self.saveMF:
+            Vu, Vd = s.computeMFGroundState(n_up, n_dn)
+            self._basis_index["mf"] = len(bases)
+            bases.append(np.stack([Vu, Vd]))
+        self._engine.set_bases(np.stack(bases) if bases else None)
 
     @property
     def engine_handle(self) -> _lib.Engine:
@@ -138,6 +183,12 @@ class FHfTWASolver:
             sim["N2iid"] = e.result("N2iid").tolist()
         if f & _lib.RES_DOUBLONS:
             sim["NuNd"] = e.result("NuNd").tolist()
+        if self._basis_index:  # src/FHSolver.cpp:242-265
+            M = e.result("basis")
+            for kind, (ku, kd) in (("tb", ("Mkku", "Mkkd")), ("mf", ("Mukku", "Mukkd"))):
+                if kind in self._basis_index:
+                    b = self._basis_index[kind]
+                    sim[ku], sim[kd] = M[b, 0].tolist(), M[b, 1].tolist()
         if f & _lib.RES_ENERGIES:
             E = e.result("E")
             sim["E"] = {k: E[i].tolist() for i, k in enumerate(("tH", "U", "V", "pe", "tot"))}

@@ -83,7 +83,10 @@ extern "C" {
 #define FPSFTWA_OUT_SUM_RHO2_D 16  /* [L][T] sum_b |rho_dn_ii|^2                    */
 #define FPSFTWA_OUT_SUM_RHOUD 17   /* [L][T] sum_b rho_up_ii rho_dn_ii               */
 #define FPSFTWA_OUT_SUM_ENERGY 18  /* [4][T] sum_b E_k                               */
-#define FPSFTWA_OUT_COUNT 19
+/* occupations of the bases given to fpsftwa_set_bases (tbStates -> Mkku/Mkkd, mfStates -> Mukku/Mukkd) */
+#define FPSFTWA_OUT_BASIS 19       /* [n_bases][2][L][T] sum_b diag(V^T rho V) / B + 1/2 (src/FHfTWASolver.cpp:531-540) */
+#define FPSFTWA_OUT_SUM_BASIS 20   /* [n_bases][2][L][T] sum_b diag(V^T rho V)                                     */
+#define FPSFTWA_OUT_COUNT 21
 
 /* System matrices, as FHSystem hands them to the solver through getUMatrix / gettMatrix /
  * getVMatrix / getExtPot (src/FHSystem.cpp:387-391).  All values are real: the reference's
@@ -134,6 +137,13 @@ int fpsftwa_set_results(fpsftwa_handle* h, uint32_t flags);
  * ([L][L] complex interleaved; NULL = identity, the givenDiagonal/spinWave/chargeWave case). */
 int fpsftwa_set_initial_state(fpsftwa_handle* h, const double* n_up, const double* n_dn,
                               const double* rot_up, const double* rot_dn);
+
+/* Fixed one-particle bases whose occupations diag(V^T rho V) are recorded at every time point, as
+ * runSimulation does for "tbStates" (tbgs_V) and "mfStates" (mfgs_Vu, mfgs_Vd)
+ * (src/FHfTWASolver.cpp:385-394, :531-540).  bases = [n_bases][2 spins][L][L] real, row-major, columns
+ * the basis states (fpsftwa_ground_state's layout); n_bases in [0, 2], 0 (or NULL) clears them.  Read
+ * the result with FPSFTWA_OUT_BASIS.  Not available with the adjoint. */
+int fpsftwa_set_bases(fpsftwa_handle* h, int32_t n_bases, const double* bases);
 
 /* Oracle mode: use the caller's noise instead of the counter-based generator.
  * noise = [n_traj][2 spins][L(L-1)/2 pairs][re, im], pair p enumerating i<j row by row;
@@ -221,6 +231,22 @@ int fpsftwa_run_json_file(const char* path, const char* out_path);
 int fpsftwa_load_system_json(const char* path, int32_t* n_sites, int32_t* coulomb_type,
                              double* hopping, double* coulomb_u, double* coulomb_v, double* potential,
                              double* n_up, double* n_dn, double* positions);
+
+/* The initial state the JSON front end builds: occupations n_up[L], n_dn[L] and the rotations
+ * R_up, R_dn ([L][L] complex interleaved; the identity unless IC type is tbGroundState or
+ * mfGroundState), as initiateSolver sets them (src/FHfTWASolver.cpp:99-153). */
+int fpsftwa_load_initial_state_json(const char* path, int32_t* n_sites, double* n_up, double* n_dn,
+                                    double* rot_up, double* rot_dn);
+
+/* Eigenbasis of FHSystem::computeTBGroundState (mean_field 0, src/FHSystem.cpp:909-937: eigenvectors
+ * of the real symmetric hopping[L][L], rot_dn = rot_up) or computeMFGroundState (mean_field 1,
+ * :940-1019: eigenvectors of hopping + diag(U n_other) per spin, iterated by plain substitution until
+ * the density residue < 1e-4 L, at most 100 iterations).  rot_* are real [L][L] row-major, columns
+ * the eigenvectors in ascending eigenvalue order, each column's largest-magnitude entry positive.
+ * Host-only (no GPU call). */
+int fpsftwa_ground_state(int32_t n_sites, const double* hopping, const double* coulomb_u, int32_t n_up,
+                         int32_t n_dn, int32_t mean_field, double* rot_up, double* rot_dn,
+                         int32_t* iterations, double* residue);
 
 #ifdef __cplusplus
 }

@@ -80,4 +80,6 @@ private:
     uint32_t flags_ = FPSFTWA_RES_DIAG | FPSFTWA_RES_DIAG2 | FPSFTWA_RES_ENERGIES;  // include/FHSolver.h:158-164
     uint64_t seed_ = 0;
     int engine_ = FPSFTWA_ENGINE_FP64_BATCHED;
+    bool saveTB_ = false, saveMF_ = false;  // "tbStates" / "mfStates" (src/FHSolver.cpp:41-42)
+    int basisTB_ = -1, basisMF_ = -1;       // their index among the engine's bases
 };

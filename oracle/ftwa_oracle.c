@@ -306,7 +306,7 @@ static void sample_state(int L, const double* nocc, const double* R, const doubl
 
 /* Per time point accumulation (src/FHfTWASolver.cpp:366-418), real parts. */
 static void record(const or_system* s, double t, double t_max, const double* ru, const double* rd, int64_t ti,
-                   int64_t T, double* acc, work* w)
+                   int64_t T, double* acc, work* w, int nb, const double* bases)
 {
     const int L = s->L;
     double* su = acc;                      /* [L][T] */
@@ -370,6 +370,28 @@ static void record(const or_system* s, double t, double t_max, const double* ru,
     sc[1 * T + ti] += ru[2 * (L - 1) + 1];
     sc[2 * T + ti] += rd[2 * (L - 1)];
     sc[3 * T + ti] += rd[2 * (L - 1) + 1];
+    /* diagvec(V^dagger rho V), real part (:385-394) */
+    double* sb = sc + (size_t)4 * T; /* [nb][2][L][T] */
+    for (int b = 0; b < nb; ++b)
+        for (int sg = 0; sg < 2; ++sg) {
+            const double* V = bases + (size_t)(b * 2 + sg) * L * L * 2;
+            const double* rho = sg ? rd : ru;
+            for (int k = 0; k < L; ++k) {
+                double m = 0.0;
+                for (int i = 0; i < L; ++i) {
+                    double xr = 0.0, xi = 0.0; /* x = (rho V)_ik */
+                    for (int j = 0; j < L; ++j) {
+                        const double pr = rho[2 * (i * L + j)], pi = rho[2 * (i * L + j) + 1];
+                        const double vr = V[2 * (j * L + k)], vi = V[2 * (j * L + k) + 1];
+                        xr += pr * vr - pi * vi;
+                        xi += pr * vi + pi * vr;
+                    }
+                    const double cr = V[2 * (i * L + k)], ci = -V[2 * (i * L + k) + 1]; /* conj(V_ik) */
+                    m += cr * xr - ci * xi;
+                }
+                sb[((size_t)(b * 2 + sg) * L + k) * T + ti] += m;
+            }
+        }
 }
 
 static int run_impl(const or_system* s, const or_params* p, const double* noise, int use_seed, uint64_t seed,
@@ -379,7 +401,8 @@ static int run_impl(const or_system* s, const or_params* p, const double* noise,
     if (L <= 0 || p->n_traj <= 0 || p->dt <= 0.0 || p->n_sub <= 0) return -1;
     const int64_t T = oracle_n_time_points(p->dt, p->t_max);
     const int64_t npairs = (int64_t)L * (L - 1) / 2;
-    const size_t acc_n = (size_t)(5 * L + 8) * T;
+    const int nb = p->bases ? p->n_bases : 0;
+    const size_t acc_n = (size_t)(5 * L + 8) * T + (size_t)nb * 2 * L * T;
     const double h = p->dt / (double)p->n_sub;
     int nth = 1;
 #ifdef _OPENMP
@@ -422,7 +445,7 @@ static int run_impl(const or_system* s, const or_params* p, const double* noise,
             sample_state(L, p->n_dn, p->rot_dn, wb + 2 * npairs, 1, rd, tmp);
             double t = 0.0;
             for (int64_t ti = 0; ti < T; ++ti) {
-                record(s, ti * p->dt, p->t_max, ru, rd, ti, T, acc, &w);
+                record(s, ti * p->dt, p->t_max, ru, rd, ti, T, acc, &w, nb, p->bases);
                 if (ti == T - 1) break; /* steps after the last record change no output */
                 for (int ss = 0; ss < p->n_sub; ++ss) {
                     step_rk4_38(s, t, p->t_max, h, ru, rd, &w);
@@ -454,6 +477,7 @@ static int run_impl(const or_system* s, const or_params* p, const double* noise,
     if (o->sum_ud) memcpy(o->sum_ud, accs + 4 * LT, sizeof(double) * LT);
     if (o->sum_e) memcpy(o->sum_e, accs + 5 * LT, sizeof(double) * 4 * T);
     if (o->sum_corner) memcpy(o->sum_corner, accs + 5 * LT + 4 * T, sizeof(double) * 4 * T);
+    if (o->sum_basis && nb) memcpy(o->sum_basis, accs + 5 * LT + 8 * T, sizeof(double) * nb * 2 * LT);
     free(accs);
     return 0;
 }

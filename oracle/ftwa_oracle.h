@@ -11,7 +11,8 @@
  *   oracle_noise          the complex white noise of src/FHfTWASolver.cpp:343-346, drawn from a
  *                         counter-based stream instead of Armadillo's randomly seeded one
  *   oracle_run            FHfTWASolver::initiateSolver (src/FHfTWASolver.cpp:99-118, :187-198),
- *                         runSimulation (:222-556), derivative (:568-600), step_RK4_38 (:629-661)
+ *                         runSimulation (:222-556; tbStates / mfStates occupations :385-394),
+ *                         derivative (:568-600), step_RK4_38 (:629-661)
  */
 #ifndef FTWA_ORACLE_H
 #define FTWA_ORACLE_H
@@ -44,6 +45,8 @@ typedef struct or_params {
     const double* n_dn;   /* [L] */
     const double* rot_up; /* [L][L] complex interleaved or NULL (identity) */
     const double* rot_dn;
+    int n_bases;          /* bases whose occupations diag(V^dagger rho V) are summed (tbStates / mfStates) */
+    const double* bases;  /* [n_bases][2 spins][L][L] complex interleaved, columns the basis states */
 } or_params;
 
 /* Raw ensemble sums (real parts), each [rows][T], any pointer may be NULL. */
@@ -56,6 +59,7 @@ typedef struct or_out {
     double* sum_e;      /* [4][T] tH, U, V, pe             */
     double* sum_corner; /* [4][T] re/im rho_up(0,L-1), re/im rho_dn(0,L-1) */
     double* final_states; /* [B][2][L][L][2]               */
+    double* sum_basis;  /* [n_bases][2][L][T] sum Re diag(V^dagger rho V) (src/FHfTWASolver.cpp:385-394) */
 } or_out;
 
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -78,12 +78,14 @@ class _OrParams(ctypes.Structure):
         ("n_sub", ctypes.c_int), ("n_traj", ctypes.c_int64),
         ("n_up", ctypes.POINTER(ctypes.c_double)), ("n_dn", ctypes.POINTER(ctypes.c_double)),
         ("rot_up", ctypes.POINTER(ctypes.c_double)), ("rot_dn", ctypes.POINTER(ctypes.c_double)),
+        ("n_bases", ctypes.c_int), ("bases", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
 class _OrOut(ctypes.Structure):
     _fields_ = [(n, ctypes.POINTER(ctypes.c_double)) for n in
-                ("sum_u", "sum_d", "sum2_u", "sum2_d", "sum_ud", "sum_e", "sum_corner", "final_states")]
+                ("sum_u", "sum_d", "sum2_u", "sum2_d", "sum_ud", "sum_e", "sum_corner", "final_states",
+                 "sum_basis")]
 
 
 def _ptr(a):
@@ -194,12 +196,16 @@ class Sums:
     sum_e: np.ndarray
     sum_corner: np.ndarray
     final_states: np.ndarray | None
+    sum_basis: np.ndarray | None = None  # [n_bases][2][L][T]
 
 
 def run(system: System, dt: float, t_max: float, n_sub: int, n_up, n_dn, *, noise_arr=None,
         n_traj: int | None = None, seed: int | None = None, rot_up=None, rot_dn=None,
-        nthreads: int = 0, keep_states: bool = False) -> Sums:
-    """Run the oracle on explicit noise ([B][2][npairs][2]) or on the seeded generator."""
+        nthreads: int = 0, keep_states: bool = False, bases=None) -> Sums:
+    """Run the oracle on explicit noise ([B][2][npairs][2]) or on the seeded generator.
+
+    bases: optional [n_bases][2][L][L] (real or complex) basis matrices whose occupations
+    diag(V^dagger rho V) are summed into Sums.sum_basis."""
     lib = _load()
     L = system.L
     T = n_time_points(dt, t_max)
@@ -219,9 +225,12 @@ def run(system: System, dt: float, t_max: float, n_sub: int, n_up, n_dn, *, nois
         return np.ascontiguousarray(np.stack([r.real, r.imag], -1))
 
     ru, rd = cplx(rot_up), cplx(rot_dn)
+    bs = cplx(bases)
+    nb = 0 if bases is None else int(np.asarray(bases).shape[0])
     p = _OrParams()
     p.dt, p.t_max, p.n_sub, p.n_traj = dt, t_max, n_sub, B
     p.n_up, p.n_dn, p.rot_up, p.rot_dn = _ptr(nu), _ptr(nd), _ptr(ru), _ptr(rd)
+    p.n_bases, p.bases = nb, _ptr(bs)
     arrays = {n: np.zeros((L, T)) for n in ("sum_u", "sum_d", "sum2_u", "sum2_d", "sum_ud")}
     arrays["sum_e"] = np.zeros((4, T))
     arrays["sum_corner"] = np.zeros((4, T))
@@ -230,6 +239,8 @@ def run(system: System, dt: float, t_max: float, n_sub: int, n_up, n_dn, *, nois
     for k, v in arrays.items():
         setattr(o, k, _ptr(v))
     o.final_states = _ptr(fs)
+    sb = np.zeros((nb, 2, L, T)) if nb else None
+    o.sum_basis = _ptr(sb)
     s = system._c()
     if noise_arr is not None:
         rc = lib.oracle_run(ctypes.byref(s), ctypes.byref(p), _ptr(noise_arr), nthreads, ctypes.byref(o))
@@ -238,7 +249,7 @@ def run(system: System, dt: float, t_max: float, n_sub: int, n_up, n_dn, *, nois
     if rc != 0:
         raise RuntimeError(f"oracle_run failed ({rc})")
     final = None if fs is None else fs[..., 0] + 1j * fs[..., 1]
-    return Sums(T=T, final_states=final, **arrays)
+    return Sums(T=T, final_states=final, sum_basis=sb, **arrays)
 
 
 def normalise(s: Sums, B: int) -> dict:
@@ -252,4 +263,48 @@ def normalise(s: Sums, B: int) -> dict:
     }
     E = s.sum_e / B
     out["E"] = np.vstack([E, E.sum(axis=0, keepdims=True)])
+    if s.sum_basis is not None:  # Mkk = mkk / nTraj + 1/2 (src/FHfTWASolver.cpp:531-540)
+        out["M"] = s.sum_basis / B + 0.5
     return out
+
+
+def gauge_columns(V: np.ndarray) -> np.ndarray:
+    """Fix each eigenvector's sign: its largest-magnitude entry (the first one within 1e-8 on ties) positive."""
+    V = np.array(V, dtype=np.float64, copy=True)
+    for c in range(V.shape[1]):
+        big = 0
+        for k in range(1, V.shape[0]):
+            if abs(V[k, c]) > abs(V[big, c]) * (1.0 + 1e-8):
+                big = k
+        if V[big, c] < 0:
+            V[:, c] = -V[:, c]
+    return V
+
+
+def ground_state(tM, U, n_up: int, n_dn: int, mean_field: bool):
+    """Eigenbasis of the tbGroundState / mfGroundState initial states with LAPACK (numpy.linalg.eigh).
+
+    tight binding: eig_sym(tM) (src/FHSystem.cpp:909-937); mean field: per spin eig_sym(tM + diag(U n_other))
+    from zero densities, the lowest n_up / n_dn states occupied, plain substitution until
+    |n_up' - n_up| + |n_dn' - n_dn| < 1e-4 L, at most 100 iterations (src/FHSystem.cpp:940-1019).
+    Returns (rot_up, rot_dn, iterations, residue) with columns in the gauge of gauge_columns.
+    """
+    tM = np.asarray(tM, dtype=np.float64)
+    L = tM.shape[0]
+    if not mean_field:
+        V = gauge_columns(np.linalg.eigh(tM)[1])
+        return V, V.copy(), 0, 0.0
+    U = np.zeros((L, L)) if U is None else np.asarray(U, dtype=np.float64)
+    nu, nd = np.zeros(L), np.zeros(L)
+    it, res = 0, 0.0
+    while it < 100:
+        Vu = gauge_columns(np.linalg.eigh(tM + np.diag(U @ nd))[1])
+        Vd = gauge_columns(np.linalg.eigh(tM + np.diag(U @ nu))[1])
+        nu_new = (Vu[:, :n_up] ** 2).sum(axis=1)
+        nd_new = (Vd[:, :n_dn] ** 2).sum(axis=1)
+        it += 1
+        res = float(np.linalg.norm(nu_new - nu) + np.linalg.norm(nd_new - nd))
+        if res < 1e-4 * L:
+            break
+        nu, nd = nu_new, nd_new
+    return Vu, Vd, it, res

@@ -33,12 +33,14 @@ def chain(L, t=-1.0, periodic=False):
     return tM
 
 
-def engine_run(eng, dt, t_max, n_sub, B, engine, seed=0, noise=None):
+def engine_run(eng, dt, t_max, n_sub, B, engine, seed=0, noise=None, bases=False):
     eng.set_params(dt, t_max, n_sub, B, seed=seed, engine=engine)
     eng.set_noise(noise)
     eng.run()
     out = {k: eng.result(k) for k in ("sum_u", "sum_d", "sum2_u", "sum_ud", "sum_e")}
     out["states"] = eng.states()
+    if bases:
+        out["sum_basis"] = eng.result("sum_basis")
     return out
 
 
@@ -51,9 +53,11 @@ def oracle_system(O, tM, coulomb_type, U=None, V=None, pot=None, pulse=None):
 
 
 def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, pulse=None, dt=0.02, t_max=0.2,
-            n_sub=1, B=6, seed=3, engines=("fp64_plain", "fp64", "fp32"), fp32_tol=2e-5, rot=None):
+            n_sub=1, B=6, seed=3, engines=("fp64_plain", "fp64", "fp32"), fp32_tol=2e-5, rot=None, bases=None):
     L = tM.shape[0]
     eng = _lib.Engine(tM, coulomb_type=coulomb_type, U=U, V=V, pot=pot, **(pulse or {}))
+    if bases is not None:
+        eng.set_bases(bases)
     if rot is None:
         eng.set_initial_state(n_up, n_dn)
     else:
@@ -62,7 +66,7 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
     noise = eng.sample_noise(0, B)
     ref = O.run(oracle_system(O, tM, coulomb_type, U, V, pot, pulse), dt, t_max, n_sub, n_up, n_dn,
                 noise_arr=noise, keep_states=True, rot_up=None if rot is None else rot[0],
-                rot_dn=None if rot is None else rot[1])
+                rot_dn=None if rot is None else rot[1], bases=bases)
     runs = []
     z = int(((tM != 0) & ~np.eye(L, dtype=bool)).sum(axis=1).max())
     stencil = L <= 64 and z <= 8  # where the LDS stencil kernels apply
@@ -77,7 +81,7 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
             runs.append((e, "auto"))
     for e, kern in runs:
         eng.set_tuning(0, 0, kern)
-        got = engine_run(eng, dt, t_max, n_sub, B, e, seed=seed, noise=noise)
+        got = engine_run(eng, dt, t_max, n_sub, B, e, seed=seed, noise=noise, bases=bases is not None)
         tol = fp32_tol if e == "fp32" else FP64_RTOL
         print(f"L={L} {e}: states {rel(got['states'], ref.final_states):.2e}")
         assert rel(got["states"], ref.final_states) < tol, e
@@ -86,6 +90,8 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
         assert rel(got["sum2_u"], ref.sum2_u) < tol, e
         assert rel(got["sum_ud"], ref.sum_ud) < tol, e
         assert rel(got["sum_e"], ref.sum_e) < tol, e
+        if bases is not None:
+            assert rel(got["sum_basis"], ref.sum_basis) < tol, (e, kern)
     eng.close()
 
 
@@ -182,6 +188,124 @@ def test_rotated_initial_state(oracle_mod):
         Q.append(q)
     n_up = np.array([1.0, 1, 1, 0, 0, 0])
     compare(oracle_mod, chain(L), n_up, n_up, U=4.0 * np.eye(L), B=4, rot=Q, engines=("fp64_plain", "fp64"))
+
+
+@pytest.mark.parametrize("kind", ["tbGroundState", "mfGroundState"])
+def test_eigenbasis_initial_state(oracle_mod, kind):
+    """tbGroundState / mfGroundState samples rotated into the eigenbasis (src/FHfTWASolver.cpp:120-153): every
+    engine and kernel against the oracle fed LAPACK's eigenvectors in the same sign gauge, to 1e-10 (FP64)."""
+    O = oracle_mod
+    L = 8
+    tM = chain(L) + np.diag(np.linspace(-0.2, 0.3, L))
+    U = 3.0 * np.eye(L)
+    Vu, Vd, _, _ = O.ground_state(tM, U, 5, 3, kind == "mfGroundState")
+    n_up = (np.arange(L) < 5).astype(float)
+    n_dn = (np.arange(L) < 3).astype(float)
+    compare(O, tM, n_up, n_dn, U=U, B=8, rot=(Vu, Vd), t_max=0.3)
+
+
+@pytest.mark.parametrize("kind", ["tbGroundState", "mfGroundState"])
+def test_eigenbasis_initial_state_json(tmp_path, oracle_mod, kind):
+    """The JSON front end builds the eigenbasis itself: its results file matches the oracle run on LAPACK's
+    basis (same gauge) and the generator's noise, to 1e-10 relative."""
+    O = oracle_mod
+    pos, _ = lat.chain(8)
+    B, dt, t_max, seed = 16, 0.02, 0.4, 7
+    sim = lat.simulation(kind, pos, None, False, U=3.0, ic={"type": kind, "nPartUp": 5, "nPartDown": 3}, dt=dt,
+                         t_max=t_max, n_traj=B, mi355x={"seed": seed, "engine": "fp64"})
+    path = lat.write_simulation(str(tmp_path), sim, pos)
+    _lib.run_json_file(path)
+    out = json.load(open(tmp_path / f"{kind}_fTWA_res.json"))["simulation"]
+    d = _lib.load_system_json(path)
+    Vu, Vd, _, _ = O.ground_state(d["tM"], d["U"], 5, 3, kind == "mfGroundState")
+    eng = _lib.Engine(d["tM"], U=d["U"])
+    eng.set_initial_state(d["n_up"], d["n_dn"], Vu, Vd)
+    eng.set_params(dt, t_max, 1, B, seed=seed, engine="fp64")
+    noise = eng.sample_noise(0, B)
+    eng.close()
+    ref = O.normalise(O.run(oracle_system(O, d["tM"], 1, d["U"]), dt, t_max, 1, d["n_up"], d["n_dn"],
+                            noise_arr=noise, rot_up=Vu, rot_dn=Vd), B)
+    for k in ("Niiu", "Niid", "NuNd"):
+        assert rel(np.asarray(out[k]), ref[k]) < FP64_RTOL, k
+    assert rel(np.asarray(out["E"]["tot"]), ref["E"][4]) < FP64_RTOL
+
+
+def test_basis_occupations(oracle_mod):
+    """tbStates / mfStates occupations diag(V^T rho V) (src/FHfTWASolver.cpp:385-394) on every engine and
+    kernel against the oracle's per-trajectory products, to 1e-10 (FP64) / 2e-5 (FP32): two real bases,
+    different per spin, on the stencil-sized chain, the dense path and a pulse."""
+    O = oracle_mod
+    L = 8
+    tM = chain(L) + np.diag(np.linspace(-0.2, 0.3, L))
+    U = 3.0 * np.eye(L)
+    n_up = (np.arange(L) < 5).astype(float)
+    n_dn = (np.arange(L) < 3).astype(float)
+    Vt, _, _, _ = O.ground_state(tM, None, 5, 3, False)
+    Vu, Vd, _, _ = O.ground_state(tM, U, 5, 3, True)
+    bases = np.stack([np.stack([Vt, Vt]), np.stack([Vu, Vd])])
+    compare(O, tM, n_up, n_dn, U=U, B=8, rot=(Vu, Vd), t_max=0.3, bases=bases)
+    rng = np.random.default_rng(3)
+    Q = np.stack([np.stack([np.linalg.qr(rng.normal(size=(L, L)))[0] for _ in range(2)])])
+    pos, _ = lat.chain(L)
+    disp = (pos[:, None, :] - pos[None, :, :])
+    pulse = {"pulse_type": _lib.PULSE_GAUSSIAN, "A0": 0.7, "t0": 0.1, "td": 0.2, "w0": 3.0, "phi": 0.2,
+             "pol": (1.0, 0.0, 0.0), "disp": disp}
+    compare(O, tM, n_up, 1 - n_up, U=U, B=5, t_max=0.2, pulse=pulse, bases=Q)
+
+
+def test_basis_occupations_trace_at_full_size():
+    """8x8 periodic Hubbard, FP32 upper-triangle kernel with the density records: the occupations of any
+    orthonormal basis sum to the trace, so sum_k M_k(t) = sum_i N_ii(t) for both spins at every time point
+    (size-independent; 1e-9 relative, both sides from the same FP32 states summed in FP64)."""
+    pos, box = lat.grid(8, 8)
+    tM, U = lat.system_arrays(pos, box, True, t=-1.0, U=5.3)
+    n_up = np.array(lat.checkerboard(8, 8)["IC_NuDiag"], dtype=float)
+    n_dn = n_up
+    V = np.linalg.eigh(tM)[1]
+    eng = _lib.Engine(tM, U=U)
+    eng.set_initial_state(n_up, n_dn)
+    eng.set_bases(np.stack([np.stack([V, V])]))
+    eng.set_tuning(0, 0, "hermitian")
+    eng.set_params(0.005, 0.1, 1, 4096, seed=9, engine="fp32")
+    eng.run()
+    M = eng.result("sum_basis")[0]
+    for sg, k in ((0, "sum_u"), (1, "sum_d")):
+        tr = eng.result(k).sum(axis=0)
+        assert np.abs(M[sg].sum(axis=0) - tr).max() <= 1e-9 * max(1.0, np.abs(tr).max()) + 1e-9 * 4096, sg
+    eng.set_bases(None)
+    eng.run()
+    with pytest.raises(_lib.FtwaError):
+        eng.result("basis")
+    eng.close()
+
+
+def test_basis_occupations_json(tmp_path, oracle_mod):
+    """"tbStates" / "mfStates" in a simulation file: Mkku/Mkkd/Mukku/Mukkd in the results file match the
+    oracle fed LAPACK's bases in the same gauge (1e-10 relative); the mean-field basis is built for the
+    IC's particle numbers (nPartUp / nPartDown of the spin wave)."""
+    O = oracle_mod
+    pos, _ = lat.chain(8)
+    B, dt, t_max, seed = 12, 0.02, 0.3, 4
+    sim = lat.simulation("states", pos, None, False, U=2.5, ic={"type": "givenDiagonal", "subtype": "spinWave",
+                                                                 "nPartUp": 4, "nPartDown": 3},
+                         dt=dt, t_max=t_max, n_traj=B, mi355x={"seed": seed, "engine": "fp64"},
+                         results={"diagDMatrix": True, "tbStates": True, "mfStates": True})
+    path = lat.write_simulation(str(tmp_path), sim, pos)
+    _lib.run_json_file(path)
+    out = json.load(open(tmp_path / "states_fTWA_res.json"))["simulation"]
+    d = _lib.load_system_json(path)
+    Vt = O.ground_state(d["tM"], None, 4, 3, False)[0]
+    Vu, Vd, _, _ = O.ground_state(d["tM"], d["U"], 4, 3, True)
+    eng = _lib.Engine(d["tM"], U=d["U"])
+    eng.set_initial_state(d["n_up"], d["n_dn"])
+    eng.set_params(dt, t_max, 1, B, seed=seed, engine="fp64")
+    noise = eng.sample_noise(0, B)
+    eng.close()
+    ref = O.normalise(O.run(oracle_system(O, d["tM"], 1, d["U"]), dt, t_max, 1, d["n_up"], d["n_dn"],
+                            noise_arr=noise, bases=np.stack([np.stack([Vt, Vt]), np.stack([Vu, Vd])])), B)
+    for key, b, sg in (("Mkku", 0, 0), ("Mkkd", 0, 1), ("Mukku", 1, 0), ("Mukkd", 1, 1)):
+        assert rel(np.asarray(out[key]), ref["M"][b, sg]) < FP64_RTOL, key
+    assert rel(np.asarray(out["Niiu"]), ref["Niiu"]) < FP64_RTOL
 
 
 def test_few_trajectories_and_single_time_point(oracle_mod):

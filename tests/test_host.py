@@ -162,10 +162,10 @@ def test_front_end_errors(tmp_path):
     with pytest.raises(_lib.FtwaError) as e:
         _lib.load_system_json(path)
     assert "e.xyz" in str(e.value)
-    sim = lat.simulation("g", pos, None, False, ic={"type": "tbGroundState"})
+    sim = lat.simulation("g", pos, None, False, ic={"type": "groundState"})
     with pytest.raises(_lib.FtwaError) as e:
         load(tmp_path, sim, pos)
-    assert e.value.code == _lib.EINVAL and "tbGroundState" in str(e.value)
+    assert e.value.code == _lib.EINVAL and "groundState" in str(e.value)
     bad = tmp_path / "bad.json"
     bad.write_text('{"system": {"lattice": ')
     with pytest.raises(_lib.FtwaError) as e:
@@ -211,3 +211,95 @@ def test_create_argument_checks():
     assert e.value.code == _lib.EINVAL
     with pytest.raises(_lib.FtwaError):
         _lib.Engine(np.zeros((3, 3)), U=np.eye(3), pulse_type=1, A0=1.0)  # pulse without displacements
+
+
+def _chain_tm(L, t=-1.0, periodic=False):
+    tM = t * (np.eye(L, k=1) + np.eye(L, k=-1))
+    if periodic:
+        tM[0, L - 1] = tM[L - 1, 0] = t
+    return tM
+
+
+def test_ground_state_matches_lapack(oracle_mod):
+    """fpsftwa_ground_state (Jacobi) against numpy's LAPACK eigh in the same sign gauge: random symmetric
+    matrices, chains, and mean-field iterations that converge (iteration counts equal too)."""
+    O = oracle_mod
+    rng = np.random.default_rng(11)
+    for L in (2, 7, 24):
+        A = rng.normal(size=(L, L))
+        A = A + A.T
+        g = _lib.ground_state(A)
+        np.testing.assert_allclose(g["rot_up"], O.ground_state(A, None, 1, 1, False)[0], atol=1e-12)
+        np.testing.assert_array_equal(g["rot_up"], g["rot_dn"])
+    for L, nu, nd in ((6, 3, 3), (6, 4, 2), (8, 5, 3), (9, 5, 4), (16, 9, 7), (30, 16, 14)):
+        tM = _chain_tm(L)
+        U = 4.0 * np.eye(L)
+        g = _lib.ground_state(tM, U, nu, nd, mean_field=True)
+        Vu, Vd, it, res = O.ground_state(tM, U, nu, nd, True)
+        assert g["iterations"] == it and abs(g["residue"] - res) < 1e-9, (L, nu, nd)
+        np.testing.assert_allclose(g["rot_up"], Vu, atol=1e-10)
+        np.testing.assert_allclose(g["rot_dn"], Vd, atol=1e-10)
+        np.testing.assert_allclose(_lib.ground_state(tM)["rot_up"], O.ground_state(tM, None, nu, nd, False)[0],
+                                   atol=1e-12)
+
+
+def test_ground_state_degenerate_spectrum(oracle_mod):
+    """Periodic ring: cos k levels are doubly degenerate, so any orthonormal basis of a level is valid.  The
+    basis is orthogonal, diagonalises tM with LAPACK's eigenvalues, and the occupied projector at a closed
+    shell (n = 5 of 10: levels 1 + 2 + 2) equals LAPACK's."""
+    O = oracle_mod
+    L = 10
+    tM = _chain_tm(L, periodic=True)
+    V = _lib.ground_state(tM)["rot_up"]
+    np.testing.assert_allclose(V.T @ V, np.eye(L), atol=1e-13)
+    np.testing.assert_allclose(V.T @ tM @ V, np.diag(np.linalg.eigvalsh(tM)), atol=1e-12)
+    Vo = O.ground_state(tM, None, 5, 5, False)[0]
+    np.testing.assert_allclose(V[:, :5] @ V[:, :5].T, Vo[:, :5] @ Vo[:, :5].T, atol=1e-12)
+
+
+def test_ground_state_argument_checks():
+    tM = _chain_tm(4)
+    with pytest.raises(_lib.FtwaError):
+        _lib.ground_state(tM, None, 5, 1)
+    with pytest.raises(_lib.FtwaError):
+        _lib.ground_state(tM, None, -1, 1)
+
+
+def test_eigenbasis_initial_conditions_from_json(tmp_path, oracle_mod):
+    """IC types tbGroundState / mfGroundState (src/FHSystem.cpp:262-270, :536-595; src/FHfTWASolver.cpp:120-153):
+    nPartUp / nPartDown lowest states occupied, rotations = eigenvectors of tM (both spins) or of the
+    self-consistent Hartree Hamiltonian per spin; other IC types keep the identity."""
+    O = oracle_mod
+    pos, _ = lat.chain(7)
+    path = write(tmp_path, lat.simulation("tb", pos, None, False, ic={"type": "tbGroundState"}), pos)
+    d = _lib.load_initial_state_json(path)
+    np.testing.assert_array_equal(d["n_up"], [1, 1, 1, 1, 0, 0, 0])  # nPartUp default ceil(L/2)
+    np.testing.assert_array_equal(d["n_dn"], [1, 1, 1, 0, 0, 0, 0])  # nPartDown default floor(L/2)
+    tM = _lib.load_system_json(path)["tM"]
+    V = O.ground_state(tM, None, 4, 3, False)[0]
+    np.testing.assert_allclose(d["rot_up"], V, atol=1e-12)
+    np.testing.assert_allclose(d["rot_dn"], V, atol=1e-12)
+    path = write(tmp_path, lat.simulation("mf", pos, None, False, U=3.0,
+                                          ic={"type": "mfGroundState", "nPartUp": 5, "nPartDown": 2}), pos)
+    d = _lib.load_initial_state_json(path)
+    np.testing.assert_array_equal(d["n_up"], [1, 1, 1, 1, 1, 0, 0])
+    np.testing.assert_array_equal(d["n_dn"], [1, 1, 0, 0, 0, 0, 0])
+    Vu, Vd, it, _ = O.ground_state(tM, 3.0 * np.eye(7), 5, 2, True)
+    assert 1 < it < 100
+    np.testing.assert_allclose(d["rot_up"], Vu, atol=1e-10)
+    np.testing.assert_allclose(d["rot_dn"], Vd, atol=1e-10)
+    assert np.abs(d["rot_up"].imag).max() == 0
+    from fermiphasespace_amd import FHSystem
+    fs = FHSystem.from_json_file(path)
+    np.testing.assert_allclose(fs.rot_up, Vu, atol=1e-10)
+    fs2 = FHSystem(tM=tM, U=3.0 * np.eye(7))
+    fs2.setGroundStateInitialConditions("mfGroundState", 5, 2)
+    np.testing.assert_allclose(fs2.rot_dn, Vd, atol=1e-10)
+    np.testing.assert_array_equal(fs2.n_up, d["n_up"])
+    d = _lib.load_initial_state_json(write(tmp_path, lat.simulation("sw", pos, None, False), pos))
+    np.testing.assert_array_equal(d["rot_up"], np.eye(7))
+    assert FHSystem.from_json_file(str(tmp_path / "sw.json")).rot_up is None
+    with pytest.raises(_lib.FtwaError) as e:
+        _lib.load_initial_state_json(write(tmp_path, lat.simulation(
+            "big", pos, None, False, ic={"type": "tbGroundState", "nPartUp": 8}), pos))
+    assert "nPartUp" in str(e.value)

@@ -281,3 +281,87 @@ def test_seeded_equals_explicit_noise(oracle_mod):
     b = O.run(s, 0.02, 0.2, 1, n_up, 1 - n_up, noise_arr=O.noise(L, 0, B, 77), nthreads=3)
     np.testing.assert_allclose(a.sum_u, b.sum_u, rtol=1e-14, atol=1e-14)
     np.testing.assert_allclose(a.sum_e, b.sum_e, rtol=1e-13, atol=1e-13)
+
+
+def test_ground_state_closed_forms(oracle_mod):
+    """Open chain: eig(tM) has the closed form v_k(i) = sqrt(2/(L+1)) sin(pi k (i+1)/(L+1)) with eigenvalue
+    2 t cos(pi k/(L+1)) (ascending for t < 0 with k = 1..L).  Mean field at half filling on the chain keeps
+    n = 1/2 everywhere (particle-hole symmetry), so the second iterate reproduces the first: two iterations,
+    residue 0, and the same eigenvectors as tight binding.  With U = 0 mean field is tight binding."""
+    O = oracle_mod
+    L, t = 9, -1.0
+    tM = t * (np.eye(L, k=1) + np.eye(L, k=-1))
+    V, Vd, it, res = O.ground_state(tM, None, 5, 4, False)
+    k = np.arange(1, L + 1)
+    closed = np.sqrt(2.0 / (L + 1)) * np.sin(np.pi * np.outer(np.arange(1, L + 1), k) / (L + 1))
+    np.testing.assert_allclose(np.abs(V), np.abs(closed), atol=1e-13)
+    np.testing.assert_allclose(np.diag(V.T @ tM @ V), 2 * t * np.cos(np.pi * k / (L + 1)), atol=1e-13)
+    assert np.array_equal(V, Vd) and it == 0
+    big = np.argmax(np.abs(V) >= np.abs(V).max(axis=0) / (1 + 1e-8), axis=0)  # first of the (tied) largest
+    assert np.all(V[big, np.arange(L)] > 0)
+    L = 10
+    tM = t * (np.eye(L, k=1) + np.eye(L, k=-1))
+    Vu, Vd, it, res = O.ground_state(tM, 4.0 * np.eye(L), 5, 5, True)
+    assert it == 2 and res < 1e-12
+    np.testing.assert_allclose(Vu, O.ground_state(tM, None, 5, 5, False)[0], atol=1e-12)
+    np.testing.assert_allclose((Vu[:, :5] ** 2).sum(axis=1), 0.5, atol=1e-13)
+    Vu0, Vd0, _, _ = O.ground_state(tM + np.diag(np.linspace(-0.3, 0.2, L)), np.zeros((L, L)), 6, 3, True)
+    np.testing.assert_allclose(Vu0, O.ground_state(tM + np.diag(np.linspace(-0.3, 0.2, L)), None, 6, 3, False)[0],
+                               atol=1e-12)
+
+
+def test_ground_state_mean_field_fixed_point(oracle_mod):
+    """A converged mean-field basis diagonalises tM + diag(U n_other) built from its own densities (to the
+    1e-4 L convergence threshold), and the spins differ when n_up != n_dn."""
+    O = oracle_mod
+    L = 8
+    tM = -(np.eye(L, k=1) + np.eye(L, k=-1))
+    U = 4.0 * np.eye(L)
+    Vu, Vd, it, res = O.ground_state(tM, U, 5, 3, True)
+    assert 1 < it < 100 and res < 1e-4 * L
+    nu, nd = (Vu[:, :5] ** 2).sum(axis=1), (Vd[:, :3] ** 2).sum(axis=1)
+    for V, other in ((Vu, nd), (Vd, nu)):
+        D = V.T @ (tM + np.diag(U @ other)) @ V
+        assert np.abs(D - np.diag(np.diag(D))).max() < 1e-2
+        assert np.all(np.diff(np.diag(D)) > -1e-2)
+    assert np.abs(nu - nd).max() > 0.1
+
+
+def test_basis_occupations(oracle_mod):
+    """tbStates / mfStates sums diag(V^dagger rho V) (src/FHfTWASolver.cpp:385-394).
+    (1) V = identity gives the site densities.  (2) Free fermions started in their own eigenbasis keep
+    every eigenstate occupation: in that basis RK4 multiplies element (k, l) by a polynomial in
+    (E_k - E_l) h, which is 1 on the diagonal, so diag(V^T rho V) = n - 1/2 per trajectory at every time
+    point (a transposed V fails this).  (3) A complex unitary basis against the final states."""
+    O = oracle_mod
+    L, B = 6, 5
+    tM = -(np.eye(L, k=1) + np.eye(L, k=-1)) + np.diag(np.linspace(-0.3, 0.4, L))
+    U = 2.0 * np.eye(L)
+    n_up = np.array([1.0, 0, 1, 1, 0, 0])
+    n_dn = np.array([0.0, 1, 0, 1, 1, 0])
+    rng = np.random.default_rng(5)
+    noise = rng.normal(size=(B, 2, L * (L - 1) // 2, 2)) / np.sqrt(2)
+    eye = np.broadcast_to(np.eye(L), (1, 2, L, L))
+    r = O.run(O.System(tM=tM, coulomb_type=1, U=U), 0.05, 0.5, 2, n_up, n_dn, noise_arr=noise, bases=eye)
+    np.testing.assert_allclose(r.sum_basis[0, 0], r.sum_u, atol=1e-13)
+    np.testing.assert_allclose(r.sum_basis[0, 1], r.sum_d, atol=1e-13)
+
+    V = O.ground_state(tM, None, 3, 3, False)[0]
+    occ = (np.arange(L) < 3).astype(float)
+    r = O.run(O.System(tM=tM, coulomb_type=0), 0.05, 1.0, 2, occ, occ, noise_arr=noise, rot_up=V, rot_dn=V,
+              bases=np.stack([np.stack([V, V])]))
+    np.testing.assert_allclose(r.sum_basis[0, 0], B * (occ[:, None] - 0.5) * np.ones((1, r.T)), atol=1e-12)
+    np.testing.assert_allclose(r.sum_basis[0, 1], B * (occ[:, None] - 0.5) * np.ones((1, r.T)), atol=1e-12)
+    assert np.abs(r.sum_u[:, -1] - r.sum_u[:, 0]).max() > 1e-3  # the site densities do move
+    Vt = np.stack([np.stack([V.T, V.T])])
+    rt = O.run(O.System(tM=tM, coulomb_type=0), 0.05, 1.0, 2, occ, occ, noise_arr=noise, rot_up=V, rot_dn=V, bases=Vt)
+    assert np.abs(rt.sum_basis[0, 0, :, -1] - B * (occ - 0.5)).max() > 1e-3
+
+    Q = [np.linalg.qr(rng.normal(size=(L, L)) + 1j * rng.normal(size=(L, L)))[0] for _ in range(2)]
+    r = O.run(O.System(tM=tM, coulomb_type=1, U=U), 0.05, 0.3, 1, n_up, n_dn, noise_arr=noise, keep_states=True,
+              bases=np.stack([np.stack(Q)]))
+    for sg in range(2):
+        want = sum(np.real(np.diag(Q[sg].conj().T @ r.final_states[b, sg] @ Q[sg])) for b in range(B))
+        np.testing.assert_allclose(r.sum_basis[0, sg, :, -1], want, atol=1e-12)
+    M = O.normalise(r, B)["M"]
+    np.testing.assert_allclose(M, r.sum_basis / B + 0.5)
