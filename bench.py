@@ -102,11 +102,193 @@ def cpu_baseline(w, dt, target_s=15.0):
                       f"{K} RK4 steps, {el:.1f} s on {nthreads} OpenMP threads"}
 
 
+# ---- the other workloads (parity cases of the build; one JSON line each, not the headline) ----------
+
+
+def _timed_run(eng, sync):
+    sync()
+    t0 = time.perf_counter()
+    eng.run()
+    sync()
+    return time.perf_counter() - t0
+
+
+def side_chain10(args, sync):
+    """10-site open chain, t = 1, U = 4, Neel, 1,000 trajectories, dt 0.01, 10 sub-steps, t_max 10 (the
+    reference's CPU-sized case): FP64 batched and plain engines, and the CPU oracle on the same run."""
+    from fermiphasespace_amd import _lib
+    from oracle import oracle as O
+    L, B, dt, nsub = 10, 1000, 0.01, 10
+    t_max = 10.0 if args.steps is None else args.steps * dt
+    tM = -(np.eye(L, k=1) + np.eye(L, k=-1))
+    nu = np.array([1.0, 0] * 5)
+    eng = _lib.Engine(tM, U=4.0 * np.eye(L))
+    eng.set_initial_state(nu, 1 - nu)
+    eng.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES)
+    steps = int(round(t_max / dt)) * nsub
+    out = {"metric": "fTWA RK4 traj-steps/s, 10-site chain 1000 traj", "unit": "traj-steps/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "fp64", "data": "synthetic",
+           "config": {"workload": "chain10_open_neel", "L": L, "n_traj": B, "dt": dt, "n_sub": nsub, "t_max": t_max,
+                      "U": 4.0}}
+    for name in ("fp64", "fp64_plain", "fp32"):
+        eng.set_params(dt, 2 * dt, nsub, B, seed=101, engine=name)
+        eng.run()
+        eng.set_params(dt, t_max, nsub, B, seed=101, engine=name)
+        el = _timed_run(eng, sync)
+        out[name] = {"value": round(B * steps / el, 1), "seconds": round(el, 3)}
+        log(f"chain10 {name}: {B * steps / el:.4e} traj-steps/s")
+    out["value"] = out["fp64"]["value"]
+    eng.close()
+    if not args.no_cpu:  # the whole configured run fits the CPU budget at L = 10
+        nth = os.cpu_count() or 1
+        t0 = time.perf_counter()
+        O.run(O.System(tM=tM, coulomb_type=1, U=4.0 * np.eye(L)), dt, t_max, nsub, nu, 1 - nu, n_traj=B, seed=101,
+              nthreads=nth)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(B * steps / el, 1), "unit": "traj-steps/s", "cores": nth, "kind": "port",
+                               "sample": f"FP64 oracle on the whole run: {B} trajectories x {steps} RK4 steps, "
+                                         f"{el:.1f} s on {nth} OpenMP threads"}
+    print(json.dumps(out), flush=True)
+
+
+def side_mixed(args, sync):
+    """Honeycomb 24 / kagome 48 / square 100 / bcc 54 with site potentials, FP32, dt 0.01, t_max 10:
+    per-system kernel by the autotuner, all four systems at once (fpsftwa_run_many)."""
+    from fermiphasespace_amd import _lib
+    from fermiphasespace_amd import lattices as lat
+    dt = 0.01
+    K = 1000 if args.steps is None else args.steps
+    ws = lat.workload_mixed()
+    engs, tune = [], {}
+    for w in ws:
+        e = _lib.Engine(w["tM"], U=w["U"], pot=w["pot"])
+        e.set_initial_state(w["n_up"], w["n_dn"])
+        e.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES)
+        e.set_params(dt, K * dt, 1, w["n_traj"], seed=303, engine="fp32")
+        tune[w["name"]] = e.autotune(trial_steps=8)
+        engs.append(e)
+    tot = sum(w["n_traj"] for w in ws) * K
+    sync()
+    t0 = time.perf_counter()
+    _lib.run_many(engs)
+    sync()
+    el = time.perf_counter() - t0
+    per = {}
+    for w, e in zip(ws, engs):
+        el1 = _timed_run(e, sync)
+        per[w["name"]] = {"L": w["L"], "n_traj": w["n_traj"], "kernel": tune[w["name"]]["kernel"],
+                          "alone_traj_steps_per_s": round(w["n_traj"] * K / el1, 1)}
+        e.close()
+    out = {"metric": "fTWA RK4 traj-steps/s, mixed-lattice batch (4 systems at once)", "value": round(tot / el, 1),
+           "unit": "traj-steps/s", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+           "seconds": round(el, 3), "config": {"workload": "mixed_honeycomb24_kagome48_square100_bcc54", "dt": dt,
+                                               "steps": K}, "systems": per}
+    log("mixed", json.dumps(out))
+    print(json.dumps(out), flush=True)
+
+
+def side_moire196(args, sync):
+    """Twisted bilayer L = 196, long-range hopping (cut-off 4a) and Coulomb V = U/(1 + r/a), Gaussian
+    pulse, 4,096 trajectories, dt 0.005, t_max 10: the dense path (one GEMM per RK stage for the batch).
+    The roofline is the GEMM's: 64 L^3 flop per trajectory-step (with the pulse, [Y_re Y_im] times the
+    2L x 2L real block matrix per spin and stage) against the FP32 MFMA peak, over the integrator time."""
+    from fermiphasespace_amd import _lib
+    from fermiphasespace_amd import lattices as lat
+    dt, B = 0.005, 4096
+    K = 2000 if args.steps is None else args.steps
+    w = lat.workload_moire196()
+    L = w["L"]
+    eng = _lib.Engine(w["tM"], coulomb_type=_lib.COULOMB_GENERAL, U=w["U"], V=w["V"], **w["pulse"])
+    eng.set_initial_state(w["n_up"], w["n_dn"])
+    eng.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES)
+    eng.set_params(dt, 2 * dt, 1, B, seed=404, engine="fp32")
+    eng.run()
+    eng.set_params(dt, K * dt, 1, B, seed=404, engine="fp32")
+    el = _timed_run(eng, sync)
+    tm = eng.timing()
+    flop = 64.0 * L ** 3 * B * K
+    achieved = flop / (tm["kernel_ms"] * 1e-3) / 1e12
+    out = {"metric": "fTWA RK4 traj-steps/s, moire L=196 4096 traj (dense path)", "value": round(B * K / el, 1),
+           "unit": "traj-steps/s", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+           "seconds": round(el, 3),
+           "config": {"workload": "moire196_longrange_pulse", "L": L, "n_traj": B, "dt": dt, "steps": K},
+           "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": 157.3, "unit": "TFLOP/s",
+                        "frac": round(achieved / 157.3, 4), "traffic": None,
+                        "note": "GEMM flop over the whole integrator time (GEMM + update + Hartree + record kernels)",
+                        "kernel_ms": round(tm["kernel_ms"], 3), "launches": tm["launches"]}}
+    log("moire196", json.dumps(out))
+    print(json.dumps(out), flush=True)
+
+
+def side_adjoint6x6(args, sync):
+    """6x6 lattice, 200 pulse knots, 16,384 trajectories, dt 0.01, t_max 10, checkpoints every 50 steps:
+    gradient of the ensemble double occupancy at t_max (FP32 and FP64), with central finite differences
+    (h = 1e-4) of the FP64 forward run for dJ/dU (uniform shift) and dJ/d(knot 50)."""
+    from fermiphasespace_amd import _lib
+    from fermiphasespace_amd import lattices as lat
+    dt, B = 0.01, 16384
+    t_max = 10.0 if args.steps is None else args.steps * dt
+    w = lat.workload_adjoint6x6()
+    L = w["L"]
+
+    def engine(U0=4.0, knots=None, name="fp64"):
+        e = _lib.Engine(w["tM"], U=U0 * np.eye(L), pulse_type=_lib.PULSE_KNOTS,
+                        knots=w["knots"] if knots is None else knots, pol=w["pol"], disp=w["disp"])
+        e.set_initial_state(w["n_up"], w["n_dn"])
+        e.set_params(dt, t_max, 1, B, seed=505, engine=name)
+        return e
+
+    def forward(**kw):
+        e = engine(**kw)
+        e.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS)
+        e.run()
+        J = float(e.result("NuNd")[:, -1].mean())
+        e.close()
+        return J
+
+    out = {"metric": "adjoint fTWA gradient, 6x6 lattice 200 knots 16384 traj", "unit": "s per gradient",
+           "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+           "config": {"workload": "adjoint6x6_knots200", "L": L, "n_traj": B, "dt": dt, "t_max": t_max,
+                      "checkpoint_every": 50}}
+    grads = {}
+    for name in ("fp32", "fp64"):
+        e = engine(name=name)
+        sync()
+        t0 = time.perf_counter()
+        g = e.adjoint(50)
+        sync()
+        el = time.perf_counter() - t0
+        e.close()
+        grads[name] = g
+        out[name] = {"seconds": round(el, 3), "J": g["J"], "dJdU_sum": float(np.sum(g["grad_u"])),
+                     "dJdk50": float(g["grad_pulse"][50])}
+        log(f"adjoint {name}: {el:.2f} s")
+    out["value"] = out["fp32"]["seconds"]
+    h = 1e-4
+    fd_u = (forward(U0=4.0 + h) - forward(U0=4.0 - h)) / (2 * h)
+    kp, km = w["knots"].copy(), w["knots"].copy()
+    kp[50] += h
+    km[50] -= h
+    fd_k = (forward(knots=kp) - forward(knots=km)) / (2 * h)
+    out["finite_differences_fp64"] = {
+        "h": h, "dJdU": fd_u, "dJdk50": fd_k,
+        "rel_diff_dJdU": abs(out["fp64"]["dJdU_sum"] - fd_u) / max(abs(fd_u), 1e-300),
+        "rel_diff_dJdk50": abs(out["fp64"]["dJdk50"] - fd_k) / max(abs(fd_k), 1e-300)}
+    gp32, gp64 = grads["fp32"]["grad_pulse"], grads["fp64"]["grad_pulse"]
+    out["fp32_vs_fp64_knots_max_rel"] = float(np.max(np.abs(gp32 - gp64)) / np.max(np.abs(gp64)))
+    log("adjoint", json.dumps(out))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000,
-                    help="timed RK4 steps of the 65,536-trajectory batch (4000 = t_max 20 at dt 0.005)")
+    ap.add_argument("--workload", default="hubbard8x8",
+                    choices=["hubbard8x8", "chain10", "mixed", "moire196", "adjoint6x6"],
+                    help="hubbard8x8 is the headline; the others print their own line (default length: the "
+                         "whole configured run)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed RK4 steps (hubbard8x8: 4000 = t_max 20 at dt 0.005)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n-traj", type=int, default=65536)
     ap.add_argument("--block", type=int, default=0, help="threads per trajectory workgroup (0, 512, 1024)")
@@ -138,6 +320,18 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); trajectories shard trivially "
               "across GPUs but multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "hubbard8x8":
+        try:
+            import torch
+
+            sync_side = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+        except Exception:
+            sync_side = lambda: None  # noqa: E731
+        {"chain10": side_chain10, "mixed": side_mixed, "moire196": side_moire196,
+         "adjoint6x6": side_adjoint6x6}[args.workload](args, sync_side)
+        return
+    if args.steps is None:
+        args.steps = 4000
 
     from fermiphasespace_amd import _lib
     try:

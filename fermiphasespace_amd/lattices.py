@@ -230,6 +230,21 @@ def workload_hubbard8x8(n_traj=65536, seed=202, engine="fp32", t_max=20.0, dt=0.
     return sim, pos
 
 
+def workload_adjoint6x6(n_knots=200, amp=0.3):
+    """6x6 periodic square lattice (L = 36), t=-1, U=4, charge-density wave up / complement down, knot pulse
+    a(t) = 0.3 sin(3 s) on n_knots equally spaced knots over [0, t_max], polarisation (1, 1, 0)/sqrt 2,
+    minimum-image displacements on the torus."""
+    pos, box = grid(6, 6)
+    tM, Um = system_arrays(pos, box, True, t=-1.0, U=4.0)
+    d = pos[:, None, :] - pos[None, :, :]
+    d = d - np.round(d / 6.0) * 6.0
+    d[..., 2] = 0.0
+    n_up = np.array(checkerboard(6, 6)["IC_NuDiag"], dtype=float)
+    knots = amp * np.sin(np.linspace(0.0, 3.0, n_knots))
+    return {"tM": tM, "U": Um, "n_up": n_up, "n_dn": 1.0 - n_up, "knots": knots, "disp": d,
+            "pol": (2 ** -0.5, 2 ** -0.5, 0.0), "L": 36}
+
+
 def system_arrays(pos, box, periodic: bool, t=-1.0, U=4.0, nn=1.0, same_point=1e-4):
     """tM (nearest neighbours at distance nn), on-site U, and min-image displacements, as arrays."""
     D = min_image_distances(pos, box if periodic else None)

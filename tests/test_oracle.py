@@ -365,3 +365,52 @@ def test_basis_occupations(oracle_mod):
         np.testing.assert_allclose(r.sum_basis[0, sg, :, -1], want, atol=1e-12)
     M = O.normalise(r, B)["M"]
     np.testing.assert_allclose(M, r.sum_basis / B + 0.5)
+
+
+@pytest.fixture(scope="module")
+def ed_mod():
+    from oracle import ed
+    return ed
+
+
+def test_ed_known_answers(ed_mod):
+    """The ED reference itself: (1) free fermions follow the one-particle propagator,
+    n_i(t) = sum_k |(e^{-i tM t})_ik|^2 n_k(0); (2) the two-site Hubbard dimer from |up, dn>: the singlet
+    half couples to the symmetric doublon state with amplitude 2t, so the total double occupancy is
+    (1/2)(4t^2/W^2) sin^2(W t), W = sqrt(U^2/4 + 4t^2)."""
+    ed = ed_mod
+    L = 6
+    tM = -(np.eye(L, k=1) + np.eye(L, k=-1)) + np.diag([0.1, -0.2, 0.0, 0.3, 0.0, -0.1])
+    occ = np.array([1, 1, 0, 1, 0, 0])
+    r = ed.evolve_product_state(L, tM, np.zeros(L), occ, 1 - occ, 1.5, 7)
+    for k, t in enumerate(r["t"]):
+        P = np.abs(expm(-1j * tM * t)) ** 2
+        np.testing.assert_allclose(r["n_up"][:, k], P @ occ, atol=1e-10)
+        np.testing.assert_allclose(r["n_dn"][:, k], P @ (1 - occ), atol=1e-10)
+    t_hop, U = 0.7, 3.0
+    r = ed.evolve_product_state(2, -t_hop * np.array([[0.0, 1.0], [1.0, 0.0]]), [U, U], [1, 0], [0, 1], 2.0, 9)
+    W = np.sqrt(U ** 2 / 4 + 4 * t_hop ** 2)
+    want = 0.5 * (4 * t_hop ** 2 / W ** 2) * np.sin(W * r["t"]) ** 2
+    np.testing.assert_allclose(r["nund"].sum(axis=0), want, atol=1e-10)
+
+
+def test_ftwa_against_exact_diagonalisation(oracle_mod, ed_mod):
+    """Physics check of the algorithm on the 10-site open chain (t = 1, Neel state): fTWA is exact for
+    U = 0, so the ensemble densities match ED at every time within the sampling error; at U = 4 it is an
+    approximation that matches ED at short times (t <= 0.5) and departs later (0.07 at t = 1)."""
+    O, ed = oracle_mod, ed_mod
+    L, B, dt = 10, 1024, 0.01
+    tM = -(np.eye(L, k=1) + np.eye(L, k=-1))
+    nu = np.array([1.0, 0] * 5)
+    nd = 1 - nu
+    for U, t_ok in ((0.0, 2.0), (4.0, 0.5)):
+        r = ed.evolve_product_state(L, tM, U * np.ones(L), nu, nd, t_ok, int(round(t_ok / 0.25)) + 1)
+        s = O.run(O.System(tM=tM, coulomb_type=1, U=U * np.eye(L)), dt, t_ok, 1, nu, nd, n_traj=B, seed=7,
+                  nthreads=8)
+        n = O.normalise(s, B)
+        idx = np.round(r["t"] / dt).astype(int)
+        for spin, su, s2 in (("n_up", s.sum_u, s.sum2_u), ("n_dn", s.sum_d, s.sum2_d)):
+            mean = su[:, idx] / B
+            err = np.sqrt(np.maximum(s2[:, idx] / B - mean ** 2, 0) / B)
+            got = n["Niiu" if spin == "n_up" else "Niid"][:, idx]
+            assert np.all(np.abs(got - r[spin]) <= 5 * err + 1e-9), (U, spin, np.abs(got - r[spin]).max())
