@@ -40,7 +40,7 @@ EXPORTED = [
     "fpsftwa_sample_noise", "fpsftwa_run", "fpsftwa_get_result", "fpsftwa_get_states", "fpsftwa_get_timing",
     "fpsftwa_set_tuning", "fpsftwa_run_json_file", "fpsftwa_load_system_json", "fpsftwa_run_adjoint",
     "fpsftwa_autotune", "fpsftwa_run_many", "fpsftwa_load_initial_state_json", "fpsftwa_ground_state",
-    "fpsftwa_set_bases",
+    "fpsftwa_set_bases", "fpsftwa_set_recorded_element",
 ]
 
 
@@ -112,6 +112,7 @@ def load(path: str = LIB_PATH):
         "fpsftwa_load_initial_state_json": ([ctypes.c_char_p, P(i32), P(d), P(d), P(d), P(d)], ctypes.c_int),
         "fpsftwa_ground_state": ([i32, P(d), P(d), i32, i32, i32, P(d), P(d), P(i32), P(d)], ctypes.c_int),
         "fpsftwa_set_bases": ([H, i32, P(d)], ctypes.c_int),
+        "fpsftwa_set_recorded_element": ([H, i32, i32], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -312,6 +313,10 @@ class Engine:
         if name in ("basis", "sum_basis"):  # [n_bases][2][L][T]
             return out.reshape(-1, 2, self.L, T)
         return out.reshape(-1, T)
+
+    def set_recorded_element(self, i: int, j: int):
+        """The element rho(i, j) behind result("N1hu") / result("N1hd") (default (0, L-1))."""
+        check(self._lib.fpsftwa_set_recorded_element(self._h, int(i), int(j)), "set_recorded_element")
 
     def set_bases(self, bases):
         """Bases whose occupations diag(V^T rho V) are recorded: [n_bases][2][L][L] real (columns the

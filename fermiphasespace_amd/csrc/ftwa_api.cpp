@@ -91,6 +91,7 @@ struct fpsftwa_handle {
     std::vector<double> n_up, n_dn, rot_up, rot_dn;
     int n_bases = 0;
     std::vector<double> bases;  // [n_bases][2][L][L] real
+    int rec_i = 0, rec_j = -1;  // recorded off-diagonal element (j < 0: L - 1)
     std::vector<double> noise;
     int nt_hint = 0;
     int time_chunk = 0;
@@ -420,6 +421,8 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     rs.h = h->dt / h->nsub;
     rs.R = R;
     rs.dens = dens;
+    rs.ci = h->rec_i;
+    rs.cj = h->rec_j < 0 ? L - 1 : h->rec_j;
     if (dense) HIP_TRY(h->dense_buf.ensure(dense_buffer_elems<Real>(L, h->B) * sizeof(Real)));
     for (int t0 = 0; t0 < h->T; t0 += Tc) {
         const int t1 = std::min(h->T, t0 + Tc);
@@ -465,6 +468,8 @@ int run_plain(fpsftwa_handle* h, EventTimer& kt)
     rs.h = h->dt / h->nsub;
     rs.R = R;
     rs.dens = dens;
+    rs.ci = h->rec_i;
+    rs.cj = h->rec_j < 0 ? L - 1 : h->rec_j;
     for (int t = 0; t < h->T; ++t) {
         HIP_TRY(kt.mark(h->stream));
         HIP_TRY(launch_plain(ds, rs, t, h->state.as<double>(), h->work.as<double>(), h->rec.as<double>(), h->stream));
@@ -780,6 +785,16 @@ int fpsftwa_set_bases(fpsftwa_handle* h, int32_t n_bases, const double* bases)
     return FPSFTWA_OK;
 }
 
+int fpsftwa_set_recorded_element(fpsftwa_handle* h, int32_t i, int32_t j)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (i < 0 || j < 0 || i >= h->L || j >= h->L) return fail(FPSFTWA_EINVAL, "element index outside [0, L)");
+    h->rec_i = i;
+    h->rec_j = j;
+    h->has_result = false;
+    return FPSFTWA_OK;
+}
+
 int fpsftwa_set_noise(fpsftwa_handle* h, const double* noise, size_t n_values)
 {
     if (int rc = check_handle(h)) return rc;
@@ -936,6 +951,7 @@ int fpsftwa_get_result(fpsftwa_handle* h, int32_t which, double* out, size_t n_v
         return FPSFTWA_OK;
     }
     auto S = [&](int t, int q) { return kahan_value(h->acc, Q, t, q); };
+    const bool on_diag = h->rec_i == (h->rec_j < 0 ? L - 1 : h->rec_j);
     auto stderr_of = [&](double s1, double s2) {
         if (B < 2) return 0.0;
         double var = (s2 - s1 * s1 / B) / (B - 1.0);
@@ -961,11 +977,11 @@ int fpsftwa_get_result(fpsftwa_handle* h, int32_t which, double* out, size_t n_v
             for (int k = 0; k < 4; ++k) out[(size_t)k * T + t] = S(t, 6 * L + k);
             break;
         case FPSFTWA_OUT_N1HU:
-            out[t] = S(t, 6 * L + 8) / B + (L == 1 ? 0.5 : 0.0);
+            out[t] = S(t, 6 * L + 8) / B + (on_diag ? 0.5 : 0.0);
             out[T + t] = S(t, 6 * L + 9) / B;
             break;
         case FPSFTWA_OUT_N1HD:
-            out[t] = S(t, 6 * L + 10) / B + (L == 1 ? 0.5 : 0.0);
+            out[t] = S(t, 6 * L + 10) / B + (on_diag ? 0.5 : 0.0);
             out[T + t] = S(t, 6 * L + 11) / B;
             break;
         default:

@@ -515,7 +515,8 @@ __global__ void __launch_bounds__(1024)
 
 // records of one time point, one trajectory per workgroup (formulas of the plain path)
 template <typename Real>
-__global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real* ycur, Real* rec, int R, int dens)
+__global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real* ycur, Real* rec, int R, int dens,
+                                    int ci, int cj)
 {
     __shared__ double ew[4 * 4];
     const int L = s.L;
@@ -526,7 +527,7 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
         rr[e] = ycur[row_off<Real>(b, sg, i, L) + i];
     }
     if (threadIdx.x == 0) {
-        const size_t ou = row_off<Real>(b, 0, 0, L) + L - 1, od = row_off<Real>(b, 1, 0, L) + L - 1;
+        const size_t ou = row_off<Real>(b, 0, ci, L) + cj, od = row_off<Real>(b, 1, ci, L) + cj;
         rr[2 * L + 4] = ycur[ou];
         rr[2 * L + 5] = ycur[ou + L];
         rr[2 * L + 6] = ycur[od];
@@ -726,7 +727,7 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     for (int t = t0; t < t1; ++t) {
         hipLaunchKernelGGL(dense_record_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256), 0, st, s,
                            static_cast<int64_t>(t) * r.nsub * 4, ycur, rec + static_cast<int64_t>(t - t0) * B * r.R,
-                           r.R, r.dens);
+                           r.R, r.dens, r.ci, r.cj);
         if (t == r.T - 1) break;
         for (int sub = 0; sub < r.nsub; ++sub) {
             for (int stg = 0; stg < 4; ++stg) {

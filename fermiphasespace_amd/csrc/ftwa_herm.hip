@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(64 * W)
                 rr[e] = yat(sg * L + i, i).x;
             }
             if (tid == 0) {
-                const C cu = yat(0, L - 1), cd = yat(L, L - 1);
+                const C cu = yat(r.ci, r.cj), cd = yat(L + r.ci, r.cj);
                 rr[R2 + 4] = cu.x;
                 rr[R2 + 5] = cu.y;
                 rr[R2 + 6] = cd.x;
@@ -655,7 +655,7 @@ __global__ void __launch_bounds__(1024)
             rr[e] = yat(sgr * L + i, i).x;
         }
         if (tid == 0) {
-            const C cu = yat(0, L - 1), cd = yat(L, L - 1);
+            const C cu = yat(r.ci, r.cj), cd = yat(L + r.ci, r.cj);
             rr[R2 + 4] = cu.x;
             rr[R2 + 5] = cu.y;
             rr[R2 + 6] = cd.x;

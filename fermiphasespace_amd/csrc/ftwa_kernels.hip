@@ -217,10 +217,11 @@ __global__ void __launch_bounds__(kPlainThreads)
             rr[e] = y[2 * (sg * LL + i * L + i)];
         }
         if (tid == 0) {
-            rr[2 * L + 4] = y[2 * (L - 1)];
-            rr[2 * L + 5] = y[2 * (L - 1) + 1];
-            rr[2 * L + 6] = y[2 * (LL + L - 1)];
-            rr[2 * L + 7] = y[2 * (LL + L - 1) + 1];
+            const int c = r.ci * L + r.cj;
+            rr[2 * L + 4] = y[2 * c];
+            rr[2 * L + 5] = y[2 * c + 1];
+            rr[2 * L + 6] = y[2 * (LL + c)];
+            rr[2 * L + 7] = y[2 * (LL + c) + 1];
         }
         if (r.dens)
             dev::record_density(rr + 2 * L + 8, L, tid & 63, tid >> 6, (NT + 63) / 64,

@@ -3,7 +3,7 @@
 // Data layout in HBM (one handle):
 //   state  [B][2 spins][L][L] complex<Real>, row-major, interleaved (re, im)
 //   rec    [Tc][B][R] Real, R = 2L + 8 per-trajectory records of one time point:
-//            [0,L) rho_up_ii  [L,2L) rho_dn_ii  2L+0..3 E_tH, E_U, E_V, E_pe  2L+4..7 corner re/im up, dn
+//            [0,L) rho_up_ii  [L,2L) rho_dn_ii  2L+0..3 E_tH, E_U, E_V, E_pe  2L+4..7 re/im of rho_up(ci, cj), rho_dn(ci, cj)
 //            and, when basis occupations are requested, R = 2L + 8 + L(L+1): [2L+8, R) Re rho_sigma(i, j),
 //            j >= i, spin-major, rows packed (dev::packed_row)
 //   part   [Tc][nchunk][Q][2] double   partial ensemble sums (sum, compensation)
@@ -60,6 +60,8 @@ struct RunShape {
     double h = 0.0;       // dt / nsub
     int R = 0;            // record width
     int dens = 0;         // 1: records also carry the packed real density matrices (R = 2L + 8 + L(L+1))
+    int ci = 0, cj = 0;   // the off-diagonal element rho(ci, cj) recorded per time point: the wholeDMatrix
+                          // corner (0, L-1), or the core-hole element (site, L-1)
 };
 
 // Sampling: rho_sigma = diag(n - 1/2) + S.W (rotation applied separately).  noise (optional,

@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(64 * W)
                 rr[e] = Y[(sg * L + i) * kRow + i].x;
             }
             if (tid == 0) {
-                const C cu = Y[L - 1], cd = Y[L * kRow + L - 1];
+                const C cu = Y[r.ci * kRow + r.cj], cd = Y[(L + r.ci) * kRow + r.cj];
                 rr[R2 + 4] = cu.x;
                 rr[R2 + 5] = cu.y;
                 rr[R2 + 6] = cd.x;

@@ -208,6 +208,16 @@ void FHfTWASolver::initiateSolver()
         bases.insert(bases.end(), ru.begin(), ru.end());
         bases.insert(bases.end(), rd.begin(), rd.end());
     };
+    if (s.core_hole_site >= 0) {
+        // NiCh = <rho_sigma(site, L-1)> (src/FHfTWASolver.cpp:413-417); it takes the engine's recorded
+        // element, which wholeDMatrix's corner (0, L-1) also needs unless the hole is linked to site 0
+        if ((flags_ & FPSFTWA_RES_WHOLE) && s.core_hole_site != 0)
+            throw std::runtime_error("wholeDMatrix together with a core hole linked to site " +
+                                     std::to_string(s.core_hole_site) + " is not supported by this build");
+        if (saveTB_ || saveMF_)  // the reference's L x L bases cannot act on the L+1 hole system either
+            throw std::runtime_error("tbStates / mfStates are not defined for a core-hole system");
+        check(fpsftwa_set_recorded_element(h_, s.core_hole_site, s.L - 1), "core-hole element");
+    }
     int nb = 0;
     if (saveTB_) {
         add_basis(false);
@@ -253,6 +263,17 @@ void FHfTWASolver::runSimulation()
         sim["N2iid"] = rowsOf(fetch(h_, FPSFTWA_OUT_N2IID), L, T);
     }
     if (flags_ & FPSFTWA_RES_DOUBLONS) sim["NuNd"] = rowsOf(fetch(h_, FPSFTWA_OUT_NUND), L, T);
+    if (sys_->data().core_hole_site >= 0) {  // "NiCh": {"u": {r, i}, "d": {r, i}} (src/FHSolver.cpp:285-291)
+        Value nich = Value::object();
+        for (int which : {FPSFTWA_OUT_N1HU, FPSFTWA_OUT_N1HD}) {
+            std::vector<double> c = fetch(h_, which);
+            Value o = Value::object();
+            o["r"] = rowOf(c, 0, T);
+            o["i"] = rowOf(c, 1, T);
+            nich[which == FPSFTWA_OUT_N1HU ? "u" : "d"] = o;
+        }
+        sim["NiCh"] = nich;
+    }
     if (basisTB_ >= 0 || basisMF_ >= 0) {  // src/FHSolver.cpp:242-265
         const std::vector<double> m = fetch(h_, FPSFTWA_OUT_BASIS);
         auto block = [&](int b, int sg) {

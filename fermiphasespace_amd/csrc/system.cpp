@@ -82,6 +82,95 @@ double siteDistance(const FtwaSystem& s, int i, int j, double* disp3)
     return dmin;
 }
 
+bool addCoreHole(const fpsjson::Value& ch, FtwaSystem& s, std::string* err)
+{
+    using fpsjson::Value;
+    // FHCoreHole(Json) defaults (src/FHCoreHole.cpp:10-26) and the extension of FHSystem(Json)
+    // (src/FHSystem.cpp:295-323): one more site L, coupled to `site` by the hopping X, on-site
+    // potential eps, interactions addSite([.. G at site .., Uc at L]) (src/FHCoulombInteraction.cpp:
+    // 189-234), occupations IC = [up, down] (default [1, 0]).
+    const int L = s.L;
+    const int64_t site = ch.get("site", Value((int64_t)0)).asInt();
+    const double eps = ch.get("eps", Value(-2.0)).asDouble();
+    const double X = ch.get("X", Value(-1.0)).asDouble();
+    const double Uc = ch.get("Uc", Value(1.0)).asDouble();
+    const double G = ch.get("G", Value(1.0)).asDouble();
+    double occ_up = 1.0, occ_dn = 0.0;
+    const Value& ic = ch["IC"];
+    if (ic.isArray() && ic.size() > 0) occ_up = ic[0].asDouble();
+    if (ic.isArray() && ic.size() > 1) occ_dn = ic[1].asDouble();
+    if (site < 0 || site >= L) {
+        if (err) *err = "coreHole site " + std::to_string(site) + " outside the lattice";
+        return false;
+    }
+    if (s.coulomb_type == 2) {
+        // for CEdistanceInteraction the reference grows U and V to L+2 while every other matrix is
+        // L+1 (src/FHCoulombInteraction.cpp:213-222): no consistent system exists
+        if (err) *err = "a core hole with a distance Coulomb interaction has no consistent form in the reference";
+        return false;
+    }
+    const int L1 = L + 1;
+    auto grow = [&](std::vector<double>& m) {
+        std::vector<double> g((size_t)L1 * L1, 0.0);
+        if (!m.empty())
+            for (int i = 0; i < L; ++i)
+                for (int j = 0; j < L; ++j) g[(size_t)i * L1 + j] = m[(size_t)i * L + j];
+        m.swap(g);
+    };
+    grow(s.tM);
+    s.tM[(size_t)site * L1 + L] = X;
+    s.tM[(size_t)L * L1 + site] = X;
+    grow(s.U);
+    s.V.assign((size_t)L1 * L1, 0.0);  // null / onSite: V starts from zero
+    s.U[(size_t)site * L1 + L] += G / 2;
+    s.U[(size_t)L * L1 + site] += G / 2;
+    s.U[(size_t)L * L1 + L] += Uc;  // Uc/2 from the column and Uc/2 from the row
+    s.V[(size_t)site * L1 + L] = G / 2;
+    s.V[(size_t)L * L1 + site] = G / 2;
+    s.coulomb_type = 2;  // CEgiven: the dynamics adds the same-spin V term
+    // the reference rebuilds the external potential from scratch before adding the hole's
+    // (src/FHSystem.cpp:311-312): a constant potential or pulse given in the file is dropped
+    if (!s.potential.empty() || s.pulse_type != 0)
+        s.warnings.push_back("coreHole: the external potential of the file is replaced by the hole potential alone "
+                             "(as in the reference)");
+    s.pulse_type = 0;
+    if (eps == 0.0) {
+        s.potential.clear();
+    } else {
+        s.potential.assign(L1, 0.0);
+        s.potential[L] = eps;
+    }
+    // geometry: the hole sits at its site (distances and displacements of the hole are the site's)
+    std::vector<double> pos = s.positions, dist((size_t)L1 * L1, 0.0), disp((size_t)L1 * L1 * 3, 0.0);
+    pos.insert(pos.end(), s.positions.begin() + 3 * site, s.positions.begin() + 3 * site + 3);
+    for (int i = 0; i < L1; ++i)
+        for (int j = 0; j < L1; ++j) {
+            const int a = i == L ? (int)site : i, b = j == L ? (int)site : j;
+            dist[(size_t)i * L1 + j] = s.distance[(size_t)a * L + b];
+            for (int k = 0; k < 3; ++k) disp[((size_t)i * L1 + j) * 3 + k] = s.displacement[((size_t)a * L + b) * 3 + k];
+        }
+    s.positions.swap(pos);
+    s.distance.swap(dist);
+    s.displacement.swap(disp);
+    // initial state (src/FHfTWASolver.cpp:156-184): occupations and rotations gain the hole
+    s.n_up.push_back(occ_up);
+    s.n_dn.push_back(occ_dn);
+    for (std::vector<double>* r : {&s.rot_up, &s.rot_dn}) {
+        if (r->empty()) continue;
+        std::vector<double> g((size_t)L1 * L1 * 2, 0.0);
+        for (int i = 0; i < L; ++i)
+            for (int j = 0; j < L; ++j) {
+                g[((size_t)i * L1 + j) * 2] = (*r)[((size_t)i * L + j) * 2];
+                g[((size_t)i * L1 + j) * 2 + 1] = (*r)[((size_t)i * L + j) * 2 + 1];
+            }
+        g[((size_t)L * L1 + L) * 2] = 1.0;
+        r->swap(g);
+    }
+    s.core_hole_site = (int)site;
+    s.L = L1;
+    return true;
+}
+
 bool computeGroundState(FtwaSystem& s, bool mean_field, int n_up, int n_dn, std::string* err)
 {
     const int L = s.L;
@@ -314,8 +403,7 @@ bool buildSystem(const fpsjson::Value& sys, const std::string& folder, FtwaSyste
         s.warnings.push_back("No Initial condition in the json file.");
     }
     if (sys.isMember("coreHole") && sys["coreHole"].get("presence", Value(false)).asBool()) {
-        if (err) *err = "core-hole systems are not supported by this build";
-        return false;
+        if (!addCoreHole(sys["coreHole"], s, err)) return false;
     }
     out = std::move(s);
     return true;

@@ -34,6 +34,7 @@ struct FtwaSystem {
     std::vector<double> rot_up, rot_dn;          // [L][L][re,im] initial-state rotations; empty = identity
     std::vector<double> tb_rot;                  // [L][L] tight-binding eigenvectors (columns), when computed
     std::vector<double> mf_rot_up, mf_rot_dn;    // [L][L] mean-field eigenvectors per spin, when computed
+    int core_hole_site = -1;                     // lattice site of the core hole (the hole is site L-1), or -1
     int part_up = 0, part_dn = 0;                // upSpinParticleNumber / downSpinParticleNumber (ground-state bases)
     int mf_iterations = 0;                       // self-consistency iterations of the mean-field state
     double mf_residue = 0;                       // its final residue
@@ -45,6 +46,9 @@ struct FtwaSystem {
 // the simulation file (with trailing '/'), prefixed to positionsFile as the reference does.
 // Returns false with err set on unreadable input or an unsupported request.
 bool buildSystem(const fpsjson::Value& systemJson, const std::string& folder, FtwaSystem& out, std::string* err);
+
+// Adds the core hole of a "coreHole" object (src/FHSystem.cpp:295-323) to a built system: L grows by one.
+bool addCoreHole(const fpsjson::Value& coreHoleJson, FtwaSystem& s, std::string* err);
 
 // The tight-binding (mean_field false: src/FHSystem.cpp:909-937) or mean-field (true: :940-1019)
 // eigenbasis of the built system for n_up / n_dn particles, into tb_rot or mf_rot_up/mf_rot_dn.

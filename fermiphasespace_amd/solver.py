@@ -27,6 +27,7 @@ class FHSystem:
     n_dn: np.ndarray | None = None
     rot_up: np.ndarray | None = None  # [L][L] initial-state rotations (tbGroundState / mfGroundState); None = identity
     rot_dn: np.ndarray | None = None
+    core_hole_site: int | None = None  # lattice site linked to the core hole (the hole is site L-1)
     pulse: dict = field(default_factory=dict)  # keys of Engine(...): pulse_type, A0, t0, td, w0, phi, pol, knots, disp
 
     @property
@@ -41,8 +42,11 @@ class FHSystem:
         eye = np.eye(d["L"])
         rot = ic["rot_up"] if not (np.allclose(ic["rot_up"], eye) and np.allclose(ic["rot_dn"], eye)) else None
         pot = d["pot"] if np.any(d["pot"]) else None
+        with open(path) as fh:
+            ch = json.load(fh).get("system", {}).get("coreHole", {})
+        site = int(ch.get("site", 0)) if ch.get("presence", False) else None
         return cls(tM=d["tM"], U=d["U"], V=d["V"], coulomb_type=d["coulomb_type"], pot=pot, n_up=d["n_up"],
-                   n_dn=d["n_dn"], rot_up=rot, rot_dn=None if rot is None else ic["rot_dn"])
+                   n_dn=d["n_dn"], rot_up=rot, rot_dn=None if rot is None else ic["rot_dn"], core_hole_site=site)
 
     def getnSites(self) -> int:
         return self.L
@@ -157,6 +161,10 @@ class FHfTWASolver:
             self._basis_index["mf"] = len(bases)
             bases.append(np.stack([Vu, Vd]))
         self._engine.set_bases(np.stack(bases) if bases else None)
+        if s.core_hole_site is not None:  # NiCh = <rho(site, L-1)> (src/FHfTWASolver.cpp:413-417)
+            if (self.flags & _lib.RES_WHOLE) and s.core_hole_site != 0:
+                raise _lib.FtwaError(_lib.EUNSUPPORTED, "wholeDMatrix with a core hole off site 0 is not supported")
+            self._engine.set_recorded_element(s.core_hole_site, L - 1)
 
     @property
     def engine_handle(self) -> _lib.Engine:
@@ -183,6 +191,11 @@ class FHfTWASolver:
             sim["N2iid"] = e.result("N2iid").tolist()
         if f & _lib.RES_DOUBLONS:
             sim["NuNd"] = e.result("NuNd").tolist()
+        if self.system.core_hole_site is not None:  # src/FHSolver.cpp:285-291
+            sim["NiCh"] = {}
+            for k, key in (("u", "N1hu"), ("d", "N1hd")):
+                c = e.result(key)
+                sim["NiCh"][k] = {"r": c[0].tolist(), "i": c[1].tolist()}
         if self._basis_index:  # src/FHSolver.cpp:242-265
             M = e.result("basis")
             for kind, (ku, kd) in (("tb", ("Mkku", "Mkkd")), ("mf", ("Mukku", "Mukkd"))):

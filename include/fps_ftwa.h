@@ -70,8 +70,8 @@ extern "C" {
 #define FPSFTWA_OUT_N2IID 4     /* [L][T]         N2iid (:521)                               */
 #define FPSFTWA_OUT_NUND 5      /* [L][T]         NuNd  (:530)                               */
 #define FPSFTWA_OUT_ENERGY 6    /* [5][T]         tH, U, V, pe, tot (:399-412, :543-547)     */
-#define FPSFTWA_OUT_N1HU 7      /* [2][T]         re, im of <N_{0,L-1}> up   (FHSolver.cpp:196) */
-#define FPSFTWA_OUT_N1HD 8      /* [2][T]         re, im of <N_{0,L-1}> down (FHSolver.cpp:197) */
+#define FPSFTWA_OUT_N1HU 7      /* [2][T]         re, im of <N_{0,L-1}> up   (FHSolver.cpp:196), or of the  */
+#define FPSFTWA_OUT_N1HD 8      /* [2][T]         ... down (:197); element set by fpsftwa_set_recorded_element */
 #define FPSFTWA_OUT_NIIU_ERR 9  /* [L][T]         statistical error sigma/sqrt(B) of Niiu    */
 #define FPSFTWA_OUT_NIID_ERR 10 /* [L][T]                                          of Niid    */
 #define FPSFTWA_OUT_NUND_ERR 11 /* [L][T]                                          of NuNd    */
@@ -144,6 +144,11 @@ int fpsftwa_set_initial_state(fpsftwa_handle* h, const double* n_up, const doubl
  * the basis states (fpsftwa_ground_state's layout); n_bases in [0, 2], 0 (or NULL) clears them.  Read
  * the result with FPSFTWA_OUT_BASIS.  Not available with the adjoint. */
 int fpsftwa_set_bases(fpsftwa_handle* h, int32_t n_bases, const double* bases);
+
+/* The off-diagonal element rho_sigma(i, j) whose ensemble mean FPSFTWA_OUT_N1HU / N1HD return.
+ * Default (0, L-1): the "wholeDMatrix" corner (src/FHSolver.cpp:196-197).  A core-hole system
+ * records (site, L-1) instead, its "NiCh" observable (src/FHfTWASolver.cpp:413-417, :548-551). */
+int fpsftwa_set_recorded_element(fpsftwa_handle* h, int32_t i, int32_t j);
 
 /* Oracle mode: use the caller's noise instead of the counter-based generator.
  * noise = [n_traj][2 spins][L(L-1)/2 pairs][re, im], pair p enumerating i<j row by row;

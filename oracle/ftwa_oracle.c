@@ -306,7 +306,7 @@ static void sample_state(int L, const double* nocc, const double* R, const doubl
 
 /* Per time point accumulation (src/FHfTWASolver.cpp:366-418), real parts. */
 static void record(const or_system* s, double t, double t_max, const double* ru, const double* rd, int64_t ti,
-                   int64_t T, double* acc, work* w, int nb, const double* bases)
+                   int64_t T, double* acc, work* w, int nb, const double* bases, int ce)
 {
     const int L = s->L;
     double* su = acc;                      /* [L][T] */
@@ -366,10 +366,10 @@ static void record(const or_system* s, double t, double t_max, const double* ru,
     se[1 * T + ti] += eu;
     se[2 * T + ti] += ev;
     se[3 * T + ti] += epe;
-    sc[0 * T + ti] += ru[2 * (L - 1)];
-    sc[1 * T + ti] += ru[2 * (L - 1) + 1];
-    sc[2 * T + ti] += rd[2 * (L - 1)];
-    sc[3 * T + ti] += rd[2 * (L - 1) + 1];
+    sc[0 * T + ti] += ru[2 * ce];
+    sc[1 * T + ti] += ru[2 * ce + 1];
+    sc[2 * T + ti] += rd[2 * ce];
+    sc[3 * T + ti] += rd[2 * ce + 1];
     /* diagvec(V^dagger rho V), real part (:385-394) */
     double* sb = sc + (size_t)4 * T; /* [nb][2][L][T] */
     for (int b = 0; b < nb; ++b)
@@ -402,6 +402,8 @@ static int run_impl(const or_system* s, const or_params* p, const double* noise,
     const int64_t T = oracle_n_time_points(p->dt, p->t_max);
     const int64_t npairs = (int64_t)L * (L - 1) / 2;
     const int nb = p->bases ? p->n_bases : 0;
+    const int ce = p->corner_j < 0 ? L - 1 : p->corner_i * L + p->corner_j; /* element index i*L + j */
+    if (p->corner_j >= L || p->corner_i < 0 || p->corner_i >= L) return -1;
     const size_t acc_n = (size_t)(5 * L + 8) * T + (size_t)nb * 2 * L * T;
     const double h = p->dt / (double)p->n_sub;
     int nth = 1;
@@ -445,7 +447,7 @@ static int run_impl(const or_system* s, const or_params* p, const double* noise,
             sample_state(L, p->n_dn, p->rot_dn, wb + 2 * npairs, 1, rd, tmp);
             double t = 0.0;
             for (int64_t ti = 0; ti < T; ++ti) {
-                record(s, ti * p->dt, p->t_max, ru, rd, ti, T, acc, &w, nb, p->bases);
+                record(s, ti * p->dt, p->t_max, ru, rd, ti, T, acc, &w, nb, p->bases, ce);
                 if (ti == T - 1) break; /* steps after the last record change no output */
                 for (int ss = 0; ss < p->n_sub; ++ss) {
                     step_rk4_38(s, t, p->t_max, h, ru, rd, &w);

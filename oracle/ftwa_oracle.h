@@ -47,6 +47,9 @@ typedef struct or_params {
     const double* rot_dn;
     int n_bases;          /* bases whose occupations diag(V^dagger rho V) are summed (tbStates / mfStates) */
     const double* bases;  /* [n_bases][2 spins][L][L] complex interleaved, columns the basis states */
+    int corner_i, corner_j; /* element summed into sum_corner: rho(corner_i, corner_j); corner_j < 0 means
+                               (0, L-1), the wholeDMatrix corner (src/FHSolver.cpp:196-197); a core-hole
+                               system sums (site, L-1) (src/FHfTWASolver.cpp:413-417) */
 } or_params;
 
 /* Raw ensemble sums (real parts), each [rows][T], any pointer may be NULL. */
@@ -57,7 +60,7 @@ typedef struct or_out {
     double* sum2_d;     /* [L][T] sum |rho_dn_ii|^2        */
     double* sum_ud;     /* [L][T] sum rho_up_ii rho_dn_ii  */
     double* sum_e;      /* [4][T] tH, U, V, pe             */
-    double* sum_corner; /* [4][T] re/im rho_up(0,L-1), re/im rho_dn(0,L-1) */
+    double* sum_corner; /* [4][T] re/im rho_up(ci,cj), re/im rho_dn(ci,cj) (default (0, L-1)) */
     double* final_states; /* [B][2][L][L][2]               */
     double* sum_basis;  /* [n_bases][2][L][T] sum Re diag(V^dagger rho V) (src/FHfTWASolver.cpp:385-394) */
 } or_out;

@@ -79,6 +79,7 @@ class _OrParams(ctypes.Structure):
         ("n_up", ctypes.POINTER(ctypes.c_double)), ("n_dn", ctypes.POINTER(ctypes.c_double)),
         ("rot_up", ctypes.POINTER(ctypes.c_double)), ("rot_dn", ctypes.POINTER(ctypes.c_double)),
         ("n_bases", ctypes.c_int), ("bases", ctypes.POINTER(ctypes.c_double)),
+        ("corner_i", ctypes.c_int), ("corner_j", ctypes.c_int),
     ]
 
 
@@ -201,11 +202,12 @@ class Sums:
 
 def run(system: System, dt: float, t_max: float, n_sub: int, n_up, n_dn, *, noise_arr=None,
         n_traj: int | None = None, seed: int | None = None, rot_up=None, rot_dn=None,
-        nthreads: int = 0, keep_states: bool = False, bases=None) -> Sums:
+        nthreads: int = 0, keep_states: bool = False, bases=None, corner=None) -> Sums:
     """Run the oracle on explicit noise ([B][2][npairs][2]) or on the seeded generator.
 
     bases: optional [n_bases][2][L][L] (real or complex) basis matrices whose occupations
-    diag(V^dagger rho V) are summed into Sums.sum_basis."""
+    diag(V^dagger rho V) are summed into Sums.sum_basis.  corner: the (i, j) element summed into
+    Sums.sum_corner (default (0, L-1))."""
     lib = _load()
     L = system.L
     T = n_time_points(dt, t_max)
@@ -231,6 +233,7 @@ def run(system: System, dt: float, t_max: float, n_sub: int, n_up, n_dn, *, nois
     p.dt, p.t_max, p.n_sub, p.n_traj = dt, t_max, n_sub, B
     p.n_up, p.n_dn, p.rot_up, p.rot_dn = _ptr(nu), _ptr(nd), _ptr(ru), _ptr(rd)
     p.n_bases, p.bases = nb, _ptr(bs)
+    p.corner_i, p.corner_j = (0, -1) if corner is None else (int(corner[0]), int(corner[1]))
     arrays = {n: np.zeros((L, T)) for n in ("sum_u", "sum_d", "sum2_u", "sum2_d", "sum_ud")}
     arrays["sum_e"] = np.zeros((4, T))
     arrays["sum_corner"] = np.zeros((4, T))

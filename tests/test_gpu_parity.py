@@ -39,6 +39,7 @@ def engine_run(eng, dt, t_max, n_sub, B, engine, seed=0, noise=None, bases=False
     eng.run()
     out = {k: eng.result(k) for k in ("sum_u", "sum_d", "sum2_u", "sum_ud", "sum_e")}
     out["states"] = eng.states()
+    out["corner"] = np.vstack([eng.result("N1hu"), eng.result("N1hd")]) * B  # [re u, im u, re d, im d][T]
     if bases:
         out["sum_basis"] = eng.result("sum_basis")
     return out
@@ -53,11 +54,14 @@ def oracle_system(O, tM, coulomb_type, U=None, V=None, pot=None, pulse=None):
 
 
 def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, pulse=None, dt=0.02, t_max=0.2,
-            n_sub=1, B=6, seed=3, engines=("fp64_plain", "fp64", "fp32"), fp32_tol=2e-5, rot=None, bases=None):
+            n_sub=1, B=6, seed=3, engines=("fp64_plain", "fp64", "fp32"), fp32_tol=2e-5, rot=None, bases=None,
+            corner=None):
     L = tM.shape[0]
     eng = _lib.Engine(tM, coulomb_type=coulomb_type, U=U, V=V, pot=pot, **(pulse or {}))
     if bases is not None:
         eng.set_bases(bases)
+    if corner is not None:
+        eng.set_recorded_element(*corner)
     if rot is None:
         eng.set_initial_state(n_up, n_dn)
     else:
@@ -66,7 +70,7 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
     noise = eng.sample_noise(0, B)
     ref = O.run(oracle_system(O, tM, coulomb_type, U, V, pot, pulse), dt, t_max, n_sub, n_up, n_dn,
                 noise_arr=noise, keep_states=True, rot_up=None if rot is None else rot[0],
-                rot_dn=None if rot is None else rot[1], bases=bases)
+                rot_dn=None if rot is None else rot[1], bases=bases, corner=corner)
     runs = []
     z = int(((tM != 0) & ~np.eye(L, dtype=bool)).sum(axis=1).max())
     stencil = L <= 64 and z <= 8  # where the LDS stencil kernels apply
@@ -92,6 +96,9 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
         assert rel(got["sum_e"], ref.sum_e) < tol, e
         if bases is not None:
             assert rel(got["sum_basis"], ref.sum_basis) < tol, (e, kern)
+        ci, cj = (0, L - 1) if corner is None else corner
+        if ci != cj:  # the recorded off-diagonal element (wholeDMatrix corner / core-hole NiCh)
+            assert np.abs(got["corner"] - ref.sum_corner).max() <= tol * max(np.abs(ref.sum_corner).max(), 1e-3), (e, kern)
     eng.close()
 
 
@@ -333,6 +340,51 @@ def test_chain10_against_exact_diagonalisation(engine):
             dev = np.abs(got - r[spin])
             assert np.all(dev <= 5 * err + 1e-9), (U, spin, float(dev.max()), float(err.max()))
         eng.close()
+
+
+def test_recorded_element(oracle_mod):
+    """The per-time-point off-diagonal element (wholeDMatrix corner, core-hole NiCh) at a chosen (i, j), on
+    every engine and kernel, against the oracle's sums of the same element."""
+    L = 9
+    tM = chain(L) + np.diag(np.linspace(-0.2, 0.3, L))
+    n_up = (np.arange(L) % 2 == 0).astype(float)
+    compare(oracle_mod, tM, n_up, 1 - n_up, U=2.5 * np.eye(L), B=6, t_max=0.3, corner=(2, 5))
+    compare(oracle_mod, tM, n_up, 1 - n_up, U=2.5 * np.eye(L), B=6, t_max=0.3, corner=(7, 1))
+
+
+def test_core_hole_json(tmp_path, oracle_mod):
+    """A core-hole system from a simulation file (src/FHSystem.cpp:295-323): the hole is site L, coupled to
+    its site by X, with potential eps and interactions G (site) and Uc (hole); the results file's NiCh is
+    <rho_sigma(site, L)> (src/FHfTWASolver.cpp:413-417, src/FHSolver.cpp:285-291).  Against the oracle on
+    the front end's own matrices and the generator's noise, 1e-10 relative (FP64)."""
+    O = oracle_mod
+    pos, _ = lat.chain(6)
+    B, dt, t_max, seed = 10, 0.02, 0.4, 8
+    sim = lat.simulation("ch", pos, None, False, U=3.0, dt=dt, t_max=t_max, n_traj=B,
+                         mi355x={"seed": seed, "engine": "fp64"},
+                         results={"diagDMatrix": True, "onSiteDoublons": True, "energies": True})
+    sim["system"]["coreHole"] = {"presence": True, "site": 2, "eps": -2.5, "X": -0.7, "Uc": 1.5, "G": 0.8,
+                                 "IC": [1, 0]}
+    path = lat.write_simulation(str(tmp_path), sim, pos)
+    _lib.run_json_file(path)
+    out = json.load(open(tmp_path / "ch_fTWA_res.json"))["simulation"]
+    d = _lib.load_system_json(path)
+    L = d["L"]
+    assert L == 7 and d["coulomb_type"] == _lib.COULOMB_GENERAL
+    eng = _lib.Engine(d["tM"], coulomb_type=_lib.COULOMB_GENERAL, U=d["U"], V=d["V"], pot=d["pot"])
+    eng.set_initial_state(d["n_up"], d["n_dn"])
+    eng.set_params(dt, t_max, 1, B, seed=seed, engine="fp64")
+    noise = eng.sample_noise(0, B)
+    eng.close()
+    s = O.run(oracle_system(O, d["tM"], 2, d["U"], d["V"], d["pot"]), dt, t_max, 1, d["n_up"], d["n_dn"],
+              noise_arr=noise, corner=(2, L - 1))
+    ref = O.normalise(s, B)
+    for k in ("Niiu", "Niid", "NuNd"):
+        assert rel(np.asarray(out[k]), ref[k]) < FP64_RTOL, k
+    assert rel(np.asarray(out["E"]["tot"]), ref["E"][4]) < FP64_RTOL
+    got = np.array([out["NiCh"]["u"]["r"], out["NiCh"]["u"]["i"], out["NiCh"]["d"]["r"], out["NiCh"]["d"]["i"]])
+    assert np.abs(got - s.sum_corner / B).max() <= 1e-10 * np.abs(s.sum_corner / B).max()
+    assert np.abs(np.asarray(out["NiCh"]["u"]["r"])[-1]) > 1e-4  # the hole is coupled: the element moves
 
 
 def test_few_trajectories_and_single_time_point(oracle_mod):

@@ -303,3 +303,42 @@ def test_eigenbasis_initial_conditions_from_json(tmp_path, oracle_mod):
         _lib.load_initial_state_json(write(tmp_path, lat.simulation(
             "big", pos, None, False, ic={"type": "tbGroundState", "nPartUp": 8}), pos))
     assert "nPartUp" in str(e.value)
+
+
+def test_core_hole_system(tmp_path):
+    """coreHole (src/FHSystem.cpp:295-323, src/FHCoreHole.cpp:10-26): site L added, tM[site, L] = X,
+    U[site, L] = U[L, site] = G/2 and U[L, L] = Uc, V[site, L] = V[L, site] = G/2 (addSite halves the vector
+    into the last row and column, src/FHCoulombInteraction.cpp:224-230), potential eps on the hole alone
+    (the file's potential is replaced), occupations IC = [up, down]; defaults eps -2, X -1, Uc 1, G 1, [1, 0]."""
+    pos, _ = lat.chain(4)
+    sim = lat.simulation("ch", pos, None, False, U=3.0, external={"type": "constant", "diagonal": [0.5, 0, 0, 0]})
+    sim["system"]["coreHole"] = {"presence": True, "site": 1, "eps": -2.5, "X": -0.7, "Uc": 1.5, "G": 0.8,
+                                 "IC": [0.75, 0.25]}
+    d = load(tmp_path, sim, pos)
+    assert d["L"] == 5 and d["coulomb_type"] == _lib.COULOMB_GENERAL
+    tM = np.zeros((5, 5))
+    tM[:4, :4] = -(np.eye(4, k=1) + np.eye(4, k=-1))
+    tM[1, 4] = tM[4, 1] = -0.7
+    np.testing.assert_array_equal(d["tM"], tM)
+    U = np.diag([3.0, 3, 3, 3, 1.5])
+    U[1, 4] = U[4, 1] = 0.4
+    np.testing.assert_allclose(d["U"], U, atol=1e-15)
+    V = np.zeros((5, 5))
+    V[1, 4] = V[4, 1] = 0.4
+    np.testing.assert_allclose(d["V"], V, atol=1e-15)
+    np.testing.assert_array_equal(d["pot"], [0, 0, 0, 0, -2.5])
+    np.testing.assert_array_equal(d["n_up"], [1, 0, 1, 0, 0.75])
+    np.testing.assert_array_equal(d["n_dn"], [0, 1, 0, 1, 0.25])
+    np.testing.assert_array_equal(d["positions"][4], d["positions"][1])
+    sim["system"]["coreHole"] = {"presence": True}
+    sim["system"]["IC"] = {"type": "tbGroundState", "nPartUp": 2, "nPartDown": 2}
+    d = load(tmp_path, sim, pos)
+    assert d["tM"][0, 4] == -1.0 and d["U"][4, 4] == 1.0 and d["U"][0, 4] == 0.5 and d["pot"][4] == -2.0
+    np.testing.assert_array_equal(d["n_up"], [1, 1, 0, 0, 1])
+    np.testing.assert_array_equal(d["n_dn"], [1, 1, 0, 0, 0])
+    ic = _lib.load_initial_state_json(str(tmp_path / "ch.json"))
+    assert ic["rot_up"].shape == (5, 5) and ic["rot_up"][4, 4] == 1 and np.all(ic["rot_up"][4, :4] == 0)
+    sim["system"]["coulombInteraction"] = {"type": "distanceInteraction", "distances": [0.0], "energies": [2.0]}
+    with pytest.raises(_lib.FtwaError) as e:
+        load(tmp_path, sim, pos)
+    assert "core hole" in str(e.value)
