@@ -17,10 +17,12 @@
 //                    written to both triangles (exactly Hermitian)
 //   record kernel    per time point: diagonal, corner, energies (plain-path formulas)
 #include <hip/hip_runtime.h>
+#define ROCBLAS_BETA_FEATURES_API  // rocblas_gemm_ex_get_solutions: the GEMM solutions for the autotuner
 #include <rocblas/rocblas.h>
 
 #include <cstdint>
 #include <type_traits>
+#include <vector>
 
 #include "ftwa_device.hpp"
 #include "ftwa_kernels.hpp"
@@ -519,13 +521,17 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
                                     int ci, int cj)
 {
     __shared__ double ew[4 * 4];
+    extern __shared__ double dg[];  // [2][L] n_sigma,j = rho_sigma,jj + 1/2 (general Coulomb energies)
     const int L = s.L;
     const int64_t b = blockIdx.x;
     Real* rr = rec + b * R;
     for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
         const int sg = e >= L ? 1 : 0, i = e - sg * L;
-        rr[e] = ycur[row_off<Real>(b, sg, i, L) + i];
+        const Real d = ycur[row_off<Real>(b, sg, i, L) + i];
+        rr[e] = d;
+        if (s.coulomb_type == 2) dg[e] = static_cast<double>(d) + 0.5;
     }
+    if (s.coulomb_type == 2) __syncthreads();
     if (threadIdx.x == 0) {
         const size_t ou = row_off<Real>(b, 0, ci, L) + cj, od = row_off<Real>(b, 1, ci, L) + cj;
         rr[2 * L + 4] = ycur[ou];
@@ -562,8 +568,8 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
             eu += static_cast<double>(s.u_diag[i]) * nu * ndn;
         } else if (s.coulomb_type == 2) {
             for (int j = 0; j < L; ++j) {
-                const double nuj = static_cast<double>(ycur[row_off<Real>(b, 0, j, L) + j]) + 0.5;
-                const double ndj = static_cast<double>(ycur[row_off<Real>(b, 1, j, L) + j]) + 0.5;
+                const double nuj = dg[j];
+                const double ndj = dg[L + j];
                 eu += static_cast<double>(s.U[i * L + j]) * (nu * ndj + ndn * nuj) / 2.0;
                 ev += static_cast<double>(s.V[i * L + j]) * (nu * nuj + ndn * ndj) / 2.0;
             }
@@ -610,11 +616,41 @@ rocblas_status gemm(rocblas_handle hb, int m, int n, int k, const double* A, int
                          ldc);
 }
 
+template <typename Real>
+constexpr rocblas_datatype blas_type()
+{
+    return std::is_same<Real, float>::value ? rocblas_datatype_f32_r : rocblas_datatype_f64_r;
+}
+
+// C = A B (column-major, no transposes) with a chosen library solution (0: the library's default);
+// with solution != 0 and list != nullptr only the solution list is returned (list_size in/out).
+template <typename Real>
+rocblas_status gemm_solution(rocblas_handle hb, int solution, int m, int n, int k, const Real* A, int lda,
+                             const Real* B, int ldb, Real* C, int ldc, rocblas_int* list = nullptr,
+                             rocblas_int* list_size = nullptr)
+{
+    const Real one = 1, zero = 0;
+    const rocblas_datatype t = blas_type<Real>();
+    if (list_size)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"  // a beta entry point of the library
+        return rocblas_gemm_ex_get_solutions(hb, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, t,
+                                             lda, B, t, ldb, &zero, C, t, ldc, C, t, ldc, t,
+                                             rocblas_gemm_algo_solution_index, rocblas_gemm_flags_none, list,
+                                             list_size);
+#pragma clang diagnostic pop
+    if (solution == 0) return gemm(hb, m, n, k, A, lda, B, ldb, C, ldc);
+    return rocblas_gemm_ex(hb, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, t, lda, B, t, ldb,
+                           &zero, C, t, ldc, C, t, ldc, t, rocblas_gemm_algo_solution_index, solution,
+                           rocblas_gemm_flags_none);
+}
+
 }  // namespace
 
 struct DenseCtx {
     rocblas_handle blas = nullptr;
     hipStream_t stream = nullptr;
+    int solution = 0;  // GEMM solution index chosen by dense_tune_gemm (0: the library's default)
 };
 
 DenseCtx* dense_create(hipStream_t st)
@@ -634,6 +670,16 @@ void dense_destroy(DenseCtx* c)
     if (!c) return;
     if (c->blas) rocblas_destroy_handle(c->blas);
     delete c;
+}
+
+template <typename Real>
+bool dense_uses_gemm(const DevSystem<Real>& s)
+{
+    // short-range lattices skip the product: X comes from rows of the stage input staged in LDS
+    const int L = s.L;
+    const size_t lds_st = 2 * kTile * 2 * static_cast<size_t>(L) * sizeof(Real) +
+                          static_cast<size_t>(s.z) * L * (2 * sizeof(Real) + sizeof(int));
+    return !(4 * s.z < L && lds_st <= 160 * 1024);
 }
 
 template <typename Real>
@@ -678,10 +724,9 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                            ycur);
     }
     const bool pulse = s.amp != nullptr;
-    // short-range lattices skip the product: X comes from rows of the stage input staged in LDS
     const size_t lds_st = 2 * kTile * 2 * static_cast<size_t>(L) * sizeof(Real) +
                           static_cast<size_t>(s.z) * L * (2 * sizeof(Real) + sizeof(int));
-    const bool stencil = 4 * s.z < L && lds_st <= 160 * 1024;
+    const bool stencil = !dense_uses_gemm(s);  // the LDS stencil form, whose tiles take lds_st bytes
     // the whole spin block in one workgroup's LDS when it fits (upper elements <= 8 per thread)
     const size_t lds_wg = 2 * static_cast<size_t>(L) * L * sizeof(Real) +
                           static_cast<size_t>(s.z) * L * (2 * sizeof(Real) + sizeof(int));
@@ -725,7 +770,8 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     Real* yin = ycur;
     Real* yout = X;
     for (int t = t0; t < t1; ++t) {
-        hipLaunchKernelGGL(dense_record_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256), 0, st, s,
+        hipLaunchKernelGGL(dense_record_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256),
+                           s.coulomb_type == 2 ? 2 * L * sizeof(double) : 0, st, s,
                            static_cast<int64_t>(t) * r.nsub * 4, ycur, rec + static_cast<int64_t>(t - t0) * B * r.R,
                            r.R, r.dens, r.ci, r.cj);
         if (t == r.T - 1) break;
@@ -767,9 +813,11 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                 rocblas_status bs;
                 if (!pulse) {
                     // rows (b, s, i, plane) of length L:  X = Z T, row-major -> column-major X^T = T^T Z^T
-                    bs = gemm(c->blas, L, static_cast<int>(4 * B * L), L, M, L, ycur, L, X, L);
+                    bs = gemm_solution<Real>(c->blas, c->solution, L, static_cast<int>(4 * B * L), L, M, L, ycur, L, X,
+                                             L);
                 } else {
-                    bs = gemm(c->blas, 2 * L, static_cast<int>(2 * B * L), 2 * L, M, 2 * L, ycur, 2 * L, X, 2 * L);
+                    bs = gemm_solution<Real>(c->blas, c->solution, 2 * L, static_cast<int>(2 * B * L), 2 * L, M, 2 * L,
+                                             ycur, 2 * L, X, 2 * L);
                 }
                 if (bs != rocblas_status_success) return hipErrorLaunchFailure;
                 switch (stg) {
@@ -870,10 +918,74 @@ hipError_t basis_project(DenseCtx* c, int L, int nt, int t0, int T, int nb, cons
 }  // namespace fpsamd
 
 namespace fpsamd {
+
+template <typename Real>
+hipError_t dense_tune_gemm(DenseCtx* c, const DevSystem<Real>& s, int64_t B, Real* buf, int* best_solution,
+                           double* best_ms, int* n_tried)
+{
+    // the stage product of dense_run for this batch, on the buffers a previous run left: every
+    // solution the library offers for the shape is timed (one warm-up call, then three), and the
+    // fastest is kept for later runs of this handle
+    const int L = s.L;
+    const size_t per = 4 * static_cast<size_t>(L) * L * static_cast<size_t>(B);
+    Real* ycur = buf + 3 * per;
+    Real* X = ycur + per;
+    Real* M = X + per;
+    const bool pulse = s.amp != nullptr;
+    const int m = pulse ? 2 * L : L, k = m;
+    const int n = static_cast<int>(pulse ? 2 * B * L : 4 * B * L);
+    rocblas_int count = 0;
+    if (gemm_solution<Real>(c->blas, 1, m, n, k, M, m, ycur, k, X, m, nullptr, &count) != rocblas_status_success)
+        count = 0;
+    std::vector<rocblas_int> list(static_cast<size_t>(count));
+    if (count > 0 && gemm_solution<Real>(c->blas, 1, m, n, k, M, m, ycur, k, X, m, list.data(), &count) !=
+                         rocblas_status_success)
+        count = 0;
+    list.resize(static_cast<size_t>(count));
+    list.insert(list.begin(), 0);  // the default first: a solution must beat it
+    hipEvent_t e0, e1;
+    hipError_t he = hipEventCreate(&e0);
+    if (he != hipSuccess) return he;
+    he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return he;
+    }
+    int best = 0, tried = 0;
+    double best_t = -1.0;
+    for (rocblas_int sol : list) {
+        if (gemm_solution<Real>(c->blas, sol, m, n, k, M, m, ycur, k, X, m) != rocblas_status_success) continue;
+        (void)hipEventRecord(e0, c->stream);
+        bool ok = true;
+        for (int rep = 0; rep < 3 && ok; ++rep)
+            ok = gemm_solution<Real>(c->blas, sol, m, n, k, M, m, ycur, k, X, m) == rocblas_status_success;
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess || !ok) continue;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ++tried;
+        if (best_t < 0.0 || ms / 3.0 < best_t) {
+            best_t = ms / 3.0;
+            best = sol;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->solution = best;
+    if (best_solution) *best_solution = best;
+    if (best_ms) *best_ms = best_t;
+    if (n_tried) *n_tried = tried;
+    return hipGetLastError();
+}
+
 template size_t dense_buffer_elems<float>(int, int64_t);
 template size_t dense_buffer_elems<double>(int, int64_t);
 template hipError_t dense_run<float>(DenseCtx*, const DevSystem<float>&, const RunShape&, float*, float*, float*, int,
                                      int, bool, bool);
 template hipError_t dense_run<double>(DenseCtx*, const DevSystem<double>&, const RunShape&, double*, double*, double*,
                                       int, int, bool, bool);
+template hipError_t dense_tune_gemm<float>(DenseCtx*, const DevSystem<float>&, int64_t, float*, int*, double*, int*);
+template bool dense_uses_gemm<float>(const DevSystem<float>&);
+template bool dense_uses_gemm<double>(const DevSystem<double>&);
+template hipError_t dense_tune_gemm<double>(DenseCtx*, const DevSystem<double>&, int64_t, double*, int*, double*, int*);
 }  // namespace fpsamd

@@ -153,6 +153,16 @@ template <typename Real>
 hipError_t launch_density_reduce(int L, int64_t B, int R, int nt, const Real* rec, double* part, int nchunk,
                                  double* S, hipStream_t st);
 
+// Whether dense_run takes the long-range form (one GEMM per stage) rather than the LDS stencil form.
+template <typename Real>
+bool dense_uses_gemm(const DevSystem<Real>& s);
+
+// Times every library solution of the dense path's stage GEMM for this batch (buffers of a previous
+// dense_run) and keeps the fastest for the context's later runs; uses the long-range (GEMM) form.
+template <typename Real>
+hipError_t dense_tune_gemm(DenseCtx* c, const DevSystem<Real>& s, int64_t B, Real* buf, int* best_solution,
+                           double* best_ms, int* n_tried);
+
 // Occupations of nb fixed bases, M[b][sg][k] = sum_ij V_ik V_jk S_sg(i, j) = diag(V^T S V) with V =
 // bases[b][sg] (real [L][L], columns the basis states), for nt time points of density sums S (as
 // above), written to out[((b * 2 + sg) * L + k) * T + t0 + t].  work holds basis_work_doubles(L, nt).

@@ -571,3 +571,36 @@ def test_autotune_picks_a_kernel_that_reproduces_the_run():
     e.run()
     assert rel(e.result("sum_u"), ref) < FP64_RTOL
     e.close()
+
+
+@pytest.mark.parametrize("engine", ["fp64", "fp32"])
+def test_autotune_gemm_solution_on_the_long_range_path(oracle_mod, engine):
+    """Long-range hopping with a pulse takes the dense path's GEMM form; the autotuner also picks the
+    library's GEMM solution there.  The tuned run still matches the oracle on identical samples
+    (1e-10 FP64, 2e-5 FP32)."""
+    O = oracle_mod
+    pos, _ = lat.grid(6, 6)
+    L = pos.shape[0]
+    d = pos[:, None, :] - pos[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    tM = np.where((r > 0) & (r <= 3.0), -np.exp(-(r - 1.0) / 0.3), 0.0)
+    U = 3.0 * np.eye(L)
+    pulse = dict(pulse_type=1, A0=0.5, t0=0.05, td=0.1, w0=1.5, phi=0.0, pol=(2 ** -0.5, 2 ** -0.5, 0.0), disp=d)
+    n_up = np.array([1.0, 0.0] * (L // 2))
+    B, dt, t_max = 64, 0.01, 0.1
+    e = _lib.Engine(tM, U=U, **pulse)
+    e.set_initial_state(n_up, 1 - n_up)
+    e.set_params(dt, t_max, 1, B, seed=21, engine=engine)
+    noise = e.sample_noise(0, B)
+    e.set_tuning(0, 0, "dense")
+    choice = e.autotune(trial_steps=3)
+    assert choice["kernel"] == "dense" and choice["traj_steps_per_s"] > 0
+    e.set_params(dt, t_max, 1, B, seed=21, engine=engine)
+    e.set_noise(noise)
+    e.run()
+    ref = O.run(oracle_system(O, tM, 1, U, pulse=pulse), dt, t_max, 1, n_up, 1 - n_up, noise_arr=noise,
+                keep_states=True)
+    tol = FP64_RTOL if engine == "fp64" else 2e-5
+    assert rel(e.states(), ref.final_states) < tol
+    assert rel(e.result("sum_u"), ref.sum_u) < tol
+    e.close()
