@@ -18,6 +18,7 @@ EINVAL, EDEVICE, ENOMEM, ESTATE, EUNSUPPORTED, EIO = -1, -2, -3, -4, -5, -6
 
 COULOMB_NULL, COULOMB_ONSITE, COULOMB_GENERAL = 0, 1, 2
 PULSE_NONE, PULSE_GAUSSIAN, PULSE_KNOTS = 0, 1, 2
+METHOD_FTWA, METHOD_HF = 0, 1
 ENGINE_FP64_PLAIN, ENGINE_FP64_BATCHED, ENGINE_FP32_BATCHED = 0, 1, 2
 ENGINES = {"fp64_plain": ENGINE_FP64_PLAIN, "fp64": ENGINE_FP64_BATCHED, "fp32": ENGINE_FP32_BATCHED}
 KERNEL_AUTO, KERNEL_FULL, KERNEL_HERMITIAN, KERNEL_DENSE, KERNEL_PIPELINED = 0, 1, 2, 3, 4
@@ -40,7 +41,7 @@ EXPORTED = [
     "fpsftwa_sample_noise", "fpsftwa_run", "fpsftwa_get_result", "fpsftwa_get_states", "fpsftwa_get_timing",
     "fpsftwa_set_tuning", "fpsftwa_run_json_file", "fpsftwa_load_system_json", "fpsftwa_run_adjoint",
     "fpsftwa_autotune", "fpsftwa_run_many", "fpsftwa_load_initial_state_json", "fpsftwa_ground_state",
-    "fpsftwa_set_bases", "fpsftwa_set_recorded_element",
+    "fpsftwa_set_bases", "fpsftwa_set_recorded_element", "fpsftwa_set_method",
 ]
 
 
@@ -113,6 +114,7 @@ def load(path: str = LIB_PATH):
         "fpsftwa_ground_state": ([i32, P(d), P(d), i32, i32, i32, P(d), P(d), P(i32), P(d)], ctypes.c_int),
         "fpsftwa_set_bases": ([H, i32, P(d)], ctypes.c_int),
         "fpsftwa_set_recorded_element": ([H, i32, i32], ctypes.c_int),
+        "fpsftwa_set_method": ([H, i32], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -313,6 +315,11 @@ class Engine:
         if name in ("basis", "sum_basis"):  # [n_bases][2][L][T]
             return out.reshape(-1, 2, self.L, T)
         return out.reshape(-1, T)
+
+    def set_method(self, method: str):
+        """"ftwa" (default) or "hf": time-dependent Hartree-Fock (one deterministic state, Fock exchange)."""
+        check(self._lib.fpsftwa_set_method(self._h, {"ftwa": METHOD_FTWA, "hf": METHOD_HF}[method.lower()]),
+              "set_method")
 
     def set_recorded_element(self, i: int, j: int):
         """The element rho(i, j) behind result("N1hu") / result("N1hd") (default (0, L-1))."""

@@ -92,6 +92,7 @@ struct fpsftwa_handle {
     int n_bases = 0;
     std::vector<double> bases;  // [n_bases][2][L][L] real
     int rec_i = 0, rec_j = -1;  // recorded off-diagonal element (j < 0: L - 1)
+    int method = FPSFTWA_METHOD_FTWA;
     std::vector<double> noise;
     int nt_hint = 0;
     int time_chunk = 0;
@@ -251,6 +252,7 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
     ds.pd = reinterpret_cast<const Real*>(base + o_pdd);
     ds.amp = amp.empty() ? nullptr : reinterpret_cast<const double*>(base + o_amp);
     ds.rows = rows.empty() ? nullptr : base + o_rows;
+    ds.fock = (h->method == FPSFTWA_METHOD_HF && h->coulomb_type == FPSFTWA_COULOMB_GENERAL) ? 1 : 0;
     return FPSFTWA_OK;
 }
 
@@ -288,8 +290,14 @@ int sample_states(fpsftwa_handle* h, const DevSystem<Real>& ds)
     const size_t NE = 2 * (size_t)L * L;
     HIP_TRY(h->state.ensure((size_t)h->B * NE * 2 * sizeof(Real)));
     const double* occ = h->occ_d.as<double>();
-    HIP_TRY(launch_sample<Real>(ds, occ, occ + L, h->noise.empty() ? nullptr : h->noise_d.as<double>(), h->seed, 0, h->B,
-                                h->state.as<Real>(), h->stream));
+    const double* noise = h->noise.empty() ? nullptr : h->noise_d.as<double>();
+    if (h->method == FPSFTWA_METHOD_HF) {  // the state itself, no fluctuations: zero noise
+        const size_t nz = (size_t)h->B * 2 * ((size_t)L * (L - 1) / 2) * 2;
+        HIP_TRY(h->noise_d.ensure(std::max<size_t>(nz, 1) * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(h->noise_d.p, 0, std::max<size_t>(nz, 1) * sizeof(double), h->stream));
+        noise = h->noise_d.as<double>();
+    }
+    HIP_TRY(launch_sample<Real>(ds, occ, occ + L, noise, h->seed, 0, h->B, h->state.as<Real>(), h->stream));
     if (!h->rot_up.empty()) {
         HIP_TRY(h->scratch.ensure((size_t)h->B * NE * 2 * sizeof(Real)));
         HIP_TRY(launch_rotate<Real>(L, h->rot_d.as<double>(), h->B, h->state.as<Real>(), h->scratch.as<Real>(),
@@ -785,6 +793,15 @@ int fpsftwa_set_bases(fpsftwa_handle* h, int32_t n_bases, const double* bases)
     return FPSFTWA_OK;
 }
 
+int fpsftwa_set_method(fpsftwa_handle* h, int32_t method)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (method != FPSFTWA_METHOD_FTWA && method != FPSFTWA_METHOD_HF) return fail(FPSFTWA_EINVAL, "unknown method");
+    h->method = method;
+    h->has_result = false;
+    return FPSFTWA_OK;
+}
+
 int fpsftwa_set_recorded_element(fpsftwa_handle* h, int32_t i, int32_t j)
 {
     if (int rc = check_handle(h)) return rc;
@@ -833,6 +850,10 @@ int fpsftwa_run(fpsftwa_handle* h)
         const size_t need = (size_t)h->B * 2 * ((size_t)h->L * (h->L - 1) / 2) * 2;
         if (h->noise.size() != need) return fail(FPSFTWA_EINVAL, "noise size does not match n_traj");
     }
+    if (h->method == FPSFTWA_METHOD_HF && h->coulomb_type == FPSFTWA_COULOMB_GENERAL &&
+        h->engine != FPSFTWA_ENGINE_FP64_PLAIN)
+        return fail(FPSFTWA_EUNSUPPORTED, "Hartree-Fock with a V interaction (the Fock term) runs on the plain FP64 "
+                                          "engine only");
     h->has_result = false;
     h->T = (int)std::floor(h->t_max / h->dt + 1.0);  // src/FHSolver.cpp:100
     const int Q = 6 * h->L + 12;

@@ -252,6 +252,16 @@ __global__ void __launch_bounds__(kPlainThreads)
                     if (general) {
                         const double v = s.V[i * L + jj];
                         ev += v * ((nu * nuj - nui * nujj) + (nd * ndj - ndi * ndjj)) / 2.0;
+                        if (s.fock) {  // - Re sum V o N o N^T / 2 per spin (src/FHHFSolver.cpp:356-357)
+                            for (int sg = 0; sg < 2; ++sg) {
+                                const double* Ym = y + 2 * sg * LL;
+                                const double ar = Ym[2 * (i * L + jj)] + (i == jj ? 0.5 : 0.0);
+                                const double ai = Ym[2 * (i * L + jj) + 1];
+                                const double br = Ym[2 * (jj * L + i)] + (i == jj ? 0.5 : 0.0);
+                                const double bi = Ym[2 * (jj * L + i) + 1];
+                                ev -= v * (ar * br - ai * bi) / 2.0;
+                            }
+                        }
                     }
                 }
             }
@@ -310,6 +320,11 @@ __global__ void __launch_bounds__(kPlainThreads)
                     sincos(a * s.pd[ij], &sn, &cs);
                     dr += tt * cs;
                     di += tt * sn;
+                }
+                if (s.fock) {  // Fock exchange -N_sigma o V (src/FHHFSolver.cpp:587-588), N = rho + I/2
+                    const double v = s.V[ij];
+                    dr -= v * (Yin[2 * e] + (i == jj ? 0.5 : 0.0));
+                    di -= v * Yin[2 * e + 1];
                 }
                 D[2 * e] = dr;
                 D[2 * e + 1] = di;

@@ -50,6 +50,7 @@ struct DevSystem {
     const Real* tM = nullptr;      // [L][L]  dense hopping (plain path)
     const Real* pd = nullptr;      // [L][L]  pol . (r_i - r_j) dense (plain path, pulse)
     const void* rows = nullptr;    // [L] RowNbrs<Real, z> (batched path)
+    int fock = 0;                  // 1: Hartree-Fock dynamics, Delta_sigma gains -V o N_sigma (plain path)
 };
 
 struct RunShape {

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <vector>
 
 #include "../../include/fps_ftwa_solver.hpp"
@@ -53,6 +54,21 @@ const char* engineName(int e)
 
 }  // namespace
 
+FHHFSolver::FHHFSolver(FHSystem* fhSystem) : FHfTWASolver(fhSystem)
+{
+    method_ = FPSFTWA_METHOD_HF;
+    nTraj_ = 1;
+    engine_ = FPSFTWA_ENGINE_FP64_PLAIN;
+}
+
+void FHHFSolver::setMethodParameters(const Value& p)
+{
+    // defaults of src/FHHFSolver.cpp:47-57; RK4 "3/8" is forced in initiateSolver (:199)
+    setTimeParameters(p.get("t_max", Value(2.0)).asDouble(), p.get("dt", Value(0.01)).asDouble(),
+                      static_cast<unsigned>(std::max<int64_t>(1, p.get("stepsParTimePoints", Value(1)).asInt())));
+    setIntegratorOrder(static_cast<int>(p.get("int_order", Value(2)).asInt()));
+}
+
 FHSystem::FHSystem(const Value& systemJson, const std::string& folderName)
 {
     std::string err;
@@ -69,7 +85,7 @@ FHfTWASolver::FHfTWASolver(FHfTWASolver&& o) noexcept
       workName_(std::move(o.workName_)), folderName_(std::move(o.folderName_)), writeJson_(o.writeJson_),
       talkative_(o.talkative_), initiated_(o.initiated_), tMax_(o.tMax_), dt_(o.dt_), nSub_(o.nSub_), nTraj_(o.nTraj_),
       flags_(o.flags_), seed_(o.seed_), engine_(o.engine_), saveTB_(o.saveTB_), saveMF_(o.saveMF_),
-      basisTB_(o.basisTB_), basisMF_(o.basisMF_)
+      basisTB_(o.basisTB_), basisMF_(o.basisMF_), method_(o.method_)
 {
     o.h_ = nullptr;
 }
@@ -176,6 +192,7 @@ void FHfTWASolver::initiateSolver()
         d.displacement = s.displacement.data();
         check(fpsftwa_create(&d, &h_), "fpsftwa_create");
     }
+    check(fpsftwa_set_method(h_, method_), "method");
     check(fpsftwa_set_initial_state(h_, s.n_up.data(), s.n_dn.data(), s.rot_up.empty() ? nullptr : s.rot_up.data(),
                                     s.rot_dn.empty() ? nullptr : s.rot_dn.data()),
           "initial state");
@@ -306,6 +323,7 @@ void FHfTWASolver::runSimulation()
     fpsftwa_get_timing(h_, &launches, &kms, &rms);
     info_ = Value::object();
     info_["engine"] = engineName(engine_);
+    if (method_ == FPSFTWA_METHOD_HF) info_["method"] = "HF (one deterministic state, Fock exchange with V)";
     info_["seed"] = static_cast<int64_t>(seed_);
     info_["kernel_ms"] = kms;
     info_["run_ms"] = rms;
@@ -371,14 +389,16 @@ extern "C" int fpsftwa_run_json_file(const char* path, const char* out_path)
         splitPath(path, folder, file);
         const std::string simName = root.get("sim_name", Value("defaultSimName")).asString();
         const std::string method = root.get("method", Value("HF")).asString();
-        if (method != "fTWA") {
-            fpsamd::set_last_error("method '" + method + "' is not provided by this engine (only fTWA)");
+        if (method != "fTWA" && method != "HF") {
+            fpsamd::set_last_error("method '" + method + "' is not provided by this engine (fTWA and HF are)");
             return FPSFTWA_EUNSUPPORTED;
         }
         FHSystem sys(root["system"], folder);
         for (const auto& w : sys.data().warnings) std::cout << "   -- note: " << w << std::endl;
-        FHfTWASolver solver(&sys);
-        std::string work = simName + "_fTWA_res";
+        // fps.cpp:88-116: the HF and fTWA branches differ only in the solver class and the file suffix
+        std::unique_ptr<FHfTWASolver> owner(method == "HF" ? new FHHFSolver(&sys) : new FHfTWASolver(&sys));
+        FHfTWASolver& solver = *owner;
+        std::string work = simName + (method == "HF" ? "_HF_res" : "_fTWA_res");
         std::string outFolder = folder;
         if (out_path) {
             std::string f2, w2;
@@ -390,7 +410,8 @@ extern "C" int fpsftwa_run_json_file(const char* path, const char* out_path)
         solver.setJsonFile(root);
         solver.setResultsWriter("json");
         solver.setResultsToSave(root["results"]);
-        solver.setMethodParameters(root["parameters"]);
+        if (method == "HF") static_cast<FHHFSolver&>(solver).setMethodParameters(root["parameters"]);
+        else solver.setMethodParameters(root["parameters"]);
         solver.initiateSolver();
         solver.runSimulation();
     } catch (const std::exception& e) {

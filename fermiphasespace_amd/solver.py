@@ -140,10 +140,13 @@ class FHfTWASolver:
         self.jsonFile = dict(file)
 
     # ---- run ----
+    method = "ftwa"
+
     def initiateSolver(self) -> None:
         s = self.system
         if self._engine is None:
             self._engine = _lib.Engine(s.tM, coulomb_type=s.coulomb_type, U=s.U, V=s.V, pot=s.pot, **s.pulse)
+        self._engine.set_method(self.method)
         L = s.L
         nu = np.zeros(L) if s.n_up is None else s.n_up
         nd = np.zeros(L) if s.n_dn is None else s.n_dn
@@ -213,3 +216,20 @@ class FHfTWASolver:
             with open(os.path.join(self.folderName, self.workName + ".json"), "w") as fh:
                 json.dump(doc, fh, indent="\t")
         return sim
+
+
+class FHHFSolver(FHfTWASolver):
+    """Time-dependent Hartree-Fock (include/FHHFSolver.h, src/FHHFSolver.cpp): one deterministic density
+    matrix, Fock exchange with V (:575-608), on the plain FP64 engine; results file `<sim>_HF_res`."""
+
+    method = "hf"
+
+    def __init__(self, fh_system: FHSystem):
+        super().__init__(fh_system)
+        self.nTraj = 1
+        self.engine = "fp64_plain"
+
+    def setMethodParameters(self, param: dict) -> None:
+        """dt, t_max, stepsParTimePoints, int_order with the defaults of src/FHHFSolver.cpp:47-57."""
+        self.setTimeParameters(param.get("t_max", 2.0), param.get("dt", 0.01), param.get("stepsParTimePoints", 1))
+        self.setIntegratorOrder(param.get("int_order", 2))

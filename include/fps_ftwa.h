@@ -55,6 +55,11 @@ extern "C" {
 #define FPSFTWA_ENGINE_FP64_BATCHED 1 /* LDS sparse-stencil kernel, FP64 trajectories                     */
 #define FPSFTWA_ENGINE_FP32_BATCHED 2 /* LDS sparse-stencil kernel, FP32 trajectories, FP64 ensemble sums */
 
+/* ---- dynamics (fpsftwa_set_method) ------------------------------------------------ */
+#define FPSFTWA_METHOD_FTWA 0 /* fTWA: sampled initial states, Hartree mean field (FHfTWASolver)       */
+#define FPSFTWA_METHOD_HF 1   /* time-dependent Hartree-Fock: one deterministic state, Fock exchange  */
+                              /* with the V interaction (FHHFSolver, src/FHHFSolver.cpp:575-608)       */
+
 /* ---- results to save (FHSolver::setResultsToSave, src/FHSolver.cpp:34-45) -------- */
 #define FPSFTWA_RES_DIAG 0x01     /* "diagDMatrix"   -> Niiu, Niid          */
 #define FPSFTWA_RES_DIAG2 0x02    /* "diag2DMatrix"  -> N2iiu, N2iid        */
@@ -145,6 +150,14 @@ int fpsftwa_set_initial_state(fpsftwa_handle* h, const double* n_up, const doubl
  * the result with FPSFTWA_OUT_BASIS.  Not available with the adjoint. */
 int fpsftwa_set_bases(fpsftwa_handle* h, int32_t n_bases, const double* bases);
 
+/* FPSFTWA_METHOD_FTWA (default) or FPSFTWA_METHOD_HF.  HF follows FHHFSolver (src/FHHFSolver.cpp:
+ * 190-420): the state is the density matrix N = R diag(n) R^dagger itself (no sampling; every
+ * trajectory is the same, so n_traj 1 is the natural choice), Delta_sigma = tM + diag(U N_sbar) +
+ * diag(V N_sigma) - N_sigma o V + extPot (:575-608), the same RK4 "3/8" steps, and E_V includes the
+ * exchange energy -sum V o N o N^T / 2 (:356-357).  With a general (V) interaction the Fock term
+ * needs the plain FP64 engine; with null or on-site interactions every engine runs HF. */
+int fpsftwa_set_method(fpsftwa_handle* h, int32_t method);
+
 /* The off-diagonal element rho_sigma(i, j) whose ensemble mean FPSFTWA_OUT_N1HU / N1HD return.
  * Default (0, L-1): the "wholeDMatrix" corner (src/FHSolver.cpp:196-197).  A core-hole system
  * records (site, L-1) instead, its "NiCh" observable (src/FHfTWASolver.cpp:413-417, :548-551). */
@@ -225,9 +238,9 @@ typedef struct fpsftwa_adjoint_spec {
 int fpsftwa_run_adjoint(fpsftwa_handle* h, const fpsftwa_adjoint_spec* spec, double* objective, double* grad_u,
                         double* grad_t, double* grad_pulse, size_t n_pulse);
 
-/* fps.cpp:88-207 for "method":"fTWA": read <path>, build the system, run, and write
- * <folder><sim_name>_fTWA_res.json (the reference's results format, src/FHSolver.cpp:171-307).
- * out_path overrides the output file name when not NULL. */
+/* fps.cpp:88-207 for "method":"fTWA" (and "HF", :88-101): read <path>, build the system, run, and
+ * write <folder><sim_name>_fTWA_res.json (_HF_res.json) in the reference's results format
+ * (src/FHSolver.cpp:171-307).  out_path overrides the output file name when not NULL. */
 int fpsftwa_run_json_file(const char* path, const char* out_path);
 
 /* The system and initial occupations the JSON front end builds from a simulation file

@@ -63,7 +63,7 @@ public:
     fpsjson::Value resultsDocument() const;
     fpsftwa_handle* handle() const { return h_; }
 
-private:
+protected:
     void check(int rc, const char* what) const;
     void writeResults();
 
@@ -82,4 +82,14 @@ private:
     int engine_ = FPSFTWA_ENGINE_FP64_BATCHED;
     bool saveTB_ = false, saveMF_ = false;  // "tbStates" / "mfStates" (src/FHSolver.cpp:41-42)
     int basisTB_ = -1, basisMF_ = -1;       // their index among the engine's bases
+    int method_ = FPSFTWA_METHOD_FTWA;
+};
+
+// FHHFSolver (include/FHHFSolver.h, src/FHHFSolver.cpp): time-dependent Hartree-Fock, one
+// deterministic density matrix with the Fock exchange of the V interaction, on the plain FP64
+// engine (one state, so none of the batched engines' parallelism applies).
+class FHHFSolver : public FHfTWASolver {
+public:
+    explicit FHHFSolver(FHSystem* fhSystem);
+    void setMethodParameters(const fpsjson::Value& p);  // src/FHHFSolver.cpp:47-57 (dt, t_max, stepsParTimePoints, int_order)
 };

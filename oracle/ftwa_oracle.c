@@ -154,6 +154,13 @@ static void build_delta(const or_system* s, const double* rho_same, const double
                 D[2 * (i * L + i)] += hr;
                 D[2 * (i * L + i) + 1] += hi;
             }
+            if (s->fock) /* - N_same o V, N = rho + I/2 */
+                for (int i = 0; i < L; ++i)
+                    for (int j = 0; j < L; ++j) {
+                        double v = s->V[i * L + j];
+                        D[2 * (i * L + j)] -= v * (rho_same[2 * (i * L + j)] + (i == j ? 0.5 : 0.0));
+                        D[2 * (i * L + j) + 1] -= v * rho_same[2 * (i * L + j) + 1];
+                    }
         }
     }
     if (s->pot)
@@ -346,6 +353,13 @@ static void record(const or_system* s, double t, double t_max, const double* ru,
                 if (s->coulomb_type == 2) {
                     double v = s->V[i * L + j];
                     ev += v * ((nui * nuj - nuii * nujj) + (ndi * ndj - ndii * ndjj)) / 2.0;
+                    if (s->fock) /* - Re sum V o N o N^T / 2 per spin */
+                        for (int sg = 0; sg < 2; ++sg) {
+                            const double* r = sg ? rd : ru;
+                            double ar = r[2 * (i * L + j)] + (i == j ? 0.5 : 0.0), ai = r[2 * (i * L + j) + 1];
+                            double br = r[2 * (j * L + i)] + (i == j ? 0.5 : 0.0), bi = r[2 * (j * L + i) + 1];
+                            ev -= v * (ar * br - ai * bi) / 2.0;
+                        }
                 }
             }
     }

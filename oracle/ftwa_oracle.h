@@ -35,6 +35,8 @@ typedef struct or_system {
     int n_knots;
     const double* knots;
     const double* disp;  /* [L][L][3] r_i - r_j                                   */
+    int fock;            /* 1: Hartree-Fock, Delta_s gains -N_s o V and E_V its exchange term
+                            (FHHFSolver::derivative, src/FHHFSolver.cpp:575-608, energies :356-357) */
 } or_system;
 
 typedef struct or_params {

@@ -69,6 +69,7 @@ class _OrSystem(ctypes.Structure):
         ("n_knots", ctypes.c_int),
         ("knots", ctypes.POINTER(ctypes.c_double)),
         ("disp", ctypes.POINTER(ctypes.c_double)),
+        ("fock", ctypes.c_int),
     ]
 
 
@@ -113,6 +114,7 @@ class System:
     pol: tuple = (0.0, 0.0, 0.0)
     knots: np.ndarray | None = None
     disp: np.ndarray | None = None
+    fock: bool = False  # Hartree-Fock dynamics (exchange with V), with zero noise: the HF solver
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -136,6 +138,7 @@ class System:
         s.L = L
         s.tM = _ptr(arrs["tM"])
         s.coulomb_type = int(self.coulomb_type)
+        s.fock = 1 if self.fock else 0
         s.U = _ptr(arrs["U"])
         s.V = _ptr(arrs["V"])
         s.pot = _ptr(arrs["pot"])

@@ -387,6 +387,70 @@ def test_core_hole_json(tmp_path, oracle_mod):
     assert np.abs(np.asarray(out["NiCh"]["u"]["r"])[-1]) > 1e-4  # the hole is coupled: the element moves
 
 
+def _hf_system(L=6):
+    tM = chain(L) + np.diag(np.linspace(-0.2, 0.3, L))
+    dist = np.abs(np.arange(L)[:, None] - np.arange(L)[None, :]).astype(float)
+    V = np.where(dist > 0, 1.2 / (1.0 + dist), 0.0)
+    return tM, 2.5 * np.eye(L) + V, V
+
+
+def test_hartree_fock_engine(oracle_mod):
+    """Time-dependent Hartree-Fock (src/FHHFSolver.cpp:575-608): with a V interaction the Fock exchange
+    runs on the plain FP64 engine and matches the oracle (fock, zero noise) state and sums to 1e-10; with
+    an on-site U (no exchange) every engine runs HF and matches too; a batched engine refuses the Fock term."""
+    O = oracle_mod
+    L, B, dt, t_max = 6, 1, 0.02, 0.4
+    tM, U, V = _hf_system(L)
+    n_up = np.array([1.0, 0, 1, 1, 0, 0])
+    n_dn = np.array([0.0, 1, 1, 0, 1, 0])
+    zero = np.zeros((B, 2, L * (L - 1) // 2, 2))
+    for coulomb, fock_ok_engines in ((2, ("fp64_plain",)), (1, ("fp64_plain", "fp64", "fp32"))):
+        Um = U if coulomb == 2 else 2.5 * np.eye(L)
+        eng = _lib.Engine(tM, coulomb_type=coulomb, U=Um, V=V if coulomb == 2 else None)
+        eng.set_method("hf")
+        eng.set_initial_state(n_up, n_dn)
+        ref = O.run(O.System(tM=tM, coulomb_type=coulomb, U=Um, V=V if coulomb == 2 else None, fock=True), dt, t_max,
+                    1, n_up, n_dn, noise_arr=zero, keep_states=True)
+        for engine in fock_ok_engines:
+            got = engine_run(eng, dt, t_max, 1, B, engine, seed=5)
+            tol = 2e-5 if engine == "fp32" else FP64_RTOL
+            assert rel(got["states"], ref.final_states) < tol, (coulomb, engine)
+            assert rel(got["sum_u"], ref.sum_u) < tol and rel(got["sum_e"], ref.sum_e) < tol, (coulomb, engine)
+        if coulomb == 2:
+            eng.set_params(dt, t_max, 1, B, seed=5, engine="fp64")
+            with pytest.raises(_lib.FtwaError) as e:
+                eng.run()
+            assert e.value.code == _lib.EUNSUPPORTED
+        eng.close()
+
+
+def test_hartree_fock_json(tmp_path, oracle_mod):
+    """"method": "HF" in a simulation file (fps.cpp:88-101): <sim>_HF_res.json with the HF observables of
+    the single state, against the oracle's Hartree-Fock run on the front end's matrices (1e-10)."""
+    O = oracle_mod
+    pos, _ = lat.chain(6)
+    dt, t_max = 0.02, 0.4
+    sim = lat.simulation("hf", pos, None, False, U=2.5, dt=dt, t_max=t_max,
+                         coulomb={"type": "distanceInteraction", "distances": [0.0, 1.0, 2.0],
+                                  "energies": [2.5, 0.6, 0.3]},
+                         results={"diagDMatrix": True, "onSiteDoublons": True, "energies": True})
+    sim["method"] = "HF"
+    path = lat.write_simulation(str(tmp_path), sim, pos)
+    _lib.run_json_file(path)
+    out = json.load(open(tmp_path / "hf_HF_res.json"))
+    assert out["method"] == "HF" and "HF" in out["mi355x"]["method"]
+    d = _lib.load_system_json(path)
+    L = d["L"]
+    s = O.run(O.System(tM=d["tM"], coulomb_type=2, U=d["U"], V=d["V"], fock=True), dt, t_max, 1, d["n_up"],
+              d["n_dn"], noise_arr=np.zeros((1, 2, L * (L - 1) // 2, 2)))
+    ref = O.normalise(s, 1)
+    sm = out["simulation"]
+    for k in ("Niiu", "Niid", "NuNd"):
+        assert rel(np.asarray(sm[k]), ref[k]) < FP64_RTOL, k
+    assert rel(np.asarray(sm["E"]["tot"]), ref["E"][4]) < FP64_RTOL
+    np.testing.assert_allclose(np.asarray(sm["NuNd"]), np.asarray(sm["Niiu"]) * np.asarray(sm["Niid"]), atol=1e-12)
+
+
 def test_few_trajectories_and_single_time_point(oracle_mod):
     L = 4
     n_up = np.array([1.0, 0, 1, 0])

@@ -414,3 +414,38 @@ def test_ftwa_against_exact_diagonalisation(oracle_mod, ed_mod):
             err = np.sqrt(np.maximum(s2[:, idx] / B - mean ** 2, 0) / B)
             got = n["Niiu" if spin == "n_up" else "Niid"][:, idx]
             assert np.all(np.abs(got - r[spin]) <= 5 * err + 1e-9), (U, spin, np.abs(got - r[spin]).max())
+
+
+def test_hartree_fock_closed_form_and_conservation(oracle_mod):
+    """Hartree-Fock (FHHFSolver::derivative, src/FHHFSolver.cpp:575-608): for one particle the Fock term
+    cancels the Hartree self-interaction exactly, so with V alone the density follows the free propagator
+    |(e^{-i tM t})_i0|^2 (without the Fock term it does not); for several particles TDHF conserves the
+    particle number and the HF energy E_tH + E_U + E_V(with exchange) + E_pe."""
+    O = oracle_mod
+    from scipy.linalg import expm as _expm
+    L = 5
+    tM = -(np.eye(L, k=1) + np.eye(L, k=-1))
+    dist = np.abs(np.arange(L)[:, None] - np.arange(L)[None, :]).astype(float)
+    V = np.where(dist > 0, 1.5 / np.maximum(dist, 1.0), 0.0)
+    nu = np.array([1.0, 0, 0, 0, 0])
+    noise = np.zeros((1, 2, L * (L - 1) // 2, 2))
+    t_end = 2.0
+    want = np.abs(_expm(-1j * tM * t_end)[:, 0]) ** 2
+    s = O.run(O.System(tM=tM, coulomb_type=2, U=np.zeros((L, L)), V=V, fock=True), 0.01, t_end, 1, nu, 0 * nu,
+              noise_arr=noise)
+    np.testing.assert_allclose(s.sum_u[:, -1] + 0.5, want, atol=1e-7)
+    s = O.run(O.System(tM=tM, coulomb_type=2, U=np.zeros((L, L)), V=V, fock=False), 0.01, t_end, 1, nu, 0 * nu,
+              noise_arr=noise)
+    assert np.abs(s.sum_u[:, -1] + 0.5 - want).max() > 1e-2
+    Um = 2.0 * np.eye(L) + V
+    n_up = np.array([1.0, 1, 0, 1, 0])
+    n_dn = np.array([0.0, 1, 1, 0, 1])
+    s = O.run(O.System(tM=tM, coulomb_type=2, U=Um, V=V, pot=np.linspace(-0.5, 0.5, L), fock=True), 0.01, 3.0, 2,
+              n_up, n_dn, noise_arr=noise)
+    E = s.sum_e.sum(axis=0)
+    assert np.abs(E - E[0]).max() < 1e-7 * max(1.0, abs(E[0]))
+    np.testing.assert_allclose(s.sum_u.sum(axis=0) + L / 2, n_up.sum(), atol=1e-10)
+    s_h = O.run(O.System(tM=tM, coulomb_type=2, U=Um, V=V, pot=np.linspace(-0.5, 0.5, L), fock=False), 0.01, 3.0, 2,
+                n_up, n_dn, noise_arr=noise)
+    assert np.abs(s_h.sum_e[2, -1] - s.sum_e[2, -1]) > 1e-3  # exchange once coherences have built up
+
