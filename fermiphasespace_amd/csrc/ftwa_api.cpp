@@ -387,8 +387,14 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
         if (h->kernel == FPSFTWA_KERNEL_PIPELINED && !pipe)
             return fail(FPSFTWA_EUNSUPPORTED, "the pipelined kernel needs L <= 64, no pulse and null or on-site "
                                               "Coulomb interactions");
-    } else if (h->kernel == FPSFTWA_KERNEL_HERMITIAN || h->kernel == FPSFTWA_KERNEL_PIPELINED) {
-        return fail(FPSFTWA_EUNSUPPORTED, "the upper-triangle kernels are FP32 only");
+    } else if (h->kernel == FPSFTWA_KERNEL_PIPELINED) {
+        return fail(FPSFTWA_EUNSUPPORTED, "the spin-pipelined kernel is FP32 only");
+    } else {
+        // FP64: the upper-triangle kernel where it applies (no pulse, null or on-site Coulomb, L <= 64)
+        herm = h->kernel != FPSFTWA_KERNEL_FULL && herm64_fits(ds) && h->n_bases == 0;
+        if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !herm)
+            return fail(FPSFTWA_EUNSUPPORTED, "the FP64 upper-triangle kernel needs L <= 64, 4 or 8 neighbours, "
+                                              "no pulse, null or on-site Coulomb and no basis occupations");
     }
     if (!dense && !herm && !stencil_fits<Real>(h->L, ds.z, &nt)) {
         if (h->kernel == FPSFTWA_KERNEL_FULL)
@@ -442,6 +448,8 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
             if (pipe) HIP_TRY(launch_herm_pipe(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->stream));
             else if (herm) HIP_TRY(launch_herm(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->nt_hint, h->stream));
             else HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
+        } else if (herm) {
+            HIP_TRY(launch_herm64(ds, rs, t0, t1, h->state.as<double>(), h->rec.as<double>(), h->nt_hint, h->stream));
         } else {
             HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
         }
@@ -1130,10 +1138,9 @@ extern "C" int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t*
     };
     std::vector<Cand> cands;
     const bool fp32 = h->engine == FPSFTWA_ENGINE_FP32_BATCHED;
-    if (fp32) {
-        cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 0, 0});
-        cands.push_back({FPSFTWA_KERNEL_PIPELINED, 0, 0});
-    }
+    cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 0, 0});
+    if (fp32) cands.push_back({FPSFTWA_KERNEL_PIPELINED, 0, 0});
+    else cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 1024, 0});
     for (int blk : {0, 512, 1024}) cands.push_back({FPSFTWA_KERNEL_FULL, blk, 0});
     cands.push_back({FPSFTWA_KERNEL_DENSE, 0, 0});
     if (fp32) cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 0, 8});

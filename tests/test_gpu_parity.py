@@ -79,12 +79,17 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
             runs += ([("fp32", "full"), ("fp32", "hermitian")] if stencil else []) + [("fp32", "dense")]
             if stencil and (pulse or {}).get("pulse_type", 0) == 0 and coulomb_type != 2:
                 runs.append(("fp32", "pipelined"))
-        elif e == "fp64":  # the stencil kernel where it fits, and the dense GEMM path
-            runs += ([("fp64", "auto")] if stencil else []) + [("fp64", "dense")]
+        elif e == "fp64":  # upper-triangle (16 and 8 waves) and stencil kernels where they fit, the dense path
+            herm64 = (stencil and L <= 64 and (pulse or {}).get("pulse_type", 0) == 0 and coulomb_type != 2
+                      and bases is None)
+            runs += ([("fp64", "auto"), ("fp64", "full")] if stencil else []) + [("fp64", "dense")]
+            if herm64:
+                runs += [("fp64", "hermitian"), ("fp64", "hermitian/1024")]
         else:
             runs.append((e, "auto"))
     for e, kern in runs:
-        eng.set_tuning(0, 0, kern)
+        kname, _, blk = kern.partition("/")
+        eng.set_tuning(int(blk or 0), 0, kname)
         got = engine_run(eng, dt, t_max, n_sub, B, e, seed=seed, noise=noise, bases=bases is not None)
         tol = fp32_tol if e == "fp32" else FP64_RTOL
         print(f"L={L} {e}: states {rel(got['states'], ref.final_states):.2e}")
@@ -494,6 +499,21 @@ def test_tuning_and_chunking_are_bitwise_invariant():
         got = engine_run(eng, 0.005, 0.1, 1, 64, "fp32", seed=9)
         for k in href:
             np.testing.assert_array_equal(got[k], href[k], err_msg=f"pipelined {chunk} {k}")
+    # FP64 upper-triangle kernel: 16 or 8 waves and any chunking give the same bits; it agrees with the
+    # FP64 full-matrix kernel to rounding and keeps rho exactly Hermitian
+    dref = None
+    for chunk, block in ((0, 0), (1, 512), (5, 1024), (2, 512)):
+        eng.set_tuning(block, chunk, "hermitian")
+        got = engine_run(eng, 0.005, 0.1, 1, 64, "fp64", seed=9)
+        if dref is None:
+            dref = got
+        else:
+            for k in dref:
+                np.testing.assert_array_equal(got[k], dref[k], err_msg=f"fp64 hermitian {chunk} {block} {k}")
+    eng.set_tuning(0, 0, "full")
+    full64 = engine_run(eng, 0.005, 0.1, 1, 64, "fp64", seed=9)
+    assert rel(dref["states"], full64["states"]) < 1e-12
+    assert np.max(np.abs(dref["states"] - np.conj(np.transpose(dref["states"], (0, 1, 3, 2))))) == 0
 
 
 def test_fp32_tracks_fp64_on_identical_samples():
@@ -631,7 +651,7 @@ def test_autotune_picks_a_kernel_that_reproduces_the_run():
     e.run()
     ref = e.result("sum_u")
     choice = e.autotune(trial_steps=3)
-    assert choice["kernel"] in ("full", "dense") and choice["traj_steps_per_s"] > 0
+    assert choice["kernel"] in ("hermitian", "full", "dense") and choice["traj_steps_per_s"] > 0
     e.run()
     assert rel(e.result("sum_u"), ref) < FP64_RTOL
     e.close()
