@@ -304,8 +304,9 @@ def main():
     ap.add_argument("--n-traj", type=int, default=65536)
     ap.add_argument("--block", type=int, default=0, help="threads per trajectory workgroup (0, 512, 1024)")
     ap.add_argument("--time-chunk", type=int, default=0)
-    ap.add_argument("--kernel", default="auto", choices=["auto", "full", "hermitian", "pipelined"],
-                    help="FP32 integrator: upper-triangle (hermitian), full-matrix, or automatic")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "full", "hermitian", "pipelined", "sequential"],
+                    help="FP32 integrator: automatic (the spin-sequential upper-triangle kernel here), "
+                         "upper-triangle with both spins (hermitian), spin-pipelined, or full-matrix")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 batched reference run")
     ap.add_argument("--no-plain", action="store_true")
@@ -380,14 +381,16 @@ def main():
     log(f"fp32 batched: {K} steps x {B} traj in {wall:.3f} s -> {value:.4e} traj-steps/s; kernel {tm}")
     res32 = {k: eng.result(k) for k in ("Niiu", "Niid", "NuNd", "E", "Niiu_err", "NuNd_err")}
 
-    # roofline of the stencil kernel: algorithmic bytes 32 L^2 per trajectory-step (FP32)
+    # roofline of the integrator: algorithmic bytes 32 L^2 per trajectory-step (FP32)
     alg_bytes = 32.0 * L * L * B * K
     kern_s = tm["kernel_ms"] * 1e-3
     achieved = alg_bytes / kern_s / 1e9
-    traffic, prof = traffic_from_profile(os.path.join(ROOT, "profiles", "herm_fp32_hubbard8x8_traffic.json"))
-    kname = "stencil_kernel<float,16,8,4,false,64>" if args.kernel == "full" else "herm_kernel<16,8,5,4,false,false>"
-    if args.kernel == "full" or (prof and prof.get("kernel") != kname):
-        traffic = None  # the committed counter figures belong to the other kernel
+    kname = {"full": "stencil_kernel<float,16,8,4,false,64>", "hermitian": "herm_kernel<16,8,5,4,false,false>",
+             "pipelined": "herm_pipe_kernel<5,4>"}.get(args.kernel, "herm_seq_kernel<float,8,4,4,2>")
+    pfile = "herm_fp32_hubbard8x8_traffic.json" if args.kernel == "hermitian" else "herm_seq_fp32_hubbard8x8_traffic.json"
+    traffic, prof = traffic_from_profile(os.path.join(ROOT, "profiles", pfile))
+    if prof is None or prof.get("kernel") != kname:
+        traffic = None  # no committed counter figures for this kernel
     z = 4
     valu_flop = (64 * z + 104) * L * L  # reference-formula flops per trajectory-step, sparse tM
     valu_achieved = valu_flop * B * K / kern_s / 1e12

@@ -21,9 +21,9 @@ PULSE_NONE, PULSE_GAUSSIAN, PULSE_KNOTS = 0, 1, 2
 METHOD_FTWA, METHOD_HF = 0, 1
 ENGINE_FP64_PLAIN, ENGINE_FP64_BATCHED, ENGINE_FP32_BATCHED = 0, 1, 2
 ENGINES = {"fp64_plain": ENGINE_FP64_PLAIN, "fp64": ENGINE_FP64_BATCHED, "fp32": ENGINE_FP32_BATCHED}
-KERNEL_AUTO, KERNEL_FULL, KERNEL_HERMITIAN, KERNEL_DENSE, KERNEL_PIPELINED = 0, 1, 2, 3, 4
+KERNEL_AUTO, KERNEL_FULL, KERNEL_HERMITIAN, KERNEL_DENSE, KERNEL_PIPELINED, KERNEL_SEQUENTIAL = 0, 1, 2, 3, 4, 5
 KERNELS = {"auto": KERNEL_AUTO, "full": KERNEL_FULL, "hermitian": KERNEL_HERMITIAN, "dense": KERNEL_DENSE,
-           "pipelined": KERNEL_PIPELINED}
+           "pipelined": KERNEL_PIPELINED, "sequential": KERNEL_SEQUENTIAL}
 
 RES_DIAG, RES_DIAG2, RES_DOUBLONS, RES_ENERGIES, RES_WHOLE = 0x01, 0x02, 0x04, 0x08, 0x10
 

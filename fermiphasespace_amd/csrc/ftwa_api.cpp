@@ -373,11 +373,20 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     int rc = upload_system<Real>(h, ds, h->T);
     if (rc) return rc;
     int nt = h->nt_hint;
-    bool herm = false, pipe = false, dense = h->kernel == FPSFTWA_KERNEL_DENSE;
+    bool herm = false, pipe = false, seq = false, dense = h->kernel == FPSFTWA_KERNEL_DENSE;
     if (dense) {
         // forced
+    } else if (h->kernel == FPSFTWA_KERNEL_SEQUENTIAL) {
+        seq = herm_seq_fits<Real>(ds) && h->n_bases == 0;
+        if (!seq)
+            return fail(FPSFTWA_EUNSUPPORTED, "the spin-sequential kernel needs L <= 64, 4 or 8 neighbours, no pulse, "
+                                              "null or on-site Coulomb and no basis occupations");
     } else if constexpr (std::is_same<Real, float>::value) {
-        herm = h->kernel != FPSFTWA_KERNEL_FULL && herm_fits(h->L, ds.z);
+        // automatic choice: the spin-sequential kernel where it applies (two workgroups per CU:
+        // 2.84e7 trajectory-steps/s on the 8x8 workload, against 2.49e7 for the both-spin kernel),
+        // else the both-spin upper-triangle kernel (pulses, V interactions, basis occupations)
+        seq = h->kernel == FPSFTWA_KERNEL_AUTO && herm_seq_fits<float>(ds) && h->n_bases == 0;
+        herm = !seq && h->kernel != FPSFTWA_KERNEL_FULL && herm_fits(h->L, ds.z);
         if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !herm)
             return fail(FPSFTWA_EUNSUPPORTED, "the upper-triangle kernel does not fit L=" + std::to_string(h->L));
         // the pipelined variant is selected only on request: on the 8x8 workload it measures 9 % slower
@@ -391,12 +400,12 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
         return fail(FPSFTWA_EUNSUPPORTED, "the spin-pipelined kernel is FP32 only");
     } else {
         // FP64: the upper-triangle kernel where it applies (no pulse, null or on-site Coulomb, L <= 64)
-        herm = h->kernel != FPSFTWA_KERNEL_FULL && herm64_fits(ds) && h->n_bases == 0;
+        herm = h->kernel != FPSFTWA_KERNEL_FULL && herm_seq_fits<double>(ds) && h->n_bases == 0;
         if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !herm)
             return fail(FPSFTWA_EUNSUPPORTED, "the FP64 upper-triangle kernel needs L <= 64, 4 or 8 neighbours, "
                                               "no pulse, null or on-site Coulomb and no basis occupations");
     }
-    if (!dense && !herm && !stencil_fits<Real>(h->L, ds.z, &nt)) {
+    if (!dense && !herm && !seq && !stencil_fits<Real>(h->L, ds.z, &nt)) {
         if (h->kernel == FPSFTWA_KERNEL_FULL)
             return fail(FPSFTWA_EUNSUPPORTED, "the full-matrix stencil kernel does not support L=" +
                                                   std::to_string(h->L) + " with " + std::to_string(ds.z) +
@@ -444,12 +453,16 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
         if (dense) {
             HIP_TRY(dense_run<Real>(h->dense, ds, rs, h->dense_buf.as<Real>(), h->state.as<Real>(), h->rec.as<Real>(),
                                     t0, t1, t0 == 0, t1 == h->T));
+        } else if (seq) {
+            HIP_TRY(launch_herm_seq<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), h->nt_hint,
+                                          h->stream));
         } else if constexpr (std::is_same<Real, float>::value) {
             if (pipe) HIP_TRY(launch_herm_pipe(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->stream));
             else if (herm) HIP_TRY(launch_herm(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->nt_hint, h->stream));
             else HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
         } else if (herm) {
-            HIP_TRY(launch_herm64(ds, rs, t0, t1, h->state.as<double>(), h->rec.as<double>(), h->nt_hint, h->stream));
+            HIP_TRY(launch_herm_seq<double>(ds, rs, t0, t1, h->state.as<double>(), h->rec.as<double>(), h->nt_hint,
+                                            h->stream));
         } else {
             HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
         }
@@ -1095,7 +1108,7 @@ double pulse_amplitude(const PulseSpec& p, double t)
 extern "C" int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel)
 {
     if (int rc = check_handle(h)) return rc;
-    if (kernel < FPSFTWA_KERNEL_AUTO || kernel > FPSFTWA_KERNEL_PIPELINED) return fail(FPSFTWA_EINVAL, "bad kernel");
+    if (kernel < FPSFTWA_KERNEL_AUTO || kernel > FPSFTWA_KERNEL_SEQUENTIAL) return fail(FPSFTWA_EINVAL, "bad kernel");
     h->kernel = kernel;
     if ((block_threads != 0 && block_threads != 512 && block_threads != 832 && block_threads != 1024) || time_chunk < 0)
         return fail(FPSFTWA_EINVAL, "block_threads must be 0 (automatic), 512, 832 or 1024, and time_chunk >= 0");
@@ -1139,7 +1152,10 @@ extern "C" int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t*
     std::vector<Cand> cands;
     const bool fp32 = h->engine == FPSFTWA_ENGINE_FP32_BATCHED;
     cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 0, 0});
-    if (fp32) cands.push_back({FPSFTWA_KERNEL_PIPELINED, 0, 0});
+    if (fp32) {
+        cands.push_back({FPSFTWA_KERNEL_PIPELINED, 0, 0});
+        cands.push_back({FPSFTWA_KERNEL_SEQUENTIAL, 0, 0});
+    }
     else cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 1024, 0});
     for (int blk : {0, 512, 1024}) cands.push_back({FPSFTWA_KERNEL_FULL, blk, 0});
     cands.push_back({FPSFTWA_KERNEL_DENSE, 0, 0});

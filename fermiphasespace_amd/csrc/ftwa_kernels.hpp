@@ -100,11 +100,14 @@ hipError_t launch_herm_pipe(const DevSystem<float>& s, const RunShape& r, int t0
 
 // Dense batched path (ftwa_dense.hip): one library GEMM X = Y T per RK stage for the whole batch,
 // hand-written Hartree, commutator/RK update and record kernels.  L <= 1024, any neighbour count.
-// FP64 upper-triangle integrator (ftwa_herm64.hip): L <= 64, z 4 or 8, no pulse, null or on-site
-// Coulomb, no density records.  nt_hint 1024 selects 16 waves per workgroup, otherwise 8.
-bool herm64_fits(const DevSystem<double>& s);
-hipError_t launch_herm64(const DevSystem<double>& s, const RunShape& r, int t0, int t1, double* state, double* rec,
-                         int nt_hint, hipStream_t st);
+// Spin-sequential upper-triangle integrator (ftwa_herm_seq.hip), FP64 and FP32: L <= 64, z 4 or 8, no
+// pulse, null or on-site Coulomb, no density records.  nt_hint 1024 selects 16 waves per workgroup,
+// otherwise 8.
+template <typename Real>
+bool herm_seq_fits(const DevSystem<Real>& s);
+template <typename Real>
+hipError_t launch_herm_seq(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
+                           int nt_hint, hipStream_t st);
 
 struct DenseCtx;
 DenseCtx* dense_create(hipStream_t st);

@@ -191,18 +191,23 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
 /* Engine tuning (no reference counterpart; the autotuner's knobs): threads per trajectory
  * workgroup for the full-matrix batched kernel (512 or 1024; 0 = automatic), time points per launch
  * (0 = automatic) and the FP32 kernel variant (FPSFTWA_KERNEL_*). */
-#define FPSFTWA_KERNEL_AUTO 0      /* upper-triangle kernel where it fits, else full-matrix       */
+#define FPSFTWA_KERNEL_AUTO 0      /* FP32: spin-sequential where it fits, else upper-triangle,
+                                    * else full-matrix; FP64 likewise without the FP32-only ones   */
 #define FPSFTWA_KERNEL_FULL 1      /* full-matrix stencil kernel                                   */
-#define FPSFTWA_KERNEL_HERMITIAN 2 /* upper-triangle kernel (FP32, L <= 64)                         */
+#define FPSFTWA_KERNEL_HERMITIAN 2 /* upper-triangle kernel (L <= 64; in FP64 the spin-sequential
+                                    * one, no pulse, null or on-site Coulomb)                         */
 #define FPSFTWA_KERNEL_DENSE 3     /* one GEMM per RK stage for the batch (any precision, L <= 1024);
                                     * chosen automatically for L > 64 or more than 8 neighbours      */
 #define FPSFTWA_KERNEL_PIPELINED 4 /* upper-triangle kernel with the two spins half a stage apart
                                     * (FP32, L <= 64, no pulse, null or on-site Coulomb)              */
+#define FPSFTWA_KERNEL_SEQUENTIAL 5 /* upper-triangle kernel with one spin's rows in LDS at a time,
+                                    * two workgroups per CU in FP32 (L <= 64, no pulse, null or
+                                    * on-site Coulomb, no basis occupations)                          */
 int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel);
 
 /* Autotuner (no reference counterpart): times every batched kernel family and shape that can run
- * this system (upper-triangle plain and spin-pipelined, full-matrix stencil at 512/1024 threads, dense; two time-chunk
- * lengths) on a shortened copy of the configured run -- trial_steps steps of at most 16,384
+ * this system (upper-triangle both-spin, spin-pipelined and spin-sequential, full-matrix stencil at
+ * 512/1024 threads, dense; two time-chunk lengths) on a shortened copy of the configured run -- trial_steps steps of at most 16,384
  * trajectories, each candidate run twice and the second timed -- and keeps the fastest as the
  * handle's tuning (fpsftwa_set_tuning). Outputs may be NULL. Requires a batched engine. */
 int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t* kernel, int32_t* block_threads,

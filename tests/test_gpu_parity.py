@@ -79,11 +79,13 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
             runs += ([("fp32", "full"), ("fp32", "hermitian")] if stencil else []) + [("fp32", "dense")]
             if stencil and (pulse or {}).get("pulse_type", 0) == 0 and coulomb_type != 2:
                 runs.append(("fp32", "pipelined"))
+                if L <= 64 and bases is None:
+                    runs.append(("fp32", "sequential"))
         elif e == "fp64":  # upper-triangle (16 and 8 waves) and stencil kernels where they fit, the dense path
-            herm64 = (stencil and L <= 64 and (pulse or {}).get("pulse_type", 0) == 0 and coulomb_type != 2
-                      and bases is None)
+            herm_seq = (stencil and L <= 64 and (pulse or {}).get("pulse_type", 0) == 0 and coulomb_type != 2
+                        and bases is None)
             runs += ([("fp64", "auto"), ("fp64", "full")] if stencil else []) + [("fp64", "dense")]
-            if herm64:
+            if herm_seq:
                 runs += [("fp64", "hermitian"), ("fp64", "hermitian/1024")]
         else:
             runs.append((e, "auto"))
