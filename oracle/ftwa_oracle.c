@@ -52,9 +52,8 @@ int64_t oracle_n_time_points(double dt, double t_max)
 
 /* One complex noise value w with E|w|^2 = 1, Re and Im independent N(0, 1/2):
  * Box-Muller (1958) on two 53-bit uniforms from one Philox block. */
-static void noise_value(uint64_t seed, int64_t traj, int spin, uint32_t pair, double* re, double* im)
+static void box_muller(const uint32_t ctr[4], uint64_t seed, double* re, double* im)
 {
-    uint32_t ctr[4] = {pair, (uint32_t)spin, (uint32_t)(uint64_t)traj, (uint32_t)((uint64_t)traj >> 32)};
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     uint32_t x[4];
     oracle_philox4x32_10(ctr, key, x);
@@ -65,6 +64,28 @@ static void noise_value(uint64_t seed, int64_t traj, int spin, uint32_t pair, do
     double th = 6.283185307179586232 * u2;
     *re = r * cos(th);
     *im = r * sin(th);
+}
+
+/* initial-state noise of pair `pair` of spin `spin` (second counter word 0 or 1) */
+static void noise_value(uint64_t seed, int64_t traj, int spin, uint32_t pair, double* re, double* im)
+{
+    uint32_t ctr[4] = {pair, (uint32_t)spin, (uint32_t)(uint64_t)traj, (uint32_t)((uint64_t)traj >> 32)};
+    box_muller(ctr, seed, re, im);
+}
+
+/* GPSR diffusion noise xi_k(p) of step `step` (bit 31 of the second counter word set: a stream apart
+ * from the initial-state noise) */
+static void gpsr_noise_value(uint64_t seed, int64_t traj, int64_t step, int k, int p, double* re, double* im)
+{
+    uint32_t ctr[4] = {(uint32_t)(uint64_t)step, 0x80000000u | ((uint32_t)k << 16) | (uint32_t)p,
+                       (uint32_t)(uint64_t)traj, (uint32_t)((uint64_t)traj >> 32)};
+    box_muller(ctr, seed, re, im);
+}
+
+void oracle_gpsr_noise(int L, int64_t traj, int64_t step, uint64_t seed, double* out)
+{
+    for (int k = 0; k < 2; ++k)
+        for (int p = 0; p < L; ++p) gpsr_noise_value(seed, traj, step, k, p, &out[2 * (k * L + p)], &out[2 * (k * L + p) + 1]);
 }
 
 void oracle_noise(int L, int64_t first, int64_t count, uint64_t seed, double* out)
@@ -132,7 +153,20 @@ static void build_delta(const or_system* s, const double* rho_same, const double
         D[2 * i] = s->tM[i];
         D[2 * i + 1] = 0.0;
     }
-    if (s->coulomb_type != 0) {
+    if (s->gpsr) {
+        /* Delta_sigma = tM + diag(U rho_sbar.diag()) (+ extPot): no 1/2, no V (src/FHGPSRSolver.cpp:112-131) */
+        if (s->coulomb_type != 0)
+            for (int i = 0; i < L; ++i) {
+                double hr = 0.0, hi = 0.0;
+                for (int j = 0; j < L; ++j) {
+                    double u = s->U[i * L + j];
+                    hr += u * rho_other[2 * (j * L + j)];
+                    hi += u * rho_other[2 * (j * L + j) + 1];
+                }
+                D[2 * (i * L + i)] += hr;
+                D[2 * (i * L + i) + 1] += hi;
+            }
+    } else if (s->coulomb_type != 0) {
         for (int i = 0; i < L; ++i) {
             double hr = 0.0, hi = 0.0;
             for (int j = 0; j < L; ++j) {
@@ -265,13 +299,106 @@ static void step_rk4_38(const or_system* s, double t, double t_max, double h, do
     }
 }
 
-/* rho = R (diag(n - 1/2) + S.W) R^dagger   (src/FHfTWASolver.cpp:348-362, :187-198) */
-static void sample_state(int L, const double* nocc, const double* R, const double* wpairs /* [npairs][2] */,
-                         int lower, double* rho, double* tmp)
+/* GPSR diffusion increment, the analytic form of FHGPSRSolver::step_Diffusion (src/FHGPSRSolver.cpp:211-232):
+ *   d_up = c (rhoH W1 rho + i rho W2 rhoH),   d_dn = c (rhoH conj(W1) rho + i rho conj(W2) rhoH),
+ * c = sqrt(i U0), U0 = Re U(0,0) (:630), rhoH = I - rho, W_k = diag(xi_k) with the step's scaled noise.
+ * The reference runs the rsvd factorisation of the same diffusion matrix (:241-287; initiateSolver
+ * :77 selects it), and adds diffusion only for an on-site Coulomb interaction (:248). */
+static void gpsr_diffusion_one(int L, const double* rho, const double* x1, const double* x2, int conj_xi,
+                               double cr, double ci, double* d, double* tmp)
 {
-    (void)lower;
+    const double sg = conj_xi ? -1.0 : 1.0;
+    /* tmp = rhoH W1 rho */
+    for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) {
+            double ar = 0.0, ai = 0.0;
+            for (int k = 0; k < L; ++k) {
+                double hr = (i == k ? 1.0 : 0.0) - rho[2 * (i * L + k)], hi = -rho[2 * (i * L + k) + 1];
+                double wr = x1[2 * k], wi = sg * x1[2 * k + 1];
+                double mr = hr * wr - hi * wi, mi = hr * wi + hi * wr; /* (rhoH W1)_ik */
+                double pr = rho[2 * (k * L + j)], pi = rho[2 * (k * L + j) + 1];
+                ar += mr * pr - mi * pi;
+                ai += mr * pi + mi * pr;
+            }
+            tmp[2 * (i * L + j)] = ar;
+            tmp[2 * (i * L + j) + 1] = ai;
+        }
+    /* d = c (tmp + i rho W2 rhoH) */
+    for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) {
+            double ar = 0.0, ai = 0.0;
+            for (int k = 0; k < L; ++k) {
+                double pr = rho[2 * (i * L + k)], pi = rho[2 * (i * L + k) + 1];
+                double wr = x2[2 * k], wi = sg * x2[2 * k + 1];
+                double mr = pr * wr - pi * wi, mi = pr * wi + pi * wr; /* (rho W2)_ik */
+                double hr = (k == j ? 1.0 : 0.0) - rho[2 * (k * L + j)], hi = -rho[2 * (k * L + j) + 1];
+                ar += mr * hr - mi * hi;
+                ai += mr * hi + mi * hr;
+            }
+            double sr = tmp[2 * (i * L + j)] - ai, si = tmp[2 * (i * L + j) + 1] + ar; /* tmp + i (ar + i ai) */
+            d[2 * (i * L + j)] = cr * sr - ci * si;
+            d[2 * (i * L + j) + 1] = cr * si + ci * sr;
+        }
+}
+
+static void gpsr_constant(const or_system* s, double* cr, double* ci)
+{
+    /* sqrt(i U0): U0 >= 0 gives sqrt(U0/2) (1 + i), U0 < 0 gives sqrt(-U0/2) (1 - i) */
+    const double u0 = s->coulomb_type == 1 ? s->U[0] : 0.0;
+    const double m = sqrt(fabs(u0) / 2.0);
+    *cr = m;
+    *ci = u0 >= 0.0 ? m : -m;
+}
+
+void oracle_gpsr_diffusion(const or_system* s, const double* rho_u, const double* rho_d, const double* xi,
+                           double* d_u, double* d_d)
+{
+    const int L = s->L;
+    double cr, ci;
+    gpsr_constant(s, &cr, &ci);
+    double* tmp = (double*)malloc(sizeof(double) * 2 * L * L);
+    if (!tmp) return;
+    gpsr_diffusion_one(L, rho_u, xi, xi + 2 * L, 0, cr, ci, d_u, tmp);
+    gpsr_diffusion_one(L, rho_d, xi, xi + 2 * L, 1, cr, ci, d_d, tmp);
+    free(tmp);
+}
+
+/* FHGPSRSolver::step (src/FHGPSRSolver.cpp:608-637): a trajectory holding a NaN or Inf becomes all NaN;
+ * otherwise rho += RK4 "3/8" drift (:168-201) + diffusion, both from the step's initial state.  xi: the
+ * step's noise [2][L][2] already scaled by sqrt(h); du, dd, tmp: scratch [L][L][2]. */
+static void gpsr_step(const or_system* s, double t, double t_max, double h, double* ru, double* rd, const double* xi,
+                      double* du, double* dd, double* tmp, work* w)
+{
+    const int n = 2 * s->L * s->L;
+    int bad = 0;
+    for (int i = 0; i < n; ++i)
+        if (!isfinite(ru[i]) || !isfinite(rd[i])) bad = 1;
+    if (bad) {
+        for (int i = 0; i < n; ++i) {
+            ru[i] = NAN;
+            rd[i] = NAN;
+        }
+        return;
+    }
+    double cr, ci;
+    gpsr_constant(s, &cr, &ci);
+    gpsr_diffusion_one(s->L, ru, xi, xi + 2 * s->L, 0, cr, ci, du, tmp);
+    gpsr_diffusion_one(s->L, rd, xi, xi + 2 * s->L, 1, cr, ci, dd, tmp);
+    step_rk4_38(s, t, t_max, h, ru, rd, w);
+    for (int i = 0; i < n; ++i) {
+        ru[i] += du[i];
+        rd[i] += dd[i];
+    }
+}
+
+/* rho = R (diag(n - shift) + S.W) R^dagger   (src/FHfTWASolver.cpp:348-362, :187-198; shift 1/2).
+ * GPSR: shift 0 and W = 0, i.e. the normally ordered R diag(n) R^dagger; the reference's GPSR starts from
+ * diag(IC occupations) (src/FHGPSRSolver.cpp:77-83), the same state for the site-diagonal ICs. */
+static void sample_state(int L, const double* nocc, const double* R, const double* wpairs /* [npairs][2] */,
+                         double shift, double* rho, double* tmp)
+{
     memset(rho, 0, sizeof(double) * 2 * L * L);
-    for (int i = 0; i < L; ++i) rho[2 * (i * L + i)] = nocc[i] - 0.5;
+    for (int i = 0; i < L; ++i) rho[2 * (i * L + i)] = nocc[i] - shift;
     int64_t p = 0;
     for (int i = 0; i < L; ++i)
         for (int j = i + 1; j < L; ++j, ++p) {
@@ -332,25 +459,26 @@ static void record(const or_system* s, double t, double t_max, const double* ru,
         s2d[(size_t)i * T + ti] += dr * dr + di * di;
         sud[(size_t)i * T + ti] += ur * dr - ui * di;
     }
-    /* energies with N = rho + I/2 (:401-411) */
+    /* energies with N = rho + I/2 (:401-411); GPSR: N = rho (src/FHGPSRSolver.cpp:811-816), no E_V */
+    const double hf = s->gpsr ? 0.0 : 0.5;
     double eth = 0.0;
     for (int i = 0; i < L; ++i)
         for (int j = 0; j < L; ++j) {
-            double nr = ru[2 * (i * L + j)] + rd[2 * (i * L + j)] + (i == j ? 1.0 : 0.0);
+            double nr = ru[2 * (i * L + j)] + rd[2 * (i * L + j)] + (i == j ? 2.0 * hf : 0.0);
             eth += nr * s->tM[i * L + j];
         }
     double eu = 0.0, ev = 0.0;
     if (s->coulomb_type != 0) {
         for (int i = 0; i < L; ++i)
             for (int j = 0; j < L; ++j) {
-                double nui = ru[2 * (i * L + i)] + 0.5, nuii = ru[2 * (i * L + i) + 1];
-                double ndi = rd[2 * (i * L + i)] + 0.5, ndii = rd[2 * (i * L + i) + 1];
-                double nuj = ru[2 * (j * L + j)] + 0.5, nujj = ru[2 * (j * L + j) + 1];
-                double ndj = rd[2 * (j * L + j)] + 0.5, ndjj = rd[2 * (j * L + j) + 1];
+                double nui = ru[2 * (i * L + i)] + hf, nuii = ru[2 * (i * L + i) + 1];
+                double ndi = rd[2 * (i * L + i)] + hf, ndii = rd[2 * (i * L + i) + 1];
+                double nuj = ru[2 * (j * L + j)] + hf, nujj = ru[2 * (j * L + j) + 1];
+                double ndj = rd[2 * (j * L + j)] + hf, ndjj = rd[2 * (j * L + j) + 1];
                 double u = s->U[i * L + j];
                 /* Re(Nu_i U_ij Nd_j + Nd_i U_ij Nu_j) / 2 */
                 eu += u * ((nui * ndj - nuii * ndjj) + (ndi * nuj - ndii * nujj)) / 2.0;
-                if (s->coulomb_type == 2) {
+                if (s->coulomb_type == 2 && !s->gpsr) {
                     double v = s->V[i * L + j];
                     ev += v * ((nui * nuj - nuii * nujj) + (ndi * ndj - ndii * ndjj)) / 2.0;
                     if (s->fock) /* - Re sum V o N o N^T / 2 per spin */
@@ -366,12 +494,12 @@ static void record(const or_system* s, double t, double t_max, const double* ru,
     double epe = 0.0;
     if (s->pot)
         for (int i = 0; i < L; ++i)
-            epe += s->pot[i] * (ru[2 * (i * L + i)] + rd[2 * (i * L + i)] + 1.0);
+            epe += s->pot[i] * (ru[2 * (i * L + i)] + rd[2 * (i * L + i)] + 2.0 * hf);
     if (s->pulse_type != 0) {
         ext_pot(s, t, t_max, w->P);
         for (int i = 0; i < L; ++i)
             for (int j = 0; j < L; ++j) {
-                double nr = ru[2 * (i * L + j)] + rd[2 * (i * L + j)] + (i == j ? 1.0 : 0.0);
+                double nr = ru[2 * (i * L + j)] + rd[2 * (i * L + j)] + (i == j ? 2.0 * hf : 0.0);
                 double ni = ru[2 * (i * L + j) + 1] + rd[2 * (i * L + j) + 1];
                 epe += nr * w->P[2 * (i * L + j)] - ni * w->P[2 * (i * L + j) + 1];
             }
@@ -438,8 +566,8 @@ static int run_impl(const or_system* s, const or_params* p, const double* noise,
 #endif
         double* acc = accs + acc_n * (size_t)tid;
         work w = {0};
-        double* ru = (double*)malloc(sizeof(double) * 2 * L * L * 3);
-        double* wbuf = (double*)malloc(sizeof(double) * 2 * (npairs > 0 ? npairs : 1) * 2);
+        double* ru = (double*)malloc(sizeof(double) * 2 * L * L * 6 + sizeof(double) * 4 * L);
+        double* wbuf = (double*)calloc((size_t)2 * (npairs > 0 ? npairs : 1) * 2, sizeof(double));
         int ok = (work_alloc(&w, L) == 0) && ru && wbuf;
         if (!ok) {
 #pragma omp atomic write
@@ -447,24 +575,40 @@ static int run_impl(const or_system* s, const or_params* p, const double* noise,
         }
         double* rd = ru + 2 * L * L;
         double* tmp = rd + 2 * L * L;
+        double* du = tmp + 2 * L * L; /* GPSR diffusion increments and noise */
+        double* dd = du + 2 * L * L;
+        double* tmp2 = dd + 2 * L * L;
+        double* xi = tmp2 + 2 * L * L;
+        const double sh = sqrt(h);
 #pragma omp for schedule(static)
         for (int64_t b = 0; b < p->n_traj; ++b) {
             if (!ok) continue;
             const double* wb;
-            if (use_seed) {
+            if (s->gpsr) {
+                wb = wbuf; /* zeros: GPSR starts from the state itself */
+            } else if (use_seed) {
                 oracle_noise(L, b, 1, seed, wbuf);
                 wb = wbuf;
             } else {
                 wb = noise + (size_t)b * 2 * npairs * 2;
             }
-            sample_state(L, p->n_up, p->rot_up, wb, 0, ru, tmp);
-            sample_state(L, p->n_dn, p->rot_dn, wb + 2 * npairs, 1, rd, tmp);
+            const double shift = s->gpsr ? 0.0 : 0.5;
+            sample_state(L, p->n_up, p->rot_up, wb, shift, ru, tmp);
+            sample_state(L, p->n_dn, p->rot_dn, wb + 2 * npairs, shift, rd, tmp);
             double t = 0.0;
             for (int64_t ti = 0; ti < T; ++ti) {
                 record(s, ti * p->dt, p->t_max, ru, rd, ti, T, acc, &w, nb, p->bases, ce);
                 if (ti == T - 1) break; /* steps after the last record change no output */
                 for (int ss = 0; ss < p->n_sub; ++ss) {
-                    step_rk4_38(s, t, p->t_max, h, ru, rd, &w);
+                    if (s->gpsr) {
+                        /* noise of variance h per step (the reference scales by sqrt(dt) of the time grid,
+                           src/FHGPSRSolver.cpp:219-220, :281, which equals h for stepsParTimePoints = 1) */
+                        oracle_gpsr_noise(L, b, ti * p->n_sub + ss, s->gpsr_seed, xi);
+                        for (int k = 0; k < 4 * L; ++k) xi[k] *= sh;
+                        gpsr_step(s, t, p->t_max, h, ru, rd, xi, du, dd, tmp2, &w);
+                    } else {
+                        step_rk4_38(s, t, p->t_max, h, ru, rd, &w);
+                    }
                     t += h;
                 }
             }

@@ -12,7 +12,14 @@
  *                         counter-based stream instead of Armadillo's randomly seeded one
  *   oracle_run            FHfTWASolver::initiateSolver (src/FHfTWASolver.cpp:99-118, :187-198),
  *                         runSimulation (:222-556; tbStates / mfStates occupations :385-394),
- *                         derivative (:568-600), step_RK4_38 (:629-661)
+ *                         derivative (:568-600), step_RK4_38 (:629-661); with or_system.gpsr set,
+ *                         FHGPSRSolver::initiateSolver (src/FHGPSRSolver.cpp:70-101), derivative
+ *                         (:110-133), step_Drift_RK4_38 (:168-201), step (:608-637), runSimulation
+ *                         (:641-930)
+ *   oracle_gpsr_diffusion the analytic diffusion increment FHGPSRSolver::step_Diffusion
+ *                         (src/FHGPSRSolver.cpp:211-232), whose second moments are the diffusion matrix
+ *                         D = [[0, A], [A^T, 0]] of the rsvd path the reference runs (:241-287, A from
+ *                         quarterDiffusionMatrix :337-383)
  */
 #ifndef FTWA_ORACLE_H
 #define FTWA_ORACLE_H
@@ -37,6 +44,10 @@ typedef struct or_system {
     const double* disp;  /* [L][L][3] r_i - r_j                                   */
     int fock;            /* 1: Hartree-Fock, Delta_s gains -N_s o V and E_V its exchange term
                             (FHHFSolver::derivative, src/FHHFSolver.cpp:575-608, energies :356-357) */
+    int gpsr;            /* 1: Gaussian phase-space representation (FHGPSRSolver): rho is the
+                            normally ordered density matrix (no 1/2 shifts), Delta_s = tM + diag(U rho_sbar_ii)
+                            + extPot (no V), and every step adds the stochastic diffusion term        */
+    uint64_t gpsr_seed;  /* key of the counter-based diffusion noise                                   */
 } or_system;
 
 typedef struct or_params {
@@ -79,6 +90,14 @@ int oracle_run(const or_system* s, const or_params* p, const double* noise, int 
 /* Same, drawing the noise from the counter-based generator (seed) trajectory by trajectory,
  * which keeps memory O(threads) for large B (used for the CPU baseline). */
 int oracle_run_seeded(const or_system* s, const or_params* p, uint64_t seed, int nthreads, or_out* o);
+/* Diffusion noise of step `step` of trajectory `traj`: xi_1 = out[0][.], xi_2 = out[1][.], complex
+ * [2][L][2], E|xi|^2 = 1 (Re and Im independent N(0, 1/2)); a step of length h scales it by sqrt(h). */
+void oracle_gpsr_noise(int L, int64_t traj, int64_t step, uint64_t seed, double* out);
+/* One GPSR diffusion increment at rho (complex [L][L][2] per spin) for the scaled noise xi ([2][L][2]):
+ * d_up = c (rhoH W1 rho + i rho W2 rhoH), d_dn the same with conj(W1), conj(W2), c = sqrt(i U(0,0)),
+ * rhoH = I - rho, W_k = diag(xi_k); zero unless the Coulomb interaction is on-site. */
+void oracle_gpsr_diffusion(const or_system* s, const double* rho_u, const double* rho_d, const double* xi,
+                           double* d_u, double* d_d);
 /* One derivative evaluation f = -i(rho Delta - Delta rho) for both spins (for tests). */
 void oracle_derivative(const or_system* s, double t, double t_max, const double* rho_u, const double* rho_d,
                        double* f_u, double* f_d);

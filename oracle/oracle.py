@@ -1,4 +1,4 @@
-"""ctypes binding of the CPU FP64 fTWA oracle (oracle/liboracle.so).
+"""ctypes binding of the CPU FP64 fTWA / GPSR oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg as the checker or the timed CPU baseline, never by the product package.
@@ -50,6 +50,10 @@ def _load():
     lib.oracle_derivative.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                       P(ctypes.c_double), P(ctypes.c_double),
                                       P(ctypes.c_double), P(ctypes.c_double)]
+    lib.oracle_gpsr_noise.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64,
+                                      P(ctypes.c_double)]
+    lib.oracle_gpsr_diffusion.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double),
+                                          P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)]
     _lib = lib
     return lib
 
@@ -70,6 +74,8 @@ class _OrSystem(ctypes.Structure):
         ("knots", ctypes.POINTER(ctypes.c_double)),
         ("disp", ctypes.POINTER(ctypes.c_double)),
         ("fock", ctypes.c_int),
+        ("gpsr", ctypes.c_int),
+        ("gpsr_seed", ctypes.c_uint64),
     ]
 
 
@@ -115,6 +121,8 @@ class System:
     knots: np.ndarray | None = None
     disp: np.ndarray | None = None
     fock: bool = False  # Hartree-Fock dynamics (exchange with V), with zero noise: the HF solver
+    gpsr: bool = False  # Gaussian phase-space representation (the GPSR solver), diffusion keyed by gpsr_seed
+    gpsr_seed: int = 0
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -139,6 +147,8 @@ class System:
         s.tM = _ptr(arrs["tM"])
         s.coulomb_type = int(self.coulomb_type)
         s.fock = 1 if self.fock else 0
+        s.gpsr = 1 if self.gpsr else 0
+        s.gpsr_seed = int(self.gpsr_seed) & 0xFFFFFFFFFFFFFFFF
         s.U = _ptr(arrs["U"])
         s.V = _ptr(arrs["V"])
         s.pot = _ptr(arrs["pot"])
@@ -187,6 +197,29 @@ def derivative(system: System, t: float, t_max: float, rho_u: np.ndarray, rho_d:
     fd = np.zeros((L, L, 2))
     _load().oracle_derivative(ctypes.byref(s), t, t_max, _ptr(ru), _ptr(rd), _ptr(fu), _ptr(fd))
     return fu[..., 0] + 1j * fu[..., 1], fd[..., 0] + 1j * fd[..., 1]
+
+
+def gpsr_noise(L: int, traj: int, step: int, seed: int) -> np.ndarray:
+    """Unscaled GPSR diffusion noise of one step, complex [2][L] (xi_1, xi_2)."""
+    out = np.zeros((2, L, 2))
+    _load().oracle_gpsr_noise(L, traj, step, seed, _ptr(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def gpsr_diffusion(system: System, rho_u: np.ndarray, rho_d: np.ndarray, xi: np.ndarray):
+    """GPSR diffusion increments (d_up, d_dn) at complex [L][L] rho for the scaled noise xi [2][L]."""
+    L = system.L
+    s = system._c()
+
+    def c2(a):
+        a = np.asarray(a, dtype=np.complex128)
+        return np.ascontiguousarray(np.stack([a.real, a.imag], -1), dtype=np.float64)
+
+    ru, rd, x = c2(rho_u), c2(rho_d), c2(xi)
+    du = np.zeros((L, L, 2))
+    dd = np.zeros((L, L, 2))
+    _load().oracle_gpsr_diffusion(ctypes.byref(s), _ptr(ru), _ptr(rd), _ptr(x), _ptr(du), _ptr(dd))
+    return du[..., 0] + 1j * du[..., 1], dd[..., 0] + 1j * dd[..., 1]
 
 
 @dataclass
@@ -258,8 +291,17 @@ def run(system: System, dt: float, t_max: float, n_sub: int, n_up, n_dn, *, nois
     return Sums(T=T, final_states=final, sum_basis=sb, **arrays)
 
 
-def normalise(s: Sums, B: int) -> dict:
-    """The reference's ensemble normalisation (src/FHfTWASolver.cpp:506-547)."""
+def normalise(s: Sums, B: int, gpsr: bool = False) -> dict:
+    """The reference's ensemble normalisation (src/FHfTWASolver.cpp:506-547); GPSR: plain averages of the
+    normally ordered quantities (src/FHGPSRSolver.cpp:895-929)."""
+    if gpsr:
+        out = {"Niiu": s.sum_u / B, "Niid": s.sum_d / B, "N2iiu": s.sum2_u / B, "N2iid": s.sum2_d / B,
+               "NuNd": s.sum_ud / B}
+        E = s.sum_e / B
+        out["E"] = np.vstack([E, E.sum(axis=0, keepdims=True)])
+        if s.sum_basis is not None:
+            out["M"] = s.sum_basis / B
+        return out
     out = {
         "Niiu": s.sum_u / B + 0.5,
         "Niid": s.sum_d / B + 0.5,

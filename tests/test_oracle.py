@@ -449,3 +449,142 @@ def test_hartree_fock_closed_form_and_conservation(oracle_mod):
                 n_up, n_dn, noise_arr=noise)
     assert np.abs(s_h.sum_e[2, -1] - s.sum_e[2, -1]) > 1e-3  # exchange once coherences have built up
 
+
+
+# ---- GPSR (FHGPSRSolver) ---------------------------------------------------------------------------
+
+def test_gpsr_free_fermions_and_initial_state(oracle_mod):
+    """Without an interaction GPSR has no diffusion and its drift is the free commutator equation, so the
+    normally ordered rho (no 1/2 shift) follows e^{i Delta t} diag(n) e^{-i Delta t}; the tbGroundState-type
+    start R diag(n) R^dagger of the tM eigenbasis is stationary and its densities are sum_occ |V_ik|^2."""
+    O = oracle_mod
+    L = 5
+    rng = np.random.default_rng(4)
+    tM = chain_tM(L)
+    tM[1, 4] = tM[4, 1] = -0.4
+    pot = rng.uniform(-1, 1, L)
+    n_up = np.array([1.0, 0.0, 1.0, 1.0, 0.0])
+    n_dn = np.array([0.0, 1.0, 1.0, 0.0, 0.0])
+    dt, t_max = 0.05, 1.0
+    for ct in (0, 1):  # null Coulomb, and on-site with U(0,0) = 0 (diffusion constant sqrt(i U0) = 0)
+        s = O.System(tM=tM, coulomb_type=ct, U=np.zeros((L, L)), pot=pot, gpsr=True, gpsr_seed=3)
+        r = O.run(s, dt, t_max, 1, n_up, n_dn, n_traj=2, seed=3, keep_states=True)
+        for sg, n in enumerate((n_up, n_dn)):
+            exact_rk4, exact = _free_expected(tM, pot, np.diag(n).astype(complex), dt, r.T - 1)
+            for b in range(2):
+                np.testing.assert_allclose(r.final_states[b, sg], exact_rk4, atol=1e-13)
+            np.testing.assert_allclose(r.final_states[0, sg], exact, atol=5e-4)
+        np.testing.assert_allclose(r.sum_u[:, 0] / 2, n_up, atol=0)  # Niiu = rho_ii, no 1/2
+    lam, V = np.linalg.eigh(tM)
+    R = V.astype(complex)
+    occ = np.array([1.0, 1.0, 0.0, 0.0, 0.0])
+    s = O.System(tM=tM, coulomb_type=0, gpsr=True)
+    r = O.run(s, 0.05, 1.0, 1, occ, occ, n_traj=1, seed=1, rot_up=R, rot_dn=R)
+    want = (V[:, :2] ** 2).sum(axis=1)
+    np.testing.assert_allclose(r.sum_u, np.repeat(want[:, None], r.T, axis=1), atol=1e-12)
+    np.testing.assert_allclose(r.sum_e[0], 2 * lam[:2].sum(), atol=1e-12)  # E_tH = Re tr(rho tM), both spins
+
+
+def _quarter_diffusion_matrix(ru, rd, U0):
+    """A of the reference's diffusion matrix D = [[0, A], [A^T, 0]] as FHGPSRSolver::quarterDiffusionMatrix
+    (src/FHGPSRSolver.cpp:337-383) builds it: line index l*L + k is element (k, l) of the column-major rho."""
+    L = ru.shape[0]
+    I = np.eye(L)
+    B1u = np.zeros((L * L, L), complex)
+    B2u = np.zeros((L * L, L), complex)
+    B1d = np.zeros((L * L, L), complex)
+    B2d = np.zeros((L * L, L), complex)
+    for l in range(L):
+        for k in range(L):
+            for p in range(L):
+                B1u[l * L + k, p] = ru[p, l] * (I - ru)[k, p]
+                B2u[l * L + k, p] = 1j * (I - ru)[p, l] * ru[k, p]
+                B1d[l * L + k, p] = rd[p, l] * (I - rd)[k, p]
+                B2d[l * L + k, p] = 1j * (I - rd)[p, l] * rd[k, p]
+    return 1j * U0 * (B1u @ B1d.T + B2u @ B2d.T)
+
+
+def test_gpsr_diffusion_second_moments(oracle_mod):
+    """The oracle's analytic diffusion increment (FHGPSRSolver::step_Diffusion) has the second moments of the
+    diffusion matrix the reference factorises on its rsvd path: E[d rho_up (x) d rho_dn] = A h and
+    E[d rho_s (x) d rho_s] = 0 (the zero diagonal blocks of D), for non-Hermitian rho and either sign of U.
+    The increment is linear in the 4L real noise variables (Re, Im of xi_1 and xi_2, each of variance h/2),
+    so the moments follow exactly from its values at unit noises."""
+    O = oracle_mod
+    L, h = 3, 0.01
+    rng = np.random.default_rng(8)
+    ru = rng.normal(size=(L, L)) + 1j * rng.normal(size=(L, L))
+    rd = rng.normal(size=(L, L)) + 1j * rng.normal(size=(L, L))
+    for U0 in (2.5, -1.5):
+        s = O.System(tM=chain_tM(L), coulomb_type=1, U=U0 * np.eye(L), gpsr=True)
+        Ju = np.zeros((L * L, 4 * L), complex)  # rows: column-major element index l*L + k
+        Jd = np.zeros((L * L, 4 * L), complex)
+        for r in range(4 * L):
+            xi = np.zeros(2 * L, complex)
+            k, part = divmod(r, 2)
+            xi[k] = 1.0 if part == 0 else 1j
+            du, dd = O.gpsr_diffusion(s, ru, rd, xi.reshape(2, L))
+            Ju[:, r] = du.T.reshape(-1)
+            Jd[:, r] = dd.T.reshape(-1)
+        A = _quarter_diffusion_matrix(ru, rd, U0)
+        np.testing.assert_allclose((h / 2) * Ju @ Jd.T, A * h, atol=1e-12 * np.abs(A).max())
+        np.testing.assert_allclose((h / 2) * Ju @ Ju.T, 0, atol=1e-12 * np.abs(A).max())
+        np.testing.assert_allclose((h / 2) * Jd @ Jd.T, 0, atol=1e-12 * np.abs(A).max())
+    s = O.System(tM=chain_tM(L), coulomb_type=2, U=np.eye(L), V=np.eye(L), gpsr=True)
+    du, dd = O.gpsr_diffusion(s, ru, rd, np.ones((2, L), complex))
+    assert np.abs(du).max() == 0 and np.abs(dd).max() == 0  # diffusion only for an on-site interaction (:248)
+
+
+def test_gpsr_noise_stream(oracle_mod):
+    """The diffusion noise: Box-Muller moments (E xi = 0, E|xi|^2 = 1, Re and Im independent with variance 1/2,
+    E xi^2 = 0) and a stream apart from the initial-state noise of the same key."""
+    O = oracle_mod
+    L = 64
+    x = np.concatenate([O.gpsr_noise(L, b, st, 5).ravel() for b in range(20) for st in range(25)])
+    n = x.size
+    assert abs(x.mean()) < 5 / np.sqrt(n)
+    assert abs(np.mean(np.abs(x) ** 2) - 1) < 5 * np.sqrt(2 / n)
+    assert abs(np.mean(x ** 2)) < 5 * np.sqrt(2 / n)
+    assert abs(np.mean(x.real * x.imag)) < 5 / np.sqrt(2 * n)
+    assert stats.kstest(x.real * np.sqrt(2), "norm").pvalue > 1e-3
+    w = O.noise(L, 0, 1, 5)[0, :, :, 0].ravel()
+    assert not np.intersect1d(np.round(w, 12), np.round(x.real, 12)).size
+
+
+def test_gpsr_against_exact_diagonalisation(oracle_mod, ed_mod):
+    """Physics check: GPSR is an exact representation, so its ensemble means follow exact diagonalisation
+    until the trajectories' known instability (4-site chain, U = 1, Neel start, to t = 0.5) -- the densities
+    within the sampling error and <n_up n_dn> within 1.5e-3 -- where the drift alone (mean field, no
+    diffusion) is off by 3e-3 or more in <n_up n_dn> and by several standard errors in the densities."""
+    O, ed = oracle_mod, ed_mod
+    L, U, B, dt, t_end = 4, 1.0, 100000, 0.01, 0.5
+    tM = chain_tM(L)
+    nu = np.array([1.0, 0, 1, 0])
+    nd = 1 - nu
+    r = ed.evolve_product_state(L, tM, U * np.ones(L), nu, nd, t_end, 6)
+    idx = np.round(r["t"] / dt).astype(int)
+    s = O.run(O.System(tM=tM, coulomb_type=1, U=U * np.eye(L), gpsr=True, gpsr_seed=11), dt, t_end, 1, nu, nd,
+              n_traj=B, seed=11, nthreads=8)
+    n = O.normalise(s, B, gpsr=True)
+    mean = s.sum_u[:, idx] / B
+    err = np.sqrt(np.maximum(s.sum2_u[:, idx] / B - mean ** 2, 0) / B)  # |rho_ii|^2 bounds (Re rho_ii)^2
+    assert np.all(np.abs(n["Niiu"][:, idx] - r["n_up"]) <= 4 * err + 1e-9)
+    assert np.all(np.abs(n["NuNd"][:, idx] - r["nund"]) <= 1.5e-3)
+    mf = O.normalise(O.run(O.System(tM=tM, coulomb_type=2, U=U * np.eye(L), V=np.zeros((L, L)), gpsr=True), dt,
+                           t_end, 1, nu, nd, n_traj=1, seed=1), 1, gpsr=True)
+    assert np.abs(mf["NuNd"][:, idx[-1]] - r["nund"][:, -1]).max() > 3e-3
+    assert np.abs(mf["Niiu"][:, idx[-1]] - r["n_up"][:, -1]).max() > 3 * err[:, -1].max()
+
+
+def test_gpsr_divergent_trajectories_become_nan(oracle_mod):
+    """FHGPSRSolver::step (src/FHGPSRSolver.cpp:611-615): a trajectory holding a NaN or Inf is set to NaN
+    everywhere, so its final state is either finite or all NaN, and the ensemble sums turn NaN from then on."""
+    O = oracle_mod
+    L = 4
+    s = O.System(tM=chain_tM(L), coulomb_type=1, U=2.0 * np.eye(L), gpsr=True, gpsr_seed=2)
+    nu = np.array([1.0, 0, 1, 0])
+    r = O.run(s, 0.02, 1.0, 1, nu, 1 - nu, n_traj=200, seed=2, keep_states=True)
+    fin = np.isfinite(r.final_states).reshape(200, -1)
+    assert np.all(fin.all(axis=1) | (~fin).all(axis=1))
+    assert (~fin).all(axis=1).any() and fin.all(axis=1).any()
+    assert np.isnan(r.sum_u[:, -1]).all() and np.isfinite(r.sum_u[:, 0]).all()
