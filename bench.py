@@ -151,6 +151,55 @@ def side_chain10(args, sync):
     print(json.dumps(out), flush=True)
 
 
+def side_gpsr4x4(args, sync):
+    """GPSR (the reference's FHGPSRSolver) on the 4x4 periodic Hubbard lattice (L = 16), t = 1, U = 2,
+    half-filled charge-density wave, 65,536 trajectories, dt 0.01, 50 steps (GPSR trajectories start to
+    diverge soon after t = 0.5 at this U): the FP64 and FP32 batched GPSR kernels over the whole run, the plain
+    FP64 engine over 2 steps, and the CPU oracle on a bounded sample."""
+    from fermiphasespace_amd import _lib
+    from fermiphasespace_amd import lattices as lat
+    from oracle import oracle as O
+    B, dt, U = args.n_traj, 0.01, 2.0
+    steps = 50 if args.steps is None else args.steps
+    pos, box = lat.grid(4, 4)
+    tM, Um = lat.system_arrays(pos, box, True, t=-1.0, U=U)
+    L = tM.shape[0]
+    nu = np.array(lat.checkerboard(4, 4)["IC_NuDiag"], dtype=float)
+    eng = _lib.Engine(tM, U=Um)
+    eng.set_method("gpsr")
+    eng.set_initial_state(nu, nu)
+    eng.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES)
+    out = {"metric": "GPSR RK4 traj-steps/s, 4x4 periodic Hubbard 65536 traj", "unit": "traj-steps/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "fp64", "data": "synthetic",
+           "config": {"workload": "gpsr4x4_periodic_cdw", "L": L, "n_traj": B, "dt": dt, "steps": steps, "U": U}}
+    for name, n in (("fp64", steps), ("fp32", steps), ("fp64_plain", 2)):
+        eng.set_params(dt, 2 * dt, 1, B, seed=31, engine=name)
+        eng.run()
+        eng.set_params(dt, n * dt, 1, B, seed=31, engine=name)
+        el = _timed_run(eng, sync)
+        kms = eng.timing()["kernel_ms"]
+        out[name] = {"value": round(B * n / el, 1), "seconds": round(el, 3), "kernel_ms": round(kms, 3), "steps": n}
+        log(f"gpsr4x4 {name}: {B * n / el:.4e} traj-steps/s ({n} steps)")
+    out["value"] = out["fp64"]["value"]
+    out["ms_per_step"] = round(1e3 * out["fp64"]["seconds"] / steps, 4)
+    out["vs_plain_fp64"] = round(out["fp64"]["value"] / out["fp64_plain"]["value"], 1)
+    eng.close()
+    if not args.no_cpu:  # bounded sample: trajectories of the same run until about cpu-seconds of work
+        nth = os.cpu_count() or 1
+        s = O.System(tM=tM, coulomb_type=1, U=Um, gpsr=True, gpsr_seed=31)
+        t0 = time.perf_counter()
+        O.run(s, dt, steps * dt, 1, nu, nu, n_traj=nth, seed=31, nthreads=nth)
+        per = (time.perf_counter() - t0) / nth
+        nb = max(nth, int(args.cpu_seconds * nth / max(per, 1e-9)))
+        t0 = time.perf_counter()
+        O.run(s, dt, steps * dt, 1, nu, nu, n_traj=nb, seed=31, nthreads=nth)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(nb * steps / el, 1), "unit": "traj-steps/s", "cores": nth, "kind": "port",
+                               "sample": f"FP64 GPSR oracle: {nb} trajectories x {steps} steps of the same run, "
+                                         f"{el:.1f} s on {nth} OpenMP threads"}
+    print(json.dumps(out), flush=True)
+
+
 def side_mixed(args, sync):
     """Honeycomb 24 / kagome 48 / square 100 / bcc 54 with site potentials, FP32, dt 0.01, t_max 10:
     per-system kernel by the autotuner, all four systems at once (fpsftwa_run_many)."""
@@ -295,7 +344,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="hubbard8x8",
-                    choices=["hubbard8x8", "chain10", "mixed", "moire196", "adjoint6x6"],
+                    choices=["hubbard8x8", "chain10", "mixed", "moire196", "adjoint6x6", "gpsr4x4"],
                     help="hubbard8x8 is the headline; the others print their own line (default length: the "
                          "whole configured run)")
     ap.add_argument("--steps", type=int, default=None,
@@ -340,7 +389,7 @@ def main():
         except Exception:
             sync_side = lambda: None  # noqa: E731
         {"chain10": side_chain10, "mixed": side_mixed, "moire196": side_moire196,
-         "adjoint6x6": side_adjoint6x6}[args.workload](args, sync_side)
+         "adjoint6x6": side_adjoint6x6, "gpsr4x4": side_gpsr4x4}[args.workload](args, sync_side)
         return
     if args.steps is None:
         args.steps = 4000

@@ -13,10 +13,10 @@ C ABI of include/fps_ftwa.h.  Nothing here computes on the CPU.
 from ._lib import (COULOMB_GENERAL, COULOMB_NULL, COULOMB_ONSITE, ENGINE_FP32_BATCHED, ENGINE_FP64_BATCHED,
                    ENGINE_FP64_PLAIN, Engine, FtwaError, device_info, ground_state, load, load_initial_state_json,
                    load_system_json, run_json_file)
-from .solver import FHHFSolver, FHSystem, FHfTWASolver
+from .solver import FHGPSRSolver, FHHFSolver, FHSystem, FHfTWASolver
 
 __all__ = [
-    "Engine", "FtwaError", "FHHFSolver", "FHSystem", "FHfTWASolver", "device_info", "ground_state", "load",
+    "Engine", "FtwaError", "FHGPSRSolver", "FHHFSolver", "FHSystem", "FHfTWASolver", "device_info", "ground_state", "load",
     "load_initial_state_json", "load_system_json", "run_json_file",
     "COULOMB_NULL", "COULOMB_ONSITE", "COULOMB_GENERAL", "ENGINE_FP64_PLAIN", "ENGINE_FP64_BATCHED",
     "ENGINE_FP32_BATCHED",

@@ -18,7 +18,7 @@ EINVAL, EDEVICE, ENOMEM, ESTATE, EUNSUPPORTED, EIO = -1, -2, -3, -4, -5, -6
 
 COULOMB_NULL, COULOMB_ONSITE, COULOMB_GENERAL = 0, 1, 2
 PULSE_NONE, PULSE_GAUSSIAN, PULSE_KNOTS = 0, 1, 2
-METHOD_FTWA, METHOD_HF = 0, 1
+METHOD_FTWA, METHOD_HF, METHOD_GPSR = 0, 1, 2
 ENGINE_FP64_PLAIN, ENGINE_FP64_BATCHED, ENGINE_FP32_BATCHED = 0, 1, 2
 ENGINES = {"fp64_plain": ENGINE_FP64_PLAIN, "fp64": ENGINE_FP64_BATCHED, "fp32": ENGINE_FP32_BATCHED}
 KERNEL_AUTO, KERNEL_FULL, KERNEL_HERMITIAN, KERNEL_DENSE, KERNEL_PIPELINED, KERNEL_SEQUENTIAL = 0, 1, 2, 3, 4, 5
@@ -317,9 +317,11 @@ class Engine:
         return out.reshape(-1, T)
 
     def set_method(self, method: str):
-        """"ftwa" (default) or "hf": time-dependent Hartree-Fock (one deterministic state, Fock exchange)."""
-        check(self._lib.fpsftwa_set_method(self._h, {"ftwa": METHOD_FTWA, "hf": METHOD_HF}[method.lower()]),
-              "set_method")
+        """"ftwa" (default), "hf" (time-dependent Hartree-Fock: one deterministic state, Fock exchange) or
+        "gpsr" (Gaussian phase-space representation: stochastic trajectories of the normally ordered density
+        matrix, diffusion noise from the run's seed)."""
+        m = {"ftwa": METHOD_FTWA, "hf": METHOD_HF, "gpsr": METHOD_GPSR}[method.lower()]
+        check(self._lib.fpsftwa_set_method(self._h, m), "set_method")
 
     def set_recorded_element(self, i: int, j: int):
         """The element rho(i, j) behind result("N1hu") / result("N1hd") (default (0, L-1))."""

@@ -93,6 +93,7 @@ struct fpsftwa_handle {
     std::vector<double> bases;  // [n_bases][2][L][L] real
     int rec_i = 0, rec_j = -1;  // recorded off-diagonal element (j < 0: L - 1)
     int method = FPSFTWA_METHOD_FTWA;
+    int method_run = FPSFTWA_METHOD_FTWA;  // the method of the completed run (result normalisation)
     std::vector<double> noise;
     int nt_hint = 0;
     int time_chunk = 0;
@@ -112,6 +113,7 @@ struct fpsftwa_handle {
     // device buffers
     DevBuf sys_buf, state, rec, part, acc_d, work, noise_d, rot_d, scratch, occ_d, dense_buf;
     DevBuf bases_d, dens_part, dens_S, basis_work, basis_out;  // basis occupations
+    DevBuf gnoise;                                              // GPSR diffusion noise of a time chunk
     bool state_double = true;
 };
 
@@ -253,6 +255,15 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
     ds.amp = amp.empty() ? nullptr : reinterpret_cast<const double*>(base + o_amp);
     ds.rows = rows.empty() ? nullptr : base + o_rows;
     ds.fock = (h->method == FPSFTWA_METHOD_HF && h->coulomb_type == FPSFTWA_COULOMB_GENERAL) ? 1 : 0;
+    ds.gpsr = h->method == FPSFTWA_METHOD_GPSR ? 1 : 0;
+    if (ds.gpsr && h->coulomb_type == FPSFTWA_COULOMB_ONSITE) {
+        // diffusion constant sqrt(i U0), U0 = Re U(0,0) (src/FHGPSRSolver.cpp:217, :630); diffusion only
+        // for an on-site interaction (:248)
+        const double u0 = h->U[0];
+        const double m = std::sqrt(std::fabs(u0) / 2.0);
+        ds.gc_re = m;
+        ds.gc_im = u0 >= 0.0 ? m : -m;
+    }
     return FPSFTWA_OK;
 }
 
@@ -291,7 +302,7 @@ int sample_states(fpsftwa_handle* h, const DevSystem<Real>& ds)
     HIP_TRY(h->state.ensure((size_t)h->B * NE * 2 * sizeof(Real)));
     const double* occ = h->occ_d.as<double>();
     const double* noise = h->noise.empty() ? nullptr : h->noise_d.as<double>();
-    if (h->method == FPSFTWA_METHOD_HF) {  // the state itself, no fluctuations: zero noise
+    if (h->method == FPSFTWA_METHOD_HF || h->method == FPSFTWA_METHOD_GPSR) {  // the state itself: zero noise
         const size_t nz = (size_t)h->B * 2 * ((size_t)L * (L - 1) / 2) * 2;
         HIP_TRY(h->noise_d.ensure(std::max<size_t>(nz, 1) * sizeof(double)));
         HIP_TRY(hipMemsetAsync(h->noise_d.p, 0, std::max<size_t>(nz, 1) * sizeof(double), h->stream));
@@ -475,6 +486,58 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     return FPSFTWA_OK;
 }
 
+// GPSR on the batched engines (ftwa_gpsr.hip): records R = 2L + 8 [+ L(L+1)] + 2L (imaginary diagonals last).
+template <typename Real>
+int run_gpsr_batched(fpsftwa_handle* h, EventTimer& kt)
+{
+    DevSystem<Real> ds;
+    int rc = upload_system<Real>(h, ds, h->T);
+    if (rc) return rc;
+    if (!gpsr_fits<Real>(ds))
+        return fail(FPSFTWA_EUNSUPPORTED, "GPSR on the batched engines needs L <= 32, at most 8 neighbours per site "
+                                          "and no pulse; the plain FP64 engine runs any system");
+    if ((rc = sample_states<Real>(h, ds))) return rc;
+    h->used_herm = false;
+    h->used_dense = false;
+    const int L = h->L;
+    const int dens = h->n_bases > 0 ? 1 : 0;
+    const int R = 2 * L + 8 + dens * L * (L + 1) + 2 * L;
+    const int Q = 6 * L + 12;
+    int Tc = h->time_chunk;
+    if (Tc <= 0) {  // records and the chunk's diffusion noise within a few GB
+        const double per_tp = (double)h->B * R * sizeof(Real) + (double)h->nsub * h->B * 4 * L * sizeof(Real);
+        Tc = (int)std::max(1.0, std::min(64.0, (dens ? 8e9 : 3e9) / per_tp));
+    }
+    Tc = std::min(Tc, h->T);
+    const int nchunk = reduce_chunks(h->B);
+    HIP_TRY(h->rec.ensure((size_t)Tc * h->B * R * sizeof(Real)));
+    HIP_TRY(h->part.ensure((size_t)Tc * nchunk * Q * 2 * sizeof(double)));
+    HIP_TRY(h->gnoise.ensure(std::max<size_t>(gpsr_noise_elems<Real>(L, h->B, h->nsub, Tc), 1) * sizeof(Real)));
+    if ((rc = prepare_bases(h, Tc))) return rc;
+    RunShape rs;
+    rs.B = h->B;
+    rs.T = h->T;
+    rs.nsub = h->nsub;
+    rs.h = h->dt / h->nsub;
+    rs.R = R;
+    rs.dens = dens;
+    rs.ci = h->rec_i;
+    rs.cj = h->rec_j < 0 ? L - 1 : h->rec_j;
+    rs.seed = h->seed;
+    for (int t0 = 0; t0 < h->T; t0 += Tc) {
+        const int t1 = std::min(h->T, t0 + Tc);
+        HIP_TRY(kt.mark(h->stream));
+        HIP_TRY(launch_gpsr<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), h->gnoise.as<Real>(),
+                                  h->stream));
+        HIP_TRY(kt.mark(h->stream));
+        ++h->launches;
+        HIP_TRY(launch_reduce<Real>(L, h->B, R, t1 - t0, t0, h->rec.as<Real>(), h->part.as<double>(), nchunk,
+                                    h->acc_d.as<double>(), h->stream, R - 2 * L));
+        if ((rc = reduce_bases<Real>(h, R, t1 - t0, t0))) return rc;
+    }
+    return FPSFTWA_OK;
+}
+
 int run_plain(fpsftwa_handle* h, EventTimer& kt)
 {
     DevSystem<double> ds;
@@ -483,7 +546,8 @@ int run_plain(fpsftwa_handle* h, EventTimer& kt)
     if ((rc = sample_states<double>(h, ds))) return rc;
     const int L = h->L;
     const int dens = h->n_bases > 0 ? 1 : 0;
-    const int R = 2 * L + 8 + dens * L * (L + 1);
+    const int gp = h->method == FPSFTWA_METHOD_GPSR ? 1 : 0;
+    const int R = 2 * L + 8 + dens * L * (L + 1) + gp * 2 * L;
     const int Q = 6 * L + 12;
     const int nchunk = reduce_chunks(h->B);
     HIP_TRY(h->rec.ensure((size_t)h->B * R * sizeof(double)));
@@ -499,13 +563,14 @@ int run_plain(fpsftwa_handle* h, EventTimer& kt)
     rs.dens = dens;
     rs.ci = h->rec_i;
     rs.cj = h->rec_j < 0 ? L - 1 : h->rec_j;
+    rs.seed = h->seed;
     for (int t = 0; t < h->T; ++t) {
         HIP_TRY(kt.mark(h->stream));
         HIP_TRY(launch_plain(ds, rs, t, h->state.as<double>(), h->work.as<double>(), h->rec.as<double>(), h->stream));
         HIP_TRY(kt.mark(h->stream));
         ++h->launches;
         HIP_TRY(launch_reduce<double>(L, h->B, R, 1, t, h->rec.as<double>(), h->part.as<double>(), nchunk,
-                                      h->acc_d.as<double>(), h->stream));
+                                      h->acc_d.as<double>(), h->stream, gp ? R - 2 * L : -1));
         if ((rc = reduce_bases<double>(h, R, 1, t))) return rc;
     }
     return FPSFTWA_OK;
@@ -741,6 +806,7 @@ int fpsftwa_destroy(fpsftwa_handle* h)
     h->dens_S.release();
     h->basis_work.release();
     h->basis_out.release();
+    h->gnoise.release();
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return FPSFTWA_OK;
@@ -817,7 +883,8 @@ int fpsftwa_set_bases(fpsftwa_handle* h, int32_t n_bases, const double* bases)
 int fpsftwa_set_method(fpsftwa_handle* h, int32_t method)
 {
     if (int rc = check_handle(h)) return rc;
-    if (method != FPSFTWA_METHOD_FTWA && method != FPSFTWA_METHOD_HF) return fail(FPSFTWA_EINVAL, "unknown method");
+    if (method != FPSFTWA_METHOD_FTWA && method != FPSFTWA_METHOD_HF && method != FPSFTWA_METHOD_GPSR)
+        return fail(FPSFTWA_EINVAL, "unknown method");
     h->method = method;
     h->has_result = false;
     return FPSFTWA_OK;
@@ -877,6 +944,8 @@ int fpsftwa_run(fpsftwa_handle* h)
                                           "engine only");
     h->has_result = false;
     h->T = (int)std::floor(h->t_max / h->dt + 1.0);  // src/FHSolver.cpp:100
+    if (h->method == FPSFTWA_METHOD_GPSR && (double)(h->T - 1) * h->nsub >= 4294967296.0)
+        return fail(FPSFTWA_EINVAL, "GPSR runs at most 2^32 steps (the diffusion noise counter)");
     const int Q = 6 * h->L + 12;
     h->launches = 0;
     h->kernel_ms = 0.0;
@@ -891,10 +960,10 @@ int fpsftwa_run(fpsftwa_handle* h)
         rc = run_plain(h, kt);
     } else if (h->engine == FPSFTWA_ENGINE_FP64_BATCHED) {
         h->state_double = true;
-        rc = run_batched<double>(h, kt);
+        rc = h->method == FPSFTWA_METHOD_GPSR ? run_gpsr_batched<double>(h, kt) : run_batched<double>(h, kt);
     } else {
         h->state_double = false;
-        rc = run_batched<float>(h, kt);
+        rc = h->method == FPSFTWA_METHOD_GPSR ? run_gpsr_batched<float>(h, kt) : run_batched<float>(h, kt);
     }
     if (rc) {
         (void)hipStreamSynchronize(h->stream);
@@ -913,6 +982,7 @@ int fpsftwa_run(fpsftwa_handle* h)
     h->kernel_ms = kt.pairs_ms();
     h->run_ms = whole.pairs_ms();
     h->B_run = h->B;
+    h->method_run = h->method;
     h->has_result = true;
     return FPSFTWA_OK;
 }
@@ -924,6 +994,7 @@ int fpsftwa_run_adjoint(fpsftwa_handle* h, const fpsftwa_adjoint_spec* spec, dou
     if (spec && spec->objective != FPSFTWA_OBJ_DOUBLE_OCCUPANCY) return fail(FPSFTWA_EINVAL, "unknown objective");
     if (h->coulomb_type == FPSFTWA_COULOMB_GENERAL)
         return fail(FPSFTWA_EUNSUPPORTED, "the adjoint supports null and on-site Coulomb interactions");
+    if (h->method != FPSFTWA_METHOD_FTWA) return fail(FPSFTWA_EUNSUPPORTED, "the adjoint is defined for fTWA runs");
     if (h->L > 256) return fail(FPSFTWA_EUNSUPPORTED, "the adjoint supports L <= 256");
     if (grad_pulse && n_pulse) {
         const size_t need = h->pulse_type == FPSFTWA_PULSE_KNOTS ? h->knots.size()
@@ -987,9 +1058,12 @@ int fpsftwa_get_result(fpsftwa_handle* h, int32_t which, double* out, size_t n_v
     if (needed) *needed = n;
     if (!out) return FPSFTWA_OK;
     if (n_values < n) return fail(FPSFTWA_EINVAL, "output buffer too small");
+    // GPSR's rho is normally ordered: plain means, no 1/2 (src/FHGPSRSolver.cpp:895-929)
+    const bool gpsr = h->method_run == FPSFTWA_METHOD_GPSR;
+    const double hf = gpsr ? 0.0 : 0.5;
     if (which == FPSFTWA_OUT_BASIS || which == FPSFTWA_OUT_SUM_BASIS) {
         for (size_t e = 0; e < n; ++e)  // Mkk = mkk / nTraj + 1/2 (src/FHfTWASolver.cpp:531-540)
-            out[e] = which == FPSFTWA_OUT_BASIS ? h->basis_acc[e] / B + 0.5 : h->basis_acc[e];
+            out[e] = which == FPSFTWA_OUT_BASIS ? h->basis_acc[e] / B + hf : h->basis_acc[e];
         return FPSFTWA_OK;
     }
     auto S = [&](int t, int q) { return kahan_value(h->acc, Q, t, q); };
@@ -1019,11 +1093,11 @@ int fpsftwa_get_result(fpsftwa_handle* h, int32_t which, double* out, size_t n_v
             for (int k = 0; k < 4; ++k) out[(size_t)k * T + t] = S(t, 6 * L + k);
             break;
         case FPSFTWA_OUT_N1HU:
-            out[t] = S(t, 6 * L + 8) / B + (on_diag ? 0.5 : 0.0);
+            out[t] = S(t, 6 * L + 8) / B + (on_diag ? hf : 0.0);
             out[T + t] = S(t, 6 * L + 9) / B;
             break;
         case FPSFTWA_OUT_N1HD:
-            out[t] = S(t, 6 * L + 10) / B + (on_diag ? 0.5 : 0.0);
+            out[t] = S(t, 6 * L + 10) / B + (on_diag ? hf : 0.0);
             out[T + t] = S(t, 6 * L + 11) / B;
             break;
         default:
@@ -1031,6 +1105,29 @@ int fpsftwa_get_result(fpsftwa_handle* h, int32_t which, double* out, size_t n_v
                 const double su = S(t, 6 * i), sd = S(t, 6 * i + 1), suu = S(t, 6 * i + 2), sdd = S(t, 6 * i + 3),
                              sud = S(t, 6 * i + 4), snn = S(t, 6 * i + 5);
                 double v = 0.0;
+                if (gpsr) {
+                    // Niiu = <rho_ii>, N2iiu = <|rho_ii|^2>, NuNd = <rho_up_ii rho_dn_ii>, real parts
+                    // (src/FHGPSRSolver.cpp:895-914); the error of Niiu uses |rho_ii|^2, an upper bound of
+                    // (Re rho_ii)^2
+                    switch (which) {
+                    case FPSFTWA_OUT_NIIU: v = su / B; break;
+                    case FPSFTWA_OUT_NIID: v = sd / B; break;
+                    case FPSFTWA_OUT_N2IIU: v = suu / B; break;
+                    case FPSFTWA_OUT_N2IID: v = sdd / B; break;
+                    case FPSFTWA_OUT_NUND: v = sud / B; break;
+                    case FPSFTWA_OUT_NIIU_ERR: v = stderr_of(su, suu); break;
+                    case FPSFTWA_OUT_NIID_ERR: v = stderr_of(sd, sdd); break;
+                    case FPSFTWA_OUT_NUND_ERR: v = stderr_of(sud, snn); break;
+                    case FPSFTWA_OUT_SUM_RHO_U: v = su; break;
+                    case FPSFTWA_OUT_SUM_RHO_D: v = sd; break;
+                    case FPSFTWA_OUT_SUM_RHO2_U: v = suu; break;
+                    case FPSFTWA_OUT_SUM_RHO2_D: v = sdd; break;
+                    case FPSFTWA_OUT_SUM_RHOUD: v = sud; break;
+                    default: break;
+                    }
+                    out[(size_t)i * T + t] = v;
+                    continue;
+                }
                 switch (which) {
                 case FPSFTWA_OUT_NIIU: v = su / B + 0.5; break;              // src/FHfTWASolver.cpp:509
                 case FPSFTWA_OUT_NIID: v = sd / B + 0.5; break;              // :510
@@ -1146,6 +1243,7 @@ extern "C" int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t*
     if (int rc = check_handle(h)) return rc;
     if (trial_steps < 1) return fail(FPSFTWA_EINVAL, "trial_steps must be >= 1");
     if (h->engine == FPSFTWA_ENGINE_FP64_PLAIN) return fail(FPSFTWA_EINVAL, "the plain engine has no tuning");
+    if (h->method == FPSFTWA_METHOD_GPSR) return fail(FPSFTWA_EINVAL, "GPSR has a single batched kernel");
     struct Cand {
         int kernel, block, chunk;
     };

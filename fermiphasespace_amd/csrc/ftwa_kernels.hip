@@ -28,42 +28,13 @@ using dev::V2;
 using dev::Neumaier;
 using dev::wave_sum;
 
-// ---- Philox4x32-10 (Salmon et al., SC'11) -------------------------------------------
-__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
-        const uint32_t lo0 = 0xD2511F53u * c[0];
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
-        const uint32_t lo1 = 0xCD9E8D57u * c[2];
-        const uint32_t n0 = hi1 ^ c[1] ^ k0;
-        const uint32_t n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0;
-        c[1] = lo1;
-        c[2] = n2;
-        c[3] = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-}
-
-// Complex noise value for (trajectory, spin, pair): Box-Muller on two 53-bit uniforms,
-// E|w|^2 = 1 with independent N(0, 1/2) real and imaginary parts.
+// Complex noise value for (trajectory, spin, pair) of the initial state (dev::box_muller).
 __device__ __forceinline__ void noise_value(uint64_t seed, int64_t traj, int spin, uint32_t pair, double& re,
                                             double& im)
 {
     uint32_t c[4] = {pair, static_cast<uint32_t>(spin), static_cast<uint32_t>(static_cast<uint64_t>(traj)),
                      static_cast<uint32_t>(static_cast<uint64_t>(traj) >> 32)};
-    philox4x32_10(c, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
-    const double two_m53 = 1.1102230246251565404e-16;
-    const double u1 = (static_cast<double>(c[0] >> 5) * 67108864.0 + static_cast<double>(c[1] >> 6) + 1.0) * two_m53;
-    const double u2 = (static_cast<double>(c[2] >> 5) * 67108864.0 + static_cast<double>(c[3] >> 6)) * two_m53;
-    const double r = sqrt(-log(u1));
-    double sn, cs;
-    sincos(6.283185307179586232 * u2, &sn, &cs);
-    re = r * cs;
-    im = r * sn;
+    dev::box_muller(c, seed, re, im);
 }
 
 __device__ __forceinline__ int64_t pair_index(int L, int i, int j)  // i < j, row-major strictly upper
@@ -74,7 +45,7 @@ __device__ __forceinline__ int64_t pair_index(int L, int i, int j)  // i < j, ro
 // ---- sampling -------------------------------------------------------------------------
 template <typename Real>
 __global__ void sample_kernel(int L, const double* __restrict__ n_up, const double* __restrict__ n_dn,
-                              const double* __restrict__ noise, uint64_t seed, int64_t b0, int64_t B,
+                              const double* __restrict__ noise, uint64_t seed, int64_t b0, int64_t B, double shift,
                               Real* __restrict__ state)
 {
     const int64_t LL = static_cast<int64_t>(L) * L;
@@ -91,7 +62,7 @@ __global__ void sample_kernel(int L, const double* __restrict__ n_up, const doub
         const double* n = sg ? n_dn : n_up;
         double re = 0.0, im = 0.0;
         if (i == j) {
-            re = n[i] - 0.5;
+            re = n[i] - shift;
         } else {
             const int a = i < j ? i : j;
             const int c = i < j ? j : i;
@@ -205,8 +176,12 @@ __global__ void __launch_bounds__(kPlainThreads)
     double* Ys = wk;
     double* K = Ys + NE * 2;       // [4][NE][2]
     double* D = K + 4 * NE * 2;    // [2][L][L][2]
-    double* hc = D + NE * 2;       // [2][L][2]
-    const bool general = s.coulomb_type == 2;
+    double* Dd = D + NE * 2;       // [2][L][L][2] GPSR diffusion increment
+    double* hc = Dd + NE * 2;      // [2][L][2]
+    double* xi = hc + 4 * L;       // [2][L][2] GPSR step noise xi_1, xi_2
+    const bool gpsr = s.gpsr != 0;
+    const bool general = s.coulomb_type == 2 && !gpsr;  // GPSR's Delta has no V (src/FHGPSRSolver.cpp:112-113)
+    const double hf = gpsr ? 0.0 : 0.5;                 // N = rho + 1/2 (fTWA, HF) or rho (GPSR)
     const double h = r.h;
 
     // ---- record (src/FHfTWASolver.cpp:366-418) ----
@@ -215,6 +190,7 @@ __global__ void __launch_bounds__(kPlainThreads)
         for (int e = tid; e < 2 * L; e += NT) {
             const int sg = e / L, i = e - sg * L;
             rr[e] = y[2 * (sg * LL + i * L + i)];
+            if (gpsr) rr[r.R - 2 * L + e] = y[2 * (sg * LL + i * L + i) + 1];
         }
         if (tid == 0) {
             const int c = r.ci * L + r.cj;
@@ -223,14 +199,16 @@ __global__ void __launch_bounds__(kPlainThreads)
             rr[2 * L + 6] = y[2 * (LL + c)];
             rr[2 * L + 7] = y[2 * (LL + c) + 1];
         }
-        if (r.dens)
-            dev::record_density(rr + 2 * L + 8, L, tid & 63, tid >> 6, (NT + 63) / 64,
-                                [&](int row, int col) { return y[2 * (row * L + col)]; });
+        if (r.dens)  // GPSR's rho is not Hermitian: the symmetric part (Re rho_ij + Re rho_ji) / 2
+            dev::record_density(rr + 2 * L + 8, L, tid & 63, tid >> 6, (NT + 63) / 64, [&](int row, int col) {
+                const int sr = row / L, i = row - sr * L;
+                return gpsr ? (y[2 * (row * L + col)] + y[2 * ((sr * L + col) * L + i)]) / 2.0 : y[2 * (row * L + col)];
+            });
         double eth = 0, eu = 0, ev = 0, epe = 0;
         const double a = s.amp ? s.amp[static_cast<int64_t>(t) * r.nsub * 4] : 0.0;
         for (int i = tid; i < L; i += NT) {
             for (int jj = 0; jj < L; ++jj) {
-                const double nr = y[2 * (i * L + jj)] + y[2 * (LL + i * L + jj)] + (i == jj ? 1.0 : 0.0);
+                const double nr = y[2 * (i * L + jj)] + y[2 * (LL + i * L + jj)] + (i == jj ? 2.0 * hf : 0.0);
                 const double ni = y[2 * (i * L + jj) + 1] + y[2 * (LL + i * L + jj) + 1];
                 const double tt = s.tM[i * L + jj];
                 eth += nr * tt;
@@ -240,13 +218,13 @@ __global__ void __launch_bounds__(kPlainThreads)
                     epe += nr * tt * cs - ni * tt * sn;
                 }
             }
-            const double nu = y[2 * (i * L + i)] + 0.5, nui = y[2 * (i * L + i) + 1];
-            const double nd = y[2 * (LL + i * L + i)] + 0.5, ndi = y[2 * (LL + i * L + i) + 1];
+            const double nu = y[2 * (i * L + i)] + hf, nui = y[2 * (i * L + i) + 1];
+            const double nd = y[2 * (LL + i * L + i)] + hf, ndi = y[2 * (LL + i * L + i) + 1];
             epe += s.pot[i] * (nu + nd);
             if (s.coulomb_type != 0) {
                 for (int jj = 0; jj < L; ++jj) {
-                    const double nuj = y[2 * (jj * L + jj)] + 0.5, nujj = y[2 * (jj * L + jj) + 1];
-                    const double ndj = y[2 * (LL + jj * L + jj)] + 0.5, ndjj = y[2 * (LL + jj * L + jj) + 1];
+                    const double nuj = y[2 * (jj * L + jj)] + hf, nujj = y[2 * (jj * L + jj) + 1];
+                    const double ndj = y[2 * (LL + jj * L + jj)] + hf, ndjj = y[2 * (LL + jj * L + jj) + 1];
                     const double u = s.U[i * L + jj];
                     eu += u * ((nu * ndj - nui * ndjj) + (nd * nuj - ndi * nujj)) / 2.0;
                     if (general) {
@@ -283,6 +261,49 @@ __global__ void __launch_bounds__(kPlainThreads)
 
     for (int sub = 0; sub < r.nsub; ++sub) {
         const int64_t g = static_cast<int64_t>(t) * r.nsub + sub;
+        if (gpsr) {
+            // FHGPSRSolver::step (src/FHGPSRSolver.cpp:608-637): a trajectory holding a NaN or Inf becomes
+            // all NaN and is not stepped; otherwise the diffusion increment from the step's initial state,
+            // the analytic form of step_Diffusion (:211-232): c (rhoH W1 rho + i rho W2 rhoH), spin down
+            // with conj(W); the noise has variance h per step
+            int bad = 0;
+            for (int e = tid; e < 2 * NE; e += NT) bad |= !isfinite(y[e]);
+            if (__syncthreads_or(bad)) {
+                for (int e = tid; e < 2 * NE; e += NT) y[e] = __builtin_nan("");
+                continue;
+            }
+            const double sh = sqrt(h);
+            for (int e = tid; e < 2 * L; e += NT) {
+                double re, im;
+                dev::gpsr_noise(r.seed, r.b0 + b, g, e / L, e % L, re, im);
+                xi[2 * e] = re * sh;
+                xi[2 * e + 1] = im * sh;
+            }
+            __syncthreads();
+            for (int e = tid; e < NE; e += NT) {
+                const int sg = e / LL, ij = e - sg * LL, i = ij / L, jj = ij - i * L;
+                const double* Ym = y + 2 * sg * LL;
+                const double cs = sg ? -1.0 : 1.0;
+                double ar = 0, ai = 0, br = 0, bi = 0;
+                for (int k = 0; k < L; ++k) {
+                    const double hr = (i == k ? 1.0 : 0.0) - Ym[2 * (i * L + k)], hi = -Ym[2 * (i * L + k) + 1];
+                    const double w1r = xi[2 * k], w1i = cs * xi[2 * k + 1];
+                    const double m1r = hr * w1r - hi * w1i, m1i = hr * w1i + hi * w1r;
+                    const double pr = Ym[2 * (k * L + jj)], pi = Ym[2 * (k * L + jj) + 1];
+                    ar += m1r * pr - m1i * pi;
+                    ai += m1r * pi + m1i * pr;
+                    const double qr = Ym[2 * (i * L + k)], qi = Ym[2 * (i * L + k) + 1];
+                    const double w2r = xi[2 * (L + k)], w2i = cs * xi[2 * (L + k) + 1];
+                    const double m2r = qr * w2r - qi * w2i, m2i = qr * w2i + qi * w2r;
+                    const double gr = (k == jj ? 1.0 : 0.0) - Ym[2 * (k * L + jj)], gi = -Ym[2 * (k * L + jj) + 1];
+                    br += m2r * gr - m2i * gi;
+                    bi += m2r * gi + m2i * gr;
+                }
+                const double sr = ar - bi, si = ai + br;  // (rhoH W1 rho) + i (rho W2 rhoH)
+                Dd[2 * e] = s.gc_re * sr - s.gc_im * si;
+                Dd[2 * e + 1] = s.gc_re * si + s.gc_im * sr;
+            }
+        }
         for (int st = 0; st < 4; ++st) {
             const double* Yin = (st == 0) ? y : Ys;
             // Hartree potential, complex as in the reference (U (rho_diag + 1/2))
@@ -292,7 +313,7 @@ __global__ void __launch_bounds__(kPlainThreads)
                 if (s.coulomb_type != 0) {
                     for (int jj = 0; jj < L; ++jj) {
                         const double u = s.U[i * L + jj];
-                        hr += u * (Yin[2 * ((1 - sg) * LL + jj * L + jj)] + 0.5);
+                        hr += u * (Yin[2 * ((1 - sg) * LL + jj * L + jj)] + hf);
                         hi += u * Yin[2 * ((1 - sg) * LL + jj * L + jj) + 1];
                     }
                     if (general)
@@ -361,6 +382,8 @@ __global__ void __launch_bounds__(kPlainThreads)
                     Ys[e] = y[e] + (-K1[e] / 3.0 + K2[e]) * h;
                 else if (st == 2)
                     Ys[e] = y[e] + (K1[e] - K2[e] + K3[e]) * h;
+                else if (gpsr)
+                    y[e] = y[e] + (K1[e] + K2[e] * 3.0 + K3[e] * 3.0 + K4[e]) * h / 8.0 + Dd[e];
                 else
                     y[e] = y[e] + (K1[e] + K2[e] * 3.0 + K3[e] * 3.0 + K4[e]) * h / 8.0;
             }
@@ -373,8 +396,11 @@ __global__ void __launch_bounds__(kPlainThreads)
 
 // ---- ensemble reduction ---------------------------------------------------------------------
 
+// im_off >= 0 (GPSR records): Im rho_up_ii, Im rho_dn_ii sit at [im_off, im_off + 2L) and the sums are
+// of the complex diagonals: S_uu = sum |rho_up_ii|^2, S_ud = sum Re(rho_up_ii rho_dn_ii), the last slot
+// sum Re(rho_up_ii rho_dn_ii)^2.
 template <typename Real>
-__global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, const Real* __restrict__ rec,
+__global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, int im_off, const Real* __restrict__ rec,
                                       double* __restrict__ part)
 {
     const int t = blockIdx.x;
@@ -388,16 +414,32 @@ __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, const
     for (int slot = threadIdx.x; slot < L + 8; slot += blockDim.x) {
         if (slot < L) {
             Neumaier su, sd, suu, sdd, sud, snn;
-            for (int64_t bi = bb0; bi < bb1; ++bi) {
-                const double u = static_cast<double>(rt[bi * R + slot]);
-                const double d = static_cast<double>(rt[bi * R + L + slot]);
-                su.add(u);
-                sd.add(d);
-                suu.add(u * u);
-                sdd.add(d * d);
-                sud.add(u * d);
-                const double nn = (u + 0.5) * (d + 0.5);
-                snn.add(nn * nn);
+            if (im_off < 0) {
+                for (int64_t bi = bb0; bi < bb1; ++bi) {
+                    const double u = static_cast<double>(rt[bi * R + slot]);
+                    const double d = static_cast<double>(rt[bi * R + L + slot]);
+                    su.add(u);
+                    sd.add(d);
+                    suu.add(u * u);
+                    sdd.add(d * d);
+                    sud.add(u * d);
+                    const double nn = (u + 0.5) * (d + 0.5);
+                    snn.add(nn * nn);
+                }
+            } else {
+                for (int64_t bi = bb0; bi < bb1; ++bi) {
+                    const double u = static_cast<double>(rt[bi * R + slot]);
+                    const double d = static_cast<double>(rt[bi * R + L + slot]);
+                    const double ui = static_cast<double>(rt[bi * R + im_off + slot]);
+                    const double di = static_cast<double>(rt[bi * R + im_off + L + slot]);
+                    su.add(u);
+                    sd.add(d);
+                    suu.add(u * u + ui * ui);
+                    sdd.add(d * d + di * di);
+                    const double ud = u * d - ui * di;
+                    sud.add(ud);
+                    snn.add(ud * ud);
+                }
             }
             const Neumaier* all[6] = {&su, &sd, &suu, &sdd, &sud, &snn};
             for (int k = 0; k < 6; ++k) {
@@ -498,8 +540,9 @@ hipError_t launch_sample(const DevSystem<Real>& s, const double* n_up, const dou
                          uint64_t seed, int64_t b0, int64_t B, Real* state, hipStream_t st)
 {
     const int64_t total = B * 2 * static_cast<int64_t>(s.L) * s.L;
+    const double shift = s.gpsr ? 0.0 : 0.5;  // GPSR: the normally ordered diag(n), no 1/2
     hipLaunchKernelGGL(sample_kernel<Real>, dim3(grid_for(total, 256)), dim3(256), 0, st, s.L, n_up, n_dn, noise, seed,
-                       b0, B, state);
+                       b0, B, shift, state);
     return hipGetLastError();
 }
 
@@ -557,12 +600,12 @@ hipError_t launch_density_reduce(int L, int64_t B, int R, int nt, const Real* re
 
 template <typename Real>
 hipError_t launch_reduce(int L, int64_t B, int R, int nt, int t0, const Real* rec, double* part, int nchunk,
-                         double* acc, hipStream_t st)
+                         double* acc, hipStream_t st, int im_off)
 {
     const int Q = 6 * L + 12;
     const int threads = ((L + 8 + 63) / 64) * 64 > 256 ? 256 : ((L + 8 + 63) / 64) * 64;
-    hipLaunchKernelGGL(reduce_partial_kernel<Real>, dim3(nt, nchunk), dim3(threads), 0, st, L, B, R, nchunk, rec,
-                       part);
+    hipLaunchKernelGGL(reduce_partial_kernel<Real>, dim3(nt, nchunk), dim3(threads), 0, st, L, B, R, nchunk, im_off,
+                       rec, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int64_t total = static_cast<int64_t>(nt) * Q;
@@ -576,7 +619,8 @@ hipError_t launch_reduce(int L, int64_t B, int R, int nt, int t0, const Real* re
 template hipError_t launch_sample<float>(const DevSystem<float>&, const double*, const double*, const double*, uint64_t,
                                          int64_t, int64_t, float*, hipStream_t);
 template hipError_t launch_rotate<float>(int, const double*, int64_t, float*, float*, hipStream_t);
-template hipError_t launch_reduce<float>(int, int64_t, int, int, int, const float*, double*, int, double*, hipStream_t);
+template hipError_t launch_reduce<float>(int, int64_t, int, int, int, const float*, double*, int, double*, hipStream_t,
+                                          int);
 template hipError_t launch_density_reduce<float>(int, int64_t, int, int, const float*, double*, int, double*,
                                                  hipStream_t);
 #else
@@ -584,7 +628,7 @@ template hipError_t launch_sample<double>(const DevSystem<double>&, const double
                                           uint64_t, int64_t, int64_t, double*, hipStream_t);
 template hipError_t launch_rotate<double>(int, const double*, int64_t, double*, double*, hipStream_t);
 template hipError_t launch_reduce<double>(int, int64_t, int, int, int, const double*, double*, int, double*,
-                                          hipStream_t);
+                                          hipStream_t, int);
 template hipError_t launch_density_reduce<double>(int, int64_t, int, int, const double*, double*, int, double*,
                                                   hipStream_t);
 #endif

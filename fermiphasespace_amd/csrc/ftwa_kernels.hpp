@@ -5,7 +5,9 @@
 //   rec    [Tc][B][R] Real, R = 2L + 8 per-trajectory records of one time point:
 //            [0,L) rho_up_ii  [L,2L) rho_dn_ii  2L+0..3 E_tH, E_U, E_V, E_pe  2L+4..7 re/im of rho_up(ci, cj), rho_dn(ci, cj)
 //            and, when basis occupations are requested, R = 2L + 8 + L(L+1): [2L+8, R) Re rho_sigma(i, j),
-//            j >= i, spin-major, rows packed (dev::packed_row)
+//            j >= i, spin-major, rows packed (dev::packed_row); GPSR records end with 2L more values,
+//            Im rho_up_ii, Im rho_dn_ii (GPSR's rho is not Hermitian; the density part then holds the
+//            symmetric part (Re rho_ij + Re rho_ji) / 2)
 //   part   [Tc][nchunk][Q][2] double   partial ensemble sums (sum, compensation)
 //   acc    [T][Q][2] double            ensemble sums (sum, compensation), Q = 6L + 12:
 //            per site i: S_u, S_d, S_uu, S_dd, S_ud, S_(nu nd)^2 at [6i .. 6i+5];
@@ -51,6 +53,9 @@ struct DevSystem {
     const Real* pd = nullptr;      // [L][L]  pol . (r_i - r_j) dense (plain path, pulse)
     const void* rows = nullptr;    // [L] RowNbrs<Real, z> (batched path)
     int fock = 0;                  // 1: Hartree-Fock dynamics, Delta_sigma gains -V o N_sigma (plain path)
+    int gpsr = 0;                  // 1: GPSR dynamics: normally ordered rho (no 1/2), Delta_sigma = tM +
+                                   //    diag(U rho_sbar_ii) + potential (no V), diffusion every step
+    double gc_re = 0.0, gc_im = 0.0;  // GPSR diffusion constant sqrt(i U(0,0)) (0: no diffusion)
 };
 
 struct RunShape {
@@ -63,10 +68,11 @@ struct RunShape {
     int dens = 0;         // 1: records also carry the packed real density matrices (R = 2L + 8 + L(L+1))
     int ci = 0, cj = 0;   // the off-diagonal element rho(ci, cj) recorded per time point: the wholeDMatrix
                           // corner (0, L-1), or the core-hole element (site, L-1)
+    uint64_t seed = 0;    // key of the GPSR diffusion noise
 };
 
-// Sampling: rho_sigma = diag(n - 1/2) + S.W (rotation applied separately).  noise (optional,
-// FP64 [B][2][npairs][2]) replaces the counter-based generator.
+// Sampling: rho_sigma = diag(n - 1/2) + S.W (rotation applied separately; GPSR: diag(n), no 1/2).  noise
+// (optional, FP64 [B][2][npairs][2]) replaces the counter-based generator.
 template <typename Real>
 hipError_t launch_sample(const DevSystem<Real>& s, const double* n_up, const double* n_dn, const double* noise,
                          uint64_t seed, int64_t b0, int64_t B, Real* state, hipStream_t st);
@@ -98,8 +104,6 @@ bool herm_pipe_fits(const DevSystem<float>& s);
 hipError_t launch_herm_pipe(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                             hipStream_t st);
 
-// Dense batched path (ftwa_dense.hip): one library GEMM X = Y T per RK stage for the whole batch,
-// hand-written Hartree, commutator/RK update and record kernels.  L <= 1024, any neighbour count.
 // Spin-sequential upper-triangle integrator (ftwa_herm_seq.hip), FP64 and FP32: L <= 64, z 4 or 8, no
 // pulse, null or on-site Coulomb, no density records.  nt_hint 1024 selects 16 waves per workgroup,
 // otherwise 8.
@@ -109,6 +113,19 @@ template <typename Real>
 hipError_t launch_herm_seq(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
                            int nt_hint, hipStream_t st);
 
+// Batched GPSR integrator (ftwa_gpsr.hip), FP64 and FP32: full (non-Hermitian) matrices in LDS, RK4 "3/8"
+// drift plus the diffusion increment every step; L <= 32, z 4 or 8, no pulse.  noise holds
+// gpsr_noise_elems(L, B, nsub, t1 - t0) Reals: the chunk's diffusion noise, drawn by the launch itself.
+template <typename Real>
+bool gpsr_fits(const DevSystem<Real>& s);
+template <typename Real>
+size_t gpsr_noise_elems(int L, int64_t B, int nsub, int nt);
+template <typename Real>
+hipError_t launch_gpsr(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
+                       Real* noise, hipStream_t st);
+
+// Dense batched path (ftwa_dense.hip): one library GEMM X = Y T per RK stage for the whole batch,
+// hand-written Hartree, commutator/RK update and record kernels.  L <= 1024, any neighbour count.
 struct DenseCtx;
 DenseCtx* dense_create(hipStream_t st);
 void dense_destroy(DenseCtx* c);
@@ -124,11 +141,12 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
 // Plain FP64 path: one launch per time point, dense Delta, global-memory products.
 hipError_t launch_plain(const DevSystem<double>& s, const RunShape& r, int t, double* state, double* work,
                         double* rec, hipStream_t st);
-// Per-trajectory FP64 scratch of the plain path: stage input, k1..k4, Delta (2 spins each) and h.
+// Per-trajectory FP64 scratch of the plain path: stage input, k1..k4, Delta, the GPSR diffusion
+// increment (2 spins each), h and the GPSR step noise.
 __host__ __device__ inline size_t plain_work_doubles(int L)
 {
     const size_t LL = static_cast<size_t>(L) * L;
-    return 6 * 2 * LL * 2 + 4 * static_cast<size_t>(L);
+    return 7 * 2 * LL * 2 + 8 * static_cast<size_t>(L);
 }
 
 // Discrete adjoint (ftwa_adjoint.hip).  Work area per trajectory: 10 + seg_steps + 1 complex matrices
@@ -151,10 +169,11 @@ hipError_t launch_adjoint_backward(const DevSystem<Real>& s, const Real* ttab, c
                                    int64_t n_stage, hipStream_t st);
 hipError_t launch_adjoint_reduce(const double* src, int64_t B, int64_t n, double* out, hipStream_t st);
 
-// Ensemble reduction of records for time points [t0, t0+nt) into acc.
+// Ensemble reduction of records for time points [t0, t0+nt) into acc.  im_off >= 0: GPSR records, whose
+// imaginary diagonals start at im_off (the sums then take the complex diagonals, see reduce_partial_kernel).
 template <typename Real>
 hipError_t launch_reduce(int L, int64_t B, int R, int nt, int t0, const Real* rec, double* part, int nchunk,
-                         double* acc, hipStream_t st);
+                         double* acc, hipStream_t st, int im_off = -1);
 int reduce_chunks(int64_t B);
 
 // Ensemble sums S[t][L(L+1)] (FP64) of the records' density part for nt time points (records with

@@ -61,6 +61,11 @@ FHHFSolver::FHHFSolver(FHSystem* fhSystem) : FHfTWASolver(fhSystem)
     engine_ = FPSFTWA_ENGINE_FP64_PLAIN;
 }
 
+FHGPSRSolver::FHGPSRSolver(FHSystem* fhSystem) : FHfTWASolver(fhSystem)
+{
+    method_ = FPSFTWA_METHOD_GPSR;
+}
+
 void FHHFSolver::setMethodParameters(const Value& p)
 {
     // defaults of src/FHHFSolver.cpp:47-57; RK4 "3/8" is forced in initiateSolver (:199)
@@ -170,7 +175,8 @@ void FHfTWASolver::setJsonFile(const Value& file) { json_ = file; }
 
 void FHfTWASolver::initiateSolver()
 {
-    std::cout << "   -- fTWA Solver initiation." << std::endl;
+    std::cout << (method_ == FPSFTWA_METHOD_GPSR ? "   -- GPSR Solver initiation." : "   -- fTWA Solver initiation.")
+              << std::endl;
     const fpsamd::FtwaSystem& s = sys_->data();
     if (!h_) {
         fpsftwa_system d{};
@@ -254,7 +260,16 @@ void FHfTWASolver::runSimulation()
     std::cout << "   -- Start simulation." << std::endl;
     check(fpsftwa_set_params(h_, dt_, tMax_, static_cast<int32_t>(nSub_), nTraj_, 4, seed_, engine_), "parameters");
     check(fpsftwa_set_results(h_, flags_), "results");
-    check(fpsftwa_run(h_), "run");
+    int rc = fpsftwa_run(h_);
+    if (rc == FPSFTWA_EUNSUPPORTED && method_ == FPSFTWA_METHOD_GPSR && engine_ != FPSFTWA_ENGINE_FP64_PLAIN) {
+        // the batched GPSR kernel covers small lattices (L <= 32, short-range hopping, no pulse); the plain
+        // FP64 engine runs the rest
+        std::cout << "   -- note: " << fpsftwa_last_error() << "; running GPSR on the fp64_plain engine." << std::endl;
+        engine_ = FPSFTWA_ENGINE_FP64_PLAIN;
+        check(fpsftwa_set_params(h_, dt_, tMax_, static_cast<int32_t>(nSub_), nTraj_, 4, seed_, engine_), "parameters");
+        rc = fpsftwa_run(h_);
+    }
+    check(rc, "run");
     const int L = sys_->data().L;
     std::vector<double> t = fetch(h_, FPSFTWA_OUT_TIME);
     const size_t T = t.size();
@@ -324,6 +339,8 @@ void FHfTWASolver::runSimulation()
     info_ = Value::object();
     info_["engine"] = engineName(engine_);
     if (method_ == FPSFTWA_METHOD_HF) info_["method"] = "HF (one deterministic state, Fock exchange with V)";
+    if (method_ == FPSFTWA_METHOD_GPSR)
+        info_["method"] = "GPSR (normally ordered density matrix, analytic diffusion of variance h per step)";
     info_["seed"] = static_cast<int64_t>(seed_);
     info_["kernel_ms"] = kms;
     info_["run_ms"] = rms;
@@ -389,16 +406,18 @@ extern "C" int fpsftwa_run_json_file(const char* path, const char* out_path)
         splitPath(path, folder, file);
         const std::string simName = root.get("sim_name", Value("defaultSimName")).asString();
         const std::string method = root.get("method", Value("HF")).asString();
-        if (method != "fTWA" && method != "HF") {
-            fpsamd::set_last_error("method '" + method + "' is not provided by this engine (fTWA and HF are)");
+        if (method != "fTWA" && method != "HF" && method != "GPSR") {
+            fpsamd::set_last_error("method '" + method + "' is not provided by this engine (fTWA, HF and GPSR are)");
             return FPSFTWA_EUNSUPPORTED;
         }
         FHSystem sys(root["system"], folder);
         for (const auto& w : sys.data().warnings) std::cout << "   -- note: " << w << std::endl;
-        // fps.cpp:88-116: the HF and fTWA branches differ only in the solver class and the file suffix
-        std::unique_ptr<FHfTWASolver> owner(method == "HF" ? new FHHFSolver(&sys) : new FHfTWASolver(&sys));
+        // fps.cpp:88-220: the HF, fTWA and GPSR branches differ only in the solver class and the file suffix
+        std::unique_ptr<FHfTWASolver> owner(method == "HF"     ? static_cast<FHfTWASolver*>(new FHHFSolver(&sys))
+                                            : method == "GPSR" ? static_cast<FHfTWASolver*>(new FHGPSRSolver(&sys))
+                                                               : new FHfTWASolver(&sys));
         FHfTWASolver& solver = *owner;
-        std::string work = simName + (method == "HF" ? "_HF_res" : "_fTWA_res");
+        std::string work = simName + "_" + method + "_res";
         std::string outFolder = folder;
         if (out_path) {
             std::string f2, w2;

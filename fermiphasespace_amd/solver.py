@@ -179,7 +179,16 @@ class FHfTWASolver:
         e = self.engine_handle
         e.set_params(self.dt, self.tMax, self.nSub, self.nTraj, seed=self.seed, engine=self.engine)
         e.set_results(self.flags)
-        e.run()
+        try:
+            e.run()
+        except _lib.FtwaError as err:
+            # the batched GPSR kernel covers small lattices (L <= 32, short-range hopping, no pulse); the plain
+            # FP64 engine runs the rest
+            if self.method != "gpsr" or err.code != _lib.EUNSUPPORTED or self.engine == "fp64_plain":
+                raise
+            self.engine = "fp64_plain"
+            e.set_params(self.dt, self.tMax, self.nSub, self.nTraj, seed=self.seed, engine=self.engine)
+            e.run()
         sim = {"t": e.result("time").tolist()}
         f = self.flags
         if f & _lib.RES_WHOLE:
@@ -233,3 +242,12 @@ class FHHFSolver(FHfTWASolver):
         """dt, t_max, stepsParTimePoints, int_order with the defaults of src/FHHFSolver.cpp:47-57."""
         self.setTimeParameters(param.get("t_max", 2.0), param.get("dt", 0.01), param.get("stepsParTimePoints", 1))
         self.setIntegratorOrder(param.get("int_order", 2))
+
+
+class FHGPSRSolver(FHfTWASolver):
+    """Gaussian phase-space representation (include/FHGPSRSolver.h, src/FHGPSRSolver.cpp): stochastic
+    trajectories of the normally ordered density matrix with the drift of derivative (:110-133) and the
+    diffusion of step_Diffusion (:211-232); same method parameters and defaults as the fTWA solver
+    (:48-58); results file `<sim>_GPSR_res`."""
+
+    method = "gpsr"

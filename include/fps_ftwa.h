@@ -59,6 +59,8 @@ extern "C" {
 #define FPSFTWA_METHOD_FTWA 0 /* fTWA: sampled initial states, Hartree mean field (FHfTWASolver)       */
 #define FPSFTWA_METHOD_HF 1   /* time-dependent Hartree-Fock: one deterministic state, Fock exchange  */
                               /* with the V interaction (FHHFSolver, src/FHHFSolver.cpp:575-608)       */
+#define FPSFTWA_METHOD_GPSR 2 /* Gaussian phase-space representation: stochastic trajectories of the   */
+                              /* normally ordered density matrix (FHGPSRSolver, src/FHGPSRSolver.cpp)  */
 
 /* ---- results to save (FHSolver::setResultsToSave, src/FHSolver.cpp:34-45) -------- */
 #define FPSFTWA_RES_DIAG 0x01     /* "diagDMatrix"   -> Niiu, Niid          */
@@ -150,12 +152,20 @@ int fpsftwa_set_initial_state(fpsftwa_handle* h, const double* n_up, const doubl
  * the result with FPSFTWA_OUT_BASIS.  Not available with the adjoint. */
 int fpsftwa_set_bases(fpsftwa_handle* h, int32_t n_bases, const double* bases);
 
-/* FPSFTWA_METHOD_FTWA (default) or FPSFTWA_METHOD_HF.  HF follows FHHFSolver (src/FHHFSolver.cpp:
- * 190-420): the state is the density matrix N = R diag(n) R^dagger itself (no sampling; every
- * trajectory is the same, so n_traj 1 is the natural choice), Delta_sigma = tM + diag(U N_sbar) +
+/* FPSFTWA_METHOD_FTWA (default), FPSFTWA_METHOD_HF or FPSFTWA_METHOD_GPSR.  HF follows FHHFSolver
+ * (src/FHHFSolver.cpp:190-420): the state is the density matrix N = R diag(n) R^dagger itself (no sampling;
+ * every trajectory is the same, so n_traj 1 is the natural choice), Delta_sigma = tM + diag(U N_sbar) +
  * diag(V N_sigma) - N_sigma o V + extPot (:575-608), the same RK4 "3/8" steps, and E_V includes the
  * exchange energy -sum V o N o N^T / 2 (:356-357).  With a general (V) interaction the Fock term
- * needs the plain FP64 engine; with null or on-site interactions every engine runs HF. */
+ * needs the plain FP64 engine; with null or on-site interactions every engine runs HF.
+ * GPSR follows FHGPSRSolver (src/FHGPSRSolver.cpp): every trajectory starts from rho = R diag(n) R^dagger
+ * (the normally ordered density matrix, no 1/2), drifts with Delta_sigma = tM + diag(U rho_sbar_ii) +
+ * extPot (no V; :110-133) by the RK4 "3/8" step and diffuses with c (rhoH W1 rho + i rho W2 rhoH), c =
+ * sqrt(i U(0,0)), W_k diagonal complex noise of variance h per step from the handle's seed (:211-232; only
+ * for an on-site interaction, :248); a trajectory that turns NaN or Inf stays NaN (:611-615).  The results
+ * are plain ensemble means: Niiu = <Re rho_ii>, N2iiu = <|rho_ii|^2>, NuNd = <Re rho_up_ii rho_dn_ii>,
+ * energies without V, Mkk = <Re diag(V^T rho V)> (:895-929).  The batched engines run GPSR for L <= 32, 4
+ * or 8 neighbours and no pulse; the plain FP64 engine runs any system.  No adjoint. */
 int fpsftwa_set_method(fpsftwa_handle* h, int32_t method);
 
 /* The off-diagonal element rho_sigma(i, j) whose ensemble mean FPSFTWA_OUT_N1HU / N1HD return.
@@ -243,9 +253,9 @@ typedef struct fpsftwa_adjoint_spec {
 int fpsftwa_run_adjoint(fpsftwa_handle* h, const fpsftwa_adjoint_spec* spec, double* objective, double* grad_u,
                         double* grad_t, double* grad_pulse, size_t n_pulse);
 
-/* fps.cpp:88-207 for "method":"fTWA" (and "HF", :88-101): read <path>, build the system, run, and
- * write <folder><sim_name>_fTWA_res.json (_HF_res.json) in the reference's results format
- * (src/FHSolver.cpp:171-307).  out_path overrides the output file name when not NULL. */
+/* fps.cpp:88-220 for "method":"fTWA" (and "HF", :88-101, "GPSR", :208-220): read <path>, build the system,
+ * run, and write <folder><sim_name>_fTWA_res.json (_HF_res.json, _GPSR_res.json) in the reference's results
+ * format (src/FHSolver.cpp:171-307).  out_path overrides the output file name when not NULL. */
 int fpsftwa_run_json_file(const char* path, const char* out_path);
 
 /* The system and initial occupations the JSON front end builds from a simulation file

@@ -93,3 +93,12 @@ public:
     explicit FHHFSolver(FHSystem* fhSystem);
     void setMethodParameters(const fpsjson::Value& p);  // src/FHHFSolver.cpp:47-57 (dt, t_max, stepsParTimePoints, int_order)
 };
+
+// FHGPSRSolver (include/FHGPSRSolver.h, src/FHGPSRSolver.cpp): stochastic trajectories of the normally ordered
+// density matrix (Gaussian phase-space representation).  Same interface and method-parameter defaults as
+// FHfTWASolver (src/FHGPSRSolver.cpp:48-58); the batched FP64 engine by default (L <= 32, short-range hopping,
+// no pulse), the plain FP64 engine otherwise.  Results file "<sim>_GPSR_res.json" (fps.cpp:213).
+class FHGPSRSolver : public FHfTWASolver {
+public:
+    explicit FHGPSRSolver(FHSystem* fhSystem);
+};
