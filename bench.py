@@ -188,8 +188,8 @@ def side_gpsr4x4(args, sync):
         nth = os.cpu_count() or 1
         s = O.System(tM=tM, coulomb_type=1, U=Um, gpsr=True, gpsr_seed=31)
         t0 = time.perf_counter()
-        O.run(s, dt, steps * dt, 1, nu, nu, n_traj=nth, seed=31, nthreads=nth)
-        per = (time.perf_counter() - t0) / nth
+        O.run(s, dt, steps * dt, 1, nu, nu, n_traj=nth, seed=31, nthreads=nth)  # one trajectory per thread
+        per = time.perf_counter() - t0
         nb = max(nth, int(args.cpu_seconds * nth / max(per, 1e-9)))
         t0 = time.perf_counter()
         O.run(s, dt, steps * dt, 1, nu, nu, n_traj=nb, seed=31, nthreads=nth)
