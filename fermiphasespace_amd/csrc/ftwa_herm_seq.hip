@@ -66,6 +66,45 @@ __device__ __forceinline__ T ldl(const unsigned char* base, int off)
     return *reinterpret_cast<const T*>(base + off);
 }
 
+// Element slots in class order.  Lane l of element row er holds the p-th element (p = 2 er + l / 32) of class
+// c = l % 32, where class c lists the strict-upper (i, j), i < j, with (i + j) mod 32 = c, ordered by i + j,
+// then i.  The rows are 129 complex long (258 dwords = 2 modulo 64 banks), so both Y_ij and Y_ji (and X_ij,
+// X_ji) of an element sit in bank pair (i + j) mod 32: a pass of 32 lanes with one element of every class
+// touches each bank pair once.  Row-major packing put several short rows of the triangle into one pass and
+// conflicted up to 8-way there (modelled: 22 % extra LDS cycles per stage at L = 64).  FP64 elements take
+// four banks and 16 lanes per pass, whose classes are distinct modulo 16 as well.
+__host__ __device__ inline bool class_slot(int L, int p, int c, int& i, int& j)
+{
+    for (int s = c; s <= 2 * L - 3; s += 32) {
+        if (s < 1) continue;
+        const int lo = s - L + 1 > 0 ? s - L + 1 : 0, hi = (s - 1) / 2;
+        const int n = hi - lo + 1;
+        if (n <= 0) continue;
+        if (p < n) {
+            i = lo + p;
+            j = s - i;
+            return true;
+        }
+        p -= n;
+    }
+    return false;
+}
+
+// Element rows of the class order: half the largest class, rounded up (32 at L = 64).
+__host__ __device__ inline int class_rows(int L)
+{
+    int most = 0;
+    for (int c = 0; c < 32; ++c) {
+        int n = 0;
+        for (int s = c; s <= 2 * L - 3; s += 32) {
+            const int lo = s - L + 1 > 0 ? s - L + 1 : 0, hi = (s - 1) / 2;
+            if (s >= 1 && hi >= lo) n += hi - lo + 1;
+        }
+        most = n > most ? n : most;
+    }
+    return (most + 1) / 2;
+}
+
 // Stage input of RK stage ST from the registers: y, y + h p/3, y + h (A - 3p)/6, y + h p.
 template <int ST, typename Real>
 __device__ __forceinline__ Real stage_input(Real y, Real A, Real p, Real h1, Real h3, Real h6)
@@ -114,8 +153,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
-    const int NS = L * (L - 1) / 2;        // strict upper elements per spin
-    const int n_off = (NS + 63) / 64;      // element rows of one spin's strict upper triangle
+    const int n_off = class_rows(L);       // element rows of one spin's strict upper triangle (class order)
     const int tid = threadIdx.x;
     const int j = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -147,9 +185,10 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + (on ? s.nbr_k[n * L + i] : 0) * kRowB;
     }
 
-    // ---- element slots of the strict upper triangle: LDS addresses of Y_ij (aij) and Y_ji (aji) in
-    // the shared rows, h offsets ah = i*8 | (j*8) << 16; padding lanes point at the spare row and at h
-    // entry 64.  The diagonal is held apart, site i by thread i (< L) of wave 0: real RK registers only.
+    // ---- element slots of the strict upper triangle (class order, class_slot): LDS addresses of Y_ij
+    // (aij) and Y_ji (aji) in the shared rows, h offsets ah = i*8 | (j*8) << 16; padding lanes point at the
+    // spare row and at h entry 64.  The diagonal is held apart, site i by thread i (< L) of wave 0: real
+    // RK registers only.
     int aij[S], aji[S], ah[S];
     C y0[S], A0[S], p0[S], y1[S], A1[S], p1[S];
     const C* gs = reinterpret_cast<const C*>(state) + b * 2 * LL;
@@ -160,16 +199,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         aji[q] = spare;
         ah[q] = 64 * kRB | (64 * kRB) << 16;
         y0[q] = A0[q] = p0[q] = y1[q] = A1[q] = p1[q] = C{0, 0};
-        if (er < n_off) {
-            const int u = er * 64 + j;
-            if (u < NS) {
-                // row i of the strict upper triangle: the largest i with i*(2L - i - 1)/2 <= u
-                const float c = static_cast<float>(2 * L - 1);
-                int i = static_cast<int>(0.5f * (c - sqrtf(fmaxf(c * c - 8.0f * static_cast<float>(u), 0.0f))));
-                if (i < 0) i = 0;
-                while (i > 0 && i * (2 * L - i - 1) / 2 > u) --i;
-                while ((i + 1) * (2 * L - i - 2) / 2 <= u) ++i;
-                const int jj = i + 1 + (u - i * (2 * L - i - 1) / 2);
+        int i = 0, jj = 0;
+        if (er < n_off && class_slot(L, 2 * er + (j >> 5), j & 31, i, jj)) {
+            {
                 aij[q] = kRows + i * kRowB + jj * kCB;
                 aji[q] = kRows + jj * kRowB + i * kCB;
                 ah[q] = i * kRB | (jj * kRB) << 16;
@@ -456,8 +488,7 @@ hipError_t launch_herm_seq(const DevSystem<Real>& s, const RunShape& r, int t0, 
     // 16 waves, whose 2 slots exceed 128 registers).  FP32: two 8-wave workgroups per CU.  16 waves on
     // request (nt_hint 1024).
     const int W = nt_hint == 1024 ? 16 : 8;
-    const int NS = s.L * (s.L - 1) / 2;
-    const int rows = (NS + 63) / 64;  // element rows of the strict upper triangle
+    const int rows = class_rows(s.L);  // element rows of the strict upper triangle (at most 32 for L <= 64)
     const int S = std::max(1, (rows + W - 1) / W);
     if (W == 16) return s.z == 4 ? run_seq_s<Real, 16, 4>(S, s, r, t0, t1, state, rec, st)
                                  : run_seq_s<Real, 16, 8>(S, s, r, t0, t1, state, rec, st);
