@@ -175,6 +175,16 @@ __global__ void __launch_bounds__(2 * RP * N)
     // noise slots of padding sites stay zero (padding elements multiply them by 0; garbage could be NaN)
     for (int q = tid; q < 2 * N; q += NT) xis[q] = C{Real(0), Real(0)};
     __syncthreads();
+    // the thread's column neighbour list, in registers where the budget allows (it is the same for the whole
+    // launch); FP64 at N = 32 reads it from the LDS tables at each use instead
+    constexpr bool kColRegs = sizeof(Real) == 4 || N <= 16;
+    int kc[Z];
+    Real tc[Z];
+#pragma unroll
+    for (int n = 0; n < Z; ++n) {
+        kc[n] = j < L ? s.nbr_k[n * L + j] : j;
+        tc[n] = j < L ? s.nbr_t[n * L + j] : Real(0);
+    }
     // state -> registers and the stage-input buffer 0
     C y[E], A[E], p[E], d[E];
     Real* gst = state + b * 2 * LL * 2 + static_cast<int64_t>(sg) * LL * 2;
@@ -262,8 +272,8 @@ __global__ void __launch_bounds__(2 * RP * N)
             C x{Real(0), Real(0)};
 #pragma unroll
             for (int n = 0; n < Z; ++n) {
-                const Real t = nbt[n * N + jc];
-                const C a = Ym[i * RS + nbk[n * N + jc]];
+                const Real t = kColRegs ? tc[n] : nbt[n * N + jc];
+                const C a = Ym[i * RS + (kColRegs ? kc[n] : nbk[n * N + jc])];
                 x.x += a.x * t;
                 x.y += a.y * t;
             }

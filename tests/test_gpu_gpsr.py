@@ -204,3 +204,43 @@ def test_gpsr_json(tmp_path, oracle_mod):
     out = json.load(open(tmp_path / "big_GPSR_res.json"))
     assert out["mi355x"]["engine"] == "fp64_plain"
     assert np.isfinite(np.asarray(out["simulation"]["Niiu"])).all()
+
+
+def test_gpsr_substeps_and_pulse_on_the_plain_engine(oracle_mod):
+    """Sub-steps (the diffusion noise of every RK step, counter t * nsub + sub, variance dt / nsub) on every
+    engine; a Gaussian pulse (Peierls phase on tM) on the plain FP64 engine, which the batched GPSR kernel
+    refuses."""
+    O = oracle_mod
+    L, B, seed, dt, t_max = 6, 16, 4, 0.04, 0.32
+    tM = chain(L)
+    nu = np.array([1.0, 0, 1, 0, 1, 0])
+    U = 1.2 * np.eye(L)
+    ref = O.run(O.System(tM=tM, coulomb_type=1, U=U, gpsr=True, gpsr_seed=seed), dt, t_max, 3, nu, 1 - nu, n_traj=B,
+                seed=seed, keep_states=True)
+    eng = _lib.Engine(tM, coulomb_type=1, U=U)
+    eng.set_method("gpsr")
+    eng.set_initial_state(nu, 1 - nu)
+    for engine, tol in (("fp64_plain", RTOL64), ("fp64", RTOL64), ("fp32", 5e-4)):
+        eng.set_params(dt, t_max, 3, B, seed=seed, engine=engine)
+        eng.run()
+        assert rel(eng.states(), ref.final_states) < tol, engine
+        assert rel(eng.result("sum_ud"), ref.sum_ud) < tol, engine
+    eng.close()
+    disp = np.zeros((L, L, 3))
+    disp[..., 0] = np.arange(L)[:, None] - np.arange(L)[None, :]
+    pulse = {"pulse_type": 1, "A0": 0.6, "t0": 0.15, "td": 0.1, "w0": 8.0, "phi": 0.0, "pol": (1.0, 0.0, 0.0),
+             "disp": disp}
+    ref = O.run(O.System(tM=tM, coulomb_type=1, U=U, gpsr=True, gpsr_seed=seed, **pulse), dt, t_max, 1, nu, 1 - nu,
+                n_traj=B, seed=seed, keep_states=True)
+    eng = _lib.Engine(tM, coulomb_type=1, U=U, **pulse)
+    eng.set_method("gpsr")
+    eng.set_initial_state(nu, 1 - nu)
+    eng.set_params(dt, t_max, 1, B, seed=seed, engine="fp64_plain")
+    eng.run()
+    assert rel(eng.states(), ref.final_states) < RTOL64
+    assert rel(eng.result("sum_e"), ref.sum_e) < RTOL64
+    eng.set_params(dt, t_max, 1, B, seed=seed, engine="fp64")
+    with pytest.raises(_lib.FtwaError) as e:
+        eng.run()
+    assert e.value.code == _lib.EUNSUPPORTED
+    eng.close()
