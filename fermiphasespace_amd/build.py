@@ -49,6 +49,7 @@ def build(verbose: bool = False) -> str:
         ("ftwa_stencil.hip", "ftwa_stencil_f64.o", ["-DFTWA_REAL=2"]),
         ("ftwa_herm.hip", "ftwa_herm.o", []),
         ("ftwa_herm_seq.hip", "ftwa_herm_seq.o", []),
+        ("ftwa_herm_wide.hip", "ftwa_herm_wide.o", []),
         ("ftwa_gpsr.hip", "ftwa_gpsr.o", []),
         ("ftwa_adjoint.hip", "ftwa_adjoint.o", []),
         ("ftwa_dense.hip", "ftwa_dense.o", []),

@@ -384,20 +384,23 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     int rc = upload_system<Real>(h, ds, h->T);
     if (rc) return rc;
     int nt = h->nt_hint;
-    bool herm = false, pipe = false, seq = false, dense = h->kernel == FPSFTWA_KERNEL_DENSE;
+    bool herm = false, pipe = false, seq = false, wide = false, dense = h->kernel == FPSFTWA_KERNEL_DENSE;
     if (dense) {
         // forced
     } else if (h->kernel == FPSFTWA_KERNEL_SEQUENTIAL) {
         seq = herm_seq_fits<Real>(ds) && h->n_bases == 0;
-        if (!seq)
-            return fail(FPSFTWA_EUNSUPPORTED, "the spin-sequential kernel needs L <= 64, 4 or 8 neighbours, no pulse, "
-                                              "null or on-site Coulomb and no basis occupations");
+        if constexpr (std::is_same<Real, float>::value) wide = !seq && herm_wide_fits(ds) && h->n_bases == 0;
+        if (!seq && !wide)
+            return fail(FPSFTWA_EUNSUPPORTED, "the spin-sequential kernel needs L <= 64 (FP32: L <= 107), 4 or 8 "
+                                              "neighbours, no pulse, null or on-site Coulomb and no basis occupations");
     } else if constexpr (std::is_same<Real, float>::value) {
         // automatic choice: the spin-sequential kernel where it applies (two workgroups per CU:
-        // 2.84e7 trajectory-steps/s on the 8x8 workload, against 2.49e7 for the both-spin kernel),
-        // else the both-spin upper-triangle kernel (pulses, V interactions, basis occupations)
+        // 2.87e7 trajectory-steps/s on the 8x8 workload, against 2.51e7 for the both-spin kernel), its
+        // column-blocked form for 64 < L <= 107, else the both-spin upper-triangle kernel (pulses, V
+        // interactions, basis occupations)
         seq = h->kernel == FPSFTWA_KERNEL_AUTO && herm_seq_fits<float>(ds) && h->n_bases == 0;
-        herm = !seq && h->kernel != FPSFTWA_KERNEL_FULL && herm_fits(h->L, ds.z);
+        wide = !seq && h->kernel == FPSFTWA_KERNEL_AUTO && herm_wide_fits(ds) && h->n_bases == 0;
+        herm = !seq && !wide && h->kernel != FPSFTWA_KERNEL_FULL && herm_fits(h->L, ds.z);
         if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !herm)
             return fail(FPSFTWA_EUNSUPPORTED, "the upper-triangle kernel does not fit L=" + std::to_string(h->L));
         // the pipelined variant is selected only on request: on the 8x8 workload it measures 9 % slower
@@ -416,7 +419,7 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
             return fail(FPSFTWA_EUNSUPPORTED, "the FP64 upper-triangle kernel needs L <= 64, 4 or 8 neighbours, "
                                               "no pulse, null or on-site Coulomb and no basis occupations");
     }
-    if (!dense && !herm && !seq && !stencil_fits<Real>(h->L, ds.z, &nt)) {
+    if (!dense && !herm && !seq && !wide && !stencil_fits<Real>(h->L, ds.z, &nt)) {
         if (h->kernel == FPSFTWA_KERNEL_FULL)
             return fail(FPSFTWA_EUNSUPPORTED, "the full-matrix stencil kernel does not support L=" +
                                                   std::to_string(h->L) + " with " + std::to_string(ds.z) +
@@ -468,7 +471,8 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
             HIP_TRY(launch_herm_seq<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), h->nt_hint,
                                           h->stream));
         } else if constexpr (std::is_same<Real, float>::value) {
-            if (pipe) HIP_TRY(launch_herm_pipe(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->stream));
+            if (wide) HIP_TRY(launch_herm_wide(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->stream));
+            else if (pipe) HIP_TRY(launch_herm_pipe(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->stream));
             else if (herm) HIP_TRY(launch_herm(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->nt_hint, h->stream));
             else HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
         } else if (herm) {

@@ -102,5 +102,74 @@ __device__ __forceinline__ void record_density(Real* dst, int L, int lane, int w
     }
 }
 
+// Element slots of the upper-triangle integrators in class order.  Lane l of element row er holds the p-th
+// element (p = 2 er + l / 32) of class c = l % 32, where class c lists the strict-upper (i, j), i < j, with
+// (i + j) mod 32 = c, ordered by i + j, then i.  With LDS rows 129 complex long (258 dwords = 2 modulo 64
+// banks), both Y_ij and Y_ji (and X_ij, X_ji) of an element sit in bank pair (i + j) mod 32: a pass of 32
+// lanes with one element of every class touches each bank pair once.  Row-major packing put several short
+// rows of the triangle into one pass and conflicted up to 8-way there (modelled: 22 % extra LDS cycles per
+// stage at L = 64).  FP64 elements take four banks and 16 lanes per pass, whose classes are distinct modulo
+// 16 as well.
+__host__ __device__ inline bool class_slot(int L, int p, int c, int& i, int& j)
+{
+    for (int s = c; s <= 2 * L - 3; s += 32) {
+        if (s < 1) continue;
+        const int lo = s - L + 1 > 0 ? s - L + 1 : 0, hi = (s - 1) / 2;
+        const int n = hi - lo + 1;
+        if (n <= 0) continue;
+        if (p < n) {
+            i = lo + p;
+            j = s - i;
+            return true;
+        }
+        p -= n;
+    }
+    return false;
+}
+
+// Element rows of the class order: half the largest class, rounded up (32 at L = 64).
+__host__ __device__ inline int class_rows(int L)
+{
+    int most = 0;
+    for (int c = 0; c < 32; ++c) {
+        int n = 0;
+        for (int s = c; s <= 2 * L - 3; s += 32) {
+            const int lo = s - L + 1 > 0 ? s - L + 1 : 0, hi = (s - 1) / 2;
+            if (s >= 1 && hi >= lo) n += hi - lo + 1;
+        }
+        most = n > most ? n : most;
+    }
+    return (most + 1) / 2;
+}
+
+// Stage input of RK stage ST from the registers: y, y + h p/3, y + h (A - 3p)/6, y + h p.
+template <int ST, typename Real>
+__device__ __forceinline__ Real stage_input(Real y, Real A, Real p, Real h1, Real h3, Real h6)
+{
+    if constexpr (ST == 0) return y;
+    else if constexpr (ST == 1) return y + h3 * p;
+    else if constexpr (ST == 2) return y + h6 * (A - Real(3) * p);
+    else return y + h1 * p;
+}
+
+// RK registers after stage ST with derivative k (the 3/8 rule in the (y, A, p) form).
+template <int ST, typename Real>
+__device__ __forceinline__ void rk_update(Real& y, Real& A, Real& p, Real k, Real h8)
+{
+    if constexpr (ST == 0) {
+        A = k;
+        p = k;
+    } else if constexpr (ST == 1) {
+        A += Real(3) * k;
+        p -= k;
+    } else if constexpr (ST == 2) {
+        A += Real(3) * k;
+        p += k;
+    } else {
+        A += k;
+        y += h8 * A;
+    }
+}
+
 }  // namespace dev
 }  // namespace fpsamd

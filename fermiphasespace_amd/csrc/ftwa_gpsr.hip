@@ -33,7 +33,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
-#include <type_traits>
 
 #include "ftwa_device.hpp"
 #include "ftwa_kernels.hpp"
@@ -46,9 +45,6 @@ using dev::V2;
 using dev::wave_sum;
 
 constexpr int kLdsLimit = 160 * 1024;
-
-template <int N>
-using Ic = std::integral_constant<int, N>;
 
 template <typename C>
 __device__ __forceinline__ C cmul(C a, C b)

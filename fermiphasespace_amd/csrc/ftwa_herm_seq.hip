@@ -66,74 +66,6 @@ __device__ __forceinline__ T ldl(const unsigned char* base, int off)
     return *reinterpret_cast<const T*>(base + off);
 }
 
-// Element slots in class order.  Lane l of element row er holds the p-th element (p = 2 er + l / 32) of class
-// c = l % 32, where class c lists the strict-upper (i, j), i < j, with (i + j) mod 32 = c, ordered by i + j,
-// then i.  The rows are 129 complex long (258 dwords = 2 modulo 64 banks), so both Y_ij and Y_ji (and X_ij,
-// X_ji) of an element sit in bank pair (i + j) mod 32: a pass of 32 lanes with one element of every class
-// touches each bank pair once.  Row-major packing put several short rows of the triangle into one pass and
-// conflicted up to 8-way there (modelled: 22 % extra LDS cycles per stage at L = 64).  FP64 elements take
-// four banks and 16 lanes per pass, whose classes are distinct modulo 16 as well.
-__host__ __device__ inline bool class_slot(int L, int p, int c, int& i, int& j)
-{
-    for (int s = c; s <= 2 * L - 3; s += 32) {
-        if (s < 1) continue;
-        const int lo = s - L + 1 > 0 ? s - L + 1 : 0, hi = (s - 1) / 2;
-        const int n = hi - lo + 1;
-        if (n <= 0) continue;
-        if (p < n) {
-            i = lo + p;
-            j = s - i;
-            return true;
-        }
-        p -= n;
-    }
-    return false;
-}
-
-// Element rows of the class order: half the largest class, rounded up (32 at L = 64).
-__host__ __device__ inline int class_rows(int L)
-{
-    int most = 0;
-    for (int c = 0; c < 32; ++c) {
-        int n = 0;
-        for (int s = c; s <= 2 * L - 3; s += 32) {
-            const int lo = s - L + 1 > 0 ? s - L + 1 : 0, hi = (s - 1) / 2;
-            if (s >= 1 && hi >= lo) n += hi - lo + 1;
-        }
-        most = n > most ? n : most;
-    }
-    return (most + 1) / 2;
-}
-
-// Stage input of RK stage ST from the registers: y, y + h p/3, y + h (A - 3p)/6, y + h p.
-template <int ST, typename Real>
-__device__ __forceinline__ Real stage_input(Real y, Real A, Real p, Real h1, Real h3, Real h6)
-{
-    if constexpr (ST == 0) return y;
-    else if constexpr (ST == 1) return y + h3 * p;
-    else if constexpr (ST == 2) return y + h6 * (A - Real(3) * p);
-    else return y + h1 * p;
-}
-
-// RK registers after stage ST with derivative k (the 3/8 rule in the (y, A, p) form).
-template <int ST, typename Real>
-__device__ __forceinline__ void rk_update(Real& y, Real& A, Real& p, Real k, Real h8)
-{
-    if constexpr (ST == 0) {
-        A = k;
-        p = k;
-    } else if constexpr (ST == 1) {
-        A += Real(3) * k;
-        p -= k;
-    } else if constexpr (ST == 2) {
-        A += Real(3) * k;
-        p += k;
-    } else {
-        A += k;
-        y += h8 * A;
-    }
-}
-
 // MINB: workgroups per CU the register budget is sized for (2 in FP32, 1 in FP64): at least
 // MINB * W / 4 waves per SIMD.
 template <typename Real, int W, int S, int Z, int MINB>
@@ -153,7 +85,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
-    const int n_off = class_rows(L);       // element rows of one spin's strict upper triangle (class order)
+    const int n_off = dev::class_rows(L);       // element rows of one spin's strict upper triangle (class order)
     const int tid = threadIdx.x;
     const int j = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -200,7 +132,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         ah[q] = 64 * kRB | (64 * kRB) << 16;
         y0[q] = A0[q] = p0[q] = y1[q] = A1[q] = p1[q] = C{0, 0};
         int i = 0, jj = 0;
-        if (er < n_off && class_slot(L, 2 * er + (j >> 5), j & 31, i, jj)) {
+        if (er < n_off && dev::class_slot(L, 2 * er + (j >> 5), j & 31, i, jj)) {
             {
                 aij[q] = kRows + i * kRowB + jj * kCB;
                 aji[q] = kRows + jj * kRowB + i * kCB;
@@ -239,18 +171,18 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
             const C& yq = SG ? y1[q] : y0[q];
             const C& Aq = SG ? A1[q] : A0[q];
             const C& pq = SG ? p1[q] : p0[q];
-            const Real yx = stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
-            const Real yy = stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
+            const Real yx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
+            const Real yy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
             *reinterpret_cast<C*>(smem + aij[q]) = C{yx, yy};
             *reinterpret_cast<C*>(smem + aji[q]) = C{yx, -yy};
         }
         if (dslot) {
-            const Real yx = SG ? stage_input<ST>(dy1, dA1, dp1, h1, h3, h6)
-                               : stage_input<ST>(dy0, dA0, dp0, h1, h3, h6);
+            const Real yx = SG ? dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6)
+                               : dev::stage_input<ST>(dy0, dA0, dp0, h1, h3, h6);
             *reinterpret_cast<C*>(smem + da) = C{yx, Real(0)};
             if constexpr (SG == 0) {
                 // h_up,i = pdiag_i + U_ii n_dn,i and h_dn,i = pdiag_i + U_ii n_up,i, n = Re y + 1/2
-                const Real y1x = stage_input<ST>(dy1, dA1, dp1, h1, h3, h6);
+                const Real y1x = dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6);
                 int o = tid * kRB;
                 asm volatile("" : "+v"(o));
                 const Real u = ldl<Real>(smem, kSite + 128 * kRB + o), pd = ldl<Real>(smem, kSite + 192 * kRB + o);
@@ -301,18 +233,18 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
             C& yq = SG ? y1[q] : y0[q];
             C& Aq = SG ? A1[q] : A0[q];
             C& pq = SG ? p1[q] : p0[q];
-            const Real ycx = stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
-            const Real ycy = stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
+            const Real ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
+            const Real ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
             // c = conj(X_ji) - X_ij + (h_j - h_i) Y_ij;  k = -i c
             const Real cx = xji.x - xij.x + dh * ycx;
             const Real cy = -xji.y - xij.y + dh * ycy;
-            rk_update<ST>(yq.x, Aq.x, pq.x, cy, h8);
-            rk_update<ST>(yq.y, Aq.y, pq.y, -cx, h8);
+            dev::rk_update<ST>(yq.x, Aq.x, pq.x, cy, h8);
+            dev::rk_update<ST>(yq.y, Aq.y, pq.y, -cx, h8);
         }
         if (dslot) {  // k_ii = -2 Im X_ii
             const Real k = Real(-2) * ldl<C>(smem, da + kXo).y;
-            if constexpr (SG == 0) rk_update<ST>(dy0, dA0, dp0, k, h8);
-            else rk_update<ST>(dy1, dA1, dp1, k, h8);
+            if constexpr (SG == 0) dev::rk_update<ST>(dy0, dA0, dp0, k, h8);
+            else dev::rk_update<ST>(dy1, dA1, dp1, k, h8);
         }
     };
 
@@ -488,7 +420,7 @@ hipError_t launch_herm_seq(const DevSystem<Real>& s, const RunShape& r, int t0, 
     // 16 waves, whose 2 slots exceed 128 registers).  FP32: two 8-wave workgroups per CU.  16 waves on
     // request (nt_hint 1024).
     const int W = nt_hint == 1024 ? 16 : 8;
-    const int rows = class_rows(s.L);  // element rows of the strict upper triangle (at most 32 for L <= 64)
+    const int rows = dev::class_rows(s.L);  // element rows of the strict upper triangle (at most 32 for L <= 64)
     const int S = std::max(1, (rows + W - 1) / W);
     if (W == 16) return s.z == 4 ? run_seq_s<Real, 16, 4>(S, s, r, t0, t1, state, rec, st)
                                  : run_seq_s<Real, 16, 8>(S, s, r, t0, t1, state, rec, st);

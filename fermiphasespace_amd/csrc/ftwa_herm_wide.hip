@@ -1,0 +1,422 @@
+// ftwa_herm_wide.hip -- batched FP32 fTWA integrator on the upper triangle for 64 < L <= 107: one spin's
+// rows in LDS at a time, X = T Y kept for one block of 64 columns at a time.
+//
+// Same physics, records and RK4 "3/8" register form (y, A, p) as the spin-sequential kernel of
+// ftwa_herm_seq.hip (src/FHfTWASolver.cpp:366-426 records, :568-600 derivative, :629-661 RK4 3/8).  Past
+// L = 64 the one-spin [Y | X] rows no longer fit LDS (160.8 KB at L = 100), so X is formed for a block of
+// 64 columns at a time and each slot accumulates its part of conj(X_ji) - X_ij:
+//   M    the spin's stage input Y = f(y, A, p) from the RK registers (upper triangle, mirror, diagonal;
+//        in the spin-up pass also the Hartree fields of both spins and the stage-input diagonals)
+//   A_b  X[:, 64b .. 64b+63] = T Y[:, 64b .. 64b+63], lane = row (two rows per lane, j and j + 64)
+//   B_b  c_ij += conj(X_ji) - X_ij for the parts in block b; after the last block
+//        k_ij = -i [c_ij + (h_j - h_i) Y_ij], k_ii = -2 Im X_ii (in the block holding column i)
+// with barriers after M, A_0, B_0 and A_1.  LDS: rows [Y (L complex) | X block (64 complex) | pad] with an
+// odd row length, so the lanes of phase A (rows k, one column) fall on distinct bank pairs.  One 8-wave
+// workgroup per CU (up to 256 VGPRs, S <= 11 slots per thread).  Element slots in class order
+// (dev::class_slot).  Null or on-site Coulomb interactions, no pulse, no density records.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "ftwa_device.hpp"
+#include "ftwa_kernels.hpp"
+
+namespace fpsamd {
+
+namespace {
+
+using dev::wave_sum;
+using C = float2;
+
+constexpr int kW = 8;            // wavefronts per workgroup
+constexpr int kNT = 64 * kW;
+constexpr int kLP = 128;         // site capacity of the tables
+constexpr int kCB = 8;           // bytes per complex
+constexpr int kRB = 4;           // bytes per real
+// LDS layout: h [2 stage parities][2 spins][144], stage-input diagonals [2][144], energy partials
+// [8 waves][4], site tables tM_ii / E_pe potential / U_ii / pdiag [4][128], neighbour hoppings [8][128],
+// neighbour row offsets [8][128] (ints), 16 zero bytes, then L rows and a spare row.
+constexpr int kHSpin = 144 * kRB;  // entries 128..143 take padding slots (always zero)
+constexpr int kHPar = 2 * kHSpin;
+constexpr int kHB = 0;
+constexpr int kDg = 2 * kHPar;
+constexpr int kEW = kDg + kHPar;
+constexpr int kSite = kEW + kW * 4 * kRB;
+constexpr int kNbrT = kSite + 4 * kLP * kRB;
+constexpr int kNbrK = kNbrT + 8 * kLP * kRB;
+constexpr int kZero = kNbrK + 8 * kLP * 4;
+constexpr int kRows = kZero + 16;
+static_assert(kRows % 16 == 0, "rows are 16-byte aligned");
+constexpr size_t kLdsLimit = 160 * 1024;
+
+// complex elements per LDS row: L of Y, 64 of the X block, padding to an odd length
+__host__ __device__ inline int wide_row_len(int L) { return (L + 65) | 1; }
+
+template <typename T>
+__device__ __forceinline__ T ldl(const unsigned char* base, int off)
+{
+    return *reinterpret_cast<const T*>(base + off);
+}
+
+template <int N>
+using Ic = std::integral_constant<int, N>;
+
+template <int S, int Z>
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
+    herm_wide_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state,
+                     float* __restrict__ rec)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int L = s.L;
+    const int LL = L * L;
+    const int R2 = 2 * L;
+    const int RB = wide_row_len(L) * kCB;  // row stride, bytes
+    const int n_off = dev::class_rows(L);
+    const int tid = threadIdx.x;
+    const int j = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t b = blockIdx.x;
+    const int spare = kRows + L * RB;
+    const float half = 0.5f;
+    float* ew = reinterpret_cast<float*>(smem + kEW);
+    auto yat = [&](int row, int col) -> C& { return *reinterpret_cast<C*>(smem + kRows + row * RB + col * kCB); };
+
+    // ---- clear everything (h and X padding stay zero), then the tables ----
+    {
+        const int n16 = (kRows + (L + 1) * RB) / 16;
+        for (int e = tid; e < n16; e += kNT) reinterpret_cast<float4*>(smem)[e] = float4{0, 0, 0, 0};
+    }
+    __syncthreads();
+    for (int i = tid; i < L; i += kNT) {
+        *reinterpret_cast<float*>(smem + kSite + i * kRB) = s.tdiag[i];
+        *reinterpret_cast<float*>(smem + kSite + (kLP + i) * kRB) = s.epot[i];
+        *reinterpret_cast<float*>(smem + kSite + (2 * kLP + i) * kRB) = s.coulomb_type == 1 ? s.u_diag[i] : 0.f;
+        *reinterpret_cast<float*>(smem + kSite + (3 * kLP + i) * kRB) = s.pdiag[i];
+    }
+    // neighbour tables for row i (rows past L: row 0 with t = 0)
+    for (int e = tid; e < Z * kLP; e += kNT) {
+        const int n = e / kLP, i = e - n * kLP;
+        const bool on = i < L;
+        *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = on ? s.nbr_t[n * L + i] : 0.f;
+        *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + (on ? s.nbr_k[n * L + i] : 0) * RB;
+    }
+
+    // ---- element slots (class order): ah = i | j << 16 of the slot's element; padding lanes hold
+    // i = j = L (the spare row; h entry L is zero).  The LDS addresses of Y_ij and Y_ji are formed from ah at
+    // each use (registers: the RK state of both spins is 12 per slot).  The diagonal of site i is held by
+    // thread i (< L): real RK registers only.
+    int ah[S];
+    C y0[S], A0[S], p0[S], y1[S], A1[S], p1[S];
+    const C* gs = reinterpret_cast<const C*>(state) + b * 2 * LL;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const int er = q * kW + w;
+        ah[q] = L | L << 16;
+        y0[q] = A0[q] = p0[q] = y1[q] = A1[q] = p1[q] = C{0, 0};
+        int i = 0, jj = 0;
+        if (er < n_off && dev::class_slot(L, 2 * er + (j >> 5), j & 31, i, jj)) {
+            ah[q] = i | jj << 16;
+            y0[q] = gs[i * L + jj];
+            y1[q] = gs[LL + i * L + jj];
+        }
+    }
+    // the slot's (i, j) and its LDS offsets, decoded where used (opaque: not kept across the time loop)
+    auto slot = [&](int q, int& i, int& jj) {
+        int a = ah[q];
+        asm volatile("" : "+v"(a));
+        i = a & 0xffff;
+        jj = a >> 16;
+    };
+    const bool dslot = tid < L;
+    const int da = dslot ? kRows + tid * RB + tid * kCB : spare;
+    const int dblk = tid >> 6;  // the X block holding column tid (wave-uniform: waves 0 and 1)
+    float dy0 = dslot ? gs[tid * L + tid].x : 0.f, dy1 = dslot ? gs[LL + tid * L + tid].x : 0.f;
+    float dA0 = 0, dp0 = 0, dA1 = 0, dp1 = 0;
+
+    const float h1 = static_cast<float>(r.h), h3 = static_cast<float>(r.h / 3.0), h6 = static_cast<float>(r.h / 6.0),
+                h8 = static_cast<float>(r.h / 8.0);
+
+    // neighbour n of row i from the LDS tables (read at each use, not held across the time loop)
+    auto nbr = [&](int n, int i, int& kb, float& tv) {
+        int o = n * kLP + i;
+        asm volatile("" : "+v"(o));
+        kb = *reinterpret_cast<const int*>(smem + kNbrK + o * 4);
+        tv = *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
+    };
+
+    // ---- M: the spin's stage input into the rows; in the spin-up pass, h of both spins ----
+    auto materialize = [&](auto stc, auto sgc) {
+        constexpr int ST = decltype(stc)::value;
+        constexpr int SG = decltype(sgc)::value;
+        constexpr int HP = kHB + (ST & 1) * kHPar;
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            if (q * kW + w >= n_off) continue;
+            const C& yq = SG ? y1[q] : y0[q];
+            const C& Aq = SG ? A1[q] : A0[q];
+            const C& pq = SG ? p1[q] : p0[q];
+            const float yx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
+            const float yy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
+            int i, jj;
+            slot(q, i, jj);
+            *reinterpret_cast<C*>(smem + kRows + i * RB + jj * kCB) = C{yx, yy};
+            *reinterpret_cast<C*>(smem + kRows + jj * RB + i * kCB) = C{yx, -yy};
+        }
+        if (dslot) {
+            const float yx = SG ? dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6)
+                                : dev::stage_input<ST>(dy0, dA0, dp0, h1, h3, h6);
+            *reinterpret_cast<C*>(smem + da) = C{yx, 0.f};
+            if constexpr (SG == 0) {
+                // h_up,i = pdiag_i + U_ii n_dn,i and h_dn,i = pdiag_i + U_ii n_up,i, n = Re y + 1/2
+                const float y1x = dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6);
+                int o = tid * kRB;
+                asm volatile("" : "+v"(o));
+                const float u = ldl<float>(smem, kSite + 2 * kLP * kRB + o), pd = ldl<float>(smem, kSite + 3 * kLP * kRB + o);
+                *reinterpret_cast<float*>(smem + HP + o) = u * (y1x + half) + pd;
+                *reinterpret_cast<float*>(smem + HP + kHSpin + o) = u * (yx + half) + pd;
+                *reinterpret_cast<float*>(smem + kDg + o) = yx;
+                *reinterpret_cast<float*>(smem + kDg + kHSpin + o) = y1x;
+            }
+        }
+    };
+
+    // ---- A_b: X block b = T Y[:, 64b ..] for the spin in the rows; lane j takes rows j and j + 64,
+    // wave w the block's columns w + 8m ----
+    auto phase_a = [&](auto bc) {
+        constexpr int BLK = decltype(bc)::value;
+#pragma unroll
+        for (int rp = 0; rp < 2; ++rp) {
+            const int i = j + 64 * rp;
+            const bool on = i < L;
+            int kb[Z];
+            float tv[Z];
+#pragma unroll
+            for (int n = 0; n < Z; ++n) nbr(n, on ? i : 0, kb[n], tv[n]);
+            if (!on) {
+#pragma unroll
+                for (int n = 0; n < Z; ++n) tv[n] = 0.f;
+            }
+            const int xb = (on ? kRows + i * RB : spare) + (L + w) * kCB;
+            const int yc = (64 * BLK + w) * kCB;
+#pragma unroll
+            for (int m = 0; m < 64 / kW; ++m) {
+                if (64 * BLK + w + kW * m >= L) break;  // wave-uniform
+                const int co = m * kW * kCB;  // immediate column offset
+                float ax = 0, ay = 0;
+#pragma unroll
+                for (int n = 0; n < Z; ++n) {
+                    const C v = ldl<C>(smem, kb[n] + yc + co);
+                    ax += tv[n] * v.x;
+                    ay += tv[n] * v.y;
+                }
+                *reinterpret_cast<C*>(smem + xb + co) = C{ax, ay};
+            }
+        }
+    };
+
+    // ---- B_b: the block's part of conj(X_ji) - X_ij; after the last block the derivative and the RK
+    // registers.  Parts outside the block read the zero bytes.  Between the blocks the partial sum of an
+    // element with i < 64 waits in Y_ji: after A_0 no phase reads the first 64 columns of Y until the next
+    // M, and an element with i >= 64 has nothing in block 0. ----
+    auto phase_b = [&](auto bc, auto stc, auto sgc) {
+        constexpr int BLK = decltype(bc)::value;
+        constexpr int ST = decltype(stc)::value;
+        constexpr int SG = decltype(sgc)::value;
+        constexpr int HS = kHB + (ST & 1) * kHPar + SG * kHSpin;
+        const int xo = (L - 64 * BLK) * kCB;  // X(i, c) - Y(i, c) for a column c of block BLK
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            if (q * kW + w >= n_off) continue;
+            asm volatile("" ::: "memory");
+            int i, jj;
+            slot(q, i, jj);
+            const int aij = kRows + i * RB + jj * kCB, aji = kRows + jj * RB + i * kCB;
+            const int ci = i - 64 * BLK, cj = jj - 64 * BLK;
+            const C xij = ldl<C>(smem, (cj >= 0 && cj < 64) ? aij + xo : kZero);
+            const C xji = ldl<C>(smem, (ci >= 0 && ci < 64) ? aji + xo : kZero);
+            if constexpr (BLK == 0) {
+                if (i < 64) *reinterpret_cast<C*>(smem + aji) = C{xji.x - xij.x, -xji.y - xij.y};
+            } else {
+                const C c0 = i < 64 ? ldl<C>(smem, aji) : C{0.f, 0.f};
+                C& yq = SG ? y1[q] : y0[q];
+                C& Aq = SG ? A1[q] : A0[q];
+                C& pq = SG ? p1[q] : p0[q];
+                const float dh = ldl<float>(smem, HS + jj * kRB) - ldl<float>(smem, HS + i * kRB);
+                const float ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
+                const float ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
+                // c = conj(X_ji) - X_ij + (h_j - h_i) Y_ij;  k = -i c
+                const float cx = c0.x + xji.x - xij.x + dh * ycx;
+                const float cy = c0.y - xji.y - xij.y + dh * ycy;
+                dev::rk_update<ST>(yq.x, Aq.x, pq.x, cy, h8);
+                dev::rk_update<ST>(yq.y, Aq.y, pq.y, -cx, h8);
+            }
+        }
+        if (dslot && dblk == BLK) {  // k_ii = -2 Im X_ii
+            const float k = -2.f * ldl<C>(smem, da + xo).y;
+            if constexpr (SG == 0) dev::rk_update<ST>(dy0, dA0, dp0, k, h8);
+            else dev::rk_update<ST>(dy1, dA1, dp1, k, h8);
+        }
+    };
+
+    // ---- records of the time point, one spin per pass (rows hold that spin's stage-0 input) ----
+    float* rr = nullptr;
+    auto record_part = [&](auto sgc) {
+        constexpr int SG = decltype(sgc)::value;
+        int tt = tid;
+        asm volatile("" : "+v"(tt));
+        if (tt < L) rr[SG * L + tt] = yat(tt, tt).x;
+        if (tid == 0) {
+            const C c = yat(r.ci, r.cj);
+            rr[R2 + 4 + 2 * SG] = c.x;
+            rr[R2 + 5 + 2 * SG] = c.y;
+        }
+        float eth = 0, eu = 0, epe = 0;
+        if (tt < L) {  // hopping terms of row tt: Y(k_n, tt) = conj(Y(tt, k_n))
+#pragma unroll
+            for (int n = 0; n < Z; ++n) {
+                int kb;
+                float tv;
+                nbr(n, tt, kb, tv);
+                eth += tv * ldl<C>(smem, kb + tt * kCB).x;
+            }
+        }
+        const int st = tt - 2 * 64;  // on-site terms on waves 2 and 3
+        if (SG == 0 && st >= 0 && st < L) {
+            const float nu = ldl<float>(smem, kDg + st * kRB) + half;
+            const float nd = ldl<float>(smem, kDg + kHSpin + st * kRB) + half;
+            eth += ldl<float>(smem, kSite + st * kRB) * (nu + nd);
+            epe += ldl<float>(smem, kSite + (kLP + st) * kRB) * (nu + nd);
+            eu += ldl<float>(smem, kSite + (2 * kLP + st) * kRB) * nu * nd;
+        }
+        eth = wave_sum(eth);
+        eu = wave_sum(eu);
+        epe = wave_sum(epe);
+        if (j == 0) {  // the same wave wrote this entry in the spin-up pass, barriers earlier
+            const float e0 = SG ? ew[w * 4 + 0] : 0.f, e1 = SG ? ew[w * 4 + 1] : 0.f, e3 = SG ? ew[w * 4 + 3] : 0.f;
+            ew[w * 4 + 0] = e0 + eth;
+            ew[w * 4 + 1] = e1 + eu;
+            ew[w * 4 + 2] = 0;
+            ew[w * 4 + 3] = e3 + epe;
+        }
+    };
+    auto flush_energies = [&]() {
+        if (tid == 0) {
+            float e4[4] = {0, 0, 0, 0};
+            for (int ww = 0; ww < kW; ++ww)
+                for (int qq = 0; qq < 4; ++qq) e4[qq] += ew[ww * 4 + qq];
+            for (int qq = 0; qq < 4; ++qq) rr[R2 + qq] = e4[qq];
+        }
+    };
+
+    auto spin_pass = [&](auto stc, auto sgc, bool record) {
+        materialize(stc, sgc);
+        __syncthreads();
+        if (record) record_part(sgc);
+        phase_a(Ic<0>{});
+        __syncthreads();
+        if (record && decltype(sgc)::value == 1) flush_energies();
+        phase_b(Ic<0>{}, stc, sgc);
+        __syncthreads();
+        phase_a(Ic<1>{});
+        __syncthreads();
+        phase_b(Ic<1>{}, stc, sgc);
+    };
+    auto stage = [&](auto stc, bool record) {
+        spin_pass(stc, Ic<0>{}, record);
+        spin_pass(stc, Ic<1>{}, record);
+    };
+
+    const int64_t Bn = r.B;
+    const int R = r.R;
+    for (int t = t0; t < t1; ++t) {
+        rr = rec + (static_cast<int64_t>(t - t0) * Bn + b) * R;
+        if (t == r.T - 1) {  // no step after the last record changes an output
+            materialize(Ic<0>{}, Ic<0>{});
+            __syncthreads();
+            record_part(Ic<0>{});
+            __syncthreads();
+            materialize(Ic<0>{}, Ic<1>{});
+            __syncthreads();
+            record_part(Ic<1>{});
+            __syncthreads();
+            flush_energies();
+            break;
+        }
+        for (int sub = 0; sub < r.nsub; ++sub) {
+            stage(Ic<0>{}, sub == 0);
+            stage(Ic<1>{}, false);
+            stage(Ic<2>{}, false);
+            stage(Ic<3>{}, false);
+        }
+    }
+    // ---- state back to HBM, from the RK registers ----
+    C* go = reinterpret_cast<C*>(state) + b * 2 * LL;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const int i = ah[q] & 0xffff, col = ah[q] >> 16;
+        if (i >= L) continue;
+        go[i * L + col] = y0[q];
+        go[col * L + i] = C{y0[q].x, -y0[q].y};
+        go[LL + i * L + col] = y1[q];
+        go[LL + col * L + i] = C{y1[q].x, -y1[q].y};
+    }
+    if (dslot) {
+        go[tid * L + tid] = C{dy0, 0.f};
+        go[LL + tid * L + tid] = C{dy1, 0.f};
+    }
+}
+
+size_t wide_lds_bytes(int L) { return kRows + static_cast<size_t>(L + 1) * wide_row_len(L) * kCB; }
+
+template <int S, int Z>
+hipError_t run_wide(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
+                    hipStream_t st)
+{
+    auto kern = herm_wide_kernel<S, Z>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(kNT), wide_lds_bytes(s.L), st, s, r, t0, t1,
+                       state, rec);
+    return hipGetLastError();
+}
+
+template <int Z>
+hipError_t run_wide_s(int S, const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
+                      hipStream_t st)
+{
+    switch (S) {  // 64 < L <= 107: 33 to 90 element rows, 5 to 12 slots per thread
+        case 5: return run_wide<5, Z>(s, r, t0, t1, state, rec, st);
+        case 6: return run_wide<6, Z>(s, r, t0, t1, state, rec, st);
+        case 7: return run_wide<7, Z>(s, r, t0, t1, state, rec, st);
+        case 8: return run_wide<8, Z>(s, r, t0, t1, state, rec, st);
+        case 9: return run_wide<9, Z>(s, r, t0, t1, state, rec, st);
+        case 10: return run_wide<10, Z>(s, r, t0, t1, state, rec, st);
+        case 11: return run_wide<11, Z>(s, r, t0, t1, state, rec, st);
+        case 12: return run_wide<12, Z>(s, r, t0, t1, state, rec, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+bool herm_wide_fits(const DevSystem<float>& s)
+{
+    return s.L > 64 && s.L <= kLP && (s.z == 4 || s.z == 8) && !s.amp && s.coulomb_type != 2 &&
+           wide_lds_bytes(s.L) <= kLdsLimit;
+}
+
+hipError_t launch_herm_wide(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
+                            hipStream_t st)
+{
+    if (!herm_wide_fits(s) || r.dens || r.B <= 0 || r.B > 0x7fffffff) return hipErrorInvalidValue;
+    const int S = std::max(5, (dev::class_rows(s.L) + kW - 1) / kW);
+    return s.z == 4 ? run_wide_s<4>(S, s, r, t0, t1, state, rec, st) : run_wide_s<8>(S, s, r, t0, t1, state, rec, st);
+}
+
+}  // namespace fpsamd

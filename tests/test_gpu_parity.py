@@ -81,6 +81,9 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
                 runs.append(("fp32", "pipelined"))
                 if L <= 64 and bases is None:
                     runs.append(("fp32", "sequential"))
+            if (64 < L <= 107 and z <= 8 and (pulse or {}).get("pulse_type", 0) == 0 and coulomb_type != 2
+                    and bases is None):  # the column-blocked spin-sequential kernel
+                runs += [("fp32", "sequential"), ("fp32", "auto")]
         elif e == "fp64":  # upper-triangle (16 and 8 waves) and stencil kernels where they fit, the dense path
             herm_seq = (stencil and L <= 64 and (pulse or {}).get("pulse_type", 0) == 0 and coulomb_type != 2
                         and bases is None)
@@ -156,6 +159,48 @@ def test_site_counts(oracle_mod, L):
     n_dn = (rng.uniform(size=L) < 0.5).astype(float)
     compare(oracle_mod, chain(L, periodic=True), n_up, n_dn, U=3.0 * np.eye(L), dt=0.02, t_max=0.1, B=3,
             engines=("fp64", "fp32"))
+
+
+@pytest.mark.parametrize("shape", ["chain80", "grid10x10", "chain90_reach4", "chain107"])
+def test_wide_lattices(oracle_mod, shape):
+    """64 < L <= 107: the FP32 column-blocked spin-sequential kernel (automatic and forced) and the dense
+    path against the oracle, with site potentials; z = 4 and 8, L at the kernel's LDS limit."""
+    rng = np.random.default_rng(hash(shape) % 1000)
+    if shape == "grid10x10":
+        pos, box = lat.grid(10, 10)
+        tM, _ = lat.system_arrays(pos, box, True)
+    elif shape == "chain90_reach4":
+        L = 90
+        tM = np.zeros((L, L))
+        for d in range(1, 5):
+            for i in range(L - d):
+                tM[i, i + d] = tM[i + d, i] = -1.0 / d
+    else:
+        tM = chain(int(shape[5:]), periodic=True)
+    L = tM.shape[0]
+    n_up = (rng.uniform(size=L) < 0.5).astype(float)
+    n_dn = (rng.uniform(size=L) < 0.5).astype(float)
+    compare(oracle_mod, tM, n_up, n_dn, U=4.0 * np.eye(L), pot=rng.uniform(-1, 1, L), dt=0.02, t_max=0.1, B=3,
+            engines=("fp64", "fp32"))
+
+
+def test_wide_kernel_is_chunking_invariant():
+    """The column-blocked kernel's result does not depend on the time chunking (bitwise)."""
+    pos, box = lat.grid(10, 10)
+    tM, Um = lat.system_arrays(pos, box, True)
+    L = tM.shape[0]
+    nu = np.array(lat.checkerboard(10, 10)["IC_NuDiag"], dtype=float)
+    eng = _lib.Engine(tM, U=Um)
+    eng.set_initial_state(nu, nu)
+    out = []
+    for chunk in (0, 3):
+        eng.set_tuning(0, chunk, "sequential")
+        eng.set_params(0.01, 0.2, 1, 512, seed=8, engine="fp32")
+        eng.run()
+        out.append((eng.states(), eng.result("sum_u"), eng.result("sum_e")))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    eng.close()
 
 
 def test_general_coulomb_and_potential(oracle_mod):
