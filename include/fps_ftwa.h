@@ -211,8 +211,9 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
 #define FPSFTWA_KERNEL_PIPELINED 4 /* upper-triangle kernel with the two spins half a stage apart
                                     * (FP32, L <= 64, no pulse, null or on-site Coulomb)              */
 #define FPSFTWA_KERNEL_SEQUENTIAL 5 /* upper-triangle kernel with one spin's rows in LDS at a time,
-                                    * two workgroups per CU in FP32 (L <= 64, no pulse, null or
-                                    * on-site Coulomb, no basis occupations)                          */
+                                    * two workgroups per CU in FP32 (L <= 64; FP32 also 64 < L <= 107
+                                    * with X formed in 64-column blocks; no pulse, null or on-site
+                                    * Coulomb, no basis occupations)                                  */
 int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel);
 
 /* Autotuner (no reference counterpart): times every batched kernel family and shape that can run
