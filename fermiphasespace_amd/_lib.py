@@ -41,7 +41,7 @@ EXPORTED = [
     "fpsftwa_sample_noise", "fpsftwa_run", "fpsftwa_get_result", "fpsftwa_get_states", "fpsftwa_get_timing",
     "fpsftwa_set_tuning", "fpsftwa_run_json_file", "fpsftwa_load_system_json", "fpsftwa_run_adjoint",
     "fpsftwa_autotune", "fpsftwa_run_many", "fpsftwa_load_initial_state_json", "fpsftwa_ground_state",
-    "fpsftwa_set_bases", "fpsftwa_set_recorded_element", "fpsftwa_set_method",
+    "fpsftwa_set_bases", "fpsftwa_set_recorded_element", "fpsftwa_set_method", "fpsftwa_set_stream",
 ]
 
 
@@ -114,6 +114,7 @@ def load(path: str = LIB_PATH):
         "fpsftwa_ground_state": ([i32, P(d), P(d), i32, i32, i32, P(d), P(d), P(i32), P(d)], ctypes.c_int),
         "fpsftwa_set_bases": ([H, i32, P(d)], ctypes.c_int),
         "fpsftwa_set_recorded_element": ([H, i32, i32], ctypes.c_int),
+        "fpsftwa_set_stream": ([H, ctypes.c_void_p], ctypes.c_int),
         "fpsftwa_set_method": ([H, i32], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -322,6 +323,11 @@ class Engine:
         matrix, diffusion noise from the run's seed)."""
         m = {"ftwa": METHOD_FTWA, "hf": METHOD_HF, "gpsr": METHOD_GPSR}[method.lower()]
         check(self._lib.fpsftwa_set_method(self._h, m), "set_method")
+
+    def set_stream(self, stream: int | None):
+        """Enqueue this engine's work on a HIP stream given by its handle (an int, e.g. a torch stream's
+        cuda_stream); None returns to the engine's own stream."""
+        check(self._lib.fpsftwa_set_stream(self._h, ctypes.c_void_p(stream) if stream else None), "set_stream")
 
     def set_recorded_element(self, i: int, j: int):
         """The element rho(i, j) behind result("N1hu") / result("N1hd") (default (0, L-1))."""

@@ -72,7 +72,8 @@ bool is_gfx950(int dev, std::string* name)
 
 struct fpsftwa_handle {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream every call enqueues on: own_stream or the caller's
+    hipStream_t own_stream = nullptr;  // created with the handle
     // system (host copy)
     int L = 0;
     int coulomb_type = 0;
@@ -759,10 +760,11 @@ int fpsftwa_create(const fpsftwa_system* sys, fpsftwa_handle** out)
     auto* h = new (std::nothrow) fpsftwa_handle();
     if (!h) return fail(FPSFTWA_ENOMEM, "host allocation failed");
     h->device = dev;
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return fail(FPSFTWA_EDEVICE, "hipStreamCreate failed");
     }
+    h->stream = h->own_stream;
     const size_t LL = (size_t)L * L;
     h->L = L;
     h->coulomb_type = sys->coulomb_type;
@@ -811,7 +813,7 @@ int fpsftwa_destroy(fpsftwa_handle* h)
     h->basis_work.release();
     h->basis_out.release();
     h->gnoise.release();
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
     return FPSFTWA_OK;
 }
@@ -891,6 +893,16 @@ int fpsftwa_set_method(fpsftwa_handle* h, int32_t method)
         return fail(FPSFTWA_EINVAL, "unknown method");
     h->method = method;
     h->has_result = false;
+    return FPSFTWA_OK;
+}
+
+int fpsftwa_set_stream(fpsftwa_handle* h, void* stream)
+{
+    if (int rc = check_handle(h)) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));  // nothing of this handle is in flight on the old stream
+    h->stream = stream ? static_cast<hipStream_t>(stream) : h->own_stream;
+    dense_set_stream(h->dense, h->stream);
     return FPSFTWA_OK;
 }
 

@@ -665,6 +665,13 @@ DenseCtx* dense_create(hipStream_t st)
     return c;
 }
 
+void dense_set_stream(DenseCtx* c, hipStream_t st)
+{
+    if (!c) return;
+    rocblas_set_stream(c->blas, st);
+    c->stream = st;
+}
+
 void dense_destroy(DenseCtx* c)
 {
     if (!c) return;

@@ -134,6 +134,7 @@ hipError_t launch_gpsr(const DevSystem<Real>& s, const RunShape& r, int t0, int 
 // hand-written Hartree, commutator/RK update and record kernels.  L <= 1024, any neighbour count.
 struct DenseCtx;
 DenseCtx* dense_create(hipStream_t st);
+void dense_set_stream(DenseCtx* c, hipStream_t st);
 void dense_destroy(DenseCtx* c);
 // elements of Real the dense path needs for B trajectories (one buffer)
 template <typename Real>

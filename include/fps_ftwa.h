@@ -183,6 +183,12 @@ int fpsftwa_set_noise(fpsftwa_handle* h, const double* noise, size_t n_values);
 /* The generator's noise for trajectories [first, first+count), same layout as above. */
 int fpsftwa_sample_noise(fpsftwa_handle* h, int64_t first, int64_t count, double* out, size_t n_values);
 
+/* The HIP stream (a hipStream_t, passed as void*) the handle enqueues its work on; NULL returns to the
+ * stream the handle created.  The reference has no streams; this is the stream argument of a run
+ * (the caller's stream, e.g. a framework's current stream).  Waits for the handle's work on the
+ * previous stream.  Runs stay synchronous: fpsftwa_run returns after its results reached the host. */
+int fpsftwa_set_stream(fpsftwa_handle* h, void* stream);
+
 /* FHfTWASolver::runSimulation (src/FHfTWASolver.cpp:222-556) without the file write:
  * samples, integrates every trajectory, reduces the ensemble.  Synchronous. */
 int fpsftwa_run(fpsftwa_handle* h);

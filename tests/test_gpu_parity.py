@@ -735,3 +735,34 @@ def test_autotune_gemm_solution_on_the_long_range_path(oracle_mod, engine):
     assert rel(e.states(), ref.final_states) < tol
     assert rel(e.result("sum_u"), ref.sum_u) < tol
     e.close()
+
+
+def test_caller_stream():
+    """fpsftwa_set_stream: the same runs on a caller's HIP stream (a torch stream, with other work queued
+    ahead of them) give bitwise the same results, on the spin-sequential kernel and on the dense path
+    (whose library GEMM follows the stream); NULL returns to the handle's own stream."""
+    import torch
+    s = torch.cuda.Stream()
+    for L, kernel in ((16, "auto"), (70, "dense")):
+        tM = chain(L, periodic=True)
+        nu = (np.arange(L) % 2 == 0).astype(float)
+        eng = _lib.Engine(tM, U=2.0 * np.eye(L))
+        eng.set_initial_state(nu, 1 - nu)
+        eng.set_tuning(0, 0, kernel)
+        eng.set_params(0.01, 0.1, 1, 256, seed=3, engine="fp32")
+        eng.run()
+        ref = (eng.states(), eng.result("sum_u"), eng.result("sum_e"))
+        eng.set_stream(s.cuda_stream)
+        with torch.cuda.stream(s):
+            x = torch.ones(1 << 22, device="cuda")
+            y = x * 2.0
+        eng.run()
+        got = (eng.states(), eng.result("sum_u"), eng.result("sum_e"))
+        torch.cuda.synchronize()
+        assert float(y.sum()) == 2.0 * (1 << 22)
+        eng.set_stream(None)
+        eng.run()
+        again = (eng.states(), eng.result("sum_u"), eng.result("sum_e"))
+        for a, b, c in zip(ref, got, again):
+            assert np.array_equal(a, b) and np.array_equal(a, c), (L, kernel)
+        eng.close()
