@@ -203,6 +203,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         for (int n = 0; n < Z; ++n) nbr(n, kb[n], tv[n]);
 #pragma unroll
         for (int m = 0; m < NCG; ++m) {
+            if (m * W + w >= L) break;  // columns past L are never read (wave-uniform)
             const int co = m * W * kCB;  // immediate column offset
             Real ax = 0, ay = 0;
 #pragma unroll
