@@ -199,6 +199,15 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
             }
             const int xb = (on ? kRows + i * RB : spare) + (L + w) * kCB;
             const int yc = (64 * BLK + w) * kCB;
+            // software pipelined as in ftwa_herm_seq.hip where the registers allow (z = 4, S <= 10): the
+            // next column's reads go out before this column's X store; the first read (column
+            // 64 BLK + w < L + 65) stays inside the row
+            constexpr bool kPipe = Z == 4 && S <= 10;
+            C v[Z];
+            if constexpr (kPipe) {
+#pragma unroll
+                for (int n = 0; n < Z; ++n) v[n] = ldl<C>(smem, kb[n] + yc);
+            }
 #pragma unroll
             for (int m = 0; m < 64 / kW; ++m) {
                 if (64 * BLK + w + kW * m >= L) break;  // wave-uniform
@@ -206,9 +215,13 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
                 float ax = 0, ay = 0;
 #pragma unroll
                 for (int n = 0; n < Z; ++n) {
-                    const C v = ldl<C>(smem, kb[n] + yc + co);
-                    ax += tv[n] * v.x;
-                    ay += tv[n] * v.y;
+                    const C vn = kPipe ? v[n] : ldl<C>(smem, kb[n] + yc + co);
+                    ax += tv[n] * vn.x;
+                    ay += tv[n] * vn.y;
+                }
+                if (kPipe && m + 1 < 64 / kW && 64 * BLK + w + kW * (m + 1) < L) {
+#pragma unroll
+                    for (int n = 0; n < Z; ++n) v[n] = ldl<C>(smem, kb[n] + yc + co + kW * kCB);
                 }
                 *reinterpret_cast<C*>(smem + xb + co) = C{ax, ay};
             }
