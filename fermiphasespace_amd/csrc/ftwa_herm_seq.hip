@@ -18,8 +18,8 @@
 // In FP32 the one-spin rows take 67 KB, so two workgroups share a CU and cover each other's barriers.
 // Null or on-site Coulomb interactions, no pulse, no density records, L <= 64 (FP32 64 < L <= 107: the
 // column-blocked form in ftwa_herm_wide.hip); everything else runs on the stencil, upper-triangle FP32
-// or dense kernels.  On the 8x8 workload the FP64 variant runs 1.35e7 trajectory-steps/s against 6.62e6
-// for the full-matrix stencil kernel, the FP32 variant 2.87e7.
+// or dense kernels.  On the 8x8 workload the FP64 variant runs 1.38e7 trajectory-steps/s against 6.62e6
+// for the full-matrix stencil kernel, the FP32 variant 3.12e7.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
