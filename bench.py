@@ -105,6 +105,16 @@ def cpu_baseline(w, dt, target_s=15.0):
 # ---- the other workloads (parity cases of the build; one JSON line each, not the headline) ----------
 
 
+def measured_peaks(args):
+    """fpsftwa_measure_peaks on this device (about a second), or None with --no-peaks."""
+    if getattr(args, "no_peaks", False):
+        return None
+    from fermiphasespace_amd import _lib
+    pk = _lib.measure_peaks()
+    log("measured peaks", {k: round(v, 2) for k, v in pk.items()})
+    return pk
+
+
 def _timed_run(eng, sync):
     sync()
     t0 = time.perf_counter()
@@ -265,6 +275,10 @@ def side_moire196(args, sync):
                         "frac": round(achieved / 157.3, 4), "traffic": None,
                         "note": "GEMM flop over the whole integrator time (GEMM + update + Hartree + record kernels)",
                         "kernel_ms": round(tm["kernel_ms"], 3), "launches": tm["launches"]}}
+    pk = measured_peaks(args)
+    if pk:
+        out["roofline"]["measured"] = {"fp32_mfma_tflops": round(pk["fp32_mfma_tflops"], 2),
+                                       "frac_of_mfma": round(achieved / pk["fp32_mfma_tflops"], 4)}
     # the autotuner's pick (kernel family and, on this path, the library's GEMM solution), same run
     t0 = time.perf_counter()
     choice = eng.autotune(trial_steps=4)
@@ -360,6 +374,8 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 batched reference run")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip the measured device peaks (copy, FMA and MFMA loops) beside the published ones")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--states-steps", type=int, default=0,
                     help="also time this many FP32 steps recording the tight-binding eigenstate occupations "
@@ -452,6 +468,15 @@ def main():
             # figure by the same fixed formula, against the FP32 vector peak
             "valu": {"achieved": round(valu_achieved, 2), "peak": 157.3, "unit": "TFLOP/s",
                      "frac": round(valu_achieved / 157.3, 4), "alg_flop_per_traj_step": valu_flop}}
+    pk = measured_peaks(args)
+    if pk:
+        # the same achieved figures against what a copy kernel and a (non-packed) FMA loop reach here
+        roof["measured"] = {"hbm_copy_gbps": round(pk["hbm_copy_gbps"], 1),
+                            "frac_of_copy": round(achieved / pk["hbm_copy_gbps"], 4),
+                            "fp32_fma_tflops": round(pk["fp32_fma_tflops"], 2),
+                            "valu_frac_of_fma": round(valu_achieved / pk["fp32_fma_tflops"], 4),
+                            "fp64_fma_tflops": round(pk["fp64_fma_tflops"], 2),
+                            "fp32_mfma_tflops": round(pk["fp32_mfma_tflops"], 2)}
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "traj-steps/s", "n_gpus": 1, "steps": K,
            "warmup": W, "ms_per_step": round(wall * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak",

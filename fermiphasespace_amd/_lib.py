@@ -42,6 +42,7 @@ EXPORTED = [
     "fpsftwa_set_tuning", "fpsftwa_run_json_file", "fpsftwa_load_system_json", "fpsftwa_run_adjoint",
     "fpsftwa_autotune", "fpsftwa_run_many", "fpsftwa_load_initial_state_json", "fpsftwa_ground_state",
     "fpsftwa_set_bases", "fpsftwa_set_recorded_element", "fpsftwa_set_method", "fpsftwa_set_stream",
+    "fpsftwa_measure_peaks",
 ]
 
 
@@ -53,6 +54,12 @@ class FtwaError(RuntimeError):
 
 class _AdjointSpec(ctypes.Structure):
     _fields_ = [("objective", ctypes.c_int32), ("checkpoint_every", ctypes.c_int32)]
+
+
+class _Peaks(ctypes.Structure):
+    _fields_ = [("hbm_copy_gbps", ctypes.c_double), ("copy_bytes", ctypes.c_double),
+                ("fp32_fma_tflops", ctypes.c_double), ("fp64_fma_tflops", ctypes.c_double),
+                ("fp32_mfma_tflops", ctypes.c_double)]
 
 
 class _System(ctypes.Structure):
@@ -92,6 +99,7 @@ def load(path: str = LIB_PATH):
         "fpsftwa_last_error": ([], ctypes.c_char_p),
         "fpsftwa_version": ([P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)], ctypes.c_int),
         "fpsftwa_device_info": ([ctypes.c_char_p, sz, P(ctypes.c_int), P(d)], ctypes.c_int),
+        "fpsftwa_measure_peaks": ([P(_Peaks)], ctypes.c_int),
         "fpsftwa_create": ([P(_System), P(H)], ctypes.c_int),
         "fpsftwa_destroy": ([H], ctypes.c_int),
         "fpsftwa_set_params": ([H, d, d, i32, i64, i32, u64, i32], ctypes.c_int),
@@ -151,6 +159,14 @@ def device_info() -> dict:
     gib = ctypes.c_double(0)
     check(lib.fpsftwa_device_info(name, 256, ctypes.byref(cu), ctypes.byref(gib)), "device_info")
     return {"name": name.value.decode(), "cu": cu.value, "hbm_gib": gib.value}
+
+
+def measure_peaks() -> dict:
+    """HBM copy bandwidth and FMA / MFMA rates measured on the current device (fpsftwa_measure_peaks)."""
+    lib = load()
+    pk = _Peaks()
+    check(lib.fpsftwa_measure_peaks(ctypes.byref(pk)), "measure_peaks")
+    return {name: getattr(pk, name) for name, _ in _Peaks._fields_}
 
 
 def load_system_json(path: str) -> dict:

@@ -51,6 +51,7 @@ def build(verbose: bool = False) -> str:
         ("ftwa_herm_seq.hip", "ftwa_herm_seq.o", []),
         ("ftwa_herm_wide.hip", "ftwa_herm_wide.o", []),
         ("ftwa_gpsr.hip", "ftwa_gpsr.o", []),
+        ("ftwa_peaks.hip", "ftwa_peaks.o", []),
         ("ftwa_adjoint.hip", "ftwa_adjoint.o", []),
         ("ftwa_dense.hip", "ftwa_dense.o", []),
         ("ftwa_api.cpp", "ftwa_api.o", ["-x", "hip"]),

@@ -131,13 +131,15 @@ __global__ void dense_hartree_kernel(DevSystem<Real> s, const Real* ycur, Real* 
     }
 }
 
-// RK stage ST on the upper triangle.  Grid: (tile pairs x B, 2 spins); 256 threads.
+// RK stage ST on the upper triangle.  Grid: (tile pairs x B, 2 spins); 256 threads.  The new stage input
+// of tile (I, J) goes back through LDS (over the X tile (I, J), which only its own thread reads) so
+// that the mirrored tile (J, I) is written row by row, coalesced, instead of one row per lane.
 template <typename Real, int ST>
 __global__ void __launch_bounds__(256)
     dense_update_kernel(int L, int npair, double hstep, const int2* pairs, const Real* X, const Real* hv, Real* y,
                         Real* A, Real* p, Real* ycur)
 {
-    __shared__ Real xa[2][kTile][kTile + 1];  // X tile (I, J), planes re/im
+    __shared__ Real xa[2][kTile][kTile + 1];  // X tile (I, J), planes re/im; then the new stage input
     __shared__ Real xb[2][kTile][kTile + 1];  // X tile (J, I)
     const int64_t b = blockIdx.x / npair;
     const int2 pr = pairs[blockIdx.x - b * npair];
@@ -234,11 +236,17 @@ __global__ void __launch_bounds__(256)
         }
         ycur[o] = nr;
         ycur[o + L] = ni;
-        if (i != j) {
-            const size_t om = row_off<Real>(b, sg, j, L) + i;
-            ycur[om] = nr;
-            ycur[om + L] = -ni;
-        }
+        xa[0][r][tx] = nr;
+        xa[1][r][tx] = ni;
+    }
+    __syncthreads();
+    // mirrored tile: Y_ji = conj(Y_ij) for i < j, row j = J0 + r, column i = I0 + tx
+    for (int r = ty; r < kTile; r += 8) {
+        const int jr = J0 + r, ic = I0 + tx;
+        if (jr >= L || ic >= L || ic >= jr) continue;
+        const size_t om = row_off<Real>(b, sg, jr, L) + ic;
+        ycur[om] = xa[0][tx][r];
+        ycur[om + L] = -xa[1][tx][r];
     }
 }
 
@@ -547,31 +555,58 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
     double eth = 0, eu = 0, ev = 0, epe = 0;
     const bool pulse = s.amp != nullptr;
     const double a = pulse ? s.amp[stage] : 0.0;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) {
-        const size_t ou = row_off<Real>(b, 0, i, L), od = row_off<Real>(b, 1, i, L);
-        for (int n = 0; n <= s.z; ++n) {  // the neighbour list of i, then the diagonal entry
-            const int j = n < s.z ? s.nbr_k[n * L + i] : i;
-            const double tt = n < s.z ? static_cast<double>(s.nbr_t[n * L + i]) : static_cast<double>(s.tdiag[i]);
-            if (tt == 0.0) continue;
-            const double nr = static_cast<double>(ycur[ou + j]) + static_cast<double>(ycur[od + j]) + (i == j ? 1.0 : 0.0);
-            const double ni = static_cast<double>(ycur[ou + L + j]) + static_cast<double>(ycur[od + L + j]);
-            eth += nr * tt;
-            if (pulse) {
-                double sn = 0.0, cs = 1.0;
-                if (n < s.z) sincos(a * static_cast<double>(s.nbr_pd[n * L + i]), &sn, &cs);
-                epe += nr * tt * cs - ni * tt * sn;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (blockDim.x + 63) / 64;
+    if (4 * s.z >= L) {
+        // long-range hopping: rows of the dense tM, one wave per row and lanes along it, so the rows of
+        // the stage input are read coalesced (one lane per row touched a cache line per neighbour and
+        // fetched 17.9 GB per time point on the 196-site bilayer)
+        for (int i = wv; i < L; i += nwv) {
+            const size_t ou = row_off<Real>(b, 0, i, L), od = row_off<Real>(b, 1, i, L);
+            for (int j = lane; j < L; j += 64) {
+                const double tt = static_cast<double>(s.tM[static_cast<size_t>(i) * L + j]);
+                if (tt == 0.0) continue;
+                const double nr = static_cast<double>(ycur[ou + j]) + static_cast<double>(ycur[od + j]) + (i == j ? 1.0 : 0.0);
+                const double ni = static_cast<double>(ycur[ou + L + j]) + static_cast<double>(ycur[od + L + j]);
+                eth += nr * tt;
+                if (pulse) {
+                    double sn = 0.0, cs = 1.0;
+                    if (i != j) sincos(a * static_cast<double>(s.pd[static_cast<size_t>(i) * L + j]), &sn, &cs);
+                    epe += nr * tt * cs - ni * tt * sn;
+                }
             }
         }
+    } else {
+        for (int i = threadIdx.x; i < L; i += blockDim.x) {
+            const size_t ou = row_off<Real>(b, 0, i, L), od = row_off<Real>(b, 1, i, L);
+            for (int n = 0; n <= s.z; ++n) {  // the neighbour list of i, then the diagonal entry
+                const int j = n < s.z ? s.nbr_k[n * L + i] : i;
+                const double tt = n < s.z ? static_cast<double>(s.nbr_t[n * L + i]) : static_cast<double>(s.tdiag[i]);
+                if (tt == 0.0) continue;
+                const double nr = static_cast<double>(ycur[ou + j]) + static_cast<double>(ycur[od + j]) + (i == j ? 1.0 : 0.0);
+                const double ni = static_cast<double>(ycur[ou + L + j]) + static_cast<double>(ycur[od + L + j]);
+                eth += nr * tt;
+                if (pulse) {
+                    double sn = 0.0, cs = 1.0;
+                    if (n < s.z) sincos(a * static_cast<double>(s.nbr_pd[n * L + i]), &sn, &cs);
+                    epe += nr * tt * cs - ni * tt * sn;
+                }
+            }
+        }
+    }
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        const size_t ou = row_off<Real>(b, 0, i, L), od = row_off<Real>(b, 1, i, L);
         const double nu = static_cast<double>(ycur[ou + i]) + 0.5, ndn = static_cast<double>(ycur[od + i]) + 0.5;
         epe += static_cast<double>(s.pot[i]) * (nu + ndn);
-        if (s.coulomb_type == 1) {
-            eu += static_cast<double>(s.u_diag[i]) * nu * ndn;
-        } else if (s.coulomb_type == 2) {
-            for (int j = 0; j < L; ++j) {
+        if (s.coulomb_type == 1) eu += static_cast<double>(s.u_diag[i]) * nu * ndn;
+    }
+    if (s.coulomb_type == 2) {  // rows of U and V along the lanes, occupations from LDS
+        for (int i = wv; i < L; i += nwv) {
+            const double nu = dg[i], ndn = dg[L + i];
+            for (int j = lane; j < L; j += 64) {
                 const double nuj = dg[j];
                 const double ndj = dg[L + j];
-                eu += static_cast<double>(s.U[i * L + j]) * (nu * ndj + ndn * nuj) / 2.0;
-                ev += static_cast<double>(s.V[i * L + j]) * (nu * nuj + ndn * ndj) / 2.0;
+                eu += static_cast<double>(s.U[static_cast<size_t>(i) * L + j]) * (nu * ndj + ndn * nuj) / 2.0;
+                ev += static_cast<double>(s.V[static_cast<size_t>(i) * L + j]) * (nu * nuj + ndn * ndj) / 2.0;
             }
         }
     }

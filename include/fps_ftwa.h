@@ -123,6 +123,19 @@ const char* fpsftwa_last_error(void);
 int fpsftwa_version(int* major, int* minor, int* patch);
 int fpsftwa_device_info(char* name, size_t name_len, int* cu_count, double* hbm_gib);
 
+/* Peaks measured on the current device (no counterpart in the reference: the roofline denominators
+ * re-measured on the machine, beside the published figures): a 1 GiB device copy (bytes read plus
+ * written per second), FP32 and FP64 FMA loops (non-packed FP32, as the integrators are built) and a
+ * 32x32x2 FP32 MFMA loop.  Allocates 2 GiB of device memory for the copy and takes about a second. */
+typedef struct fpsftwa_peaks {
+    double hbm_copy_gbps;     /* (read + written bytes) / s, in GB/s                          */
+    double copy_bytes;        /* bytes moved per copy launch                                  */
+    double fp32_fma_tflops;   /* v_fma_f32, 2 flop per lane and instruction                   */
+    double fp64_fma_tflops;   /* v_fma_f64                                                    */
+    double fp32_mfma_tflops;  /* v_mfma_f32_32x32x2_f32, 4,096 flop per wave and instruction  */
+} fpsftwa_peaks;
+int fpsftwa_measure_peaks(fpsftwa_peaks* out);
+
 /* FHfTWASolver::FHfTWASolver(FHSystem*) -- include/FHfTWASolver.h:28, src/FHfTWASolver.cpp:4-16.
  * Copies the system in; fails with FPSFTWA_EDEVICE when no gfx950 device is usable. */
 int fpsftwa_create(const fpsftwa_system* system, fpsftwa_handle** out);

@@ -766,3 +766,15 @@ def test_caller_stream():
         for a, b, c in zip(ref, got, again):
             assert np.array_equal(a, b) and np.array_equal(a, c), (L, kernel)
         eng.close()
+
+
+def test_measured_peaks_are_below_the_published_figures():
+    """fpsftwa_measure_peaks: each measured rate lies between a tenth of MI355X's published figure and the
+    figure itself (5 % allowance for clock spread). A wrong flop or byte count per launch (a factor of 2
+    or more) leaves this window: 8 TB/s HBM, 78.6 TFLOP/s non-packed FP32 FMA (256 CUs x 4 SIMDs x 16
+    lanes x 2 flop x 2.4 GHz), 78.6 TFLOP/s FP64 vector, 157.3 TFLOP/s FP32 MFMA."""
+    pk = _lib.measure_peaks()
+    assert pk["copy_bytes"] == 2.0 * (1 << 30)
+    for key, published in (("hbm_copy_gbps", 8000.0), ("fp32_fma_tflops", 78.6), ("fp64_fma_tflops", 78.6),
+                           ("fp32_mfma_tflops", 157.3)):
+        assert 0.1 * published < pk[key] < 1.05 * published, (key, pk[key])
