@@ -396,7 +396,7 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
                                               "neighbours, no pulse, null or on-site Coulomb and no basis occupations");
     } else if constexpr (std::is_same<Real, float>::value) {
         // automatic choice: the spin-sequential kernel where it applies (two workgroups per CU:
-        // 3.12e7 trajectory-steps/s on the 8x8 workload, against 2.51e7 for the both-spin kernel), its
+        // 3.12e7 trajectory-steps/s on the 8x8 workload, against 2.63e7 for the both-spin kernel), its
         // column-blocked form for 64 < L <= 107, else the both-spin upper-triangle kernel (pulses, V
         // interactions, basis occupations)
         seq = h->kernel == FPSFTWA_KERNEL_AUTO && herm_seq_fits<float>(ds) && h->n_bases == 0;

@@ -233,22 +233,35 @@ __global__ void __launch_bounds__(64 * W)
 #pragma unroll
                 for (int n = 0; n < Z; ++n) T[n] = Tc[n * L + il];
             }
+            // software pipelined where the registers allow: the next column group's reads go out before
+            // this group's X store (the compiler keeps reads behind an earlier store to the same array)
+            constexpr int NCG = EPTA / 2;
+            constexpr bool kPipe = Z == 4 ? (S <= 4 || !PULSE) : (!PULSE && S <= 3);
+            C v[Z];
+            if constexpr (kPipe) {
+#pragma unroll
+                for (int n = 0; n < Z; ++n) v[n] = lds_c(smem, kb0[n]);
+            }
 #pragma unroll
             for (int m = 0; m < EPTA; ++m) {
-                constexpr int NCG = EPTA / 2;
                 const int sg = m / NCG;
                 const int co = (m % NCG) * W * 8;  // immediate column offset
                 float ax = 0.0f, ay = 0.0f;
 #pragma unroll
                 for (int n = 0; n < Z; ++n) {
-                    const C v = lds_c(smem, (sg ? kb1[n] : kb0[n]) + co);
+                    const C vn = kPipe ? v[n] : lds_c(smem, (sg ? kb1[n] : kb0[n]) + co);
                     if constexpr (PULSE) {
-                        ax += T[n].x * v.x - T[n].y * v.y;
-                        ay += T[n].x * v.y + T[n].y * v.x;
+                        ax += T[n].x * vn.x - T[n].y * vn.y;
+                        ay += T[n].x * vn.y + T[n].y * vn.x;
                     } else {
-                        ax += tv[n] * v.x;
-                        ay += tv[n] * v.y;
+                        ax += tv[n] * vn.x;
+                        ay += tv[n] * vn.y;
                     }
+                }
+                if (kPipe && m + 1 < EPTA) {
+                    const int sn = (m + 1) / NCG, cn = ((m + 1) % NCG) * W * 8;
+#pragma unroll
+                    for (int n = 0; n < Z; ++n) v[n] = lds_c(smem, (sn ? kb1[n] : kb0[n]) + cn);
                 }
                 *reinterpret_cast<C*>(smem + (sg ? xb1 : xb0) + co) = C{ax, ay};
             }
@@ -551,16 +564,26 @@ __global__ void __launch_bounds__(1024)
     }
     __syncthreads();
 
-    auto phase_a = [&]() {
+    auto phase_a = [&]() {  // software pipelined as in herm_kernel where the registers allow
+        constexpr bool kPipe = Z == 4 || S <= 2;
+        C v[Z];
+        if constexpr (kPipe) {
+#pragma unroll
+            for (int n = 0; n < Z; ++n) v[n] = lds_c(smem, kb[n]);
+        }
 #pragma unroll
         for (int mm = 0; mm < kNCG; ++mm) {
             const int co = mm * 64;  // 8 columns of 8 bytes
             float ax = 0.0f, ay = 0.0f;
 #pragma unroll
             for (int n = 0; n < Z; ++n) {
-                const C v = lds_c(smem, kb[n] + co);
-                ax += tv[n] * v.x;
-                ay += tv[n] * v.y;
+                const C vn = kPipe ? v[n] : lds_c(smem, kb[n] + co);
+                ax += tv[n] * vn.x;
+                ay += tv[n] * vn.y;
+            }
+            if (kPipe && mm + 1 < kNCG) {
+#pragma unroll
+                for (int n = 0; n < Z; ++n) v[n] = lds_c(smem, kb[n] + co + 64);
             }
             *reinterpret_cast<C*>(smem + xb + co) = C{ax, ay};
         }
