@@ -117,16 +117,30 @@ __global__ void dense_hartree_kernel(DevSystem<Real> s, const Real* ycur, Real* 
         nd[e] = ycur[row_off<Real>(b, sg, i, L) + i] + Real(0.5);
     }
     __syncthreads();
+    if (s.coulomb_type == 2) {
+        // one wave per row of U and V, lanes along it (coalesced), both spins from the same reads:
+        // h_up,i = pdiag_i + sum_j U_ij n_dn,j + V_ij n_up,j, h_dn,i with the spins exchanged
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+        for (int i = wv; i < L; i += nwv) {
+            Real au = 0, ad = 0;
+            for (int j = lane; j < L; j += 64) {
+                const Real u = s.U[static_cast<size_t>(i) * L + j], v = s.V[static_cast<size_t>(i) * L + j];
+                au += u * nd[L + j] + v * nd[j];
+                ad += u * nd[j] + v * nd[L + j];
+            }
+            au = wave_sum(au);
+            ad = wave_sum(ad);
+            if (lane == 0) {
+                hv[(b * 2) * L + i] = s.pdiag[i] + au;
+                hv[(b * 2 + 1) * L + i] = s.pdiag[i] + ad;
+            }
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
         const int sg = e >= L ? 1 : 0, i = e - sg * L;
         Real v = s.pdiag[i];
-        if (s.coulomb_type == 1) {
-            v += s.u_diag[i] * nd[(1 - sg) * L + i];
-        } else if (s.coulomb_type == 2) {
-            Real acc = 0;
-            for (int j = 0; j < L; ++j) acc += s.U[i * L + j] * nd[(1 - sg) * L + j] + s.V[i * L + j] * nd[sg * L + j];
-            v += acc;
-        }
+        if (s.coulomb_type == 1) v += s.u_diag[i] * nd[(1 - sg) * L + i];
         hv[(b * 2 + sg) * L + i] = v;
     }
 }
