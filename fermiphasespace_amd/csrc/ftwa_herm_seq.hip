@@ -18,7 +18,7 @@
 // In FP32 the one-spin rows take 67 KB, so two workgroups share a CU and cover each other's barriers.
 // Null or on-site Coulomb interactions, no pulse, no density records, L <= 64 (FP32 64 < L <= 107: the
 // column-blocked form in ftwa_herm_wide.hip); everything else runs on the stencil, upper-triangle FP32
-// or dense kernels.  On the 8x8 workload the FP64 variant runs 1.38e7 trajectory-steps/s against 6.62e6
+// or dense kernels.  On the 8x8 workload the FP64 variant runs 1.40e7 trajectory-steps/s against 6.62e6
 // for the full-matrix stencil kernel, the FP32 variant 3.12e7.
 #include <hip/hip_runtime.h>
 
@@ -235,27 +235,60 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         constexpr int ST = decltype(stc)::value;
         constexpr int SG = decltype(sgc)::value;
         constexpr int HS = kHB + (ST & 1) * kHPar + SG * kHSpin;
-        // one slot at a time (the memory clobber keeps the next slot's loads behind this slot's
-        // arithmetic): in FP64 the reads of every slot in flight at once would not fit the registers
+        // FP64: one slot ahead -- the next slot's reads go out before this slot's arithmetic, and the
+        // memory clobber keeps the slot after that behind it (every slot in flight at once would not fit
+        // the registers; 8x8 lattice: 1.38e7 -> 1.40e7 trajectory-steps/s).  FP32 (two workgroups per
+        // CU, 128 registers): one slot at a time.
+        if constexpr (sizeof(Real) == 8) {
+            C nij{0, 0}, nji{0, 0};
+            Real nhj = 0, nhi = 0;
+            auto fetch = [&](int q) {
+                nij = ldl<C>(smem, aij[q] + kXo);
+                nji = ldl<C>(smem, aji[q] + kXo);
+                int ahq = ah[q];  // opaque: decoded here, not kept decoded across the time loop
+                asm volatile("" : "+v"(ahq));
+                nhj = ldl<Real>(smem, HS + (ahq >> 16));
+                nhi = ldl<Real>(smem, HS + (ahq & 0xffff));
+            };
+            if (w < n_off) fetch(0);
 #pragma unroll
-        for (int q = 0; q < S; ++q) {
-            if (q * W + w >= n_off) continue;
-            asm volatile("" ::: "memory");
-            const C xij = ldl<C>(smem, aij[q] + kXo);
-            const C xji = ldl<C>(smem, aji[q] + kXo);
-            int ahq = ah[q];  // opaque: decoded here, not kept decoded across the time loop
-            asm volatile("" : "+v"(ahq));
-            const Real dh = ldl<Real>(smem, HS + (ahq >> 16)) - ldl<Real>(smem, HS + (ahq & 0xffff));
-            C& yq = SG ? y1[q] : y0[q];
-            C& Aq = SG ? A1[q] : A0[q];
-            C& pq = SG ? p1[q] : p0[q];
-            const Real ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
-            const Real ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
-            // c = conj(X_ji) - X_ij + (h_j - h_i) Y_ij;  k = -i c
-            const Real cx = xji.x - xij.x + dh * ycx;
-            const Real cy = -xji.y - xij.y + dh * ycy;
-            dev::rk_update<ST>(yq.x, Aq.x, pq.x, cy, h8);
-            dev::rk_update<ST>(yq.y, Aq.y, pq.y, -cx, h8);
+            for (int q = 0; q < S; ++q) {
+                if (q * W + w >= n_off) break;  // slot rows grow with q (wave-uniform)
+                const C xij = nij, xji = nji;
+                const Real dh = nhj - nhi;
+                if (q + 1 < S && (q + 1) * W + w < n_off) fetch(q + 1);
+                asm volatile("" ::: "memory");
+                C& yq = SG ? y1[q] : y0[q];
+                C& Aq = SG ? A1[q] : A0[q];
+                C& pq = SG ? p1[q] : p0[q];
+                const Real ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
+                const Real ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
+                const Real cx = xji.x - xij.x + dh * ycx;
+                const Real cy = -xji.y - xij.y + dh * ycy;
+                dev::rk_update<ST>(yq.x, Aq.x, pq.x, cy, h8);
+                dev::rk_update<ST>(yq.y, Aq.y, pq.y, -cx, h8);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                if (q * W + w >= n_off) continue;
+                asm volatile("" ::: "memory");
+                const C xij = ldl<C>(smem, aij[q] + kXo);
+                const C xji = ldl<C>(smem, aji[q] + kXo);
+                int ahq = ah[q];  // opaque: decoded here, not kept decoded across the time loop
+                asm volatile("" : "+v"(ahq));
+                const Real dh = ldl<Real>(smem, HS + (ahq >> 16)) - ldl<Real>(smem, HS + (ahq & 0xffff));
+                C& yq = SG ? y1[q] : y0[q];
+                C& Aq = SG ? A1[q] : A0[q];
+                C& pq = SG ? p1[q] : p0[q];
+                const Real ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
+                const Real ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
+                // c = conj(X_ji) - X_ij + (h_j - h_i) Y_ij;  k = -i c
+                const Real cx = xji.x - xij.x + dh * ycx;
+                const Real cy = -xji.y - xij.y + dh * ycy;
+                dev::rk_update<ST>(yq.x, Aq.x, pq.x, cy, h8);
+                dev::rk_update<ST>(yq.y, Aq.y, pq.y, -cx, h8);
+            }
         }
         if (dslot) {  // k_ii = -2 Im X_ii
             const Real k = Real(-2) * ldl<C>(smem, da + kXo).y;
