@@ -119,7 +119,8 @@ def load(path: str = LIB_PATH):
         "fpsftwa_load_system_json": ([ctypes.c_char_p, P(i32), P(i32), P(d), P(d), P(d), P(d), P(d), P(d), P(d)],
                                      ctypes.c_int),
         "fpsftwa_load_initial_state_json": ([ctypes.c_char_p, P(i32), P(d), P(d), P(d), P(d)], ctypes.c_int),
-        "fpsftwa_ground_state": ([i32, P(d), P(d), i32, i32, i32, P(d), P(d), P(i32), P(d)], ctypes.c_int),
+        "fpsftwa_ground_state": ([i32, P(d), P(d), i32, i32, i32, P(d), P(d), P(i32), P(d), P(d), P(d)],
+                                 ctypes.c_int),
         "fpsftwa_set_bases": ([H, i32, P(d)], ctypes.c_int),
         "fpsftwa_set_recorded_element": ([H, i32, i32], ctypes.c_int),
         "fpsftwa_set_stream": ([H, ctypes.c_void_p], ctypes.c_int),
@@ -207,7 +208,7 @@ def load_initial_state_json(path: str) -> dict:
 
 def ground_state(tM, U=None, n_up=None, n_dn=None, mean_field=False) -> dict:
     """Tight-binding or mean-field eigenbasis (fpsftwa_ground_state): real [L][L] eigenvector columns,
-    ascending eigenvalues, each column's largest-magnitude entry positive.  Host only."""
+    ascending eigenvalues (energy_up / energy_dn), each column's largest-magnitude entry positive.  Host only."""
     lib = load()
     tM = _f64(tM)
     L = tM.shape[0]
@@ -215,10 +216,12 @@ def ground_state(tM, U=None, n_up=None, n_dn=None, mean_field=False) -> dict:
     n_up = int(np.ceil(L / 2)) if n_up is None else int(n_up)
     n_dn = int(np.floor(L / 2)) if n_dn is None else int(n_dn)
     ru, rd = np.zeros((L, L)), np.zeros((L, L))
+    eu, ed = np.zeros(L), np.zeros(L)
     it, res = ctypes.c_int32(0), ctypes.c_double(0)
     check(lib.fpsftwa_ground_state(L, _dptr(tM), _dptr(U), n_up, n_dn, 1 if mean_field else 0, _dptr(ru), _dptr(rd),
-                                   ctypes.byref(it), ctypes.byref(res)), "ground_state")
-    return {"rot_up": ru, "rot_dn": rd, "iterations": it.value, "residue": res.value}
+                                   ctypes.byref(it), ctypes.byref(res), _dptr(eu), _dptr(ed)), "ground_state")
+    return {"rot_up": ru, "rot_dn": rd, "energy_up": eu, "energy_dn": ed, "iterations": it.value,
+            "residue": res.value}
 
 
 def run_many(engines) -> None:

@@ -82,14 +82,17 @@ int symmetric_eigen(int n, const double* a, double* evals, double* evecs)
 }
 
 int ground_state_rotation(int L, const double* tM, const double* U, int n_up, int n_dn, bool mean_field,
-                          double* rot_up, double* rot_dn, int* iterations, double* residue)
+                          double* rot_up, double* rot_dn, int* iterations, double* residue, double* e_up,
+                          double* e_dn)
 {
-    std::vector<double> ev(L), M(static_cast<size_t>(L) * L);
+    std::vector<double> ev(L), ev_dn(L), M(static_cast<size_t>(L) * L);
     if (iterations) *iterations = 0;
     if (residue) *residue = 0.0;
     if (!mean_field) {  // src/FHSystem.cpp:911-937: eig_sym(tM), both spins share the basis
         if (symmetric_eigen(L, tM, ev.data(), rot_up) != 0) return -1;
         std::copy(rot_up, rot_up + static_cast<size_t>(L) * L, rot_dn);
+        if (e_up) std::copy(ev.begin(), ev.end(), e_up);
+        if (e_dn) std::copy(ev.begin(), ev.end(), e_dn);
         return 0;
     }
     // src/FHSystem.cpp:940-1019: Hartree fields from the other spin's density, eig_sym per spin,
@@ -115,7 +118,8 @@ int ground_state_rotation(int L, const double* tM, const double* U, int n_up, in
                 for (int j = 0; j < L; ++j) h += (U ? U[static_cast<size_t>(i) * L + j] : 0.0) * other[j];
                 M[static_cast<size_t>(i) * L + i] += h;
             }
-            if (symmetric_eigen(L, M.data(), ev.data(), spin == 0 ? rot_up : rot_dn) != 0) return -1;
+            if (symmetric_eigen(L, M.data(), spin == 0 ? ev.data() : ev_dn.data(), spin == 0 ? rot_up : rot_dn) != 0)
+                return -1;
         }
         density(rot_up, n_up, nu_new);
         density(rot_dn, n_dn, nd_new);
@@ -132,6 +136,8 @@ int ground_state_rotation(int L, const double* tM, const double* U, int n_up, in
     }
     if (iterations) *iterations = it;
     if (residue) *residue = res;
+    if (e_up) std::copy(ev.begin(), ev.end(), e_up);
+    if (e_dn) std::copy(ev_dn.begin(), ev_dn.end(), e_dn);
     return 0;
 }
 

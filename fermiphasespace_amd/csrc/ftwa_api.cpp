@@ -602,8 +602,9 @@ void pulse_gradient(const fpsftwa_handle* h, const std::vector<double>& dJda, do
                 out[0] += dJda[q];
                 continue;
             }
-            const double u = t / (h->t_max / (double)(n - 1));
-            if (u <= 0.0) {
+            const double step = h->t_max / (double)(n - 1);
+            const double u = t / step;
+            if (!(step > 0.0) || !std::isfinite(u) || u <= 0.0) {  // same rule as pulse_amplitude
                 out[0] += dJda[q];
                 continue;
             }
@@ -1208,7 +1209,8 @@ double pulse_amplitude(const PulseSpec& p, double t)
         if (n == 1) return p.knots[0];
         const double step = p.t_max / (double)(n - 1);
         const double u = t / step;
-        if (u <= 0.0) return p.knots[0];
+        // t_max == 0 puts every knot at t = 0 (u is 0/0 or infinite): the first knot holds
+        if (!(step > 0.0) || !std::isfinite(u) || u <= 0.0) return p.knots[0];
         const int k = (int)std::floor(u);
         if (k >= n - 1) return p.knots[n - 1];
         const double f = u - (double)k;

@@ -43,6 +43,49 @@ std::vector<double> fetch(fpsftwa_handle* h, int which)
     return v;
 }
 
+Value vectorJson(const std::vector<double>& v)
+{
+    Value a = Value::array();
+    for (double x : v) a.append(x);
+    return a;
+}
+
+// [n][n] row-major -> array of rows (FHSystem::writeTBGroundStateJson writes V row by row, real parts)
+Value matrixJson(const std::vector<double>& m)
+{
+    const size_t n = static_cast<size_t>(std::llround(std::sqrt(static_cast<double>(m.size()))));
+    Value a = Value::array();
+    for (size_t i = 0; i < n; ++i) {
+        Value r = Value::array();
+        for (size_t j = 0; j < n; ++j) r.append(m[i * n + j]);
+        a.append(r);
+    }
+    return a;
+}
+
+// "tbGroundState" block (src/FHSystem.cpp:1143-1168)
+Value tbGroundStateJson(const std::vector<double>& energy, const std::vector<double>& rot)
+{
+    Value o = Value::object();
+    o["energyVector"] = vectorJson(energy);
+    o["rotationMatrix"] = matrixJson(rot);
+    return o;
+}
+
+// "mfGroundState" block (src/FHSystem.cpp:1171-1230)
+Value mfGroundStateJson(const std::vector<double>& eu, const std::vector<double>& ed, const std::vector<double>& ru,
+                        const std::vector<double>& rd, int iterations, double residue)
+{
+    Value o = Value::object();
+    o["energyVectorUp"] = vectorJson(eu);
+    o["energyVectorDown"] = vectorJson(ed);
+    o["rotationMatrixUp"] = matrixJson(ru);
+    o["rotationMatrixDown"] = matrixJson(rd);
+    o["iterationsNeeded"] = iterations;
+    o["residueComputation"] = residue;
+    return o;
+}
+
 const char* engineName(int e)
 {
     switch (e) {
@@ -208,6 +251,12 @@ void FHfTWASolver::initiateSolver()
     const size_t LL = (size_t)s.L * s.L;
     std::vector<double> bases;
     basisTB_ = basisMF_ = -1;
+    // eigenbases the system computed for its initial state go into the results file as
+    // "tbGroundState" / "mfGroundState" (src/FHSolver.cpp:177-185), as do those computed below
+    gsTB_ = s.tb_rot.empty() ? Value() : tbGroundStateJson(s.tb_energy, s.tb_rot);
+    gsMF_ = s.mf_rot_up.empty() ? Value()
+                                : mfGroundStateJson(s.mf_energy_up, s.mf_energy_dn, s.mf_rot_up, s.mf_rot_dn,
+                                                    s.mf_iterations, s.mf_residue);
     auto add_basis = [&](bool mean_field) {
         std::vector<double> ru, rd;
         if (!mean_field && !s.tb_rot.empty()) {
@@ -220,13 +269,16 @@ void FHfTWASolver::initiateSolver()
                 throw std::runtime_error("particle numbers outside [0, L] for the ground-state bases");
             ru.assign(LL, 0.0);
             rd.assign(LL, 0.0);
+            std::vector<double> eu(s.L), ed(s.L);
             int it = 0;
             double res = 0.0;
             if (fpsamd::ground_state_rotation(s.L, s.tM.data(), s.U.data(), s.part_up, s.part_dn, mean_field, ru.data(),
-                                              rd.data(), &it, &res) != 0)
+                                              rd.data(), &it, &res, eu.data(), ed.data()) != 0)
                 throw std::runtime_error("eigen-decomposition of the hopping matrix did not converge");
             if (mean_field && it >= 100)
                 std::cout << "!! the MF ground state computation has not converged. Residue=" << res << std::endl;
+            if (mean_field) gsMF_ = mfGroundStateJson(eu, ed, ru, rd, it, res);
+            else gsTB_ = tbGroundStateJson(eu, ru);
         }
         bases.insert(bases.end(), ru.begin(), ru.end());
         bases.insert(bases.end(), rd.begin(), rd.end());
@@ -362,6 +414,8 @@ Value FHfTWASolver::resultsDocument() const
     Value res = json_;
     if (!res.isObject()) res = Value::object();
     res["fpsVersion"] = kFormatVersion;
+    if (!gsTB_.isNull()) res["tbGroundState"] = gsTB_;  // src/FHSolver.cpp:177-185
+    if (!gsMF_.isNull()) res["mfGroundState"] = gsMF_;
     res["simulation"] = sim_;
     res["mi355x"] = info_;
     return res;
@@ -516,7 +570,7 @@ extern "C" int fpsftwa_load_initial_state_json(const char* path, int32_t* n_site
 
 extern "C" int fpsftwa_ground_state(int32_t n_sites, const double* hopping, const double* coulomb_u, int32_t n_up,
                                     int32_t n_dn, int32_t mean_field, double* rot_up, double* rot_dn,
-                                    int32_t* iterations, double* residue)
+                                    int32_t* iterations, double* residue, double* energy_up, double* energy_dn)
 {
     if (n_sites <= 0 || !hopping || !rot_up || !rot_dn || (mean_field && !coulomb_u)) {
         fpsamd::set_last_error("fpsftwa_ground_state: null argument or n_sites <= 0");
@@ -529,7 +583,7 @@ extern "C" int fpsftwa_ground_state(int32_t n_sites, const double* hopping, cons
     int it = 0;
     double res = 0.0;
     if (fpsamd::ground_state_rotation(n_sites, hopping, coulomb_u, n_up, n_dn, mean_field != 0, rot_up, rot_dn, &it,
-                                      &res) != 0) {
+                                      &res, energy_up, energy_dn) != 0) {
         fpsamd::set_last_error("fpsftwa_ground_state: eigen-decomposition did not converge");
         return FPSFTWA_EINVAL;
     }

@@ -174,20 +174,23 @@ bool addCoreHole(const fpsjson::Value& ch, FtwaSystem& s, std::string* err)
 bool computeGroundState(FtwaSystem& s, bool mean_field, int n_up, int n_dn, std::string* err)
 {
     const int L = s.L;
-    std::vector<double> ru((size_t)L * L), rd((size_t)L * L);
+    std::vector<double> ru((size_t)L * L), rd((size_t)L * L), eu(L), ed(L);
     int it = 0;
     double res = 0.0;
     if (ground_state_rotation(L, s.tM.data(), s.U.empty() ? nullptr : s.U.data(), n_up, n_dn, mean_field, ru.data(),
-                              rd.data(), &it, &res) != 0) {
+                              rd.data(), &it, &res, eu.data(), ed.data()) != 0) {
         if (err) *err = "eigen-decomposition of the hopping matrix did not converge";
         return false;
     }
     if (!mean_field) {
         s.tb_rot = std::move(ru);
+        s.tb_energy = std::move(eu);
         return true;
     }
     s.mf_rot_up = std::move(ru);
     s.mf_rot_dn = std::move(rd);
+    s.mf_energy_up = std::move(eu);
+    s.mf_energy_dn = std::move(ed);
     s.mf_iterations = it;
     s.mf_residue = res;
     if (it >= 100)

@@ -33,7 +33,9 @@ struct FtwaSystem {
     std::vector<double> n_up, n_dn;              // [L] initial occupations
     std::vector<double> rot_up, rot_dn;          // [L][L][re,im] initial-state rotations; empty = identity
     std::vector<double> tb_rot;                  // [L][L] tight-binding eigenvectors (columns), when computed
+    std::vector<double> tb_energy;               // [L] their eigenvalues, ascending
     std::vector<double> mf_rot_up, mf_rot_dn;    // [L][L] mean-field eigenvectors per spin, when computed
+    std::vector<double> mf_energy_up, mf_energy_dn;  // [L] their eigenvalues (last iteration), ascending
     int core_hole_site = -1;                     // lattice site of the core hole (the hole is site L-1), or -1
     int part_up = 0, part_dn = 0;                // upSpinParticleNumber / downSpinParticleNumber (ground-state bases)
     int mf_iterations = 0;                       // self-consistency iterations of the mean-field state

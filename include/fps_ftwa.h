@@ -296,10 +296,12 @@ int fpsftwa_load_initial_state_json(const char* path, int32_t* n_sites, double* 
  * :940-1019: eigenvectors of hopping + diag(U n_other) per spin, iterated by plain substitution until
  * the density residue < 1e-4 L, at most 100 iterations).  rot_* are real [L][L] row-major, columns
  * the eigenvectors in ascending eigenvalue order, each column's largest-magnitude entry positive.
- * Host-only (no GPU call). */
+ * energy_up / energy_dn ([L], may be NULL) receive the eigenvalues (tbgs_E for both spins, or the last
+ * iteration's mfgs_Eu / mfgs_Ed), which the results file echoes as "energyVector" / "energyVectorUp" /
+ * "energyVectorDown" (src/FHSystem.cpp:1143-1230).  Host-only (no GPU call). */
 int fpsftwa_ground_state(int32_t n_sites, const double* hopping, const double* coulomb_u, int32_t n_up,
                          int32_t n_dn, int32_t mean_field, double* rot_up, double* rot_dn,
-                         int32_t* iterations, double* residue);
+                         int32_t* iterations, double* residue, double* energy_up, double* energy_dn);
 
 #ifdef __cplusplus
 }

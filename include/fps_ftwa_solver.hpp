@@ -59,7 +59,8 @@ public:
     void setEngine(int engine) { engine_ = engine; }
     // The "simulation" object of the results file after runSimulation().
     const fpsjson::Value& results() const { return sim_; }
-    // Whole results document (input echo + "fpsVersion" + "simulation" + "mi355x").
+    // Whole results document (input echo + "fpsVersion" + "tbGroundState"/"mfGroundState" when an
+    // eigenbasis was computed + "simulation" + "mi355x").
     fpsjson::Value resultsDocument() const;
     fpsftwa_handle* handle() const { return h_; }
 
@@ -72,6 +73,7 @@ protected:
     fpsjson::Value json_ = fpsjson::Value::object();
     fpsjson::Value sim_;
     fpsjson::Value info_;
+    fpsjson::Value gsTB_, gsMF_;           // "tbGroundState" / "mfGroundState" blocks, null when not computed
     std::string workName_, folderName_;
     bool writeJson_ = false, talkative_ = false, initiated_ = false;
     double tMax_ = 1.0, dt_ = 0.01;        // include/FHSolver.h:179-180 defaults

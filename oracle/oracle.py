@@ -329,25 +329,29 @@ def gauge_columns(V: np.ndarray) -> np.ndarray:
     return V
 
 
-def ground_state(tM, U, n_up: int, n_dn: int, mean_field: bool):
+def ground_state(tM, U, n_up: int, n_dn: int, mean_field: bool, energies: bool = False):
     """Eigenbasis of the tbGroundState / mfGroundState initial states with LAPACK (numpy.linalg.eigh).
 
     tight binding: eig_sym(tM) (src/FHSystem.cpp:909-937); mean field: per spin eig_sym(tM + diag(U n_other))
     from zero densities, the lowest n_up / n_dn states occupied, plain substitution until
     |n_up' - n_up| + |n_dn' - n_dn| < 1e-4 L, at most 100 iterations (src/FHSystem.cpp:940-1019).
-    Returns (rot_up, rot_dn, iterations, residue) with columns in the gauge of gauge_columns.
+    Returns (rot_up, rot_dn, iterations, residue) with columns in the gauge of gauge_columns; with
+    energies=True also (energy_up, energy_dn), the ascending eigenvalues the results file echoes
+    (tbgs_E for both spins, or the last iteration's mfgs_Eu / mfgs_Ed; src/FHSystem.cpp:1143-1230).
     """
     tM = np.asarray(tM, dtype=np.float64)
     L = tM.shape[0]
     if not mean_field:
-        V = gauge_columns(np.linalg.eigh(tM)[1])
-        return V, V.copy(), 0, 0.0
+        E, V = np.linalg.eigh(tM)
+        V = gauge_columns(V)
+        return (V, V.copy(), 0, 0.0, E, E.copy()) if energies else (V, V.copy(), 0, 0.0)
     U = np.zeros((L, L)) if U is None else np.asarray(U, dtype=np.float64)
     nu, nd = np.zeros(L), np.zeros(L)
     it, res = 0, 0.0
     while it < 100:
-        Vu = gauge_columns(np.linalg.eigh(tM + np.diag(U @ nd))[1])
-        Vd = gauge_columns(np.linalg.eigh(tM + np.diag(U @ nu))[1])
+        Eu, Vu = np.linalg.eigh(tM + np.diag(U @ nd))
+        Ed, Vd = np.linalg.eigh(tM + np.diag(U @ nu))
+        Vu, Vd = gauge_columns(Vu), gauge_columns(Vd)
         nu_new = (Vu[:, :n_up] ** 2).sum(axis=1)
         nd_new = (Vd[:, :n_dn] ** 2).sum(axis=1)
         it += 1
@@ -355,4 +359,4 @@ def ground_state(tM, U, n_up: int, n_dn: int, mean_field: bool):
         if res < 1e-4 * L:
             break
         nu, nd = nu_new, nd_new
-    return Vu, Vd, it, res
+    return (Vu, Vd, it, res, Eu, Ed) if energies else (Vu, Vd, it, res)

@@ -231,14 +231,18 @@ def test_ground_state_matches_lapack(oracle_mod):
         g = _lib.ground_state(A)
         np.testing.assert_allclose(g["rot_up"], O.ground_state(A, None, 1, 1, False)[0], atol=1e-12)
         np.testing.assert_array_equal(g["rot_up"], g["rot_dn"])
+        np.testing.assert_allclose(g["energy_up"], np.linalg.eigvalsh(A), atol=1e-12 * np.abs(A).max())
+        np.testing.assert_array_equal(g["energy_up"], g["energy_dn"])
     for L, nu, nd in ((6, 3, 3), (6, 4, 2), (8, 5, 3), (9, 5, 4), (16, 9, 7), (30, 16, 14)):
         tM = _chain_tm(L)
         U = 4.0 * np.eye(L)
         g = _lib.ground_state(tM, U, nu, nd, mean_field=True)
-        Vu, Vd, it, res = O.ground_state(tM, U, nu, nd, True)
+        Vu, Vd, it, res, Eu, Ed = O.ground_state(tM, U, nu, nd, True, energies=True)
         assert g["iterations"] == it and abs(g["residue"] - res) < 1e-9, (L, nu, nd)
         np.testing.assert_allclose(g["rot_up"], Vu, atol=1e-10)
         np.testing.assert_allclose(g["rot_dn"], Vd, atol=1e-10)
+        np.testing.assert_allclose(g["energy_up"], Eu, atol=1e-12)
+        np.testing.assert_allclose(g["energy_dn"], Ed, atol=1e-12)
         np.testing.assert_allclose(_lib.ground_state(tM)["rot_up"], O.ground_state(tM, None, nu, nd, False)[0],
                                    atol=1e-12)
 

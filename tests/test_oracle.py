@@ -291,19 +291,25 @@ def test_ground_state_closed_forms(oracle_mod):
     O = oracle_mod
     L, t = 9, -1.0
     tM = t * (np.eye(L, k=1) + np.eye(L, k=-1))
-    V, Vd, it, res = O.ground_state(tM, None, 5, 4, False)
+    V, Vd, it, res, E, Ed = O.ground_state(tM, None, 5, 4, False, energies=True)
     k = np.arange(1, L + 1)
     closed = np.sqrt(2.0 / (L + 1)) * np.sin(np.pi * np.outer(np.arange(1, L + 1), k) / (L + 1))
     np.testing.assert_allclose(np.abs(V), np.abs(closed), atol=1e-13)
     np.testing.assert_allclose(np.diag(V.T @ tM @ V), 2 * t * np.cos(np.pi * k / (L + 1)), atol=1e-13)
+    # the echoed energies ("energyVector") are the same closed form, ascending, equal for both spins
+    np.testing.assert_allclose(E, 2 * t * np.cos(np.pi * k / (L + 1)), atol=1e-13)
+    assert np.array_equal(E, Ed)
     assert np.array_equal(V, Vd) and it == 0
     big = np.argmax(np.abs(V) >= np.abs(V).max(axis=0) / (1 + 1e-8), axis=0)  # first of the (tied) largest
     assert np.all(V[big, np.arange(L)] > 0)
     L = 10
     tM = t * (np.eye(L, k=1) + np.eye(L, k=-1))
-    Vu, Vd, it, res = O.ground_state(tM, 4.0 * np.eye(L), 5, 5, True)
+    Vu, Vd, it, res, Eu, Ed = O.ground_state(tM, 4.0 * np.eye(L), 5, 5, True, energies=True)
     assert it == 2 and res < 1e-12
     np.testing.assert_allclose(Vu, O.ground_state(tM, None, 5, 5, False)[0], atol=1e-12)
+    # n = 1/2 everywhere: the mean-field levels are the tight-binding ones shifted by U/2
+    np.testing.assert_allclose(Eu, 2 * t * np.cos(np.pi * np.arange(1, L + 1) / (L + 1)) + 2.0, atol=1e-12)
+    np.testing.assert_allclose(Ed, Eu, atol=1e-13)
     np.testing.assert_allclose((Vu[:, :5] ** 2).sum(axis=1), 0.5, atol=1e-13)
     Vu0, Vd0, _, _ = O.ground_state(tM + np.diag(np.linspace(-0.3, 0.2, L)), np.zeros((L, L)), 6, 3, True)
     np.testing.assert_allclose(Vu0, O.ground_state(tM + np.diag(np.linspace(-0.3, 0.2, L)), None, 6, 3, False)[0],
