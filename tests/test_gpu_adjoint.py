@@ -212,3 +212,77 @@ def test_torch_autograd_function():
         fd = (ag.double_occupancy(Up, tM.detach(), knots.detach(), prob)
               - ag.double_occupancy(Um, tM.detach(), knots.detach(), prob)).item() / (2 * h)
     assert abs(fd - U.grad[0].item()) <= 1e-6 * abs(U.grad).max().item()
+
+
+# ---- the 6x6 pulse workload (L = 36, 200 knots, checkpoints every 50 steps) ----------------------------
+C5_DT, C5_TMAX, C5_B, C5_SEED, C5_CK = 0.01, 1.0, 64, 505, 50
+
+
+def c5_engine(w, U=None, tM=None, knots=None):
+    eng = _lib.Engine(w["tM"] if tM is None else tM, coulomb_type=_lib.COULOMB_ONSITE,
+                      U=w["U"] if U is None else U, pulse_type=_lib.PULSE_KNOTS,
+                      knots=w["knots"] if knots is None else knots, pol=w["pol"], disp=w["disp"])
+    eng.set_initial_state(w["n_up"], w["n_dn"])
+    return eng
+
+
+def c5_oracle_J(O, w, noise, U=None, tM=None, knots=None):
+    sysO = O.System(tM=w["tM"] if tM is None else tM, coulomb_type=1, U=w["U"] if U is None else U,
+                    pulse_type=_lib.PULSE_KNOTS, pol=w["pol"], knots=w["knots"] if knots is None else knots,
+                    disp=w["disp"])
+    s = O.run(sysO, C5_DT, C5_TMAX, 1, w["n_up"], w["n_dn"], noise_arr=noise)
+    return float(O.normalise(s, noise.shape[0])["NuNd"][:, -1].mean())
+
+
+def test_adjoint_6x6_pulse_workload_vs_finite_differences(oracle_mod):
+    """The adjoint workload's own system: 6x6 periodic square lattice (L = 36), t = -1, U = 4, charge-density
+    wave, 200 knots, checkpoints every 50 steps over 100 steps (two segments), B = 64, dt = 0.01, t_max = 1.
+    The FP64 adjoint against central differences (h = 1e-5) of the oracle's J on the same samples for the
+    uniform U, the uniform hopping scale (d/d ln t), two bonds (x and y, both directions) and three knots:
+    <= 1e-6 relative to the largest entry.  The FP32 adjoint agrees with the FP64 one to 1e-4 of the largest
+    entry of each gradient."""
+    O = oracle_mod
+    w = lat.workload_adjoint6x6()
+    L = w["L"]
+    assert L == 36 and w["knots"].size == 200
+    eng = c5_engine(w)
+    eng.set_params(C5_DT, C5_TMAX, 1, C5_B, seed=C5_SEED, engine="fp64")
+    noise = eng.sample_noise(0, C5_B)
+    r64 = eng.adjoint(C5_CK)
+    eng.set_params(C5_DT, C5_TMAX, 1, C5_B, seed=C5_SEED, engine="fp32")
+    r32 = eng.adjoint(C5_CK)
+    eng.close()
+    J0 = c5_oracle_J(O, w, noise)
+    assert abs(r64["J"] - J0) <= 1e-12 * abs(J0)
+
+    h = 1e-5
+    checks = []  # (label, central difference, adjoint)
+
+    def fd(**kw_pm):
+        plus = {k: v[0] for k, v in kw_pm.items()}
+        minus = {k: v[1] for k, v in kw_pm.items()}
+        return (c5_oracle_J(O, w, noise, **plus) - c5_oracle_J(O, w, noise, **minus)) / (2 * h)
+
+    eye = np.eye(L)
+    checks.append(("U", fd(U=(w["U"] + h * eye, w["U"] - h * eye)), r64["grad_u"].sum()))
+    checks.append(("ln t", fd(tM=(w["tM"] * (1 + h), w["tM"] * (1 - h))), float((r64["grad_t"] * w["tM"]).sum())))
+    for k, l in ((0, 1), (0, 6)):  # an x bond and a y bond of site 0
+        assert w["tM"][k, l] != 0
+        e = np.zeros((L, L))
+        e[k, l] = e[l, k] = 1.0
+        checks.append((f"bond{k}-{l}", fd(tM=(w["tM"] + h * e, w["tM"] - h * e)),
+                       r64["grad_t"][k, l] + r64["grad_t"][l, k]))
+    for q in (3, 101, 198):
+        e = np.zeros(w["knots"].size)
+        e[q] = 1.0
+        checks.append((f"knot{q}", fd(knots=(w["knots"] + h * e, w["knots"] - h * e)), r64["grad_pulse"][q]))
+    scale = max(abs(c[1]) for c in checks)
+    for name, f, a in checks:
+        print(f"{name}: fd {f:+.10e} adjoint {a:+.10e} diff {abs(f - a) / scale:.2e}", flush=True)
+    for name, f, a in checks:
+        assert abs(f - a) <= 1e-6 * scale, name
+
+    assert abs(r32["J"] - r64["J"]) <= 1e-5
+    for k in ("grad_u", "grad_t", "grad_pulse"):
+        sc = np.max(np.abs(r64[k]))
+        assert np.max(np.abs(r32[k] - r64[k])) <= 1e-4 * sc, k
