@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -118,7 +119,34 @@ struct fpsftwa_handle {
     bool state_double = true;
 };
 
+// Work memory of fpsftwa_run_noalloc: system tables, occupations, the trajectory records of a time chunk
+// and their partial ensemble sums.  One copy per GPU, held until the process ends.  Host code never
+// passes the array's name to a kernel or copy (on the host it is a stand-in, not a device address): it
+// uses the per-device address that scratch_address() obtained from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[FPSFTWA_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on device dev, looked up once per device with dev current (the caller
+// has made it current).
+int scratch_address(int dev, void** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(FPSFTWA_EDEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FPSFTWA_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return FPSFTWA_OK;
+}
 
 int check_handle(fpsftwa_handle* h)
 {
@@ -127,8 +155,11 @@ int check_handle(fpsftwa_handle* h)
 }
 
 // Upload the static system in precision Real into one device allocation and fill the view.
+// dst (capacity cap bytes), when given, receives the tables instead of the handle's own buffer; *used is
+// set to the bytes taken there.
 template <typename Real>
-int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
+int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T, char* dst = nullptr, size_t cap = 0,
+                  size_t* used = nullptr)
 {
     const int L = h->L;
     // ELL neighbour lists of tM (off-diagonal non-zeros), entry-major
@@ -218,8 +249,16 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T)
                  o_tM = take(tMd.size() * sizeof(Real)), o_pdd = take(pdd.size() * sizeof(Real)),
                  o_amp = take(std::max<size_t>(amp.size(), 1) * sizeof(double)),
                  o_rows = take(std::max<size_t>(rows.size(), 1));
-    HIP_TRY(h->sys_buf.ensure(off));
-    char* base = h->sys_buf.as<char>();
+    char* base = dst;
+    if (dst) {
+        if (off > cap)
+            return fail(FPSFTWA_EINVAL, "the system tables (" + std::to_string(off) + " B) exceed the " +
+                                            std::to_string(cap) + " B left in the fixed scratch");
+        if (used) *used = off;
+    } else {
+        HIP_TRY(h->sys_buf.ensure(off));
+        base = h->sys_buf.as<char>();
+    }
     auto up = [&](size_t o, const void* src, size_t bytes) {
         return hipMemcpyAsync(base + o, src, bytes, hipMemcpyHostToDevice, h->stream);
     };
@@ -378,20 +417,24 @@ int reduce_bases(fpsftwa_handle* h, int R, int nt, int t0)
     return FPSFTWA_OK;
 }
 
+// Kernel family chosen for a batched run.
+struct KernelChoice {
+    bool herm = false, pipe = false, seq = false, wide = false, dense = false;
+    int nt = 0;  // threads per workgroup of the full-matrix stencil kernel
+};
+
 template <typename Real>
-int run_batched(fpsftwa_handle* h, EventTimer& kt)
+int choose_kernel(fpsftwa_handle* h, const DevSystem<Real>& ds, KernelChoice& k)
 {
-    DevSystem<Real> ds;
-    int rc = upload_system<Real>(h, ds, h->T);
-    if (rc) return rc;
-    int nt = h->nt_hint;
-    bool herm = false, pipe = false, seq = false, wide = false, dense = h->kernel == FPSFTWA_KERNEL_DENSE;
-    if (dense) {
+    k = KernelChoice{};
+    k.nt = h->nt_hint;
+    k.dense = h->kernel == FPSFTWA_KERNEL_DENSE;
+    if (k.dense) {
         // forced
     } else if (h->kernel == FPSFTWA_KERNEL_SEQUENTIAL) {
-        seq = herm_seq_fits<Real>(ds) && h->n_bases == 0;
-        if constexpr (std::is_same<Real, float>::value) wide = !seq && herm_wide_fits(ds) && h->n_bases == 0;
-        if (!seq && !wide)
+        k.seq = herm_seq_fits<Real>(ds) && h->n_bases == 0;
+        if constexpr (std::is_same<Real, float>::value) k.wide = !k.seq && herm_wide_fits(ds) && h->n_bases == 0;
+        if (!k.seq && !k.wide)
             return fail(FPSFTWA_EUNSUPPORTED, "the spin-sequential kernel needs L <= 64 (FP32: L <= 107), 4 or 8 "
                                               "neighbours, no pulse, null or on-site Coulomb and no basis occupations");
     } else if constexpr (std::is_same<Real, float>::value) {
@@ -399,38 +442,97 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
         // 3.12e7 trajectory-steps/s on the 8x8 workload, against 2.63e7 for the both-spin kernel), its
         // column-blocked form for 64 < L <= 107, else the both-spin upper-triangle kernel (pulses, V
         // interactions, basis occupations)
-        seq = h->kernel == FPSFTWA_KERNEL_AUTO && herm_seq_fits<float>(ds) && h->n_bases == 0;
-        wide = !seq && h->kernel == FPSFTWA_KERNEL_AUTO && herm_wide_fits(ds) && h->n_bases == 0;
-        herm = !seq && !wide && h->kernel != FPSFTWA_KERNEL_FULL && herm_fits(h->L, ds.z);
-        if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !herm)
+        k.seq = h->kernel == FPSFTWA_KERNEL_AUTO && herm_seq_fits<float>(ds) && h->n_bases == 0;
+        k.wide = !k.seq && h->kernel == FPSFTWA_KERNEL_AUTO && herm_wide_fits(ds) && h->n_bases == 0;
+        k.herm = !k.seq && !k.wide && h->kernel != FPSFTWA_KERNEL_FULL && herm_fits(h->L, ds.z);
+        if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !k.herm)
             return fail(FPSFTWA_EUNSUPPORTED, "the upper-triangle kernel does not fit L=" + std::to_string(h->L));
         // the pipelined variant is selected only on request: on the 8x8 workload it measures 9 % slower
         // than herm_kernel (2.27e7 vs 2.48e7 trajectory-steps/s), its extra barrier and register
         // pressure outweighing the overlap of the two spins' phases
-        pipe = herm && h->kernel == FPSFTWA_KERNEL_PIPELINED && herm_pipe_fits(ds);
-        if (h->kernel == FPSFTWA_KERNEL_PIPELINED && !pipe)
+        k.pipe = k.herm && h->kernel == FPSFTWA_KERNEL_PIPELINED && herm_pipe_fits(ds);
+        if (h->kernel == FPSFTWA_KERNEL_PIPELINED && !k.pipe)
             return fail(FPSFTWA_EUNSUPPORTED, "the pipelined kernel needs L <= 64, no pulse and null or on-site "
                                               "Coulomb interactions");
     } else if (h->kernel == FPSFTWA_KERNEL_PIPELINED) {
         return fail(FPSFTWA_EUNSUPPORTED, "the spin-pipelined kernel is FP32 only");
     } else {
         // FP64: the upper-triangle kernel where it applies (no pulse, null or on-site Coulomb, L <= 64)
-        herm = h->kernel != FPSFTWA_KERNEL_FULL && herm_seq_fits<double>(ds) && h->n_bases == 0;
-        if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !herm)
+        k.herm = h->kernel != FPSFTWA_KERNEL_FULL && herm_seq_fits<double>(ds) && h->n_bases == 0;
+        if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !k.herm)
             return fail(FPSFTWA_EUNSUPPORTED, "the FP64 upper-triangle kernel needs L <= 64, 4 or 8 neighbours, "
                                               "no pulse, null or on-site Coulomb and no basis occupations");
     }
-    if (!dense && !herm && !seq && !wide && !stencil_fits<Real>(h->L, ds.z, &nt)) {
+    if (!k.dense && !k.herm && !k.seq && !k.wide && !stencil_fits<Real>(h->L, ds.z, &k.nt)) {
         if (h->kernel == FPSFTWA_KERNEL_FULL)
             return fail(FPSFTWA_EUNSUPPORTED, "the full-matrix stencil kernel does not support L=" +
                                                   std::to_string(h->L) + " with " + std::to_string(ds.z) +
                                                   " neighbours per site");
-        dense = true;  // large or long-range lattices: one GEMM per stage for the batch
+        k.dense = true;  // large or long-range lattices: one GEMM per stage for the batch
     }
-    if (dense && h->L > 1024) return fail(FPSFTWA_EUNSUPPORTED, "the batched engines support L <= 1024");
-    h->used_herm = herm;
-    h->used_dense = dense;
-    if (dense && !h->dense) {
+    if (k.dense && h->L > 1024) return fail(FPSFTWA_EUNSUPPORTED, "the batched engines support L <= 1024");
+    return FPSFTWA_OK;
+}
+
+RunShape run_shape(const fpsftwa_handle* h, int R, int dens)
+{
+    RunShape rs;
+    rs.B = h->B;
+    rs.T = h->T;
+    rs.nsub = h->nsub;
+    rs.h = h->dt / h->nsub;
+    rs.R = R;
+    rs.dens = dens;
+    rs.ci = h->rec_i;
+    rs.cj = h->rec_j < 0 ? h->L - 1 : h->rec_j;
+    return rs;
+}
+
+// The time grid in launches of Tc time points: integrator, then the records' ensemble reduction into
+// acc [T][Q][2] (and the basis occupations).  kt (may be null) brackets the integrator launches.
+template <typename Real>
+int integrate_chunks(fpsftwa_handle* h, const DevSystem<Real>& ds, const KernelChoice& k, const RunShape& rs, int Tc,
+                     Real* state, Real* rec, double* part, double* acc, EventTimer* kt)
+{
+    const int L = h->L;
+    const int nchunk = reduce_chunks(h->B);
+    for (int t0 = 0; t0 < h->T; t0 += Tc) {
+        const int t1 = std::min(h->T, t0 + Tc);
+        if (kt) HIP_TRY(kt->mark(h->stream));
+        if (k.dense) {
+            HIP_TRY(dense_run<Real>(h->dense, ds, rs, h->dense_buf.as<Real>(), state, rec, t0, t1, t0 == 0,
+                                    t1 == h->T));
+        } else if (k.seq) {
+            HIP_TRY(launch_herm_seq<Real>(ds, rs, t0, t1, state, rec, h->nt_hint, h->stream));
+        } else if constexpr (std::is_same<Real, float>::value) {
+            if (k.wide) HIP_TRY(launch_herm_wide(ds, rs, t0, t1, state, rec, h->stream));
+            else if (k.pipe) HIP_TRY(launch_herm_pipe(ds, rs, t0, t1, state, rec, h->stream));
+            else if (k.herm) HIP_TRY(launch_herm(ds, rs, t0, t1, state, rec, h->nt_hint, h->stream));
+            else HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, state, rec, k.nt, h->stream));
+        } else if (k.herm) {
+            HIP_TRY(launch_herm_seq<double>(ds, rs, t0, t1, state, rec, h->nt_hint, h->stream));
+        } else {
+            HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, state, rec, k.nt, h->stream));
+        }
+        if (kt) HIP_TRY(kt->mark(h->stream));
+        ++h->launches;
+        HIP_TRY(launch_reduce<Real>(L, h->B, rs.R, t1 - t0, t0, rec, part, nchunk, acc, h->stream));
+        if (int rc = reduce_bases<Real>(h, rs.R, t1 - t0, t0)) return rc;
+    }
+    return FPSFTWA_OK;
+}
+
+template <typename Real>
+int run_batched(fpsftwa_handle* h, EventTimer& kt)
+{
+    DevSystem<Real> ds;
+    int rc = upload_system<Real>(h, ds, h->T);
+    if (rc) return rc;
+    KernelChoice k;
+    if ((rc = choose_kernel<Real>(h, ds, k))) return rc;
+    h->used_herm = k.herm;
+    h->used_dense = k.dense;
+    if (k.dense && !h->dense) {
         h->dense = dense_create(h->stream);
         if (!h->dense) return fail(FPSFTWA_EDEVICE, "rocBLAS handle creation failed");
     }
@@ -452,42 +554,68 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     HIP_TRY(h->rec.ensure((size_t)Tc * h->B * R * sizeof(Real)));
     HIP_TRY(h->part.ensure((size_t)Tc * nchunk * Q * 2 * sizeof(double)));
     if ((rc = prepare_bases(h, Tc))) return rc;
-    RunShape rs;
-    rs.B = h->B;
-    rs.T = h->T;
-    rs.nsub = h->nsub;
-    rs.h = h->dt / h->nsub;
-    rs.R = R;
-    rs.dens = dens;
-    rs.ci = h->rec_i;
-    rs.cj = h->rec_j < 0 ? L - 1 : h->rec_j;
-    if (dense) HIP_TRY(h->dense_buf.ensure(dense_buffer_elems<Real>(L, h->B) * sizeof(Real)));
-    for (int t0 = 0; t0 < h->T; t0 += Tc) {
-        const int t1 = std::min(h->T, t0 + Tc);
-        HIP_TRY(kt.mark(h->stream));
-        if (dense) {
-            HIP_TRY(dense_run<Real>(h->dense, ds, rs, h->dense_buf.as<Real>(), h->state.as<Real>(), h->rec.as<Real>(),
-                                    t0, t1, t0 == 0, t1 == h->T));
-        } else if (seq) {
-            HIP_TRY(launch_herm_seq<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), h->nt_hint,
-                                          h->stream));
-        } else if constexpr (std::is_same<Real, float>::value) {
-            if (wide) HIP_TRY(launch_herm_wide(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->stream));
-            else if (pipe) HIP_TRY(launch_herm_pipe(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->stream));
-            else if (herm) HIP_TRY(launch_herm(ds, rs, t0, t1, h->state.as<float>(), h->rec.as<float>(), h->nt_hint, h->stream));
-            else HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
-        } else if (herm) {
-            HIP_TRY(launch_herm_seq<double>(ds, rs, t0, t1, h->state.as<double>(), h->rec.as<double>(), h->nt_hint,
-                                            h->stream));
-        } else {
-            HIP_TRY(launch_stencil<Real>(ds, rs, t0, t1, h->state.as<Real>(), h->rec.as<Real>(), nt, h->stream));
-        }
-        HIP_TRY(kt.mark(h->stream));
-        ++h->launches;
-        HIP_TRY(launch_reduce<Real>(L, h->B, R, t1 - t0, t0, h->rec.as<Real>(), h->part.as<double>(), nchunk,
-                                    h->acc_d.as<double>(), h->stream));
-        if ((rc = reduce_bases<Real>(h, R, t1 - t0, t0))) return rc;
+    const RunShape rs = run_shape(h, R, dens);
+    if (k.dense) HIP_TRY(h->dense_buf.ensure(dense_buffer_elems<Real>(L, h->B) * sizeof(Real)));
+    return integrate_chunks<Real>(h, ds, k, rs, Tc, h->state.as<Real>(), h->rec.as<Real>(), h->part.as<double>(),
+                                  h->acc_d.as<double>(), &kt);
+}
+
+// fpsftwa_run_noalloc: the batched run with the caller's state and sum buffers and the fixed scratch.
+template <typename Real>
+int run_noalloc(fpsftwa_handle* h, void* state, size_t state_bytes, double* sums, size_t n_sums, char* scr)
+{
+    const int L = h->L;
+    const int R = 2 * L + 8, Q = 6 * L + 12;
+    const size_t cap = FPSFTWA_NOALLOC_SCRATCH_BYTES;
+    auto r256 = [](size_t b) { return (b + 255) / 256 * 256; };
+    // one time point of records and partial sums must fit beside the occupations and the system tables
+    const int nchunk = reduce_chunks(h->B);
+    const size_t rec_tp = (size_t)h->B * R * sizeof(Real), part_tp = (size_t)nchunk * Q * 2 * sizeof(double);
+    const size_t occ_bytes = r256(2 * (size_t)L * sizeof(double));
+    if (occ_bytes + r256(rec_tp) + r256(part_tp) > cap)
+        return fail(FPSFTWA_EINVAL, "n_traj = " + std::to_string(h->B) + " needs " + std::to_string(rec_tp + part_tp) +
+                                        " B of records per time point; the fixed scratch holds " + std::to_string(cap) +
+                                        " B (fpsftwa_run has no such limit)");
+    if (n_sums < (size_t)h->T * Q * 2)
+        return fail(FPSFTWA_EINVAL, "sums holds " + std::to_string(n_sums) + " doubles; the run needs T*(6L+12)*2 = " +
+                                        std::to_string((size_t)h->T * Q * 2));
+    const size_t need_state = (size_t)h->B * 2 * L * L * 2 * sizeof(Real);
+    if (state_bytes < need_state)
+        return fail(FPSFTWA_EINVAL, "state holds " + std::to_string(state_bytes) + " B; n_traj x 2 x L x L complex " +
+                                        (sizeof(Real) == 4 ? "FP32" : "FP64") + " needs " + std::to_string(need_state));
+    std::vector<double> occ(2 * (size_t)L);
+    for (int i = 0; i < L; ++i) {
+        occ[i] = h->n_up[i];
+        occ[L + i] = h->n_dn[i];
     }
+    HIP_TRY(hipMemcpyAsync(scr, occ.data(), occ.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));  // occ is a local
+    size_t off = occ_bytes, used = 0;
+    DevSystem<Real> ds;
+    int rc = upload_system<Real>(h, ds, h->T, scr + off, cap - off, &used);
+    if (rc) return rc;
+    off += r256(used);
+    KernelChoice k;
+    if ((rc = choose_kernel<Real>(h, ds, k))) return rc;
+    if (k.dense)
+        return fail(FPSFTWA_EUNSUPPORTED, "the dense path (L > 64 or long-range hopping) runs on a GEMM library "
+                                          "that allocates its workspace: use fpsftwa_run");
+    h->used_herm = k.herm;
+    h->used_dense = false;
+    // time points per launch: as many as the rest of the scratch holds (at most 64, or the tuned chunk)
+    int Tc = h->time_chunk > 0 ? h->time_chunk : 64;
+    Tc = std::min(Tc, h->T);
+    while (Tc > 1 && off + r256((size_t)Tc * rec_tp) + r256((size_t)Tc * part_tp) > cap) --Tc;
+    if (off + r256((size_t)Tc * rec_tp) + r256((size_t)Tc * part_tp) > cap)
+        return fail(FPSFTWA_EINVAL, "the records of one time point do not fit the fixed scratch beside the system "
+                                    "tables");
+    Real* rec = reinterpret_cast<Real*>(scr + off);
+    double* part = reinterpret_cast<double*>(scr + off + r256((size_t)Tc * rec_tp));
+    const double* occ_d = reinterpret_cast<const double*>(scr);
+    HIP_TRY(launch_sample<Real>(ds, occ_d, occ_d + L, nullptr, h->seed, 0, h->B, static_cast<Real*>(state), h->stream));
+    const RunShape rs = run_shape(h, R, 0);
+    if ((rc = integrate_chunks<Real>(h, ds, k, rs, Tc, static_cast<Real*>(state), rec, part, sums, nullptr))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return FPSFTWA_OK;
 }
 
@@ -758,6 +886,8 @@ int fpsftwa_create(const fpsftwa_system* sys, fpsftwa_handle** out)
     std::string arch;
     if (!is_gfx950(dev, &arch))
         return fail(FPSFTWA_EDEVICE, "device " + std::to_string(dev) + " is " + arch + ", not gfx950");
+    void* scr = nullptr;  // the no-allocation entry point's scratch on this device (looked up once)
+    if (int rc = scratch_address(dev, &scr)) return rc;
     auto* h = new (std::nothrow) fpsftwa_handle();
     if (!h) return fail(FPSFTWA_ENOMEM, "host allocation failed");
     h->device = dev;
@@ -1002,6 +1132,31 @@ int fpsftwa_run(fpsftwa_handle* h)
     h->method_run = h->method;
     h->has_result = true;
     return FPSFTWA_OK;
+}
+
+int fpsftwa_run_noalloc(fpsftwa_handle* h, void* state, size_t state_bytes, double* sums, size_t n_sums)
+{
+    if (int rc = check_handle(h)) return rc;
+    if (!state || !sums) return fail(FPSFTWA_EINVAL, "state and sums must be device buffers of the caller");
+    HIP_TRY(hipSetDevice(h->device));
+    void* scr = nullptr;
+    if (int rc = scratch_address(h->device, &scr)) return rc;
+    if (h->engine == FPSFTWA_ENGINE_FP64_PLAIN)
+        return fail(FPSFTWA_EUNSUPPORTED, "fpsftwa_run_noalloc runs the batched engines (FP32, FP64) only");
+    if (h->method != FPSFTWA_METHOD_FTWA)
+        return fail(FPSFTWA_EUNSUPPORTED, "fpsftwa_run_noalloc runs the fTWA method only");
+    if (!h->noise.empty() || !h->rot_up.empty() || h->n_bases > 0)
+        return fail(FPSFTWA_EUNSUPPORTED, "fpsftwa_run_noalloc takes the generator's noise, diagonal initial states "
+                                          "(no rotations) and no basis occupations");
+    h->has_result = false;
+    h->T = (int)std::floor(h->t_max / h->dt + 1.0);  // src/FHSolver.cpp:100
+    h->launches = 0;
+    h->kernel_ms = h->run_ms = 0.0;
+    int rc = h->engine == FPSFTWA_ENGINE_FP32_BATCHED
+                 ? run_noalloc<float>(h, state, state_bytes, sums, n_sums, static_cast<char*>(scr))
+                 : run_noalloc<double>(h, state, state_bytes, sums, n_sums, static_cast<char*>(scr));
+    if (rc) (void)hipStreamSynchronize(h->stream);
+    return rc;
 }
 
 int fpsftwa_run_adjoint(fpsftwa_handle* h, const fpsftwa_adjoint_spec* spec, double* objective, double* grad_u,

@@ -249,6 +249,29 @@ int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t* kernel, in
  * Returns the first failure (its message names the system). */
 int fpsftwa_run_many(fpsftwa_handle** handles, int32_t n);
 
+/* No counterpart in the reference: fpsftwa_run for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It makes no device
+ * allocation: the caller passes the trajectory state and the ensemble sums as device buffers, and the
+ * library's work memory (system tables, occupations, the trajectory records of a time chunk and their
+ * partial sums) is a fixed device array of FPSFTWA_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends.  Same samples, kernels and sums as fpsftwa_run with the same handle settings.
+ *   state  device, >= n_traj * 2 * L * L complex of the engine's precision (FP32 engine: float, FP64
+ *          batched: double; interleaved re, im; [traj][spin][i][j]), state_bytes its size: receives the
+ *          sampled initial states and holds the final states on return;
+ *   sums   device, n_sums >= T * (6L + 12) * 2 doubles, T = floor(t_max/dt + 1): the ensemble sums per
+ *          time point as (sum, compensation) pairs [T][Q][2], Q = 6L + 12; for site i, q = 6i + m with
+ *          m = 0..5: sum rho_up_ii, sum rho_dn_ii, sum |rho_up_ii|^2, sum |rho_dn_ii|^2,
+ *          sum rho_up_ii rho_dn_ii, sum ((rho_up_ii + 1/2)(rho_dn_ii + 1/2))^2; q = 6L + k (k < 4): sum E_k
+ *          (tH, U, V, pe);
+ *          q = 6L + 4 + k: sum E_k^2; q = 6L + 8 + c: the recorded element's sums (re, im up; re, im down).
+ *          Every entry is written (nothing needs clearing); fpsftwa_get_result does not see this run.
+ * Limits: the batched FP32 and FP64 engines on the stencil-sized kernels (not the dense path), the fTWA
+ * method, the generator's noise, diagonal initial states, no basis occupations, and one time point's
+ * records n_traj * (2L + 8) reals plus their partial sums within the scratch (FPSFTWA_EINVAL beyond).
+ * Synchronous.  Calls that overlap on one GPU share its scratch and race: serialise them. */
+#define FPSFTWA_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int fpsftwa_run_noalloc(fpsftwa_handle* h, void* state, size_t state_bytes, double* sums, size_t n_sums);
+
 /* Discrete adjoint of a run (no reference counterpart; it differentiates what runSimulation
  * computes, src/FHfTWASolver.cpp:342-426 with the derivative :568-600 and the step :629-661).
  * Objective: the ensemble double occupancy at the last time point,
