@@ -334,12 +334,15 @@ int prepare_initial(fpsftwa_handle* h)
     return FPSFTWA_OK;
 }
 
+// The state buffer, and the sampled (and rotated) initial states in it unless the integrator samples
+// them in its first launch (in_kernel).
 template <typename Real>
-int sample_states(fpsftwa_handle* h, const DevSystem<Real>& ds)
+int sample_states(fpsftwa_handle* h, const DevSystem<Real>& ds, bool in_kernel = false)
 {
     const int L = h->L;
     const size_t NE = 2 * (size_t)L * L;
     HIP_TRY(h->state.ensure((size_t)h->B * NE * 2 * sizeof(Real)));
+    if (in_kernel) return FPSFTWA_OK;
     const double* occ = h->occ_d.as<double>();
     const double* noise = h->noise.empty() ? nullptr : h->noise_d.as<double>();
     if (h->method == FPSFTWA_METHOD_HF || h->method == FPSFTWA_METHOD_GPSR) {  // the state itself: zero noise
@@ -421,7 +424,18 @@ int reduce_bases(fpsftwa_handle* h, int R, int nt, int t0)
 struct KernelChoice {
     bool herm = false, pipe = false, seq = false, wide = false, dense = false;
     int nt = 0;  // threads per workgroup of the full-matrix stencil kernel
+    // the spin-sequential kernel (FP32 seq; FP64 herm runs it too), which can sample the initial state
+    bool spin_sequential(bool fp64) const { return seq || (fp64 && herm); }
 };
+
+// Whether the run's first integrator launch samples the initial states itself: fTWA with the
+// generator's noise and diagonal initial states on the spin-sequential kernel.
+template <typename Real>
+bool samples_in_kernel(const fpsftwa_handle* h, const KernelChoice& k)
+{
+    return k.spin_sequential(std::is_same<Real, double>::value) && h->method == FPSFTWA_METHOD_FTWA &&
+           h->noise.empty() && h->rot_up.empty();
+}
 
 template <typename Real>
 int choose_kernel(fpsftwa_handle* h, const DevSystem<Real>& ds, KernelChoice& k)
@@ -485,6 +499,7 @@ RunShape run_shape(const fpsftwa_handle* h, int R, int dens)
     rs.dens = dens;
     rs.ci = h->rec_i;
     rs.cj = h->rec_j < 0 ? h->L - 1 : h->rec_j;
+    rs.seed = h->seed;
     return rs;
 }
 
@@ -536,7 +551,8 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
         h->dense = dense_create(h->stream);
         if (!h->dense) return fail(FPSFTWA_EDEVICE, "rocBLAS handle creation failed");
     }
-    if ((rc = sample_states<Real>(h, ds))) return rc;
+    const bool fused = samples_in_kernel<Real>(h, k);
+    if ((rc = sample_states<Real>(h, ds, fused))) return rc;
     const int L = h->L;
     const int dens = h->n_bases > 0 ? 1 : 0;
     const int R = 2 * L + 8 + dens * L * (L + 1);
@@ -549,12 +565,15 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
         const double per_tp = (double)h->B * R * sizeof(Real);
         Tc = (int)std::max(1.0, std::min(64.0, (dens ? 8e9 : 1.5e9) / per_tp));
     }
+    // buffers for a whole chunk even when this run is shorter, so a longer run on the handle reuses them
+    const int Tc_alloc = Tc;
     Tc = std::min(Tc, h->T);
     const int nchunk = reduce_chunks(h->B);
-    HIP_TRY(h->rec.ensure((size_t)Tc * h->B * R * sizeof(Real)));
-    HIP_TRY(h->part.ensure((size_t)Tc * nchunk * Q * 2 * sizeof(double)));
+    HIP_TRY(h->rec.ensure((size_t)Tc_alloc * h->B * R * sizeof(Real)));
+    HIP_TRY(h->part.ensure((size_t)Tc_alloc * nchunk * Q * 2 * sizeof(double)));
     if ((rc = prepare_bases(h, Tc))) return rc;
-    const RunShape rs = run_shape(h, R, dens);
+    RunShape rs = run_shape(h, R, dens);
+    if (fused) rs.sample_occ = h->occ_d.as<double>();
     if (k.dense) HIP_TRY(h->dense_buf.ensure(dense_buffer_elems<Real>(L, h->B) * sizeof(Real)));
     return integrate_chunks<Real>(h, ds, k, rs, Tc, h->state.as<Real>(), h->rec.as<Real>(), h->part.as<double>(),
                                   h->acc_d.as<double>(), &kt);
@@ -612,8 +631,10 @@ int run_noalloc(fpsftwa_handle* h, void* state, size_t state_bytes, double* sums
     Real* rec = reinterpret_cast<Real*>(scr + off);
     double* part = reinterpret_cast<double*>(scr + off + r256((size_t)Tc * rec_tp));
     const double* occ_d = reinterpret_cast<const double*>(scr);
-    HIP_TRY(launch_sample<Real>(ds, occ_d, occ_d + L, nullptr, h->seed, 0, h->B, static_cast<Real*>(state), h->stream));
-    const RunShape rs = run_shape(h, R, 0);
+    RunShape rs = run_shape(h, R, 0);
+    if (samples_in_kernel<Real>(h, k)) rs.sample_occ = occ_d;
+    else HIP_TRY(launch_sample<Real>(ds, occ_d, occ_d + L, nullptr, h->seed, 0, h->B, static_cast<Real*>(state),
+                                     h->stream));
     if ((rc = integrate_chunks<Real>(h, ds, k, rs, Tc, static_cast<Real*>(state), rec, part, sums, nullptr))) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     return FPSFTWA_OK;
@@ -1096,7 +1117,8 @@ int fpsftwa_run(fpsftwa_handle* h)
     const int Q = 6 * h->L + 12;
     h->launches = 0;
     h->kernel_ms = 0.0;
-    HIP_TRY(h->acc_d.ensure((size_t)h->T * Q * 2 * sizeof(double)));
+    // sized in steps of 1,024 time points, so a somewhat longer run on the handle reuses the buffer
+    HIP_TRY(h->acc_d.ensure((size_t)((h->T + 1023) / 1024 * 1024) * Q * 2 * sizeof(double)));
     HIP_TRY(hipMemsetAsync(h->acc_d.p, 0, (size_t)h->T * Q * 2 * sizeof(double), h->stream));
     int rc = prepare_initial(h);
     if (rc) return rc;

@@ -57,6 +57,32 @@ __device__ __forceinline__ void box_muller(uint32_t c[4], uint64_t seed, double&
     im = r * sn;
 }
 
+// Index of the pair (a, c), a < c, among the strictly upper entries of an L x L matrix, row by row.
+__device__ __forceinline__ int64_t pair_index(int L, int a, int c)
+{
+    return static_cast<int64_t>(a) * (2 * L - a - 1) / 2 + (c - a - 1);
+}
+
+// Complex noise value for (trajectory, spin, pair) of the initial state: counter (pair, spin, traj lo,
+// traj hi), key seed.
+__device__ __forceinline__ void initial_noise(uint64_t seed, int64_t traj, int spin, uint32_t pair, double& re,
+                                              double& im)
+{
+    uint32_t c[4] = {pair, static_cast<uint32_t>(spin), static_cast<uint32_t>(static_cast<uint64_t>(traj)),
+                     static_cast<uint32_t>(static_cast<uint64_t>(traj) >> 32)};
+    box_muller(c, seed, re, im);
+}
+
+// Sampled element rho_sigma(a, c), a < c, of the initial state before any rotation
+// (src/FHfTWASolver.cpp:342-362): S_ac w with S_ac = sqrt((n_a + n_c - 2 n_a n_c) / 2) and w the complex
+// normal (wr, wi); rho_sigma(c, a) is its conjugate.  The one formula every sampling path uses.
+__device__ __forceinline__ void sampled_offdiag(double na, double nc, double wr, double wi, double& re, double& im)
+{
+    const double sij = sqrt((na + nc - 2.0 * na * nc) / 2.0);
+    re = sij * wr;
+    im = sij * wi;
+}
+
 // GPSR diffusion noise xi_k(p) (k = 0, 1) of RK step `step` of trajectory `traj`, unscaled.  Bit 31 of
 // the second counter word keeps this stream apart from the initial-state noise (words 0 and 1 there).
 __device__ __forceinline__ void gpsr_noise(uint64_t seed, int64_t traj, int64_t step, int k, int p, double& re,

@@ -118,6 +118,37 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + (on ? s.nbr_k[n * L + i] : 0) * kRowB;
     }
 
+    // ---- first launch of a run: the initial state is sampled here (src/FHfTWASolver.cpp:342-362), the
+    // same FP64 values as launch_sample's, rounded to Real: spin up into the Y half of row i, spin down
+    // into its X half (free until the first phase A); the slots pick them up below ----
+    const bool sampled = t0 == 0 && r.sample_occ != nullptr;
+    if (sampled) {
+        const double* occ = r.sample_occ;
+        // the strict upper triangle folded onto rows i0 < ceil(L/2): (i0, c0) with c0 > i0 is itself, with
+        // c0 < i0 it stands for (L-1-i0, L-1-c0) (the middle row of an odd L only once), so nearly every lane
+        // draws a pair
+        const int half = (L + 1) / 2;
+        for (int e = tid; e < half * L; e += kNT) {
+            const int i0 = e / L, c0 = e - i0 * L;
+            int i = i0, c = c0;
+            if (c0 < i0 && 2 * i0 != L - 1) {
+                i = L - 1 - i0;
+                c = L - 1 - c0;
+            } else if (c0 <= i0) {
+                continue;
+            }
+            const uint32_t p = static_cast<uint32_t>(dev::pair_index(L, i, c));
+            for (int sg = 0; sg < 2; ++sg) {
+                double wr, wi, re, im;
+                dev::initial_noise(r.seed, r.b0 + b, sg, p, wr, wi);
+                dev::sampled_offdiag(occ[sg * L + i], occ[sg * L + c], wr, wi, re, im);
+                *reinterpret_cast<C*>(smem + kRows + i * kRowB + c * kCB + sg * kXo) =
+                    C{static_cast<Real>(re), static_cast<Real>(im)};
+            }
+        }
+        __syncthreads();
+    }
+
     // ---- element slots of the strict upper triangle (class order, class_slot): LDS addresses of Y_ij
     // (aij) and Y_ji (aji) in the shared rows, h offsets ah = i*8 | (j*8) << 16; padding lanes point at the
     // spare row and at h entry 64.  The diagonal is held apart, site i by thread i (< L) of wave 0: real
@@ -138,14 +169,23 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
                 aij[q] = kRows + i * kRowB + jj * kCB;
                 aji[q] = kRows + jj * kRowB + i * kCB;
                 ah[q] = i * kRB | (jj * kRB) << 16;
-                y0[q] = gs[i * L + jj];
-                y1[q] = gs[LL + i * L + jj];
+                if (sampled) {
+                    y0[q] = *reinterpret_cast<const C*>(smem + aij[q]);
+                    y1[q] = *reinterpret_cast<const C*>(smem + aij[q] + kXo);
+                } else {
+                    y0[q] = gs[i * L + jj];
+                    y1[q] = gs[LL + i * L + jj];
+                }
             }
         }
     }
     const bool dslot = tid < L;
     const int da = dslot ? kRows + tid * kRowB + tid * kCB : spare;
-    Real dy0 = dslot ? gs[tid * L + tid].x : Real(0), dy1 = dslot ? gs[LL + tid * L + tid].x : Real(0);
+    Real dy0 = 0, dy1 = 0;
+    if (dslot) {  // diagonal: n - 1/2
+        dy0 = sampled ? static_cast<Real>(r.sample_occ[tid] - 0.5) : gs[tid * L + tid].x;
+        dy1 = sampled ? static_cast<Real>(r.sample_occ[L + tid] - 0.5) : gs[LL + tid * L + tid].x;
+    }
     Real dA0 = 0, dp0 = 0, dA1 = 0, dp1 = 0;
     // ---- phase A: lane j is row i = j; wave w covers columns w + W*m.  The neighbour rows and
     // hoppings come from the LDS tables at each use (not held in registers across the time loop);

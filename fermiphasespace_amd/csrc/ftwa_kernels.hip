@@ -28,18 +28,11 @@ using dev::V2;
 using dev::Neumaier;
 using dev::wave_sum;
 
-// Complex noise value for (trajectory, spin, pair) of the initial state (dev::box_muller).
+using dev::pair_index;
 __device__ __forceinline__ void noise_value(uint64_t seed, int64_t traj, int spin, uint32_t pair, double& re,
                                             double& im)
 {
-    uint32_t c[4] = {pair, static_cast<uint32_t>(spin), static_cast<uint32_t>(static_cast<uint64_t>(traj)),
-                     static_cast<uint32_t>(static_cast<uint64_t>(traj) >> 32)};
-    dev::box_muller(c, seed, re, im);
-}
-
-__device__ __forceinline__ int64_t pair_index(int L, int i, int j)  // i < j, row-major strictly upper
-{
-    return static_cast<int64_t>(i) * (2 * L - i - 1) / 2 + (j - i - 1);
+    dev::initial_noise(seed, traj, spin, pair, re, im);
 }
 
 // ---- sampling -------------------------------------------------------------------------
@@ -75,10 +68,8 @@ __global__ void sample_kernel(int L, const double* __restrict__ n_up, const doub
             } else {
                 noise_value(seed, b0 + b, sg, static_cast<uint32_t>(p), wr, wi);
             }
-            const double sa = n[a], sc = n[c];
-            const double sij = sqrt((sa + sc - 2.0 * sa * sc) / 2.0);
-            re = sij * wr;
-            im = (i < j) ? sij * wi : -sij * wi;
+            dev::sampled_offdiag(n[a], n[c], wr, wi, re, im);
+            if (i > j) im = -im;
         }
         state[2 * e] = static_cast<Real>(re);
         state[2 * e + 1] = static_cast<Real>(im);
@@ -415,9 +406,7 @@ __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, int i
         if (slot < L) {
             Neumaier su, sd, suu, sdd, sud, snn;
             if (im_off < 0) {
-                for (int64_t bi = bb0; bi < bb1; ++bi) {
-                    const double u = static_cast<double>(rt[bi * R + slot]);
-                    const double d = static_cast<double>(rt[bi * R + L + slot]);
+                auto add = [&](double u, double d) {
                     su.add(u);
                     sd.add(d);
                     suu.add(u * u);
@@ -425,7 +414,22 @@ __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, int i
                     sud.add(u * d);
                     const double nn = (u + 0.5) * (d + 0.5);
                     snn.add(nn * nn);
+                };
+                // rows read 8 at a time before they are added (in order): the loads overlap instead of
+                // each waiting out the memory latency behind the previous row's sums
+                constexpr int kRows8 = 8;
+                int64_t bi = bb0;
+                for (; bi + kRows8 <= bb1; bi += kRows8) {
+                    Real u8[kRows8], d8[kRows8];
+#pragma unroll
+                    for (int q = 0; q < kRows8; ++q) {
+                        u8[q] = rt[(bi + q) * R + slot];
+                        d8[q] = rt[(bi + q) * R + L + slot];
+                    }
+#pragma unroll
+                    for (int q = 0; q < kRows8; ++q) add(static_cast<double>(u8[q]), static_cast<double>(d8[q]));
                 }
+                for (; bi < bb1; ++bi) add(static_cast<double>(rt[bi * R + slot]), static_cast<double>(rt[bi * R + L + slot]));
             } else {
                 for (int64_t bi = bb0; bi < bb1; ++bi) {
                     const double u = static_cast<double>(rt[bi * R + slot]);
@@ -449,7 +453,20 @@ __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, int i
         } else if (slot < L + 4) {
             const int k = slot - L;
             Neumaier se, se2;
-            for (int64_t bi = bb0; bi < bb1; ++bi) {
+            constexpr int kRows8 = 8;
+            int64_t bi = bb0;
+            for (; bi + kRows8 <= bb1; bi += kRows8) {
+                Real e8[kRows8];
+#pragma unroll
+                for (int q = 0; q < kRows8; ++q) e8[q] = rt[(bi + q) * R + 2 * L + k];
+#pragma unroll
+                for (int q = 0; q < kRows8; ++q) {
+                    const double e = static_cast<double>(e8[q]);
+                    se.add(e);
+                    se2.add(e * e);
+                }
+            }
+            for (; bi < bb1; ++bi) {
                 const double e = static_cast<double>(rt[bi * R + 2 * L + k]);
                 se.add(e);
                 se2.add(e * e);
@@ -461,7 +478,16 @@ __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, int i
         } else {
             const int k = slot - L - 4;
             Neumaier sc;
-            for (int64_t bi = bb0; bi < bb1; ++bi) sc.add(static_cast<double>(rt[bi * R + 2 * L + 4 + k]));
+            constexpr int kRows8 = 8;
+            int64_t bi = bb0;
+            for (; bi + kRows8 <= bb1; bi += kRows8) {
+                Real c8[kRows8];
+#pragma unroll
+                for (int q = 0; q < kRows8; ++q) c8[q] = rt[(bi + q) * R + 2 * L + 4 + k];
+#pragma unroll
+                for (int q = 0; q < kRows8; ++q) sc.add(static_cast<double>(c8[q]));
+            }
+            for (; bi < bb1; ++bi) sc.add(static_cast<double>(rt[bi * R + 2 * L + 4 + k]));
             po[(6 * L + 8 + k) * 2] = sc.s;
             po[(6 * L + 8 + k) * 2 + 1] = sc.c;
         }
@@ -577,8 +603,9 @@ hipError_t launch_plain(const DevSystem<double>& s, const RunShape& r, int t, do
 #if FTWA_REAL != 1
 int reduce_chunks(int64_t B)
 {
-    int64_t n = (B + 1023) / 1024;
-    if (n > 256) n = 256;
+    // chunks of 256 trajectories (at most 1024 chunks): enough workgroups to cover the records' read
+    int64_t n = (B + 255) / 256;
+    if (n > 1024) n = 1024;
     if (n < 1) n = 1;
     return static_cast<int>(n);
 }

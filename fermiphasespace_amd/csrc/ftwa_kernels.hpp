@@ -68,7 +68,10 @@ struct RunShape {
     int dens = 0;         // 1: records also carry the packed real density matrices (R = 2L + 8 + L(L+1))
     int ci = 0, cj = 0;   // the off-diagonal element rho(ci, cj) recorded per time point: the wholeDMatrix
                           // corner (0, L-1), or the core-hole element (site, L-1)
-    uint64_t seed = 0;    // key of the GPSR diffusion noise
+    uint64_t seed = 0;    // the run's noise key (initial-state noise; GPSR diffusion noise)
+    const double* sample_occ = nullptr;  // [2][L] occupations: when set, the launch starting at time point 0
+                                         // samples the initial state itself instead of reading it (kernels
+                                         // that support it: herm_seq; launch_sample otherwise)
 };
 
 // Sampling: rho_sigma = diag(n - 1/2) + S.W (rotation applied separately; GPSR: diag(n), no 1/2).  noise

@@ -1,5 +1,5 @@
 // json.cpp -- reader and writer for fpsjson::Value (see json.hpp).
-#include "json.hpp"
+#include "../../include/fps_json.hpp"
 
 #include <cmath>
 #include <cstdio>

@@ -130,7 +130,7 @@ FHfTWASolver::FHfTWASolver(FHSystem* fhSystem) : sys_(fhSystem)
 
 FHfTWASolver::FHfTWASolver(FHfTWASolver&& o) noexcept
     : sys_(o.sys_), h_(o.h_), json_(std::move(o.json_)), sim_(std::move(o.sim_)), info_(std::move(o.info_)),
-      workName_(std::move(o.workName_)), folderName_(std::move(o.folderName_)), writeJson_(o.writeJson_),
+      gsTB_(std::move(o.gsTB_)), gsMF_(std::move(o.gsMF_)), workName_(std::move(o.workName_)), folderName_(std::move(o.folderName_)), writeJson_(o.writeJson_),
       talkative_(o.talkative_), initiated_(o.initiated_), tMax_(o.tMax_), dt_(o.dt_), nSub_(o.nSub_), nTraj_(o.nTraj_),
       flags_(o.flags_), seed_(o.seed_), engine_(o.engine_), saveTB_(o.saveTB_), saveMF_(o.saveMF_),
       basisTB_(o.basisTB_), basisMF_(o.basisMF_), method_(o.method_)

@@ -1,5 +1,5 @@
 // system.cpp -- JSON -> fTWA system (see system.hpp for the reference functions restated).
-#include "system.hpp"
+#include "../../include/fps_system.hpp"
 
 #include "eigen.hpp"
 

@@ -16,8 +16,8 @@
 #include <stdexcept>
 #include <string>
 
-#include "../fermiphasespace_amd/csrc/json.hpp"
-#include "../fermiphasespace_amd/csrc/system.hpp"
+#include "fps_json.hpp"
+#include "fps_system.hpp"
 #include "fps_ftwa.h"
 
 // FHSystem(Json, folder) -- src/FHSystem.cpp:123-326 (the parts the fTWA path reads).

@@ -1,4 +1,4 @@
-// json.hpp -- a small JSON value, reader and writer for the reference's .json formats.
+// fps_json.hpp -- a small JSON value, reader and writer for the reference's .json formats (csrc/json.cpp).
 //
 // The reference uses jsoncpp 1.9.5 (include/json/json.h:96).  This is an independent,
 // minimal replacement covering what the simulation and results files need: objects with

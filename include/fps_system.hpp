@@ -1,4 +1,4 @@
-// system.hpp -- the fTWA system built from the reference's simulation JSON.
+// fps_system.hpp -- the fTWA system built from the reference's simulation JSON (csrc/system.cpp).
 //
 // Restates, for the fTWA path only, what FHSystem(Json, folder) (src/FHSystem.cpp:123-326)
 // builds: lattice positions from the .xyz file (src/FHLattice.cpp:646-710), distances with the
@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "json.hpp"
+#include "fps_json.hpp"
 
 namespace fpsamd {
 

@@ -185,6 +185,33 @@ def test_wide_lattices(oracle_mod, shape):
             engines=("fp64", "fp32"))
 
 
+@pytest.mark.parametrize("engine", ["fp32", "fp64"])
+@pytest.mark.parametrize("chunk", [0, 3])
+def test_in_kernel_sampling_matches_sampling_kernel(engine, chunk):
+    """The spin-sequential kernel samples the initial states in its first launch.  The same run fed the
+    generator's own noise through the separate sampling kernel (fpsftwa_set_noise) gives bitwise the same
+    final states and ensemble sums: 8x8 periodic lattice, 512 trajectories, whole-run and 3-time-point
+    launches."""
+    pos, box = lat.grid(8, 8)
+    tM, Um = lat.system_arrays(pos, box, True, t=-1.0, U=4.4)
+    occ = np.array(lat.checkerboard(8, 8)["IC_NuDiag"], dtype=float)
+    occ[5], occ[6] = 0.3, 0.8  # fractional occupations: S_ij takes every branch of its formula
+    eng = _lib.Engine(tM, U=Um)
+    eng.set_initial_state(occ, 1.0 - occ)
+    eng.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES | _lib.RES_WHOLE)
+    eng.set_tuning(0, chunk, "sequential")
+    out = []
+    for explicit in (False, True):
+        eng.set_params(0.01, 0.1, 1, 512, seed=31, engine=engine)
+        eng.set_noise(eng.sample_noise(0, 512) if explicit else None)
+        eng.run()
+        out.append((eng.states(), [eng.result(k) for k in ("sum_u", "sum_d", "sum2_u", "sum_ud", "sum_e", "N1hu")]))
+    eng.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+
+
 def test_wide_kernel_is_chunking_invariant():
     """The column-blocked kernel's result does not depend on the time chunking (bitwise)."""
     pos, box = lat.grid(10, 10)
@@ -655,6 +682,28 @@ def test_json_run_writes_reference_format(tmp_path):
     eng.run()
     np.testing.assert_allclose(np.asarray(s["Niiu"]), eng.result("Niiu"), rtol=0, atol=0)
     np.testing.assert_allclose(np.asarray(s["NuNd"]), eng.result("NuNd"), rtol=0, atol=0)
+
+
+def test_cli_binary_writes_the_results_file(tmp_path):
+    """The fps_mi355x binary (fps.cpp's fTWA branch) on a simulation file: exit 0, the reference's
+    results-file name next to the simulation, and the same numbers as the in-process front end (same
+    seed, bitwise)."""
+    import subprocess
+    sim, pos = lat.workload_chain10(n_traj=20, seed=5, engine="fp64")
+    sim["parameters"]["t_max"] = 0.2
+    path = lat.write_simulation(str(tmp_path / "cli"), sim, pos)
+    exe = os.path.join(ROOT, "fermiphasespace_amd", "fps_mi355x")
+    r = subprocess.run([exe, path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Program closing" in r.stdout
+    cli = json.load(open(tmp_path / "cli" / "chain10_fTWA_res.json"))
+    path2 = lat.write_simulation(str(tmp_path / "lib"), sim, pos)
+    _lib.run_json_file(path2)
+    ref = json.load(open(tmp_path / "lib" / "chain10_fTWA_res.json"))
+    assert cli["system"] == ref["system"] and cli["fpsVersion"] == ref["fpsVersion"]
+    for k in ("t", "Niiu", "Niid", "NuNd"):
+        assert cli["simulation"][k] == ref["simulation"][k], k
+    assert cli["simulation"]["E"] == ref["simulation"]["E"]
 
 
 def test_beyond_stencil_capacity_runs_dense(oracle_mod):

@@ -36,6 +36,15 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTED) == syms
 
 
+def test_cli_without_arguments():
+    """The fps_mi355x binary, as fps.cpp without a simulation file: the message and a non-zero exit, before
+    any device call."""
+    import subprocess
+    exe = os.path.join(ROOT, "fermiphasespace_amd", "fps_mi355x")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "Need a json file." in r.stdout
+
+
 def test_version_and_last_error():
     import ctypes
     lib = _lib.load()
