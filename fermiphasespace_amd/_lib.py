@@ -42,7 +42,7 @@ EXPORTED = [
     "fpsftwa_set_tuning", "fpsftwa_run_json_file", "fpsftwa_load_system_json", "fpsftwa_run_adjoint",
     "fpsftwa_autotune", "fpsftwa_run_many", "fpsftwa_load_initial_state_json", "fpsftwa_ground_state",
     "fpsftwa_set_bases", "fpsftwa_set_recorded_element", "fpsftwa_set_method", "fpsftwa_set_stream",
-    "fpsftwa_measure_peaks", "fpsftwa_run_noalloc",
+    "fpsftwa_measure_peaks", "fpsftwa_run_noalloc", "fpsftwa_set_graph",
 ]
 
 
@@ -126,6 +126,7 @@ def load(path: str = LIB_PATH):
         "fpsftwa_set_stream": ([H, ctypes.c_void_p], ctypes.c_int),
         "fpsftwa_set_method": ([H, i32], ctypes.c_int),
         "fpsftwa_run_noalloc": ([H, ctypes.c_void_p, sz, ctypes.c_void_p, sz], ctypes.c_int),
+        "fpsftwa_set_graph": ([H, i32], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -402,6 +403,10 @@ class Engine:
         check(self._lib.fpsftwa_run_adjoint(self._h, ctypes.byref(spec), ctypes.byref(J), _dptr(gu), _dptr(gt),
                                             _dptr(gp) if npulse else None, npulse), "run_adjoint")
         return {"J": J.value, "grad_u": gu, "grad_t": gt, "grad_pulse": gp[:npulse] if npulse else None}
+
+    def set_graph(self, on: bool) -> None:
+        """fpsftwa_set_graph: launch a batched run's chunks as one HIP graph."""
+        check(self._lib.fpsftwa_set_graph(self._h, 1 if on else 0), "set_graph")
 
     def run_noalloc(self, state_ptr: int, state_bytes: int, sums_ptr: int, n_sums: int) -> None:
         """fpsftwa_run_noalloc: the run into caller-owned device buffers (raw device addresses), with the

@@ -117,6 +117,10 @@ struct fpsftwa_handle {
     DevBuf bases_d, dens_part, dens_S, basis_work, basis_out;  // basis occupations
     DevBuf gnoise;                                              // GPSR diffusion noise of a time chunk
     bool state_double = true;
+    bool graph = false;           // capture a run's integrator and reduction launches in one HIP graph
+    bool graph_used = false;      // the last run launched its chunks as a graph
+    hipGraph_t graph_g = nullptr;  // the last run's graph and executable, released after the run
+    hipGraphExec_t graph_x = nullptr;
 };
 
 // Work memory of fpsftwa_run_noalloc: system tables, occupations, the trajectory records of a time chunk
@@ -575,8 +579,32 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
     RunShape rs = run_shape(h, R, dens);
     if (fused) rs.sample_occ = h->occ_d.as<double>();
     if (k.dense) HIP_TRY(h->dense_buf.ensure(dense_buffer_elems<Real>(L, h->B) * sizeof(Real)));
-    return integrate_chunks<Real>(h, ds, k, rs, Tc, h->state.as<Real>(), h->rec.as<Real>(), h->part.as<double>(),
-                                  h->acc_d.as<double>(), &kt);
+    h->graph_used = h->graph && !k.dense && h->n_bases == 0;  // the GEMM library stays out of the graph
+    if (!h->graph_used)
+        return integrate_chunks<Real>(h, ds, k, rs, Tc, h->state.as<Real>(), h->rec.as<Real>(), h->part.as<double>(),
+                                      h->acc_d.as<double>(), &kt);
+    // every integrator and reduction launch of the run captured on the handle's stream (thread-local
+    // capture: other handles capture their own streams from their own threads), then launched at once
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    rc = integrate_chunks<Real>(h, ds, k, rs, Tc, h->state.as<Real>(), h->rec.as<Real>(), h->part.as<double>(),
+                                h->acc_d.as<double>(), nullptr);
+    const hipError_t ec = hipStreamEndCapture(h->stream, &h->graph_g);
+    if (rc) return rc;
+    HIP_TRY(ec);
+    HIP_TRY(hipGraphInstantiate(&h->graph_x, h->graph_g, nullptr, nullptr, 0));
+    HIP_TRY(kt.mark(h->stream));
+    HIP_TRY(hipGraphLaunch(h->graph_x, h->stream));
+    HIP_TRY(kt.mark(h->stream));
+    return FPSFTWA_OK;
+}
+
+// The last run's graph, once its stream has finished with it.
+void release_graph(fpsftwa_handle* h)
+{
+    if (h->graph_x) (void)hipGraphExecDestroy(h->graph_x);
+    if (h->graph_g) (void)hipGraphDestroy(h->graph_g);
+    h->graph_x = nullptr;
+    h->graph_g = nullptr;
 }
 
 // fpsftwa_run_noalloc: the batched run with the caller's state and sum buffers and the fixed scratch.
@@ -794,8 +822,6 @@ int run_adjoint(fpsftwa_handle* h, int ck_every, double* objective, double* grad
     rs.nsub = h->nsub;
     rs.h = h->dt / h->nsub;
     DevBuf ck, wk, gacc, jb, out, tt;
-    HIP_TRY(ck.ensure(std::max<int64_t>(nseg, 1) * B * NE * 2 * sizeof(Real)));
-    HIP_TRY(wk.ensure(adjoint_work_bytes<Real>(L, B, K)));
     HIP_TRY(gacc.ensure((size_t)B * G * sizeof(double)));
     HIP_TRY(jb.ensure((size_t)B * sizeof(double)));
     HIP_TRY(out.ensure((size_t)(G + 1) * sizeof(double)));
@@ -807,20 +833,43 @@ int run_adjoint(fpsftwa_handle* h, int ck_every, double* objective, double* grad
         HIP_TRY(launch_adjoint_table<Real>(ds, n_stage, ttab, h->stream));
     }
     Real* state = h->state.as<Real>();
-    Real* work = wk.as<Real>();
-    Real* segbuf = work + (size_t)B * NE * 2 * 10;
-    const size_t ck_bytes = (size_t)B * NE * 2 * sizeof(Real);
     EventTimer kt;
-    HIP_TRY(kt.mark(h->stream));
-    for (int64_t s = 0; s < nseg; ++s) {
-        HIP_TRY(hipMemcpyAsync(ck.as<char>() + s * ck_bytes, state, ck_bytes, hipMemcpyDeviceToDevice, h->stream));
-        HIP_TRY(launch_adjoint_forward<Real>(ds, ttab, rs, s * K, std::min(nst, (s + 1) * K), state, work, h->stream));
+    const int want_nt = h->nt_hint == 512 || h->nt_hint == 1024 ? h->nt_hint : 0;
+    int lds_slots = 0;
+    const int lds_nt = adjoint_lds_threads<Real>(L, ds.z, &lds_slots, want_nt);
+    if (lds_nt) {
+        // LDS-resident form: one forward pass writes the checkpoints, then each segment is recomputed and
+        // reversed in one launch; states and cotangents in the kernel's slot order between launches
+        const size_t SL = (size_t)lds_nt * lds_slots * 2;  // reals per trajectory state
+        DevBuf seg, lam;
+        HIP_TRY(ck.ensure(std::max<int64_t>(nseg, 1) * B * SL * sizeof(Real)));
+        HIP_TRY(seg.ensure((size_t)B * K * SL * sizeof(Real)));
+        HIP_TRY(lam.ensure((size_t)B * SL * sizeof(Real)));
+        HIP_TRY(kt.mark(h->stream));
+        HIP_TRY(launch_adjoint_lds<Real>(ds, ttab, rs, want_nt, 0, 0, nst, nst, K, n_stage, state, ck.as<Real>(),
+                                         seg.as<Real>(), lam.as<Real>(), gacc.as<double>(), jb.as<double>(), h->stream));
+        for (int64_t s = nseg - 1; s >= 0; --s)
+            HIP_TRY(launch_adjoint_lds<Real>(ds, ttab, rs, want_nt, 1, s * K, std::min(nst, (s + 1) * K), nst, K,
+                                             n_stage, state, ck.as<Real>(), seg.as<Real>(), lam.as<Real>(),
+                                             gacc.as<double>(), jb.as<double>(), h->stream));
+    } else {
+        HIP_TRY(wk.ensure(adjoint_work_bytes<Real>(L, B, K)));
+        HIP_TRY(ck.ensure(std::max<int64_t>(nseg, 1) * B * NE * 2 * sizeof(Real)));
+        Real* work = wk.as<Real>();
+        Real* segbuf = work + (size_t)B * NE * 2 * 10;
+        const size_t ck_bytes = (size_t)B * NE * 2 * sizeof(Real);
+        HIP_TRY(kt.mark(h->stream));
+        for (int64_t s = 0; s < nseg; ++s) {
+            HIP_TRY(hipMemcpyAsync(ck.as<char>() + s * ck_bytes, state, ck_bytes, hipMemcpyDeviceToDevice, h->stream));
+            HIP_TRY(launch_adjoint_forward<Real>(ds, ttab, rs, s * K, std::min(nst, (s + 1) * K), state, work,
+                                                 h->stream));
+        }
+        HIP_TRY(launch_adjoint_terminal<Real>(ds, B, state, work, jb.as<double>(), h->stream));
+        for (int64_t s = nseg - 1; s >= 0; --s)
+            HIP_TRY(launch_adjoint_backward<Real>(ds, ttab, rs, s * K, std::min(nst, (s + 1) * K),
+                                                  reinterpret_cast<const Real*>(ck.as<char>() + s * ck_bytes), segbuf,
+                                                  work, gacc.as<double>(), n_stage, h->stream));
     }
-    HIP_TRY(launch_adjoint_terminal<Real>(ds, B, state, work, jb.as<double>(), h->stream));
-    for (int64_t s = nseg - 1; s >= 0; --s)
-        HIP_TRY(launch_adjoint_backward<Real>(ds, ttab, rs, s * K, std::min(nst, (s + 1) * K),
-                                              reinterpret_cast<const Real*>(ck.as<char>() + s * ck_bytes), segbuf,
-                                              work, gacc.as<double>(), n_stage, h->stream));
     HIP_TRY(launch_adjoint_reduce(gacc.as<double>(), B, G, out.as<double>(), h->stream));
     HIP_TRY(launch_adjoint_reduce(jb.as<double>(), B, 1, out.as<double>() + G, h->stream));
     HIP_TRY(kt.mark(h->stream));
@@ -828,7 +877,7 @@ int run_adjoint(fpsftwa_handle* h, int ck_every, double* objective, double* grad
     HIP_TRY(hipMemcpyAsync(g.data(), out.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->kernel_ms = kt.pairs_ms();
-    h->launches = 2 * nseg + 1;
+    h->launches = lds_nt ? nseg + 1 : 2 * nseg + 1;
     const double inv = 1.0 / (double)B;
     if (objective) *objective = g[G] * inv;
     if (grad_u)
@@ -1048,6 +1097,13 @@ int fpsftwa_set_method(fpsftwa_handle* h, int32_t method)
     return FPSFTWA_OK;
 }
 
+int fpsftwa_set_graph(fpsftwa_handle* h, int32_t on)
+{
+    if (int rc = check_handle(h)) return rc;
+    h->graph = on != 0;
+    return FPSFTWA_OK;
+}
+
 int fpsftwa_set_stream(fpsftwa_handle* h, void* stream)
 {
     if (int rc = check_handle(h)) return rc;
@@ -1136,6 +1192,7 @@ int fpsftwa_run(fpsftwa_handle* h)
     }
     if (rc) {
         (void)hipStreamSynchronize(h->stream);
+        release_graph(h);
         return rc;
     }
     HIP_TRY(whole.mark(h->stream));
@@ -1147,7 +1204,9 @@ int fpsftwa_run(fpsftwa_handle* h)
     if (h->n_bases)
         HIP_TRY(hipMemcpyAsync(h->basis_acc.data(), h->basis_out.p, h->basis_acc.size() * sizeof(double),
                                hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    const hipError_t es = hipStreamSynchronize(h->stream);
+    release_graph(h);
+    HIP_TRY(es);
     h->kernel_ms = kt.pairs_ms();
     h->run_ms = whole.pairs_ms();
     h->B_run = h->B;

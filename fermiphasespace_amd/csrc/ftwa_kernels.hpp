@@ -178,6 +178,16 @@ hipError_t launch_adjoint_backward(const DevSystem<Real>& s, const Real* ttab, c
                                    int64_t n1, const Real* ckpt, Real* segbuf, Real* work, double* gacc,
                                    int64_t n_stage, hipStream_t st);
 hipError_t launch_adjoint_reduce(const double* src, int64_t B, int64_t n, double* out, hipStream_t st);
+// LDS-resident form of the adjoint: threads per workgroup (0 when it does not apply: then the form above
+// runs) and slots per thread; want_nt forces a workgroup size (0: the smallest that fits).  Per-trajectory
+// buffers hold nt * slots complex values each: ckpt [nseg][B], seg [B][K], lam [B].  mode 0 runs the
+// forward pass (checkpoints every K steps, the objective into jb), mode 1 reverses the segment [n0, n1).
+template <typename Real>
+int adjoint_lds_threads(int L, int z, int* slots, int want_nt = 0);
+template <typename Real>
+hipError_t launch_adjoint_lds(const DevSystem<Real>& s, const Real* ttab, const RunShape& r, int want_nt, int mode,
+                              int64_t n0, int64_t n1, int64_t nst, int64_t K, int64_t n_stage, const Real* state,
+                              Real* ckpt, Real* seg, Real* lam, double* gacc, double* jb, hipStream_t st);
 
 // Ensemble reduction of records for time points [t0, t0+nt) into acc.  im_off >= 0: GPSR records, whose
 // imaginary diagonals start at im_off (the sums then take the complex diagonals, see reduce_partial_kernel).

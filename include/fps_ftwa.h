@@ -202,6 +202,12 @@ int fpsftwa_sample_noise(fpsftwa_handle* h, int64_t first, int64_t count, double
  * previous stream.  Runs stay synchronous: fpsftwa_run returns after its results reached the host. */
 int fpsftwa_set_stream(fpsftwa_handle* h, void* stream);
 
+/* No counterpart in the reference: with on != 0, a batched run captures its integrator and ensemble-
+ * reduction launches (every time chunk) on the handle's stream into one HIP graph and launches that graph
+ * instead of the individual kernels; runs that use the GEMM library (the dense path, basis occupations)
+ * launch directly as before.  Results are bitwise the same either way.  Off by default. */
+int fpsftwa_set_graph(fpsftwa_handle* h, int32_t on);
+
 /* FHfTWASolver::runSimulation (src/FHfTWASolver.cpp:222-556) without the file write:
  * samples, integrates every trajectory, reduces the ensemble.  Synchronous. */
 int fpsftwa_run(fpsftwa_handle* h);

@@ -212,6 +212,43 @@ def test_in_kernel_sampling_matches_sampling_kernel(engine, chunk):
         assert np.array_equal(a, b)
 
 
+def test_graph_launch_is_bitwise_neutral():
+    """fpsftwa_set_graph: the run's integrator and reduction launches (here 7 time chunks of 3 points)
+    replayed as one HIP graph give bitwise the results of direct launches, alone and in fpsftwa_run_many
+    with a system that the graph does not cover (dense path)."""
+    pos, box = lat.grid(8, 8)
+    tM, Um = lat.system_arrays(pos, box, True, t=-1.0, U=4.4)
+    occ = np.array(lat.checkerboard(8, 8)["IC_NuDiag"], dtype=float)
+    keys = ("sum_u", "sum_d", "sum2_u", "sum_ud", "sum_e", "N1hu")
+    eng = _lib.Engine(tM, U=Um)
+    eng.set_initial_state(occ, occ)
+    eng.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES | _lib.RES_WHOLE)
+    eng.set_tuning(0, 3, "auto")
+    eng.set_params(0.01, 0.2, 1, 1024, seed=17, engine="fp32")
+    out = []
+    for graph in (False, True, False):
+        eng.set_graph(graph)
+        eng.run()
+        out.append((eng.states(), [eng.result(k) for k in keys]))
+        assert eng.timing()["launches"] == 7
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0])
+        for a, b in zip(o[1], out[0][1]):
+            assert np.array_equal(a, b)
+    other = _lib.Engine(chain(80), U=np.eye(80))  # L = 80, FP64: the dense path, launched directly
+    other.set_initial_state(np.array([1.0, 0] * 40), np.array([0.0, 1] * 40))
+    other.set_params(0.02, 0.06, 1, 8, seed=3, engine="fp64")
+    other.run()
+    ref_other = other.result("sum_u")
+    eng.set_graph(True)
+    other.set_graph(True)
+    _lib.run_many([eng, other])
+    assert np.array_equal(eng.states(), out[0][0])
+    assert np.array_equal(other.result("sum_u"), ref_other)
+    eng.close()
+    other.close()
+
+
 def test_wide_kernel_is_chunking_invariant():
     """The column-blocked kernel's result does not depend on the time chunking (bitwise)."""
     pos, box = lat.grid(10, 10)
