@@ -22,8 +22,9 @@ METHOD_FTWA, METHOD_HF, METHOD_GPSR = 0, 1, 2
 ENGINE_FP64_PLAIN, ENGINE_FP64_BATCHED, ENGINE_FP32_BATCHED = 0, 1, 2
 ENGINES = {"fp64_plain": ENGINE_FP64_PLAIN, "fp64": ENGINE_FP64_BATCHED, "fp32": ENGINE_FP32_BATCHED}
 KERNEL_AUTO, KERNEL_FULL, KERNEL_HERMITIAN, KERNEL_DENSE, KERNEL_PIPELINED, KERNEL_SEQUENTIAL = 0, 1, 2, 3, 4, 5
+KERNEL_DENSE_MFMA = 6
 KERNELS = {"auto": KERNEL_AUTO, "full": KERNEL_FULL, "hermitian": KERNEL_HERMITIAN, "dense": KERNEL_DENSE,
-           "pipelined": KERNEL_PIPELINED, "sequential": KERNEL_SEQUENTIAL}
+           "pipelined": KERNEL_PIPELINED, "sequential": KERNEL_SEQUENTIAL, "dense_mfma": KERNEL_DENSE_MFMA}
 
 RES_DIAG, RES_DIAG2, RES_DOUBLONS, RES_ENERGIES, RES_WHOLE = 0x01, 0x02, 0x04, 0x08, 0x10
 
@@ -90,6 +91,13 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise FtwaError(EDEVICE, f"{path} is missing: build it with `python -m fermiphasespace_amd.build`")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 under the same soname.  Loaded first,
+    # it also serves this library; loaded after the system runtime this library links, it would be a
+    # second runtime in the process and fail to enumerate the device.  So torch, when installed, goes first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     P = ctypes.POINTER
     d, i32, i64, u32, u64, sz = (ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32,

@@ -446,7 +446,7 @@ int choose_kernel(fpsftwa_handle* h, const DevSystem<Real>& ds, KernelChoice& k)
 {
     k = KernelChoice{};
     k.nt = h->nt_hint;
-    k.dense = h->kernel == FPSFTWA_KERNEL_DENSE;
+    k.dense = h->kernel == FPSFTWA_KERNEL_DENSE || h->kernel == FPSFTWA_KERNEL_DENSE_MFMA;
     if (k.dense) {
         // forced
     } else if (h->kernel == FPSFTWA_KERNEL_SEQUENTIAL) {
@@ -555,6 +555,7 @@ int run_batched(fpsftwa_handle* h, EventTimer& kt)
         h->dense = dense_create(h->stream);
         if (!h->dense) return fail(FPSFTWA_EDEVICE, "rocBLAS handle creation failed");
     }
+    if (h->dense) dense_set_fused_mfma(h->dense, h->kernel == FPSFTWA_KERNEL_DENSE_MFMA);
     const bool fused = samples_in_kernel<Real>(h, k);
     if ((rc = sample_states<Real>(h, ds, fused))) return rc;
     const int L = h->L;
@@ -1459,7 +1460,7 @@ double pulse_amplitude(const PulseSpec& p, double t)
 extern "C" int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel)
 {
     if (int rc = check_handle(h)) return rc;
-    if (kernel < FPSFTWA_KERNEL_AUTO || kernel > FPSFTWA_KERNEL_SEQUENTIAL) return fail(FPSFTWA_EINVAL, "bad kernel");
+    if (kernel < FPSFTWA_KERNEL_AUTO || kernel > FPSFTWA_KERNEL_DENSE_MFMA) return fail(FPSFTWA_EINVAL, "bad kernel");
     h->kernel = kernel;
     if ((block_threads != 0 && block_threads != 512 && block_threads != 832 && block_threads != 1024) || time_chunk < 0)
         return fail(FPSFTWA_EINVAL, "block_threads must be 0 (automatic), 512, 832 or 1024, and time_chunk >= 0");
@@ -1511,7 +1512,10 @@ extern "C" int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t*
     else cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 1024, 0});
     for (int blk : {0, 512, 1024}) cands.push_back({FPSFTWA_KERNEL_FULL, blk, 0});
     cands.push_back({FPSFTWA_KERNEL_DENSE, 0, 0});
-    if (fp32) cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 0, 8});
+    if (fp32) {
+        cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 0, 8});
+        cands.push_back({FPSFTWA_KERNEL_DENSE_MFMA, 0, 0});  // the fused matrix-core form of the dense path
+    }
     // a shortened copy of the configured run: trial_steps steps of at most 16,384 trajectories
     const double t_max = h->t_max;
     const int64_t B = h->B;

@@ -264,6 +264,177 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// ---- fused product and update on the matrix cores (FP32) -------------------------------------------
+//
+// One workgroup per tile pair {(I, J), (J, I)}, I <= J, of one (trajectory, spin): X = Y T for both tiles
+// from v_mfma_f32_32x32x2f32, then the RK update of dense_update_kernel, without X leaving the chip.
+// Each of the 4 waves forms one real 32x32 output tile over the whole K: Re or Im of X(I, J) from rows I
+// of the stage input and columns J of the stage's hopping, or of X(J, I); a diagonal pair splits K
+// between its wave pairs instead and adds the halves in a fixed order.  With a pulse K = 2L: row i of the
+// stage input is [Y_re Y_im], and M = [[Tr, Ti], [-Ti, Tr]] (symmetric, since T is Hermitian) gives
+// Re X_ij from column j and Im X_ij from column L + j; without one K = L, M = T real symmetric, the real
+// and imaginary planes of Y times T.  Operands come straight from memory, one 16-byte load per lane and
+// operand for 4 instructions: lane l supplies k = k0 + 4 (l / 32) + m to instruction m of an 8-k block
+// (the same permutation of k for both factors leaves the product unchanged), rows and columns past L
+// read zeros.  Block b of the grid is remapped so that the pairs of one trajectory run on one XCD
+// (dispatch is round-robin over the 8 XCDs) and its stage input is served from that XCD's L2.  The new
+// stage input goes to the other buffer (other workgroups still read this one).
+typedef float fps_f32x16 __attribute__((ext_vector_type(16)));
+
+template <int ST, bool PULSE>
+__global__ void __launch_bounds__(256)
+    dense_mfma_update_kernel(int L, int64_t B, int npair, double hstep, const int2* pairs, const float* M,
+                             const float* hv, const float* yin, float* y, float* A, float* p, float* yout)
+{
+    __shared__ float xs[2][2][kTile][kTile + 1];  // [side][re/im][row][col]: X(I, J), X(J, I); then new Y
+    const int64_t nblk = B * 2 * npair;
+    int64_t blk = blockIdx.x;
+    if ((nblk & 7) == 0) blk = (blk & 7) * (nblk >> 3) + (blk >> 3);  // XCD-major: one trajectory per XCD
+    const int pair = static_cast<int>(blk % npair);
+    const int64_t bs = blk / npair;
+    const int sg = static_cast<int>(bs & 1);
+    const int64_t b = bs >> 1;
+    const int2 pr = pairs[pair];
+    const int I0 = pr.x * kTile, J0 = pr.y * kTile;
+    const bool diag = pr.x == pr.y;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int part = w & 1;   // 0: Re X, 1: Im X
+    const int side = w >> 1;  // off-diagonal: 0 tile (I, J), 1 tile (J, I); diagonal: half of K
+    const int K = PULSE ? 2 * L : L;
+    const int r0 = (diag || side == 0) ? I0 : J0;  // rows of Y
+    const int c0 = (diag || side == 0) ? J0 : I0;  // columns of X
+    const int kmid = (K / 2) & ~7;  // blocks of 8 k on both sides of the split
+    const int ka = diag && side ? kmid : 0, kb = diag && !side ? kmid : K;
+    const int arow = r0 + (lane & 31), bcol = c0 + (lane & 31);
+    const int koff = 4 * (lane >> 5);
+    const bool aon = arow < L, bon = bcol < L;
+    // A: row arow of the stage input (plane part without a pulse); B: row bcol (+ L for Im with a pulse) of M
+    const float* ap = yin + row_off<float>(b, sg, aon ? arow : 0, L) + (PULSE ? 0 : part * L) + koff;
+    const float* bp = M + static_cast<size_t>(bon ? bcol + (PULSE ? part * L : 0) : 0) * (PULSE ? 2 * L : L) + koff;
+    fps_f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    const float4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    float4 a = aon ? *reinterpret_cast<const float4*>(ap + ka) : zero4;
+    float4 bv = bon ? *reinterpret_cast<const float4*>(bp + ka) : zero4;
+    for (int k0 = ka; k0 < kb; k0 += 8) {
+        // the next block's operands in flight during this block's instructions (a ring of several blocks
+        // measured 3x slower: the conditional refills left the waits unresolved)
+        float4 an = zero4, bn = zero4;
+        if (k0 + 8 < kb) {
+            if (aon) an = *reinterpret_cast<const float4*>(ap + k0 + 8);
+            if (bon) bn = *reinterpret_cast<const float4*>(bp + k0 + 8);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, bv.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, bv.y, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, bv.z, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, bv.w, acc, 0, 0, 0);
+        a = an;
+        bv = bn;
+    }
+    // accumulator lane layout: register r holds row 8 (r / 4) + 4 (lane / 32) + r % 4, column lane % 32
+    const int col = lane & 31;
+    if (!diag || side == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) xs[diag ? 0 : side][part][8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)][col] = acc[r];
+    }
+    __syncthreads();
+    if (diag && side == 1) {  // the second half of K, added in a fixed order
+#pragma unroll
+        for (int r = 0; r < 16; ++r) xs[0][part][8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)][col] += acc[r];
+    }
+    __syncthreads();
+    const int tsrc = diag ? 0 : 1;  // tile holding X_ji, read transposed
+    const float h = static_cast<float>(hstep);
+    const float h3 = static_cast<float>(hstep / 3.0), h6 = static_cast<float>(hstep / 6.0),
+                h8 = static_cast<float>(hstep / 8.0);
+    const float* hs = hv + (b * 2 + sg) * L;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32 per pass
+    for (int r = ty; r < kTile; r += 8) {
+        const int i = I0 + r, j = J0 + tx;
+        if (i >= L || j >= L || i > j) continue;
+        const size_t o = row_off<float>(b, sg, i, L) + j;
+        const float xr = xs[0][0][r][tx], xi = xs[0][1][r][tx];       // X_ij
+        const float tr = xs[tsrc][0][tx][r], ti = xs[tsrc][1][tx][r];  // X_ji
+        const float yr = y[o], yi = y[o + L];
+        float Ar = 0, Ai = 0, pr_ = 0, pi = 0;
+        if (ST > 0) {
+            Ar = A[o];
+            Ai = A[o + L];
+            pr_ = p[o];
+            pi = p[o + L];
+        }
+        float cr, ci;  // current stage input
+        if (ST == 0) {
+            cr = yr;
+            ci = yi;
+        } else if (ST == 1) {
+            cr = yr + h3 * pr_;
+            ci = yi + h3 * pi;
+        } else if (ST == 2) {
+            cr = yr + h6 * (Ar - 3.f * pr_);
+            ci = yi + h6 * (Ai - 3.f * pi);
+        } else {
+            cr = yr + h * pr_;
+            ci = yi + h * pi;
+        }
+        // c = X_ij - conj(X_ji) + (h_j - h_i) Y_ij;  k = -i c
+        const float dh = hs[j] - hs[i];
+        const float cx = xr - tr + dh * cr, cy = xi + ti + dh * ci;
+        float kx = cy, ky = -cx;
+        if (i == j) ky = 0;  // the diagonal slope is real
+        float nr, ni;
+        if (ST == 0) {
+            Ar = kx;
+            Ai = ky;
+            pr_ = kx;
+            pi = ky;
+            nr = yr + h3 * kx;
+            ni = yi + h3 * ky;
+        } else if (ST == 1) {
+            Ar += 3.f * kx;
+            Ai += 3.f * ky;
+            pr_ -= kx;
+            pi -= ky;
+            nr = yr + h6 * (Ar - 3.f * pr_);
+            ni = yi + h6 * (Ai - 3.f * pi);
+        } else if (ST == 2) {
+            Ar += 3.f * kx;
+            Ai += 3.f * ky;
+            pr_ += kx;
+            pi += ky;
+            nr = yr + h * pr_;
+            ni = yi + h * pi;
+        } else {
+            Ar += kx;
+            Ai += ky;
+            nr = yr + h8 * Ar;
+            ni = yi + h8 * Ai;
+            y[o] = nr;
+            y[o + L] = ni;
+        }
+        if (ST < 3) {
+            A[o] = Ar;
+            A[o + L] = Ai;
+            p[o] = pr_;
+            p[o + L] = pi;
+        }
+        yout[o] = nr;
+        yout[o + L] = ni;
+        xs[0][0][r][tx] = nr;  // only this thread reads (r, tx) of X(I, J)
+        xs[0][1][r][tx] = ni;
+    }
+    __syncthreads();
+    // mirrored tile: Y_ji = conj(Y_ij) for i < j, row j = J0 + r, column i = I0 + tx
+    for (int r = ty; r < kTile; r += 8) {
+        const int jr = J0 + r, ic = I0 + tx;
+        if (jr >= L || ic >= L || ic >= jr) continue;
+        const size_t om = row_off<float>(b, sg, jr, L) + ic;
+        yout[om] = xs[0][0][tx][r];
+        yout[om + L] = -xs[0][1][tx][r];
+    }
+}
+
 // Hopping of the stage, T_ik for the ELL entries [z][L] (k = nb_n(i)): tM_ik (1 + e^{i a pd_ik}) with a
 // pulse, tM_ik without.
 template <typename Real>
@@ -699,8 +870,20 @@ rocblas_status gemm_solution(rocblas_handle hb, int solution, int m, int n, int 
 struct DenseCtx {
     rocblas_handle blas = nullptr;
     hipStream_t stream = nullptr;
-    int solution = 0;  // GEMM solution index chosen by dense_tune_gemm (0: the library's default)
+    int solution = 0;        // GEMM solution index chosen by dense_tune_gemm (0: the library's default)
+    bool fused_mfma = false;  // FP32: product and update fused on the matrix cores instead of the library GEMM
 };
+
+void dense_set_fused_mfma(DenseCtx* c, bool on)
+{
+    if (c) c->fused_mfma = on;
+}
+
+bool dense_mfma_fits(int L, bool pulse)
+{
+    // K = 2L (pulse) or L in blocks of 8, rows of the stage input and of M 16-byte aligned
+    return L >= 4 && L <= 1024 && (pulse ? L % 4 == 0 : L % 8 == 0);
+}
 
 DenseCtx* dense_create(hipStream_t st)
 {
@@ -866,6 +1049,31 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                 if (pulse)
                     hipLaunchKernelGGL(dense_hop_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)),
                                        dim3(256), 0, st, s, g, M);
+                if constexpr (std::is_same<Real, float>::value) {
+                    if (c->fused_mfma && dense_mfma_fits(L, pulse)) {
+                        // product and update fused on the matrix cores; the stage input alternates
+                        // between ycur and X (four swaps per step)
+                        const dim3 g1(static_cast<unsigned>(static_cast<int64_t>(npair) * B * 2));
+#define FPS_MFMA_STAGE(STG)                                                                                    \
+    if (pulse)                                                                                                   \
+        hipLaunchKernelGGL((dense_mfma_update_kernel<STG, true>), g1, dim3(256), 0, st, L, B, npair, r.h, pairs, \
+                           M, hv, yin, y, A, p, yout);                                                           \
+    else                                                                                                         \
+        hipLaunchKernelGGL((dense_mfma_update_kernel<STG, false>), g1, dim3(256), 0, st, L, B, npair, r.h,       \
+                           pairs, M, hv, yin, y, A, p, yout)
+                        switch (stg) {
+                            case 0: FPS_MFMA_STAGE(0); break;
+                            case 1: FPS_MFMA_STAGE(1); break;
+                            case 2: FPS_MFMA_STAGE(2); break;
+                            default: FPS_MFMA_STAGE(3); break;
+                        }
+#undef FPS_MFMA_STAGE
+                        Real* tmp = yin;
+                        yin = yout;
+                        yout = tmp;
+                        continue;
+                    }
+                }
                 rocblas_status bs;
                 if (!pulse) {
                     // rows (b, s, i, plane) of length L:  X = Z T, row-major -> column-major X^T = T^T Z^T

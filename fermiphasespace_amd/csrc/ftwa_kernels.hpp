@@ -133,12 +133,18 @@ template <typename Real>
 hipError_t launch_gpsr(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
                        Real* noise, hipStream_t st);
 
-// Dense batched path (ftwa_dense.hip): one library GEMM X = Y T per RK stage for the whole batch,
-// hand-written Hartree, commutator/RK update and record kernels.  L <= 1024, any neighbour count.
+// Dense batched path (ftwa_dense.hip): X = Y T per RK stage for the whole batch -- a library GEMM followed
+// by the hand-written commutator/RK update kernel, or (FP32, on request, where L allows) one hand-written
+// kernel fusing product and update on the matrix cores -- plus hand-written Hartree and record kernels.
+// L <= 1024, any neighbour count.
 struct DenseCtx;
 DenseCtx* dense_create(hipStream_t st);
 void dense_set_stream(DenseCtx* c, hipStream_t st);
 void dense_destroy(DenseCtx* c);
+// FP32: fuse product and update on the matrix cores (where dense_mfma_fits) instead of the library GEMM
+void dense_set_fused_mfma(DenseCtx* c, bool on);
+// whether the fused FP32 kernel runs this system (L % 4 == 0 with a pulse, L % 8 == 0 without)
+bool dense_mfma_fits(int L, bool pulse);
 // elements of Real the dense path needs for B trajectories (one buffer)
 template <typename Real>
 size_t dense_buffer_elems(int L, int64_t B);

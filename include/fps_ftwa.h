@@ -231,7 +231,7 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
 #define FPSFTWA_KERNEL_FULL 1      /* full-matrix stencil kernel                                   */
 #define FPSFTWA_KERNEL_HERMITIAN 2 /* upper-triangle kernel (L <= 64; in FP64 the spin-sequential
                                     * one, no pulse, null or on-site Coulomb)                         */
-#define FPSFTWA_KERNEL_DENSE 3     /* one GEMM per RK stage for the batch (any precision, L <= 1024);
+#define FPSFTWA_KERNEL_DENSE 3     /* X = Y T per RK stage for the batch (any precision, L <= 1024);
                                     * chosen automatically for L > 64 or more than 8 neighbours      */
 #define FPSFTWA_KERNEL_PIPELINED 4 /* upper-triangle kernel with the two spins half a stage apart
                                     * (FP32, L <= 64, no pulse, null or on-site Coulomb)              */
@@ -239,11 +239,16 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
                                     * two workgroups per CU in FP32 (L <= 64; FP32 also 64 < L <= 107
                                     * with X formed in 64-column blocks; no pulse, null or on-site
                                     * Coulomb, no basis occupations)                                  */
+#define FPSFTWA_KERNEL_DENSE_MFMA 6 /* FP32 dense path with the product X = Y T and the RK update fused in
+                                    * one hand-written matrix-core kernel per stage (v_mfma_f32_32x32x2f32;
+                                    * L % 4 == 0 with a pulse, % 8 without; else as DENSE).  The default
+                                    * DENSE form (library GEMM, then the update kernel) measures faster  */
 int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel);
 
 /* Autotuner (no reference counterpart): times every batched kernel family and shape that can run
  * this system (upper-triangle both-spin, spin-pipelined and spin-sequential, full-matrix stencil at
- * 512/1024 threads, dense; two time-chunk lengths) on a shortened copy of the configured run -- trial_steps steps of at most 16,384
+ * 512/1024 threads, dense with the library GEMM and, in FP32, fused on the matrix cores; two time-chunk
+ * lengths) on a shortened copy of the configured run -- trial_steps steps of at most 16,384
  * trajectories, each candidate run twice and the second timed -- and keeps the fastest as the
  * handle's tuning (fpsftwa_set_tuning). Outputs may be NULL. Requires a batched engine. */
 int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t* kernel, int32_t* block_threads,

@@ -76,8 +76,10 @@ def compare(O, tM, n_up, n_dn, *, coulomb_type=1, U=None, V=None, pot=None, puls
     z = int(((tM != 0) & ~np.eye(L, dtype=bool)).sum(axis=1).max())
     stencil = L <= 64 and z <= 8  # where the LDS stencil kernels apply
     for e in engines:
-        if e == "fp32":  # every FP32 kernel: full-matrix and upper-triangle (where they fit), dense GEMM
-            runs += ([("fp32", "full"), ("fp32", "hermitian")] if stencil else []) + [("fp32", "dense")]
+        if e == "fp32":  # every FP32 kernel: full-matrix and upper-triangle (where they fit), the dense path
+            # with the library GEMM and with its fused matrix-core kernel (where L allows)
+            runs += ([("fp32", "full"), ("fp32", "hermitian")] if stencil else []) + [("fp32", "dense"),
+                                                                                     ("fp32", "dense_mfma")]
             if stencil and (pulse or {}).get("pulse_type", 0) == 0 and coulomb_type != 2:
                 runs.append(("fp32", "pipelined"))
                 if L <= 64 and bases is None:
@@ -767,6 +769,19 @@ def test_long_range_hopping_general_coulomb_and_pulse_dense(oracle_mod):
             B=4)
 
 
+def test_long_range_hopping_without_pulse_dense(oracle_mod):
+    """Long-range hopping (more than 8 neighbours per site) without a pulse, on-site U, L = 24 (a multiple
+    of 8): the dense path's product of the real and imaginary planes with a real hopping matrix, through
+    the library GEMM and fused on the matrix cores in FP32 (K = L)."""
+    pos, box = lat.grid(6, 4)
+    L = pos.shape[0]
+    r = np.linalg.norm(pos[:, None, :] - pos[None, :, :], axis=-1)
+    tM = np.where((r > 0) & (r <= 2.3), -np.exp(-(r - 1.0) / 0.4), 0.0)
+    n_up = np.array([1.0, 0.0] * (L // 2))
+    compare(oracle_mod, tM, n_up, 1 - n_up, U=2.5 * np.eye(L), pot=np.linspace(-0.5, 0.5, L), dt=0.01, t_max=0.1,
+            B=5)
+
+
 def test_moire196_shape(oracle_mod):
     """The long-range twisted bilayer (L = 196, 17-65 neighbours per site, dense distance Coulomb U and V,
     Gaussian pulse) through the dense path, two trajectories over a few steps."""
@@ -821,8 +836,9 @@ def test_autotune_picks_a_kernel_that_reproduces_the_run():
 @pytest.mark.parametrize("engine", ["fp64", "fp32"])
 def test_autotune_gemm_solution_on_the_long_range_path(oracle_mod, engine):
     """Long-range hopping with a pulse takes the dense path's GEMM form; the autotuner also picks the
-    library's GEMM solution there.  The tuned run still matches the oracle on identical samples
-    (1e-10 FP64, 2e-5 FP32)."""
+    library's GEMM solution there, or in FP32 the fused matrix-core form when that measures faster (it
+    does at this L = 36).  The tuned run still matches the oracle on identical samples (1e-10 FP64,
+    2e-5 FP32)."""
     O = oracle_mod
     pos, _ = lat.grid(6, 6)
     L = pos.shape[0]
@@ -839,7 +855,8 @@ def test_autotune_gemm_solution_on_the_long_range_path(oracle_mod, engine):
     noise = e.sample_noise(0, B)
     e.set_tuning(0, 0, "dense")
     choice = e.autotune(trial_steps=3)
-    assert choice["kernel"] == "dense" and choice["traj_steps_per_s"] > 0
+    assert choice["kernel"] in (("dense", "dense_mfma") if engine == "fp32" else ("dense",))
+    assert choice["traj_steps_per_s"] > 0
     e.set_params(dt, t_max, 1, B, seed=21, engine=engine)
     e.set_noise(noise)
     e.run()
