@@ -16,6 +16,11 @@
 // the same LDS addresses), and each thread holds the RK registers of its S slots for both spins: the
 // upper triangle halves the register state of the full-matrix stencil kernel, which spills in FP64.
 // In FP32 the one-spin rows take 67 KB, so two workgroups share a CU and cover each other's barriers.
+// For L <= 32 one workgroup holds P = 2 (L <= 16: 4) trajectories: trajectory t's site i is row t L + i
+// of a block-diagonal system of P L <= 64 rows, phase A forms only the diagonal blocks (lane = row, the
+// lane's column block), and the slots of all P trajectories share the waves.  Every element sees the
+// same operands in the same order as with one trajectory per workgroup, and the energy sums of
+// trajectory t run on waves 2t and 2t + 1 with its sites on the same lanes, so packing changes no bit.
 // Null or on-site Coulomb interactions, no pulse, no density records, L <= 64 (FP32 64 < L <= 107: the
 // column-blocked form in ftwa_herm_wide.hip); everything else runs on the stencil, upper-triangle FP32
 // or dense kernels.  On the 8x8 workload the FP64 variant runs 1.40e7 trajectory-steps/s against 6.62e6
@@ -68,8 +73,8 @@ __device__ __forceinline__ T ldl(const unsigned char* base, int off)
 }
 
 // MINB: workgroups per CU the register budget is sized for (2 in FP32, 1 in FP64): at least
-// MINB * W / 4 waves per SIMD.
-template <typename Real, int W, int S, int Z, int MINB>
+// MINB * W / 4 waves per SIMD.  P: trajectories per workgroup (P L <= 64).
+template <typename Real, int W, int S, int Z, int MINB, int P>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MINB * W / 4)))
     herm_seq_kernel(DevSystem<Real> s, RunShape r, int t0, int t1, Real* __restrict__ state,
                     Real* __restrict__ rec)
@@ -86,12 +91,15 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
+    const int PL = P * L;                       // rows held: site i of trajectory t is row t L + i
     const int n_off = dev::class_rows(L);       // element rows of one spin's strict upper triangle (class order)
+    const int n_el = P * n_off;                 // the same for all held trajectories, trajectory-major
     const int tid = threadIdx.x;
     const int j = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int64_t b = blockIdx.x;
-    const int spare = kRows + L * kRowB;
+    const int64_t b0 = static_cast<int64_t>(blockIdx.x) * P;  // first trajectory of the workgroup
+    const int spare = kRows + PL * kRowB;
+    auto live = [&](int t) { return P == 1 || b0 + t < r.B; };  // the last workgroup may hold fewer
     Real* ew = reinterpret_cast<Real*>(smem + kEW);
 
     auto yat = [&](int row, int col) -> C& {
@@ -100,22 +108,26 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
 
     // ---- clear rows and tables (columns past L stay zero) ----
     {
-        const int n16 = (kRows + (L + 1) * kRowB) / 16;
+        const int n16 = (kRows + (PL + 1) * kRowB) / 16;
         for (int e = tid; e < n16; e += kNT) reinterpret_cast<float4*>(smem)[e] = float4{0, 0, 0, 0};
     }
     __syncthreads();
-    for (int i = tid; i < L; i += kNT) {
-        *reinterpret_cast<Real*>(smem + kSite + i * kRB) = s.tdiag[i];
-        *reinterpret_cast<Real*>(smem + kSite + (64 + i) * kRB) = s.epot[i];
-        *reinterpret_cast<Real*>(smem + kSite + (128 + i) * kRB) = s.coulomb_type == 1 ? s.u_diag[i] : Real(0);
-        *reinterpret_cast<Real*>(smem + kSite + (192 + i) * kRB) = s.pdiag[i];
+    for (int rw = tid; rw < PL; rw += kNT) {  // site tables per row
+        const int i = P == 1 ? rw : rw % L;
+        *reinterpret_cast<Real*>(smem + kSite + rw * kRB) = s.tdiag[i];
+        *reinterpret_cast<Real*>(smem + kSite + (64 + rw) * kRB) = s.epot[i];
+        *reinterpret_cast<Real*>(smem + kSite + (128 + rw) * kRB) = s.coulomb_type == 1 ? s.u_diag[i] : Real(0);
+        *reinterpret_cast<Real*>(smem + kSite + (192 + rw) * kRB) = s.pdiag[i];
     }
-    // neighbour tables for lane = row i (lanes past L: row 0 with t = 0)
+    // neighbour tables for lane = row t L + i: the neighbour row t L + k and the start of the row's column
+    // block t L (rows past P L: row 0 with t = 0)
     for (int e = tid; e < Z * 64; e += kNT) {
-        const int n = e >> 6, i = e & 63;
-        const bool on = i < L;
+        const int n = e >> 6, rw = e & 63;
+        const bool on = rw < PL;
+        const int tb = P == 1 || !on ? 0 : rw / L, i = rw - tb * L;
         *reinterpret_cast<Real*>(smem + kNbrT + e * kRB) = on ? s.nbr_t[n * L + i] : Real(0);
-        *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + (on ? s.nbr_k[n * L + i] : 0) * kRowB;
+        *reinterpret_cast<int*>(smem + kNbrK + e * 4) =
+            kRows + (on ? tb * L + s.nbr_k[n * L + i] : 0) * kRowB + tb * L * kCB;
     }
 
     // ---- first launch of a run: the initial state is sampled here (src/FHfTWASolver.cpp:342-362), the
@@ -128,8 +140,10 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         // c0 < i0 it stands for (L-1-i0, L-1-c0) (the middle row of an odd L only once), so nearly every lane
         // draws a pair
         const int half = (L + 1) / 2;
-        for (int e = tid; e < half * L; e += kNT) {
-            const int i0 = e / L, c0 = e - i0 * L;
+        for (int e = tid; e < P * half * L; e += kNT) {
+            const int tb = P == 1 ? 0 : e / (half * L);
+            const int e2 = e - tb * half * L;
+            const int i0 = e2 / L, c0 = e2 - i0 * L;
             int i = i0, c = c0;
             if (c0 < i0 && 2 * i0 != L - 1) {
                 i = L - 1 - i0;
@@ -140,22 +154,22 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
             const uint32_t p = static_cast<uint32_t>(dev::pair_index(L, i, c));
             for (int sg = 0; sg < 2; ++sg) {
                 double wr, wi, re, im;
-                dev::initial_noise(r.seed, r.b0 + b, sg, p, wr, wi);
+                dev::initial_noise(r.seed, r.b0 + b0 + tb, sg, p, wr, wi);
                 dev::sampled_offdiag(occ[sg * L + i], occ[sg * L + c], wr, wi, re, im);
-                *reinterpret_cast<C*>(smem + kRows + i * kRowB + c * kCB + sg * kXo) =
+                *reinterpret_cast<C*>(smem + kRows + (tb * L + i) * kRowB + (tb * L + c) * kCB + sg * kXo) =
                     C{static_cast<Real>(re), static_cast<Real>(im)};
             }
         }
         __syncthreads();
     }
 
-    // ---- element slots of the strict upper triangle (class order, class_slot): LDS addresses of Y_ij
-    // (aij) and Y_ji (aji) in the shared rows, h offsets ah = i*8 | (j*8) << 16; padding lanes point at the
-    // spare row and at h entry 64.  The diagonal is held apart, site i by thread i (< L) of wave 0: real
-    // RK registers only.
+    // ---- element slots of the strict upper triangles (class order, class_slot; element row er holds
+    // trajectory er / n_off): LDS addresses of Y_ij (aij) and Y_ji (aji) in the shared rows, h offsets
+    // ah = (t L + i)*8 | ((t L + j)*8) << 16; padding lanes point at the spare row and at h entry 64.  The
+    // diagonal is held apart, row t L + i by thread t L + i (< P L): real RK registers only.
     int aij[S], aji[S], ah[S];
     C y0[S], A0[S], p0[S], y1[S], A1[S], p1[S];
-    const C* gs = reinterpret_cast<const C*>(state) + b * 2 * LL;
+    const C* gs = reinterpret_cast<const C*>(state) + b0 * 2 * LL;
 #pragma unroll
     for (int q = 0; q < S; ++q) {
         const int er = q * W + w;
@@ -163,39 +177,50 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         aji[q] = spare;
         ah[q] = 64 * kRB | (64 * kRB) << 16;
         y0[q] = A0[q] = p0[q] = y1[q] = A1[q] = p1[q] = C{0, 0};
+        const int tb = P == 1 ? 0 : er / n_off, e2 = er - tb * n_off;
         int i = 0, jj = 0;
-        if (er < n_off && dev::class_slot(L, 2 * er + (j >> 5), j & 31, i, jj)) {
-            {
-                aij[q] = kRows + i * kRowB + jj * kCB;
-                aji[q] = kRows + jj * kRowB + i * kCB;
-                ah[q] = i * kRB | (jj * kRB) << 16;
-                if (sampled) {
-                    y0[q] = *reinterpret_cast<const C*>(smem + aij[q]);
-                    y1[q] = *reinterpret_cast<const C*>(smem + aij[q] + kXo);
-                } else {
-                    y0[q] = gs[i * L + jj];
-                    y1[q] = gs[LL + i * L + jj];
-                }
+        if (er < n_el && dev::class_slot(L, 2 * e2 + (j >> 5), j & 31, i, jj)) {
+            const int ri = tb * L + i, rj = tb * L + jj;
+            aij[q] = kRows + ri * kRowB + rj * kCB;
+            aji[q] = kRows + rj * kRowB + ri * kCB;
+            ah[q] = ri * kRB | (rj * kRB) << 16;
+            if (sampled) {
+                y0[q] = *reinterpret_cast<const C*>(smem + aij[q]);
+                y1[q] = *reinterpret_cast<const C*>(smem + aij[q] + kXo);
+            } else if (live(tb)) {
+                y0[q] = gs[tb * 2 * LL + i * L + jj];
+                y1[q] = gs[tb * 2 * LL + LL + i * L + jj];
             }
         }
     }
-    const bool dslot = tid < L;
+    const bool dslot = tid < PL;
     const int da = dslot ? kRows + tid * kRowB + tid * kCB : spare;
+    const int dt = P == 1 || !dslot ? 0 : tid / L, di = tid - dt * L;  // the diagonal's trajectory and site
     Real dy0 = 0, dy1 = 0;
     if (dslot) {  // diagonal: n - 1/2
-        dy0 = sampled ? static_cast<Real>(r.sample_occ[tid] - 0.5) : gs[tid * L + tid].x;
-        dy1 = sampled ? static_cast<Real>(r.sample_occ[L + tid] - 0.5) : gs[LL + tid * L + tid].x;
+        if (sampled) {
+            dy0 = static_cast<Real>(r.sample_occ[di] - 0.5);
+            dy1 = static_cast<Real>(r.sample_occ[L + di] - 0.5);
+        } else if (live(dt)) {
+            dy0 = gs[dt * 2 * LL + di * L + di].x;
+            dy1 = gs[dt * 2 * LL + LL + di * L + di].x;
+        }
     }
     Real dA0 = 0, dp0 = 0, dA1 = 0, dp1 = 0;
-    // ---- phase A: lane j is row i = j; wave w covers columns w + W*m.  The neighbour rows and
-    // hoppings come from the LDS tables at each use (not held in registers across the time loop);
-    // lanes past L read row 0 with t = 0 and write the spare row.
-    const int xb = (j < L ? kRows + j * kRowB : spare) + kXo + w * kCB;
-    auto nbr = [&](int n, int& kb, Real& tv) {
-        int o = n * 64 + j;
+    // ---- phase A: lane j is row j (site j - t L of trajectory t); wave w covers columns t L + w + W*m of
+    // the lane's block.  The neighbour rows and hoppings come from the LDS tables at each use (not held
+    // in registers across the time loop); lanes past P L read row 0 with t = 0 and write the spare row.
+    const int xb = (j < PL ? kRows + j * kRowB + (P == 1 ? 0 : j / L * L * kCB) : spare) + kXo + w * kCB;
+    // neighbour n of row rw: LDS offset of Y(k_n, t L) and the hopping
+    auto nbr_row = [&](int n, int rw, int& kb, Real& tv) {
+        int o = n * 64 + rw;
         asm volatile("" : "+v"(o));  // opaque: loaded here, not hoisted out of the time loop
-        kb = *reinterpret_cast<const int*>(smem + kNbrK + o * 4) + w * kCB;
+        kb = *reinterpret_cast<const int*>(smem + kNbrK + o * 4);
         tv = *reinterpret_cast<const Real*>(smem + kNbrT + o * kRB);
+    };
+    auto nbr = [&](int n, int& kb, Real& tv) {
+        nbr_row(n, j, kb, tv);
+        kb += w * kCB;
     };
 
     const Real h1 = static_cast<Real>(r.h), h3 = static_cast<Real>(r.h / 3.0), h6 = static_cast<Real>(r.h / 6.0),
@@ -208,7 +233,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         constexpr int HP = kHB + (ST & 1) * kHPar;
 #pragma unroll
         for (int q = 0; q < S; ++q) {
-            if (q * W + w >= n_off) continue;
+            if (q * W + w >= n_el) continue;
             const C& yq = SG ? y1[q] : y0[q];
             const C& Aq = SG ? A1[q] : A0[q];
             const C& pq = SG ? p1[q] : p0[q];
@@ -290,13 +315,13 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
                 nhj = ldl<Real>(smem, HS + (ahq >> 16));
                 nhi = ldl<Real>(smem, HS + (ahq & 0xffff));
             };
-            if (w < n_off) fetch(0);
+            if (w < n_el) fetch(0);
 #pragma unroll
             for (int q = 0; q < S; ++q) {
-                if (q * W + w >= n_off) break;  // slot rows grow with q (wave-uniform)
+                if (q * W + w >= n_el) break;  // slot rows grow with q (wave-uniform)
                 const C xij = nij, xji = nji;
                 const Real dh = nhj - nhi;
-                if (q + 1 < S && (q + 1) * W + w < n_off) fetch(q + 1);
+                if (q + 1 < S && (q + 1) * W + w < n_el) fetch(q + 1);
                 asm volatile("" ::: "memory");
                 C& yq = SG ? y1[q] : y0[q];
                 C& Aq = SG ? A1[q] : A0[q];
@@ -311,7 +336,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         } else {
 #pragma unroll
             for (int q = 0; q < S; ++q) {
-                if (q * W + w >= n_off) continue;
+                if (q * W + w >= n_el) continue;
                 asm volatile("" ::: "memory");
                 const C xij = ldl<C>(smem, aij[q] + kXo);
                 const C xji = ldl<C>(smem, aji[q] + kXo);
@@ -339,35 +364,41 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
 
     // ---- records of the time point, one spin per pass (rows hold that spin's stage-0 input); the
     // per-wave energy partials go through ew, so nothing stays live between the passes ----
-    Real* rr = nullptr;
+    // (trajectory t: hopping terms on wave 2t, on-site terms on wave 2t + 1, lane = site)
+    Real* rr = nullptr;  // records of the workgroup's first trajectory; trajectory t's follow at t R
     auto record_part = [&](auto sgc) {
         constexpr int SG = decltype(sgc)::value;
         int jt = j, tt = tid;
         asm volatile("" : "+v"(jt), "+v"(tt));
-        for (int e = tt; e < L; e += kNT) rr[SG * L + e] = yat(e, e).x;
-        if (tid == 0) {
-            const C c = yat(r.ci, r.cj);
-            rr[R2 + 4 + 2 * SG] = c.x;
-            rr[R2 + 5 + 2 * SG] = c.y;
+        for (int e = tt; e < PL; e += kNT) {
+            const int te = P == 1 ? 0 : e / L;
+            if (live(te)) rr[te * r.R + SG * L + e - te * L] = yat(e, e).x;
         }
+        if (tid < P && live(tid)) {
+            const C c = yat(tid * L + r.ci, tid * L + r.cj);
+            rr[tid * r.R + R2 + 4 + 2 * SG] = c.x;
+            rr[tid * r.R + R2 + 5 + 2 * SG] = c.y;
+        }
+        const int tw = w >> 1;  // the trajectory whose energy terms this wave holds (if tw < P)
+        const int rw = tw * L + jt;
         Real eth = 0, eu = 0, epe = 0;
-        if (w == 0 && jt < L) {  // hopping terms: Y(k_n(i), i) = conj(Y(i, k_n(i)))
+        if (tw < P && (w & 1) == 0 && jt < L) {  // hopping terms: Y(k_n(i), i) = conj(Y(i, k_n(i)))
 #pragma unroll
             for (int n = 0; n < Z; ++n) {
                 int kb;
                 Real tv;
-                nbr(n, kb, tv);
+                nbr_row(n, rw, kb, tv);
                 eth += tv * ldl<C>(smem, kb + jt * kCB).x;
             }
         }
-        if (SG == 0 && w == 1 && jt < L) {  // on-site terms from the stage-input diagonals
-            const Real nu = ldl<Real>(smem, kDg + jt * kRB) + half;
-            const Real nd = ldl<Real>(smem, kDg + kHSpin + jt * kRB) + half;
-            eth += ldl<Real>(smem, kSite + jt * kRB) * (nu + nd);
-            epe += ldl<Real>(smem, kSite + (64 + jt) * kRB) * (nu + nd);
-            eu += ldl<Real>(smem, kSite + (128 + jt) * kRB) * nu * nd;
+        if (SG == 0 && tw < P && (w & 1) == 1 && jt < L) {  // on-site terms from the stage-input diagonals
+            const Real nu = ldl<Real>(smem, kDg + rw * kRB) + half;
+            const Real nd = ldl<Real>(smem, kDg + kHSpin + rw * kRB) + half;
+            eth += ldl<Real>(smem, kSite + rw * kRB) * (nu + nd);
+            epe += ldl<Real>(smem, kSite + (64 + rw) * kRB) * (nu + nd);
+            eu += ldl<Real>(smem, kSite + (128 + rw) * kRB) * nu * nd;
         }
-        if (w < 2) {  // the other waves hold no energy terms
+        if (w < 2 * P) {  // the other waves hold no energy terms
             eth = wave_sum(eth);
             eu = wave_sum(eu);
             epe = wave_sum(epe);
@@ -382,11 +413,12 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         }
     };
     auto flush_energies = [&]() {
-        if (tid == 0) {
+        if (tid < P && live(tid)) {  // one trajectory per thread: its two waves (P = 1: every wave, the rest 0)
             Real e4[4] = {0, 0, 0, 0};
-            for (int ww = 0; ww < W; ++ww)
+            const int w0 = P == 1 ? 0 : 2 * tid, w1 = P == 1 ? W : 2 * tid + 2;
+            for (int ww = w0; ww < w1; ++ww)
                 for (int qq = 0; qq < 4; ++qq) e4[qq] += ew[ww * 4 + qq];
-            for (int qq = 0; qq < 4; ++qq) rr[R2 + qq] = e4[qq];
+            for (int qq = 0; qq < 4; ++qq) rr[tid * r.R + R2 + qq] = e4[qq];
         }
     };
 
@@ -409,7 +441,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
     const int64_t Bn = r.B;
     const int R = r.R;
     for (int t = t0; t < t1; ++t) {
-        rr = rec + (static_cast<int64_t>(t - t0) * Bn + b) * R;
+        rr = rec + (static_cast<int64_t>(t - t0) * Bn + b0) * R;
         if (t == r.T - 1) {  // no step after the last record changes an output
             materialize(Ic<0>{}, Ic<0>{});
             __syncthreads();
@@ -430,34 +462,38 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         }
     }
     // ---- state back to HBM, from the RK registers ----
-    C* go = reinterpret_cast<C*>(state) + b * 2 * LL;
+    C* go = reinterpret_cast<C*>(state) + b0 * 2 * LL;
 #pragma unroll
     for (int q = 0; q < S; ++q) {
         if (aij[q] == spare) continue;
-        const int i = (ah[q] & 0xffff) / kRB, col = (ah[q] >> 16) / kRB;
-        go[i * L + col] = y0[q];
-        go[col * L + i] = C{y0[q].x, -y0[q].y};
-        go[LL + i * L + col] = y1[q];
-        go[LL + col * L + i] = C{y1[q].x, -y1[q].y};
+        const int ri = (ah[q] & 0xffff) / kRB, rj = (ah[q] >> 16) / kRB;
+        const int tb = P == 1 ? 0 : ri / L, i = ri - tb * L, col = rj - tb * L;
+        if (!live(tb)) continue;
+        C* g = go + tb * 2 * LL;
+        g[i * L + col] = y0[q];
+        g[col * L + i] = C{y0[q].x, -y0[q].y};
+        g[LL + i * L + col] = y1[q];
+        g[LL + col * L + i] = C{y1[q].x, -y1[q].y};
     }
-    if (dslot) {
-        go[tid * L + tid] = C{dy0, Real(0)};
-        go[LL + tid * L + tid] = C{dy1, Real(0)};
+    if (dslot && live(dt)) {
+        C* g = go + dt * 2 * LL;
+        g[di * L + di] = C{dy0, Real(0)};
+        g[LL + di * L + di] = C{dy1, Real(0)};
     }
 }
 
 template <typename Real>
-size_t seq_lds_bytes(int L)
+size_t seq_lds_bytes(int rows)
 {
-    return Lay<Real>::kRows + static_cast<size_t>(L + 1) * Lay<Real>::kRowB;
+    return Lay<Real>::kRows + static_cast<size_t>(rows + 1) * Lay<Real>::kRowB;
 }
 
-template <typename Real, int W, int S, int Z>
+template <typename Real, int W, int S, int Z, int P>
 hipError_t run_seq(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
                    hipStream_t st)
 {
     constexpr int MINB = sizeof(Real) == 4 ? 2 : 1;
-    auto kern = herm_seq_kernel<Real, W, S, Z, MINB>;
+    auto kern = herm_seq_kernel<Real, W, S, Z, MINB, P>;
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -465,30 +501,49 @@ hipError_t run_seq(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, 
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(64 * W), seq_lds_bytes<Real>(s.L), st, s, r, t0,
-                       t1, state, rec);
+    const int64_t grid = (r.B + P - 1) / P;
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(64 * W), seq_lds_bytes<Real>(P * s.L), st, s, r,
+                       t0, t1, state, rec);
     return hipGetLastError();
 }
 
-template <typename Real, int W, int Z>
+template <typename Real, int W, int Z, int P>
 hipError_t run_seq_s(int S, const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
                      hipStream_t st)
 {
-    // L <= 64: at most 32 element rows, so S <= 2 with 16 waves and S <= 4 with 8
+    // P L <= 64: at most 32 element rows, so S <= 2 with 16 waves and S <= 4 with 8
     if constexpr (W == 16) {
         switch (S) {
-            case 1: return run_seq<Real, 16, 1, Z>(s, r, t0, t1, state, rec, st);
-            case 2: return run_seq<Real, 16, 2, Z>(s, r, t0, t1, state, rec, st);
+            case 1: return run_seq<Real, 16, 1, Z, P>(s, r, t0, t1, state, rec, st);
+            case 2: return run_seq<Real, 16, 2, Z, P>(s, r, t0, t1, state, rec, st);
+            default: return hipErrorInvalidValue;
+        }
+    } else if constexpr (P > 1) {  // L <= 32: S <= 2
+        switch (S) {
+            case 1: return run_seq<Real, 8, 1, Z, P>(s, r, t0, t1, state, rec, st);
+            case 2: return run_seq<Real, 8, 2, Z, P>(s, r, t0, t1, state, rec, st);
             default: return hipErrorInvalidValue;
         }
     } else {
         switch (S) {
-            case 1: return run_seq<Real, 8, 1, Z>(s, r, t0, t1, state, rec, st);
-            case 2: return run_seq<Real, 8, 2, Z>(s, r, t0, t1, state, rec, st);
-            case 3: return run_seq<Real, 8, 3, Z>(s, r, t0, t1, state, rec, st);
-            case 4: return run_seq<Real, 8, 4, Z>(s, r, t0, t1, state, rec, st);
+            case 1: return run_seq<Real, 8, 1, Z, 1>(s, r, t0, t1, state, rec, st);
+            case 2: return run_seq<Real, 8, 2, Z, 1>(s, r, t0, t1, state, rec, st);
+            case 3: return run_seq<Real, 8, 3, Z, 1>(s, r, t0, t1, state, rec, st);
+            case 4: return run_seq<Real, 8, 4, Z, 1>(s, r, t0, t1, state, rec, st);
             default: return hipErrorInvalidValue;
         }
+    }
+}
+
+template <typename Real, int W, int Z>
+hipError_t run_seq_p(int P, int S, const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
+                     hipStream_t st)
+{
+    switch (P) {
+        case 1: return run_seq_s<Real, W, Z, 1>(S, s, r, t0, t1, state, rec, st);
+        case 2: return run_seq_s<Real, W, Z, 2>(S, s, r, t0, t1, state, rec, st);
+        case 4: return run_seq_s<Real, W, Z, 4>(S, s, r, t0, t1, state, rec, st);
+        default: return hipErrorInvalidValue;
     }
 }
 
@@ -509,14 +564,16 @@ hipError_t launch_herm_seq(const DevSystem<Real>& s, const RunShape& r, int t0, 
     // 8 waves by default.  FP64: 4 slots per thread at L = 64 fit the 256 registers of a one-workgroup-
     // per-CU shape without spilling (1.34e7 trajectory-steps/s on the 8x8 workload, against 1.12e7 with
     // 16 waves, whose 2 slots exceed 128 registers).  FP32: two 8-wave workgroups per CU.  16 waves on
-    // request (nt_hint 1024).
+    // request (nt_hint 1024); one trajectory per workgroup for every L on request (nt_hint 512).
     const int W = nt_hint == 1024 ? 16 : 8;
-    const int rows = dev::class_rows(s.L);  // element rows of the strict upper triangle (at most 32 for L <= 64)
+    // trajectories per workgroup: as many as fit 64 rows, up to 4 (one on request, nt_hint 512)
+    const int P = nt_hint == 512 || s.L > 32 ? 1 : (s.L <= 16 ? 4 : 2);
+    const int rows = P * dev::class_rows(s.L);  // element rows of the strict upper triangles (at most 32)
     const int S = std::max(1, (rows + W - 1) / W);
-    if (W == 16) return s.z == 4 ? run_seq_s<Real, 16, 4>(S, s, r, t0, t1, state, rec, st)
-                                 : run_seq_s<Real, 16, 8>(S, s, r, t0, t1, state, rec, st);
-    return s.z == 4 ? run_seq_s<Real, 8, 4>(S, s, r, t0, t1, state, rec, st)
-                    : run_seq_s<Real, 8, 8>(S, s, r, t0, t1, state, rec, st);
+    if (W == 16) return s.z == 4 ? run_seq_p<Real, 16, 4>(P, S, s, r, t0, t1, state, rec, st)
+                                 : run_seq_p<Real, 16, 8>(P, S, s, r, t0, t1, state, rec, st);
+    return s.z == 4 ? run_seq_p<Real, 8, 4>(P, S, s, r, t0, t1, state, rec, st)
+                    : run_seq_p<Real, 8, 8>(P, S, s, r, t0, t1, state, rec, st);
 }
 
 template bool herm_seq_fits<float>(const DevSystem<float>&);

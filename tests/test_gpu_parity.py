@@ -214,6 +214,50 @@ def test_in_kernel_sampling_matches_sampling_kernel(engine, chunk):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("engine", ["fp32", "fp64"])
+@pytest.mark.parametrize("shape,B", [("honeycomb24", 6), ("honeycomb24", 7), ("grid4x4", 9), ("chain5", 8),
+                                     ("chain32", 5)])
+def test_packed_trajectories_are_bitwise_neutral(oracle_mod, engine, shape, B):
+    """L <= 32: the spin-sequential kernel holds 2 trajectories per workgroup (L <= 16: 4), the last
+    workgroup ragged where that does not divide B.  Against one trajectory per workgroup (block hint 512)
+    it gives bitwise the same final states, ensemble sums, energies and recorded element, with the initial
+    states sampled in the kernel and from explicit noise, in whole-run and 3-time-point launches; and it
+    matches the oracle on the explicit samples (FP64 1e-10, FP32 2e-5).  Honeycomb: z = 3, padded to 4."""
+    if shape == "honeycomb24":
+        pos, box = lat.honeycomb(4, 3)
+        tM, Um = lat.system_arrays(pos, box, True, t=-1.0, U=4.0)
+    elif shape == "grid4x4":
+        pos, box = lat.grid(4, 4)
+        tM, Um = lat.system_arrays(pos, box, True, t=-1.0, U=4.0)
+    else:
+        tM = chain(int(shape[5:]), periodic=True)
+        Um = 4.0 * np.eye(tM.shape[0])
+    L = tM.shape[0]
+    rng = np.random.default_rng(zlib.crc32(shape.encode()))
+    n_up = (rng.uniform(size=L) < 0.5).astype(float)
+    n_up[1] = 0.3
+    n_dn = 1.0 - n_up
+    pot = rng.uniform(-1, 1, L)
+    eng = _lib.Engine(tM, U=Um, pot=pot)
+    eng.set_initial_state(n_up, n_dn)
+    eng.set_results(_lib.RES_DIAG | _lib.RES_DOUBLONS | _lib.RES_ENERGIES | _lib.RES_WHOLE)
+    keys = ("sum_u", "sum_d", "sum2_u", "sum_ud", "sum_e", "N1hu", "N1hd")
+    for explicit in (False, True):
+        for chunk in (0, 3):
+            out = []
+            for block in (0, 512):
+                eng.set_tuning(block, chunk, "sequential")
+                eng.set_params(0.02, 0.12, 1, B, seed=5, engine=engine)
+                eng.set_noise(eng.sample_noise(0, B) if explicit else None)
+                eng.run()
+                out.append((eng.states(), [eng.result(k) for k in keys]))
+            assert np.array_equal(out[0][0], out[1][0]), (explicit, chunk)
+            for k, a, b in zip(keys, out[0][1], out[1][1]):
+                assert np.array_equal(a, b), (explicit, chunk, k)
+    eng.close()
+    compare(oracle_mod, tM, n_up, n_dn, U=Um, pot=pot, dt=0.02, t_max=0.12, B=B, engines=(engine,))
+
+
 def test_graph_launch_is_bitwise_neutral():
     """fpsftwa_set_graph: the run's integrator and reduction launches (here 7 time chunks of 3 points)
     replayed as one HIP graph give bitwise the results of direct launches, alone and in fpsftwa_run_many
