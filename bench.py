@@ -522,7 +522,7 @@ def main():
     kern_s = tm["kernel_ms"] * 1e-3
     achieved = alg_bytes / kern_s / 1e9
     kname = {"full": "stencil_kernel<float,16,8,4,false,64>", "hermitian": "herm_kernel<16,8,5,4,false,false>",
-             "pipelined": "herm_pipe_kernel<5,4>"}.get(args.kernel, "herm_seq_kernel<float,8,4,4,2>")
+             "pipelined": "herm_pipe_kernel<5,4>"}.get(args.kernel, "herm_seq_kernel<float,8,4,4,2,1>")
     pfile = "herm_fp32_hubbard8x8_traffic.json" if args.kernel == "hermitian" else "herm_seq_fp32_hubbard8x8_traffic.json"
     traffic, prof = traffic_from_profile(os.path.join(ROOT, "profiles", pfile))
     if prof is None or prof.get("kernel") != kname:

@@ -80,7 +80,10 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
     const int spare = kRows + L * RB;
     const float half = 0.5f;
     float* ew = reinterpret_cast<float*>(smem + kEW);
-    auto yat = [&](int row, int col) -> C& { return *reinterpret_cast<C*>(smem + kRows + row * RB + col * kCB); };
+    // byte offset of a row: rows < 128 and strides < 1,400 B, so the full-rate 24-bit multiply (a 32-bit
+    // multiply issues at a quarter of the rate, and the slots form these offsets at every use)
+    auto rowb = [&](int row) { return static_cast<int>(__umul24(static_cast<unsigned>(row), static_cast<unsigned>(RB))); };
+    auto yat = [&](int row, int col) -> C& { return *reinterpret_cast<C*>(smem + kRows + rowb(row) + col * kCB); };
 
     // ---- clear everything (h and X padding stay zero), then the tables ----
     {
@@ -160,8 +163,8 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
             const float yy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
             int i, jj;
             slot(q, i, jj);
-            *reinterpret_cast<C*>(smem + kRows + i * RB + jj * kCB) = C{yx, yy};
-            *reinterpret_cast<C*>(smem + kRows + jj * RB + i * kCB) = C{yx, -yy};
+            *reinterpret_cast<C*>(smem + kRows + rowb(i) + jj * kCB) = C{yx, yy};
+            *reinterpret_cast<C*>(smem + kRows + rowb(jj) + i * kCB) = C{yx, -yy};
         }
         if (dslot) {
             const float yx = SG ? dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6)
@@ -197,7 +200,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
                 for (int n = 0; n < Z; ++n) tv[n] = 0.f;
             }
-            const int xb = (on ? kRows + i * RB : spare) + (L + w) * kCB;
+            const int xb = (on ? kRows + rowb(i) : spare) + (L + w) * kCB;
             const int yc = (64 * BLK + w) * kCB;
             // software pipelined as in ftwa_herm_seq.hip where the registers allow (z = 4, S <= 10): the
             // next column's reads go out before this column's X store; the first read (column
@@ -244,7 +247,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
             asm volatile("" ::: "memory");
             int i, jj;
             slot(q, i, jj);
-            const int aij = kRows + i * RB + jj * kCB, aji = kRows + jj * RB + i * kCB;
+            const int aij = kRows + rowb(i) + jj * kCB, aji = kRows + rowb(jj) + i * kCB;
             const int ci = i - 64 * BLK, cj = jj - 64 * BLK;
             const C xij = ldl<C>(smem, (cj >= 0 && cj < 64) ? aij + xo : kZero);
             const C xji = ldl<C>(smem, (ci >= 0 && ci < 64) ? aji + xo : kZero);
