@@ -264,175 +264,393 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// ---- fused product and update on the matrix cores (FP32) -------------------------------------------
+// ---- fused product and update on the matrix cores, several (trajectory, spin) slices per workgroup ----
 //
-// One workgroup per tile pair {(I, J), (J, I)}, I <= J, of one (trajectory, spin): X = Y T for both tiles
-// from v_mfma_f32_32x32x2f32, then the RK update of dense_update_kernel, without X leaving the chip.
-// Each of the 4 waves forms one real 32x32 output tile over the whole K: Re or Im of X(I, J) from rows I
-// of the stage input and columns J of the stage's hopping, or of X(J, I); a diagonal pair splits K
-// between its wave pairs instead and adds the halves in a fixed order.  With a pulse K = 2L: row i of the
-// stage input is [Y_re Y_im], and M = [[Tr, Ti], [-Ti, Tr]] (symmetric, since T is Hermitian) gives
-// Re X_ij from column j and Im X_ij from column L + j; without one K = L, M = T real symmetric, the real
-// and imaginary planes of Y times T.  Operands come straight from memory, one 16-byte load per lane and
-// operand for 4 instructions: lane l supplies k = k0 + 4 (l / 32) + m to instruction m of an 8-k block
-// (the same permutation of k for both factors leaves the product unchanged), rows and columns past L
-// read zeros.  Block b of the grid is remapped so that the pairs of one trajectory run on one XCD
-// (dispatch is round-robin over the 8 XCDs) and its stage input is served from that XCD's L2.  The new
-// stage input goes to the other buffer (other workgroups still read this one).
-typedef float fps_f32x16 __attribute__((ext_vector_type(16)));
+// One workgroup per (tile pair {(I, J), (J, I)} of 16 x 16 complex tiles, group of kGrpSlices slices),
+// 4 waves of kWaveSlices slices each.  The stage's hopping operand M is the same for every slice, so the
+// workgroup stages the columns of M it needs (those of tiles J and I, Re and Im with a pulse) through LDS in
+// chunks of kKc k, double-buffered (one barrier per chunk), and every wave reads them back for each of its
+// slices; the rows of the stage input are per slice and come straight from memory (the pairs of one group
+// run on one XCD, so after the first pair they are served from its L2).  v_mfma_f32_16x16x4f32: a 16 x 16
+// output block per 4 k, lane l supplying row/column l % 16 at k = k0 + 4 (l / 16) + m to instruction m of a
+// 16-k block from one 16-byte load (the same permutation of k in both factors leaves the product
+// unchanged).  The accumulators of both tiles stay in registers; X_ji is brought to the lane holding X_ij
+// through a small per-wave LDS transpose, the RK update of dense_update_kernel follows, and the new stage
+// input goes to the other buffer (its mirror through the same transpose, so rows are written contiguously).
+// With a pulse K = 2L: row i of the stage input is [Y_re Y_im] and M = [[Tr, Ti], [-Ti, Tr]] is symmetric
+// (T is Hermitian), so Re X_ij = row i . M row j and Im X_ij = row i . M row L + j.  Without one K = L and
+// M = T real symmetric: Re X = Y_re T, Im X = Y_im T.
+constexpr int kMt = 16;           // complex tile edge
+constexpr int kKc = 64;           // k per LDS chunk
+constexpr int kKcPad = kKc + 4;   // row stride of the LDS chunk (17 16-byte units: conflict-free b128 reads)
+constexpr int kWaveSlices = 2;    // (trajectory, spin) slices per wave
+constexpr int kGrpSlices = 4 * kWaveSlices;
+
+typedef float fps_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ fps_f32x4 ld4(const float* p) { return *reinterpret_cast<const fps_f32x4*>(p); }
+
+__device__ __forceinline__ float ld1b(const float* base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+__device__ __forceinline__ void st1b(float* base, uint32_t byte_off, float v)
+{
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+// 16 bytes at a 32-bit byte offset from a wave-uniform base (the scalar-base form of the global load)
+__device__ __forceinline__ fps_f32x4 ld4b(const float* base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const fps_f32x4*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+__device__ __forceinline__ fps_f32x4 keep4(bool on, fps_f32x4 v)  // v or zeros, per component (no branch)
+{
+    v.x = on ? v.x : 0.f;
+    v.y = on ? v.y : 0.f;
+    v.z = on ? v.z : 0.f;
+    v.w = on ? v.w : 0.f;
+    return v;
+}
 
 template <int ST, bool PULSE>
 __global__ void __launch_bounds__(256)
-    dense_mfma_update_kernel(int L, int64_t B, int npair, double hstep, const int2* pairs, const float* M,
-                             const float* hv, const float* yin, float* y, float* A, float* p, float* yout)
+    dense_mfma_group_kernel(int L, int64_t nbs, int npair, double hstep, const int2* pairs, const float* M,
+                            const float* hv, const float* yin, float* y, float* A, float* p, float* yout)
 {
-    __shared__ float xs[2][2][kTile][kTile + 1];  // [side][re/im][row][col]: X(I, J), X(J, I); then new Y
-    const int64_t nblk = B * 2 * npair;
-    int64_t blk = blockIdx.x;
-    if ((nblk & 7) == 0) blk = (blk & 7) * (nblk >> 3) + (blk >> 3);  // XCD-major: one trajectory per XCD
-    const int pair = static_cast<int>(blk % npair);
-    const int64_t bs = blk / npair;
-    const int sg = static_cast<int>(bs & 1);
-    const int64_t b = bs >> 1;
-    const int2 pr = pairs[pair];
-    const int I0 = pr.x * kTile, J0 = pr.y * kTile;
-    const bool diag = pr.x == pr.y;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int part = w & 1;   // 0: Re X, 1: Im X
-    const int side = w >> 1;  // off-diagonal: 0 tile (I, J), 1 tile (J, I); diagonal: half of K
+    constexpr int NCOL = PULSE ? 4 * kMt : 2 * kMt;  // staged columns of M: J (Re, Im), I (Re, Im) | J, I
+    constexpr int NA = PULSE ? 2 : 4;                // row streams per slice: I, J | I re, I im, J re, J im
+    constexpr int NB = PULSE ? 4 : 2;                // staged column blocks
+    constexpr int F4 = NCOL * kKc / 4;               // float4 per chunk of M
+    static_assert(F4 % 256 == 0, "whole float4 per thread");
+    constexpr int NLD = F4 / 256;
+    static_assert(kKc == 64, "four 16-k blocks per chunk");
+    __shared__ float ms[2][NCOL][kKcPad];
+    __shared__ float xt[4][2][kMt][kMt + 1];  // per wave: transpose of one tile (re, im)
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: slice bases live in scalar registers
     const int K = PULSE ? 2 * L : L;
-    const int r0 = (diag || side == 0) ? I0 : J0;  // rows of Y
-    const int c0 = (diag || side == 0) ? J0 : I0;  // columns of X
-    const int kmid = (K / 2) & ~7;  // blocks of 8 k on both sides of the split
-    const int ka = diag && side ? kmid : 0, kb = diag && !side ? kmid : K;
-    const int arow = r0 + (lane & 31), bcol = c0 + (lane & 31);
-    const int koff = 4 * (lane >> 5);
-    const bool aon = arow < L, bon = bcol < L;
-    // A: row arow of the stage input (plane part without a pulse); B: row bcol (+ L for Im with a pulse) of M
-    const float* ap = yin + row_off<float>(b, sg, aon ? arow : 0, L) + (PULSE ? 0 : part * L) + koff;
-    const float* bp = M + static_cast<size_t>(bon ? bcol + (PULSE ? part * L : 0) : 0) * (PULSE ? 2 * L : L) + koff;
-    fps_f32x16 acc;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-    const float4 zero4 = {0.f, 0.f, 0.f, 0.f};
-    float4 a = aon ? *reinterpret_cast<const float4*>(ap + ka) : zero4;
-    float4 bv = bon ? *reinterpret_cast<const float4*>(bp + ka) : zero4;
-    for (int k0 = ka; k0 < kb; k0 += 8) {
-        // the next block's operands in flight during this block's instructions (a ring of several blocks
-        // measured 3x slower: the conditional refills left the waits unresolved)
-        float4 an = zero4, bn = zero4;
-        if (k0 + 8 < kb) {
-            if (aon) an = *reinterpret_cast<const float4*>(ap + k0 + 8);
-            if (bon) bn = *reinterpret_cast<const float4*>(bp + k0 + 8);
-        }
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, bv.x, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, bv.y, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, bv.z, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, bv.w, acc, 0, 0, 0);
-        a = an;
-        bv = bn;
-    }
-    // accumulator lane layout: register r holds row 8 (r / 4) + 4 (lane / 32) + r % 4, column lane % 32
-    const int col = lane & 31;
-    if (!diag || side == 0) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) xs[diag ? 0 : side][part][8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)][col] = acc[r];
-    }
-    __syncthreads();
-    if (diag && side == 1) {  // the second half of K, added in a fixed order
-#pragma unroll
-        for (int r = 0; r < 16; ++r) xs[0][part][8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)][col] += acc[r];
-    }
-    __syncthreads();
-    const int tsrc = diag ? 0 : 1;  // tile holding X_ji, read transposed
+    const int row2 = 2 * L;  // floats per row of the stage input (Re plane, Im plane)
+    const int q = lane >> 4, c16 = lane & 15;
+    const int kq = 4 * (tid & (kKc / 4 - 1));  // k offset of this thread's float4 within a chunk of M
     const float h = static_cast<float>(hstep);
     const float h3 = static_cast<float>(hstep / 3.0), h6 = static_cast<float>(hstep / 6.0),
                 h8 = static_cast<float>(hstep / 8.0);
-    const float* hs = hv + (b * 2 + sg) * L;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows of 32 per pass
-    for (int r = ty; r < kTile; r += 8) {
-        const int i = I0 + r, j = J0 + tx;
-        if (i >= L || j >= L || i > j) continue;
-        const size_t o = row_off<float>(b, sg, i, L) + j;
-        const float xr = xs[0][0][r][tx], xi = xs[0][1][r][tx];       // X_ij
-        const float tr = xs[tsrc][0][tx][r], ti = xs[tsrc][1][tx][r];  // X_ji
-        const float yr = y[o], yi = y[o + L];
-        float Ar = 0, Ai = 0, pr_ = 0, pi = 0;
-        if (ST > 0) {
-            Ar = A[o];
-            Ai = A[o + L];
-            pr_ = p[o];
-            pi = p[o + L];
-        }
-        float cr, ci;  // current stage input
-        if (ST == 0) {
-            cr = yr;
-            ci = yi;
-        } else if (ST == 1) {
-            cr = yr + h3 * pr_;
-            ci = yi + h3 * pi;
-        } else if (ST == 2) {
-            cr = yr + h6 * (Ar - 3.f * pr_);
-            ci = yi + h6 * (Ai - 3.f * pi);
-        } else {
-            cr = yr + h * pr_;
-            ci = yi + h * pi;
-        }
-        // c = X_ij - conj(X_ji) + (h_j - h_i) Y_ij;  k = -i c
-        const float dh = hs[j] - hs[i];
-        const float cx = xr - tr + dh * cr, cy = xi + ti + dh * ci;
-        float kx = cy, ky = -cx;
-        if (i == j) ky = 0;  // the diagonal slope is real
-        float nr, ni;
-        if (ST == 0) {
-            Ar = kx;
-            Ai = ky;
-            pr_ = kx;
-            pi = ky;
-            nr = yr + h3 * kx;
-            ni = yi + h3 * ky;
-        } else if (ST == 1) {
-            Ar += 3.f * kx;
-            Ai += 3.f * ky;
-            pr_ -= kx;
-            pi -= ky;
-            nr = yr + h6 * (Ar - 3.f * pr_);
-            ni = yi + h6 * (Ai - 3.f * pi);
-        } else if (ST == 2) {
-            Ar += 3.f * kx;
-            Ai += 3.f * ky;
-            pr_ += kx;
-            pi += ky;
-            nr = yr + h * pr_;
-            ni = yi + h * pi;
-        } else {
-            Ar += kx;
-            Ai += ky;
-            nr = yr + h8 * Ar;
-            ni = yi + h8 * Ai;
-            y[o] = nr;
-            y[o + L] = ni;
-        }
-        if (ST < 3) {
-            A[o] = Ar;
-            A[o + L] = Ai;
-            p[o] = pr_;
-            p[o + L] = pi;
-        }
-        yout[o] = nr;
-        yout[o + L] = ni;
-        xs[0][0][r][tx] = nr;  // only this thread reads (r, tx) of X(I, J)
-        xs[0][1][r][tx] = ni;
+    float* ms0 = &ms[0][0][0];
+    int mst[NLD];  // LDS float offset of this thread's float4 m within a chunk buffer
+#pragma unroll
+    for (int m = 0; m < NLD; ++m) mst[m] = ((tid + 256 * m) / (kKc / 4)) * kKcPad + kq;
+
+    // ---- the item of this workgroup: (group of kGrpSlices slices, tile pair), group-major.  With a whole
+    // multiple of 8 items, XCD x (workgroups b = x mod 8; dispatch is round-robin over the 8 XCDs) runs items
+    // [x n/8, (x+1) n/8), so the pairs of a group run on one XCD and its stage input is served from that L2.
+    const int64_t nitem = ((nbs + kGrpSlices - 1) / kGrpSlices) * npair;
+    int64_t it = blockIdx.x;
+    if ((nitem & 7) == 0) it = (it & 7) * (nitem >> 3) + (it >> 3);
+
+    // ---- per-item state.  Off-diagonal pairs run blocks 0, 1 (rows I, columns J) and 2, 3 (rows J, columns I)
+    // over the whole K.  A diagonal pair needs only X(I, I): blocks 0, 1 take k < kh and blocks 2, 3 the rest
+    // (rows J = rows I, columns I = columns J, read from k + kh), and the halves are added at the end.  Stream
+    // g (0: blocks 0, 1; 1: blocks 2, 3) is read at k + kofs[g] while k < kend[g].  Loads are unconditional from
+    // clamped addresses, and what lies outside is replaced by zeros where the value is used (a conditional
+    // 16-byte load compiles to a branch around a flat load).
+    int I0, J0, kend0, kend1, klen;
+    bool diag;
+    uint32_t mp[NLD];  // byte offset in M of the staged row (= column, M symmetric) at k = kq (+ kofs)
+    bool mon[NLD];     // column inside L (else row 0 is read and zeros are stored)
+    int mend[NLD];     // loop k below which the column's values are inside K
+    const float* ys[kWaveSlices];  // wave-uniform start of each slice's stage input
+    int64_t bsl[kWaveSlices];
+    bool son[kWaveSlices];
+    uint32_t ap[NA];   // per-lane byte offsets of the row streams within a slice
+    uint32_t eo[4];    // byte offset of this lane's element r (row I0 + 4 q + r, column J0 + c16) in a slice's planes
+    bool eon[4];       // element r in the upper triangle
+#define FPS_GROUP_SETUP(ITEM)                                                                                 \
+    {                                                                                                         \
+        const int pair_ = static_cast<int>((ITEM) % npair);                                                   \
+        const int64_t grp_ = (ITEM) / npair;                                                                  \
+        const int2 pr_ = pairs[pair_];                                                                        \
+        I0 = pr_.x * kMt;                                                                                     \
+        J0 = pr_.y * kMt;                                                                                     \
+        diag = pr_.x == pr_.y;                                                                                \
+        const bool diag_ = diag;                                                                              \
+        const int kh_ = min(((K / 2) + 15) & ~15, K); /* K < 16: the whole K on blocks 0, 1 */                \
+        const int kofs1_ = diag_ ? kh_ : 0;                                                                   \
+        kend0 = diag_ ? kh_ : K;                                                                              \
+        kend1 = diag_ ? K - kh_ : K;                                                                          \
+        klen = kend0;                                                                                         \
+        _Pragma("unroll") for (int m = 0; m < NLD; ++m) {                                                     \
+            const int f = tid + 256 * m, cc = f / (kKc / 4);                                                  \
+            const int blkc = cc / kMt, c = cc % kMt;                                                          \
+            const bool second = PULSE ? (blkc & 2) != 0 : blkc != 0; /* columns I: blocks 2, 3 */             \
+            const int col = (second ? I0 : J0) + c;                                                           \
+            mon[m] = col < L;                                                                                 \
+            const int mr = PULSE && (blkc & 1) ? col + L : col;                                               \
+            mp[m] = 4u * static_cast<uint32_t>((mon[m] ? mr : 0) * K + kq + (second ? kofs1_ : 0));           \
+            mend[m] = second ? kend1 : kend0;                                                                 \
+        }                                                                                                     \
+        _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s) {                                             \
+            const int64_t bs = grp_ * kGrpSlices + w * kWaveSlices + s;                                       \
+            son[s] = bs < nbs;                                                                                \
+            bsl[s] = son[s] ? bs : 0;                                                                         \
+            ys[s] = yin + static_cast<size_t>(bsl[s]) * L * row2;                                             \
+        }                                                                                                     \
+        const int ri_ = I0 + c16 < L ? I0 + c16 : L - 1, rj_ = J0 + c16 < L ? J0 + c16 : L - 1;               \
+        ap[0] = 4u * static_cast<uint32_t>(ri_ * row2 + 4 * q);                                               \
+        if (PULSE) {                                                                                          \
+            ap[1] = 4u * static_cast<uint32_t>(rj_ * row2 + 4 * q + kofs1_);                                  \
+        } else {                                                                                              \
+            ap[1] = ap[0] + 4u * L;                                                                           \
+            ap[NA - 2] = 4u * static_cast<uint32_t>(rj_ * row2 + 4 * q + kofs1_);                             \
+            ap[NA - 1] = ap[NA - 2] + 4u * L;                                                                 \
+        }                                                                                                     \
+        const int jc_ = J0 + c16;                                                                             \
+        _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                                       \
+            const int i = I0 + 4 * q + r;                                                                     \
+            eon[r] = i < L && jc_ < L && i <= jc_;                                                            \
+            eo[r] = eon[r] ? 4u * static_cast<uint32_t>(i * row2 + jc_) : 0u;                                 \
+        }                                                                                                     \
     }
-    __syncthreads();
-    // mirrored tile: Y_ji = conj(Y_ij) for i < j, row j = J0 + r, column i = I0 + tx
-    for (int r = ty; r < kTile; r += 8) {
-        const int jr = J0 + r, ic = I0 + tx;
-        if (jr >= L || ic >= L || ic >= jr) continue;
-        const size_t om = row_off<float>(b, sg, jr, L) + ic;
-        yout[om] = xs[0][0][tx][r];
-        yout[om + L] = -xs[0][1][tx][r];
+
+    // the first chunk of M and the first 16-k block of rows of the current item
+    fps_f32x4 mreg[NLD];
+    int mokb = 0;  // bit m: float4 m of the chunk in mreg lies inside its column
+    fps_f32x4 an[kWaveSlices][NA];
+    bool aok0, aok1;  // the prefetched rows of streams 0 and 1 lie inside K
+#define FPS_GROUP_FIRST_LOADS()                                                                               \
+    {                                                                                                         \
+        mokb = 0;                                                                                             \
+        _Pragma("unroll") for (int m = 0; m < NLD; ++m) {                                                     \
+            const bool ok = kq < mend[m];                                                                     \
+            mokb |= ok ? 1 << m : 0;                                                                          \
+            mreg[m] = ld4b(M, mp[m] - (ok ? 0u : 4u * kq));                                                   \
+        }                                                                                                     \
+        aok0 = 4 * q < kend0;                                                                                 \
+        aok1 = 4 * q < kend1;                                                                                 \
+        _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s)                                               \
+            _Pragma("unroll") for (int n = 0; n < NA; ++n)                                                    \
+                an[s][n] = ld4b(ys[s], ap[n] - ((n < NA / 2 ? aok0 : aok1) ? 0u : 16u * q));                  \
     }
+
+    // the RK registers and Hartree fields of this lane's elements of the current item, read ahead of its
+    // products (elements outside the upper triangle read element (0, 0) and are not used)
+    float ry[kWaveSlices][4][2], rA[kWaveSlices][4][2], rp[kWaveSlices][4][2], rhi[kWaveSlices][4], rhj[kWaveSlices];
+#define FPS_GROUP_PREFETCH_RK()                                                                               \
+    _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s) {                                                 \
+        const size_t so = static_cast<size_t>(bsl[s]) * L * row2;                                             \
+        const float* hs = hv + bsl[s] * L;                                                                    \
+        rhj[s] = hs[J0 + c16 < L ? J0 + c16 : L - 1];                                                         \
+        _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                                       \
+            const int i = I0 + 4 * q + r;                                                                     \
+            rhi[s][r] = hs[i < L ? i : L - 1];                                                                \
+            ry[s][r][0] = ld1b(y + so, eo[r]);                                                                \
+            ry[s][r][1] = ld1b(y + so, eo[r] + 4u * L);                                                       \
+            if (ST > 0) {                                                                                     \
+                rA[s][r][0] = ld1b(A + so, eo[r]);                                                            \
+                rA[s][r][1] = ld1b(A + so, eo[r] + 4u * L);                                                   \
+                rp[s][r][0] = ld1b(p + so, eo[r]);                                                            \
+                rp[s][r][1] = ld1b(p + so, eo[r] + 4u * L);                                                   \
+            } else {                                                                                          \
+                rA[s][r][0] = rA[s][r][1] = rp[s][r][0] = rp[s][r][1] = 0.f;                                  \
+            }                                                                                                 \
+        }                                                                                                     \
+    }
+
+    // One 16-k block: take the prefetched rows (waiting for them here, a block after their loads), on the
+    // first block of a chunk start the next chunk of M, prefetch the next block's rows, read the staged
+    // columns, then the products.
+#define FPS_GROUP_BLOCK(KB, MLOAD)                                                                          \
+    {                                                                                                        \
+        const int k0 = kc0 + 16 * (KB);                                                                      \
+        __builtin_amdgcn_sched_barrier(0); /* this block's waits stay behind the previous block's products */ \
+        fps_f32x4 a[kWaveSlices][NA];                                                                        \
+        _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s)                                              \
+            _Pragma("unroll") for (int n = 0; n < NA; ++n) a[s][n] = keep4(n < NA / 2 ? aok0 : aok1, an[s][n]); \
+        if (MLOAD) {                                                                                         \
+            mokb = 0;                                                                                        \
+            _Pragma("unroll") for (int m = 0; m < NLD; ++m) {                                                \
+                const bool ok = more && kc0 + kKc + kq < mend[m];                                            \
+                mokb |= ok ? 1 << m : 0;                                                                     \
+                mreg[m] = ld4b(M, mp[m] + (ok ? 4u * (kc0 + kKc) : -4u * kq));                              \
+            }                                                                                                \
+        }                                                                                                    \
+        const int kn = k0 + 16 + 4 * q;                                                                      \
+        aok0 = kn < kend0;                                                                                   \
+        aok1 = kn < kend1;                                                                                   \
+        _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s)                                              \
+            _Pragma("unroll") for (int n = 0; n < NA; ++n)                                                   \
+                an[s][n] = ld4b(ys[s], ap[n] + ((n < NA / 2 ? aok0 : aok1) ? 4u * (k0 + 16) : -16u * q));     \
+        fps_f32x4 bm[NB];                                                                                    \
+        _Pragma("unroll") for (int n = 0; n < NB; ++n) bm[n] = ld4(mb + n * kMt * kKcPad + 16 * (KB));      \
+        __builtin_amdgcn_sched_barrier(0); /* the prefetches stay ahead of this block's products */          \
+        _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s) {                                            \
+            _Pragma("unroll") for (int n = 0; n < 4; ++n) {                                                  \
+                const fps_f32x4 av = PULSE ? a[s][n >> 1] : a[s][n];                                         \
+                const fps_f32x4 bv = PULSE ? bm[n] : bm[n >> 1];                                             \
+                acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[s][n], 0, 0, 0);            \
+                acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[s][n], 0, 0, 0);            \
+                acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[s][n], 0, 0, 0);            \
+                acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[s][n], 0, 0, 0);            \
+            }                                                                                                \
+        }                                                                                                    \
+    }
+
+    FPS_GROUP_SETUP(it)
+    FPS_GROUP_FIRST_LOADS()
+    FPS_GROUP_PREFETCH_RK()
+    float(*tw)[kMt][kMt + 1] = xt[w];
+    {
+        // ---- the first chunk of M to LDS
+#pragma unroll
+        for (int m = 0; m < NLD; ++m)
+            *reinterpret_cast<fps_f32x4*>(ms0 + mst[m]) = keep4(mon[m] && ((mokb >> m) & 1), mreg[m]);
+        __syncthreads();
+        fps_f32x4 acc[kWaveSlices][4];
+#pragma unroll
+        for (int s = 0; s < kWaveSlices; ++s)
+#pragma unroll
+            for (int n = 0; n < 4; ++n) acc[s][n] = fps_f32x4{0.f, 0.f, 0.f, 0.f};
+        const int nch = (klen + kKc - 1) / kKc;
+        for (int ch = 0; ch < nch; ++ch) {
+            const int kc0 = ch * kKc;
+            const bool more = ch + 1 < nch;
+            const float* mb = ms0 + (ch & 1) * (NCOL * kKcPad) + c16 * kKcPad + 4 * q;
+            const int nkb = klen - kc0 >= kKc ? 4 : (klen - kc0 + 15) / 16;  // fewer only in the last chunk
+            FPS_GROUP_BLOCK(0, true)
+            if (nkb > 1) FPS_GROUP_BLOCK(1, false)
+            if (nkb > 2) FPS_GROUP_BLOCK(2, false)
+            if (nkb > 3) FPS_GROUP_BLOCK(3, false)
+            if (more) {
+                float* dst = ms0 + ((ch + 1) & 1) * (NCOL * kKcPad);
+#pragma unroll
+                for (int m = 0; m < NLD; ++m)
+                    *reinterpret_cast<fps_f32x4*>(dst + mst[m]) = keep4(mon[m] && ((mokb >> m) & 1), mreg[m]);
+            }
+            __syncthreads();
+        }
+        if (diag) {  // X(I, I) from the two halves of K, added in a fixed order
+#pragma unroll
+            for (int s = 0; s < kWaveSlices; ++s) {
+                acc[s][0] += acc[s][2];
+                acc[s][1] += acc[s][3];
+                acc[s][2] = acc[s][0];
+                acc[s][3] = acc[s][1];
+            }
+        }
+
+        // ---- RK update.  Accumulator register r of a lane holds row 4 (lane / 16) + r, column lane % 16.
+#pragma unroll
+        for (int s = 0; s < kWaveSlices; ++s) {
+            if (!son[s]) continue;
+            const int64_t bs = bsl[s];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {  // blocks 2, 3 hold X(J, I) (X(I, I) for a diagonal pair)
+                tw[0][4 * q + r][c16] = acc[s][2][r];
+                tw[1][4 * q + r][c16] = acc[s][3][r];
+            }
+            __builtin_amdgcn_wave_barrier();
+            float xjr[4], xji[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                xjr[r] = tw[0][c16][4 * q + r];
+                xji[r] = tw[1][c16][4 * q + r];
+            }
+            __builtin_amdgcn_wave_barrier();
+            const int j = J0 + c16;
+            const float hj = rhj[s];
+            const size_t so = static_cast<size_t>(bs) * L * row2;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = I0 + 4 * q + r;
+                float nr = 0.f, ni = 0.f;
+                if (eon[r]) {
+                    const uint32_t o = eo[r], oi = eo[r] + 4u * L;
+                    const float xr = acc[s][0][r], xi = acc[s][1][r];  // X_ij
+                    const float tr = xjr[r], ti = xji[r];              // X_ji
+                    const float yr = ry[s][r][0], yi = ry[s][r][1];
+                    float Ar = rA[s][r][0], Ai = rA[s][r][1], pr_ = rp[s][r][0], pi = rp[s][r][1];
+                    float cr, ci;  // current stage input
+                    if (ST == 0) {
+                        cr = yr;
+                        ci = yi;
+                    } else if (ST == 1) {
+                        cr = yr + h3 * pr_;
+                        ci = yi + h3 * pi;
+                    } else if (ST == 2) {
+                        cr = yr + h6 * (Ar - 3.f * pr_);
+                        ci = yi + h6 * (Ai - 3.f * pi);
+                    } else {
+                        cr = yr + h * pr_;
+                        ci = yi + h * pi;
+                    }
+                    // c = X_ij - conj(X_ji) + (h_j - h_i) Y_ij;  k = -i c
+                    const float dh = hj - rhi[s][r];
+                    const float cx = xr - tr + dh * cr, cy = xi + ti + dh * ci;
+                    float kx = cy, ky = -cx;
+                    if (i == j) ky = 0;  // the diagonal slope is real
+                    if (ST == 0) {
+                        Ar = kx;
+                        Ai = ky;
+                        pr_ = kx;
+                        pi = ky;
+                        nr = yr + h3 * kx;
+                        ni = yi + h3 * ky;
+                    } else if (ST == 1) {
+                        Ar += 3.f * kx;
+                        Ai += 3.f * ky;
+                        pr_ -= kx;
+                        pi -= ky;
+                        nr = yr + h6 * (Ar - 3.f * pr_);
+                        ni = yi + h6 * (Ai - 3.f * pi);
+                    } else if (ST == 2) {
+                        Ar += 3.f * kx;
+                        Ai += 3.f * ky;
+                        pr_ += kx;
+                        pi += ky;
+                        nr = yr + h * pr_;
+                        ni = yi + h * pi;
+                    } else {
+                        Ar += kx;
+                        Ai += ky;
+                        nr = yr + h8 * Ar;
+                        ni = yi + h8 * Ai;
+                        st1b(y + so, o, nr);
+                        st1b(y + so, oi, ni);
+                    }
+                    if (ST < 3) {
+                        st1b(A + so, o, Ar);
+                        st1b(A + so, oi, Ai);
+                        st1b(p + so, o, pr_);
+                        st1b(p + so, oi, pi);
+                    }
+                    st1b(yout + so, o, nr);
+                    st1b(yout + so, oi, ni);
+                }
+                tw[0][4 * q + r][c16] = nr;
+                tw[1][4 * q + r][c16] = ni;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // mirrored tile: Y_ji = conj(Y_ij) for i < j; this lane writes row j = J0 + 4 q + r, column i = I0 + c16
+            const int ic = I0 + c16;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int jr = J0 + 4 * q + r;
+                const float mr = tw[0][c16][4 * q + r], mi = tw[1][c16][4 * q + r];
+                if (jr < L && ic < L && ic < jr) {
+                    const uint32_t om = 4u * static_cast<uint32_t>(jr * row2 + ic);
+                    st1b(yout + so, om, mr);
+                    st1b(yout + so, om + 4u * L, -mi);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+#undef FPS_GROUP_BLOCK
+#undef FPS_GROUP_PREFETCH_RK
+#undef FPS_GROUP_FIRST_LOADS
+#undef FPS_GROUP_SETUP
 }
 
 // Hopping of the stage, T_ik for the ELL entries [z][L] (k = nb_n(i)): tM_ik (1 + e^{i a pd_ik}) with a
@@ -881,8 +1099,9 @@ void dense_set_fused_mfma(DenseCtx* c, bool on)
 
 bool dense_mfma_fits(int L, bool pulse)
 {
-    // K = 2L (pulse) or L in blocks of 8, rows of the stage input and of M 16-byte aligned
-    return L >= 4 && L <= 1024 && (pulse ? L % 4 == 0 : L % 8 == 0);
+    // rows of the stage input and of M 16-byte aligned and K = 2L (pulse) or L a multiple of 4: whole 16-byte
+    // loads of the row streams (without a pulse the imaginary plane starts L floats into a row)
+    return L >= 4 && L <= 1024 && (pulse ? L % 2 == 0 : L % 4 == 0);
 }
 
 DenseCtx* dense_create(hipStream_t st)
@@ -927,8 +1146,8 @@ size_t dense_buffer_elems(int L, int64_t B)
     // y, A, p, Ycur, X: 4 L^2 values per trajectory each; the GEMM factor (2L)^2; h: 2 L per trajectory;
     // tile pairs
     const size_t per = 4 * static_cast<size_t>(L) * L * static_cast<size_t>(B);
-    const int nt = (L + kTile - 1) / kTile;
-    const size_t pairs = static_cast<size_t>(nt) * (nt + 1) / 2;
+    const int nt = (L + kTile - 1) / kTile, nt16 = (L + kMt - 1) / kMt;
+    const size_t pairs = static_cast<size_t>(nt) * (nt + 1) / 2 + static_cast<size_t>(nt16) * (nt16 + 1) / 2;
     return 5 * per + 4 * static_cast<size_t>(L) * L + 2 * static_cast<size_t>(L) * B + pairs * 2 + 64;
 }
 
@@ -947,15 +1166,18 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     Real* M = X + per;
     Real* hv = M + 4 * static_cast<size_t>(L) * L;
     int2* pairs = reinterpret_cast<int2*>(hv + 2 * static_cast<size_t>(L) * B);
-    const int nt = (L + kTile - 1) / kTile;
-    const int npair = nt * (nt + 1) / 2;
+    const int nt = (L + kTile - 1) / kTile, nt16 = (L + kMt - 1) / kMt;
+    const int npair = nt * (nt + 1) / 2, npair16 = nt16 * (nt16 + 1) / 2;
+    int2* pairs16 = pairs + npair;  // the 16 x 16 tile pairs of the grouped matrix-core kernel
     hipStream_t st = c->stream;
     if (first) {
-        int2 hp[(1024 / kTile) * (1024 / kTile + 1) / 2];
-        int q = 0;
+        std::vector<int2> hp;
+        hp.reserve(static_cast<size_t>(npair + npair16));
         for (int I = 0; I < nt; ++I)
-            for (int J = I; J < nt; ++J) hp[q++] = int2{I, J};
-        hipError_t e = hipMemcpyAsync(pairs, hp, npair * sizeof(int2), hipMemcpyHostToDevice, st);
+            for (int J = I; J < nt; ++J) hp.push_back(int2{I, J});
+        for (int I = 0; I < nt16; ++I)
+            for (int J = I; J < nt16; ++J) hp.push_back(int2{I, J});
+        hipError_t e = hipMemcpyAsync(pairs, hp.data(), hp.size() * sizeof(int2), hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);  // hp goes out of scope
         if (e != hipSuccess) return e;
@@ -1053,14 +1275,15 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                     if (c->fused_mfma && dense_mfma_fits(L, pulse)) {
                         // product and update fused on the matrix cores; the stage input alternates
                         // between ycur and X (four swaps per step)
-                        const dim3 g1(static_cast<unsigned>(static_cast<int64_t>(npair) * B * 2));
+                        const int64_t ngrp = (2 * B + kGrpSlices - 1) / kGrpSlices;
+                        const dim3 g2(static_cast<unsigned>(ngrp * npair16));
 #define FPS_MFMA_STAGE(STG)                                                                                    \
     if (pulse)                                                                                                   \
-        hipLaunchKernelGGL((dense_mfma_update_kernel<STG, true>), g1, dim3(256), 0, st, L, B, npair, r.h, pairs, \
-                           M, hv, yin, y, A, p, yout);                                                           \
+        hipLaunchKernelGGL((dense_mfma_group_kernel<STG, true>), g2, dim3(256), 0, st, L, 2 * B, npair16, r.h,   \
+                           pairs16, M, hv, yin, y, A, p, yout);                                                  \
     else                                                                                                         \
-        hipLaunchKernelGGL((dense_mfma_update_kernel<STG, false>), g1, dim3(256), 0, st, L, B, npair, r.h,       \
-                           pairs, M, hv, yin, y, A, p, yout)
+        hipLaunchKernelGGL((dense_mfma_group_kernel<STG, false>), g2, dim3(256), 0, st, L, 2 * B, npair16, r.h,  \
+                           pairs16, M, hv, yin, y, A, p, yout)
                         switch (stg) {
                             case 0: FPS_MFMA_STAGE(0); break;
                             case 1: FPS_MFMA_STAGE(1); break;

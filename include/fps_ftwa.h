@@ -231,8 +231,9 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
 #define FPSFTWA_KERNEL_FULL 1      /* full-matrix stencil kernel                                   */
 #define FPSFTWA_KERNEL_HERMITIAN 2 /* upper-triangle kernel (L <= 64; in FP64 the spin-sequential
                                     * one, no pulse, null or on-site Coulomb)                         */
-#define FPSFTWA_KERNEL_DENSE 3     /* X = Y T per RK stage for the batch (any precision, L <= 1024);
-                                    * chosen automatically for L > 64 or more than 8 neighbours      */
+#define FPSFTWA_KERNEL_DENSE 3     /* X = Y T per RK stage for the batch (any precision, L <= 1024)
+                                    * with the library GEMM; AUTO takes the dense path for L > 64 or
+                                    * more than 8 neighbours (in FP32 as DENSE_MFMA where L allows)   */
 #define FPSFTWA_KERNEL_PIPELINED 4 /* upper-triangle kernel with the two spins half a stage apart
                                     * (FP32, L <= 64, no pulse, null or on-site Coulomb)              */
 #define FPSFTWA_KERNEL_SEQUENTIAL 5 /* upper-triangle kernel with one spin's rows in LDS at a time,
@@ -240,9 +241,11 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
                                     * with X formed in 64-column blocks; no pulse, null or on-site
                                     * Coulomb, no basis occupations)                                  */
 #define FPSFTWA_KERNEL_DENSE_MFMA 6 /* FP32 dense path with the product X = Y T and the RK update fused in
-                                    * one hand-written matrix-core kernel per stage (v_mfma_f32_32x32x2f32;
-                                    * L % 4 == 0 with a pulse, % 8 without; else as DENSE).  The default
-                                    * DENSE form (library GEMM, then the update kernel) measures faster  */
+                                    * one hand-written matrix-core kernel per stage (v_mfma_f32_16x16x4f32,
+                                    * 8 trajectory-spin slices per workgroup sharing the hopping operand
+                                    * in LDS; L even with a pulse, L % 4 == 0 without; else as DENSE).
+                                    * What AUTO runs on the dense path in FP32; DENSE keeps the library
+                                    * GEMM followed by the update kernel                                 */
 int fpsftwa_set_tuning(fpsftwa_handle* h, int32_t block_threads, int32_t time_chunk, int32_t kernel);
 
 /* Autotuner (no reference counterpart): times every batched kernel family and shape that can run

@@ -813,6 +813,23 @@ def test_long_range_hopping_general_coulomb_and_pulse_dense(oracle_mod):
             B=4)
 
 
+@pytest.mark.parametrize("nx,ny,with_pulse", [(4, 4, True), (8, 4, False), (8, 4, True)])
+def test_long_range_matrix_core_split_diagonal(oracle_mod, nx, ny, with_pulse):
+    """Long-range hopping where K (2L with a pulse, L without) is a multiple of 32, so the two halves of K
+    that the fused matrix-core kernel gives a diagonal tile pair are equal (L = 16 with a pulse, L = 32
+    with and without): the halves must still be added.  Seven trajectories (a partial group of slices)."""
+    pos, box = lat.grid(nx, ny)
+    L = pos.shape[0]
+    d = pos[:, None, :] - pos[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    tM = np.where((r > 0) & (r <= 2.5), -np.exp(-(r - 1.0) / 0.3), 0.0)
+    pulse = (dict(pulse_type=1, A0=0.5, t0=0.05, td=0.1, w0=1.5, phi=0.0, pol=(2 ** -0.5, 2 ** -0.5, 0.0), disp=d)
+             if with_pulse else None)
+    n_up = np.array([1.0, 0.0] * (L // 2))
+    compare(oracle_mod, tM, n_up, 1 - n_up, U=3.0 * np.eye(L), pot=np.linspace(-0.5, 0.5, L), pulse=pulse,
+            dt=0.01, t_max=0.05, B=7, engines=("fp32",))
+
+
 def test_long_range_hopping_without_pulse_dense(oracle_mod):
     """Long-range hopping (more than 8 neighbours per site) without a pulse, on-site U, L = 24 (a multiple
     of 8): the dense path's product of the real and imaginary planes with a real hopping matrix, through
