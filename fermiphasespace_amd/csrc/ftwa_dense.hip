@@ -1194,6 +1194,8 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     const bool whole = stencil && lds_wg <= 160 * 1024 && L * (L + 1) / 2 <= 8 * 1024;
     const size_t lds_half = 2 * static_cast<size_t>(L) * L * sizeof(Real);
     const bool two = whole && 2 * lds_half <= 160 * 1024;  // tables through the caches, two workgroups per CU
+    // the block alone fits (FP64 up to L = 101): tables through the caches, one workgroup per CU
+    const bool alone = stencil && !whole && lds_half <= 160 * 1024 && L * (L + 1) / 2 <= 8 * 1024;
     const dim3 grid(static_cast<unsigned>(npair * B), 2);
     if (first && !pulse) {
         if (stencil)
@@ -1247,7 +1249,7 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                                            dim3(grid_for(static_cast<int64_t>(s.z) * L, 256)), dim3(256), 0, st, s, g,
                                            M);
 #define FPS_STENCIL_STAGE(STG)                                                                                   \
-    if (two)                                                                                                       \
+    if (two || alone)                                                                                              \
         hipLaunchKernelGGL((dense_wg_update_kernel<Real, STG, 8, false>), dim3(static_cast<unsigned>(2 * B)),       \
                            dim3(1024), lds_half, st, L, s.z, r.h, s.nbr_k, M, hv, yin, y, A, p, yout);             \
     else if (whole)                                                                                                \
