@@ -20,6 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libfps_ftwa.so")
 CLI = os.path.join(PKG, "fps_mi355x")
+MAKE_CLI = os.path.join(PKG, "fps_make_system")  # the system generator (makeSitesSystem)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -57,7 +58,7 @@ def build(verbose: bool = False) -> str:
         ("ftwa_api.cpp", "ftwa_api.o", ["-x", "hip"]),
     ]
     cxx_units = [("json.cpp", "json.o"), ("system.cpp", "system.o"), ("eigen.cpp", "eigen.o"),
-                 ("solver.cpp", "solver.o")]
+                 ("solver.cpp", "solver.o"), ("lattice_gen.cpp", "lattice_gen.o")]
     jobs = []
     for src, obj, extra in hip_units:
         jobs.append([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", os.path.join(OUT, obj)])
@@ -75,6 +76,9 @@ def build(verbose: bool = False) -> str:
     _run([cxx, *CXX_FLAGS, os.path.join(CSRC, "fps_main.cpp"), "-o", CLI + ".tmp", "-L" + PKG, "-lfps_ftwa",
           "-Wl,-rpath,$ORIGIN"])
     os.replace(CLI + ".tmp", CLI)
+    _run([cxx, *CXX_FLAGS, os.path.join(CSRC, "make_system_main.cpp"), "-o", MAKE_CLI + ".tmp", "-L" + PKG,
+          "-lfps_ftwa", "-Wl,-rpath,$ORIGIN"])
+    os.replace(MAKE_CLI + ".tmp", MAKE_CLI)
     if verbose:
         print("built", LIB)
     return LIB

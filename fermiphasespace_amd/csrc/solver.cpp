@@ -43,49 +43,6 @@ std::vector<double> fetch(fpsftwa_handle* h, int which)
     return v;
 }
 
-Value vectorJson(const std::vector<double>& v)
-{
-    Value a = Value::array();
-    for (double x : v) a.append(x);
-    return a;
-}
-
-// [n][n] row-major -> array of rows (FHSystem::writeTBGroundStateJson writes V row by row, real parts)
-Value matrixJson(const std::vector<double>& m)
-{
-    const size_t n = static_cast<size_t>(std::llround(std::sqrt(static_cast<double>(m.size()))));
-    Value a = Value::array();
-    for (size_t i = 0; i < n; ++i) {
-        Value r = Value::array();
-        for (size_t j = 0; j < n; ++j) r.append(m[i * n + j]);
-        a.append(r);
-    }
-    return a;
-}
-
-// "tbGroundState" block (src/FHSystem.cpp:1143-1168)
-Value tbGroundStateJson(const std::vector<double>& energy, const std::vector<double>& rot)
-{
-    Value o = Value::object();
-    o["energyVector"] = vectorJson(energy);
-    o["rotationMatrix"] = matrixJson(rot);
-    return o;
-}
-
-// "mfGroundState" block (src/FHSystem.cpp:1171-1230)
-Value mfGroundStateJson(const std::vector<double>& eu, const std::vector<double>& ed, const std::vector<double>& ru,
-                        const std::vector<double>& rd, int iterations, double residue)
-{
-    Value o = Value::object();
-    o["energyVectorUp"] = vectorJson(eu);
-    o["energyVectorDown"] = vectorJson(ed);
-    o["rotationMatrixUp"] = matrixJson(ru);
-    o["rotationMatrixDown"] = matrixJson(rd);
-    o["iterationsNeeded"] = iterations;
-    o["residueComputation"] = residue;
-    return o;
-}
-
 const char* engineName(int e)
 {
     switch (e) {
@@ -253,9 +210,9 @@ void FHfTWASolver::initiateSolver()
     basisTB_ = basisMF_ = -1;
     // eigenbases the system computed for its initial state go into the results file as
     // "tbGroundState" / "mfGroundState" (src/FHSolver.cpp:177-185), as do those computed below
-    gsTB_ = s.tb_rot.empty() ? Value() : tbGroundStateJson(s.tb_energy, s.tb_rot);
+    gsTB_ = s.tb_rot.empty() ? Value() : fpsamd::tbGroundStateJson(s.tb_energy, s.tb_rot);
     gsMF_ = s.mf_rot_up.empty() ? Value()
-                                : mfGroundStateJson(s.mf_energy_up, s.mf_energy_dn, s.mf_rot_up, s.mf_rot_dn,
+                                : fpsamd::mfGroundStateJson(s.mf_energy_up, s.mf_energy_dn, s.mf_rot_up, s.mf_rot_dn,
                                                     s.mf_iterations, s.mf_residue);
     auto add_basis = [&](bool mean_field) {
         std::vector<double> ru, rd;
@@ -277,8 +234,8 @@ void FHfTWASolver::initiateSolver()
                 throw std::runtime_error("eigen-decomposition of the hopping matrix did not converge");
             if (mean_field && it >= 100)
                 std::cout << "!! the MF ground state computation has not converged. Residue=" << res << std::endl;
-            if (mean_field) gsMF_ = mfGroundStateJson(eu, ed, ru, rd, it, res);
-            else gsTB_ = tbGroundStateJson(eu, ru);
+            if (mean_field) gsMF_ = fpsamd::mfGroundStateJson(eu, ed, ru, rd, it, res);
+            else gsTB_ = fpsamd::tbGroundStateJson(eu, ru);
         }
         bases.insert(bases.end(), ru.begin(), ru.end());
         bases.insert(bases.end(), rd.begin(), rd.end());

@@ -171,6 +171,52 @@ bool addCoreHole(const fpsjson::Value& ch, FtwaSystem& s, std::string* err)
     return true;
 }
 
+namespace {
+
+fpsjson::Value vectorJson(const std::vector<double>& v)
+{
+    fpsjson::Value a = fpsjson::Value::array();
+    for (double x : v) a.append(x);
+    return a;
+}
+
+// [n][n] row-major -> array of rows (FHSystem::writeTBGroundStateJson writes V row by row, real parts)
+fpsjson::Value matrixJson(const std::vector<double>& m)
+{
+    const size_t n = static_cast<size_t>(std::llround(std::sqrt(static_cast<double>(m.size()))));
+    fpsjson::Value a = fpsjson::Value::array();
+    for (size_t i = 0; i < n; ++i) {
+        fpsjson::Value r = fpsjson::Value::array();
+        for (size_t j = 0; j < n; ++j) r.append(m[i * n + j]);
+        a.append(r);
+    }
+    return a;
+}
+
+}  // namespace
+
+fpsjson::Value tbGroundStateJson(const std::vector<double>& energy, const std::vector<double>& rot)
+{
+    fpsjson::Value o = fpsjson::Value::object();
+    o["energyVector"] = vectorJson(energy);
+    o["rotationMatrix"] = matrixJson(rot);
+    return o;
+}
+
+fpsjson::Value mfGroundStateJson(const std::vector<double>& eu, const std::vector<double>& ed,
+                                 const std::vector<double>& ru, const std::vector<double>& rd, int iterations,
+                                 double residue)
+{
+    fpsjson::Value o = fpsjson::Value::object();
+    o["energyVectorUp"] = vectorJson(eu);
+    o["energyVectorDown"] = vectorJson(ed);
+    o["rotationMatrixUp"] = matrixJson(ru);
+    o["rotationMatrixDown"] = matrixJson(rd);
+    o["iterationsNeeded"] = iterations;
+    o["residueComputation"] = residue;
+    return o;
+}
+
 bool computeGroundState(FtwaSystem& s, bool mean_field, int n_up, int n_dn, std::string* err)
 {
     const int L = s.L;

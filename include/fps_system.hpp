@@ -59,4 +59,37 @@ bool computeGroundState(FtwaSystem& s, bool mean_field, int n_up, int n_dn, std:
 // Minimum-image distance between two sites (src/FHLattice.cpp:516-539).
 double siteDistance(const FtwaSystem& s, int i, int j, double* disp3);
 
+// The "tbGroundState" (src/FHSystem.cpp:1143-1168) and "mfGroundState" (:1171-1230) blocks of a system or
+// results file: eigenvalues ascending, rotation matrices [L][L] row-major (eigenvectors in the columns).
+fpsjson::Value tbGroundStateJson(const std::vector<double>& energy, const std::vector<double>& rot);
+fpsjson::Value mfGroundStateJson(const std::vector<double>& eu, const std::vector<double>& ed,
+                                 const std::vector<double>& ru, const std::vector<double>& rd, int iterations,
+                                 double residue);
+
+// ---- the system generator of makeSitesSystem (csrc/lattice_gen.cpp; the fps_make_system CLI) ----
+
+// A generated lattice: its type name as the system file writes it ("grid", "bcc", "fcc", "triangular",
+// "honeyCombs", "graphene", "moireHC", "moire"; "kagome" and unknown names ("none") keep the reference's
+// two-site default), positions [L][3] and the box, row-major [3][3] with the lattice vectors in columns.
+struct GeneratedLattice {
+    std::string type;
+    std::vector<double> positions;
+    double matrix[9] = {0};
+};
+
+// FHLattice(std::string, std::vector<double>) (src/FHLattice.cpp:169-321): parameters padded to five
+// with 1 (makeSitesSystem.cpp:128-131) and read per type as
+//   grid, bcc, fcc, triangular: nx ny nz a;  honeyCombs: nCombs a;  graphene: nx ny a;
+//   moireHC: nCombs a b theta;  moire: nx ny a b theta (theta in degrees, b the layer spacing).
+bool generateLattice(const std::string& typeName, const std::vector<double>& parameters, GeneratedLattice& out);
+
+// The .xyz file FHLattice::writePositions writes (6 significant digits, src/FHLattice.cpp:715-738).
+bool writeXyzFile(const std::string& path, const std::vector<double>& positions, const std::string& siteName);
+
+// The simulation file makeSitesSystem writes (makeSitesSystem.cpp:170-205): method "HF", its default
+// parameters and results, and a system of nearest-neighbour hopping -1 at distance 1, on-site U = 1, no
+// external potential, with the initial condition ic ("spinWave", "chargeWave", "tbGroundState" or
+// "mfGroundState").  The ground-state blocks of the last two are added by the caller.
+fpsjson::Value sitesSystemJson(const std::string& name, const GeneratedLattice& lat, const std::string& ic);
+
 }  // namespace fpsamd

@@ -355,3 +355,147 @@ def test_core_hole_system(tmp_path):
     with pytest.raises(_lib.FtwaError) as e:
         load(tmp_path, sim, pos)
     assert "core hole" in str(e.value)
+
+
+# ---- the system generator (fps_make_system, makeSitesSystem.cpp) ----
+
+MAKE = os.path.join(ROOT, "fermiphasespace_amd", "fps_make_system")
+
+
+def make_system(tmp_path, *args):
+    """Runs fps_make_system in tmp_path; returns (simulation dict, positions [L][3] read from the .xyz)."""
+    import subprocess
+    r = subprocess.run([MAKE, *map(str, args)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    name = args[0]
+    sim = json.load(open(tmp_path / "systemsFolder" / f"{name}.json"))
+    lines = open(tmp_path / "systemsFolder" / f"{name}.xyz").read().split("\n")
+    n = int(lines[0])
+    assert lines[1] == "" and len(lines) == n + 2  # count, empty comment, one line per site, no final newline
+    pos = np.array([[float(x) for x in ln.split("\t")[1:]] for ln in lines[2:]])
+    assert all(ln.split("\t")[0] == "C" for ln in lines[2:])
+    return sim, pos, r.stdout
+
+
+def _distances(pos):
+    return np.linalg.norm(pos[:, None, :] - pos[None, :, :], axis=-1)
+
+
+def _unique_points(pts, tol=1e-6):
+    """Brute-force count of distinct points (greedy clustering, independent of the generator's order)."""
+    keep = []
+    for p in pts:
+        if not any(np.abs(p - q).sum() < tol for q in keep):
+            keep.append(p)
+    return np.array(keep)
+
+
+def test_make_system_matches_the_reference_file(tmp_path):
+    """`honeyCombs 16` (the lattice behind the reference's systemsFolder/hc16_CH.json, written by
+    makeSitesSystem): the same lattice block (box digits included), hopping, Coulomb and potential blocks;
+    16 combs of the square-as-possible sequence hold 48 distinct sites with coordination <= 3."""
+    sim, pos, out = make_system(tmp_path, "hc16", "honeyCombs", 16)
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_inputs", "hc16_CH.json")))
+    for key in ("lattice", "tunnelingInteraction", "coulombInteraction", "externalPotential"):
+        assert sim["system"][key] == ref["system"][key], key
+    assert sim["method"] == "HF" and sim["sim_name"] == "hc16"
+    assert sim["parameters"] == {"dt": 0.01, "int_order": 4, "n_traj": 1000, "stepsPerTimePoints": 1, "t_max": 2.0}
+    assert sim["system"]["IC"] == {"type": "givenDiagonal", "subtype": "spinWave"}
+    assert "== hc16 system created with 48 sites and 48 particles." in out
+    d = _distances(pos)
+    np.fill_diagonal(d, np.inf)
+    assert len(pos) == 48 and d.min() > 0.99999
+    assert ((np.abs(d - 1.0) < 1e-4).sum(axis=1) <= 3).all()
+
+
+@pytest.mark.parametrize("args,L,nn,zmax", [
+    (("grid", 3, 4, 3, 1.5), 36, 1.5, 6),
+    (("bcc", 2, 2, 2, 1.0), 27 + 8, math.sqrt(3) / 2, 8),
+    (("fcc", 2, 2, 2, 1.0), 27 + 3 * 3 * 2 * 2, 1 / math.sqrt(2), 12),
+    (("triangular", 4, 3, 1, 1.0), 12, 1.0, 6),
+    (("triangular", 3, 3, 3, 1.0), 27, 1.0, 12),
+    (("graphene", 4, 3, 1.0), 24, 1.0, 3),
+])
+def test_make_system_generators(tmp_path, args, L, nn, zmax):
+    """Site counts of the closed forms (bcc: corners + cube centres; fcc: corners + face centres), the
+    nearest-neighbour distance of each lattice (grid a, bcc a sqrt3/2, fcc a/sqrt2, triangular and
+    tetrahedral a, graphene a) and the coordination bound, with no site repeated."""
+    sim, pos, _ = make_system(tmp_path, "s", *args)
+    assert len(pos) == L
+    d = _distances(pos)
+    np.fill_diagonal(d, np.inf)
+    assert abs(d.min() - nn) < 1e-5
+    z = (np.abs(d - nn) < 1e-4).sum(axis=1)
+    assert z.max() == zmax
+    assert sim["system"]["lattice"]["latticeType"] == args[0]
+
+
+def test_make_system_honeycomb_sequence(tmp_path):
+    """The comb sequence (0,0), (1,0), (0,1), (1,1), (2,0), ... with shared corners removed: every prefix of
+    n combs holds exactly the distinct corners of those combs (brute force over the union), and the box
+    is the ceil(sqrt n) x (floor(sqrt n) + ...) parallelogram of the reference."""
+    s3 = math.sqrt(3)
+    corners = np.array([[-s3 / 2, -0.5, 0], [-s3 / 2, 0.5, 0], [0, 1, 0], [0, -1, 0], [s3 / 2, -0.5, 0],
+                        [s3 / 2, 0.5, 0]]) * 0.8
+    seq = [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (2, 1), (0, 2), (1, 2), (2, 2), (3, 0)]
+    for n in (1, 2, 5, 9, 10):
+        sim, pos, _ = make_system(tmp_path, f"h{n}", "honeyCombs", n, 0.8)
+        pts = np.vstack([corners + np.array([s3 * 0.8 * i + s3 / 2 * 0.8 * j, 1.5 * 0.8 * j, 0]) for i, j in seq[:n]])
+        want = _unique_points(pts)
+        assert len(pos) == len(want), n
+        assert all(np.abs(want - p).sum(axis=1).min() < 1e-5 for p in pos)
+        r = math.sqrt(n)
+        nx, ny = math.ceil(r), math.floor(r) + (1 if n - math.floor(r) ** 2 > math.floor(r) else 0)
+        box = np.array(sim["system"]["lattice"]["latticeMatrix"])
+        assert np.allclose(box, [[s3 * 0.8 * nx, s3 / 2 * 0.8 * ny, 0], [0, 1.5 * 0.8 * ny, 0], [0, 0, 8.0]])
+
+
+def test_make_system_bilayers(tmp_path):
+    """moire nx ny a b theta: the package's twisted-bilayer generator (the 196-site workload's) site by site;
+    moireHC n a b theta: the second flake is the first rotated by theta about z and lifted by b (a = 1),
+    each flake's sites distinct apart from the reference's unchecked first point."""
+    sim, pos, _ = make_system(tmp_path, "m", "moire", 3, 2, 1.0, 2.35, 5.0)
+    want, box = lat.moire(3, 2, 1.0, 2.35, 5.0)
+    assert np.allclose(pos, want, atol=5e-6)
+    assert np.allclose(np.array(sim["system"]["lattice"]["latticeMatrix"]), box)
+    sim, pos, _ = make_system(tmp_path, "mh", "moireHC", 7, 1.0, 3.0, 10.0)
+    half = len(pos) // 2
+    assert len(pos) == 2 * half
+    l0, l1 = pos[:half], pos[half:]
+    c, s = math.cos(math.radians(10)), math.sin(math.radians(10))
+    R = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    assert np.allclose(l0 @ R.T + [0, 0, 3.0], l1, atol=5e-6)
+    assert len(_unique_points(l0[1:], 1e-4)) == half - 1
+
+
+def test_make_system_initial_conditions(tmp_path):
+    """ic_cw: charge wave (ceil(L/2) of each spin); ic_tbgs / ic_mfgs: the ground-state blocks, whose
+    tight-binding energies are the eigenvalues of the nearest-neighbour hopping built from the written
+    positions (numpy, to 1e-12); the file loads through the JSON front end.  After a word that is not a
+    number, later numbers are not read (the reference's argument loop)."""
+    sim, pos, out = make_system(tmp_path, "cw", "graphene", 3, 2, "ic_cw")
+    assert sim["system"]["IC"] == {"type": "givenDiagonal", "subtype": "chargeWave"}
+    assert "with 12 sites and 12 particles" in out
+    s = _lib.load_system_json(str(tmp_path / "systemsFolder" / "cw.json"))
+    assert np.array_equal(s["n_up"], s["n_dn"]) and s["n_up"].sum() == 6
+    sim, pos, _ = make_system(tmp_path, "tb", "graphene", 3, 2, "ic_tbgs")
+    d = _distances(pos)
+    tM = np.where(np.abs(d - 1.0) < 1e-4, -1.0, 0.0)
+    e = np.linalg.eigvalsh(tM)
+    assert np.allclose(sim["system"]["tbGroundState"]["energyVector"], e, atol=1e-12)
+    V = np.array(sim["system"]["tbGroundState"]["rotationMatrix"])
+    assert np.allclose(V.T @ tM @ V, np.diag(e), atol=1e-10)
+    sim, pos, _ = make_system(tmp_path, "mf", "graphene", 3, 2, "ic_mfgs")
+    mf = sim["system"]["mfGroundState"]
+    assert set(mf) == {"energyVectorUp", "energyVectorDown", "rotationMatrixUp", "rotationMatrixDown",
+                       "iterationsNeeded", "residueComputation"}
+    assert mf["iterationsNeeded"] >= 1
+    sim, pos, _ = make_system(tmp_path, "q", "grid", 3, "x", 4, 2)
+    assert len(pos) == 3  # 3 x 1 x 1: the numbers after "x" are not read
+
+
+def test_make_system_without_arguments():
+    """fps_make_system alone prints the usage, as makeSitesSystem does."""
+    import subprocess
+    r = subprocess.run([MAKE], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "./makeSitesSystem <nameOfTheSystem> <geometryOfTheSites>" in r.stdout
