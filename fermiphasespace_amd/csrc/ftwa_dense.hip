@@ -41,7 +41,8 @@ __device__ __forceinline__ size_t row_off(int64_t b, int sg, int i, int L)  // s
     return ((static_cast<size_t>(b) * 2 + sg) * L + i) * 2 * static_cast<size_t>(L);
 }
 
-// interleaved [b][s][i][j] complex <-> planes [b][s][i][plane][j]; to_planes also sets Ycur = y
+// interleaved [b][s][i][j] complex <-> planes [b][s][i][plane][j]; the pack also sets Ycur = y (y may be null:
+// Ycur only)
 template <typename Real>
 __global__ void dense_pack_kernel(int L, int64_t B, const Real* inter, Real* y, Real* ycur)
 {
@@ -52,8 +53,10 @@ __global__ void dense_pack_kernel(int L, int64_t B, const Real* inter, Real* y, 
         const int ij = static_cast<int>(e - bs * L * L), i = ij / L, j = ij - i * L;
         const size_t o = (static_cast<size_t>(bs) * L + i) * 2 * L + j;
         const Real re = inter[2 * e], im = inter[2 * e + 1];
-        y[o] = re;
-        y[o + L] = im;
+        if (y) {
+            y[o] = re;
+            y[o + L] = im;
+        }
         ycur[o] = re;
         ycur[o + L] = im;
     }
@@ -285,6 +288,10 @@ constexpr int kKc = 64;           // k per LDS chunk
 constexpr int kKcPad = kKc + 4;   // row stride of the LDS chunk (17 16-byte units: conflict-free b128 reads)
 constexpr int kWaveSlices = 2;    // (trajectory, spin) slices per wave
 constexpr int kGrpSlices = 4 * kWaveSlices;
+// y, A and p of the grouped kernel are tile-major: per slice and 16 x 16 tile pair, a Re plane then an Im
+// plane of 256 floats, element (row 4 q + r, column c) of the tile at 64 r + 16 q + c.  One wave's access
+// for one r is then 256 contiguous bytes instead of 64-byte pieces of four rows.
+constexpr int kTileElems = 2 * kMt * kMt;
 
 typedef float fps_f32x4 __attribute__((ext_vector_type(4)));
 
@@ -366,6 +373,7 @@ __global__ void __launch_bounds__(256)
     bool son[kWaveSlices];
     uint32_t ap[NA];   // per-lane byte offsets of the row streams within a slice
     uint32_t eo[4];    // byte offset of this lane's element r (row I0 + 4 q + r, column J0 + c16) in a slice's planes
+    uint32_t tp;       // byte offset of this lane's element r = 0 in the slice's tile-major RK registers
     bool eon[4];       // element r in the upper triangle
 #define FPS_GROUP_SETUP(ITEM)                                                                                 \
     {                                                                                                         \
@@ -406,6 +414,7 @@ __global__ void __launch_bounds__(256)
             ap[NA - 2] = 4u * static_cast<uint32_t>(rj_ * row2 + 4 * q + kofs1_);                             \
             ap[NA - 1] = ap[NA - 2] + 4u * L;                                                                 \
         }                                                                                                     \
+        tp = 4u * static_cast<uint32_t>(pair_ * kTileElems + lane);                                           \
         const int jc_ = J0 + c16;                                                                             \
         _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                                       \
             const int i = I0 + 4 * q + r;                                                                     \
@@ -439,19 +448,20 @@ __global__ void __launch_bounds__(256)
     float ry[kWaveSlices][4][2], rA[kWaveSlices][4][2], rp[kWaveSlices][4][2], rhi[kWaveSlices][4], rhj[kWaveSlices];
 #define FPS_GROUP_PREFETCH_RK()                                                                               \
     _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s) {                                                 \
-        const size_t so = static_cast<size_t>(bsl[s]) * L * row2;                                             \
+        const size_t to = static_cast<size_t>(bsl[s]) * npair * kTileElems;                                   \
         const float* hs = hv + bsl[s] * L;                                                                    \
         rhj[s] = hs[J0 + c16 < L ? J0 + c16 : L - 1];                                                         \
         _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                                       \
             const int i = I0 + 4 * q + r;                                                                     \
             rhi[s][r] = hs[i < L ? i : L - 1];                                                                \
-            ry[s][r][0] = ld1b(y + so, eo[r]);                                                                \
-            ry[s][r][1] = ld1b(y + so, eo[r] + 4u * L);                                                       \
+            const uint32_t te = tp + 256u * r;                                                                \
+            ry[s][r][0] = ld1b(y + to, te);                                                                   \
+            ry[s][r][1] = ld1b(y + to, te + 4u * kMt * kMt);                                                  \
             if (ST > 0) {                                                                                     \
-                rA[s][r][0] = ld1b(A + so, eo[r]);                                                            \
-                rA[s][r][1] = ld1b(A + so, eo[r] + 4u * L);                                                   \
-                rp[s][r][0] = ld1b(p + so, eo[r]);                                                            \
-                rp[s][r][1] = ld1b(p + so, eo[r] + 4u * L);                                                   \
+                rA[s][r][0] = ld1b(A + to, te);                                                               \
+                rA[s][r][1] = ld1b(A + to, te + 4u * kMt * kMt);                                              \
+                rp[s][r][0] = ld1b(p + to, te);                                                               \
+                rp[s][r][1] = ld1b(p + to, te + 4u * kMt * kMt);                                              \
             } else {                                                                                          \
                 rA[s][r][0] = rA[s][r][1] = rp[s][r][0] = rp[s][r][1] = 0.f;                                  \
             }                                                                                                 \
@@ -561,12 +571,14 @@ __global__ void __launch_bounds__(256)
             const int j = J0 + c16;
             const float hj = rhj[s];
             const size_t so = static_cast<size_t>(bs) * L * row2;
+            const size_t to = static_cast<size_t>(bs) * npair * kTileElems;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = I0 + 4 * q + r;
                 float nr = 0.f, ni = 0.f;
                 if (eon[r]) {
                     const uint32_t o = eo[r], oi = eo[r] + 4u * L;
+                    const uint32_t te = tp + 256u * r, tei = te + 4u * kMt * kMt;
                     const float xr = acc[s][0][r], xi = acc[s][1][r];  // X_ij
                     const float tr = xjr[r], ti = xji[r];              // X_ji
                     const float yr = ry[s][r][0], yi = ry[s][r][1];
@@ -616,14 +628,14 @@ __global__ void __launch_bounds__(256)
                         Ai += ky;
                         nr = yr + h8 * Ar;
                         ni = yi + h8 * Ai;
-                        st1b(y + so, o, nr);
-                        st1b(y + so, oi, ni);
+                        st1b(y + to, te, nr);
+                        st1b(y + to, tei, ni);
                     }
                     if (ST < 3) {
-                        st1b(A + so, o, Ar);
-                        st1b(A + so, oi, Ai);
-                        st1b(p + so, o, pr_);
-                        st1b(p + so, oi, pi);
+                        st1b(A + to, te, Ar);
+                        st1b(A + to, tei, Ai);
+                        st1b(p + to, te, pr_);
+                        st1b(p + to, tei, pi);
                     }
                     st1b(yout + so, o, nr);
                     st1b(yout + so, oi, ni);
@@ -651,6 +663,29 @@ __global__ void __launch_bounds__(256)
 #undef FPS_GROUP_PREFETCH_RK
 #undef FPS_GROUP_FIRST_LOADS
 #undef FPS_GROUP_SETUP
+}
+
+// y of the grouped kernel from the interleaved state [b][s][i][j]: every element of the tile-major form,
+// zero outside the upper triangle.  Grid-stride over nbs * npair * 256 complex elements.
+__global__ void dense_tile_pack_kernel(int L, int64_t nbs, int npair, const int2* pairs, const float* inter,
+                                       float* yt)
+{
+    const int64_t n = nbs * npair * (kMt * kMt);
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t t = e / (kMt * kMt);  // slice * npair + pair
+        const int k = static_cast<int>(e - t * (kMt * kMt)), r = k >> 6, lane = k & 63;
+        const int2 pr = pairs[t % npair];
+        const int i = pr.x * kMt + 4 * (lane >> 4) + r, j = pr.y * kMt + (lane & 15);
+        float re = 0.f, im = 0.f;
+        if (i < L && j < L && i <= j) {
+            const int64_t o = ((t / npair) * L + i) * static_cast<int64_t>(L) + j;
+            re = inter[2 * o];
+            im = inter[2 * o + 1];
+        }
+        yt[t * kTileElems + k] = re;
+        yt[t * kTileElems + kMt * kMt + k] = im;
+    }
 }
 
 // Hopping of the stage, T_ik for the ELL entries [z][L] (k = nb_n(i)): tM_ik (1 + e^{i a pd_ik}) with a
@@ -1140,12 +1175,20 @@ bool dense_uses_gemm(const DevSystem<Real>& s)
     return !(4 * s.z < L && lds_st <= 160 * 1024);
 }
 
+// values in each of y, A, p, Ycur and X: 4 L^2 per trajectory as planes, or the tile-major form of the
+// grouped kernel's y, A and p when that is larger (L not a multiple of 16)
+static size_t dense_per(int L, int64_t B)
+{
+    const size_t nt16 = static_cast<size_t>((L + kMt - 1) / kMt);
+    return std::max(4 * static_cast<size_t>(L) * L, nt16 * (nt16 + 1) / 2 * 2 * kTileElems) *
+           static_cast<size_t>(B);
+}
+
 template <typename Real>
 size_t dense_buffer_elems(int L, int64_t B)
 {
-    // y, A, p, Ycur, X: 4 L^2 values per trajectory each; the GEMM factor (2L)^2; h: 2 L per trajectory;
-    // tile pairs
-    const size_t per = 4 * static_cast<size_t>(L) * L * static_cast<size_t>(B);
+    // y, A, p, Ycur, X (dense_per values each); the GEMM factor (2L)^2; h: 2 L per trajectory; tile pairs
+    const size_t per = dense_per(L, B);
     const int nt = (L + kTile - 1) / kTile, nt16 = (L + kMt - 1) / kMt;
     const size_t pairs = static_cast<size_t>(nt) * (nt + 1) / 2 + static_cast<size_t>(nt16) * (nt16 + 1) / 2;
     return 5 * per + 4 * static_cast<size_t>(L) * L + 2 * static_cast<size_t>(L) * B + pairs * 2 + 64;
@@ -1157,7 +1200,7 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
 {
     const int L = s.L;
     const int64_t B = r.B;
-    const size_t per = 4 * static_cast<size_t>(L) * L * static_cast<size_t>(B);
+    const size_t per = dense_per(L, B);
     Real* y = buf;
     Real* A = y + per;
     Real* p = A + per;
@@ -1170,6 +1213,10 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     const int npair = nt * (nt + 1) / 2, npair16 = nt16 * (nt16 + 1) / 2;
     int2* pairs16 = pairs + npair;  // the 16 x 16 tile pairs of the grouped matrix-core kernel
     hipStream_t st = c->stream;
+    // FP32 long-range form with the product fused on the matrix cores: y, A, p tile-major (the state is read
+    // back from Ycur, which holds it in planes after every whole step)
+    const bool tiled = std::is_same<Real, float>::value && c->fused_mfma && dense_mfma_fits(L, s.amp != nullptr) &&
+                       dense_uses_gemm(s);
     if (first) {
         std::vector<int2> hp;
         hp.reserve(static_cast<size_t>(npair + npair16));
@@ -1181,8 +1228,13 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);  // hp goes out of scope
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(dense_pack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B, state, y,
-                           ycur);
+        hipLaunchKernelGGL(dense_pack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B, state,
+                           tiled ? nullptr : y, ycur);
+        if constexpr (std::is_same<Real, float>::value) {
+            if (tiled)
+                hipLaunchKernelGGL(dense_tile_pack_kernel, dim3(grid_for(2 * B * npair16 * kMt * kMt, 256)), dim3(256),
+                                   0, st, L, 2 * B, npair16, pairs16, state, y);
+        }
     }
     const bool pulse = s.amp != nullptr;
     const size_t lds_st = 2 * kTile * 2 * static_cast<size_t>(L) * sizeof(Real) +
@@ -1274,7 +1326,7 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                     hipLaunchKernelGGL(dense_hop_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)),
                                        dim3(256), 0, st, s, g, M);
                 if constexpr (std::is_same<Real, float>::value) {
-                    if (c->fused_mfma && dense_mfma_fits(L, pulse)) {
+                    if (tiled) {
                         // product and update fused on the matrix cores; the stage input alternates
                         // between ycur and X (four swaps per step)
                         const int64_t ngrp = (2 * B + kGrpSlices - 1) / kGrpSlices;
@@ -1330,8 +1382,8 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
             }
         }
     }
-    if (last) hipLaunchKernelGGL(dense_unpack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B, y,
-                                 state);
+    if (last) hipLaunchKernelGGL(dense_unpack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B,
+                                 tiled ? ycur : y, state);
     return hipGetLastError();
 }
 
@@ -1416,7 +1468,7 @@ hipError_t dense_tune_gemm(DenseCtx* c, const DevSystem<Real>& s, int64_t B, Rea
     // solution the library offers for the shape is timed (one warm-up call, then three), and the
     // fastest is kept for later runs of this handle
     const int L = s.L;
-    const size_t per = 4 * static_cast<size_t>(L) * L * static_cast<size_t>(B);
+    const size_t per = dense_per(L, B);
     Real* ycur = buf + 3 * per;
     Real* X = ycur + per;
     Real* M = X + per;
