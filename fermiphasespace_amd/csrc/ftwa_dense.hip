@@ -107,44 +107,70 @@ __global__ void dense_hop_kernel(DevSystem<Real> s, int64_t stage, Real* M)
     }
 }
 
-// h of both spins of one trajectory per workgroup, from the diagonal of Ycur
-template <typename Real>
-__global__ void dense_hartree_kernel(DevSystem<Real> s, const Real* ycur, Real* hv)
+// h of both spins of TB trajectories per workgroup (b0 = TB blockIdx), one thread per site i:
+// h_up,i = pdiag_i + sum_j U_ij n_dn,j + V_ij n_up,j and h_dn,i with the spins exchanged (general Coulomb),
+// pdiag_i + U_ii n_sbar,i (on-site), pdiag_i (none).  UT, VT: U and V transposed, so a wave reads a row
+// of each per j, coalesced; the occupations n = Re rho_jj + 1/2 of the TB trajectories are LDS
+// broadcasts.  The diagonal comes from dg [b][s][L] when given (written with the stage input by the
+// matrix-core kernel), else from the rows of Ycur.
+template <typename Real, int TB>
+__global__ void __launch_bounds__(256)
+    dense_hartree_kernel(DevSystem<Real> s, int64_t B, const Real* UT, const Real* VT, const Real* dg,
+                         const Real* ycur, Real* hv)
 {
     extern __shared__ unsigned char dsm[];
-    Real* nd = reinterpret_cast<Real*>(dsm);  // [2][L] occupations n = Re rho_ii + 1/2
+    Real* nd = reinterpret_cast<Real*>(dsm);  // [L][TB][2]
     const int L = s.L;
-    const int64_t b = blockIdx.x;
-    for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
-        const int sg = e >= L ? 1 : 0, i = e - sg * L;
-        nd[e] = ycur[row_off<Real>(b, sg, i, L) + i] + Real(0.5);
+    const int64_t b0 = static_cast<int64_t>(blockIdx.x) * TB;
+    const int nb = static_cast<int>(B - b0 < TB ? B - b0 : TB);
+    for (int e = threadIdx.x; e < TB * 2 * L; e += blockDim.x) {
+        const int t = e / (2 * L), r = e - t * 2 * L, sg = r >= L ? 1 : 0, i = r - sg * L;
+        Real v = 0;
+        if (t < nb) v = (dg ? dg[(b0 + t) * 2 * L + r] : ycur[row_off<Real>(b0 + t, sg, i, L) + i]) + Real(0.5);
+        nd[(i * TB + t) * 2 + sg] = v;
     }
     __syncthreads();
-    if (s.coulomb_type == 2) {
-        // one wave per row of U and V, lanes along it (coalesced), both spins from the same reads:
-        // h_up,i = pdiag_i + sum_j U_ij n_dn,j + V_ij n_up,j, h_dn,i with the spins exchanged
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-        for (int i = wv; i < L; i += nwv) {
-            Real au = 0, ad = 0;
-            for (int j = lane; j < L; j += 64) {
-                const Real u = s.U[static_cast<size_t>(i) * L + j], v = s.V[static_cast<size_t>(i) * L + j];
-                au += u * nd[L + j] + v * nd[j];
-                ad += u * nd[j] + v * nd[L + j];
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        Real hu[TB], hd[TB];
+        const Real pd = s.pdiag[i];
+#pragma unroll
+        for (int t = 0; t < TB; ++t) hu[t] = hd[t] = 0;
+        if (s.coulomb_type == 2) {
+#pragma unroll 4
+            for (int j = 0; j < L; ++j) {
+                const Real u = UT[static_cast<size_t>(j) * L + i], v = VT[static_cast<size_t>(j) * L + i];
+                const Real* nj = nd + j * TB * 2;
+#pragma unroll
+                for (int t = 0; t < TB; ++t) {
+                    hu[t] += u * nj[2 * t + 1] + v * nj[2 * t];
+                    hd[t] += u * nj[2 * t] + v * nj[2 * t + 1];
+                }
             }
-            au = wave_sum(au);
-            ad = wave_sum(ad);
-            if (lane == 0) {
-                hv[(b * 2) * L + i] = s.pdiag[i] + au;
-                hv[(b * 2 + 1) * L + i] = s.pdiag[i] + ad;
+        } else if (s.coulomb_type == 1) {
+            const Real u = s.u_diag[i];
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                hu[t] = u * nd[(i * TB + t) * 2 + 1];
+                hd[t] = u * nd[(i * TB + t) * 2];
             }
         }
-        return;
+#pragma unroll
+        for (int t = 0; t < TB; ++t) {
+            if (t < nb) {
+                hv[((b0 + t) * 2) * L + i] = pd + hu[t];
+                hv[((b0 + t) * 2 + 1) * L + i] = pd + hd[t];
+            }
+        }
     }
-    for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
-        const int sg = e >= L ? 1 : 0, i = e - sg * L;
-        Real v = s.pdiag[i];
-        if (s.coulomb_type == 1) v += s.u_diag[i] * nd[(1 - sg) * L + i];
-        hv[(b * 2 + sg) * L + i] = v;
+}
+
+// T = transpose of S, both [L][L]
+template <typename Real>
+__global__ void dense_transpose_kernel(int L, const Real* S, Real* T)
+{
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L * L; e += gridDim.x * blockDim.x) {
+        const int i = e / L, j = e - i * L;
+        T[static_cast<size_t>(j) * L + i] = S[e];
     }
 }
 
@@ -278,8 +304,14 @@ __global__ void __launch_bounds__(256)
 // output block per 4 k, lane l supplying row/column l % 16 at k = k0 + 4 (l / 16) + m to instruction m of a
 // 16-k block from one 16-byte load (the same permutation of k in both factors leaves the product
 // unchanged).  The accumulators of both tiles stay in registers; X_ji is brought to the lane holding X_ij
-// through a small per-wave LDS transpose, the RK update of dense_update_kernel follows, and the new stage
-// input goes to the other buffer (its mirror through the same transpose, so rows are written contiguously).
+// through a small per-wave LDS transpose, the RK update follows, and the new stage input goes to the other
+// buffer (its mirror through the same transpose, so rows are written contiguously).
+// The update is the 3/8 rule of dense_update_kernel written around the stage input c (read from the stage
+// input itself, which the products have just brought into the caches) so that each stage touches fewer
+// registers: stage 0 keeps A = k1; stage 1 writes Y3 = c + h (k2 - 2/3 k1), p = 4/3 k1 - 2 k2 and
+// A = k1 + 3 k2; stage 2 writes Y4 = c + h (p + k3) and A += 3 k3; stage 3 writes y + h/8 (A + k4).  The
+// step result has the same single rounding at the size of y as dense_update_kernel's; only the stage
+// inputs are formed from the previous one.
 // With a pulse K = 2L: row i of the stage input is [Y_re Y_im] and M = [[Tr, Ti], [-Ti, Tr]] is symmetric
 // (T is Hermitian), so Re X_ij = row i . M row j and Im X_ij = row i . M row L + j.  Without one K = L and
 // M = T real symmetric: Re X = Y_re T, Im X = Y_im T.
@@ -325,7 +357,7 @@ __device__ __forceinline__ fps_f32x4 keep4(bool on, fps_f32x4 v)  // v or zeros,
 template <int ST, bool PULSE>
 __global__ void __launch_bounds__(256)
     dense_mfma_group_kernel(int L, int64_t nbs, int npair, double hstep, const int2* pairs, const float* M,
-                            const float* hv, const float* yin, float* y, float* A, float* p, float* yout)
+                            const float* hv, const float* yin, float* y, float* A, float* p, float* yout, float* dg)
 {
     constexpr int NCOL = PULSE ? 4 * kMt : 2 * kMt;  // staged columns of M: J (Re, Im), I (Re, Im) | J, I
     constexpr int NA = PULSE ? 2 : 4;                // row streams per slice: I, J | I re, I im, J re, J im
@@ -343,8 +375,8 @@ __global__ void __launch_bounds__(256)
     const int q = lane >> 4, c16 = lane & 15;
     const int kq = 4 * (tid & (kKc / 4 - 1));  // k offset of this thread's float4 within a chunk of M
     const float h = static_cast<float>(hstep);
-    const float h3 = static_cast<float>(hstep / 3.0), h6 = static_cast<float>(hstep / 6.0),
-                h8 = static_cast<float>(hstep / 8.0);
+    const float h3 = static_cast<float>(hstep / 3.0), h8 = static_cast<float>(hstep / 8.0);
+    const float c43 = 4.f / 3.f, c23 = 2.f / 3.f;
     float* ms0 = &ms[0][0][0];
     int mst[NLD];  // LDS float offset of this thread's float4 m within a chunk buffer
 #pragma unroll
@@ -445,7 +477,8 @@ __global__ void __launch_bounds__(256)
 
     // the RK registers and Hartree fields of this lane's elements of the current item, read ahead of its
     // products (elements outside the upper triangle read element (0, 0) and are not used)
-    float ry[kWaveSlices][4][2], rA[kWaveSlices][4][2], rp[kWaveSlices][4][2], rhi[kWaveSlices][4], rhj[kWaveSlices];
+    float rc[kWaveSlices][4][2], ry[kWaveSlices][4][2], rA[kWaveSlices][4][2], rp[kWaveSlices][4][2];
+    float rhi[kWaveSlices][4], rhj[kWaveSlices];
 #define FPS_GROUP_PREFETCH_RK()                                                                               \
     _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s) {                                                 \
         const size_t to = static_cast<size_t>(bsl[s]) * npair * kTileElems;                                   \
@@ -455,15 +488,20 @@ __global__ void __launch_bounds__(256)
             const int i = I0 + 4 * q + r;                                                                     \
             rhi[s][r] = hs[i < L ? i : L - 1];                                                                \
             const uint32_t te = tp + 256u * r;                                                                \
-            ry[s][r][0] = ld1b(y + to, te);                                                                   \
-            ry[s][r][1] = ld1b(y + to, te + 4u * kMt * kMt);                                                  \
+            rc[s][r][0] = ld1b(ys[s], eo[r]);                                                                 \
+            rc[s][r][1] = ld1b(ys[s], eo[r] + 4u * L);                                                        \
+            ry[s][r][0] = ry[s][r][1] = rA[s][r][0] = rA[s][r][1] = rp[s][r][0] = rp[s][r][1] = 0.f;          \
+            if (ST == 3) {                                                                                    \
+                ry[s][r][0] = ld1b(y + to, te);                                                               \
+                ry[s][r][1] = ld1b(y + to, te + 4u * kMt * kMt);                                              \
+            }                                                                                                 \
             if (ST > 0) {                                                                                     \
                 rA[s][r][0] = ld1b(A + to, te);                                                               \
                 rA[s][r][1] = ld1b(A + to, te + 4u * kMt * kMt);                                              \
+            }                                                                                                 \
+            if (ST == 2) {                                                                                    \
                 rp[s][r][0] = ld1b(p + to, te);                                                               \
                 rp[s][r][1] = ld1b(p + to, te + 4u * kMt * kMt);                                              \
-            } else {                                                                                          \
-                rA[s][r][0] = rA[s][r][1] = rp[s][r][0] = rp[s][r][1] = 0.f;                                  \
             }                                                                                                 \
         }                                                                                                     \
     }
@@ -581,64 +619,49 @@ __global__ void __launch_bounds__(256)
                     const uint32_t te = tp + 256u * r, tei = te + 4u * kMt * kMt;
                     const float xr = acc[s][0][r], xi = acc[s][1][r];  // X_ij
                     const float tr = xjr[r], ti = xji[r];              // X_ji
-                    const float yr = ry[s][r][0], yi = ry[s][r][1];
+                    const float cr = rc[s][r][0], ci = rc[s][r][1];  // the stage input
                     float Ar = rA[s][r][0], Ai = rA[s][r][1], pr_ = rp[s][r][0], pi = rp[s][r][1];
-                    float cr, ci;  // current stage input
-                    if (ST == 0) {
-                        cr = yr;
-                        ci = yi;
-                    } else if (ST == 1) {
-                        cr = yr + h3 * pr_;
-                        ci = yi + h3 * pi;
-                    } else if (ST == 2) {
-                        cr = yr + h6 * (Ar - 3.f * pr_);
-                        ci = yi + h6 * (Ai - 3.f * pi);
-                    } else {
-                        cr = yr + h * pr_;
-                        ci = yi + h * pi;
-                    }
                     // c = X_ij - conj(X_ji) + (h_j - h_i) Y_ij;  k = -i c
                     const float dh = hj - rhi[s][r];
                     const float cx = xr - tr + dh * cr, cy = xi + ti + dh * ci;
                     float kx = cy, ky = -cx;
                     if (i == j) ky = 0;  // the diagonal slope is real
-                    if (ST == 0) {
+                    if (ST == 0) {  // Y2 = y + h/3 k1
                         Ar = kx;
                         Ai = ky;
-                        pr_ = kx;
-                        pi = ky;
-                        nr = yr + h3 * kx;
-                        ni = yi + h3 * ky;
-                    } else if (ST == 1) {
+                        nr = cr + h3 * kx;
+                        ni = ci + h3 * ky;
+                    } else if (ST == 1) {  // Y3 = Y2 + h (k2 - 2/3 k1); p = 4/3 k1 - 2 k2 = (Y4 - Y3)/h - k3
+                        pr_ = c43 * Ar - 2.f * kx;
+                        pi = c43 * Ai - 2.f * ky;
+                        nr = cr + h * (kx - c23 * Ar);
+                        ni = ci + h * (ky - c23 * Ai);
                         Ar += 3.f * kx;
                         Ai += 3.f * ky;
-                        pr_ -= kx;
-                        pi -= ky;
-                        nr = yr + h6 * (Ar - 3.f * pr_);
-                        ni = yi + h6 * (Ai - 3.f * pi);
-                    } else if (ST == 2) {
+                    } else if (ST == 2) {  // Y4 = Y3 + h (p + k3)
+                        nr = cr + h * (pr_ + kx);
+                        ni = ci + h * (pi + ky);
                         Ar += 3.f * kx;
                         Ai += 3.f * ky;
-                        pr_ += kx;
-                        pi += ky;
-                        nr = yr + h * pr_;
-                        ni = yi + h * pi;
-                    } else {
+                    } else {  // y + h/8 (k1 + 3 k2 + 3 k3 + k4)
                         Ar += kx;
                         Ai += ky;
-                        nr = yr + h8 * Ar;
-                        ni = yi + h8 * Ai;
+                        nr = ry[s][r][0] + h8 * Ar;
+                        ni = ry[s][r][1] + h8 * Ai;
                         st1b(y + to, te, nr);
                         st1b(y + to, tei, ni);
                     }
                     if (ST < 3) {
                         st1b(A + to, te, Ar);
                         st1b(A + to, tei, Ai);
+                    }
+                    if (ST == 1) {
                         st1b(p + to, te, pr_);
                         st1b(p + to, tei, pi);
                     }
                     st1b(yout + so, o, nr);
                     st1b(yout + so, oi, ni);
+                    if (i == j) dg[bs * L + i] = nr;  // the diagonal again, compact, for the Hartree fields
                 }
                 tw[0][4 * q + r][c16] = nr;
                 tw[1][4 * q + r][c16] = ni;
@@ -961,10 +984,27 @@ __global__ void __launch_bounds__(1024)
     copy_block(yout + base, Ys, n2);
 }
 
-// records of one time point, one trajectory per workgroup (formulas of the plain path)
+// The pulse factors of the long-range energy at one time point, shared by every trajectory:
+// et[i L + j] = (tM_ij cos(a pd_ij), tM_ij sin(a pd_ij)) (cos 1, sin 0 on the diagonal).
+template <typename Real>
+__global__ void dense_energy_table_kernel(DevSystem<Real> s, int64_t stage, double2* et)
+{
+    const int L = s.L;
+    const double a = s.amp[stage];
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L * L; e += gridDim.x * blockDim.x) {
+        const int i = e / L, j = e - i * L;
+        const double tt = static_cast<double>(s.tM[e]);
+        double sn = 0.0, cs = 1.0;
+        if (i != j) sincos(a * static_cast<double>(s.pd[e]), &sn, &cs);
+        et[e] = double2{tt * cs, tt * sn};
+    }
+}
+
+// records of one time point, one trajectory per workgroup (formulas of the plain path); et: the table of
+// dense_energy_table_kernel for this time point (long-range hopping with a pulse)
 template <typename Real>
 __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real* ycur, Real* rec, int R, int dens,
-                                    int ci, int cj)
+                                    int ci, int cj, const double2* et)
 {
     __shared__ double ew[4 * 4];
     extern __shared__ double dg[];  // [2][L] n_sigma,j = rho_sigma,jj + 1/2 (general Coulomb energies)
@@ -1007,9 +1047,8 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
                 const double ni = static_cast<double>(ycur[ou + L + j]) + static_cast<double>(ycur[od + L + j]);
                 eth += nr * tt;
                 if (pulse) {
-                    double sn = 0.0, cs = 1.0;
-                    if (i != j) sincos(a * static_cast<double>(s.pd[static_cast<size_t>(i) * L + j]), &sn, &cs);
-                    epe += nr * tt * cs - ni * tt * sn;
+                    const double2 f = et[static_cast<size_t>(i) * L + j];
+                    epe += nr * f.x - ni * f.y;
                 }
             }
         }
@@ -1187,11 +1226,13 @@ static size_t dense_per(int L, int64_t B)
 template <typename Real>
 size_t dense_buffer_elems(int L, int64_t B)
 {
-    // y, A, p, Ycur, X (dense_per values each); the GEMM factor (2L)^2; h: 2 L per trajectory; tile pairs
+    // y, A, p, Ycur, X (dense_per values each); the GEMM factor (2L)^2; the energy table (L^2 double2); h:
+    // 2 L per trajectory; U and V transposed; the diagonal of the stage input (2 L per trajectory); tile pairs
     const size_t per = dense_per(L, B);
     const int nt = (L + kTile - 1) / kTile, nt16 = (L + kMt - 1) / kMt;
     const size_t pairs = static_cast<size_t>(nt) * (nt + 1) / 2 + static_cast<size_t>(nt16) * (nt16 + 1) / 2;
-    return 5 * per + 4 * static_cast<size_t>(L) * L + 2 * static_cast<size_t>(L) * B + pairs * 2 + 64;
+    return 5 * per + 4 * static_cast<size_t>(L) * L + 2 * static_cast<size_t>(L) * L * sizeof(double) / sizeof(Real) +
+           4 * static_cast<size_t>(L) * B + 2 * static_cast<size_t>(L) * L + pairs * 2 + 64;
 }
 
 template <typename Real>
@@ -1207,8 +1248,12 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     Real* ycur = p + per;
     Real* X = ycur + per;
     Real* M = X + per;
-    Real* hv = M + 4 * static_cast<size_t>(L) * L;
-    int2* pairs = reinterpret_cast<int2*>(hv + 2 * static_cast<size_t>(L) * B);
+    double2* et = reinterpret_cast<double2*>(M + 4 * static_cast<size_t>(L) * L);  // 16-byte aligned
+    Real* hv = reinterpret_cast<Real*>(et + static_cast<size_t>(L) * L);
+    Real* UT = hv + 2 * static_cast<size_t>(L) * B;
+    Real* VT = UT + static_cast<size_t>(L) * L;
+    Real* dg = VT + static_cast<size_t>(L) * L;
+    int2* pairs = reinterpret_cast<int2*>(dg + 2 * static_cast<size_t>(L) * B);
     const int nt = (L + kTile - 1) / kTile, nt16 = (L + kMt - 1) / kMt;
     const int npair = nt * (nt + 1) / 2, npair16 = nt16 * (nt16 + 1) / 2;
     int2* pairs16 = pairs + npair;  // the 16 x 16 tile pairs of the grouped matrix-core kernel
@@ -1218,6 +1263,12 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     const bool tiled = std::is_same<Real, float>::value && c->fused_mfma && dense_mfma_fits(L, s.amp != nullptr) &&
                        dense_uses_gemm(s);
     if (first) {
+        if (s.coulomb_type == 2) {
+            hipLaunchKernelGGL(dense_transpose_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)),
+                               dim3(256), 0, st, L, s.U, UT);
+            hipLaunchKernelGGL(dense_transpose_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)),
+                               dim3(256), 0, st, L, s.V, VT);
+        }
         std::vector<int2> hp;
         hp.reserve(static_cast<size_t>(npair + npair16));
         for (int I = 0; I < nt; ++I)
@@ -1284,17 +1335,30 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     // time point finds it in ycur)
     Real* yin = ycur;
     Real* yout = X;
+    // trajectories per Hartree workgroup: occupations of up to 8 in 48 KB of LDS
+    // Hartree fields: 8 trajectories per workgroup while their occupations fit 48 KB of LDS; the diagonal
+    // from dg once the matrix-core kernel has written it
+    const bool h8 = 8 * 2 * static_cast<size_t>(L) * sizeof(Real) <= 48 * 1024;
+    bool dg_ok = false;
     for (int t = t0; t < t1; ++t) {
+        if (pulse && 4 * s.z >= L)
+            hipLaunchKernelGGL(dense_energy_table_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)),
+                               dim3(256), 0, st, s, static_cast<int64_t>(t) * r.nsub * 4, et);
         hipLaunchKernelGGL(dense_record_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256),
                            s.coulomb_type == 2 ? 2 * L * sizeof(double) : 0, st, s,
                            static_cast<int64_t>(t) * r.nsub * 4, ycur, rec + static_cast<int64_t>(t - t0) * B * r.R,
-                           r.R, r.dens, r.ci, r.cj);
+                           r.R, r.dens, r.ci, r.cj, et);
         if (t == r.T - 1) break;
         for (int sub = 0; sub < r.nsub; ++sub) {
             for (int stg = 0; stg < 4; ++stg) {
                 const int64_t g = (static_cast<int64_t>(t) * r.nsub + sub) * 4 + stg;
-                hipLaunchKernelGGL(dense_hartree_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256),
-                                   2 * L * sizeof(Real), st, s, yin, hv);
+                if (h8)
+                    hipLaunchKernelGGL((dense_hartree_kernel<Real, 8>), dim3(static_cast<unsigned>((B + 7) / 8)),
+                                       dim3(256), 8 * 2 * L * sizeof(Real), st, s, B, UT, VT, dg_ok ? dg : nullptr,
+                                       yin, hv);
+                else
+                    hipLaunchKernelGGL((dense_hartree_kernel<Real, 1>), dim3(static_cast<unsigned>(B)), dim3(256),
+                                       2 * L * sizeof(Real), st, s, B, UT, VT, dg_ok ? dg : nullptr, yin, hv);
                 if (stencil) {
                     if (pulse)
                         hipLaunchKernelGGL(dense_ell_table_kernel<Real>,
@@ -1334,10 +1398,10 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
 #define FPS_MFMA_STAGE(STG)                                                                                    \
     if (pulse)                                                                                                   \
         hipLaunchKernelGGL((dense_mfma_group_kernel<STG, true>), g2, dim3(256), 0, st, L, 2 * B, npair16, r.h,   \
-                           pairs16, M, hv, yin, y, A, p, yout);                                                  \
+                           pairs16, M, hv, yin, y, A, p, yout, dg);                                              \
     else                                                                                                         \
         hipLaunchKernelGGL((dense_mfma_group_kernel<STG, false>), g2, dim3(256), 0, st, L, 2 * B, npair16, r.h,  \
-                           pairs16, M, hv, yin, y, A, p, yout)
+                           pairs16, M, hv, yin, y, A, p, yout, dg)
                         switch (stg) {
                             case 0: FPS_MFMA_STAGE(0); break;
                             case 1: FPS_MFMA_STAGE(1); break;
@@ -1348,6 +1412,7 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                         Real* tmp = yin;
                         yin = yout;
                         yout = tmp;
+                        dg_ok = true;
                         continue;
                     }
                 }
