@@ -112,12 +112,15 @@ __global__ void dense_hop_kernel(DevSystem<Real> s, int64_t stage, Real* M)
 // pdiag_i + U_ii n_sbar,i (on-site), pdiag_i (none).  UT, VT: U and V transposed, so a wave reads a row
 // of each per j, coalesced; the occupations n = Re rho_jj + 1/2 of the TB trajectories are LDS
 // broadcasts.  The diagonal comes from dg [b][s][L] when given (written with the stage input by the
-// matrix-core kernel), else from the rows of Ycur.
-template <typename Real, int TB>
+// matrix-core kernel), else from the rows of Ycur.  With ecl (general Coulomb; the stage input is a time
+// point) also the energies E_U = sum_ij U_ij (n_up,i n_dn,j + n_dn,i n_up,j) / 2 and
+// E_V = sum_ij V_ij (n_up,i n_up,j + n_dn,i n_dn,j) / 2 of the same sums, into ecl [b][2] (double).
+template <typename Real, int TB, bool ECL>
 __global__ void __launch_bounds__(256)
     dense_hartree_kernel(DevSystem<Real> s, int64_t B, const Real* UT, const Real* VT, const Real* dg,
-                         const Real* ycur, Real* hv)
+                         const Real* ycur, Real* hv, double* ecl)
 {
+    __shared__ double red[4][TB][2];
     extern __shared__ unsigned char dsm[];
     Real* nd = reinterpret_cast<Real*>(dsm);  // [L][TB][2]
     const int L = s.L;
@@ -130,12 +133,15 @@ __global__ void __launch_bounds__(256)
         nd[(i * TB + t) * 2 + sg] = v;
     }
     __syncthreads();
+    double eu[TB], ev[TB];
+#pragma unroll
+    for (int t = 0; t < TB; ++t) eu[t] = ev[t] = 0.0;
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
         Real hu[TB], hd[TB];
         const Real pd = s.pdiag[i];
 #pragma unroll
         for (int t = 0; t < TB; ++t) hu[t] = hd[t] = 0;
-        if (s.coulomb_type == 2) {
+        if (s.coulomb_type == 2 && !ECL) {
 #pragma unroll 4
             for (int j = 0; j < L; ++j) {
                 const Real u = UT[static_cast<size_t>(j) * L + i], v = VT[static_cast<size_t>(j) * L + i];
@@ -145,6 +151,31 @@ __global__ void __launch_bounds__(256)
                     hu[t] += u * nj[2 * t + 1] + v * nj[2 * t];
                     hd[t] += u * nj[2 * t] + v * nj[2 * t + 1];
                 }
+            }
+        } else if (s.coulomb_type == 2) {
+            Real ud[TB], uu[TB], vu[TB], vd[TB];  // sum_j U_ij n_dn,j, U_ij n_up,j, V_ij n_up,j, V_ij n_dn,j
+#pragma unroll
+            for (int t = 0; t < TB; ++t) ud[t] = uu[t] = vu[t] = vd[t] = 0;
+#pragma unroll 4
+            for (int j = 0; j < L; ++j) {
+                const Real u = UT[static_cast<size_t>(j) * L + i], v = VT[static_cast<size_t>(j) * L + i];
+                const Real* nj = nd + j * TB * 2;
+#pragma unroll
+                for (int t = 0; t < TB; ++t) {
+                    ud[t] += u * nj[2 * t + 1];
+                    uu[t] += u * nj[2 * t];
+                    vu[t] += v * nj[2 * t];
+                    vd[t] += v * nj[2 * t + 1];
+                }
+            }
+            const Real* ni = nd + i * TB * 2;
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                hu[t] = ud[t] + vu[t];
+                hd[t] = uu[t] + vd[t];
+                const double nu = static_cast<double>(ni[2 * t]), ndn = static_cast<double>(ni[2 * t + 1]);
+                eu[t] += (nu * static_cast<double>(ud[t]) + ndn * static_cast<double>(uu[t])) / 2.0;
+                ev[t] += (nu * static_cast<double>(vu[t]) + ndn * static_cast<double>(vd[t])) / 2.0;
             }
         } else if (s.coulomb_type == 1) {
             const Real u = s.u_diag[i];
@@ -160,6 +191,22 @@ __global__ void __launch_bounds__(256)
                 hv[((b0 + t) * 2) * L + i] = pd + hu[t];
                 hv[((b0 + t) * 2 + 1) * L + i] = pd + hd[t];
             }
+        }
+    }
+    if (ECL && s.coulomb_type == 2) {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int t = 0; t < TB; ++t) {
+            const double su = wave_sum(eu[t]), sv = wave_sum(ev[t]);
+            if (lane == 0) {
+                red[w][t][0] = su;
+                red[w][t][1] = sv;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * nb) {
+            const int t = threadIdx.x >> 1, k = threadIdx.x & 1;
+            ecl[(b0 + t) * 2 + k] = red[0][t][k] + red[1][t][k] + red[2][t][k] + red[3][t][k];
         }
     }
 }
@@ -357,7 +404,8 @@ __device__ __forceinline__ fps_f32x4 keep4(bool on, fps_f32x4 v)  // v or zeros,
 template <int ST, bool PULSE>
 __global__ void __launch_bounds__(256)
     dense_mfma_group_kernel(int L, int64_t nbs, int npair, double hstep, const int2* pairs, const float* M,
-                            const float* hv, const float* yin, float* y, float* A, float* p, float* yout, float* dg)
+                            const float* hv, const float* yin, float* y, float* A, float* p, float* yout, float* dg,
+                            const float4* ew, double* epart)
 {
     constexpr int NCOL = PULSE ? 4 * kMt : 2 * kMt;  // staged columns of M: J (Re, Im), I (Re, Im) | J, I
     constexpr int NA = PULSE ? 2 : 4;                // row streams per slice: I, J | I re, I im, J re, J im
@@ -406,6 +454,7 @@ __global__ void __launch_bounds__(256)
     uint32_t ap[NA];   // per-lane byte offsets of the row streams within a slice
     uint32_t eo[4];    // byte offset of this lane's element r (row I0 + 4 q + r, column J0 + c16) in a slice's planes
     uint32_t tp;       // byte offset of this lane's element r = 0 in the slice's tile-major RK registers
+    int pix;           // tile pair index
     bool eon[4];       // element r in the upper triangle
 #define FPS_GROUP_SETUP(ITEM)                                                                                 \
     {                                                                                                         \
@@ -447,6 +496,7 @@ __global__ void __launch_bounds__(256)
             ap[NA - 1] = ap[NA - 2] + 4u * L;                                                                 \
         }                                                                                                     \
         tp = 4u * static_cast<uint32_t>(pair_ * kTileElems + lane);                                           \
+        pix = pair_;                                                                                          \
         const int jc_ = J0 + c16;                                                                             \
         _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                                       \
             const int i = I0 + 4 * q + r;                                                                     \
@@ -479,7 +529,13 @@ __global__ void __launch_bounds__(256)
     // products (elements outside the upper triangle read element (0, 0) and are not used)
     float rc[kWaveSlices][4][2], ry[kWaveSlices][4][2], rA[kWaveSlices][4][2], rp[kWaveSlices][4][2];
     float rhi[kWaveSlices][4], rhj[kWaveSlices];
+    float4 rw[4];  // stage 3 with epart: the energy weights of this lane's elements (shared by the slices)
 #define FPS_GROUP_PREFETCH_RK()                                                                               \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r) rw[r] = float4{0.f, 0.f, 0.f, 0.f};                         \
+    if (ST == 3 && epart) {                                                                                   \
+        _Pragma("unroll") for (int r = 0; r < 4; ++r)                                                         \
+            rw[r] = ew[eon[r] ? static_cast<size_t>(I0 + 4 * q + r) * L + J0 + c16 : 0];                      \
+    }                                                                                                         \
     _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s) {                                                 \
         const size_t to = static_cast<size_t>(bsl[s]) * npair * kTileElems;                                   \
         const float* hs = hv + bsl[s] * L;                                                                    \
@@ -610,6 +666,7 @@ __global__ void __launch_bounds__(256)
             const float hj = rhj[s];
             const size_t so = static_cast<size_t>(bs) * L * row2;
             const size_t to = static_cast<size_t>(bs) * npair * kTileElems;
+            double eks = 0.0, eps = 0.0;  // stage 3 with epart: this lane's share of the hopping energies
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = I0 + 4 * q + r;
@@ -650,6 +707,10 @@ __global__ void __launch_bounds__(256)
                         ni = ry[s][r][1] + h8 * Ai;
                         st1b(y + to, te, nr);
                         st1b(y + to, tei, ni);
+                        if (epart) {
+                            eks += static_cast<double>(rw[r].x) * nr;
+                            eps += static_cast<double>(rw[r].y) * nr - static_cast<double>(rw[r].z) * ni;
+                        }
                     }
                     if (ST < 3) {
                         st1b(A + to, te, Ar);
@@ -665,6 +726,14 @@ __global__ void __launch_bounds__(256)
                 }
                 tw[0][4 * q + r][c16] = nr;
                 tw[1][4 * q + r][c16] = ni;
+            }
+            if (ST == 3 && epart) {  // one partial sum per (slice, tile pair), summed by the record in order
+                eks = wave_sum(eks);
+                eps = wave_sum(eps);
+                if (lane == 0) {
+                    epart[(bs * npair + pix) * 2] = eks;
+                    epart[(bs * npair + pix) * 2 + 1] = eps;
+                }
             }
             __builtin_amdgcn_wave_barrier();
             // mirrored tile: Y_ji = conj(Y_ij) for i < j; this lane writes row j = J0 + 4 q + r, column i = I0 + c16
@@ -1000,24 +1069,57 @@ __global__ void dense_energy_table_kernel(DevSystem<Real> s, int64_t stage, doub
     }
 }
 
-// records of one time point, one trajectory per workgroup (formulas of the plain path); et: the table of
-// dense_energy_table_kernel for this time point (long-range hopping with a pulse)
+// The hopping energies' weights over the upper triangle at one time point, for the partial sums of the
+// matrix-core kernel (rho is Hermitian, so (i, j) and (j, i) fold into one term):
+// ew[i L + j] = (tM_ij + tM_ji, f_ij + f_ji, g_ij - g_ji) for i < j and (tM_ii, tM_ii, 0) on the diagonal,
+// f + i g = tM e^{i a pd} (a = 0 without a pulse); the lower triangle is not read.
+template <typename Real>
+__global__ void dense_energy_weights_kernel(DevSystem<Real> s, int64_t stage, float4* ew)
+{
+    const int L = s.L;
+    const double a = s.amp ? s.amp[stage] : 0.0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < L * L; e += gridDim.x * blockDim.x) {
+        const int i = e / L, j = e - i * L;
+        if (i > j) continue;
+        const double tij = static_cast<double>(s.tM[e]);
+        if (i == j) {
+            ew[e] = float4{static_cast<float>(tij), static_cast<float>(tij), 0.f, 0.f};
+            continue;
+        }
+        const size_t et_ = static_cast<size_t>(j) * L + i;
+        const double tji = static_cast<double>(s.tM[et_]);
+        double sij = 0.0, cij = 1.0, sji = 0.0, cji = 1.0;
+        if (s.amp) {
+            sincos(a * static_cast<double>(s.pd[e]), &sij, &cij);
+            sincos(a * static_cast<double>(s.pd[et_]), &sji, &cji);
+        }
+        ew[e] = float4{static_cast<float>(tij + tji), static_cast<float>(tij * cij + tji * cji),
+                       static_cast<float>(tij * sij - tji * sji), 0.f};
+    }
+}
+
+// records of one time point, one trajectory per workgroup (formulas of the plain path).  et: the table of
+// dense_energy_table_kernel for this time point (long-range hopping with a pulse).  dgp (may be null): the
+// diagonal [b][s][L] of Ycur, compact.  epart (may be null): the hopping energies' partial sums of the
+// matrix-core kernel, [b][s][npart] x (kinetic, with the pulse), used instead of the rows of Ycur.  ecl: the
+// general Coulomb energies [b][2] of dense_hartree_kernel for this time point (general Coulomb only).
 template <typename Real>
 __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real* ycur, Real* rec, int R, int dens,
-                                    int ci, int cj, const double2* et)
+                                    int ci, int cj, const double2* et, const Real* dgp, const double* epart, int npart,
+                                    const double* ecl)
 {
     __shared__ double ew[4 * 4];
-    extern __shared__ double dg[];  // [2][L] n_sigma,j = rho_sigma,jj + 1/2 (general Coulomb energies)
+    extern __shared__ double dg[];  // [2][L] n_sigma,j = rho_sigma,jj + 1/2
     const int L = s.L;
     const int64_t b = blockIdx.x;
     Real* rr = rec + b * R;
     for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
         const int sg = e >= L ? 1 : 0, i = e - sg * L;
-        const Real d = ycur[row_off<Real>(b, sg, i, L) + i];
+        const Real d = dgp ? dgp[b * 2 * L + e] : ycur[row_off<Real>(b, sg, i, L) + i];
         rr[e] = d;
-        if (s.coulomb_type == 2) dg[e] = static_cast<double>(d) + 0.5;
+        dg[e] = static_cast<double>(d) + 0.5;
     }
-    if (s.coulomb_type == 2) __syncthreads();
+    __syncthreads();
     if (threadIdx.x == 0) {
         const size_t ou = row_off<Real>(b, 0, ci, L) + cj, od = row_off<Real>(b, 1, ci, L) + cj;
         rr[2 * L + 4] = ycur[ou];
@@ -1034,7 +1136,19 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
     const bool pulse = s.amp != nullptr;
     const double a = pulse ? s.amp[stage] : 0.0;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (blockDim.x + 63) / 64;
-    if (4 * s.z >= L) {
+    if (epart) {
+        // the tiles' partial sums of both spins in a fixed order, and the 1 of n = rho + 1/2 on the diagonal
+        const double* ep = epart + b * 2 * npart * 2;
+        for (int e = threadIdx.x; e < 2 * npart; e += blockDim.x) {
+            eth += ep[2 * e];
+            if (pulse) epe += ep[2 * e + 1];
+        }
+        for (int i = threadIdx.x; i < L; i += blockDim.x) {
+            const double tii = static_cast<double>(s.tM[static_cast<size_t>(i) * L + i]);
+            eth += tii;
+            if (pulse) epe += tii;
+        }
+    } else if (4 * s.z >= L) {
         // long-range hopping: rows of the dense tM, one wave per row and lanes along it, so the rows of
         // the stage input are read coalesced (one lane per row touched a cache line per neighbour and
         // fetched 17.9 GB per time point on the 196-site bilayer)
@@ -1071,21 +1185,13 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
         }
     }
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
-        const size_t ou = row_off<Real>(b, 0, i, L), od = row_off<Real>(b, 1, i, L);
-        const double nu = static_cast<double>(ycur[ou + i]) + 0.5, ndn = static_cast<double>(ycur[od + i]) + 0.5;
+        const double nu = dg[i], ndn = dg[L + i];
         epe += static_cast<double>(s.pot[i]) * (nu + ndn);
         if (s.coulomb_type == 1) eu += static_cast<double>(s.u_diag[i]) * nu * ndn;
     }
-    if (s.coulomb_type == 2) {  // rows of U and V along the lanes, occupations from LDS
-        for (int i = wv; i < L; i += nwv) {
-            const double nu = dg[i], ndn = dg[L + i];
-            for (int j = lane; j < L; j += 64) {
-                const double nuj = dg[j];
-                const double ndj = dg[L + j];
-                eu += static_cast<double>(s.U[static_cast<size_t>(i) * L + j]) * (nu * ndj + ndn * nuj) / 2.0;
-                ev += static_cast<double>(s.V[static_cast<size_t>(i) * L + j]) * (nu * nuj + ndn * ndj) / 2.0;
-            }
-        }
+    if (s.coulomb_type == 2 && threadIdx.x == 0) {  // general Coulomb: from the Hartree kernel's sums
+        eu += ecl[2 * b];
+        ev += ecl[2 * b + 1];
     }
     eth = wave_sum(eth);
     eu = wave_sum(eu);
@@ -1223,16 +1329,46 @@ static size_t dense_per(int L, int64_t B)
            static_cast<size_t>(B);
 }
 
+// Byte offsets of the dense path's buffers: y, A, p, Ycur, X (dense_per values each) and the GEMM factor
+// M ((2L)^2) back to back, then 256-byte aligned: the energy tables (L^2 double2 and L^2 float4), the
+// hopping energies' partial sums ([2B][tile pairs] x 2 doubles), the Coulomb energies ([B][2] doubles),
+// h (2 L per trajectory), U and V
+// transposed, the compact diagonal of the stage input (2 L per trajectory) and the tile pairs.
+struct DenseLayout {
+    size_t per, et, ew, ep, ecl, hv, ut, vt, dg, pairs, bytes;
+};
+
+template <typename Real>
+DenseLayout dense_layout(int L, int64_t B)
+{
+    DenseLayout d;
+    const size_t LL = static_cast<size_t>(L) * L;
+    const int nt = (L + kTile - 1) / kTile, nt16 = (L + kMt - 1) / kMt;
+    const size_t np = static_cast<size_t>(nt) * (nt + 1) / 2, np16 = static_cast<size_t>(nt16) * (nt16 + 1) / 2;
+    d.per = dense_per(L, B);
+    size_t o = (5 * d.per + 4 * LL) * sizeof(Real);
+    auto take = [&](size_t bytes) {
+        const size_t r = (o + 255) / 256 * 256;
+        o = r + bytes;
+        return r;
+    };
+    d.et = take(LL * sizeof(double2));
+    d.ew = take(LL * sizeof(float4));
+    d.ep = take(2 * static_cast<size_t>(B) * np16 * 2 * sizeof(double));
+    d.ecl = take(2 * static_cast<size_t>(B) * sizeof(double));
+    d.hv = take(2 * static_cast<size_t>(L) * B * sizeof(Real));
+    d.ut = take(LL * sizeof(Real));
+    d.vt = take(LL * sizeof(Real));
+    d.dg = take(2 * static_cast<size_t>(L) * B * sizeof(Real));
+    d.pairs = take((np + np16) * sizeof(int2));
+    d.bytes = o;
+    return d;
+}
+
 template <typename Real>
 size_t dense_buffer_elems(int L, int64_t B)
 {
-    // y, A, p, Ycur, X (dense_per values each); the GEMM factor (2L)^2; the energy table (L^2 double2); h:
-    // 2 L per trajectory; U and V transposed; the diagonal of the stage input (2 L per trajectory); tile pairs
-    const size_t per = dense_per(L, B);
-    const int nt = (L + kTile - 1) / kTile, nt16 = (L + kMt - 1) / kMt;
-    const size_t pairs = static_cast<size_t>(nt) * (nt + 1) / 2 + static_cast<size_t>(nt16) * (nt16 + 1) / 2;
-    return 5 * per + 4 * static_cast<size_t>(L) * L + 2 * static_cast<size_t>(L) * L * sizeof(double) / sizeof(Real) +
-           4 * static_cast<size_t>(L) * B + 2 * static_cast<size_t>(L) * L + pairs * 2 + 64;
+    return (dense_layout<Real>(L, B).bytes + sizeof(Real) - 1) / sizeof(Real);
 }
 
 template <typename Real>
@@ -1248,12 +1384,17 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     Real* ycur = p + per;
     Real* X = ycur + per;
     Real* M = X + per;
-    double2* et = reinterpret_cast<double2*>(M + 4 * static_cast<size_t>(L) * L);  // 16-byte aligned
-    Real* hv = reinterpret_cast<Real*>(et + static_cast<size_t>(L) * L);
-    Real* UT = hv + 2 * static_cast<size_t>(L) * B;
-    Real* VT = UT + static_cast<size_t>(L) * L;
-    Real* dg = VT + static_cast<size_t>(L) * L;
-    int2* pairs = reinterpret_cast<int2*>(dg + 2 * static_cast<size_t>(L) * B);
+    const DenseLayout lay = dense_layout<Real>(L, B);
+    char* bb = reinterpret_cast<char*>(buf);
+    double2* et = reinterpret_cast<double2*>(bb + lay.et);
+    float4* ewt = reinterpret_cast<float4*>(bb + lay.ew);
+    double* epart = reinterpret_cast<double*>(bb + lay.ep);
+    double* ecl = reinterpret_cast<double*>(bb + lay.ecl);
+    Real* hv = reinterpret_cast<Real*>(bb + lay.hv);
+    Real* UT = reinterpret_cast<Real*>(bb + lay.ut);
+    Real* VT = reinterpret_cast<Real*>(bb + lay.vt);
+    Real* dg = reinterpret_cast<Real*>(bb + lay.dg);
+    int2* pairs = reinterpret_cast<int2*>(bb + lay.pairs);
     const int nt = (L + kTile - 1) / kTile, nt16 = (L + kMt - 1) / kMt;
     const int npair = nt * (nt + 1) / 2, npair16 = nt16 * (nt16 + 1) / 2;
     int2* pairs16 = pairs + npair;  // the 16 x 16 tile pairs of the grouped matrix-core kernel
@@ -1340,25 +1481,52 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     // from dg once the matrix-core kernel has written it
     const bool h8 = 8 * 2 * static_cast<size_t>(L) * sizeof(Real) <= 48 * 1024;
     bool dg_ok = false;
+    auto hartree = [&](const Real* yst, double* ec) {
+        const Real* d = dg_ok ? dg : nullptr;
+        const unsigned g8 = static_cast<unsigned>((B + 7) / 8), g1 = static_cast<unsigned>(B);
+        const size_t l8 = 8 * 2 * L * sizeof(Real), l1 = 2 * L * sizeof(Real);
+        if (h8 && ec)
+            hipLaunchKernelGGL((dense_hartree_kernel<Real, 8, true>), g8, dim3(256), l8, st, s, B, UT, VT, d, yst, hv,
+                               ec);
+        else if (h8)
+            hipLaunchKernelGGL((dense_hartree_kernel<Real, 8, false>), g8, dim3(256), l8, st, s, B, UT, VT, d, yst, hv,
+                               ec);
+        else if (ec)
+            hipLaunchKernelGGL((dense_hartree_kernel<Real, 1, true>), g1, dim3(256), l1, st, s, B, UT, VT, d, yst, hv,
+                               ec);
+        else
+            hipLaunchKernelGGL((dense_hartree_kernel<Real, 1, false>), g1, dim3(256), l1, st, s, B, UT, VT, d, yst, hv,
+                               ec);
+    };
+    bool ep_ok = false;  // epart holds the hopping energies of this time point
     for (int t = t0; t < t1; ++t) {
-        if (pulse && 4 * s.z >= L)
+        if (pulse && 4 * s.z >= L && !ep_ok)
             hipLaunchKernelGGL(dense_energy_table_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)),
                                dim3(256), 0, st, s, static_cast<int64_t>(t) * r.nsub * 4, et);
+        // general Coulomb: the fields of the next stage (the time point's state) and the time point's
+        // Coulomb energies from the same sums
+        bool hv_ready = false;
+        if (s.coulomb_type == 2) {
+            hartree(ycur, ecl);
+            hv_ready = true;
+        }
         hipLaunchKernelGGL(dense_record_kernel<Real>, dim3(static_cast<unsigned>(B)), dim3(256),
-                           s.coulomb_type == 2 ? 2 * L * sizeof(double) : 0, st, s,
-                           static_cast<int64_t>(t) * r.nsub * 4, ycur, rec + static_cast<int64_t>(t - t0) * B * r.R,
-                           r.R, r.dens, r.ci, r.cj, et);
+                           2 * L * sizeof(double), st, s, static_cast<int64_t>(t) * r.nsub * 4, ycur,
+                           rec + static_cast<int64_t>(t - t0) * B * r.R, r.R, r.dens, r.ci, r.cj, et,
+                           dg_ok ? dg : nullptr, ep_ok ? epart : nullptr, npair16,
+                           s.coulomb_type == 2 ? ecl : nullptr);
+        ep_ok = false;
         if (t == r.T - 1) break;
+        // the next time point's hopping energies as partial sums of the matrix-core kernel's last stage
+        const bool want_ep = tiled && 4 * s.z >= L && t + 1 < t1;
+        if (want_ep)
+            hipLaunchKernelGGL(dense_energy_weights_kernel<Real>, dim3(grid_for(static_cast<int64_t>(L) * L, 256)),
+                               dim3(256), 0, st, s, static_cast<int64_t>(t + 1) * r.nsub * 4, ewt);
         for (int sub = 0; sub < r.nsub; ++sub) {
             for (int stg = 0; stg < 4; ++stg) {
                 const int64_t g = (static_cast<int64_t>(t) * r.nsub + sub) * 4 + stg;
-                if (h8)
-                    hipLaunchKernelGGL((dense_hartree_kernel<Real, 8>), dim3(static_cast<unsigned>((B + 7) / 8)),
-                                       dim3(256), 8 * 2 * L * sizeof(Real), st, s, B, UT, VT, dg_ok ? dg : nullptr,
-                                       yin, hv);
-                else
-                    hipLaunchKernelGGL((dense_hartree_kernel<Real, 1>), dim3(static_cast<unsigned>(B)), dim3(256),
-                                       2 * L * sizeof(Real), st, s, B, UT, VT, dg_ok ? dg : nullptr, yin, hv);
+                const bool erec = want_ep && sub == r.nsub - 1 && stg == 3;
+                if (!(hv_ready && sub == 0 && stg == 0)) hartree(yin, nullptr);
                 if (stencil) {
                     if (pulse)
                         hipLaunchKernelGGL(dense_ell_table_kernel<Real>,
@@ -1398,10 +1566,10 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
 #define FPS_MFMA_STAGE(STG)                                                                                    \
     if (pulse)                                                                                                   \
         hipLaunchKernelGGL((dense_mfma_group_kernel<STG, true>), g2, dim3(256), 0, st, L, 2 * B, npair16, r.h,   \
-                           pairs16, M, hv, yin, y, A, p, yout, dg);                                              \
+                           pairs16, M, hv, yin, y, A, p, yout, dg, ewt, erec ? epart : nullptr);                \
     else                                                                                                         \
         hipLaunchKernelGGL((dense_mfma_group_kernel<STG, false>), g2, dim3(256), 0, st, L, 2 * B, npair16, r.h,  \
-                           pairs16, M, hv, yin, y, A, p, yout, dg)
+                           pairs16, M, hv, yin, y, A, p, yout, dg, ewt, erec ? epart : nullptr)
                         switch (stg) {
                             case 0: FPS_MFMA_STAGE(0); break;
                             case 1: FPS_MFMA_STAGE(1); break;
@@ -1413,6 +1581,7 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                         yin = yout;
                         yout = tmp;
                         dg_ok = true;
+                        ep_ok = ep_ok || erec;
                         continue;
                     }
                 }
