@@ -41,6 +41,24 @@ __device__ __forceinline__ size_t row_off(int64_t b, int sg, int i, int L)  // s
     return ((static_cast<size_t>(b) * 2 + sg) * L + i) * 2 * static_cast<size_t>(L);
 }
 
+// The stage input of the matrix-core kernel (L even): per slice, row blocks of 16 rows (the last one padded),
+// each row [Re Y_i0 .. Re Y_i,L-1, Im Y_i0 .. Im Y_i,L-1] split into groups of 4 k, and within a row block
+// the groups in k order, each group 16 rows x 4 k.  Element (i, k) sits at ki4(i, k, L); a 16-row by 16-k
+// block is 1 KB contiguous, so a tile's reads and writes are whole cache lines.
+__host__ __device__ inline size_t ki4_slice(int L) { return 2 * static_cast<size_t>(L) * ((L + 15) & ~15); }
+__host__ __device__ inline uint32_t ki4(int i, int k, int L)
+{
+    return static_cast<uint32_t>(((i >> 4) * (L >> 1) + (k >> 2)) * 64 + (i & 15) * 4 + (k & 3));
+}
+
+// element (i, k) of slice (b, sg) of the stage input (k < L: Re Y_ik, L + k: Im Y_ik): rows of planes, or
+// (kil) the layout of the matrix-core kernel
+template <typename Real>
+__device__ __forceinline__ size_t y_off(int64_t b, int sg, int i, int k, int L, int kil)
+{
+    return kil ? (static_cast<size_t>(b) * 2 + sg) * ki4_slice(L) + ki4(i, k, L) : row_off<Real>(b, sg, i, L) + k;
+}
+
 // interleaved [b][s][i][j] complex <-> planes [b][s][i][plane][j]; the pack also sets Ycur = y (y may be null:
 // Ycur only)
 template <typename Real>
@@ -112,13 +130,13 @@ __global__ void dense_hop_kernel(DevSystem<Real> s, int64_t stage, Real* M)
 // pdiag_i + U_ii n_sbar,i (on-site), pdiag_i (none).  UT, VT: U and V transposed, so a wave reads a row
 // of each per j, coalesced; the occupations n = Re rho_jj + 1/2 of the TB trajectories are LDS
 // broadcasts.  The diagonal comes from dg [b][s][L] when given (written with the stage input by the
-// matrix-core kernel), else from the rows of Ycur.  With ecl (general Coulomb; the stage input is a time
+// matrix-core kernel), else from Ycur (layout kil as for y_off).  With ecl (general Coulomb; the stage input is a time
 // point) also the energies E_U = sum_ij U_ij (n_up,i n_dn,j + n_dn,i n_up,j) / 2 and
 // E_V = sum_ij V_ij (n_up,i n_up,j + n_dn,i n_dn,j) / 2 of the same sums, into ecl [b][2] (double).
 template <typename Real, int TB, bool ECL>
 __global__ void __launch_bounds__(256)
     dense_hartree_kernel(DevSystem<Real> s, int64_t B, const Real* UT, const Real* VT, const Real* dg,
-                         const Real* ycur, Real* hv, double* ecl)
+                         const Real* ycur, int kil, Real* hv, double* ecl)
 {
     __shared__ double red[4][TB][2];
     extern __shared__ unsigned char dsm[];
@@ -129,7 +147,7 @@ __global__ void __launch_bounds__(256)
     for (int e = threadIdx.x; e < TB * 2 * L; e += blockDim.x) {
         const int t = e / (2 * L), r = e - t * 2 * L, sg = r >= L ? 1 : 0, i = r - sg * L;
         Real v = 0;
-        if (t < nb) v = (dg ? dg[(b0 + t) * 2 * L + r] : ycur[row_off<Real>(b0 + t, sg, i, L) + i]) + Real(0.5);
+        if (t < nb) v = (dg ? dg[(b0 + t) * 2 * L + r] : ycur[y_off<Real>(b0 + t, sg, i, i, L, kil)]) + Real(0.5);
         nd[(i * TB + t) * 2 + sg] = v;
     }
     __syncthreads();
@@ -346,8 +364,9 @@ __global__ void __launch_bounds__(256)
 // 4 waves of kWaveSlices slices each.  The stage's hopping operand M is the same for every slice, so the
 // workgroup stages the columns of M it needs (those of tiles J and I, Re and Im with a pulse) through LDS in
 // chunks of kKc k, double-buffered (one barrier per chunk), and every wave reads them back for each of its
-// slices; the rows of the stage input are per slice and come straight from memory (the pairs of one group
-// run on one XCD, so after the first pair they are served from its L2).  v_mfma_f32_16x16x4f32: a 16 x 16
+// slices; the rows of the stage input are per slice and come straight from memory, in the row-block layout
+// of ki4 (a 16-k block of a tile's 16 rows is one 1 KB load per wave; the pairs of one group run on one
+// XCD, so after the first pair they are served from its L2).  v_mfma_f32_16x16x4f32: a 16 x 16
 // output block per 4 k, lane l supplying row/column l % 16 at k = k0 + 4 (l / 16) + m to instruction m of a
 // 16-k block from one 16-byte load (the same permutation of k in both factors leaves the product
 // unchanged).  The accumulators of both tiles stay in registers; X_ji is brought to the lane holding X_ij
@@ -419,7 +438,6 @@ __global__ void __launch_bounds__(256)
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: slice bases live in scalar registers
     const int K = PULSE ? 2 * L : L;
-    const int row2 = 2 * L;  // floats per row of the stage input (Re plane, Im plane)
     const int q = lane >> 4, c16 = lane & 15;
     const int kq = 4 * (tid & (kKc / 4 - 1));  // k offset of this thread's float4 within a chunk of M
     const float h = static_cast<float>(hstep);
@@ -452,7 +470,7 @@ __global__ void __launch_bounds__(256)
     int64_t bsl[kWaveSlices];
     bool son[kWaveSlices];
     uint32_t ap[NA];   // per-lane byte offsets of the row streams within a slice
-    uint32_t eo[4];    // byte offset of this lane's element r (row I0 + 4 q + r, column J0 + c16) in a slice's planes
+    uint32_t eo[4], eoi[4];  // byte offsets of Re, Im of this lane's element r (row I0 + 4 q + r, column J0 + c16)
     uint32_t tp;       // byte offset of this lane's element r = 0 in the slice's tile-major RK registers
     int pix;           // tile pair index
     bool eon[4];       // element r in the upper triangle
@@ -484,16 +502,16 @@ __global__ void __launch_bounds__(256)
             const int64_t bs = grp_ * kGrpSlices + w * kWaveSlices + s;                                       \
             son[s] = bs < nbs;                                                                                \
             bsl[s] = son[s] ? bs : 0;                                                                         \
-            ys[s] = yin + static_cast<size_t>(bsl[s]) * L * row2;                                             \
+            ys[s] = yin + static_cast<size_t>(bsl[s]) * ki4_slice(L);                                          \
         }                                                                                                     \
-        const int ri_ = I0 + c16 < L ? I0 + c16 : L - 1, rj_ = J0 + c16 < L ? J0 + c16 : L - 1;               \
-        ap[0] = 4u * static_cast<uint32_t>(ri_ * row2 + 4 * q);                                               \
+        /* rows past L lie in the padded row block; they only feed outputs that are not stored */            \
+        ap[0] = 4u * ki4(I0 + c16, 4 * q, L);                                                                 \
         if (PULSE) {                                                                                          \
-            ap[1] = 4u * static_cast<uint32_t>(rj_ * row2 + 4 * q + kofs1_);                                  \
+            ap[1] = 4u * ki4(J0 + c16, 4 * q + kofs1_, L);                                                    \
         } else {                                                                                              \
-            ap[1] = ap[0] + 4u * L;                                                                           \
-            ap[NA - 2] = 4u * static_cast<uint32_t>(rj_ * row2 + 4 * q + kofs1_);                             \
-            ap[NA - 1] = ap[NA - 2] + 4u * L;                                                                 \
+            ap[1] = 4u * ki4(I0 + c16, L + 4 * q, L);                                                         \
+            ap[NA - 2] = 4u * ki4(J0 + c16, 4 * q + kofs1_, L);                                               \
+            ap[NA - 1] = 4u * ki4(J0 + c16, L + 4 * q + kofs1_, L);                                           \
         }                                                                                                     \
         tp = 4u * static_cast<uint32_t>(pair_ * kTileElems + lane);                                           \
         pix = pair_;                                                                                          \
@@ -501,7 +519,8 @@ __global__ void __launch_bounds__(256)
         _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                                       \
             const int i = I0 + 4 * q + r;                                                                     \
             eon[r] = i < L && jc_ < L && i <= jc_;                                                            \
-            eo[r] = eon[r] ? 4u * static_cast<uint32_t>(i * row2 + jc_) : 0u;                                 \
+            eo[r] = eon[r] ? 4u * ki4(i, jc_, L) : 0u;                                                        \
+            eoi[r] = eon[r] ? 4u * ki4(i, L + jc_, L) : 0u;                                                   \
         }                                                                                                     \
     }
 
@@ -522,7 +541,7 @@ __global__ void __launch_bounds__(256)
         aok1 = 4 * q < kend1;                                                                                 \
         _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s)                                               \
             _Pragma("unroll") for (int n = 0; n < NA; ++n)                                                    \
-                an[s][n] = ld4b(ys[s], ap[n] - ((n < NA / 2 ? aok0 : aok1) ? 0u : 16u * q));                  \
+                an[s][n] = ld4b(ys[s], ap[n] - ((n < NA / 2 ? aok0 : aok1) ? 0u : 256u * q));                 \
     }
 
     // the RK registers and Hartree fields of this lane's elements of the current item, read ahead of its
@@ -545,7 +564,7 @@ __global__ void __launch_bounds__(256)
             rhi[s][r] = hs[i < L ? i : L - 1];                                                                \
             const uint32_t te = tp + 256u * r;                                                                \
             rc[s][r][0] = ld1b(ys[s], eo[r]);                                                                 \
-            rc[s][r][1] = ld1b(ys[s], eo[r] + 4u * L);                                                        \
+            rc[s][r][1] = ld1b(ys[s], eoi[r]);                                                                \
             ry[s][r][0] = ry[s][r][1] = rA[s][r][0] = rA[s][r][1] = rp[s][r][0] = rp[s][r][1] = 0.f;          \
             if (ST == 3) {                                                                                    \
                 ry[s][r][0] = ld1b(y + to, te);                                                               \
@@ -585,7 +604,7 @@ __global__ void __launch_bounds__(256)
         aok1 = kn < kend1;                                                                                   \
         _Pragma("unroll") for (int s = 0; s < kWaveSlices; ++s)                                              \
             _Pragma("unroll") for (int n = 0; n < NA; ++n)                                                   \
-                an[s][n] = ld4b(ys[s], ap[n] + ((n < NA / 2 ? aok0 : aok1) ? 4u * (k0 + 16) : -16u * q));     \
+                an[s][n] = ld4b(ys[s], ap[n] + ((n < NA / 2 ? aok0 : aok1) ? 64u * (k0 + 16) : -256u * q));   \
         fps_f32x4 bm[NB];                                                                                    \
         _Pragma("unroll") for (int n = 0; n < NB; ++n) bm[n] = ld4(mb + n * kMt * kKcPad + 16 * (KB));      \
         __builtin_amdgcn_sched_barrier(0); /* the prefetches stay ahead of this block's products */          \
@@ -664,7 +683,7 @@ __global__ void __launch_bounds__(256)
             __builtin_amdgcn_wave_barrier();
             const int j = J0 + c16;
             const float hj = rhj[s];
-            const size_t so = static_cast<size_t>(bs) * L * row2;
+            const size_t so = static_cast<size_t>(bs) * ki4_slice(L);
             const size_t to = static_cast<size_t>(bs) * npair * kTileElems;
             double eks = 0.0, eps = 0.0;  // stage 3 with epart: this lane's share of the hopping energies
 #pragma unroll
@@ -672,7 +691,7 @@ __global__ void __launch_bounds__(256)
                 const int i = I0 + 4 * q + r;
                 float nr = 0.f, ni = 0.f;
                 if (eon[r]) {
-                    const uint32_t o = eo[r], oi = eo[r] + 4u * L;
+                    const uint32_t o = eo[r], oi = eoi[r];
                     const uint32_t te = tp + 256u * r, tei = te + 4u * kMt * kMt;
                     const float xr = acc[s][0][r], xi = acc[s][1][r];  // X_ij
                     const float tr = xjr[r], ti = xji[r];              // X_ji
@@ -743,9 +762,8 @@ __global__ void __launch_bounds__(256)
                 const int jr = J0 + 4 * q + r;
                 const float mr = tw[0][c16][4 * q + r], mi = tw[1][c16][4 * q + r];
                 if (jr < L && ic < L && ic < jr) {
-                    const uint32_t om = 4u * static_cast<uint32_t>(jr * row2 + ic);
-                    st1b(yout + so, om, mr);
-                    st1b(yout + so, om + 4u * L, -mi);
+                    st1b(yout + so, 4u * ki4(jr, ic, L), mr);
+                    st1b(yout + so, 4u * ki4(jr, L + ic, L), -mi);
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -777,6 +795,37 @@ __global__ void dense_tile_pack_kernel(int L, int64_t nbs, int npair, const int2
         }
         yt[t * kTileElems + k] = re;
         yt[t * kTileElems + kMt * kMt + k] = im;
+    }
+}
+
+// the stage input of the matrix-core kernel from the interleaved state [b][s][i][j] (zeros in the padding
+// rows of the last row block)
+__global__ void dense_pack_ki4_kernel(int L, int64_t nbs, const float* inter, float* yk)
+{
+    const int L16 = (L + 15) & ~15;
+    const int64_t n = nbs * L16 * 2 * static_cast<int64_t>(L);
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t bs = e / (static_cast<int64_t>(L16) * 2 * L);
+        const int ik = static_cast<int>(e - bs * L16 * 2 * L), i = ik / (2 * L), k = ik - i * 2 * L;
+        const int j = k < L ? k : k - L;
+        const float v = i < L ? inter[2 * ((bs * L + i) * static_cast<int64_t>(L) + j) + (k < L ? 0 : 1)] : 0.f;
+        yk[bs * ki4_slice(L) + ki4(i, k, L)] = v;
+    }
+}
+
+// the interleaved state from the upper triangle of that stage input (the lower is its mirror)
+__global__ void dense_unpack_ki4_kernel(int L, int64_t nbs, const float* yk, float* inter)
+{
+    const int64_t n = nbs * static_cast<int64_t>(L) * L;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t bs = e / (static_cast<int64_t>(L) * L);
+        const int ij = static_cast<int>(e - bs * L * L), i = ij / L, j = ij - i * L;
+        const int a = i <= j ? i : j, c = i <= j ? j : i;
+        const float* ys = yk + bs * ki4_slice(L);
+        inter[2 * e] = ys[ki4(a, c, L)];
+        inter[2 * e + 1] = i == j ? 0.f : (i < j ? 1.f : -1.f) * ys[ki4(a, L + c, L)];
     }
 }
 
@@ -1102,11 +1151,12 @@ __global__ void dense_energy_weights_kernel(DevSystem<Real> s, int64_t stage, fl
 // dense_energy_table_kernel for this time point (long-range hopping with a pulse).  dgp (may be null): the
 // diagonal [b][s][L] of Ycur, compact.  epart (may be null): the hopping energies' partial sums of the
 // matrix-core kernel, [b][s][npart] x (kinetic, with the pulse), used instead of the rows of Ycur.  ecl: the
-// general Coulomb energies [b][2] of dense_hartree_kernel for this time point (general Coulomb only).
+// general Coulomb energies [b][2] of dense_hartree_kernel for this time point (general Coulomb only).  kil: the
+// layout of Ycur, as for y_off.
 template <typename Real>
 __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real* ycur, Real* rec, int R, int dens,
                                     int ci, int cj, const double2* et, const Real* dgp, const double* epart, int npart,
-                                    const double* ecl)
+                                    const double* ecl, int kil)
 {
     __shared__ double ew[4 * 4];
     extern __shared__ double dg[];  // [2][L] n_sigma,j = rho_sigma,jj + 1/2
@@ -1115,22 +1165,22 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
     Real* rr = rec + b * R;
     for (int e = threadIdx.x; e < 2 * L; e += blockDim.x) {
         const int sg = e >= L ? 1 : 0, i = e - sg * L;
-        const Real d = dgp ? dgp[b * 2 * L + e] : ycur[row_off<Real>(b, sg, i, L) + i];
+        const Real d = dgp ? dgp[b * 2 * L + e] : ycur[y_off<Real>(b, sg, i, i, L, kil)];
         rr[e] = d;
         dg[e] = static_cast<double>(d) + 0.5;
     }
     __syncthreads();
+    auto Y = [&](int sg, int i, int k) { return ycur[y_off<Real>(b, sg, i, k, L, kil)]; };
     if (threadIdx.x == 0) {
-        const size_t ou = row_off<Real>(b, 0, ci, L) + cj, od = row_off<Real>(b, 1, ci, L) + cj;
-        rr[2 * L + 4] = ycur[ou];
-        rr[2 * L + 5] = ycur[ou + L];
-        rr[2 * L + 6] = ycur[od];
-        rr[2 * L + 7] = ycur[od + L];
+        rr[2 * L + 4] = Y(0, ci, cj);
+        rr[2 * L + 5] = Y(0, ci, L + cj);
+        rr[2 * L + 6] = Y(1, ci, cj);
+        rr[2 * L + 7] = Y(1, ci, L + cj);
     }
     if (dens)  // block of 256 threads: 4 waves
         dev::record_density(rr + 2 * L + 8, L, threadIdx.x & 63, threadIdx.x >> 6, 4, [&](int row, int col) {
             const int sg = row >= L ? 1 : 0;
-            return ycur[row_off<Real>(b, sg, row - sg * L, L) + col];
+            return Y(sg, row - sg * L, col);
         });
     double eth = 0, eu = 0, ev = 0, epe = 0;
     const bool pulse = s.amp != nullptr;
@@ -1153,12 +1203,12 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
         // the stage input are read coalesced (one lane per row touched a cache line per neighbour and
         // fetched 17.9 GB per time point on the 196-site bilayer)
         for (int i = wv; i < L; i += nwv) {
-            const size_t ou = row_off<Real>(b, 0, i, L), od = row_off<Real>(b, 1, i, L);
             for (int j = lane; j < L; j += 64) {
                 const double tt = static_cast<double>(s.tM[static_cast<size_t>(i) * L + j]);
                 if (tt == 0.0) continue;
-                const double nr = static_cast<double>(ycur[ou + j]) + static_cast<double>(ycur[od + j]) + (i == j ? 1.0 : 0.0);
-                const double ni = static_cast<double>(ycur[ou + L + j]) + static_cast<double>(ycur[od + L + j]);
+                const double nr =
+                    static_cast<double>(Y(0, i, j)) + static_cast<double>(Y(1, i, j)) + (i == j ? 1.0 : 0.0);
+                const double ni = static_cast<double>(Y(0, i, L + j)) + static_cast<double>(Y(1, i, L + j));
                 eth += nr * tt;
                 if (pulse) {
                     const double2 f = et[static_cast<size_t>(i) * L + j];
@@ -1168,13 +1218,13 @@ __global__ void dense_record_kernel(DevSystem<Real> s, int64_t stage, const Real
         }
     } else {
         for (int i = threadIdx.x; i < L; i += blockDim.x) {
-            const size_t ou = row_off<Real>(b, 0, i, L), od = row_off<Real>(b, 1, i, L);
             for (int n = 0; n <= s.z; ++n) {  // the neighbour list of i, then the diagonal entry
                 const int j = n < s.z ? s.nbr_k[n * L + i] : i;
                 const double tt = n < s.z ? static_cast<double>(s.nbr_t[n * L + i]) : static_cast<double>(s.tdiag[i]);
                 if (tt == 0.0) continue;
-                const double nr = static_cast<double>(ycur[ou + j]) + static_cast<double>(ycur[od + j]) + (i == j ? 1.0 : 0.0);
-                const double ni = static_cast<double>(ycur[ou + L + j]) + static_cast<double>(ycur[od + L + j]);
+                const double nr =
+                    static_cast<double>(Y(0, i, j)) + static_cast<double>(Y(1, i, j)) + (i == j ? 1.0 : 0.0);
+                const double ni = static_cast<double>(Y(0, i, L + j)) + static_cast<double>(Y(1, i, L + j));
                 eth += nr * tt;
                 if (pulse) {
                     double sn = 0.0, cs = 1.0;
@@ -1320,12 +1370,12 @@ bool dense_uses_gemm(const DevSystem<Real>& s)
     return !(4 * s.z < L && lds_st <= 160 * 1024);
 }
 
-// values in each of y, A, p, Ycur and X: 4 L^2 per trajectory as planes, or the tile-major form of the
-// grouped kernel's y, A and p when that is larger (L not a multiple of 16)
+// values in each of y, A, p, Ycur and X: 4 L^2 per trajectory as planes; the matrix-core kernel's stage
+// input (row blocks padded to 16 rows) and its tile-major y, A and p can take more
 static size_t dense_per(int L, int64_t B)
 {
     const size_t nt16 = static_cast<size_t>((L + kMt - 1) / kMt);
-    return std::max(4 * static_cast<size_t>(L) * L, nt16 * (nt16 + 1) / 2 * 2 * kTileElems) *
+    return std::max(4 * static_cast<size_t>(L) * (nt16 * kMt), nt16 * (nt16 + 1) / 2 * 2 * kTileElems) *
            static_cast<size_t>(B);
 }
 
@@ -1420,13 +1470,17 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
         if (e != hipSuccess) return e;
         e = hipStreamSynchronize(st);  // hp goes out of scope
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(dense_pack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B, state,
-                           tiled ? nullptr : y, ycur);
         if constexpr (std::is_same<Real, float>::value) {
-            if (tiled)
+            if (tiled) {
+                hipLaunchKernelGGL(dense_pack_ki4_kernel, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, 2 * B,
+                                   state, ycur);
                 hipLaunchKernelGGL(dense_tile_pack_kernel, dim3(grid_for(2 * B * npair16 * kMt * kMt, 256)), dim3(256),
                                    0, st, L, 2 * B, npair16, pairs16, state, y);
+            }
         }
+        if (!tiled)
+            hipLaunchKernelGGL(dense_pack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B, state, y,
+                               ycur);
     }
     const bool pulse = s.amp != nullptr;
     const size_t lds_st = 2 * kTile * 2 * static_cast<size_t>(L) * sizeof(Real) +
@@ -1483,20 +1537,21 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
     bool dg_ok = false;
     auto hartree = [&](const Real* yst, double* ec) {
         const Real* d = dg_ok ? dg : nullptr;
+        const int kil = tiled ? 1 : 0;
         const unsigned g8 = static_cast<unsigned>((B + 7) / 8), g1 = static_cast<unsigned>(B);
         const size_t l8 = 8 * 2 * L * sizeof(Real), l1 = 2 * L * sizeof(Real);
         if (h8 && ec)
-            hipLaunchKernelGGL((dense_hartree_kernel<Real, 8, true>), g8, dim3(256), l8, st, s, B, UT, VT, d, yst, hv,
-                               ec);
+            hipLaunchKernelGGL((dense_hartree_kernel<Real, 8, true>), g8, dim3(256), l8, st, s, B, UT, VT, d, yst, kil,
+                               hv, ec);
         else if (h8)
-            hipLaunchKernelGGL((dense_hartree_kernel<Real, 8, false>), g8, dim3(256), l8, st, s, B, UT, VT, d, yst, hv,
-                               ec);
+            hipLaunchKernelGGL((dense_hartree_kernel<Real, 8, false>), g8, dim3(256), l8, st, s, B, UT, VT, d, yst,
+                               kil, hv, ec);
         else if (ec)
-            hipLaunchKernelGGL((dense_hartree_kernel<Real, 1, true>), g1, dim3(256), l1, st, s, B, UT, VT, d, yst, hv,
-                               ec);
+            hipLaunchKernelGGL((dense_hartree_kernel<Real, 1, true>), g1, dim3(256), l1, st, s, B, UT, VT, d, yst, kil,
+                               hv, ec);
         else
-            hipLaunchKernelGGL((dense_hartree_kernel<Real, 1, false>), g1, dim3(256), l1, st, s, B, UT, VT, d, yst, hv,
-                               ec);
+            hipLaunchKernelGGL((dense_hartree_kernel<Real, 1, false>), g1, dim3(256), l1, st, s, B, UT, VT, d, yst,
+                               kil, hv, ec);
     };
     bool ep_ok = false;  // epart holds the hopping energies of this time point
     for (int t = t0; t < t1; ++t) {
@@ -1514,7 +1569,7 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                            2 * L * sizeof(double), st, s, static_cast<int64_t>(t) * r.nsub * 4, ycur,
                            rec + static_cast<int64_t>(t - t0) * B * r.R, r.R, r.dens, r.ci, r.cj, et,
                            dg_ok ? dg : nullptr, ep_ok ? epart : nullptr, npair16,
-                           s.coulomb_type == 2 ? ecl : nullptr);
+                           s.coulomb_type == 2 ? ecl : nullptr, tiled ? 1 : 0);
         ep_ok = false;
         if (t == r.T - 1) break;
         // the next time point's hopping energies as partial sums of the matrix-core kernel's last stage
@@ -1597,27 +1652,35 @@ hipError_t dense_run(DenseCtx* c, const DevSystem<Real>& s, const RunShape& r, R
                 if (bs != rocblas_status_success) return hipErrorLaunchFailure;
                 switch (stg) {
                     case 0:
-                        hipLaunchKernelGGL((dense_update_kernel<Real, 0>), grid, dim3(256), 0, st, L, npair, r.h, pairs, X,
-                                           hv, y, A, p, ycur);
+                        hipLaunchKernelGGL((dense_update_kernel<Real, 0>), grid, dim3(256), 0, st, L, npair, r.h,
+                                           pairs, X, hv, y, A, p, ycur);
                         break;
                     case 1:
-                        hipLaunchKernelGGL((dense_update_kernel<Real, 1>), grid, dim3(256), 0, st, L, npair, r.h, pairs, X,
-                                           hv, y, A, p, ycur);
+                        hipLaunchKernelGGL((dense_update_kernel<Real, 1>), grid, dim3(256), 0, st, L, npair, r.h,
+                                           pairs, X, hv, y, A, p, ycur);
                         break;
                     case 2:
-                        hipLaunchKernelGGL((dense_update_kernel<Real, 2>), grid, dim3(256), 0, st, L, npair, r.h, pairs, X,
-                                           hv, y, A, p, ycur);
+                        hipLaunchKernelGGL((dense_update_kernel<Real, 2>), grid, dim3(256), 0, st, L, npair, r.h,
+                                           pairs, X, hv, y, A, p, ycur);
                         break;
                     default:
-                        hipLaunchKernelGGL((dense_update_kernel<Real, 3>), grid, dim3(256), 0, st, L, npair, r.h, pairs, X,
-                                           hv, y, A, p, ycur);
+                        hipLaunchKernelGGL((dense_update_kernel<Real, 3>), grid, dim3(256), 0, st, L, npair, r.h,
+                                           pairs, X, hv, y, A, p, ycur);
                         break;
                 }
             }
         }
     }
-    if (last) hipLaunchKernelGGL(dense_unpack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B,
-                                 tiled ? ycur : y, state);
+    if (last) {
+        if constexpr (std::is_same<Real, float>::value) {
+            if (tiled)
+                hipLaunchKernelGGL(dense_unpack_ki4_kernel, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, 2 * B,
+                                   ycur, state);
+        }
+        if (!tiled)
+            hipLaunchKernelGGL(dense_unpack_kernel<Real>, dim3(grid_for(per / 2, 256)), dim3(256), 0, st, L, B, y,
+                               state);
+    }
     return hipGetLastError();
 }
 
