@@ -1,19 +1,20 @@
 // ftwa_herm_wide.hip -- batched FP32 fTWA integrator on the upper triangle for 64 < L <= 107: one spin's
-// rows in LDS at a time, X = T Y kept for one block of 64 columns at a time.
+// rows in LDS at a time, the commutator of each element formed directly from them.
 //
 // Same physics, records and RK4 "3/8" register form (y, A, p) as the spin-sequential kernel of
 // ftwa_herm_seq.hip (src/FHfTWASolver.cpp:366-426 records, :568-600 derivative, :629-661 RK4 3/8).  Past
-// L = 64 the one-spin [Y | X] rows no longer fit LDS (160.8 KB at L = 100), so X is formed for a block of
-// 64 columns at a time and each slot accumulates its part of conj(X_ji) - X_ij:
+// L = 64 the one-spin [Y | X] rows of that kernel no longer fit LDS (160.8 KB at L = 100), so X = T Y is
+// not stored: each element slot sums its own neighbour terms from the rows of Y,
 //   M    the spin's stage input Y = f(y, A, p) from the RK registers (upper triangle, mirror, diagonal;
 //        in the spin-up pass also the Hartree fields of both spins and the stage-input diagonals)
-//   A_b  X[:, 64b .. 64b+63] = T Y[:, 64b .. 64b+63], lane = row (two rows per lane, j and j + 64)
-//   B_b  c_ij += conj(X_ji) - X_ij for the parts in block b; after the last block
-//        k_ij = -i [c_ij + (h_j - h_i) Y_ij], k_ii = -2 Im X_ii (in the block holding column i)
-// with barriers after M, A_0, B_0 and A_1.  LDS: rows [Y (L complex) | X block (64 complex) | pad] with an
-// odd row length, so the lanes of phase A (rows k, one column) fall on distinct bank pairs.  One 8-wave
-// workgroup per CU (up to 256 VGPRs, S <= 11 slots per thread).  Element slots in class order
-// (dev::class_slot).  Null or on-site Coulomb interactions, no pulse, no density records.
+//   C    c_ij = sum_n t_{j,n} Y_{i,k_n(j)} - sum_n t_{i,n} Y_{k_n(i),j} + (h_j - h_i) Y_ij
+//        (the first sum is conj(X_ji), Y being Hermitian; the second is X_ij), k_ij = -i c_ij,
+//        k_ii = -2 Im X_ii, and the RK registers
+// with a barrier after M and one after C (before the next M rewrites the rows).  Per element that is the
+// same 2z complex x real multiply-adds as forming X, with the neighbour tables (row offsets, column
+// offsets, hoppings) read from LDS per slot.  LDS: rows of Y with an odd row length.  One 8-wave workgroup
+// per CU (up to 256 VGPRs, S <= 12 slots per thread).  Element slots in class order (dev::class_slot).
+// Null or on-site Coulomb interactions, no pulse, no density records.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,7 +37,8 @@ constexpr int kCB = 8;           // bytes per complex
 constexpr int kRB = 4;           // bytes per real
 // LDS layout: h [2 stage parities][2 spins][144], stage-input diagonals [2][144], energy partials
 // [8 waves][4], site tables tM_ii / E_pe potential / U_ii / pdiag [4][128], neighbour hoppings [8][128],
-// neighbour row offsets [8][128] (ints), 16 zero bytes, then L rows and a spare row.
+// neighbour row offsets [8][128] and column offsets [8][128] (ints), 16 spare bytes, then L rows and a
+// spare row.
 constexpr int kHSpin = 144 * kRB;  // entries 128..143 take padding slots (always zero)
 constexpr int kHPar = 2 * kHSpin;
 constexpr int kHB = 0;
@@ -45,13 +47,14 @@ constexpr int kEW = kDg + kHPar;
 constexpr int kSite = kEW + kW * 4 * kRB;
 constexpr int kNbrT = kSite + 4 * kLP * kRB;
 constexpr int kNbrK = kNbrT + 8 * kLP * kRB;
-constexpr int kZero = kNbrK + 8 * kLP * 4;
+constexpr int kNbrC = kNbrK + 8 * kLP * 4;
+constexpr int kZero = kNbrC + 8 * kLP * 4;
 constexpr int kRows = kZero + 16;
 static_assert(kRows % 16 == 0, "rows are 16-byte aligned");
 constexpr size_t kLdsLimit = 160 * 1024;
 
-// complex elements per LDS row: L of Y, 64 of the X block, padding to an odd length
-__host__ __device__ inline int wide_row_len(int L) { return (L + 65) | 1; }
+// complex elements per LDS row: L of Y and at least one more (the padding slots' column L), odd
+__host__ __device__ inline int wide_row_len(int L) { return (L + 1) | 1; }
 
 template <typename T>
 __device__ __forceinline__ T ldl(const unsigned char* base, int off)
@@ -97,12 +100,14 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
         *reinterpret_cast<float*>(smem + kSite + (2 * kLP + i) * kRB) = s.coulomb_type == 1 ? s.u_diag[i] : 0.f;
         *reinterpret_cast<float*>(smem + kSite + (3 * kLP + i) * kRB) = s.pdiag[i];
     }
-    // neighbour tables for row i (rows past L: row 0 with t = 0)
+    // neighbour tables for row i (rows past L: row 0 with t = 0): row and column byte offsets of k_n(i)
     for (int e = tid; e < Z * kLP; e += kNT) {
         const int n = e / kLP, i = e - n * kLP;
         const bool on = i < L;
+        const int k = on ? s.nbr_k[n * L + i] : 0;
         *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = on ? s.nbr_t[n * L + i] : 0.f;
-        *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + (on ? s.nbr_k[n * L + i] : 0) * RB;
+        *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + k * RB;
+        *reinterpret_cast<int*>(smem + kNbrC + e * 4) = k * kCB;
     }
 
     // ---- element slots (class order): ah = i | j << 16 of the slot's element; padding lanes hold
@@ -133,18 +138,24 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
     };
     const bool dslot = tid < L;
     const int da = dslot ? kRows + tid * RB + tid * kCB : spare;
-    const int dblk = tid >> 6;  // the X block holding column tid (wave-uniform: waves 0 and 1)
     float dy0 = dslot ? gs[tid * L + tid].x : 0.f, dy1 = dslot ? gs[LL + tid * L + tid].x : 0.f;
     float dA0 = 0, dp0 = 0, dA1 = 0, dp1 = 0;
 
     const float h1 = static_cast<float>(r.h), h3 = static_cast<float>(r.h / 3.0), h6 = static_cast<float>(r.h / 6.0),
                 h8 = static_cast<float>(r.h / 8.0);
 
-    // neighbour n of row i from the LDS tables (read at each use, not held across the time loop)
+    // neighbour n of row i from the LDS tables (read at each use, not held across the time loop): the row
+    // offset of k_n(i) (kb) or its column offset (kc), and t_{i,n}
     auto nbr = [&](int n, int i, int& kb, float& tv) {
         int o = n * kLP + i;
         asm volatile("" : "+v"(o));
         kb = *reinterpret_cast<const int*>(smem + kNbrK + o * 4);
+        tv = *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
+    };
+    auto nbrc = [&](int n, int i, int& kc, float& tv) {
+        int o = n * kLP + i;
+        asm volatile("" : "+v"(o));
+        kc = *reinterpret_cast<const int*>(smem + kNbrC + o * 4);
         tv = *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
     };
 
@@ -184,92 +195,53 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
         }
     };
 
-    // ---- A_b: X block b = T Y[:, 64b ..] for the spin in the rows; lane j takes rows j and j + 64,
-    // wave w the block's columns w + 8m ----
-    auto phase_a = [&](auto bc) {
-        constexpr int BLK = decltype(bc)::value;
-#pragma unroll
-        for (int rp = 0; rp < 2; ++rp) {
-            const int i = j + 64 * rp;
-            const bool on = i < L;
-            int kb[Z];
-            float tv[Z];
-#pragma unroll
-            for (int n = 0; n < Z; ++n) nbr(n, on ? i : 0, kb[n], tv[n]);
-            if (!on) {
-#pragma unroll
-                for (int n = 0; n < Z; ++n) tv[n] = 0.f;
-            }
-            const int xb = (on ? kRows + rowb(i) : spare) + (L + w) * kCB;
-            const int yc = (64 * BLK + w) * kCB;
-            // software pipelined as in ftwa_herm_seq.hip where the registers allow (z = 4, S <= 10): the
-            // next column's reads go out before this column's X store; the first read (column
-            // 64 BLK + w < L + 65) stays inside the row
-            constexpr bool kPipe = Z == 4 && S <= 10;
-            C v[Z];
-            if constexpr (kPipe) {
-#pragma unroll
-                for (int n = 0; n < Z; ++n) v[n] = ldl<C>(smem, kb[n] + yc);
-            }
-#pragma unroll
-            for (int m = 0; m < 64 / kW; ++m) {
-                if (64 * BLK + w + kW * m >= L) break;  // wave-uniform
-                const int co = m * kW * kCB;  // immediate column offset
-                float ax = 0, ay = 0;
-#pragma unroll
-                for (int n = 0; n < Z; ++n) {
-                    const C vn = kPipe ? v[n] : ldl<C>(smem, kb[n] + yc + co);
-                    ax += tv[n] * vn.x;
-                    ay += tv[n] * vn.y;
-                }
-                if (kPipe && m + 1 < 64 / kW && 64 * BLK + w + kW * (m + 1) < L) {
-#pragma unroll
-                    for (int n = 0; n < Z; ++n) v[n] = ldl<C>(smem, kb[n] + yc + co + kW * kCB);
-                }
-                *reinterpret_cast<C*>(smem + xb + co) = C{ax, ay};
-            }
-        }
-    };
-
-    // ---- B_b: the block's part of conj(X_ji) - X_ij; after the last block the derivative and the RK
-    // registers.  Parts outside the block read the zero bytes.  Between the blocks the partial sum of an
-    // element with i < 64 waits in Y_ji: after A_0 no phase reads the first 64 columns of Y until the next
-    // M, and an element with i >= 64 has nothing in block 0. ----
-    auto phase_b = [&](auto bc, auto stc, auto sgc) {
-        constexpr int BLK = decltype(bc)::value;
+    // ---- C: each slot's commutator from the rows of Y, then its RK registers ----
+    auto phase_c = [&](auto stc, auto sgc) {
         constexpr int ST = decltype(stc)::value;
         constexpr int SG = decltype(sgc)::value;
         constexpr int HS = kHB + (ST & 1) * kHPar + SG * kHSpin;
-        const int xo = (L - 64 * BLK) * kCB;  // X(i, c) - Y(i, c) for a column c of block BLK
 #pragma unroll
         for (int q = 0; q < S; ++q) {
             if (q * kW + w >= n_off) continue;
-            asm volatile("" ::: "memory");
             int i, jj;
             slot(q, i, jj);
-            const int aij = kRows + rowb(i) + jj * kCB, aji = kRows + rowb(jj) + i * kCB;
-            const int ci = i - 64 * BLK, cj = jj - 64 * BLK;
-            const C xij = ldl<C>(smem, (cj >= 0 && cj < 64) ? aij + xo : kZero);
-            const C xji = ldl<C>(smem, (ci >= 0 && ci < 64) ? aji + xo : kZero);
-            if constexpr (BLK == 0) {
-                if (i < 64) *reinterpret_cast<C*>(smem + aji) = C{xji.x - xij.x, -xji.y - xij.y};
-            } else {
-                const C c0 = i < 64 ? ldl<C>(smem, aji) : C{0.f, 0.f};
-                C& yq = SG ? y1[q] : y0[q];
-                C& Aq = SG ? A1[q] : A0[q];
-                C& pq = SG ? p1[q] : p0[q];
-                const float dh = ldl<float>(smem, HS + jj * kRB) - ldl<float>(smem, HS + i * kRB);
-                const float ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
-                const float ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
-                // c = conj(X_ji) - X_ij + (h_j - h_i) Y_ij;  k = -i c
-                const float cx = c0.x + xji.x - xij.x + dh * ycx;
-                const float cy = c0.y - xji.y - xij.y + dh * ycy;
-                dev::rk_update<ST>(yq.x, Aq.x, pq.x, cy, h8);
-                dev::rk_update<ST>(yq.y, Aq.y, pq.y, -cx, h8);
+            const int ri = kRows + rowb(i), cj = jj * kCB;
+            float ax = 0, ay = 0, bx = 0, by = 0;  // conj(X_ji), X_ij
+#pragma unroll
+            for (int n = 0; n < Z; ++n) {
+                int kc, kb;
+                float tj, ti;
+                nbrc(n, jj, kc, tj);
+                nbr(n, i, kb, ti);
+                const C u = ldl<C>(smem, ri + kc);  // Y_{i, k_n(j)} = conj(Y_{k_n(j), i})
+                const C v = ldl<C>(smem, kb + cj);  // Y_{k_n(i), j}
+                ax += tj * u.x;
+                ay += tj * u.y;
+                bx += ti * v.x;
+                by += ti * v.y;
             }
+            C& yq = SG ? y1[q] : y0[q];
+            C& Aq = SG ? A1[q] : A0[q];
+            C& pq = SG ? p1[q] : p0[q];
+            const float dh = ldl<float>(smem, HS + jj * kRB) - ldl<float>(smem, HS + i * kRB);
+            const float ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
+            const float ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
+            // c = conj(X_ji) - X_ij + (h_j - h_i) Y_ij;  k = -i c
+            const float cx = ax - bx + dh * ycx;
+            const float cy = ay - by + dh * ycy;
+            dev::rk_update<ST>(yq.x, Aq.x, pq.x, cy, h8);
+            dev::rk_update<ST>(yq.y, Aq.y, pq.y, -cx, h8);
         }
-        if (dslot && dblk == BLK) {  // k_ii = -2 Im X_ii
-            const float k = -2.f * ldl<C>(smem, da + xo).y;
+        if (dslot) {  // k_ii = -2 Im X_ii, X_ii = sum_n t_{i,n} Y_{k_n(i), i}
+            float xi = 0;
+#pragma unroll
+            for (int n = 0; n < Z; ++n) {
+                int kb;
+                float ti;
+                nbr(n, tid, kb, ti);
+                xi += ti * ldl<C>(smem, kb + tid * kCB).y;
+            }
+            const float k = -2.f * xi;
             if constexpr (SG == 0) dev::rk_update<ST>(dy0, dA0, dp0, k, h8);
             else dev::rk_update<ST>(dy1, dA1, dp1, k, h8);
         }
@@ -329,14 +301,9 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
         materialize(stc, sgc);
         __syncthreads();
         if (record) record_part(sgc);
-        phase_a(Ic<0>{});
-        __syncthreads();
+        phase_c(stc, sgc);
+        __syncthreads();  // the next M rewrites rows that other slots have read
         if (record && decltype(sgc)::value == 1) flush_energies();
-        phase_b(Ic<0>{}, stc, sgc);
-        __syncthreads();
-        phase_a(Ic<1>{});
-        __syncthreads();
-        phase_b(Ic<1>{}, stc, sgc);
     };
     auto stage = [&](auto stc, bool record) {
         spin_pass(stc, Ic<0>{}, record);
