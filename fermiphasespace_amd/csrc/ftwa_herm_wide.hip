@@ -12,7 +12,7 @@
 //        k_ii = -2 Im X_ii, and the RK registers
 // with a barrier after M and one after C (before the next M rewrites the rows).  Per element that is the
 // same 2z complex x real multiply-adds as forming X, with the neighbour tables (row offsets, column
-// offsets, hoppings) read from LDS per slot.  LDS: rows of Y with an odd row length.  One 8-wave workgroup
+// offsets, hoppings) read from LDS per slot.  LDS: rows of Y, 1 mod 32 complex long.  One 8-wave workgroup
 // per CU (up to 256 VGPRs, S <= 12 slots per thread).  Element slots in class order (dev::class_slot).
 // Null or on-site Coulomb interactions, no pulse, no density records.
 #include <hip/hip_runtime.h>
@@ -37,8 +37,8 @@ constexpr int kCB = 8;           // bytes per complex
 constexpr int kRB = 4;           // bytes per real
 // LDS layout: h [2 stage parities][2 spins][144], stage-input diagonals [2][144], energy partials
 // [8 waves][4], site tables tM_ii / E_pe potential / U_ii / pdiag [4][128], neighbour hoppings [8][128],
-// neighbour row offsets [8][128] and column offsets [8][128] (ints), 16 spare bytes, then L rows and a
-// spare row.
+// neighbour row offsets [8][128] (ints), the neighbour sites packed a byte each [2][128] (word w of site i
+// holds k_{4w .. 4w+3}(i)), 16 spare bytes (a flag at set-up), then L rows and a spare row.
 constexpr int kHSpin = 144 * kRB;  // entries 128..143 take padding slots (always zero)
 constexpr int kHPar = 2 * kHSpin;
 constexpr int kHB = 0;
@@ -47,14 +47,16 @@ constexpr int kEW = kDg + kHPar;
 constexpr int kSite = kEW + kW * 4 * kRB;
 constexpr int kNbrT = kSite + 4 * kLP * kRB;
 constexpr int kNbrK = kNbrT + 8 * kLP * kRB;
-constexpr int kNbrC = kNbrK + 8 * kLP * 4;
-constexpr int kZero = kNbrC + 8 * kLP * 4;
+constexpr int kNbrP = kNbrK + 8 * kLP * 4;
+constexpr int kZero = kNbrP + 2 * kLP * 4;
 constexpr int kRows = kZero + 16;
 static_assert(kRows % 16 == 0, "rows are 16-byte aligned");
 constexpr size_t kLdsLimit = 160 * 1024;
 
-// complex elements per LDS row: L of Y and at least one more (the padding slots' column L), odd
-__host__ __device__ inline int wide_row_len(int L) { return (L + 1) | 1; }
+// complex elements per LDS row: L of Y and at least one more (the padding slots' column L), rounded up to
+// 1 mod 32.  Y_ij then sits in the bank pair of i + j, the same for Y_ji: the lanes of a half-wave hold
+// elements of distinct classes (i + j mod 32), and a neighbour read shifts them all alike.
+__host__ __device__ inline int wide_row_len(int L) { return (L + 31) / 32 * 32 + 1; }
 
 template <typename T>
 __device__ __forceinline__ T ldl(const unsigned char* base, int off)
@@ -100,15 +102,33 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
         *reinterpret_cast<float*>(smem + kSite + (2 * kLP + i) * kRB) = s.coulomb_type == 1 ? s.u_diag[i] : 0.f;
         *reinterpret_cast<float*>(smem + kSite + (3 * kLP + i) * kRB) = s.pdiag[i];
     }
-    // neighbour tables for row i (rows past L: row 0 with t = 0): row and column byte offsets of k_n(i)
+    // neighbour tables for row i (rows past L: row 0 with t = 0): the row byte offset of k_n(i), and k_n(i)
+    // packed four to a word.  Whether every hopping of the lattice is the same value (then phase C takes
+    // it from a register instead of the table).
+    bool same_t = true;
+    const float t_first = s.nbr_t[0];
     for (int e = tid; e < Z * kLP; e += kNT) {
         const int n = e / kLP, i = e - n * kLP;
         const bool on = i < L;
         const int k = on ? s.nbr_k[n * L + i] : 0;
-        *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = on ? s.nbr_t[n * L + i] : 0.f;
+        const float tv = on ? s.nbr_t[n * L + i] : 0.f;
+        same_t = same_t && (!on || tv == t_first);
+        *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = tv;
         *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + k * RB;
-        *reinterpret_cast<int*>(smem + kNbrC + e * 4) = k * kCB;
     }
+    for (int e = tid; e < (Z / 4) * kLP; e += kNT) {
+        const int wd = e / kLP, i = e - wd * kLP;
+        uint32_t pk = 0;
+        if (i < L)
+            for (int m = 0; m < 4; ++m) pk |= static_cast<uint32_t>(s.nbr_k[(4 * wd + m) * L + i]) << (8 * m);
+        *reinterpret_cast<uint32_t*>(smem + kNbrP + e * 4) = pk;
+    }
+    // (a flag in the spare bytes: no static LDS, the whole 160 KB stay available to the rows)
+    if (tid == 0) *reinterpret_cast<int*>(smem + kZero) = 1;
+    __syncthreads();
+    if (!same_t) *reinterpret_cast<int*>(smem + kZero) = 0;
+    __syncthreads();
+    const bool tuni = *reinterpret_cast<const int*>(smem + kZero) != 0;
 
     // ---- element slots (class order): ah = i | j << 16 of the slot's element; padding lanes hold
     // i = j = L (the spare row; h entry L is zero).  The LDS addresses of Y_ij and Y_ji are formed from ah at
@@ -152,11 +172,11 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
         kb = *reinterpret_cast<const int*>(smem + kNbrK + o * 4);
         tv = *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
     };
-    auto nbrc = [&](int n, int i, int& kc, float& tv) {
+    auto hop = [&](int n, int i) -> float {  // t_{i,n}: the lattice's one value, or from the table
+        if (tuni) return t_first;
         int o = n * kLP + i;
         asm volatile("" : "+v"(o));
-        kc = *reinterpret_cast<const int*>(smem + kNbrC + o * 4);
-        tv = *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
+        return *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
     };
 
     // ---- M: the spin's stage input into the rows; in the spin-up pass, h of both spins ----
@@ -205,16 +225,20 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
             if (q * kW + w >= n_off) continue;
             int i, jj;
             slot(q, i, jj);
-            const int ri = kRows + rowb(i), cj = jj * kCB;
+            const int ri = kRows + rowb(i), rj = kRows + jj * kCB;
             float ax = 0, ay = 0, bx = 0, by = 0;  // conj(X_ji), X_ij
+            uint32_t pi[Z / 4], pj[Z / 4];  // the neighbour sites of i and j
+#pragma unroll
+            for (int wd = 0; wd < Z / 4; ++wd) {
+                pi[wd] = ldl<uint32_t>(smem, kNbrP + (wd * kLP + i) * 4);
+                pj[wd] = ldl<uint32_t>(smem, kNbrP + (wd * kLP + jj) * 4);
+            }
 #pragma unroll
             for (int n = 0; n < Z; ++n) {
-                int kc, kb;
-                float tj, ti;
-                nbrc(n, jj, kc, tj);
-                nbr(n, i, kb, ti);
-                const C u = ldl<C>(smem, ri + kc);  // Y_{i, k_n(j)} = conj(Y_{k_n(j), i})
-                const C v = ldl<C>(smem, kb + cj);  // Y_{k_n(i), j}
+                const int ki = (pi[n / 4] >> (8 * (n % 4))) & 0xff, kj = (pj[n / 4] >> (8 * (n % 4))) & 0xff;
+                const float tj = hop(n, jj), ti = hop(n, i);
+                const C u = ldl<C>(smem, ri + kj * kCB);     // Y_{i, k_n(j)} = conj(Y_{k_n(j), i})
+                const C v = ldl<C>(smem, rj + rowb(ki));    // Y_{k_n(i), j}
                 ax += tj * u.x;
                 ay += tj * u.y;
                 bx += ti * v.x;
