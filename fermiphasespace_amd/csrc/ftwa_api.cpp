@@ -298,6 +298,9 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T, char* dst = nul
     ds.pd = reinterpret_cast<const Real*>(base + o_pdd);
     ds.amp = amp.empty() ? nullptr : reinterpret_cast<const double*>(base + o_amp);
     ds.rows = rows.empty() ? nullptr : base + o_rows;
+    ds.hop_uniform = 1;
+    for (const Real t : nt) ds.hop_uniform &= t == nt[0] ? 1 : 0;
+    ds.hop_value = nt.empty() ? Real(0) : nt[0];
     ds.fock = (h->method == FPSFTWA_METHOD_HF && h->coulomb_type == FPSFTWA_COULOMB_GENERAL) ? 1 : 0;
     ds.gpsr = h->method == FPSFTWA_METHOD_GPSR ? 1 : 0;
     if (ds.gpsr && h->coulomb_type == FPSFTWA_COULOMB_ONSITE) {

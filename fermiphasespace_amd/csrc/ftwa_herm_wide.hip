@@ -38,7 +38,7 @@ constexpr int kRB = 4;           // bytes per real
 // LDS layout: h [2 stage parities][2 spins][144], stage-input diagonals [2][144], energy partials
 // [8 waves][4], site tables tM_ii / E_pe potential / U_ii / pdiag [4][128], neighbour hoppings [8][128],
 // neighbour row offsets [8][128] (ints), the neighbour sites packed a byte each [2][128] (word w of site i
-// holds k_{4w .. 4w+3}(i)), 16 spare bytes (a flag at set-up), then L rows and a spare row.
+// holds k_{4w .. 4w+3}(i)), 16 spare bytes, then L rows and a spare row.
 constexpr int kHSpin = 144 * kRB;  // entries 128..143 take padding slots (always zero)
 constexpr int kHPar = 2 * kHSpin;
 constexpr int kHB = 0;
@@ -67,7 +67,9 @@ __device__ __forceinline__ T ldl(const unsigned char* base, int off)
 template <int N>
 using Ic = std::integral_constant<int, N>;
 
-template <int S, int Z>
+// UNI: every neighbour entry holds the same hopping s.hop_value (a regular lattice with one bond type),
+// taken from a register instead of the table.
+template <int S, int Z, bool UNI>
 __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
     herm_wide_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state,
                      float* __restrict__ rec)
@@ -103,17 +105,12 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
         *reinterpret_cast<float*>(smem + kSite + (3 * kLP + i) * kRB) = s.pdiag[i];
     }
     // neighbour tables for row i (rows past L: row 0 with t = 0): the row byte offset of k_n(i), and k_n(i)
-    // packed four to a word.  Whether every hopping of the lattice is the same value (then phase C takes
-    // it from a register instead of the table).
-    bool same_t = true;
-    const float t_first = s.nbr_t[0];
+    // packed four to a word
     for (int e = tid; e < Z * kLP; e += kNT) {
         const int n = e / kLP, i = e - n * kLP;
         const bool on = i < L;
         const int k = on ? s.nbr_k[n * L + i] : 0;
-        const float tv = on ? s.nbr_t[n * L + i] : 0.f;
-        same_t = same_t && (!on || tv == t_first);
-        *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = tv;
+        *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = on ? s.nbr_t[n * L + i] : 0.f;
         *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + k * RB;
     }
     for (int e = tid; e < (Z / 4) * kLP; e += kNT) {
@@ -123,12 +120,6 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
             for (int m = 0; m < 4; ++m) pk |= static_cast<uint32_t>(s.nbr_k[(4 * wd + m) * L + i]) << (8 * m);
         *reinterpret_cast<uint32_t*>(smem + kNbrP + e * 4) = pk;
     }
-    // (a flag in the spare bytes: no static LDS, the whole 160 KB stay available to the rows)
-    if (tid == 0) *reinterpret_cast<int*>(smem + kZero) = 1;
-    __syncthreads();
-    if (!same_t) *reinterpret_cast<int*>(smem + kZero) = 0;
-    __syncthreads();
-    const bool tuni = *reinterpret_cast<const int*>(smem + kZero) != 0;
 
     // ---- element slots (class order): ah = i | j << 16 of the slot's element; padding lanes hold
     // i = j = L (the spare row; h entry L is zero).  The LDS addresses of Y_ij and Y_ji are formed from ah at
@@ -172,8 +163,9 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
         kb = *reinterpret_cast<const int*>(smem + kNbrK + o * 4);
         tv = *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
     };
+    const float t_uni = s.hop_value;
     auto hop = [&](int n, int i) -> float {  // t_{i,n}: the lattice's one value, or from the table
-        if (tuni) return t_first;
+        if (UNI) return t_uni;
         int o = n * kLP + i;
         asm volatile("" : "+v"(o));
         return *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
@@ -244,7 +236,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
                 bx += ti * v.x;
                 by += ti * v.y;
             }
-            C& yq = SG ? y1[q] : y0[q];
+        C& yq = SG ? y1[q] : y0[q];
             C& Aq = SG ? A1[q] : A0[q];
             C& pq = SG ? p1[q] : p0[q];
             const float dh = ldl<float>(smem, HS + jj * kRB) - ldl<float>(smem, HS + i * kRB);
@@ -376,11 +368,11 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
 
 size_t wide_lds_bytes(int L) { return kRows + static_cast<size_t>(L + 1) * wide_row_len(L) * kCB; }
 
-template <int S, int Z>
+template <int S, int Z, bool UNI>
 hipError_t run_wide(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                     hipStream_t st)
 {
-    auto kern = herm_wide_kernel<S, Z>;
+    auto kern = herm_wide_kernel<S, Z, UNI>;
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -393,19 +385,19 @@ hipError_t run_wide(const DevSystem<float>& s, const RunShape& r, int t0, int t1
     return hipGetLastError();
 }
 
-template <int Z>
+template <int Z, bool UNI>
 hipError_t run_wide_s(int S, const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                       hipStream_t st)
 {
     switch (S) {  // 64 < L <= 107: 33 to 90 element rows, 5 to 12 slots per thread
-        case 5: return run_wide<5, Z>(s, r, t0, t1, state, rec, st);
-        case 6: return run_wide<6, Z>(s, r, t0, t1, state, rec, st);
-        case 7: return run_wide<7, Z>(s, r, t0, t1, state, rec, st);
-        case 8: return run_wide<8, Z>(s, r, t0, t1, state, rec, st);
-        case 9: return run_wide<9, Z>(s, r, t0, t1, state, rec, st);
-        case 10: return run_wide<10, Z>(s, r, t0, t1, state, rec, st);
-        case 11: return run_wide<11, Z>(s, r, t0, t1, state, rec, st);
-        case 12: return run_wide<12, Z>(s, r, t0, t1, state, rec, st);
+        case 5: return run_wide<5, Z, UNI>(s, r, t0, t1, state, rec, st);
+        case 6: return run_wide<6, Z, UNI>(s, r, t0, t1, state, rec, st);
+        case 7: return run_wide<7, Z, UNI>(s, r, t0, t1, state, rec, st);
+        case 8: return run_wide<8, Z, UNI>(s, r, t0, t1, state, rec, st);
+        case 9: return run_wide<9, Z, UNI>(s, r, t0, t1, state, rec, st);
+        case 10: return run_wide<10, Z, UNI>(s, r, t0, t1, state, rec, st);
+        case 11: return run_wide<11, Z, UNI>(s, r, t0, t1, state, rec, st);
+        case 12: return run_wide<12, Z, UNI>(s, r, t0, t1, state, rec, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -423,7 +415,11 @@ hipError_t launch_herm_wide(const DevSystem<float>& s, const RunShape& r, int t0
 {
     if (!herm_wide_fits(s) || r.dens || r.B <= 0 || r.B > 0x7fffffff) return hipErrorInvalidValue;
     const int S = std::max(5, (dev::class_rows(s.L) + kW - 1) / kW);
-    return s.z == 4 ? run_wide_s<4>(S, s, r, t0, t1, state, rec, st) : run_wide_s<8>(S, s, r, t0, t1, state, rec, st);
+    if (s.hop_uniform)
+        return s.z == 4 ? run_wide_s<4, true>(S, s, r, t0, t1, state, rec, st)
+                        : run_wide_s<8, true>(S, s, r, t0, t1, state, rec, st);
+    return s.z == 4 ? run_wide_s<4, false>(S, s, r, t0, t1, state, rec, st)
+                    : run_wide_s<8, false>(S, s, r, t0, t1, state, rec, st);
 }
 
 }  // namespace fpsamd

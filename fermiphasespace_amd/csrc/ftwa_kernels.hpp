@@ -52,6 +52,8 @@ struct DevSystem {
     const Real* tM = nullptr;      // [L][L]  dense hopping (plain path)
     const Real* pd = nullptr;      // [L][L]  pol . (r_i - r_j) dense (plain path, pulse)
     const void* rows = nullptr;    // [L] RowNbrs<Real, z> (batched path)
+    int hop_uniform = 0;           // 1: every ELL entry (padding included) holds the hopping hop_value
+    Real hop_value = 0;
     int fock = 0;                  // 1: Hartree-Fock dynamics, Delta_sigma gains -V o N_sigma (plain path)
     int gpsr = 0;                  // 1: GPSR dynamics: normally ordered rho (no 1/2), Delta_sigma = tM +
                                    //    diag(U rho_sbar_ii) + potential (no V), diffusion every step
