@@ -118,8 +118,8 @@ template <typename Real>
 hipError_t launch_herm_seq(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, Real* state, Real* rec,
                            int nt_hint, hipStream_t st);
 
-// Its FP32 form for 64 < L <= 107 (ftwa_herm_wide.hip): X = T Y formed one block of 64 columns at a time;
-// z 4 or 8, no pulse, null or on-site Coulomb, no density records.
+// Its FP32 form for 64 < L <= 107 (ftwa_herm_wide.hip): only Y in LDS, each element's commutator formed
+// directly from its rows; z 4 or 8, no pulse, null or on-site Coulomb, no density records.
 bool herm_wide_fits(const DevSystem<float>& s);
 hipError_t launch_herm_wide(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                             hipStream_t st);
