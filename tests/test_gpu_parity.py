@@ -164,14 +164,24 @@ def test_site_counts(oracle_mod, L):
             engines=("fp64", "fp32"))
 
 
-@pytest.mark.parametrize("shape", ["chain80", "grid10x10", "chain90_reach4", "chain107"])
+@pytest.mark.parametrize("shape", ["chain80", "grid10x10", "chain90_reach4", "chain107", "grid10x10_bonds",
+                                   "grid10x10_diag"])
 def test_wide_lattices(oracle_mod, shape):
-    """64 < L <= 107: the FP32 column-blocked spin-sequential kernel (automatic and forced) and the dense
-    path against the oracle, with site potentials; z = 4 and 8, L at the kernel's LDS limit."""
+    """64 < L <= 107: the FP32 spin-sequential kernel for L > 64 (automatic and forced) and the dense path
+    against the oracle, with site potentials; z = 4 and 8, one hopping value on every neighbour entry
+    (grid10x10: z = 4; grid10x10_diag, nearest and diagonal neighbours alike: z = 8) or several (chains,
+    whose padded entries are zero; grid10x10_bonds: a different hopping per bond), L at the kernel's LDS
+    limit."""
     rng = np.random.default_rng(zlib.crc32(shape.encode()) % 1000)  # stable across processes
-    if shape == "grid10x10":
+    if shape.startswith("grid10x10"):
         pos, box = lat.grid(10, 10)
         tM, _ = lat.system_arrays(pos, box, True)
+        if shape == "grid10x10_bonds":
+            f = rng.uniform(0.5, 1.5, tM.shape)
+            tM = tM * np.triu(f, 1)
+            tM = tM + tM.T
+        elif shape == "grid10x10_diag":
+            tM = tM + lat.system_arrays(pos, box, True, nn=2 ** 0.5)[0]
     elif shape == "chain90_reach4":
         L = 90
         tM = np.zeros((L, L))
