@@ -435,7 +435,8 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="threads per trajectory workgroup (0, 512, 1024)")
     ap.add_argument("--time-chunk", type=int, default=0)
     ap.add_argument("--kernel", default="auto", choices=["auto", "full", "hermitian", "pipelined", "sequential"],
-                    help="FP32 integrator: automatic (the spin-sequential upper-triangle kernel here), "
+                    help="FP32 integrator: automatic (here the spin-sequential upper-triangle kernel in its "
+                         "direct-commutator form), the spin-sequential kernel with X = T Y in LDS (sequential), "
                          "upper-triangle with both spins (hermitian), spin-pipelined, or full-matrix")
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 batched reference run")
@@ -522,8 +523,10 @@ def main():
     kern_s = tm["kernel_ms"] * 1e-3
     achieved = alg_bytes / kern_s / 1e9
     kname = {"full": "stencil_kernel<float,16,8,4,false,64>", "hermitian": "herm_kernel<16,8,5,4,false,false>",
-             "pipelined": "herm_pipe_kernel<5,4>"}.get(args.kernel, "herm_seq_kernel<float,8,4,4,2,1>")
-    pfile = "herm_fp32_hubbard8x8_traffic.json" if args.kernel == "hermitian" else "herm_seq_fp32_hubbard8x8_traffic.json"
+             "pipelined": "herm_pipe_kernel<5,4>", "sequential": "herm_seq_kernel<float,8,4,4,2,1>"}.get(
+                 args.kernel, "herm_wide_kernel<4,4,true>")
+    pfile = {"hermitian": "herm_fp32_hubbard8x8_traffic.json",
+             "sequential": "herm_seq_fp32_hubbard8x8_traffic.json"}.get(args.kernel, "herm_wide_fp32_hubbard8x8_traffic.json")
     traffic, prof = traffic_from_profile(os.path.join(ROOT, "profiles", pfile))
     if prof is None or prof.get("kernel") != kname:
         traffic = None  # no committed counter figures for this kernel

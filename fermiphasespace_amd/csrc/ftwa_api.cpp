@@ -432,7 +432,7 @@ struct KernelChoice {
     bool herm = false, pipe = false, seq = false, wide = false, dense = false;
     int nt = 0;  // threads per workgroup of the full-matrix stencil kernel
     // the spin-sequential kernel (FP32 seq; FP64 herm runs it too), which can sample the initial state
-    bool spin_sequential(bool fp64) const { return seq || (fp64 && herm); }
+    bool spin_sequential(bool fp64) const { return seq || wide || (fp64 && herm); }
 };
 
 // Whether the run's first integrator launch samples the initial states itself: fTWA with the
@@ -461,9 +461,11 @@ int choose_kernel(fpsftwa_handle* h, const DevSystem<Real>& ds, KernelChoice& k)
     } else if constexpr (std::is_same<Real, float>::value) {
         // automatic choice: the spin-sequential kernel where it applies (two workgroups per CU:
         // 3.12e7 trajectory-steps/s on the 8x8 workload, against 2.63e7 for the both-spin kernel), its
-        // column-blocked form for 64 < L <= 107, else the both-spin upper-triangle kernel (pulses, V
-        // interactions, basis occupations)
-        k.seq = h->kernel == FPSFTWA_KERNEL_AUTO && herm_seq_fits<float>(ds) && h->n_bases == 0;
+        // direct-commutator form for 64 < L <= 107 and where that form measured faster at L <= 64 (the
+        // 8x8 workload), else the both-spin upper-triangle kernel (pulses, V interactions, basis
+        // occupations)
+        k.seq = h->kernel == FPSFTWA_KERNEL_AUTO && herm_seq_fits<float>(ds) && h->n_bases == 0 &&
+                !herm_wide_direct_preferred(ds);
         k.wide = !k.seq && h->kernel == FPSFTWA_KERNEL_AUTO && herm_wide_fits(ds) && h->n_bases == 0;
         k.herm = !k.seq && !k.wide && h->kernel != FPSFTWA_KERNEL_FULL && herm_fits(h->L, ds.z);
         if (h->kernel == FPSFTWA_KERNEL_HERMITIAN && !k.herm)

@@ -70,7 +70,7 @@ using Ic = std::integral_constant<int, N>;
 // UNI: every neighbour entry holds the same hopping s.hop_value (a regular lattice with one bond type),
 // taken from a register instead of the table.
 template <int S, int Z, bool UNI>
-__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 : 2)))
     herm_wide_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state,
                      float* __restrict__ rec)
 {
@@ -125,6 +125,9 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
     // i = j = L (the spare row; h entry L is zero).  The LDS addresses of Y_ij and Y_ji are formed from ah at
     // each use (registers: the RK state of both spins is 12 per slot).  The diagonal of site i is held by
     // thread i (< L): real RK registers only.
+    // The first launch of a run samples the initial state here (src/FHfTWASolver.cpp:342-362), the same
+    // FP64 values as launch_sample's rounded to float, one spin at a time through the rows.
+    const bool sampled = t0 == 0 && r.sample_occ != nullptr;
     int ah[S];
     C y0[S], A0[S], p0[S], y1[S], A1[S], p1[S];
     const C* gs = reinterpret_cast<const C*>(state) + b * 2 * LL;
@@ -136,9 +139,44 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
         int i = 0, jj = 0;
         if (er < n_off && dev::class_slot(L, 2 * er + (j >> 5), j & 31, i, jj)) {
             ah[q] = i | jj << 16;
-            y0[q] = gs[i * L + jj];
-            y1[q] = gs[LL + i * L + jj];
+            if (!sampled) {
+                y0[q] = gs[i * L + jj];
+                y1[q] = gs[LL + i * L + jj];
+            }
         }
+    }
+    auto sample_pass = [&](auto sgc) {
+        constexpr int SG = decltype(sgc)::value;
+        const double* occ = r.sample_occ + SG * L;
+        // the strict upper triangle of spin SG into the rows, folded onto rows i0 < ceil(L/2): (i0, c0) with
+        // c0 > i0 is itself, with c0 < i0 it stands for (L-1-i0, L-1-c0) (the middle row of an odd L only
+        // once), so nearly every lane draws an element
+        const int half = (L + 1) / 2;
+        for (int e = tid; e < half * L; e += kNT) {
+            const int i0 = e / L, c0 = e - i0 * L;
+            int i = i0, c = c0;
+            if (c0 < i0 && 2 * i0 != L - 1) {
+                i = L - 1 - i0;
+                c = L - 1 - c0;
+            } else if (c0 <= i0) {
+                continue;
+            }
+            double wr, wi, re, im;
+            dev::initial_noise(r.seed, r.b0 + b, SG, static_cast<uint32_t>(dev::pair_index(L, i, c)), wr, wi);
+            dev::sampled_offdiag(occ[i], occ[c], wr, wi, re, im);
+            yat(i, c) = C{static_cast<float>(re), static_cast<float>(im)};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            const int i = ah[q] & 0xffff, c = ah[q] >> 16;
+            if (i < L) (SG ? y1[q] : y0[q]) = yat(i, c);
+        }
+        __syncthreads();  // the rows are rewritten next
+    };
+    if (sampled) {
+        sample_pass(Ic<0>{});
+        sample_pass(Ic<1>{});
     }
     // the slot's (i, j) and its LDS offsets, decoded where used (opaque: not kept across the time loop)
     auto slot = [&](int q, int& i, int& jj) {
@@ -149,7 +187,11 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2)))
     };
     const bool dslot = tid < L;
     const int da = dslot ? kRows + tid * RB + tid * kCB : spare;
-    float dy0 = dslot ? gs[tid * L + tid].x : 0.f, dy1 = dslot ? gs[LL + tid * L + tid].x : 0.f;
+    float dy0 = 0.f, dy1 = 0.f;  // diagonal: n - 1/2
+    if (dslot) {
+        dy0 = sampled ? static_cast<float>(r.sample_occ[tid] - 0.5) : gs[tid * L + tid].x;
+        dy1 = sampled ? static_cast<float>(r.sample_occ[L + tid] - 0.5) : gs[LL + tid * L + tid].x;
+    }
     float dA0 = 0, dp0 = 0, dA1 = 0, dp1 = 0;
 
     const float h1 = static_cast<float>(r.h), h3 = static_cast<float>(r.h / 3.0), h6 = static_cast<float>(r.h / 6.0),
@@ -389,7 +431,8 @@ template <int Z, bool UNI>
 hipError_t run_wide_s(int S, const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                       hipStream_t st)
 {
-    switch (S) {  // 64 < L <= 107: 33 to 90 element rows, 5 to 12 slots per thread
+    switch (S) {  // L = 64: 32 element rows, 4 slots; 64 < L <= 107: 33 to 91 element rows, 5 to 12 slots
+        case 4: return run_wide<4, Z, UNI>(s, r, t0, t1, state, rec, st);
         case 5: return run_wide<5, Z, UNI>(s, r, t0, t1, state, rec, st);
         case 6: return run_wide<6, Z, UNI>(s, r, t0, t1, state, rec, st);
         case 7: return run_wide<7, Z, UNI>(s, r, t0, t1, state, rec, st);
@@ -404,17 +447,27 @@ hipError_t run_wide_s(int S, const DevSystem<float>& s, const RunShape& r, int t
 
 }  // namespace
 
+// L <= 64 where it measured faster than the spin-sequential kernel of ftwa_herm_seq.hip: four whole slot
+// rows per wave (L = 64), four neighbours, one hopping value (the 8x8 periodic square: 3.25e7 against
+// 3.08e7 trajectory-steps/s; with unequal slot rows per wave, kagome 48 and bcc 54, it lost)
+bool herm_wide_direct_preferred(const DevSystem<float>& s)
+{
+    return s.L <= 64 && dev::class_rows(s.L) == 4 * kW && s.z == 4 && s.hop_uniform && !s.amp &&
+           s.coulomb_type != 2 && wide_lds_bytes(s.L) <= kLdsLimit;
+}
+
 bool herm_wide_fits(const DevSystem<float>& s)
 {
-    return s.L > 64 && s.L <= kLP && (s.z == 4 || s.z == 8) && !s.amp && s.coulomb_type != 2 &&
-           wide_lds_bytes(s.L) <= kLdsLimit;
+    return ((s.L > 64 && s.L <= kLP && (s.z == 4 || s.z == 8) && !s.amp && s.coulomb_type != 2 &&
+             wide_lds_bytes(s.L) <= kLdsLimit) ||
+            herm_wide_direct_preferred(s));
 }
 
 hipError_t launch_herm_wide(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                             hipStream_t st)
 {
     if (!herm_wide_fits(s) || r.dens || r.B <= 0 || r.B > 0x7fffffff) return hipErrorInvalidValue;
-    const int S = std::max(5, (dev::class_rows(s.L) + kW - 1) / kW);
+    const int S = std::max(4, (dev::class_rows(s.L) + kW - 1) / kW);
     if (s.hop_uniform)
         return s.z == 4 ? run_wide_s<4, true>(S, s, r, t0, t1, state, rec, st)
                         : run_wide_s<8, true>(S, s, r, t0, t1, state, rec, st);

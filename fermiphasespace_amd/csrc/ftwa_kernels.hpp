@@ -119,8 +119,11 @@ hipError_t launch_herm_seq(const DevSystem<Real>& s, const RunShape& r, int t0, 
                            int nt_hint, hipStream_t st);
 
 // Its FP32 form for 64 < L <= 107 (ftwa_herm_wide.hip): only Y in LDS, each element's commutator formed
-// directly from its rows; z 4 or 8, no pulse, null or on-site Coulomb, no density records.
+// directly from its rows; z 4 or 8, no pulse, null or on-site Coulomb, no density records; samples the
+// initial state in a run's first launch like the spin-sequential kernel.
 bool herm_wide_fits(const DevSystem<float>& s);
+// The same kernel where it beats the spin-sequential kernel for L <= 64 (L = 64, z = 4, one hopping value).
+bool herm_wide_direct_preferred(const DevSystem<float>& s);
 hipError_t launch_herm_wide(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec,
                             hipStream_t st);
 

@@ -226,8 +226,10 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
 /* Engine tuning (no reference counterpart; the autotuner's knobs): threads per trajectory
  * workgroup for the full-matrix batched kernel (512 or 1024; 0 = automatic), time points per launch
  * (0 = automatic) and the FP32 kernel variant (FPSFTWA_KERNEL_*). */
-#define FPSFTWA_KERNEL_AUTO 0      /* FP32: spin-sequential where it fits, else upper-triangle,
-                                    * else full-matrix; FP64 likewise without the FP32-only ones   */
+#define FPSFTWA_KERNEL_AUTO 0      /* FP32: spin-sequential where it fits (at L = 64 with four
+                                    * neighbours and one hopping value in its direct-commutator
+                                    * form), else upper-triangle, else full-matrix; FP64 likewise
+                                    * without the FP32-only ones                                    */
 #define FPSFTWA_KERNEL_FULL 1      /* full-matrix stencil kernel                                   */
 #define FPSFTWA_KERNEL_HERMITIAN 2 /* upper-triangle kernel (L <= 64; in FP64 the spin-sequential
                                     * one, no pulse, null or on-site Coulomb)                         */
@@ -237,9 +239,10 @@ int fpsftwa_get_timing(fpsftwa_handle* h, int64_t* launches, double* kernel_ms, 
 #define FPSFTWA_KERNEL_PIPELINED 4 /* upper-triangle kernel with the two spins half a stage apart
                                     * (FP32, L <= 64, no pulse, null or on-site Coulomb)              */
 #define FPSFTWA_KERNEL_SEQUENTIAL 5 /* upper-triangle kernel with one spin's rows in LDS at a time,
-                                    * two workgroups per CU in FP32 (L <= 64; FP32 also 64 < L <= 107
-                                    * with X formed in 64-column blocks; no pulse, null or on-site
-                                    * Coulomb, no basis occupations)                                  */
+                                    * two workgroups per CU in FP32 (L <= 64, X = T Y in LDS; FP32
+                                    * also 64 < L <= 107, each element's commutator formed directly
+                                    * from the rows; no pulse, null or on-site Coulomb, no basis
+                                    * occupations)                                                    */
 #define FPSFTWA_KERNEL_DENSE_MFMA 6 /* FP32 dense path with the product X = Y T and the RK update fused in
                                     * one hand-written matrix-core kernel per stage (v_mfma_f32_16x16x4f32,
                                     * 8 trajectory-spin slices per workgroup sharing the hopping operand
