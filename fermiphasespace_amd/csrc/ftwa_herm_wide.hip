@@ -37,8 +37,9 @@ constexpr int kCB = 8;           // bytes per complex
 constexpr int kRB = 4;           // bytes per real
 // LDS layout: h [2 stage parities][2 spins][144], stage-input diagonals [2][144], energy partials
 // [8 waves][4], site tables tM_ii / E_pe potential / U_ii / pdiag [4][128], neighbour hoppings [8][128],
-// neighbour row offsets [8][128] (ints), the neighbour sites packed a byte each [2][128] (word w of site i
-// holds k_{4w .. 4w+3}(i)), 16 spare bytes, then L rows and a spare row.
+// neighbour row offsets [8][128] (ints), the neighbours' row and column offsets packed two to a word
+// [128][4] each (site-major: word w of site i holds k_{2w}(i) and k_{2w+1}(i)), 16 spare bytes, then L rows
+// and a spare row.
 constexpr int kHSpin = 144 * kRB;  // entries 128..143 take padding slots (always zero)
 constexpr int kHPar = 2 * kHSpin;
 constexpr int kHB = 0;
@@ -47,8 +48,9 @@ constexpr int kEW = kDg + kHPar;
 constexpr int kSite = kEW + kW * 4 * kRB;
 constexpr int kNbrT = kSite + 4 * kLP * kRB;
 constexpr int kNbrK = kNbrT + 8 * kLP * kRB;
-constexpr int kNbrP = kNbrK + 8 * kLP * 4;
-constexpr int kZero = kNbrP + 2 * kLP * 4;
+constexpr int kNbrR = kNbrK + 8 * kLP * 4;
+constexpr int kNbrC = kNbrR + 4 * kLP * 4;
+constexpr int kZero = kNbrC + 4 * kLP * 4;
 constexpr int kRows = kZero + 16;
 static_assert(kRows % 16 == 0, "rows are 16-byte aligned");
 constexpr size_t kLdsLimit = 160 * 1024;
@@ -75,6 +77,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
                      float* __restrict__ rec)
 {
     extern __shared__ __align__(16) unsigned char smem[];
+    constexpr bool kRowBytes = S <= 4;  // L <= 64: neighbour row offsets in bytes fit 16 bits
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
@@ -113,12 +116,19 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = on ? s.nbr_t[n * L + i] : 0.f;
         *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + k * RB;
     }
-    for (int e = tid; e < (Z / 4) * kLP; e += kNT) {
-        const int wd = e / kLP, i = e - wd * kLP;
-        uint32_t pk = 0;
+    // 16-bit offsets of Y(k_n(i), 0) from the rows (bytes for L <= 64, else complex elements: at most
+    // 127 x 129) and of Y(0, k_n(i)) within a row (bytes), two neighbours to a word
+    for (int e = tid; e < (Z / 2) * kLP; e += kNT) {
+        const int i = e / (Z / 2), wd = e - i * (Z / 2);
+        uint32_t rw = 0, cw = 0;
         if (i < L)
-            for (int m = 0; m < 4; ++m) pk |= static_cast<uint32_t>(s.nbr_k[(4 * wd + m) * L + i]) << (8 * m);
-        *reinterpret_cast<uint32_t*>(smem + kNbrP + e * 4) = pk;
+            for (int m = 0; m < 2; ++m) {
+                const int k = s.nbr_k[(2 * wd + m) * L + i];
+                rw |= static_cast<uint32_t>(kRowBytes ? k * RB : k * (RB / kCB)) << (16 * m);
+                cw |= static_cast<uint32_t>(k * kCB) << (16 * m);
+            }
+        *reinterpret_cast<uint32_t*>(smem + kNbrR + (i * 4 + wd) * 4) = rw;
+        *reinterpret_cast<uint32_t*>(smem + kNbrC + (i * 4 + wd) * 4) = cw;
     }
 
     // ---- element slots (class order): ah = i | j << 16 of the slot's element; padding lanes hold
@@ -261,18 +271,30 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
             slot(q, i, jj);
             const int ri = kRows + rowb(i), rj = kRows + jj * kCB;
             float ax = 0, ay = 0, bx = 0, by = 0;  // conj(X_ji), X_ij
-            uint32_t pi[Z / 4], pj[Z / 4];  // the neighbour sites of i and j
-#pragma unroll
-            for (int wd = 0; wd < Z / 4; ++wd) {
-                pi[wd] = ldl<uint32_t>(smem, kNbrP + (wd * kLP + i) * 4);
-                pj[wd] = ldl<uint32_t>(smem, kNbrP + (wd * kLP + jj) * 4);
+            uint32_t pr[Z / 2], pc[Z / 2];  // row offsets of i's neighbours, column offsets of j's
+            if constexpr (Z == 4) {
+                const uint2 a = ldl<uint2>(smem, kNbrR + i * 16), c = ldl<uint2>(smem, kNbrC + jj * 16);
+                pr[0] = a.x;
+                pr[1] = a.y;
+                pc[0] = c.x;
+                pc[1] = c.y;
+            } else {
+                const uint4 a = ldl<uint4>(smem, kNbrR + i * 16), c = ldl<uint4>(smem, kNbrC + jj * 16);
+                pr[0] = a.x;
+                pr[1] = a.y;
+                pr[2] = a.z;
+                pr[3] = a.w;
+                pc[0] = c.x;
+                pc[1] = c.y;
+                pc[2] = c.z;
+                pc[3] = c.w;
             }
 #pragma unroll
             for (int n = 0; n < Z; ++n) {
-                const int ki = (pi[n / 4] >> (8 * (n % 4))) & 0xff, kj = (pj[n / 4] >> (8 * (n % 4))) & 0xff;
+                const uint32_t ro = (pr[n / 2] >> (16 * (n % 2))) & 0xffff, co = (pc[n / 2] >> (16 * (n % 2))) & 0xffff;
                 const float tj = hop(n, jj), ti = hop(n, i);
-                const C u = ldl<C>(smem, ri + kj * kCB);     // Y_{i, k_n(j)} = conj(Y_{k_n(j), i})
-                const C v = ldl<C>(smem, rj + rowb(ki));    // Y_{k_n(i), j}
+                const C u = ldl<C>(smem, ri + co);                          // Y_{i, k_n(j)} = conj(Y_{k_n(j), i})
+                const C v = ldl<C>(smem, rj + (kRowBytes ? ro : ro * kCB));  // Y_{k_n(i), j}
                 ax += tj * u.x;
                 ay += tj * u.y;
                 bx += ti * v.x;
