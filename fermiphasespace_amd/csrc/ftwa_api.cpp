@@ -1526,6 +1526,9 @@ extern "C" int fpsftwa_autotune(fpsftwa_handle* h, int32_t trial_steps, int32_t*
         cands.push_back({FPSFTWA_KERNEL_HERMITIAN, 0, 8});
         cands.push_back({FPSFTWA_KERNEL_DENSE_MFMA, 0, 0});  // the fused matrix-core form of the dense path
     }
+    // the automatic choice last, kept only when strictly faster than every named kernel: it holds a form no
+    // named kernel selects (the direct-commutator kernel on the 8x8 lattice)
+    cands.push_back({FPSFTWA_KERNEL_AUTO, 0, 0});
     // a shortened copy of the configured run: trial_steps steps of at most 16,384 trajectories
     const double t_max = h->t_max;
     const int64_t B = h->B;
