@@ -853,6 +853,51 @@ def test_long_range_hopping_without_pulse_dense(oracle_mod):
             B=5)
 
 
+def _long_range_coulomb_system(nx, ny, with_pulse):
+    """Long-range hopping to 2.5a and a distance Coulomb V = 1/(1 + r) with on-site U = 3."""
+    pos, box = lat.grid(nx, ny)
+    L = pos.shape[0]
+    d = pos[:, None, :] - pos[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    tM = np.where((r > 0) & (r <= 2.5), -np.exp(-(r - 1.0) / 0.3), 0.0)
+    V = np.where(r > 0, 1.0 / (1.0 + r), 0.0)
+    pulse = (dict(pulse_type=1, A0=0.5, t0=0.05, td=0.1, w0=1.5, phi=0.0, pol=(2 ** -0.5, 2 ** -0.5, 0.0), disp=d)
+             if with_pulse else None)
+    n_up = np.array([1.0, 0.0] * (L // 2))
+    return tM, 3.0 * np.eye(L), V, pulse, n_up
+
+
+@pytest.mark.parametrize("with_pulse", [True, False])
+def test_long_range_general_coulomb_substeps(oracle_mod, with_pulse):
+    """The dense path with a distance Coulomb table and two RK sub-steps per time point: the fused
+    matrix-core kernel's Hartree fields, Coulomb energies and hopping-energy partial sums (formed in the
+    last stage of the last sub-step) against the oracle, with nine trajectories (a partial Hartree
+    workgroup of 8 and a partial group of slices); L = 32, K = 64 with a pulse and 32 without."""
+    tM, U, V, pulse, n_up = _long_range_coulomb_system(8, 4, with_pulse)
+    compare(oracle_mod, tM, n_up, 1 - n_up, coulomb_type=2, U=U, V=V, pot=np.linspace(-0.3, 0.3, tM.shape[0]),
+            pulse=pulse, dt=0.01, t_max=0.06, n_sub=2, B=9, engines=("fp64", "fp32"))
+
+
+def test_dense_matrix_core_chunking():
+    """The fused matrix-core kernel across launches of 1, 2 and 3 time points and in one launch: the
+    state, the diagonal records and the doublons are bitwise the same (the first record of a launch reads
+    the stage input instead of the previous launch's partial sums, so the energies agree to rounding)."""
+    tM, U, V, pulse, n_up = _long_range_coulomb_system(8, 4, True)
+    eng = _lib.Engine(tM, coulomb_type=2, U=U, V=V, **pulse)
+    eng.set_initial_state(n_up, 1 - n_up)
+    ref = None
+    for chunk in (0, 1, 2, 3):
+        eng.set_tuning(0, chunk, "dense_mfma")
+        got = engine_run(eng, 0.01, 0.07, 2, 11, "fp32", seed=5)
+        if ref is None:
+            ref = got
+            continue
+        for k in ("states", "sum_u", "sum_d", "sum2_u", "sum_ud", "corner"):
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=f"chunk {chunk} {k}")
+        assert rel(got["sum_e"], ref["sum_e"]) < 1e-6, chunk
+    eng.close()
+
+
 def test_moire196_shape(oracle_mod):
     """The long-range twisted bilayer (L = 196, 17-65 neighbours per site, dense distance Coulomb U and V,
     Gaussian pulse) through the dense path, two trajectories over a few steps."""
