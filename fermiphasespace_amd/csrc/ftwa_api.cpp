@@ -656,8 +656,9 @@ int run_noalloc(fpsftwa_handle* h, void* state, size_t state_bytes, double* sums
     KernelChoice k;
     if ((rc = choose_kernel<Real>(h, ds, k))) return rc;
     if (k.dense)
-        return fail(FPSFTWA_EUNSUPPORTED, "the dense path (L > 64 or long-range hopping) runs on a GEMM library "
-                                          "that allocates its workspace: use fpsftwa_run");
+        return fail(FPSFTWA_EUNSUPPORTED, "the dense path (L > 64 or long-range hopping) keeps its stage input and "
+                                          "RK registers (several n_traj x L x L buffers) beside the state, and its "
+                                          "library GEMM form a workspace: use fpsftwa_run");
     h->used_herm = k.herm;
     h->used_dense = false;
     // time points per launch: as many as the rest of the scratch holds (at most 64, or the tuned chunk)
