@@ -35,22 +35,25 @@ constexpr int kNT = 64 * kW;
 constexpr int kLP = 128;         // site capacity of the tables
 constexpr int kCB = 8;           // bytes per complex
 constexpr int kRB = 4;           // bytes per real
-// LDS layout: h [2 stage parities][2 spins][144], stage-input diagonals [2][144], energy partials
-// [8 waves][4], site tables tM_ii / E_pe potential / U_ii / pdiag [4][128], neighbour hoppings [8][128],
-// neighbour row offsets [8][128] (ints), the neighbours' row and column offsets packed two to a word
-// [128][4] each (site-major: word w of site i holds k_{2w}(i) and k_{2w+1}(i)), 16 spare bytes, then L rows
-// and a spare row.
-constexpr int kHSpin = 144 * kRB;  // entries 128..143 take padding slots (always zero)
-constexpr int kHPar = 2 * kHSpin;
+// LDS layout: h [2 stage parities][144 sites][2 spins], stage-input diagonals [2 spins][144], energy
+// partials [8 waves][4], site tables tM_ii / E_pe potential / U_ii / pdiag [4][128], neighbour hoppings
+// [8][128], neighbour row offsets [8][128] (ints), the neighbours' row and column offsets, 16 bits each, four
+// to an 8-byte entry [2][128] each (entry p of site i holds k_{4p}(i) .. k_{4p+3}(i)), 16 spare bytes, then L
+// rows and a spare row.  Every per-site entry the element slots read is 8 bytes, one complex: a slot holds
+// 8 i and 8 j, which address its rows' tables and h as they address the columns of Y.
+constexpr int kHSite = 2 * kRB;      // h of one site, both spins
+constexpr int kHPar = 144 * kHSite;  // sites 128..143 take padding slots (always zero)
 constexpr int kHB = 0;
+constexpr int kHSpin = 144 * kRB;
+static_assert(kHSite == kCB, "h entries are addressed like complex columns");
 constexpr int kDg = 2 * kHPar;
-constexpr int kEW = kDg + kHPar;
+constexpr int kEW = kDg + 2 * kHSpin;
 constexpr int kSite = kEW + kW * 4 * kRB;
 constexpr int kNbrT = kSite + 4 * kLP * kRB;
 constexpr int kNbrK = kNbrT + 8 * kLP * kRB;
 constexpr int kNbrR = kNbrK + 8 * kLP * 4;
-constexpr int kNbrC = kNbrR + 4 * kLP * 4;
-constexpr int kZero = kNbrC + 4 * kLP * 4;
+constexpr int kNbrC = kNbrR + 2 * kLP * kCB;
+constexpr int kZero = kNbrC + 2 * kLP * kCB;
 constexpr int kRows = kZero + 16;
 static_assert(kRows % 16 == 0, "rows are 16-byte aligned");
 constexpr size_t kLdsLimit = 160 * 1024;
@@ -117,21 +120,21 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + k * RB;
     }
     // 16-bit offsets of Y(k_n(i), 0) from the rows (bytes for L <= 64, else complex elements: at most
-    // 127 x 129) and of Y(0, k_n(i)) within a row (bytes), two neighbours to a word
-    for (int e = tid; e < (Z / 2) * kLP; e += kNT) {
-        const int i = e / (Z / 2), wd = e - i * (Z / 2);
-        uint32_t rw = 0, cw = 0;
+    // 127 x 129) and of Y(0, k_n(i)) within a row (bytes), four neighbours to an entry
+    for (int e = tid; e < (Z / 4) * kLP; e += kNT) {
+        const int p = e / kLP, i = e - p * kLP;
+        uint32_t rw[2] = {0, 0}, cw[2] = {0, 0};
         if (i < L)
-            for (int m = 0; m < 2; ++m) {
-                const int k = s.nbr_k[(2 * wd + m) * L + i];
-                rw |= static_cast<uint32_t>(kRowBytes ? k * RB : k * (RB / kCB)) << (16 * m);
-                cw |= static_cast<uint32_t>(k * kCB) << (16 * m);
+            for (int m = 0; m < 4; ++m) {
+                const int k = s.nbr_k[(4 * p + m) * L + i];
+                rw[m / 2] |= static_cast<uint32_t>(kRowBytes ? k * RB : k * (RB / kCB)) << (16 * (m % 2));
+                cw[m / 2] |= static_cast<uint32_t>(k * kCB) << (16 * (m % 2));
             }
-        *reinterpret_cast<uint32_t*>(smem + kNbrR + (i * 4 + wd) * 4) = rw;
-        *reinterpret_cast<uint32_t*>(smem + kNbrC + (i * 4 + wd) * 4) = cw;
+        *reinterpret_cast<uint2*>(smem + kNbrR + e * kCB) = uint2{rw[0], rw[1]};
+        *reinterpret_cast<uint2*>(smem + kNbrC + e * kCB) = uint2{cw[0], cw[1]};
     }
 
-    // ---- element slots (class order): ah = i | j << 16 of the slot's element; padding lanes hold
+    // ---- element slots (class order): ah = 8 i | 8 j << 16 of the slot's element; padding lanes hold
     // i = j = L (the spare row; h entry L is zero).  The LDS addresses of Y_ij and Y_ji are formed from ah at
     // each use (registers: the RK state of both spins is 12 per slot).  The diagonal of site i is held by
     // thread i (< L): real RK registers only.
@@ -144,11 +147,11 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
 #pragma unroll
     for (int q = 0; q < S; ++q) {
         const int er = q * kW + w;
-        ah[q] = L | L << 16;
+        ah[q] = L * kCB | (L * kCB) << 16;
         y0[q] = A0[q] = p0[q] = y1[q] = A1[q] = p1[q] = C{0, 0};
         int i = 0, jj = 0;
         if (er < n_off && dev::class_slot(L, 2 * er + (j >> 5), j & 31, i, jj)) {
-            ah[q] = i | jj << 16;
+            ah[q] = i * kCB | (jj * kCB) << 16;
             if (!sampled) {
                 y0[q] = gs[i * L + jj];
                 y1[q] = gs[LL + i * L + jj];
@@ -179,7 +182,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < S; ++q) {
-            const int i = ah[q] & 0xffff, c = ah[q] >> 16;
+            const int i = (ah[q] & 0xffff) / kCB, c = (ah[q] >> 16) / kCB;
             if (i < L) (SG ? y1[q] : y0[q]) = yat(i, c);
         }
         __syncthreads();  // the rows are rewritten next
@@ -188,13 +191,16 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         sample_pass(Ic<0>{});
         sample_pass(Ic<1>{});
     }
-    // the slot's (i, j) and its LDS offsets, decoded where used (opaque: not kept across the time loop)
-    auto slot = [&](int q, int& i, int& jj) {
+    // the slot's 8 i and 8 j, decoded where used (opaque: not kept across the time loop); 8 i times the row
+    // length in complex elements is the byte offset of row i
+    auto slot = [&](int q, int& i8, int& j8) {
         int a = ah[q];
         asm volatile("" : "+v"(a));
-        i = a & 0xffff;
-        jj = a >> 16;
+        i8 = a & 0xffff;
+        j8 = a >> 16;
     };
+    const unsigned RL = static_cast<unsigned>(wide_row_len(L));
+    auto rowb8 = [&](int x8) { return static_cast<int>(__umul24(static_cast<unsigned>(x8), RL)); };
     const bool dslot = tid < L;
     const int da = dslot ? kRows + tid * RB + tid * kCB : spare;
     float dy0 = 0.f, dy1 = 0.f;  // diagonal: n - 1/2
@@ -216,11 +222,11 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         tv = *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
     };
     const float t_uni = s.hop_value;
-    auto hop = [&](int n, int i) -> float {  // t_{i,n}: the lattice's one value, or from the table
+    auto hop = [&](int n, int x8) -> float {  // t_{x,n} of site x = x8 / 8: the lattice's one value, or the table's
         if (UNI) return t_uni;
-        int o = n * kLP + i;
+        int o = static_cast<int>(static_cast<unsigned>(x8) >> 1);
         asm volatile("" : "+v"(o));
-        return *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
+        return ldl<float>(smem, kNbrT + n * kLP * kRB + o);
     };
 
     // ---- M: the spin's stage input into the rows; in the spin-up pass, h of both spins ----
@@ -236,10 +242,10 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
             const C& pq = SG ? p1[q] : p0[q];
             const float yx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
             const float yy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
-            int i, jj;
-            slot(q, i, jj);
-            *reinterpret_cast<C*>(smem + kRows + rowb(i) + jj * kCB) = C{yx, yy};
-            *reinterpret_cast<C*>(smem + kRows + rowb(jj) + i * kCB) = C{yx, -yy};
+            int i8, j8;
+            slot(q, i8, j8);
+            *reinterpret_cast<C*>(smem + kRows + rowb8(i8) + j8) = C{yx, yy};
+            *reinterpret_cast<C*>(smem + kRows + rowb8(j8) + i8) = C{yx, -yy};
         }
         if (dslot) {
             const float yx = SG ? dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6)
@@ -251,8 +257,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
                 int o = tid * kRB;
                 asm volatile("" : "+v"(o));
                 const float u = ldl<float>(smem, kSite + 2 * kLP * kRB + o), pd = ldl<float>(smem, kSite + 3 * kLP * kRB + o);
-                *reinterpret_cast<float*>(smem + HP + o) = u * (y1x + half) + pd;
-                *reinterpret_cast<float*>(smem + HP + kHSpin + o) = u * (yx + half) + pd;
+                *reinterpret_cast<float2*>(smem + HP + 2 * o) = float2{u * (y1x + half) + pd, u * (yx + half) + pd};
                 *reinterpret_cast<float*>(smem + kDg + o) = yx;
                 *reinterpret_cast<float*>(smem + kDg + kHSpin + o) = y1x;
             }
@@ -263,36 +268,27 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
     auto phase_c = [&](auto stc, auto sgc) {
         constexpr int ST = decltype(stc)::value;
         constexpr int SG = decltype(sgc)::value;
-        constexpr int HS = kHB + (ST & 1) * kHPar + SG * kHSpin;
+        constexpr int HS = kHB + (ST & 1) * kHPar + SG * kRB;
 #pragma unroll
         for (int q = 0; q < S; ++q) {
             if (q * kW + w >= n_off) continue;
-            int i, jj;
-            slot(q, i, jj);
-            const int ri = kRows + rowb(i), rj = kRows + jj * kCB;
+            int i8, j8;
+            slot(q, i8, j8);
+            const int ri = kRows + rowb8(i8), rj = kRows + j8;
             float ax = 0, ay = 0, bx = 0, by = 0;  // conj(X_ji), X_ij
             uint32_t pr[Z / 2], pc[Z / 2];  // row offsets of i's neighbours, column offsets of j's
-            if constexpr (Z == 4) {
-                const uint2 a = ldl<uint2>(smem, kNbrR + i * 16), c = ldl<uint2>(smem, kNbrC + jj * 16);
-                pr[0] = a.x;
-                pr[1] = a.y;
-                pc[0] = c.x;
-                pc[1] = c.y;
-            } else {
-                const uint4 a = ldl<uint4>(smem, kNbrR + i * 16), c = ldl<uint4>(smem, kNbrC + jj * 16);
-                pr[0] = a.x;
-                pr[1] = a.y;
-                pr[2] = a.z;
-                pr[3] = a.w;
-                pc[0] = c.x;
-                pc[1] = c.y;
-                pc[2] = c.z;
-                pc[3] = c.w;
+#pragma unroll
+            for (int e = 0; e < Z / 4; ++e) {
+                const uint2 a = ldl<uint2>(smem, kNbrR + e * kLP * kCB + i8), c = ldl<uint2>(smem, kNbrC + e * kLP * kCB + j8);
+                pr[2 * e] = a.x;
+                pr[2 * e + 1] = a.y;
+                pc[2 * e] = c.x;
+                pc[2 * e + 1] = c.y;
             }
 #pragma unroll
             for (int n = 0; n < Z; ++n) {
                 const uint32_t ro = (pr[n / 2] >> (16 * (n % 2))) & 0xffff, co = (pc[n / 2] >> (16 * (n % 2))) & 0xffff;
-                const float tj = hop(n, jj), ti = hop(n, i);
+                const float tj = hop(n, j8), ti = hop(n, i8);
                 const C u = ldl<C>(smem, ri + co);                          // Y_{i, k_n(j)} = conj(Y_{k_n(j), i})
                 const C v = ldl<C>(smem, rj + (kRowBytes ? ro : ro * kCB));  // Y_{k_n(i), j}
                 ax += tj * u.x;
@@ -303,7 +299,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         C& yq = SG ? y1[q] : y0[q];
             C& Aq = SG ? A1[q] : A0[q];
             C& pq = SG ? p1[q] : p0[q];
-            const float dh = ldl<float>(smem, HS + jj * kRB) - ldl<float>(smem, HS + i * kRB);
+            const float dh = ldl<float>(smem, HS + j8) - ldl<float>(smem, HS + i8);  // h entries are 8 bytes
             const float ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
             const float ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
             // c = conj(X_ji) - X_ij + (h_j - h_i) Y_ij;  k = -i c
@@ -417,7 +413,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
     C* go = reinterpret_cast<C*>(state) + b * 2 * LL;
 #pragma unroll
     for (int q = 0; q < S; ++q) {
-        const int i = ah[q] & 0xffff, col = ah[q] >> 16;
+        const int i = (ah[q] & 0xffff) / kCB, col = (ah[q] >> 16) / kCB;
         if (i >= L) continue;
         go[i * L + col] = y0[q];
         go[col * L + i] = C{y0[q].x, -y0[q].y};
