@@ -400,6 +400,7 @@ __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, int i
     const int64_t bb0 = static_cast<int64_t>(c) * CB;
     const int64_t bb1 = (bb0 + CB < B) ? bb0 + CB : B;
     const int Q = 6 * L + 12;
+    constexpr int kRowsB = 16;  // rows loaded per batch
     const Real* rt = rec + static_cast<int64_t>(t) * B * R;
     double* po = part + (static_cast<int64_t>(t) * nchunk + c) * Q * 2;
     for (int slot = threadIdx.x; slot < L + 8; slot += blockDim.x) {
@@ -415,19 +416,35 @@ __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, int i
                     const double nn = (u + 0.5) * (d + 0.5);
                     snn.add(nn * nn);
                 };
-                // rows read 8 at a time before they are added (in order): the loads overlap instead of
-                // each waiting out the memory latency behind the previous row's sums
-                constexpr int kRows8 = 8;
+                // rows read 16 at a time before they are added (the loads overlap instead of each waiting
+                // out the memory latency behind the previous row's sums); a batch's terms summed in plain
+                // FP64 in row order, the batch sums compensated
                 int64_t bi = bb0;
-                for (; bi + kRows8 <= bb1; bi += kRows8) {
-                    Real u8[kRows8], d8[kRows8];
+                for (; bi + kRowsB <= bb1; bi += kRowsB) {
+                    Real ub[kRowsB], db[kRowsB];
 #pragma unroll
-                    for (int q = 0; q < kRows8; ++q) {
-                        u8[q] = rt[(bi + q) * R + slot];
-                        d8[q] = rt[(bi + q) * R + L + slot];
+                    for (int q = 0; q < kRowsB; ++q) {
+                        ub[q] = rt[(bi + q) * R + slot];
+                        db[q] = rt[(bi + q) * R + L + slot];
                     }
+                    double bu = 0, bd = 0, buu = 0, bdd = 0, bud = 0, bnn = 0;
 #pragma unroll
-                    for (int q = 0; q < kRows8; ++q) add(static_cast<double>(u8[q]), static_cast<double>(d8[q]));
+                    for (int q = 0; q < kRowsB; ++q) {
+                        const double u = static_cast<double>(ub[q]), d = static_cast<double>(db[q]);
+                        bu += u;
+                        bd += d;
+                        buu += u * u;
+                        bdd += d * d;
+                        bud += u * d;
+                        const double nn = (u + 0.5) * (d + 0.5);
+                        bnn += nn * nn;
+                    }
+                    su.add(bu);
+                    sd.add(bd);
+                    suu.add(buu);
+                    sdd.add(bdd);
+                    sud.add(bud);
+                    snn.add(bnn);
                 }
                 for (; bi < bb1; ++bi) add(static_cast<double>(rt[bi * R + slot]), static_cast<double>(rt[bi * R + L + slot]));
             } else {
@@ -453,18 +470,20 @@ __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, int i
         } else if (slot < L + 4) {
             const int k = slot - L;
             Neumaier se, se2;
-            constexpr int kRows8 = 8;
             int64_t bi = bb0;
-            for (; bi + kRows8 <= bb1; bi += kRows8) {
-                Real e8[kRows8];
+            for (; bi + kRowsB <= bb1; bi += kRowsB) {
+                Real eb[kRowsB];
 #pragma unroll
-                for (int q = 0; q < kRows8; ++q) e8[q] = rt[(bi + q) * R + 2 * L + k];
+                for (int q = 0; q < kRowsB; ++q) eb[q] = rt[(bi + q) * R + 2 * L + k];
+                double be = 0, be2 = 0;
 #pragma unroll
-                for (int q = 0; q < kRows8; ++q) {
-                    const double e = static_cast<double>(e8[q]);
-                    se.add(e);
-                    se2.add(e * e);
+                for (int q = 0; q < kRowsB; ++q) {
+                    const double e = static_cast<double>(eb[q]);
+                    be += e;
+                    be2 += e * e;
                 }
+                se.add(be);
+                se2.add(be2);
             }
             for (; bi < bb1; ++bi) {
                 const double e = static_cast<double>(rt[bi * R + 2 * L + k]);
@@ -478,14 +497,15 @@ __global__ void reduce_partial_kernel(int L, int64_t B, int R, int nchunk, int i
         } else {
             const int k = slot - L - 4;
             Neumaier sc;
-            constexpr int kRows8 = 8;
             int64_t bi = bb0;
-            for (; bi + kRows8 <= bb1; bi += kRows8) {
-                Real c8[kRows8];
+            for (; bi + kRowsB <= bb1; bi += kRowsB) {
+                Real cb[kRowsB];
 #pragma unroll
-                for (int q = 0; q < kRows8; ++q) c8[q] = rt[(bi + q) * R + 2 * L + 4 + k];
+                for (int q = 0; q < kRowsB; ++q) cb[q] = rt[(bi + q) * R + 2 * L + 4 + k];
+                double bc = 0;
 #pragma unroll
-                for (int q = 0; q < kRows8; ++q) sc.add(static_cast<double>(c8[q]));
+                for (int q = 0; q < kRowsB; ++q) bc += static_cast<double>(cb[q]);
+                sc.add(bc);
             }
             for (; bi < bb1; ++bi) sc.add(static_cast<double>(rt[bi * R + 2 * L + 4 + k]));
             po[(6 * L + 8 + k) * 2] = sc.s;
@@ -502,10 +522,24 @@ __global__ void reduce_final_kernel(int Q, int nchunk, int nt, int t0, const dou
     const int t = static_cast<int>(idx / Q);
     const int q = static_cast<int>(idx - static_cast<int64_t>(t) * Q);
     Neumaier a;
-    for (int c = 0; c < nchunk; ++c) {
-        const double* pp = part + ((static_cast<int64_t>(t) * nchunk + c) * Q + q) * 2;
-        a.add(pp[0]);
-        a.add(pp[1]);
+    // chunks loaded 16 at a time before they are added in order (the loads overlap; the sum is unchanged)
+    constexpr int kC = 16;
+    const double2* pp = reinterpret_cast<const double2*>(part) + static_cast<int64_t>(t) * nchunk * Q + q;
+    int c = 0;
+    for (; c + kC <= nchunk; c += kC) {
+        double2 v[kC];
+#pragma unroll
+        for (int k = 0; k < kC; ++k) v[k] = pp[static_cast<int64_t>(c + k) * Q];
+#pragma unroll
+        for (int k = 0; k < kC; ++k) {
+            a.add(v[k].x);
+            a.add(v[k].y);
+        }
+    }
+    for (; c < nchunk; ++c) {
+        const double2 v = pp[static_cast<int64_t>(c) * Q];
+        a.add(v.x);
+        a.add(v.y);
     }
     double* o = acc + (static_cast<int64_t>(t0 + t) * Q + q) * 2;
     o[0] = a.s;
