@@ -535,8 +535,9 @@ def main():
     valu_achieved = valu_flop * B * K / kern_s / 1e12
     # The state stays on chip for a whole launch (measured HBM traffic is a few per cent of the streamed
     # state's bytes), so HBM does not bind: the line reports the arithmetic figure by the fixed formula
-    # (64 z + 104) L^2 flop per trajectory-step against the FP32 vector peak, with the LDS activity the
-    # committed counters measured beside it, and the streamed-state HBM figure as a secondary field.
+    # (64 z + 104) L^2 flop per trajectory-step against the FP32 vector peak, with the VALU issue, LDS
+    # activity and LDS bank-conflict cycles the committed counters measured beside it (issue and LDS are
+    # both near their limits), and the streamed-state HBM figure as a secondary field.
     roof = {"bound": "valu", "achieved": round(valu_achieved, 2), "peak": 157.3, "unit": "TFLOP/s",
             "frac": round(valu_achieved / 157.3, 4), "traffic": traffic,
             "kernel": kname, "launches": tm["launches"],
@@ -544,6 +545,7 @@ def main():
             "alg_flop_per_traj_step": valu_flop,
             "lds_active_frac": prof.get("lds_active_frac") if prof else None,
             "valu_issue_frac": prof.get("valu_issue_frac") if prof else None,
+            "lds_bank_conflict_frac": prof.get("lds_bank_conflict_frac") if prof else None,
             "hbm": {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "alg_bytes_per_traj_step": 32 * L * L,
                     "note": "streamed-state model: one read and one write of both spins' L x L blocks per step"}}
