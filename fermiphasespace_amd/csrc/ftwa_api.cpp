@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -158,6 +159,55 @@ int check_handle(fpsftwa_handle* h)
     return FPSFTWA_OK;
 }
 
+// The neighbour entries of every site (ELL form of tM's off-diagonal non-zeros), z entries per site with
+// z the batched kernels' compiled width (4 or 8 where it applies); -1 marks an empty entry.
+// In index order (by_shift false) the entries of a site are its neighbours in column order, empty ones
+// last.  By shift, entry n of every site holds, where the site has one, the neighbour at the n-th most
+// common shift k - i modulo 32 (ties: the smaller shift first); the others fill the remaining entries in
+// index order.  The direct-commutator kernel, whose rows are 1 mod 32 complex long, then reads entry n of
+// every lane's site at one bank offset from the lane's own element, so a wave's neighbour reads are
+// conflict-free except where a neighbour wraps around a periodic boundary (on the 8x8 square: entries by
+// direction, the x wraps excepted; in index order the directions of sites on the boundary rows and
+// columns land in different entries).
+std::vector<std::vector<int>> neighbour_entries(const fpsftwa_handle* h, int& z, bool by_shift)
+{
+    const int L = h->L;
+    std::vector<std::vector<int>> nb(L);
+    z = 0;
+    for (int i = 0; i < L; ++i) {
+        for (int k = 0; k < L; ++k)
+            if (k != i && h->tM[(size_t)i * L + k] != 0.0) nb[i].push_back(k);
+        z = std::max<int>(z, (int)nb[i].size());
+    }
+    z = std::max(z, 1);
+    if (const int zp = stencil_padded_z(z)) z = zp;  // the batched kernel's compiled width
+    if (!by_shift) {
+        for (int i = 0; i < L; ++i) nb[i].resize(z, -1);
+        return nb;
+    }
+    std::array<int, 32> hist{};
+    for (int i = 0; i < L; ++i)
+        for (int k : nb[i]) ++hist[((k - i) % 32 + 32) % 32];
+    std::vector<int> shifts(32);
+    for (int d = 0; d < 32; ++d) shifts[d] = d;
+    std::stable_sort(shifts.begin(), shifts.end(), [&](int a, int b) { return hist[a] > hist[b]; });
+    for (int i = 0; i < L; ++i) {
+        std::vector<int> left = nb[i], slot(z, -1);
+        for (int n = 0; n < z && n < 32 && hist[shifts[n]] > 0; ++n)
+            for (size_t m = 0; m < left.size(); ++m)
+                if (((left[m] - i) % 32 + 32) % 32 == shifts[n]) {
+                    slot[n] = left[m];
+                    left.erase(left.begin() + (std::ptrdiff_t)m);
+                    break;
+                }
+        size_t m = 0;
+        for (int n = 0; n < z && m < left.size(); ++n)
+            if (slot[n] < 0) slot[n] = left[m++];
+        nb[i] = slot;
+    }
+    return nb;
+}
+
 // Upload the static system in precision Real into one device allocation and fill the view.
 // dst (capacity cap bytes), when given, receives the tables instead of the handle's own buffer; *used is
 // set to the bytes taken there.
@@ -166,31 +216,26 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T, char* dst = nul
                   size_t* used = nullptr)
 {
     const int L = h->L;
-    // ELL neighbour lists of tM (off-diagonal non-zeros), entry-major
-    std::vector<std::vector<int>> nb(L);
     int z = 0;
-    for (int i = 0; i < L; ++i) {
-        for (int k = 0; k < L; ++k)
-            if (k != i && h->tM[(size_t)i * L + k] != 0.0) nb[i].push_back(k);
-        z = std::max<int>(z, (int)nb[i].size());
-    }
-    z = std::max(z, 1);
-    if (const int zp = stencil_padded_z(z)) z = zp;  // the batched kernel's compiled width
+    const std::vector<std::vector<int>> nb = neighbour_entries(h, z, false), nbs = neighbour_entries(h, z, true);
     const double* pol = h->pol;
     auto pdot = [&](int i, int k) {
         if (h->disp.empty()) return 0.0;
         const double* d = &h->disp[((size_t)i * L + k) * 3];
         return pol[0] * d[0] + pol[1] * d[1] + pol[2] * d[2];
     };
-    std::vector<int> nk((size_t)z * L);
-    std::vector<Real> nt((size_t)z * L), npd((size_t)z * L);
+    std::vector<int> nk((size_t)z * L), nks((size_t)z * L);
+    std::vector<Real> nt((size_t)z * L), npd((size_t)z * L), nts((size_t)z * L);
     for (int i = 0; i < L; ++i)
         for (int n = 0; n < z; ++n) {
-            const bool on = n < (int)nb[i].size();
+            const bool on = nb[i][n] >= 0;
             const int k = on ? nb[i][n] : i;
             nk[(size_t)n * L + i] = k;
             nt[(size_t)n * L + i] = on ? (Real)h->tM[(size_t)i * L + k] : Real(0);
             npd[(size_t)n * L + i] = on ? (Real)pdot(i, k) : Real(0);
+            const int ks = nbs[i][n] >= 0 ? nbs[i][n] : i;
+            nks[(size_t)n * L + i] = ks;
+            nts[(size_t)n * L + i] = nbs[i][n] >= 0 ? (Real)h->tM[(size_t)i * L + ks] : Real(0);
         }
     // per-site records for the batched kernel (z is 4 or 8 there; other widths use the plain path)
     std::vector<unsigned char> rows;
@@ -247,6 +292,7 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T, char* dst = nul
         return o;
     };
     const size_t o_nk = take(nk.size() * sizeof(int)), o_nt = take(nt.size() * sizeof(Real)),
+                 o_nks = take(nks.size() * sizeof(int)), o_nts = take(nts.size() * sizeof(Real)),
                  o_npd = take(npd.size() * sizeof(Real)), o_ud = take(L * sizeof(Real)),
                  o_pd = take(L * sizeof(Real)), o_td = take(L * sizeof(Real)), o_pot = take(L * sizeof(Real)), o_epot = take(L * sizeof(Real)),
                  o_U = take(Ud.size() * sizeof(Real)), o_V = take(Vd.size() * sizeof(Real)),
@@ -268,6 +314,8 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T, char* dst = nul
     };
     HIP_TRY(up(o_nk, nk.data(), nk.size() * sizeof(int)));
     HIP_TRY(up(o_nt, nt.data(), nt.size() * sizeof(Real)));
+    HIP_TRY(up(o_nks, nks.data(), nks.size() * sizeof(int)));
+    HIP_TRY(up(o_nts, nts.data(), nts.size() * sizeof(Real)));
     HIP_TRY(up(o_npd, npd.data(), npd.size() * sizeof(Real)));
     HIP_TRY(up(o_ud, udiag.data(), L * sizeof(Real)));
     HIP_TRY(up(o_pd, pdiag.data(), L * sizeof(Real)));
@@ -286,6 +334,8 @@ int upload_system(fpsftwa_handle* h, DevSystem<Real>& ds, int T, char* dst = nul
     ds.coulomb_type = h->coulomb_type;
     ds.nbr_k = reinterpret_cast<const int*>(base + o_nk);
     ds.nbr_t = reinterpret_cast<const Real*>(base + o_nt);
+    ds.nbr_ks = reinterpret_cast<const int*>(base + o_nks);
+    ds.nbr_ts = reinterpret_cast<const Real*>(base + o_nts);
     ds.nbr_pd = reinterpret_cast<const Real*>(base + o_npd);
     ds.u_diag = reinterpret_cast<const Real*>(base + o_ud);
     ds.pdiag = reinterpret_cast<const Real*>(base + o_pd);
@@ -897,12 +947,12 @@ int run_adjoint(fpsftwa_handle* h, int ck_every, double* objective, double* grad
     if (grad_t) {
         for (size_t e = 0; e < (size_t)L * L; ++e) grad_t[e] = 0.0;
         for (int k = 0; k < L; ++k) grad_t[(size_t)k * L + k] = g[L + k] * inv;
-        // the ELL order of upload_system: off-diagonal non-zeros of row k in column order
-        for (int k = 0; k < L; ++k) {
-            int n = 0;
-            for (int l = 0; l < L && n < ds.z; ++l)
-                if (l != k && h->tM[(size_t)k * L + l] != 0.0) grad_t[(size_t)k * L + l] = g[2 * L + (size_t)n++ * L + k] * inv;
-        }
+        // the entries of upload_system's neighbour tables
+        int zz = 0;
+        const std::vector<std::vector<int>> ent = neighbour_entries(h, zz, false);
+        for (int k = 0; k < L; ++k)
+            for (int n = 0; n < ds.z && n < zz; ++n)
+                if (ent[k][n] >= 0) grad_t[(size_t)k * L + ent[k][n]] = g[2 * L + (size_t)n * L + k] * inv;
     }
     if (grad_pulse && n_pulse && n_stage) {
         std::vector<double> dJda(g.begin() + 2 * L + (size_t)ds.z * L, g.begin() + G);

@@ -115,8 +115,8 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
     for (int e = tid; e < Z * kLP; e += kNT) {
         const int n = e / kLP, i = e - n * kLP;
         const bool on = i < L;
-        const int k = on ? s.nbr_k[n * L + i] : 0;
-        *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = on ? s.nbr_t[n * L + i] : 0.f;
+        const int k = on ? s.nbr_ks[n * L + i] : 0;
+        *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = on ? s.nbr_ts[n * L + i] : 0.f;
         *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + k * RB;
     }
     // 16-bit offsets of Y(k_n(i), 0) from the rows (bytes for L <= 64, else complex elements: at most
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         uint32_t rw[2] = {0, 0}, cw[2] = {0, 0};
         if (i < L)
             for (int m = 0; m < 4; ++m) {
-                const int k = s.nbr_k[(4 * p + m) * L + i];
+                const int k = s.nbr_ks[(4 * p + m) * L + i];
                 rw[m / 2] |= static_cast<uint32_t>(kRowBytes ? k * RB : k * (RB / kCB)) << (16 * (m % 2));
                 cw[m / 2] |= static_cast<uint32_t>(k * kCB) << (16 * (m % 2));
             }

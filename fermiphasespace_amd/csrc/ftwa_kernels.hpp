@@ -38,6 +38,8 @@ struct DevSystem {
     int coulomb_type = 0;          // 0 null, 1 onSite, 2 general
     const int* nbr_k = nullptr;    // [z][L]  neighbour k of site i (tM_ik != 0, k != i)
     const Real* nbr_t = nullptr;   // [z][L]  tM_ik
+    const int* nbr_ks = nullptr;   // [z][L]  the same neighbours, entry n at the n-th most common shift
+    const Real* nbr_ts = nullptr;  // [z][L]  k - i mod 32 where the site has one (the direct-commutator kernel)
     const Real* nbr_pd = nullptr;  // [z][L]  pol . (r_i - r_k)  (pulse phase per unit amplitude)
     const Real* u_diag = nullptr;  // [L]     U_ii
     const Real* U = nullptr;       // [L][L]  dense U (general)
