@@ -72,6 +72,14 @@ __device__ __forceinline__ T ldl(const unsigned char* base, int off)
 template <int N>
 using Ic = std::integral_constant<int, N>;
 
+// LDS of the L <= 64 instances, a static array whose offsets the compiler folds into the LDS instructions:
+// the tables, then L + 1 <= 65 rows of wide_row_len(L) <= 65 complex.  Past L = 64 the launch sizes the
+// dynamic array.  The kernel names either array directly (a pointer variable that the lambdas capture
+// costs the L = 100 instance 3 % more VALU instructions).
+constexpr int kStaticLds = kRows + 65 * 65 * kCB;
+__shared__ __align__(16) unsigned char wide_static_lds[kStaticLds];
+extern __shared__ __align__(16) unsigned char wide_dyn_lds[];
+
 // UNI: every neighbour entry holds the same hopping s.hop_value (a regular lattice with one bond type),
 // taken from a register instead of the table.
 template <int S, int Z, bool UNI>
@@ -79,7 +87,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
     herm_wide_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state,
                      float* __restrict__ rec)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
+#define WIDE_LDS (S <= 4 ? wide_static_lds : wide_dyn_lds)  // this kernel's LDS (a global array, not captured)
     constexpr bool kRowBytes = S <= 4;  // L <= 64: neighbour row offsets in bytes fit 16 bits
     const int L = s.L;
     const int LL = L * L;
@@ -92,23 +100,23 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
     const int64_t b = blockIdx.x;
     const int spare = kRows + L * RB;
     const float half = 0.5f;
-    float* ew = reinterpret_cast<float*>(smem + kEW);
+    float* ew = reinterpret_cast<float*>(WIDE_LDS + kEW);
     // byte offset of a row: rows < 128 and strides < 1,400 B, so the full-rate 24-bit multiply (a 32-bit
     // multiply issues at a quarter of the rate, and the slots form these offsets at every use)
     auto rowb = [&](int row) { return static_cast<int>(__umul24(static_cast<unsigned>(row), static_cast<unsigned>(RB))); };
-    auto yat = [&](int row, int col) -> C& { return *reinterpret_cast<C*>(smem + kRows + rowb(row) + col * kCB); };
+    auto yat = [&](int row, int col) -> C& { return *reinterpret_cast<C*>(WIDE_LDS + kRows + rowb(row) + col * kCB); };
 
     // ---- clear everything (h and X padding stay zero), then the tables ----
     {
         const int n16 = (kRows + (L + 1) * RB) / 16;
-        for (int e = tid; e < n16; e += kNT) reinterpret_cast<float4*>(smem)[e] = float4{0, 0, 0, 0};
+        for (int e = tid; e < n16; e += kNT) reinterpret_cast<float4*>(WIDE_LDS)[e] = float4{0, 0, 0, 0};
     }
     __syncthreads();
     for (int i = tid; i < L; i += kNT) {
-        *reinterpret_cast<float*>(smem + kSite + i * kRB) = s.tdiag[i];
-        *reinterpret_cast<float*>(smem + kSite + (kLP + i) * kRB) = s.epot[i];
-        *reinterpret_cast<float*>(smem + kSite + (2 * kLP + i) * kRB) = s.coulomb_type == 1 ? s.u_diag[i] : 0.f;
-        *reinterpret_cast<float*>(smem + kSite + (3 * kLP + i) * kRB) = s.pdiag[i];
+        *reinterpret_cast<float*>(WIDE_LDS + kSite + i * kRB) = s.tdiag[i];
+        *reinterpret_cast<float*>(WIDE_LDS + kSite + (kLP + i) * kRB) = s.epot[i];
+        *reinterpret_cast<float*>(WIDE_LDS + kSite + (2 * kLP + i) * kRB) = s.coulomb_type == 1 ? s.u_diag[i] : 0.f;
+        *reinterpret_cast<float*>(WIDE_LDS + kSite + (3 * kLP + i) * kRB) = s.pdiag[i];
     }
     // neighbour tables for row i (rows past L: row 0 with t = 0): the row byte offset of k_n(i), and k_n(i)
     // packed four to a word
@@ -116,8 +124,8 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         const int n = e / kLP, i = e - n * kLP;
         const bool on = i < L;
         const int k = on ? s.nbr_ks[n * L + i] : 0;
-        *reinterpret_cast<float*>(smem + kNbrT + e * kRB) = on ? s.nbr_ts[n * L + i] : 0.f;
-        *reinterpret_cast<int*>(smem + kNbrK + e * 4) = kRows + k * RB;
+        *reinterpret_cast<float*>(WIDE_LDS + kNbrT + e * kRB) = on ? s.nbr_ts[n * L + i] : 0.f;
+        *reinterpret_cast<int*>(WIDE_LDS + kNbrK + e * 4) = kRows + k * RB;
     }
     // 16-bit offsets of Y(k_n(i), 0) from the rows (bytes for L <= 64, else complex elements: at most
     // 127 x 129) and of Y(0, k_n(i)) within a row (bytes), four neighbours to an entry
@@ -130,8 +138,8 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
                 rw[m / 2] |= static_cast<uint32_t>(kRowBytes ? k * RB : k * (RB / kCB)) << (16 * (m % 2));
                 cw[m / 2] |= static_cast<uint32_t>(k * kCB) << (16 * (m % 2));
             }
-        *reinterpret_cast<uint2*>(smem + kNbrR + e * kCB) = uint2{rw[0], rw[1]};
-        *reinterpret_cast<uint2*>(smem + kNbrC + e * kCB) = uint2{cw[0], cw[1]};
+        *reinterpret_cast<uint2*>(WIDE_LDS + kNbrR + e * kCB) = uint2{rw[0], rw[1]};
+        *reinterpret_cast<uint2*>(WIDE_LDS + kNbrC + e * kCB) = uint2{cw[0], cw[1]};
     }
 
     // ---- element slots (class order): ah = 8 i | 8 j << 16 of the slot's element; padding lanes hold
@@ -218,15 +226,15 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
     auto nbr = [&](int n, int i, int& kb, float& tv) {
         int o = n * kLP + i;
         asm volatile("" : "+v"(o));
-        kb = *reinterpret_cast<const int*>(smem + kNbrK + o * 4);
-        tv = *reinterpret_cast<const float*>(smem + kNbrT + o * kRB);
+        kb = *reinterpret_cast<const int*>(WIDE_LDS + kNbrK + o * 4);
+        tv = *reinterpret_cast<const float*>(WIDE_LDS + kNbrT + o * kRB);
     };
     const float t_uni = s.hop_value;
     auto hop = [&](int n, int x8) -> float {  // t_{x,n} of site x = x8 / 8: the lattice's one value, or the table's
         if (UNI) return t_uni;
         int o = static_cast<int>(static_cast<unsigned>(x8) >> 1);
         asm volatile("" : "+v"(o));
-        return ldl<float>(smem, kNbrT + n * kLP * kRB + o);
+        return ldl<float>(WIDE_LDS, kNbrT + n * kLP * kRB + o);
     };
 
     // ---- M: the spin's stage input into the rows; in the spin-up pass, h of both spins ----
@@ -244,22 +252,23 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
             const float yy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
             int i8, j8;
             slot(q, i8, j8);
-            *reinterpret_cast<C*>(smem + kRows + rowb8(i8) + j8) = C{yx, yy};
-            *reinterpret_cast<C*>(smem + kRows + rowb8(j8) + i8) = C{yx, -yy};
+            *reinterpret_cast<C*>(WIDE_LDS + kRows + rowb8(i8) + j8) = C{yx, yy};
+            *reinterpret_cast<C*>(WIDE_LDS + kRows + rowb8(j8) + i8) = C{yx, -yy};
         }
         if (dslot) {
             const float yx = SG ? dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6)
                                 : dev::stage_input<ST>(dy0, dA0, dp0, h1, h3, h6);
-            *reinterpret_cast<C*>(smem + da) = C{yx, 0.f};
+            *reinterpret_cast<C*>(WIDE_LDS + da) = C{yx, 0.f};
             if constexpr (SG == 0) {
                 // h_up,i = pdiag_i + U_ii n_dn,i and h_dn,i = pdiag_i + U_ii n_up,i, n = Re y + 1/2
                 const float y1x = dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6);
                 int o = tid * kRB;
                 asm volatile("" : "+v"(o));
-                const float u = ldl<float>(smem, kSite + 2 * kLP * kRB + o), pd = ldl<float>(smem, kSite + 3 * kLP * kRB + o);
-                *reinterpret_cast<float2*>(smem + HP + 2 * o) = float2{u * (y1x + half) + pd, u * (yx + half) + pd};
-                *reinterpret_cast<float*>(smem + kDg + o) = yx;
-                *reinterpret_cast<float*>(smem + kDg + kHSpin + o) = y1x;
+                const float u = ldl<float>(WIDE_LDS, kSite + 2 * kLP * kRB + o);
+                const float pd = ldl<float>(WIDE_LDS, kSite + 3 * kLP * kRB + o);
+                *reinterpret_cast<float2*>(WIDE_LDS + HP + 2 * o) = float2{u * (y1x + half) + pd, u * (yx + half) + pd};
+                *reinterpret_cast<float*>(WIDE_LDS + kDg + o) = yx;
+                *reinterpret_cast<float*>(WIDE_LDS + kDg + kHSpin + o) = y1x;
             }
         }
     };
@@ -274,12 +283,21 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
             if (q * kW + w >= n_off) continue;
             int i8, j8;
             slot(q, i8, j8);
-            const int ri = kRows + rowb8(i8), rj = kRows + j8;
+            int ri = kRows + rowb8(i8);
+            if constexpr (S <= 4) {
+                // L <= 64: one multiply per slot, each neighbour's address one add from it (otherwise the
+                // compiler forms every address as its own multiply-add)
+                ri = rowb8(i8);
+                asm volatile("" : "+v"(ri));
+                ri += kRows;
+            }
+            const int rj = kRows + j8;
             float ax = 0, ay = 0, bx = 0, by = 0;  // conj(X_ji), X_ij
             uint32_t pr[Z / 2], pc[Z / 2];  // row offsets of i's neighbours, column offsets of j's
 #pragma unroll
             for (int e = 0; e < Z / 4; ++e) {
-                const uint2 a = ldl<uint2>(smem, kNbrR + e * kLP * kCB + i8), c = ldl<uint2>(smem, kNbrC + e * kLP * kCB + j8);
+                const uint2 a = ldl<uint2>(WIDE_LDS, kNbrR + e * kLP * kCB + i8);
+                const uint2 c = ldl<uint2>(WIDE_LDS, kNbrC + e * kLP * kCB + j8);
                 pr[2 * e] = a.x;
                 pr[2 * e + 1] = a.y;
                 pc[2 * e] = c.x;
@@ -289,8 +307,8 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
             for (int n = 0; n < Z; ++n) {
                 const uint32_t ro = (pr[n / 2] >> (16 * (n % 2))) & 0xffff, co = (pc[n / 2] >> (16 * (n % 2))) & 0xffff;
                 const float tj = hop(n, j8), ti = hop(n, i8);
-                const C u = ldl<C>(smem, ri + co);                          // Y_{i, k_n(j)} = conj(Y_{k_n(j), i})
-                const C v = ldl<C>(smem, rj + (kRowBytes ? ro : ro * kCB));  // Y_{k_n(i), j}
+                const C u = ldl<C>(WIDE_LDS, ri + co);                          // Y_{i, k_n(j)} = conj(Y_{k_n(j), i})
+                const C v = ldl<C>(WIDE_LDS, rj + (kRowBytes ? ro : ro * kCB));  // Y_{k_n(i), j}
                 ax += tj * u.x;
                 ay += tj * u.y;
                 bx += ti * v.x;
@@ -299,7 +317,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         C& yq = SG ? y1[q] : y0[q];
             C& Aq = SG ? A1[q] : A0[q];
             C& pq = SG ? p1[q] : p0[q];
-            const float dh = ldl<float>(smem, HS + j8) - ldl<float>(smem, HS + i8);  // h entries are 8 bytes
+            const float dh = ldl<float>(WIDE_LDS, HS + j8) - ldl<float>(WIDE_LDS, HS + i8);  // h entries are 8 bytes
             const float ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
             const float ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
             // c = conj(X_ji) - X_ij + (h_j - h_i) Y_ij;  k = -i c
@@ -315,7 +333,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
                 int kb;
                 float ti;
                 nbr(n, tid, kb, ti);
-                xi += ti * ldl<C>(smem, kb + tid * kCB).y;
+                xi += ti * ldl<C>(WIDE_LDS, kb + tid * kCB).y;
             }
             const float k = -2.f * xi;
             if constexpr (SG == 0) dev::rk_update<ST>(dy0, dA0, dp0, k, h8);
@@ -342,16 +360,16 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
                 int kb;
                 float tv;
                 nbr(n, tt, kb, tv);
-                eth += tv * ldl<C>(smem, kb + tt * kCB).x;
+                eth += tv * ldl<C>(WIDE_LDS, kb + tt * kCB).x;
             }
         }
         const int st = tt - 2 * 64;  // on-site terms on waves 2 and 3
         if (SG == 0 && st >= 0 && st < L) {
-            const float nu = ldl<float>(smem, kDg + st * kRB) + half;
-            const float nd = ldl<float>(smem, kDg + kHSpin + st * kRB) + half;
-            eth += ldl<float>(smem, kSite + st * kRB) * (nu + nd);
-            epe += ldl<float>(smem, kSite + (kLP + st) * kRB) * (nu + nd);
-            eu += ldl<float>(smem, kSite + (2 * kLP + st) * kRB) * nu * nd;
+            const float nu = ldl<float>(WIDE_LDS, kDg + st * kRB) + half;
+            const float nd = ldl<float>(WIDE_LDS, kDg + kHSpin + st * kRB) + half;
+            eth += ldl<float>(WIDE_LDS, kSite + st * kRB) * (nu + nd);
+            epe += ldl<float>(WIDE_LDS, kSite + (kLP + st) * kRB) * (nu + nd);
+            eu += ldl<float>(WIDE_LDS, kSite + (2 * kLP + st) * kRB) * nu * nd;
         }
         eth = wave_sum(eth);
         eu = wave_sum(eu);
@@ -425,6 +443,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
         go[LL + tid * L + tid] = C{dy1, 0.f};
     }
 }
+#undef WIDE_LDS
 
 size_t wide_lds_bytes(int L) { return kRows + static_cast<size_t>(L + 1) * wide_row_len(L) * kCB; }
 
@@ -433,14 +452,16 @@ hipError_t run_wide(const DevSystem<float>& s, const RunShape& r, int t0, int t1
                     hipStream_t st)
 {
     auto kern = herm_wide_kernel<S, Z, UNI>;
-    static bool attr_set = false;
+    static bool attr_set = S <= 4;  // the static array of the L <= 64 instances needs no opt-in
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(kNT), wide_lds_bytes(s.L), st, s, r, t0, t1,
+    if (S <= 4 && wide_lds_bytes(s.L) > static_cast<size_t>(kStaticLds)) return hipErrorInvalidValue;
+    const size_t dyn = S <= 4 ? 0 : wide_lds_bytes(s.L);  // the L <= 64 instances hold a static array
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(kNT), dyn, st, s, r, t0, t1,
                        state, rec);
     return hipGetLastError();
 }
