@@ -89,6 +89,9 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
 {
 #define WIDE_LDS (S <= 4 ? wide_static_lds : wide_dyn_lds)  // this kernel's LDS (a global array, not captured)
     constexpr bool kRowBytes = S <= 4;  // L <= 64: neighbour row offsets in bytes fit 16 bits
+    // one hopping value at L <= 64: neighbour terms summed first, then multiplied (2 fewer VALU
+    // instructions per slot; at L = 100 the same form measured 2.5 % slower)
+    constexpr bool kSumFirst = UNI && S <= 4;
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
@@ -309,10 +312,17 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
                 const float tj = hop(n, j8), ti = hop(n, i8);
                 const C u = ldl<C>(WIDE_LDS, ri + co);                          // Y_{i, k_n(j)} = conj(Y_{k_n(j), i})
                 const C v = ldl<C>(WIDE_LDS, rj + (kRowBytes ? ro : ro * kCB));  // Y_{k_n(i), j}
-                ax += tj * u.x;
-                ay += tj * u.y;
-                bx += ti * v.x;
-                by += ti * v.y;
+                if constexpr (kSumFirst) {  // the one hopping multiplies the sums below
+                    ax = n == 0 ? u.x : ax + u.x;
+                    ay = n == 0 ? u.y : ay + u.y;
+                    bx = n == 0 ? v.x : bx + v.x;
+                    by = n == 0 ? v.y : by + v.y;
+                } else {
+                    ax += tj * u.x;
+                    ay += tj * u.y;
+                    bx += ti * v.x;
+                    by += ti * v.y;
+                }
             }
         C& yq = SG ? y1[q] : y0[q];
             C& Aq = SG ? A1[q] : A0[q];
@@ -321,8 +331,8 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
             const float ycx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
             const float ycy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
             // c = conj(X_ji) - X_ij + (h_j - h_i) Y_ij;  k = -i c
-            const float cx = ax - bx + dh * ycx;
-            const float cy = ay - by + dh * ycy;
+            const float cx = kSumFirst ? t_uni * (ax - bx) + dh * ycx : ax - bx + dh * ycx;
+            const float cy = kSumFirst ? t_uni * (ay - by) + dh * ycy : ay - by + dh * ycy;
             dev::rk_update<ST>(yq.x, Aq.x, pq.x, cy, h8);
             dev::rk_update<ST>(yq.y, Aq.y, pq.y, -cx, h8);
         }
