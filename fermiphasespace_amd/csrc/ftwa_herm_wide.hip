@@ -1,5 +1,6 @@
-// ftwa_herm_wide.hip -- batched FP32 fTWA integrator on the upper triangle for 64 < L <= 107: one spin's
-// rows in LDS at a time, the commutator of each element formed directly from them.
+// ftwa_herm_wide.hip -- batched FP32 fTWA integrator on the upper triangle for 64 < L <= 107, and for the
+// balanced L = 64 lattices with one hopping value (the 8x8 periodic square): one spin's rows in LDS at a
+// time, the commutator of each element formed directly from them.
 //
 // Same physics, records and RK4 "3/8" register form (y, A, p) as the spin-sequential kernel of
 // ftwa_herm_seq.hip (src/FHfTWASolver.cpp:366-426 records, :568-600 derivative, :629-661 RK4 3/8).  Past
@@ -12,9 +13,11 @@
 //        k_ii = -2 Im X_ii, and the RK registers
 // with a barrier after M and one after C (before the next M rewrites the rows).  Per element that is the
 // same 2z complex x real multiply-adds as forming X, with the neighbour tables (row offsets, column
-// offsets, hoppings) read from LDS per slot.  LDS: rows of Y, 1 mod 32 complex long.  One 8-wave workgroup
-// per CU (up to 256 VGPRs, S <= 12 slots per thread).  Element slots in class order (dev::class_slot).
-// Null or on-site Coulomb interactions, no pulse, no density records.
+// offsets, hoppings; entry n of every site at the n-th most common shift k - i mod 32) read from LDS per
+// slot.  LDS: rows of Y, 1 mod 32 complex long.  Past L = 64 one 8-wave workgroup per CU (up to 256 VGPRs,
+// S <= 12 slots per thread) and a dynamic LDS array; at L = 64 two per CU (4 slots, 128 VGPRs) and a
+// static one.  Element slots in class order (dev::class_slot).  Null or on-site Coulomb interactions, no
+// pulse, no density records.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -498,7 +501,8 @@ hipError_t run_wide_s(int S, const DevSystem<float>& s, const RunShape& r, int t
 
 // L <= 64 where it measured faster than the spin-sequential kernel of ftwa_herm_seq.hip: four whole slot
 // rows per wave (L = 64), four neighbours, one hopping value (the 8x8 periodic square: 3.25e7 against
-// 3.08e7 trajectory-steps/s; with unequal slot rows per wave, kagome 48 and bcc 54, it lost)
+// 3.08e7 trajectory-steps/s when first measured, 3.56e7 now; with unequal slot rows per wave, kagome 48
+// and bcc 54, it lost)
 bool herm_wide_direct_preferred(const DevSystem<float>& s)
 {
     return s.L <= 64 && dev::class_rows(s.L) == 4 * kW && s.z == 4 && s.hop_uniform && !s.amp &&
