@@ -63,6 +63,14 @@ struct Lay {
 };
 constexpr size_t kLdsLimit = 160 * 1024;
 
+// LDS of the FP32 instances, a static array whose offsets the compiler folds into the LDS instructions:
+// the tables, then P L + 1 <= 65 rows (two workgroups per CU still fit).  FP64 (one workgroup per CU,
+// 134 KB at L = 64) keeps the dynamic array the launch sizes.  The kernel names either array directly:
+// a pointer variable captured by its lambdas costs VALU instructions.
+constexpr int kSeqStaticLds = Lay<float>::kRows + 65 * Lay<float>::kRowB;
+__shared__ __align__(16) unsigned char seq_static_lds[kSeqStaticLds];
+extern __shared__ __align__(16) unsigned char seq_dyn_lds[];
+
 template <int N>
 using Ic = std::integral_constant<int, N>;
 
@@ -87,7 +95,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
     const Real half = Real(0.5);
     constexpr int kNT = 64 * W;
     constexpr int NCG = 64 / W;  // columns per lane in phase A
-    extern __shared__ __align__(16) unsigned char smem[];
+#define SEQ_LDS (sizeof(Real) == 4 ? seq_static_lds : seq_dyn_lds)  // this kernel's LDS (a global array)
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
@@ -100,24 +108,24 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
     const int64_t b0 = static_cast<int64_t>(blockIdx.x) * P;  // first trajectory of the workgroup
     const int spare = kRows + PL * kRowB;
     auto live = [&](int t) { return P == 1 || b0 + t < r.B; };  // the last workgroup may hold fewer
-    Real* ew = reinterpret_cast<Real*>(smem + kEW);
+    Real* ew = reinterpret_cast<Real*>(SEQ_LDS + kEW);
 
     auto yat = [&](int row, int col) -> C& {
-        return *reinterpret_cast<C*>(smem + kRows + row * kRowB + col * kCB);
+        return *reinterpret_cast<C*>(SEQ_LDS + kRows + row * kRowB + col * kCB);
     };
 
     // ---- clear rows and tables (columns past L stay zero) ----
     {
         const int n16 = (kRows + (PL + 1) * kRowB) / 16;
-        for (int e = tid; e < n16; e += kNT) reinterpret_cast<float4*>(smem)[e] = float4{0, 0, 0, 0};
+        for (int e = tid; e < n16; e += kNT) reinterpret_cast<float4*>(SEQ_LDS)[e] = float4{0, 0, 0, 0};
     }
     __syncthreads();
     for (int rw = tid; rw < PL; rw += kNT) {  // site tables per row
         const int i = P == 1 ? rw : rw % L;
-        *reinterpret_cast<Real*>(smem + kSite + rw * kRB) = s.tdiag[i];
-        *reinterpret_cast<Real*>(smem + kSite + (64 + rw) * kRB) = s.epot[i];
-        *reinterpret_cast<Real*>(smem + kSite + (128 + rw) * kRB) = s.coulomb_type == 1 ? s.u_diag[i] : Real(0);
-        *reinterpret_cast<Real*>(smem + kSite + (192 + rw) * kRB) = s.pdiag[i];
+        *reinterpret_cast<Real*>(SEQ_LDS + kSite + rw * kRB) = s.tdiag[i];
+        *reinterpret_cast<Real*>(SEQ_LDS + kSite + (64 + rw) * kRB) = s.epot[i];
+        *reinterpret_cast<Real*>(SEQ_LDS + kSite + (128 + rw) * kRB) = s.coulomb_type == 1 ? s.u_diag[i] : Real(0);
+        *reinterpret_cast<Real*>(SEQ_LDS + kSite + (192 + rw) * kRB) = s.pdiag[i];
     }
     // neighbour tables for lane = row t L + i: the neighbour row t L + k and the start of the row's column
     // block t L (rows past P L: row 0 with t = 0)
@@ -125,8 +133,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         const int n = e >> 6, rw = e & 63;
         const bool on = rw < PL;
         const int tb = P == 1 || !on ? 0 : rw / L, i = rw - tb * L;
-        *reinterpret_cast<Real*>(smem + kNbrT + e * kRB) = on ? s.nbr_t[n * L + i] : Real(0);
-        *reinterpret_cast<int*>(smem + kNbrK + e * 4) =
+        *reinterpret_cast<Real*>(SEQ_LDS + kNbrT + e * kRB) = on ? s.nbr_t[n * L + i] : Real(0);
+        *reinterpret_cast<int*>(SEQ_LDS + kNbrK + e * 4) =
             kRows + (on ? tb * L + s.nbr_k[n * L + i] : 0) * kRowB + tb * L * kCB;
     }
 
@@ -156,7 +164,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
                 double wr, wi, re, im;
                 dev::initial_noise(r.seed, r.b0 + b0 + tb, sg, p, wr, wi);
                 dev::sampled_offdiag(occ[sg * L + i], occ[sg * L + c], wr, wi, re, im);
-                *reinterpret_cast<C*>(smem + kRows + (tb * L + i) * kRowB + (tb * L + c) * kCB + sg * kXo) =
+                *reinterpret_cast<C*>(SEQ_LDS + kRows + (tb * L + i) * kRowB + (tb * L + c) * kCB + sg * kXo) =
                     C{static_cast<Real>(re), static_cast<Real>(im)};
             }
         }
@@ -185,8 +193,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
             aji[q] = kRows + rj * kRowB + ri * kCB;
             ah[q] = ri * kRB | (rj * kRB) << 16;
             if (sampled) {
-                y0[q] = *reinterpret_cast<const C*>(smem + aij[q]);
-                y1[q] = *reinterpret_cast<const C*>(smem + aij[q] + kXo);
+                y0[q] = *reinterpret_cast<const C*>(SEQ_LDS + aij[q]);
+                y1[q] = *reinterpret_cast<const C*>(SEQ_LDS + aij[q] + kXo);
             } else if (live(tb)) {
                 y0[q] = gs[tb * 2 * LL + i * L + jj];
                 y1[q] = gs[tb * 2 * LL + LL + i * L + jj];
@@ -215,8 +223,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
     auto nbr_row = [&](int n, int rw, int& kb, Real& tv) {
         int o = n * 64 + rw;
         asm volatile("" : "+v"(o));  // opaque: loaded here, not hoisted out of the time loop
-        kb = *reinterpret_cast<const int*>(smem + kNbrK + o * 4);
-        tv = *reinterpret_cast<const Real*>(smem + kNbrT + o * kRB);
+        kb = *reinterpret_cast<const int*>(SEQ_LDS + kNbrK + o * 4);
+        tv = *reinterpret_cast<const Real*>(SEQ_LDS + kNbrT + o * kRB);
     };
     auto nbr = [&](int n, int& kb, Real& tv) {
         nbr_row(n, j, kb, tv);
@@ -239,23 +247,23 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
             const C& pq = SG ? p1[q] : p0[q];
             const Real yx = dev::stage_input<ST>(yq.x, Aq.x, pq.x, h1, h3, h6);
             const Real yy = dev::stage_input<ST>(yq.y, Aq.y, pq.y, h1, h3, h6);
-            *reinterpret_cast<C*>(smem + aij[q]) = C{yx, yy};
-            *reinterpret_cast<C*>(smem + aji[q]) = C{yx, -yy};
+            *reinterpret_cast<C*>(SEQ_LDS + aij[q]) = C{yx, yy};
+            *reinterpret_cast<C*>(SEQ_LDS + aji[q]) = C{yx, -yy};
         }
         if (dslot) {
             const Real yx = SG ? dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6)
                                : dev::stage_input<ST>(dy0, dA0, dp0, h1, h3, h6);
-            *reinterpret_cast<C*>(smem + da) = C{yx, Real(0)};
+            *reinterpret_cast<C*>(SEQ_LDS + da) = C{yx, Real(0)};
             if constexpr (SG == 0) {
                 // h_up,i = pdiag_i + U_ii n_dn,i and h_dn,i = pdiag_i + U_ii n_up,i, n = Re y + 1/2
                 const Real y1x = dev::stage_input<ST>(dy1, dA1, dp1, h1, h3, h6);
                 int o = tid * kRB;
                 asm volatile("" : "+v"(o));
-                const Real u = ldl<Real>(smem, kSite + 128 * kRB + o), pd = ldl<Real>(smem, kSite + 192 * kRB + o);
-                *reinterpret_cast<Real*>(smem + HP + o) = u * (y1x + half) + pd;
-                *reinterpret_cast<Real*>(smem + HP + kHSpin + o) = u * (yx + half) + pd;
-                *reinterpret_cast<Real*>(smem + kDg + o) = yx;
-                *reinterpret_cast<Real*>(smem + kDg + kHSpin + o) = y1x;
+                const Real u = ldl<Real>(SEQ_LDS, kSite + 128 * kRB + o), pd = ldl<Real>(SEQ_LDS, kSite + 192 * kRB + o);
+                *reinterpret_cast<Real*>(SEQ_LDS + HP + o) = u * (y1x + half) + pd;
+                *reinterpret_cast<Real*>(SEQ_LDS + HP + kHSpin + o) = u * (yx + half) + pd;
+                *reinterpret_cast<Real*>(SEQ_LDS + kDg + o) = yx;
+                *reinterpret_cast<Real*>(SEQ_LDS + kDg + kHSpin + o) = y1x;
             }
         }
     };
@@ -274,7 +282,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         C v[Z];
         if constexpr (kPipe) {
 #pragma unroll
-            for (int n = 0; n < Z; ++n) v[n] = ldl<C>(smem, kb[n]);
+            for (int n = 0; n < Z; ++n) v[n] = ldl<C>(SEQ_LDS, kb[n]);
         }
 #pragma unroll
         for (int m = 0; m < NCG; ++m) {
@@ -283,15 +291,15 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
             Real ax = 0, ay = 0;
 #pragma unroll
             for (int n = 0; n < Z; ++n) {
-                const C vn = kPipe ? v[n] : ldl<C>(smem, kb[n] + co);
+                const C vn = kPipe ? v[n] : ldl<C>(SEQ_LDS, kb[n] + co);
                 ax += tv[n] * vn.x;
                 ay += tv[n] * vn.y;
             }
             if (kPipe && m + 1 < NCG && (m + 1) * W + w < L) {
 #pragma unroll
-                for (int n = 0; n < Z; ++n) v[n] = ldl<C>(smem, kb[n] + co + W * kCB);
+                for (int n = 0; n < Z; ++n) v[n] = ldl<C>(SEQ_LDS, kb[n] + co + W * kCB);
             }
-            *reinterpret_cast<C*>(smem + xb + co) = C{ax, ay};
+            *reinterpret_cast<C*>(SEQ_LDS + xb + co) = C{ax, ay};
         }
     };
 
@@ -308,12 +316,12 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
             C nij{0, 0}, nji{0, 0};
             Real nhj = 0, nhi = 0;
             auto fetch = [&](int q) {
-                nij = ldl<C>(smem, aij[q] + kXo);
-                nji = ldl<C>(smem, aji[q] + kXo);
+                nij = ldl<C>(SEQ_LDS, aij[q] + kXo);
+                nji = ldl<C>(SEQ_LDS, aji[q] + kXo);
                 int ahq = ah[q];  // opaque: decoded here, not kept decoded across the time loop
                 asm volatile("" : "+v"(ahq));
-                nhj = ldl<Real>(smem, HS + (ahq >> 16));
-                nhi = ldl<Real>(smem, HS + (ahq & 0xffff));
+                nhj = ldl<Real>(SEQ_LDS, HS + (ahq >> 16));
+                nhi = ldl<Real>(SEQ_LDS, HS + (ahq & 0xffff));
             };
             if (w < n_el) fetch(0);
 #pragma unroll
@@ -338,11 +346,11 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
             for (int q = 0; q < S; ++q) {
                 if (q * W + w >= n_el) continue;
                 asm volatile("" ::: "memory");
-                const C xij = ldl<C>(smem, aij[q] + kXo);
-                const C xji = ldl<C>(smem, aji[q] + kXo);
+                const C xij = ldl<C>(SEQ_LDS, aij[q] + kXo);
+                const C xji = ldl<C>(SEQ_LDS, aji[q] + kXo);
                 int ahq = ah[q];  // opaque: decoded here, not kept decoded across the time loop
                 asm volatile("" : "+v"(ahq));
-                const Real dh = ldl<Real>(smem, HS + (ahq >> 16)) - ldl<Real>(smem, HS + (ahq & 0xffff));
+                const Real dh = ldl<Real>(SEQ_LDS, HS + (ahq >> 16)) - ldl<Real>(SEQ_LDS, HS + (ahq & 0xffff));
                 C& yq = SG ? y1[q] : y0[q];
                 C& Aq = SG ? A1[q] : A0[q];
                 C& pq = SG ? p1[q] : p0[q];
@@ -356,7 +364,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
             }
         }
         if (dslot) {  // k_ii = -2 Im X_ii
-            const Real k = Real(-2) * ldl<C>(smem, da + kXo).y;
+            const Real k = Real(-2) * ldl<C>(SEQ_LDS, da + kXo).y;
             if constexpr (SG == 0) dev::rk_update<ST>(dy0, dA0, dp0, k, h8);
             else dev::rk_update<ST>(dy1, dA1, dp1, k, h8);
         }
@@ -388,15 +396,15 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
                 int kb;
                 Real tv;
                 nbr_row(n, rw, kb, tv);
-                eth += tv * ldl<C>(smem, kb + jt * kCB).x;
+                eth += tv * ldl<C>(SEQ_LDS, kb + jt * kCB).x;
             }
         }
         if (SG == 0 && tw < P && (w & 1) == 1 && jt < L) {  // on-site terms from the stage-input diagonals
-            const Real nu = ldl<Real>(smem, kDg + rw * kRB) + half;
-            const Real nd = ldl<Real>(smem, kDg + kHSpin + rw * kRB) + half;
-            eth += ldl<Real>(smem, kSite + rw * kRB) * (nu + nd);
-            epe += ldl<Real>(smem, kSite + (64 + rw) * kRB) * (nu + nd);
-            eu += ldl<Real>(smem, kSite + (128 + rw) * kRB) * nu * nd;
+            const Real nu = ldl<Real>(SEQ_LDS, kDg + rw * kRB) + half;
+            const Real nd = ldl<Real>(SEQ_LDS, kDg + kHSpin + rw * kRB) + half;
+            eth += ldl<Real>(SEQ_LDS, kSite + rw * kRB) * (nu + nd);
+            epe += ldl<Real>(SEQ_LDS, kSite + (64 + rw) * kRB) * (nu + nd);
+            eu += ldl<Real>(SEQ_LDS, kSite + (128 + rw) * kRB) * nu * nd;
         }
         if (w < 2 * P) {  // the other waves hold no energy terms
             eth = wave_sum(eth);
@@ -481,6 +489,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
         g[LL + di * L + di] = C{dy1, Real(0)};
     }
 }
+#undef SEQ_LDS
 
 template <typename Real>
 size_t seq_lds_bytes(int rows)
@@ -494,15 +503,17 @@ hipError_t run_seq(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, 
 {
     constexpr int MINB = sizeof(Real) == 4 ? 2 : 1;
     auto kern = herm_seq_kernel<Real, W, S, Z, MINB, P>;
-    static bool attr_set = false;
+    constexpr bool kStatic = sizeof(Real) == 4;  // the FP32 instances hold a static array
+    static bool attr_set = kStatic;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
+    if (kStatic && seq_lds_bytes<Real>(P * s.L) > static_cast<size_t>(kSeqStaticLds)) return hipErrorInvalidValue;
     const int64_t grid = (r.B + P - 1) / P;
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(64 * W), seq_lds_bytes<Real>(P * s.L), st, s, r,
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(64 * W), kStatic ? 0 : seq_lds_bytes<Real>(P * s.L), st, s, r,
                        t0, t1, state, rec);
     return hipGetLastError();
 }
