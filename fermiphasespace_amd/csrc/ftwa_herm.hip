@@ -51,6 +51,11 @@ constexpr int kEW = 1280;           // energy partials
 constexpr int kSite = 1536;         // tM_ii [64], potential [64], U_ii [64]
 constexpr int kRows = 2304;         // first row
 constexpr size_t kLdsLimit = 160 * 1024;
+// LDS of the spin-pipelined kernel (no pulse table), a static array of the largest size whose offsets the
+// compiler folds into the LDS instructions (one workgroup per CU at any L <= 64).  The kernel names the
+// array directly: a pointer variable captured by its lambdas costs VALU instructions.
+constexpr int kPipeStaticLds = kRows + 129 * kRowBytes;
+__shared__ __align__(16) unsigned char pipe_static_lds[kPipeStaticLds];
 
 template <int N>
 using Ic = std::integral_constant<int, N>;
@@ -470,7 +475,7 @@ __global__ void __launch_bounds__(1024)
 {
     constexpr int kNT = 1024;
     constexpr int kNCG = 8;  // column groups per wave in phase A
-    extern __shared__ __align__(16) unsigned char smem[];
+#define smem pipe_static_lds
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
@@ -793,6 +798,7 @@ __global__ void __launch_bounds__(1024)
         }
     }
 }
+#undef smem
 
 size_t herm_lds_bytes(int L, int z, bool pulse)
 {
@@ -875,15 +881,8 @@ hipError_t run_pipe(const DevSystem<float>& s, const RunShape& r, int t0, int t1
                     hipStream_t st)
 {
     auto kern = herm_pipe_kernel<S, Z>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(1024), herm_lds_bytes(s.L, s.z, false), st, s, r,
-                       t0, t1, state, rec);
+    if (herm_lds_bytes(s.L, s.z, false) > static_cast<size_t>(kPipeStaticLds)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(1024), 0, st, s, r, t0, t1, state, rec);
     return hipGetLastError();
 }
 
