@@ -63,13 +63,14 @@ struct Lay {
 };
 constexpr size_t kLdsLimit = 160 * 1024;
 
-// LDS of the FP32 instances, a static array whose offsets the compiler folds into the LDS instructions:
-// the tables, then P L + 1 <= 65 rows (two workgroups per CU still fit).  FP64 (one workgroup per CU,
-// 134 KB at L = 64) keeps the dynamic array the launch sizes.  The kernel names either array directly:
-// a pointer variable captured by its lambdas costs VALU instructions.
+// LDS as a static array of the largest size, one per precision, whose offsets the compiler folds into
+// the LDS instructions: the tables, then P L + 1 <= 65 rows (FP32: two workgroups per CU still fit; FP64:
+// one).  The kernel names the array directly: a pointer variable captured by its lambdas costs VALU
+// instructions.
 constexpr int kSeqStaticLds = Lay<float>::kRows + 65 * Lay<float>::kRowB;
+constexpr int kSeqStaticLds64 = Lay<double>::kRows + 65 * Lay<double>::kRowB;
 __shared__ __align__(16) unsigned char seq_static_lds[kSeqStaticLds];
-extern __shared__ __align__(16) unsigned char seq_dyn_lds[];
+__shared__ __align__(16) unsigned char seq_static_lds64[kSeqStaticLds64];
 
 template <int N>
 using Ic = std::integral_constant<int, N>;
@@ -95,7 +96,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MIN
     const Real half = Real(0.5);
     constexpr int kNT = 64 * W;
     constexpr int NCG = 64 / W;  // columns per lane in phase A
-#define SEQ_LDS (sizeof(Real) == 4 ? seq_static_lds : seq_dyn_lds)  // this kernel's LDS (a global array)
+#define SEQ_LDS (sizeof(Real) == 4 ? seq_static_lds : seq_static_lds64)  // this kernel's LDS (a global array)
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
@@ -503,17 +504,10 @@ hipError_t run_seq(const DevSystem<Real>& s, const RunShape& r, int t0, int t1, 
 {
     constexpr int MINB = sizeof(Real) == 4 ? 2 : 1;
     auto kern = herm_seq_kernel<Real, W, S, Z, MINB, P>;
-    constexpr bool kStatic = sizeof(Real) == 4;  // the FP32 instances hold a static array
-    static bool attr_set = kStatic;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    if (kStatic && seq_lds_bytes<Real>(P * s.L) > static_cast<size_t>(kSeqStaticLds)) return hipErrorInvalidValue;
+    constexpr int kStatic = sizeof(Real) == 4 ? kSeqStaticLds : kSeqStaticLds64;  // the kernel's static array
+    if (seq_lds_bytes<Real>(P * s.L) > static_cast<size_t>(kStatic)) return hipErrorInvalidValue;
     const int64_t grid = (r.B + P - 1) / P;
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(64 * W), kStatic ? 0 : seq_lds_bytes<Real>(P * s.L), st, s, r,
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(64 * W), 0, st, s, r,
                        t0, t1, state, rec);
     return hipGetLastError();
 }
