@@ -51,11 +51,12 @@ constexpr int kEW = 1280;           // energy partials
 constexpr int kSite = 1536;         // tM_ii [64], potential [64], U_ii [64]
 constexpr int kRows = 2304;         // first row
 constexpr size_t kLdsLimit = 160 * 1024;
-// LDS of the spin-pipelined kernel (no pulse table), a static array of the largest size whose offsets the
-// compiler folds into the LDS instructions (one workgroup per CU at any L <= 64).  The kernel names the
-// array directly: a pointer variable captured by its lambdas costs VALU instructions.
-constexpr int kPipeStaticLds = kRows + 129 * kRowBytes;
-__shared__ __align__(16) unsigned char pipe_static_lds[kPipeStaticLds];
+// LDS of both kernels, a static array of the largest size (2L + 1 <= 129 rows and a pulse table of 8 x 64
+// complex) whose offsets the compiler folds into the LDS instructions; one workgroup per CU at any
+// L <= 64 as with a dynamic array.  The kernels name the array directly: a pointer variable captured by
+// their lambdas costs VALU instructions.
+constexpr int kHermStaticLds = kRows + 129 * kRowBytes + 8 * 64 * 8;
+__shared__ __align__(16) unsigned char herm_static_lds[kHermStaticLds];
 
 template <int N>
 using Ic = std::integral_constant<int, N>;
@@ -77,7 +78,7 @@ __global__ void __launch_bounds__(64 * W)
     herm_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state, float* __restrict__ rec)
 {
     constexpr int kNT = 64 * W;
-    extern __shared__ __align__(16) unsigned char smem[];
+#define smem herm_static_lds
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
@@ -457,6 +458,7 @@ __global__ void __launch_bounds__(64 * W)
         }
     }
 }
+#undef smem
 
 // Spin-pipelined variant (no pulse; null or on-site Coulomb).  Waves 0-7 carry spin up, waves 8-15
 // spin down, and the down group runs half a stage behind, so every interval between two barriers pairs
@@ -475,7 +477,7 @@ __global__ void __launch_bounds__(1024)
 {
     constexpr int kNT = 1024;
     constexpr int kNCG = 8;  // column groups per wave in phase A
-#define smem pipe_static_lds
+#define smem herm_static_lds
     const int L = s.L;
     const int LL = L * L;
     const int R2 = 2 * L;
@@ -831,15 +833,8 @@ template <int W, int EA, int S, int Z, bool PULSE, bool DENS>
 hipError_t run(const DevSystem<float>& s, const RunShape& r, int t0, int t1, float* state, float* rec, hipStream_t st)
 {
     auto kern = herm_kernel<W, EA, S, Z, PULSE, DENS>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(64 * W), herm_lds_bytes(s.L, s.z, PULSE), st, s,
-                       r, t0, t1, state, rec);
+    if (herm_lds_bytes(s.L, s.z, PULSE) > static_cast<size_t>(kHermStaticLds)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(64 * W), 0, st, s, r, t0, t1, state, rec);
     return hipGetLastError();
 }
 
@@ -881,7 +876,7 @@ hipError_t run_pipe(const DevSystem<float>& s, const RunShape& r, int t0, int t1
                     hipStream_t st)
 {
     auto kern = herm_pipe_kernel<S, Z>;
-    if (herm_lds_bytes(s.L, s.z, false) > static_cast<size_t>(kPipeStaticLds)) return hipErrorInvalidValue;
+    if (herm_lds_bytes(s.L, s.z, false) > static_cast<size_t>(kHermStaticLds)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(1024), 0, st, s, r, t0, t1, state, rec);
     return hipGetLastError();
 }
