@@ -14,10 +14,10 @@
 // with a barrier after M and one after C (before the next M rewrites the rows).  Per element that is the
 // same 2z complex x real multiply-adds as forming X, with the neighbour tables (row offsets, column
 // offsets, hoppings; entry n of every site at the n-th most common shift k - i mod 32) read from LDS per
-// slot.  LDS: rows of Y, 1 mod 32 complex long.  Past L = 64 one 8-wave workgroup per CU (up to 256 VGPRs,
-// S <= 12 slots per thread) and a dynamic LDS array; at L = 64 two per CU (4 slots, 128 VGPRs) and a
-// static one.  Element slots in class order (dev::class_slot).  Null or on-site Coulomb interactions, no
-// pulse, no density records.
+// slot.  LDS: rows of Y, 1 mod 32 complex long, in a static array.  Past L = 64 one 8-wave workgroup per
+// CU (up to 256 VGPRs, S <= 12 slots per thread); at L = 64 two per CU (4 slots, 128 VGPRs).  Element
+// slots in class order (dev::class_slot).  Null or on-site Coulomb interactions, no pulse, no density
+// records.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -75,13 +75,14 @@ __device__ __forceinline__ T ldl(const unsigned char* base, int off)
 template <int N>
 using Ic = std::integral_constant<int, N>;
 
-// LDS of the L <= 64 instances, a static array whose offsets the compiler folds into the LDS instructions:
-// the tables, then L + 1 <= 65 rows of wide_row_len(L) <= 65 complex.  Past L = 64 the launch sizes the
-// dynamic array.  The kernel names either array directly (a pointer variable that the lambdas capture
-// costs the L = 100 instance 3 % more VALU instructions).
+// LDS as static arrays whose offsets the compiler folds into the LDS instructions: the tables, then L + 1
+// rows of wide_row_len(L) complex.  L <= 64: at most 65 rows of 65 (51.7 KB, two workgroups per CU); past
+// L = 64: at most 129 rows of 129 (151 KB, one workgroup per CU).  The kernel names the array directly (a
+// pointer variable that the lambdas capture costs the L = 100 instance 3 % more VALU instructions).
 constexpr int kStaticLds = kRows + 65 * 65 * kCB;
+constexpr int kStaticLdsBig = kRows + (kLP + 1) * 129 * kCB;
 __shared__ __align__(16) unsigned char wide_static_lds[kStaticLds];
-extern __shared__ __align__(16) unsigned char wide_dyn_lds[];
+__shared__ __align__(16) unsigned char wide_static_lds_big[kStaticLdsBig];
 
 // UNI: every neighbour entry holds the same hopping s.hop_value (a regular lattice with one bond type),
 // taken from a register instead of the table.
@@ -90,7 +91,7 @@ __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(S <= 4
     herm_wide_kernel(DevSystem<float> s, RunShape r, int t0, int t1, float* __restrict__ state,
                      float* __restrict__ rec)
 {
-#define WIDE_LDS (S <= 4 ? wide_static_lds : wide_dyn_lds)  // this kernel's LDS (a global array, not captured)
+#define WIDE_LDS (S <= 4 ? wide_static_lds : wide_static_lds_big)  // this kernel's LDS (a global array)
     constexpr bool kRowBytes = S <= 4;  // L <= 64: neighbour row offsets in bytes fit 16 bits
     // one hopping value at L <= 64: neighbour terms summed first, then multiplied (2 fewer VALU
     // instructions per slot; at L = 100 the same form measured 2.5 % slower)
@@ -465,17 +466,8 @@ hipError_t run_wide(const DevSystem<float>& s, const RunShape& r, int t0, int t1
                     hipStream_t st)
 {
     auto kern = herm_wide_kernel<S, Z, UNI>;
-    static bool attr_set = S <= 4;  // the static array of the L <= 64 instances needs no opt-in
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsLimit));
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    if (S <= 4 && wide_lds_bytes(s.L) > static_cast<size_t>(kStaticLds)) return hipErrorInvalidValue;
-    const size_t dyn = S <= 4 ? 0 : wide_lds_bytes(s.L);  // the L <= 64 instances hold a static array
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(kNT), dyn, st, s, r, t0, t1,
-                       state, rec);
+    if (wide_lds_bytes(s.L) > static_cast<size_t>(S <= 4 ? kStaticLds : kStaticLdsBig)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(r.B)), dim3(kNT), 0, st, s, r, t0, t1, state, rec);
     return hipGetLastError();
 }
 
